@@ -1,0 +1,244 @@
+/*
+ * melon3d_ext.h - extension entry points of the MI355X backend (m3dx_*).
+ *
+ * None of these exist in the reference C API; they sit beside the unchanged
+ * m3d_* functions of melon3d.h and serve four callers:
+ *   - parity tests: full-state snapshot/restore so one step can be compared
+ *     against the reference engine from identical state, plus read-only
+ *     views of per-stage outputs (grid tiers, islands, colors);
+ *   - the benchmark: device/stage counters for the HBM-traffic model;
+ *   - device selection and error reporting.
+ * Plain C types only; no torch types cross this boundary.
+ */
+#pragma once
+
+#ifndef M3D_VERSION_MAJOR /* melon3d.h already seen (possibly from another include path) */
+#include "melon3d.h"
+#endif
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/* ------------------------------------------------------------ snapshot */
+
+/* One body slot (in use or free), mirroring the reference Body
+ * (src/m3d_internal.h:77-119) plus its broadphase sidecars
+ * (src/m3d_internal.h:627-639). All fields are 4 bytes wide. */
+typedef struct m3dx_body_state
+{
+	float position[3];
+	float rotation[4];
+	float linearVelocity[3];
+	float angularVelocity[3];
+	float prevPosition[3];
+	float prevRotation[4];
+	float prevLinearVelocity[3];
+	float prevAngularVelocity[3];
+	float invMass;
+	float invInertiaLocal[3];
+	float invInertiaWorld[9]; /* columns cx, cy, cz */
+	int32_t shapeType;
+	float radius;
+	float halfHeight;
+	float halfExtents[3];
+	float friction;
+	float restitution;
+	float rollingResistance;
+	float mass;
+	float force[3];
+	float torque[3];
+	uint32_t categoryBits;
+	uint32_t maskBits;
+	int32_t groupIndex;
+	int32_t isSensor;
+	int32_t type;
+	float linearDamping;
+	float angularDamping;
+	float gravityScale;
+	float sleepTime;
+	float smoothedEnergy;
+	int32_t enableSleep;
+	int32_t isAwake;
+	int32_t inUse;
+	uint32_t generation;
+	uint32_t userDataLo;
+	uint32_t userDataHi;
+	/* broadphase sidecars */
+	float fatAABB[6]; /* lower xyz, upper xyz */
+	int32_t hasFatAABB;
+	int32_t inStableTier; /* migrated into the stable tier at its last build */
+	int32_t inStableCells; /* has entries in the stable tier (not oversized) */
+} m3dx_body_state;
+
+/* One manifold point (src/m3d_internal.h:154-187). */
+typedef struct m3dx_point_state
+{
+	float localAnchorA[3];
+	float localAnchorB[3];
+	float frictionAnchorA[3];
+	float frictionAnchorB[3];
+	float separation;
+	uint32_t id;
+	float lambdaN;
+	float lambdaT;
+	float separationOffset;
+	float slipDir[3];
+	float rA[3];
+	float rB[3];
+	float wN;
+	float angJA[3];
+	float angJB[3];
+} m3dx_point_state;
+
+/* One contact (src/m3d_internal.h:189-229). */
+typedef struct m3dx_contact_state
+{
+	int32_t bodyA;
+	int32_t bodyB;
+	float friction;
+	float restitution;
+	float rollingResistance;
+	int32_t touching;
+	int32_t isSensor;
+	int32_t tunnelGuard;
+	int32_t curved;
+	int32_t hasCache;
+	float cachePosA[3];
+	float cachePosB[3];
+	float cacheRotA[4];
+	float cacheRotB[4];
+	float normal[3];
+	int32_t pointCount;
+	m3dx_point_state points[4];
+} m3dx_contact_state;
+
+/* One joint slot (src/m3d_internal.h:313-349). type: 0 distance, 1 ball,
+ * 2 hinge, 3 weld. */
+typedef struct m3dx_joint_state
+{
+	int32_t type;
+	int32_t bodyA;
+	int32_t bodyB;
+	float localAnchorA[3];
+	float localAnchorB[3];
+	float length;
+	float hertz;
+	float dampingRatio;
+	float localAxisA[3];
+	float localAxisB[3];
+	float localRefA[3];
+	float localRefB[3];
+	int32_t enableLimit;
+	float lowerAngle;
+	float upperAngle;
+	int32_t enableMotor;
+	float motorSpeed;
+	float maxMotorTorque;
+	float relRot0[4];
+	float compliance;
+	float lambda;
+	float motorLambda;
+	int32_t inUse;
+	uint32_t generation;
+} m3dx_joint_state;
+
+/* Broadphase grid entry: packed cell key (21 bits per axis) and body. */
+typedef struct m3dx_grid_entry
+{
+	uint64_t key;
+	int32_t body;
+	int32_t pad;
+} m3dx_grid_entry;
+
+typedef struct m3dx_world_header
+{
+	int32_t bodyCapacity;
+	int32_t contactCount;
+	int32_t jointCapacity;
+	int32_t freeBodyCount;
+	int32_t freeJointCount;
+	int32_t stableEntryCount;
+	int32_t stableOversizedCount;
+	int32_t gridDirty;
+	int32_t stableDirty;
+	int32_t stableForce;
+	int32_t stepsSinceStableBuild;
+	int32_t anyForcesApplied;
+	int32_t jointedPairsDirty;
+	float gravity[3];
+	int32_t workerCount;
+	int32_t positionIterations;
+	float gridCellSize;
+	int32_t enableSleep;
+} m3dx_world_header;
+
+/* Caller-owned arrays sized from header (fill the header with a first call
+ * whose array pointers are NULL). */
+typedef struct m3dx_snapshot
+{
+	m3dx_world_header header;
+	m3dx_body_state* bodies;          /* bodyCapacity */
+	m3dx_contact_state* contacts;     /* contactCount */
+	m3dx_joint_state* joints;         /* jointCapacity */
+	int32_t* freeBodies;              /* freeBodyCount, in free-list order */
+	int32_t* freeJoints;              /* freeJointCount */
+	m3dx_grid_entry* stableEntries;   /* stableEntryCount, sorted */
+	int32_t* stableOversized;         /* stableOversizedCount */
+} m3dx_snapshot;
+
+/* Fills snapshot->header always and every non-NULL array. Returns 0 on
+ * success, -1 on error (see m3dx_last_error). */
+M3D_API int m3dx_world_export(m3d_world* world, m3dx_snapshot* snapshot);
+
+/* Creates a new world whose full state equals the snapshot. NULL on error. */
+M3D_API m3d_world* m3dx_world_import(const m3dx_snapshot* snapshot);
+
+/* ------------------------------------------------- per-stage views */
+
+/* Sorted grid tiers as of the last broadphase refresh. tier 0 = active,
+ * 1 = stable. Returns the entry count; copies at most cap entries. */
+M3D_API int m3dx_debug_grid_entries(m3d_world* world, int tier, m3dx_grid_entry* out, int cap);
+/* Oversized (coarse) list of a tier, in list order. */
+M3D_API int m3dx_debug_oversized(m3d_world* world, int tier, int32_t* out, int cap);
+/* Island of every body slot at the last solve (-1: not a dynamic body, or no
+ * awake dynamic body that step). Returns the island count. islandAwake gets
+ * one flag per island when non-NULL. */
+M3D_API int m3dx_debug_islands(m3d_world* world, int32_t* islandOfBody, int bodyCap, uint8_t* islandAwake,
+							   int islandCap);
+/* Graph color of every contact at the last solve: -1 not in a colored
+ * island, 0..7 a color, 8 the serial overflow pass. Returns contact count. */
+M3D_API int m3dx_debug_contact_colors(m3d_world* world, int32_t* colorOfContact, int cap);
+
+/* ------------------------------------------------------ counters */
+
+typedef struct m3dx_step_stats
+{
+	int32_t bodyCapacity;
+	int32_t awakeDynamicBodies;   /* bodies integrated per substep */
+	int32_t contacts;
+	int32_t solvedContacts;       /* touching, non-sensor, in awake islands */
+	int32_t solvedPoints;         /* manifold points of solved contacts */
+	int32_t boxBoxPoints;         /* points of solved box-box contacts */
+	int32_t regeneratedManifolds; /* narrowphase regenerations this step */
+	int32_t activeEntries;
+	int32_t stableEntries;
+	int32_t refattedBodies;
+	int32_t islands;
+	int32_t coloredIslands;
+	int32_t sweeps;               /* Gauss-Seidel sweeps per iteration */
+	int32_t colorRounds;          /* rounds of the parallel coloring fixpoint */
+} m3dx_step_stats;
+
+M3D_API void m3dx_world_stats(m3d_world* world, m3dx_step_stats* out);
+
+/* Last error message of this thread ("" when none). */
+M3D_API const char* m3dx_last_error(void);
+
+/* Device the next m3d_world_create binds to (default 0). */
+M3D_API int m3dx_set_device(int device);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,1148 @@
+// Broadphase and contact-set maintenance kernels.
+//
+//   refresh     fat-AABB breach test and tier flags, one thread per body
+//               (src/world.cpp:156-229)
+//   tiers       count -> scan -> emit of (cell key, body) entries in body
+//               order, then a stable radix sort on a compressed cell key,
+//               which yields the reference's unique (key, body) order
+//               (src/grid.cpp:39-131)
+//   pairs       one thread per cell run of the active tier enumerates the
+//               run's candidates in the reference's serial order (active x
+//               active, then active x stable), count pass + emit pass;
+//               coarse-list candidates follow in list order
+//               (src/world.cpp:1570-1702)
+//   create      first admitted occurrence of each pair key wins, preserving
+//               candidate order (the serial creation re-check)
+//   remove      separated awake pairs are swap-removed in descending index
+//               order; the k-th largest removed slot below the new size
+//               receives the k-th largest surviving slot above it
+//               (src/world.cpp:1705-1735, :100-119)
+#include "mx_steps.h"
+
+#include <algorithm>
+
+namespace mx
+{
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, int64_t n)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s)
+{
+	if (n > 0)
+		k_fill_i32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
+{
+	if (n > 0)
+		k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+}
+static void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s)
+{
+	if (n > 0)
+		k_fill_u64<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+}
+
+static DBodies body_view(World& w)
+{
+	DBodies B;
+	B.pos = w.bc.pos.p;
+	B.rot = w.bc.rot.p;
+	B.vel = w.bc.vel.p;
+	B.angVel = w.bc.angVel.p;
+	B.prevPos = w.bc.prevPos.p;
+	B.prevRot = w.bc.prevRot.p;
+	B.prevVel = w.bc.prevVel.p;
+	B.prevAngVel = w.bc.prevAngVel.p;
+	B.invIL = w.bc.invIL.p;
+	B.iw0 = w.bc.iw0.p;
+	B.iw1 = w.bc.iw1.p;
+	B.iw2 = w.bc.iw2.p;
+	B.shape = w.bc.shape.p;
+	B.mat = w.bc.mat.p;
+	B.force = w.bc.force.p;
+	B.torque = w.bc.torque.p;
+	B.flags = w.bc.flags.p;
+	B.filter = w.bc.filter.p;
+	B.fatLo = w.bc.fatLo.p;
+	B.fatHi = w.bc.fatHi.p;
+	B.bp = w.bc.bp.p;
+	B.sb = w.bc.sb.p;
+	return B;
+}
+
+static DContacts contact_view(World& w)
+{
+	DContacts C;
+	C.cap = w.contactCap;
+	C.bodies = w.cc.bodies.p;
+	C.mat = w.cc.mat.p;
+	C.normal = w.cc.normal.p;
+	C.cachePA = w.cc.cachePA.p;
+	C.cachePB = w.cc.cachePB.p;
+	C.cacheQA = w.cc.cacheQA.p;
+	C.cacheQB = w.cc.cacheQB.p;
+	C.pA = w.cc.pA.p;
+	C.pB = w.cc.pB.p;
+	C.pJA = w.cc.pJA.p;
+	C.pJB = w.cc.pJB.p;
+	C.pRA = w.cc.pRA.p;
+	C.pRB = w.cc.pRB.p;
+	C.pFA = w.cc.pFA.p;
+	C.pFB = w.cc.pFB.p;
+	C.pSlip = w.cc.pSlip.p;
+	return C;
+}
+
+DBodies make_body_view(World& w) { return body_view(w); }
+DContacts make_contact_view(World& w) { return contact_view(w); }
+
+// ------------------------------------------------------------ refresh
+
+MX_HD m3d_aabb fat_lo_hi(float4 lo, float4 hi)
+{
+	m3d_aabb a;
+	a.lowerBound = f4_v3(lo);
+	a.upperBound = f4_v3(hi);
+	return a;
+}
+
+__global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint32_t fl = B.flags[i];
+	bool inUse = (fl & BF_IN_USE) != 0;
+	bool awake = (fl & BF_AWAKE) != 0;
+	int type = bf_type(fl);
+	uint8_t bp = B.bp[i] & (BP_HAS_FAT | BP_IN_STABLE_TIER | BP_IN_STABLE_CELLS);
+	bool stable = inUse && !awake;
+	bool active = inUse && (awake || !(bp & BP_IN_STABLE_TIER));
+	if (active)
+		bp |= BP_ACTIVE;
+	if (stable)
+		bp |= BP_STABLE;
+	uint8_t sb = (uint8_t)((inUse ? SB_IN_USE : 0) | ((inUse && type == M3D_BODY_DYNAMIC) ? SB_DYNAMIC : 0) |
+						   ((inUse && type == M3D_BODY_KINEMATIC) ? SB_KINEMATIC : 0) | (awake ? SB_AWAKE : 0));
+	if (active)
+	{
+		Shape sh = load_shape(B, i);
+		m3d_transform xf = load_xf(B, i);
+		m3d_vec3 v = f4_v3(B.vel[i]);
+		m3d_aabb pred = shape_aabb(sh, xf);
+		m3d_vec3 st = m3d_scale(dt, v);
+		if (st.x < 0.0f) pred.lowerBound.x += st.x; else pred.upperBound.x += st.x;
+		if (st.y < 0.0f) pred.lowerBound.y += st.y; else pred.upperBound.y += st.y;
+		if (st.z < 0.0f) pred.lowerBound.z += st.z; else pred.upperBound.z += st.z;
+		if (!(bp & BP_HAS_FAT) || !m3d_aabb_contains(fat_lo_hi(B.fatLo[i], B.fatHi[i]), pred))
+		{
+			m3d_vec3 mg = m3d_v3(kAABBMargin, kAABBMargin, kAABBMargin);
+			m3d_aabb fat;
+			fat.lowerBound = m3d_sub(pred.lowerBound, mg);
+			fat.upperBound = m3d_add(pred.upperBound, mg);
+			m3d_vec3 sl = m3d_scale(2.0f * dt, v);
+			if (sl.x < 0.0f) fat.lowerBound.x += sl.x; else fat.upperBound.x += sl.x;
+			if (sl.y < 0.0f) fat.lowerBound.y += sl.y; else fat.upperBound.y += sl.y;
+			if (sl.z < 0.0f) fat.lowerBound.z += sl.z; else fat.upperBound.z += sl.z;
+			B.fatLo[i] = v3_f4(fat.lowerBound, 0.0f);
+			B.fatHi[i] = v3_f4(fat.upperBound, 0.0f);
+			bp |= BP_HAS_FAT | BP_MOVED;
+			atomicAdd(&counters[CT_REFAT], 1u);
+		}
+	}
+	else if (stable && rebuildStable && !(bp & BP_HAS_FAT))
+	{
+		// ExpandedAABB: margin first, then the one-step velocity extension
+		Shape sh = load_shape(B, i);
+		m3d_transform xf = load_xf(B, i);
+		m3d_aabb a = shape_aabb(sh, xf);
+		m3d_vec3 mg = m3d_v3(kAABBMargin, kAABBMargin, kAABBMargin);
+		a.lowerBound = m3d_sub(a.lowerBound, mg);
+		a.upperBound = m3d_add(a.upperBound, mg);
+		m3d_vec3 d = m3d_scale(dt, f4_v3(B.vel[i]));
+		if (d.x < 0.0f) a.lowerBound.x += d.x; else a.upperBound.x += d.x;
+		if (d.y < 0.0f) a.lowerBound.y += d.y; else a.upperBound.y += d.y;
+		if (d.z < 0.0f) a.lowerBound.z += d.z; else a.upperBound.z += d.z;
+		B.fatLo[i] = v3_f4(a.lowerBound, 0.0f);
+		B.fatHi[i] = v3_f4(a.upperBound, 0.0f);
+		bp |= BP_HAS_FAT;
+	}
+	B.bp[i] = bp;
+	B.sb[i] = sb;
+}
+
+void launch_refresh(World& w, float dt, bool rebuildStable)
+{
+	int32_t n = w.deviceBodyCount;
+	if (n > 0)
+	{
+		k_refresh<<<grid_for(n), kBlock, 0, w.stream>>>(body_view(w), n, dt, rebuildStable ? 1 : 0, w.counters.p);
+	}
+}
+
+__global__ void k_stable_migrate(uint8_t* bp, int32_t n)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint8_t b = bp[i];
+	bp[i] = (b & BP_STABLE) ? (uint8_t)(b | BP_IN_STABLE_TIER) : (uint8_t)(b & ~BP_IN_STABLE_TIER);
+}
+
+void launch_stable_migrate(World& w)
+{
+	int32_t n = w.deviceBodyCount;
+	if (n > 0)
+		k_stable_migrate<<<grid_for(n), kBlock, 0, w.stream>>>(w.bc.bp.p, n);
+}
+
+// -------------------------------------------------------------- tiers
+
+MX_HD int64_t cell_coord(float v, float inv) { return (int64_t)floorf(v * inv); }
+
+MX_HD uint64_t pack_cell(int64_t x, int64_t y, int64_t z)
+{
+	const int64_t off = (int64_t)1 << 20;
+	uint64_t ux = (uint64_t)(x + off) & 0x1FFFFFu;
+	uint64_t uy = (uint64_t)(y + off) & 0x1FFFFFu;
+	uint64_t uz = (uint64_t)(z + off) & 0x1FFFFFu;
+	return (ux << 42) | (uy << 21) | uz;
+}
+
+MX_HD uint64_t cell_key_of(m3d_vec3 p, float cellSize)
+{
+	float inv = 1.0f / cellSize;
+	return pack_cell(cell_coord(p.x, inv), cell_coord(p.y, inv), cell_coord(p.z, inv));
+}
+
+__global__ void k_tier_count(DBodies B, int32_t n, uint8_t includeBit, float cellSize, uint32_t* cnt, uint32_t* ov)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint32_t c = 0, o = 0;
+	if (B.bp[i] & includeBit)
+	{
+		float inv = 1.0f / cellSize;
+		float4 lo = B.fatLo[i], hi = B.fatHi[i];
+		int64_t x0 = cell_coord(lo.x, inv), y0 = cell_coord(lo.y, inv), z0 = cell_coord(lo.z, inv);
+		int64_t x1 = cell_coord(hi.x, inv), y1 = cell_coord(hi.y, inv), z1 = cell_coord(hi.z, inv);
+		int64_t span = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+		if (span > kOversizedCellSpan)
+			o = 1;
+		else
+			c = (uint32_t)span;
+	}
+	cnt[i] = c;
+	ov[i] = o;
+}
+
+__global__ void k_compact_indices(const uint32_t* flags, const uint32_t* offs, int64_t n, int32_t* out)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flags[i])
+		out[offs[i]] = (int32_t)i;
+}
+
+__device__ __forceinline__ void block_minmax(uint32_t v, bool valid, uint32_t* gmin, uint32_t* gmax)
+{
+	__shared__ uint32_t smin, smax;
+	if (threadIdx.x == 0)
+	{
+		smin = 0xFFFFFFFFu;
+		smax = 0u;
+	}
+	__syncthreads();
+	if (valid)
+	{
+		atomicMin(&smin, v);
+		atomicMax(&smax, v);
+	}
+	__syncthreads();
+	if (threadIdx.x == 0 && smin <= smax)
+	{
+		atomicMin(gmin, smin);
+		atomicMax(gmax, smax);
+	}
+	__syncthreads();
+}
+
+__global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, float cellSize, const uint32_t* cnt,
+							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
+	bool any = false;
+	if (i < n && cnt[i] > 0 && (B.bp[i] & includeBit))
+	{
+		float inv = 1.0f / cellSize;
+		float4 lo = B.fatLo[i], hi = B.fatHi[i];
+		int64_t x0 = cell_coord(lo.x, inv), y0 = cell_coord(lo.y, inv), z0 = cell_coord(lo.z, inv);
+		int64_t x1 = cell_coord(hi.x, inv), y1 = cell_coord(hi.y, inv), z1 = cell_coord(hi.z, inv);
+		uint32_t o = offs[i];
+		for (int64_t x = x0; x <= x1; ++x)
+			for (int64_t y = y0; y <= y1; ++y)
+				for (int64_t z = z0; z <= z1; ++z)
+				{
+					uint64_t k = pack_cell(x, y, z);
+					keys[o] = k;
+					bodies[o] = (uint32_t)i;
+					++o;
+					uint32_t ux = (uint32_t)(k >> 42) & 0x1FFFFFu, uy = (uint32_t)(k >> 21) & 0x1FFFFFu,
+							 uz = (uint32_t)k & 0x1FFFFFu;
+					mnx = min(mnx, ux);
+					mxx = max(mxx, ux);
+					mny = min(mny, uy);
+					mxy = max(mxy, uy);
+					mnz = min(mnz, uz);
+					mxz = max(mxz, uz);
+					any = true;
+				}
+	}
+	block_minmax(mnx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
+	block_minmax(mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
+	block_minmax(mny, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
+	block_minmax(mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
+	block_minmax(mnz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+	block_minmax(mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+}
+
+struct KeyCompress
+{
+	uint32_t ux0, uy0, uz0;
+	uint64_t ny, nz;
+};
+
+__global__ void k_key_compress(uint64_t* keys, int64_t n, KeyCompress kc)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint64_t k = keys[i];
+	uint64_t ux = (k >> 42) & 0x1FFFFFu, uy = (k >> 21) & 0x1FFFFFu, uz = k & 0x1FFFFFu;
+	keys[i] = ((ux - kc.ux0) * kc.ny + (uy - kc.uy0)) * kc.nz + (uz - kc.uz0);
+}
+
+__global__ void k_key_expand(uint64_t* keys, int64_t n, KeyCompress kc)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint64_t c = keys[i];
+	uint64_t cz = c % kc.nz;
+	uint64_t r = c / kc.nz;
+	uint64_t cy = r % kc.ny;
+	uint64_t cx = r / kc.ny;
+	keys[i] = ((cx + kc.ux0) << 42) | ((cy + kc.uy0) << 21) | (cz + kc.uz0);
+}
+
+__global__ void k_mark_cells(DBodies B, int32_t n, uint8_t includeBit, uint8_t cellBit, const uint32_t* cnt)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint8_t bp = B.bp[i];
+	bool in = (bp & includeBit) && cnt[i] > 0;
+	B.bp[i] = in ? (uint8_t)(bp | cellBit) : (uint8_t)(bp & ~cellBit);
+}
+
+bool sync_counters(World& w)
+{
+	if (!hip_ok(hipMemcpyAsync(w.hostCounters, w.counters.p, kCountersSize * sizeof(uint32_t), hipMemcpyDeviceToHost,
+							   w.stream),
+				"counter readback"))
+		return false;
+	return hip_ok(hipStreamSynchronize(w.stream), "stream sync");
+}
+
+static bool ensure_prim_scratch(World& w, int64_t n)
+{
+	size_t need = std::max(scan_scratch_bytes(n), sort_scratch_bytes(n)) + 1024;
+	return w.primScratch.ensure(need);
+}
+
+bool build_tier(World& w, int tier)
+{
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	uint8_t includeBit = tier == 0 ? BP_ACTIVE : BP_STABLE;
+	uint8_t cellBit = tier == 0 ? BP_IN_ACTIVE_CELLS : BP_IN_STABLE_CELLS;
+	DBuf<uint64_t>& keys = tier == 0 ? w.activeKeys : w.stableKeys;
+	DBuf<uint32_t>& vals = tier == 0 ? w.activeBodies : w.stableBodies;
+	std::vector<int32_t>& over = tier == 0 ? w.activeOversized : w.stableOversized;
+	DBuf<int32_t>& dOver = tier == 0 ? w.dActiveOversized : w.dStableOversized;
+	int64_t& count = tier == 0 ? w.activeCount : w.stableCount;
+	DBodies B = body_view(w);
+	if (n == 0)
+	{
+		count = 0;
+		over.clear();
+		return true;
+	}
+	if (!w.u32a.ensure(n) || !w.u32b.ensure(n) || !w.u32c.ensure(n) || !w.i32a.ensure(n) || !ensure_prim_scratch(w, n))
+		return false;
+	uint32_t* cnt = w.u32a.p;
+	uint32_t* offs = w.u32b.p;
+	uint32_t* ov = w.u32c.p;
+	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, w.def.gridCellSize, cnt, ov);
+	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.primScratch.p, s);
+	// oversized list in body order
+	uint32_t* ovOff = w.u32d.ensure(n) ? w.u32d.p : nullptr;
+	if (ovOff == nullptr)
+		return false;
+	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.primScratch.p, s);
+	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
+	if (!sync_counters(w))
+		return false;
+	int64_t E = w.hostCounters[CT_ENTRIES];
+	int32_t nov = (int32_t)w.hostCounters[CT_OVERSIZED];
+	over.resize(nov);
+	if (nov > 0)
+	{
+		if (!dOver.ensure(nov))
+			return false;
+		hipMemcpyAsync(dOver.p, w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+		hipMemcpyAsync(over.data(), w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+	}
+	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, cellBit, cnt);
+	count = E;
+	if (E == 0)
+	{
+		return hip_ok(hipStreamSynchronize(s), "tier");
+	}
+	if (!keys.ensure(E) || !vals.ensure(E) || !w.sortKeysAlt.ensure(E) || !w.sortValsAlt.ensure(E) ||
+		!ensure_prim_scratch(w, E))
+		return false;
+	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
+	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
+	k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, w.def.gridCellSize, cnt, offs, keys.p, vals.p,
+											   w.counters.p);
+	if (!sync_counters(w))
+		return false;
+	KeyCompress kc;
+	kc.ux0 = w.hostCounters[CT_UX_MIN];
+	kc.uy0 = w.hostCounters[CT_UY_MIN];
+	kc.uz0 = w.hostCounters[CT_UZ_MIN];
+	uint64_t nx = (uint64_t)(w.hostCounters[CT_UX_MAX] - kc.ux0) + 1;
+	kc.ny = (uint64_t)(w.hostCounters[CT_UY_MAX] - kc.uy0) + 1;
+	kc.nz = (uint64_t)(w.hostCounters[CT_UZ_MAX] - kc.uz0) + 1;
+	uint64_t range = nx * kc.ny * kc.nz; // <= 2^63
+	int bits = 0;
+	while (bits < 64 && (range - 1) >> bits)
+		++bits;
+	k_key_compress<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
+	radix_sort_pairs(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p, s);
+	k_key_expand<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
+	return true;
+}
+
+// ------------------------------------------------------------ hashing
+
+constexpr uint64_t kEmptyKey = ~0ull;
+
+MX_HD uint32_t hash_key(uint64_t k)
+{
+	k ^= k >> 33;
+	k *= 0xff51afd7ed558ccdull;
+	k ^= k >> 33;
+	k *= 0xc4ceb9fe1a85ec53ull;
+	k ^= k >> 33;
+	return (uint32_t)k;
+}
+
+__device__ __forceinline__ uint32_t hash_insert(uint64_t* table, uint32_t mask, uint64_t key)
+{
+	uint32_t h = hash_key(key) & mask;
+	for (;;)
+	{
+		uint64_t prev = atomicCAS((unsigned long long*)&table[h], (unsigned long long)kEmptyKey,
+								  (unsigned long long)key);
+		if (prev == kEmptyKey || prev == key)
+			return h;
+		h = (h + 1) & mask;
+	}
+}
+
+__device__ __forceinline__ int64_t hash_find(const uint64_t* table, uint32_t mask, uint64_t key)
+{
+	uint32_t h = hash_key(key) & mask;
+	for (;;)
+	{
+		uint64_t k = table[h];
+		if (k == key)
+			return h;
+		if (k == kEmptyKey)
+			return -1;
+		h = (h + 1) & mask;
+	}
+}
+
+__global__ void k_hash_contacts(const int2* bodies, int32_t C, uint64_t* table, uint32_t mask)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int2 b = bodies[c];
+	hash_insert(table, mask, contact_key(b.x, b.y));
+}
+
+static uint32_t table_size_for(int64_t n)
+{
+	uint32_t s = 1024;
+	while ((int64_t)s < 2 * n + 64)
+		s <<= 1;
+	return s;
+}
+
+// ------------------------------------------------------------ pairs
+
+struct PairCtx
+{
+	DBodies B;
+	int32_t n;
+	float cellSize;
+	const uint64_t* act;
+	const uint32_t* actBody;
+	int64_t nAct;
+	const uint64_t* stab;
+	const uint32_t* stabBody;
+	int64_t nStab;
+	const uint64_t* contactTable;
+	uint32_t contactMask;
+	const uint64_t* jointed;
+	int32_t nJointed;
+};
+
+// Read-only admission filter; src/world.cpp:1486-1538.
+__device__ bool pair_admissible(const PairCtx& P, int32_t a, int32_t b)
+{
+	if (a == b)
+		return false;
+	const DBodies& B = P.B;
+	if (!(B.bp[a] & BP_MOVED) && !(B.bp[b] & BP_MOVED))
+		return false;
+	int32_t lo = a < b ? a : b;
+	int32_t hi = a < b ? b : a;
+	uint32_t fl = B.flags[lo], fh = B.flags[hi];
+	if (!(fl & BF_IN_USE) || !(fh & BF_IN_USE))
+		return false;
+	if (bf_type(fl) != M3D_BODY_DYNAMIC && bf_type(fh) != M3D_BODY_DYNAMIC)
+		return false;
+	if (!(fl & BF_AWAKE) && !(fh & BF_AWAKE))
+		return false;
+	float4 lL = B.fatLo[lo], lH = B.fatHi[lo], hL = B.fatLo[hi], hH = B.fatHi[hi];
+	if (!(lL.x <= hH.x && lL.y <= hH.y && lL.z <= hH.z && hL.x <= lH.x && hL.y <= lH.y && hL.z <= lH.z))
+		return false;
+	uint4 fa = B.filter[lo], fb = B.filter[hi];
+	int32_t ga = (int32_t)fa.z, gb = (int32_t)fb.z;
+	if (ga != 0 && ga == gb)
+	{
+		if (ga < 0)
+			return false;
+	}
+	else if ((fa.x & fb.y) == 0 || (fb.x & fa.y) == 0)
+	{
+		return false;
+	}
+	uint64_t key = contact_key(lo, hi);
+	if (hash_find(P.contactTable, P.contactMask, key) >= 0)
+		return false;
+	if (P.nJointed > 0)
+	{
+		int32_t l = 0, r = P.nJointed;
+		while (l < r)
+		{
+			int32_t m = (l + r) >> 1;
+			if (P.jointed[m] < key)
+				l = m + 1;
+			else
+				r = m;
+		}
+		if (l < P.nJointed && P.jointed[l] == key)
+			return false;
+	}
+	return true;
+}
+
+__device__ __forceinline__ bool dedupe_ok(const PairCtx& P, int32_t a, int32_t b, uint64_t runKey)
+{
+	float4 la = P.B.fatLo[a], lb = P.B.fatLo[b];
+	m3d_vec3 lo2 = m3d_max(f4_v3(la), f4_v3(lb));
+	return cell_key_of(lo2, P.cellSize) == runKey;
+}
+
+// One thread per run start; emit=0 counts, emit=1 writes keys at offs[i].
+__global__ void k_pairs(PairCtx P, int emit, uint32_t* cnt, const uint32_t* offs, uint64_t* out)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.nAct)
+		return;
+	uint64_t key = P.act[i];
+	if (i > 0 && P.act[i - 1] == key)
+	{
+		if (!emit)
+			cnt[i] = 0;
+		return;
+	}
+	int64_t e = i + 1;
+	while (e < P.nAct && P.act[e] == key)
+		++e;
+	uint32_t k = 0;
+	uint32_t o = emit ? offs[i] : 0;
+	for (int64_t x = i; x < e; ++x)
+	{
+		int32_t a = (int32_t)P.actBody[x];
+		for (int64_t y = x + 1; y < e; ++y)
+		{
+			int32_t b = (int32_t)P.actBody[y];
+			if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
+			{
+				if (emit)
+					out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
+				++k;
+			}
+		}
+	}
+	// stable entries of the same cell: lower_bound
+	int64_t l = 0, r = P.nStab;
+	while (l < r)
+	{
+		int64_t m = (l + r) >> 1;
+		if (P.stab[m] < key)
+			l = m + 1;
+		else
+			r = m;
+	}
+	for (int64_t s = l; s < P.nStab && P.stab[s] == key; ++s)
+	{
+		int32_t b = (int32_t)P.stabBody[s];
+		for (int64_t x = i; x < e; ++x)
+		{
+			int32_t a = (int32_t)P.actBody[x];
+			if (a == b)
+				continue;
+			if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
+			{
+				if (emit)
+					out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
+				++k;
+			}
+		}
+	}
+	if (!emit)
+		cnt[i] = k;
+}
+
+// Coarse-list candidate segments, laid out back to back in one index space.
+struct OverSeg
+{
+	int32_t a;
+	int32_t kind;  // 0: all bodies in either tier's cells, 1: list, 2: bodies in active cells
+	int32_t list;  // 0 active oversized list, 1 stable oversized list
+	int32_t first; // first list index (kind 1)
+	int64_t start; // first global candidate index
+	int64_t len;
+};
+
+constexpr int kMaxSegs = 64;
+
+struct OverCtx
+{
+	OverSeg seg[kMaxSegs];
+	int32_t nseg;
+	const int32_t* actOver;
+	const int32_t* stabOver;
+};
+
+__device__ bool over_candidate(const PairCtx& P, const OverCtx& O, int64_t g, uint64_t* keyOut)
+{
+	int s = 0;
+	while (s + 1 < O.nseg && O.seg[s + 1].start <= g)
+		++s;
+	const OverSeg& S = O.seg[s];
+	int64_t k = g - S.start;
+	int32_t a = S.a, b;
+	if (S.kind == 1)
+	{
+		b = (S.list == 0 ? O.actOver : O.stabOver)[S.first + k];
+		if (S.list == 1 && b == a)
+			return false;
+	}
+	else
+	{
+		b = (int32_t)k;
+		uint8_t bp = P.B.bp[b];
+		if (S.kind == 0)
+		{
+			if (b == a || !(bp & (BP_IN_ACTIVE_CELLS | BP_IN_STABLE_CELLS)))
+				return false;
+		}
+		else if (!(bp & BP_IN_ACTIVE_CELLS))
+		{
+			return false;
+		}
+	}
+	if (!pair_admissible(P, a, b))
+		return false;
+	*keyOut = contact_key(a < b ? a : b, a < b ? b : a);
+	return true;
+}
+
+__global__ void k_over_flags(PairCtx P, OverCtx O, int64_t total, uint32_t* flags)
+{
+	int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= total)
+		return;
+	uint64_t key;
+	flags[g] = over_candidate(P, O, g, &key) ? 1u : 0u;
+}
+
+__global__ void k_over_emit(PairCtx P, OverCtx O, int64_t total, const uint32_t* flags, const uint32_t* offs,
+							uint64_t* out)
+{
+	int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= total || !flags[g])
+		return;
+	uint64_t key;
+	over_candidate(P, O, g, &key);
+	out[offs[g]] = key;
+}
+
+__global__ void k_first_insert(const uint64_t* cand, int64_t n, uint64_t* table, uint32_t* pos, uint32_t mask)
+{
+	int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	uint32_t h = hash_insert(table, mask, cand[k]);
+	atomicMin(&pos[h], (uint32_t)k);
+}
+
+__global__ void k_first_flag(const uint64_t* cand, int64_t n, const uint64_t* table, const uint32_t* pos,
+							 uint32_t mask, uint32_t* flags)
+{
+	int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	int64_t h = hash_find(table, mask, cand[k]);
+	flags[k] = (h >= 0 && pos[h] == (uint32_t)k) ? 1u : 0u;
+}
+
+__global__ void k_compact_u64(const uint64_t* in, const uint32_t* flags, const uint32_t* offs, int64_t n,
+							  uint64_t* out)
+{
+	int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n && flags[k])
+		out[offs[k]] = in[k];
+}
+
+// Contact initialization; src/world.cpp:1540-1568 (value-initialized
+// manifold, combined materials, curved flag).
+__global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t count, int32_t first)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	uint64_t key = keys[k];
+	int32_t lo = (int32_t)(key >> 32), hi = (int32_t)(uint32_t)key;
+	int32_t c = first + k;
+	float4 ml = B.mat[lo], mh = B.mat[hi];
+	uint32_t fl = B.flags[lo], fh = B.flags[hi];
+	float friction = sqrtf(ml.y * mh.y);
+	float restitution = ml.z > mh.z ? ml.z : mh.z;
+	float rolling = sqrtf(ml.w * mh.w);
+	uint32_t cf = 0;
+	if ((fl & BF_SENSOR) || (fh & BF_SENSOR))
+		cf |= CF_SENSOR;
+	if (bf_shape(fl) != M3D_SHAPE_BOX || bf_shape(fh) != M3D_SHAPE_BOX)
+		cf |= CF_CURVED;
+	C.bodies[c] = make_int2(lo, hi);
+	C.mat[c] = make_float4(friction, restitution, rolling, u2f(cf));
+	C.normal[c] = make_float4(0.0f, 1.0f, 0.0f, i2f(0));
+	float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	C.cachePA[c] = z;
+	C.cachePB[c] = z;
+	C.cacheQA[c] = z;
+	C.cacheQB[c] = z;
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = pidx(C, c, j);
+		C.pA[p] = z;
+		C.pB[p] = z;
+		C.pJA[p] = z;
+		C.pJB[p] = z;
+		C.pRA[p] = z;
+		C.pRB[p] = z;
+		C.pFA[p] = z;
+		C.pFB[p] = z;
+		C.pSlip[p] = z;
+	}
+}
+
+bool enumerate_and_create(World& w)
+{
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	int32_t C0 = w.contactCount;
+	// existing-contact map
+	uint32_t tsz = table_size_for(C0);
+	if (!w.hashKeys.ensure(tsz) || !w.hashVals.ensure(tsz))
+		return false;
+	launch_fill_u64(w.hashKeys.p, kEmptyKey, tsz, s);
+	if (C0 > 0)
+		k_hash_contacts<<<grid_for(C0), kBlock, 0, s>>>(w.cc.bodies.p, C0, w.hashKeys.p, tsz - 1);
+
+	PairCtx P;
+	P.B = body_view(w);
+	P.n = n;
+	P.cellSize = w.def.gridCellSize;
+	P.act = w.activeKeys.p;
+	P.actBody = w.activeBodies.p;
+	P.nAct = w.activeCount;
+	P.stab = w.stableKeys.p;
+	P.stabBody = w.stableBodies.p;
+	P.nStab = w.stableCount;
+	P.contactTable = w.hashKeys.p;
+	P.contactMask = tsz - 1;
+	P.jointed = w.jointedKeys.p;
+	P.nJointed = w.jointedCount;
+
+	int64_t E = w.activeCount;
+	int64_t gridTotal = 0;
+	if (E > 0)
+	{
+		if (!w.u32a.ensure(E) || !w.u32b.ensure(E) || !ensure_prim_scratch(w, E))
+			return false;
+		k_pairs<<<grid_for(E), kBlock, 0, s>>>(P, 0, w.u32a.p, nullptr, nullptr);
+		scan_exclusive(w.u32a.p, w.u32b.p, E, w.counters.p + CT_CANDIDATES, w.primScratch.p, s);
+		if (!sync_counters(w))
+			return false;
+		gridTotal = w.hostCounters[CT_CANDIDATES];
+	}
+
+	// coarse-list segments in the reference's serial order
+	OverCtx O;
+	O.nseg = 0;
+	O.actOver = w.dActiveOversized.p;
+	O.stabOver = w.dStableOversized.p;
+	int64_t ovTotal = 0;
+	auto addSeg = [&](int32_t a, int kind, int list, int first, int64_t len) {
+		if (len <= 0)
+			return true;
+		if (O.nseg >= kMaxSegs)
+		{
+			set_error("too many oversized bodies (%d segments)", O.nseg);
+			return false;
+		}
+		OverSeg& S = O.seg[O.nseg++];
+		S.a = a;
+		S.kind = kind;
+		S.list = list;
+		S.first = first;
+		S.start = ovTotal;
+		S.len = len;
+		ovTotal += len;
+		return true;
+	};
+	const std::vector<int32_t>& ao = w.activeOversized;
+	const std::vector<int32_t>& so = w.stableOversized;
+	bool segOk = true;
+	for (size_t oi = 0; oi < ao.size(); ++oi)
+	{
+		segOk &= addSeg(ao[oi], 0, 0, 0, n);
+		segOk &= addSeg(ao[oi], 1, 0, (int)oi + 1, (int64_t)ao.size() - (int64_t)oi - 1);
+		segOk &= addSeg(ao[oi], 1, 1, 0, (int64_t)so.size());
+	}
+	for (size_t si = 0; si < so.size(); ++si)
+	{
+		segOk &= addSeg(so[si], 2, 0, 0, n);
+	}
+	if (!segOk)
+		return false;
+	int64_t total = gridTotal;
+	uint32_t* ovFlags = nullptr;
+	uint32_t* ovOffs = nullptr;
+	int64_t ovAdmitted = 0;
+	if (ovTotal > 0)
+	{
+		if (!w.u32c.ensure(ovTotal) || !w.u32d.ensure(ovTotal) || !ensure_prim_scratch(w, ovTotal))
+			return false;
+		ovFlags = w.u32c.p;
+		ovOffs = w.u32d.p;
+		k_over_flags<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags);
+		scan_exclusive(ovFlags, ovOffs, ovTotal, w.counters.p + CT_SEG_COUNT, w.primScratch.p, s);
+		if (!sync_counters(w))
+			return false;
+		ovAdmitted = w.hostCounters[CT_SEG_COUNT];
+		total += ovAdmitted;
+	}
+	if (total == 0)
+	{
+		return true;
+	}
+	if (!w.u64a.ensure(total) || !w.u64b.ensure(total))
+		return false;
+	if (gridTotal > 0)
+		k_pairs<<<grid_for(E), kBlock, 0, s>>>(P, 1, nullptr, w.u32b.p, w.u64a.p);
+	if (ovAdmitted > 0)
+		k_over_emit<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags, ovOffs, w.u64a.p + gridTotal);
+
+	// first admitted occurrence of each key wins
+	uint32_t csz = table_size_for(total);
+	if (!w.u64c.ensure(csz) || !w.u32a.ensure(std::max<int64_t>(csz, total)) || !w.u32b.ensure(total) ||
+		!w.u32c.ensure(total) || !ensure_prim_scratch(w, total))
+		return false;
+	launch_fill_u64(w.u64c.p, kEmptyKey, csz, s);
+	launch_fill_u32(w.u32a.p, 0xFFFFFFFFu, csz, s);
+	k_first_insert<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1);
+	k_first_flag<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1, w.u32b.p);
+	scan_exclusive(w.u32b.p, w.u32c.p, total, w.counters.p + CT_NEW_CONTACTS, w.primScratch.p, s);
+	k_compact_u64<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, w.u32b.p, w.u32c.p, total, w.u64b.p);
+	if (!sync_counters(w))
+		return false;
+	int32_t K = (int32_t)w.hostCounters[CT_NEW_CONTACTS];
+	if (K == 0)
+		return true;
+	if (!ensure_device_contacts(w, C0 + K))
+		return false;
+	k_create<<<grid_for(K), kBlock, 0, s>>>(body_view(w), contact_view(w), w.u64b.p, K, C0);
+	w.contactCount = C0 + K;
+	return true;
+}
+
+// ------------------------------------------------------------ removal
+
+__global__ void k_remove_flags(DBodies B, const int2* bodies, int32_t C, uint32_t* flags)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int2 ab = bodies[c];
+	uint32_t f = 0;
+	if ((B.sb[ab.x] & SB_AWAKE) || (B.sb[ab.y] & SB_AWAKE))
+	{
+		float4 aL = B.fatLo[ab.x], aH = B.fatHi[ab.x], bL = B.fatLo[ab.y], bH = B.fatHi[ab.y];
+		bool ov = aL.x <= bH.x && aL.y <= bH.y && aL.z <= bH.z && bL.x <= aH.x && bL.y <= aH.y && bL.z <= aH.z;
+		f = ov ? 0u : 1u;
+	}
+	flags[c] = f;
+}
+
+__global__ void k_hole_tail_flags(const uint32_t* rm, const float4* mat, int32_t C, int32_t newC, uint32_t* hole,
+								  uint32_t* tail, uint32_t* ev)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	uint32_t r = rm[c];
+	hole[c] = (r && c < newC) ? 1u : 0u;
+	tail[c] = (!r && c >= newC) ? 1u : 0u;
+	ev[c] = (r && (f2u(mat[c].w) & CF_TOUCHING)) ? 1u : 0u;
+}
+
+__global__ void k_make_moves(const uint32_t* hole, const uint32_t* holeOff, const uint32_t* tail,
+							 const uint32_t* tailOff, int32_t C, int2* moves)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	if (hole[c])
+		moves[holeOff[c]].x = c;
+	if (tail[c])
+		moves[tailOff[c]].y = c;
+}
+
+__global__ void k_apply_moves(DContacts C, const int2* moves, int32_t count)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	int32_t d = moves[k].x, s = moves[k].y;
+	C.bodies[d] = C.bodies[s];
+	C.mat[d] = C.mat[s];
+	C.normal[d] = C.normal[s];
+	C.cachePA[d] = C.cachePA[s];
+	C.cachePB[d] = C.cachePB[s];
+	C.cacheQA[d] = C.cacheQA[s];
+	C.cacheQB[d] = C.cacheQB[s];
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t pd = pidx(C, d, j), ps = pidx(C, s, j);
+		C.pA[pd] = C.pA[ps];
+		C.pB[pd] = C.pB[ps];
+		C.pJA[pd] = C.pJA[ps];
+		C.pJB[pd] = C.pJB[ps];
+		C.pRA[pd] = C.pRA[ps];
+		C.pRB[pd] = C.pRB[ps];
+		C.pFA[pd] = C.pFA[ps];
+		C.pFB[pd] = C.pFB[ps];
+		C.pSlip[pd] = C.pSlip[ps];
+	}
+}
+
+bool apply_moves(World& w, const int2* dMoves, int32_t count)
+{
+	if (count > 0)
+		k_apply_moves<<<grid_for(count), kBlock, 0, w.stream>>>(contact_view(w), dMoves, count);
+	return true;
+}
+
+__global__ void k_gather_events(const int2* bodies, const float4* mat, const int32_t* idx, int32_t count, int4* out)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	int32_t c = idx[k];
+	int2 b = bodies[c];
+	uint32_t cf = f2u(mat[c].w);
+	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, c);
+}
+
+bool remove_separated(World& w)
+{
+	hipStream_t s = w.stream;
+	int32_t C = w.contactCount;
+	if (C == 0)
+		return true;
+	if (!w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.u32c.ensure(C) || !w.u32d.ensure(C) || !w.i32a.ensure(C) ||
+		!w.i32b.ensure(C) || !w.i32c.ensure(C) || !ensure_prim_scratch(w, C))
+		return false;
+	uint32_t* rm = w.u32a.p;
+	k_remove_flags<<<grid_for(C), kBlock, 0, s>>>(body_view(w), w.cc.bodies.p, C, rm);
+	scan_exclusive(rm, w.u32b.p, C, w.counters.p + CT_REMOVED, w.primScratch.p, s);
+	if (!sync_counters(w))
+		return false;
+	int32_t R = (int32_t)w.hostCounters[CT_REMOVED];
+	if (R == 0)
+		return true;
+	int32_t newC = C - R;
+	uint32_t* hole = w.u32b.p;
+	uint32_t* tail = w.u32c.p;
+	uint32_t* ev = w.u32d.p;
+	k_hole_tail_flags<<<grid_for(C), kBlock, 0, s>>>(rm, w.cc.mat.p, C, newC, hole, tail, ev);
+	uint32_t* holeOff = (uint32_t*)w.i32a.p;
+	uint32_t* tailOff = (uint32_t*)w.i32b.p;
+	uint32_t* evOff = (uint32_t*)w.i32c.p;
+	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.primScratch.p, s);
+	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.primScratch.p, s);
+	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.primScratch.p, s);
+	if (!sync_counters(w))
+		return false;
+	int32_t H = (int32_t)w.hostCounters[CT_HOLES];
+	int32_t NE = (int32_t)w.hostCounters[CT_END_EVENTS];
+	if (H != (int32_t)w.hostCounters[CT_TAILS])
+	{
+		set_error("removal bookkeeping mismatch (%d holes, %u tails)", H, w.hostCounters[CT_TAILS]);
+		return false;
+	}
+	// end events of touching removed pairs, descending contact order
+	if (NE > 0)
+	{
+		if (!w.eventIdx.ensure(NE))
+			return false;
+		k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
+	}
+	std::vector<int4> evHost(NE);
+	if (NE > 0)
+	{
+		if (!w.u64a.ensure((size_t)NE * 2))
+			return false;
+		k_gather_events<<<grid_for(NE), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p, NE,
+														 (int4*)w.u64a.p);
+		hipMemcpyAsync(evHost.data(), w.u64a.p, NE * sizeof(int4), hipMemcpyDeviceToHost, s);
+	}
+	if (H > 0)
+	{
+		if (!w.moves.ensure(H))
+			return false;
+		k_make_moves<<<grid_for(C), kBlock, 0, s>>>(hole, holeOff, tail, tailOff, C, w.moves.p);
+		k_apply_moves<<<grid_for(H), kBlock, 0, s>>>(contact_view(w), w.moves.p, H);
+	}
+	if (!hip_ok(hipStreamSynchronize(s), "contact removal"))
+		return false;
+	for (int32_t k = NE - 1; k >= 0; --k)
+	{
+		const int4& e = evHost[k];
+		m3d_contact_event ce;
+		ce.bodyA = { e.x, w.bodies[e.x].generation };
+		ce.bodyB = { e.y, w.bodies[e.y].generation };
+		ce.isSensor = e.z != 0;
+		w.endEvents.push_back(ce);
+	}
+	w.contactCount = newC;
+	return true;
+}
+
+// ------------------------------------------------------------ host sync helpers
+
+__global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const uint8_t* flag)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= count)
+		return;
+	int2 ab = C.bodies[c];
+	if (!flag[ab.x] && !flag[ab.y])
+		return;
+	float4 ma = B.mat[ab.x], mb = B.mat[ab.y];
+	float4 m = C.mat[c];
+	m.x = sqrtf(ma.y * mb.y);
+	m.y = ma.z > mb.z ? ma.z : mb.z;
+	m.z = sqrtf(ma.w * mb.w);
+	C.mat[c] = m;
+}
+
+void launch_material_refresh(World& w, const uint8_t* dFlags)
+{
+	if (w.contactCount > 0)
+		k_material_refresh<<<grid_for(w.contactCount), kBlock, 0, w.stream>>>(body_view(w), contact_view(w),
+																			   w.contactCount, dFlags);
+}
+
+// stage layout: 18 float4 columns back to back, each `count` long, in the
+// order pos, rot, vel, angVel, prevPos, prevRot, prevVel, prevAngVel, invIL,
+// iw0, iw1, iw2, shape, mat, force, torque, fatLo, fatHi
+__global__ void k_upload_scatter(DBodies B, int32_t count, const int32_t* idx, const float4* st, const uint32_t* sf,
+								 const uint4* sfil, const uint8_t* sbp)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	int32_t i = idx[k];
+	float4* cols[18] = { B.pos, B.rot, B.vel, B.angVel, B.prevPos, B.prevRot, B.prevVel, B.prevAngVel, B.invIL,
+						 B.iw0, B.iw1, B.iw2, B.shape, B.mat, B.force, B.torque, B.fatLo, B.fatHi };
+	for (int c = 0; c < 18; ++c)
+		cols[c][i] = st[(int64_t)c * count + k];
+	B.flags[i] = sf[k];
+	B.filter[i] = sfil[k];
+	B.bp[i] = sbp[k];
+}
+
+void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const float4* stage, const uint32_t* stageFlags,
+						   const uint4* stageFilter, const uint8_t* stageBp)
+{
+	if (count > 0)
+		k_upload_scatter<<<grid_for(count), kBlock, 0, w.stream>>>(body_view(w), count, dIdx, stage, stageFlags,
+																	stageFilter, stageBp);
+}
+
+} // namespace mx

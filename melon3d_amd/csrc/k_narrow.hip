@@ -1,0 +1,218 @@
+// Step narrowphase: one thread per contact (src/world.cpp:1738-1815), then
+// touch transitions in contact order: begin/end events and the
+// wake-on-touch pass (src/world.cpp:1817-1845).
+//
+// The reference walks transitions serially and wakes a sleeping partner of
+// an awake body, so a body woken by an earlier transition can wake another
+// at a later one. The device computes the same result as the least fixpoint
+// of wakeAt[x] = min(initial, t : transition t touches x and its partner was
+// awake before t), iterated until no value changes.
+#include "mx_steps.h"
+
+namespace mx
+{
+
+DBodies make_body_view(World& w);
+DContacts make_contact_view(World& w);
+
+constexpr int kBlockN = 256;
+static inline unsigned gridn(int64_t n) { return (unsigned)((n + kBlockN - 1) / kBlockN); }
+
+__global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t count, float dt, int8_t* trans,
+												uint32_t* counters)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= count)
+		return;
+	int2 ab = C.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	if (!(B.sb[ia] & SB_AWAKE) && !(B.sb[ib] & SB_AWAKE))
+	{
+		trans[c] = 0;
+		return;
+	}
+	float4 mat = C.mat[c];
+	uint32_t cf = f2u(mat.w);
+	m3d_vec3 pa = f4_v3(B.pos[ia]), pb = f4_v3(B.pos[ib]);
+	m3d_quat qa = f4_q(B.rot[ia]), qb = f4_q(B.rot[ib]);
+	int32_t pointCount = f2i(C.normal[c].w);
+
+	if (cf & CF_TUNNEL_GUARD)
+	{
+		Manifold m;
+		load_manifold(C, c, m);
+		if (tracked_min_sep(pa, qa, pb, qb, m) < -kLinearSlop)
+		{
+			trans[c] = 0;
+			return;
+		}
+		cf &= ~CF_TUNNEL_GUARD;
+	}
+
+	bool regen = !(cf & CF_HAS_CACHE);
+	if (!regen)
+	{
+		regen = pose_changed(pa, qa, f4_v3(C.cachePA[c]), f4_q(C.cacheQA[c])) ||
+				pose_changed(pb, qb, f4_v3(C.cachePB[c]), f4_q(C.cacheQB[c]));
+	}
+	if (regen)
+	{
+		atomicAdd(&counters[CT_REGEN], 1u);
+		Manifold old;
+		load_manifold(C, c, old);
+		Manifold m = old;
+		Shape sA = load_shape(B, ia), sB = load_shape(B, ib);
+		float relSpeed = m3d_length(m3d_sub(f4_v3(B.vel[ia]), f4_v3(B.vel[ib])));
+		float margin = kSpeculativeBase + dt * relSpeed;
+		m3d_transform xfA = { pa, qa }, xfB = { pb, qb };
+		collide_shapes(sA, xfA, sB, xfB, margin, &m);
+		inherit_friction_anchors(old, m, pa, qa, pb, qb, sA.type == M3D_SHAPE_BOX && sB.type == M3D_SHAPE_BOX);
+		cf |= CF_HAS_CACHE;
+		C.cachePA[c] = v3_f4(pa, 0.0f);
+		C.cacheQA[c] = q_f4(qa);
+		C.cachePB[c] = v3_f4(pb, 0.0f);
+		C.cacheQB[c] = q_f4(qb);
+		set_separation_offsets(m, pa, qa, pb, qb);
+		store_manifold(C, c, m);
+		pointCount = m.pointCount;
+	}
+	bool was = (cf & CF_TOUCHING) != 0;
+	bool now = pointCount > 0;
+	cf = now ? (cf | CF_TOUCHING) : (cf & ~CF_TOUCHING);
+	trans[c] = (int8_t)(now && !was ? 1 : (!now && was ? -1 : 0));
+	C.mat[c] = make_float4(mat.x, mat.y, mat.z, u2f(cf));
+}
+
+__global__ void k_trans_flags(const int8_t* trans, int32_t count, uint32_t* flags)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c < count)
+		flags[c] = trans[c] != 0 ? 1u : 0u;
+}
+
+__global__ void k_trans_compact(const int8_t* trans, const uint32_t* offs, int32_t count, int32_t* list)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c < count && trans[c] != 0)
+		list[offs[c]] = c;
+}
+
+__global__ void k_trans_info(const int2* bodies, const float4* mat, const int8_t* trans, const int32_t* list,
+							 int32_t T, int4* out)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= T)
+		return;
+	int32_t c = list[t];
+	int2 b = bodies[c];
+	uint32_t cf = f2u(mat[c].w);
+	out[t] = make_int4(b.x, b.y, (int)trans[c] * (((cf & CF_SENSOR) != 0) ? 2 : 1), c);
+}
+
+__global__ void k_wake_init(const uint32_t* flags, int32_t n, int32_t* wakeAt)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n)
+		wakeAt[i] = (flags[i] & BF_AWAKE) ? -1 : 0x7FFFFFFF;
+}
+
+// info.z: +1 begin, +2 begin (sensor), -1/-2 end
+__global__ void k_wake_iter(const int4* info, int32_t T, const uint32_t* flags, int32_t* wakeAt, uint32_t* changed)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= T)
+		return;
+	int4 e = info[t];
+	if (e.z != 1)
+		return;
+	int32_t a = e.x, b = e.y;
+	int32_t wa = __atomic_load_n(&wakeAt[a], __ATOMIC_RELAXED);
+	int32_t wb = __atomic_load_n(&wakeAt[b], __ATOMIC_RELAXED);
+	bool awA = wa < t, awB = wb < t;
+	if (awA && !awB)
+	{
+		if (bf_type(flags[b]) != M3D_BODY_STATIC && atomicMin(&wakeAt[b], t) > t)
+			atomicAdd(changed, 1u);
+	}
+	else if (awB && !awA)
+	{
+		if (bf_type(flags[a]) != M3D_BODY_STATIC && atomicMin(&wakeAt[a], t) > t)
+			atomicAdd(changed, 1u);
+	}
+}
+
+// WakeBodyInternal for bodies woken by a transition; src/world.cpp:121-140
+__global__ void k_wake_apply(DBodies B, int32_t n, const int32_t* wakeAt, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t wv = wakeAt[i];
+	if (wv < 0 || wv == 0x7FFFFFFF)
+		return;
+	B.flags[i] |= BF_AWAKE;
+	B.sb[i] |= SB_AWAKE;
+	float4 a = B.iw1[i];
+	a.w = 0.0f; // sleepTime
+	B.iw1[i] = a;
+	float4 b = B.iw2[i];
+	b.w = 4.0f * kSleepEnergyThresh; // smoothedEnergy
+	B.iw2[i] = b;
+	counters[CT_STABLE_DIRTY] = 1u;
+}
+
+bool narrowphase(World& w, float dt)
+{
+	hipStream_t s = w.stream;
+	int32_t C = w.contactCount;
+	int32_t n = w.deviceBodyCount;
+	if (C == 0)
+		return true;
+	if (!w.u8a.ensure(C) || !w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.i32a.ensure(C) ||
+		!w.primScratch.ensure(scan_scratch_bytes(C) + 1024))
+		return false;
+	int8_t* trans = (int8_t*)w.u8a.p;
+	k_narrow<<<gridn(C), kBlockN, 0, s>>>(make_body_view(w), make_contact_view(w), C, dt, trans, w.counters.p);
+	k_trans_flags<<<gridn(C), kBlockN, 0, s>>>(trans, C, w.u32a.p);
+	scan_exclusive(w.u32a.p, w.u32b.p, C, w.counters.p + CT_TRANSITIONS, w.primScratch.p, s);
+	if (!sync_counters(w))
+		return false;
+	int32_t T = (int32_t)w.hostCounters[CT_TRANSITIONS];
+	if (T == 0)
+		return true;
+	if (!w.i32b.ensure(T) || !w.u64a.ensure((size_t)T * 2) || !w.i32c.ensure(n))
+		return false;
+	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(trans, w.u32b.p, C, w.i32b.p);
+	int4* info = (int4*)w.u64a.p;
+	k_trans_info<<<gridn(T), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.i32b.p, T, info);
+	std::vector<int4> host(T);
+	hipMemcpyAsync(host.data(), info, T * sizeof(int4), hipMemcpyDeviceToHost, s);
+	// wake-on-touch fixpoint
+	int32_t* wakeAt = w.i32c.p;
+	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
+	for (int round = 0; round < 1 << 20; ++round)
+	{
+		hipMemsetAsync(w.counters.p + CT_CHANGED, 0, sizeof(uint32_t), s);
+		k_wake_iter<<<gridn(T), kBlockN, 0, s>>>(info, T, w.bc.flags.p, wakeAt, w.counters.p + CT_CHANGED);
+		if (!sync_counters(w))
+			return false;
+		if (w.hostCounters[CT_CHANGED] == 0)
+			break;
+	}
+	k_wake_apply<<<gridn(n), kBlockN, 0, s>>>(make_body_view(w), n, wakeAt, w.counters.p);
+	for (int32_t t = 0; t < T; ++t)
+	{
+		const int4& e = host[t];
+		m3d_contact_event ce;
+		ce.bodyA = { e.x, w.bodies[e.x].generation };
+		ce.bodyB = { e.y, w.bodies[e.y].generation };
+		ce.isSensor = e.z == 2 || e.z == -2;
+		if (e.z > 0)
+			w.beginEvents.push_back(ce);
+		else
+			w.endEvents.push_back(ce);
+	}
+	return true;
+}
+
+} // namespace mx

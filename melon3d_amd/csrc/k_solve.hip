@@ -1,0 +1,784 @@
+// Islands, island solve, sleeping and the end-of-step body passes.
+//
+//   islands   lock-free union-find (hooking always attaches the larger root
+//             under the smaller, so every root is its component's smallest
+//             body index) over touching non-sensor dynamic-dynamic contacts
+//             and dynamic-dynamic joints; islands numbered in order of their
+//             smallest body, as the reference's ascending scan assigns them
+//             (src/world.cpp:1868-1958)
+//   lists     island body/contact lists by a stable sort on the island id,
+//             so each list stays ascending (src/world.cpp:1974-2010)
+//   solve     one lane per awake island runs the reference island solve
+//             (serial Gauss-Seidel, or greedy coloring with packet-form
+//             normals and serial overflow for islands with >= 300 contacts)
+//             (src/xpbd.cpp:898-974, :1167-1369)
+//   sleep     low-pass energy, timers and whole-island sleep
+//             (src/world.cpp:2060-2108)
+#include "mx_steps.h"
+
+#include <algorithm>
+
+namespace mx
+{
+
+DBodies make_body_view(World& w);
+DContacts make_contact_view(World& w);
+
+constexpr int kBlockS = 256;
+static inline unsigned grids(int64_t n) { return (unsigned)((n + kBlockS - 1) / kBlockS); }
+
+// ----------------------------------------------------------- islands
+
+__global__ void k_any_awake_dyn(const uint8_t* sb, int32_t n, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n && (sb[i] & (SB_DYNAMIC | SB_AWAKE)) == (SB_DYNAMIC | SB_AWAKE))
+		counters[CT_ANY_AWAKE_DYN] = 1u;
+}
+
+__global__ void k_uf_init(int32_t* parent, int32_t n)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n)
+		parent[i] = i;
+}
+
+__device__ __forceinline__ int32_t uf_find(int32_t* parent, int32_t x)
+{
+	for (;;)
+	{
+		int32_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+		if (p == x)
+			return x;
+		int32_t gp = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+		if (gp != p)
+			__atomic_store_n(&parent[x], gp, __ATOMIC_RELAXED); // halving: still an ancestor
+		x = p;
+	}
+}
+
+__device__ __forceinline__ void uf_union(int32_t* parent, int32_t a, int32_t b)
+{
+	for (;;)
+	{
+		int32_t ra = uf_find(parent, a);
+		int32_t rb = uf_find(parent, b);
+		if (ra == rb)
+			return;
+		int32_t lo = ra < rb ? ra : rb;
+		int32_t hi = ra < rb ? rb : ra;
+		if (atomicCAS(&parent[hi], hi, lo) == hi)
+			return;
+	}
+}
+
+__global__ void k_uf_contacts(const int2* bodies, const float4* mat, const uint8_t* sb, int32_t C, int32_t* parent)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	uint32_t cf = f2u(mat[c].w);
+	if (!(cf & CF_TOUCHING) || (cf & CF_SENSOR))
+		return;
+	int2 ab = bodies[c];
+	if ((sb[ab.x] & SB_DYNAMIC) && (sb[ab.y] & SB_DYNAMIC))
+		uf_union(parent, ab.x, ab.y);
+}
+
+__global__ void k_uf_joints(const DJoint* J, int32_t nJ, const uint32_t* flags, int32_t* parent)
+{
+	int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j >= nJ)
+		return;
+	const DJoint& jt = J[j];
+	if (jt.inUse && bf_type(flags[jt.bodyA]) == M3D_BODY_DYNAMIC && bf_type(flags[jt.bodyB]) == M3D_BODY_DYNAMIC)
+		uf_union(parent, jt.bodyA, jt.bodyB);
+}
+
+__global__ void k_uf_roots(int32_t* parent, const uint8_t* sb, int32_t n, uint32_t* rootFlag)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t r = uf_find(parent, i);
+	parent[i] = r;
+	rootFlag[i] = ((sb[i] & SB_DYNAMIC) && r == i) ? 1u : 0u;
+}
+
+__global__ void k_island_of(const int32_t* parent, const uint8_t* sb, const uint32_t* rootRank, int32_t n,
+							int32_t* islandOf, uint32_t* islandAwake)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint8_t f = sb[i];
+	if (!(f & SB_DYNAMIC))
+	{
+		islandOf[i] = -1;
+		return;
+	}
+	int32_t isl = (int32_t)rootRank[parent[i]];
+	islandOf[i] = isl;
+	if (f & SB_AWAKE)
+		islandAwake[isl] = 1u;
+}
+
+// WakeBodyInternal for every not-yet-awake body of an awake island
+__global__ void k_island_wake(DBodies B, int32_t n, const int32_t* islandOf, const uint32_t* islandAwake,
+							  uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t isl = islandOf[i];
+	if (isl < 0 || !islandAwake[isl])
+		return;
+	if (B.flags[i] & BF_AWAKE)
+		return;
+	B.flags[i] |= BF_AWAKE;
+	B.sb[i] |= SB_AWAKE;
+	float4 a = B.iw1[i];
+	a.w = 0.0f;
+	B.iw1[i] = a;
+	float4 b = B.iw2[i];
+	b.w = 4.0f * kSleepEnergyThresh;
+	B.iw2[i] = b;
+	counters[CT_STABLE_DIRTY] = 1u;
+}
+
+// solver contacts: touching, non-sensor, dynamic body's island awake
+__global__ void k_contact_island(const int2* bodies, const float4* mat, const float4* normal, const uint8_t* sb,
+								 const int32_t* islandOf, const uint32_t* islandAwake, int32_t C, uint64_t* keys,
+								 uint32_t* vals, uint32_t* flag, uint32_t* islandContacts, uint32_t* counters)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	uint32_t cf = f2u(mat[c].w);
+	uint32_t f = 0;
+	int32_t isl = -1;
+	if (cf & CF_TOUCHING)
+	{
+		atomicAdd(&counters[CT_TOUCHING], 1u);
+		if (!(cf & CF_SENSOR))
+		{
+			int2 ab = bodies[c];
+			int32_t dyn = (sb[ab.x] & SB_DYNAMIC) ? ab.x : ab.y;
+			isl = islandOf[dyn];
+			if (isl >= 0 && islandAwake[isl])
+			{
+				f = 1;
+				atomicAdd(&islandContacts[isl], 1u);
+				atomicAdd(&counters[CT_SOLVED_CONTACTS], 1u);
+			}
+		}
+	}
+	flag[c] = f;
+	keys[c] = f ? (uint64_t)isl : 0ull;
+	vals[c] = (uint32_t)c;
+}
+
+__global__ void k_body_island_keys(const int32_t* islandOf, const uint32_t* islandAwake, int32_t n, uint64_t* keys,
+								   uint32_t* vals, uint32_t* flag, uint32_t* islandBodies)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t isl = islandOf[i];
+	uint32_t f = (isl >= 0 && islandAwake[isl]) ? 1u : 0u;
+	flag[i] = f;
+	keys[i] = f ? (uint64_t)isl : 0ull;
+	vals[i] = (uint32_t)i;
+	if (f)
+		atomicAdd(&islandBodies[isl], 1u);
+}
+
+__global__ void k_compact_pairs(const uint64_t* keys, const uint32_t* vals, const uint32_t* flag, const uint32_t* offs,
+								int64_t n, uint64_t* ko, uint32_t* vo)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flag[i])
+	{
+		ko[offs[i]] = keys[i];
+		vo[offs[i]] = vals[i];
+	}
+}
+
+__global__ void k_joint_island(const DJoint* J, int32_t nJ, const uint32_t* flags, const int32_t* islandOf,
+							   const uint32_t* islandAwake, int32_t* jointIsland)
+{
+	int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j >= nJ)
+		return;
+	const DJoint& jt = J[j];
+	int32_t isl = -1;
+	if (jt.inUse)
+	{
+		int32_t dyn = bf_type(flags[jt.bodyA]) == M3D_BODY_DYNAMIC ? jt.bodyA : jt.bodyB;
+		if (bf_type(flags[dyn]) == M3D_BODY_DYNAMIC)
+		{
+			int32_t k = islandOf[dyn];
+			if (k >= 0 && islandAwake[k])
+				isl = k;
+		}
+	}
+	jointIsland[j] = isl;
+}
+
+// ------------------------------------------------------- island solve
+
+struct SolveCtx
+{
+	DBodies B;
+	DContacts C;
+	DJoint* J;
+	int32_t nJ;
+	const int32_t* jointIsland;
+	const uint32_t* bodyList;
+	const uint32_t* bodyStart; // per island, exclusive scan (size islands+1)
+	const uint32_t* contactList;
+	const uint32_t* contactStart;
+	uint32_t* colorMask; // per body
+	int32_t* contactColor; // per contact
+	m3d_vec3 gravity;
+	float h, invH, gravityLen, maxCorrection;
+	int32_t iterations;
+	int32_t substeps;
+};
+
+__device__ __forceinline__ void load_wb(const DBodies& B, int32_t i, WB& b)
+{
+	float4 P = B.pos[i];
+	b.p = f4_v3(P);
+	b.invMass = P.w;
+	b.q = f4_q(B.rot[i]);
+	b.v = f4_v3(B.vel[i]);
+	b.w = f4_v3(B.angVel[i]);
+	b.iw = load_iw(B, i);
+}
+
+__device__ __forceinline__ void store_pose(const DBodies& B, int32_t i, const WB& b)
+{
+	if (b.invMass == 0.0f)
+		return;
+	B.pos[i] = v3_f4(b.p, b.invMass);
+	B.rot[i] = q_f4(b.q);
+}
+
+__device__ __forceinline__ void store_vel(const DBodies& B, int32_t i, const WB& b)
+{
+	if (b.invMass == 0.0f)
+		return;
+	float4 V = B.vel[i], W = B.angVel[i];
+	B.vel[i] = v3_f4(b.v, V.w);
+	B.angVel[i] = v3_f4(b.w, W.w);
+}
+
+__device__ void contact_normals_serial(const SolveCtx& X, int32_t c)
+{
+	int2 ab = X.C.bodies[c];
+	float4 PA = X.B.pos[ab.x], PB = X.B.pos[ab.y];
+	m3d_vec3 pa = f4_v3(PA), pb = f4_v3(PB);
+	m3d_quat qa = f4_q(X.B.rot[ab.x]), qb = f4_q(X.B.rot[ab.y]);
+	solve_normals_serial(X.C, c, pa, qa, PA.w, pb, qb, PB.w, X.maxCorrection);
+	if (PA.w != 0.0f)
+	{
+		X.B.pos[ab.x] = v3_f4(pa, PA.w);
+		X.B.rot[ab.x] = q_f4(qa);
+	}
+	if (PB.w != 0.0f)
+	{
+		X.B.pos[ab.y] = v3_f4(pb, PB.w);
+		X.B.rot[ab.y] = q_f4(qb);
+	}
+}
+
+__device__ void contact_normals_packet(const SolveCtx& X, int32_t c, int32_t maxPoints)
+{
+	int2 ab = X.C.bodies[c];
+	float4 PA = X.B.pos[ab.x], PB = X.B.pos[ab.y];
+	m3d_vec3 pa = f4_v3(PA), pb = f4_v3(PB);
+	m3d_quat qa = f4_q(X.B.rot[ab.x]), qb = f4_q(X.B.rot[ab.y]);
+	solve_normals_packet(X.C, c, maxPoints, pa, qa, PA.w, pb, qb, PB.w, X.maxCorrection);
+	if (PA.w > 0.0f)
+	{
+		X.B.pos[ab.x] = v3_f4(pa, PA.w);
+		X.B.rot[ab.x] = q_f4(qa);
+	}
+	if (PB.w > 0.0f)
+	{
+		X.B.pos[ab.y] = v3_f4(pb, PB.w);
+		X.B.rot[ab.y] = q_f4(qb);
+	}
+}
+
+__device__ void contact_friction(const SolveCtx& X, int32_t c)
+{
+	int2 ab = X.C.bodies[c];
+	bool boxBox = bf_shape(X.B.flags[ab.x]) == M3D_SHAPE_BOX && bf_shape(X.B.flags[ab.y]) == M3D_SHAPE_BOX;
+	if (!boxBox)
+		return;
+	WB a, b;
+	load_wb(X.B, ab.x, a);
+	load_wb(X.B, ab.y, b);
+	apply_friction(X.C, c, boxBox, a, b);
+	store_pose(X.B, ab.x, a);
+	store_pose(X.B, ab.y, b);
+}
+
+__device__ void contact_velocity(const SolveCtx& X, int32_t c)
+{
+	int2 ab = X.C.bodies[c];
+	WB a, b;
+	load_wb(X.B, ab.x, a);
+	load_wb(X.B, ab.y, b);
+	m3d_vec3 pvA = f4_v3(X.B.prevVel[ab.x]), pwA = f4_v3(X.B.prevAngVel[ab.x]);
+	m3d_vec3 pvB = f4_v3(X.B.prevVel[ab.y]), pwB = f4_v3(X.B.prevAngVel[ab.y]);
+	VelShape sa, sbv;
+	sa.type = bf_shape(X.B.flags[ab.x]);
+	sa.radius = X.B.shape[ab.x].w;
+	sa.halfHeight = X.B.mat[ab.x].x;
+	sbv.type = bf_shape(X.B.flags[ab.y]);
+	sbv.radius = X.B.shape[ab.y].w;
+	sbv.halfHeight = X.B.mat[ab.y].x;
+	solve_velocity(X.C, c, X.h, X.gravityLen, a, b, pvA, pwA, pvB, pwB, sa, sbv);
+	store_vel(X.B, ab.x, a);
+	store_vel(X.B, ab.y, b);
+}
+
+__device__ void joint_op(const SolveCtx& X, int32_t j, int op)
+{
+	DJoint& jt = X.J[j];
+	WB a, b;
+	load_wb(X.B, jt.bodyA, a);
+	load_wb(X.B, jt.bodyB, b);
+	if (op == 0)
+	{
+		prepare_joint(jt, a, b);
+		return;
+	}
+	if (op == 1)
+	{
+		solve_joint_position(jt, a, b, X.h);
+		store_pose(X.B, jt.bodyA, a);
+		store_pose(X.B, jt.bodyB, b);
+	}
+	else
+	{
+		solve_joint_velocity(jt, a, b, X.h);
+		store_vel(X.B, jt.bodyA, a);
+		store_vel(X.B, jt.bodyB, b);
+	}
+}
+
+// One lane solves one island with the reference's exact serial schedule.
+__global__ void k_solve_islands_serial(SolveCtx X, const int32_t* islands, int32_t nIslands)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= nIslands)
+		return;
+	int32_t isl = islands[t];
+	uint32_t b0 = X.bodyStart[isl], b1 = X.bodyStart[isl + 1];
+	uint32_t c0 = X.contactStart[isl], c1 = X.contactStart[isl + 1];
+	bool colored = (int32_t)(c1 - c0) >= kBigIslandContacts;
+	bool anyJoint = false;
+	for (int32_t j = 0; j < X.nJ; ++j)
+		anyJoint |= X.jointIsland[j] == isl;
+
+	if (anyJoint)
+	{
+		for (uint32_t k = b0; k < b1; ++k)
+		{
+			int32_t bi = (int32_t)X.bodyList[k];
+			m3d_mat3 m = inv_inertia_world(f4_q(X.B.rot[bi]), f4_v3(X.B.invIL[bi]));
+			store_iw(X.B, bi, m);
+		}
+		for (int32_t j = 0; j < X.nJ; ++j)
+			if (X.jointIsland[j] == isl)
+				joint_op(X, j, 0);
+	}
+
+	int32_t colorCount = 0;
+	if (colored)
+	{
+		// greedy first-fit in island contact order; src/xpbd.cpp:1179-1235
+		for (uint32_t k = b0; k < b1; ++k)
+			X.colorMask[X.bodyList[k]] = 0;
+		for (uint32_t k = c0; k < c1; ++k)
+		{
+			int32_t c = (int32_t)X.contactList[k];
+			int2 ab = X.C.bodies[c];
+			bool dA = X.B.pos[ab.x].w > 0.0f, dB = X.B.pos[ab.y].w > 0.0f;
+			uint32_t used = 0;
+			if (dA)
+				used |= X.colorMask[ab.x];
+			if (dB)
+				used |= X.colorMask[ab.y];
+			uint32_t fb = ~used;
+			int32_t color = fb != 0 ? __builtin_ctz(fb) : kMaxGraphColors;
+			if (color >= kMaxGraphColors)
+			{
+				X.contactColor[c] = kMaxGraphColors;
+				continue;
+			}
+			X.contactColor[c] = color;
+			if (dA)
+				X.colorMask[ab.x] |= 1u << color;
+			if (dB)
+				X.colorMask[ab.y] |= 1u << color;
+			colorCount = color + 1 > colorCount ? color + 1 : colorCount;
+		}
+	}
+
+	for (int sub = 0; sub < X.substeps; ++sub)
+	{
+		for (uint32_t k = b0; k < b1; ++k)
+			integrate_body(X.B, (int32_t)X.bodyList[k], X.h, X.gravity);
+		for (uint32_t k = c0; k < c1; ++k)
+			prepare_contact(X.B, X.C, (int32_t)X.contactList[k], X.h);
+		for (int32_t j = 0; j < X.nJ; ++j)
+		{
+			if (X.jointIsland[j] == isl)
+			{
+				X.J[j].lambda = 0.0f;
+				X.J[j].motorLambda = 0.0f;
+			}
+		}
+		for (int it = 0; it < X.iterations; ++it)
+		{
+			for (int32_t j = 0; j < X.nJ; ++j)
+				if (X.jointIsland[j] == isl)
+					joint_op(X, j, 1);
+			if (!colored)
+			{
+				for (uint32_t k = c0; k < c1; ++k)
+					contact_normals_serial(X, (int32_t)X.contactList[k]);
+				continue;
+			}
+			for (int32_t color = 0; color < colorCount; ++color)
+			{
+				// packets of 8 consecutive bucket members
+				int32_t lane[8];
+				int32_t nl = 0;
+				for (uint32_t k = c0; k <= c1; ++k)
+				{
+					bool end = k == c1;
+					if (!end)
+					{
+						int32_t c = (int32_t)X.contactList[k];
+						if (X.contactColor[c] != color)
+							continue;
+						lane[nl++] = c;
+					}
+					if (nl == 8 || (end && nl > 0))
+					{
+						int32_t mp = 0;
+						for (int l = 0; l < nl; ++l)
+						{
+							int32_t pc = f2i(X.C.normal[lane[l]].w);
+							mp = pc > mp ? pc : mp;
+						}
+						for (int l = 0; l < nl; ++l)
+							contact_normals_packet(X, lane[l], mp);
+						nl = 0;
+					}
+				}
+			}
+			for (uint32_t k = c0; k < c1; ++k)
+			{
+				int32_t c = (int32_t)X.contactList[k];
+				if (X.contactColor[c] == kMaxGraphColors)
+					contact_normals_serial(X, c);
+			}
+		}
+		if (!colored)
+		{
+			for (uint32_t k = c0; k < c1; ++k)
+				contact_friction(X, (int32_t)X.contactList[k]);
+		}
+		else
+		{
+			for (int32_t color = 0; color <= kMaxGraphColors; ++color)
+			{
+				if (color == colorCount && color < kMaxGraphColors)
+					color = kMaxGraphColors;
+				for (uint32_t k = c0; k < c1; ++k)
+				{
+					int32_t c = (int32_t)X.contactList[k];
+					if (X.contactColor[c] == color)
+						contact_friction(X, c);
+				}
+			}
+		}
+		for (uint32_t k = b0; k < b1; ++k)
+			reconstruct_velocity(X.B, (int32_t)X.bodyList[k], X.invH);
+		if (!colored)
+		{
+			for (uint32_t k = c0; k < c1; ++k)
+				contact_velocity(X, (int32_t)X.contactList[k]);
+		}
+		else
+		{
+			for (int32_t color = 0; color <= kMaxGraphColors; ++color)
+			{
+				if (color == colorCount && color < kMaxGraphColors)
+					color = kMaxGraphColors;
+				for (uint32_t k = c0; k < c1; ++k)
+				{
+					int32_t c = (int32_t)X.contactList[k];
+					if (X.contactColor[c] == color)
+						contact_velocity(X, c);
+				}
+			}
+		}
+		for (int32_t j = 0; j < X.nJ; ++j)
+			if (X.jointIsland[j] == isl)
+				joint_op(X, j, 2);
+	}
+}
+
+// ------------------------------------------------------------- sleep
+
+__global__ void k_sleep_prep(const uint32_t* flags, const int32_t* islandOf, const uint32_t* islandAwake, int32_t n,
+							 int32_t* firstNoSleep)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t isl = islandOf[i];
+	if (isl < 0 || !islandAwake[isl])
+		return;
+	if (!(flags[i] & BF_ENABLE_SLEEP))
+		atomicMin(&firstNoSleep[isl], i);
+}
+
+__global__ void k_sleep_update(DBodies B, const int32_t* islandOf, const uint32_t* islandAwake, int32_t n, float dt,
+							   const int32_t* firstNoSleep, uint32_t* minSleepBits)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t isl = islandOf[i];
+	if (isl < 0 || !islandAwake[isl] || i >= firstNoSleep[isl])
+		return;
+	m3d_vec3 v = f4_v3(B.vel[i]), w = f4_v3(B.angVel[i]);
+	float4 T = B.iw1[i], E = B.iw2[i];
+	float sleepTime = T.w, se = E.w;
+	float energy = m3d_length_sq(v) + m3d_length_sq(w);
+	se += kSleepEnergyAlpha * (energy - se);
+	if (se > kSleepWakeFactor * kSleepEnergyThresh)
+		sleepTime = 0.0f;
+	else if (se < kSleepEnergyThresh)
+		sleepTime += dt;
+	else
+		sleepTime += 0.5f * dt;
+	T.w = sleepTime;
+	E.w = se;
+	B.iw1[i] = T;
+	B.iw2[i] = E;
+	atomicMin(&minSleepBits[isl], f2u(sleepTime));
+}
+
+__global__ void k_sleep_apply(DBodies B, const int32_t* islandOf, const uint32_t* islandAwake, int32_t n,
+							  const int32_t* firstNoSleep, const uint32_t* minSleepBits, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	int32_t isl = islandOf[i];
+	if (isl < 0 || !islandAwake[isl])
+		return;
+	if (firstNoSleep[isl] != 0x7FFFFFFF)
+		return;
+	if (!(u2f(minSleepBits[isl]) >= kTimeToSleep))
+		return;
+	B.flags[i] &= ~BF_AWAKE;
+	B.sb[i] &= (uint8_t)~SB_AWAKE;
+	float4 V = B.vel[i], W = B.angVel[i];
+	B.vel[i] = make_float4(0.0f, 0.0f, 0.0f, V.w);
+	B.angVel[i] = make_float4(0.0f, 0.0f, 0.0f, W.w);
+	counters[CT_STABLE_DIRTY] = 1u;
+}
+
+// ---------------------------------------------------------- end of step
+
+__global__ void k_finish(DBodies B, int32_t n, float dt, int clearForces, uint32_t* counters)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint8_t f = B.sb[i];
+	if ((f & SB_KINEMATIC) && (f & SB_AWAKE))
+	{
+		float4 P = B.pos[i];
+		m3d_vec3 p = m3d_mul_add(f4_v3(P), dt, f4_v3(B.vel[i]));
+		m3d_quat q = m3d_quat_integrate(f4_q(B.rot[i]), f4_v3(B.angVel[i]), dt);
+		B.pos[i] = v3_f4(p, P.w);
+		B.rot[i] = q_f4(q);
+	}
+	if (clearForces && (f & SB_DYNAMIC))
+	{
+		B.force[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		B.torque[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
+	if (f & SB_IN_USE)
+	{
+		atomicAdd(&counters[CT_BODY_COUNT], 1u);
+		if (f & SB_AWAKE)
+			atomicAdd(&counters[CT_AWAKE_BODIES], 1u);
+	}
+}
+
+__global__ void k_mark_flag_list(const uint32_t* flag, int32_t n, const uint32_t* offs, int32_t* out)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n && flag[i])
+		out[offs[i]] = i;
+}
+
+bool islands_and_solve(World& w, float dt, int substeps)
+{
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	int32_t C = w.contactCount;
+	w.lastHadIslands = false;
+	w.lastIslandCount = 0;
+	if (n == 0)
+		return true;
+	k_any_awake_dyn<<<grids(n), kBlockS, 0, s>>>(w.bc.sb.p, n, w.counters.p);
+	if (!sync_counters(w))
+		return false;
+	if (!w.hostCounters[CT_ANY_AWAKE_DYN])
+		return true;
+
+	int64_t m = std::max<int64_t>(n, C);
+	if (!w.i32a.ensure(n) || !w.u32a.ensure(m + 1) || !w.u32b.ensure(m + 1) || !w.islandOf.ensure(n) ||
+		!w.primScratch.ensure(std::max(scan_scratch_bytes(m + 1), sort_scratch_bytes(m + 1)) + 1024))
+		return false;
+	int32_t* parent = w.i32a.p;
+	k_uf_init<<<grids(n), kBlockS, 0, s>>>(parent, n);
+	if (C > 0)
+		k_uf_contacts<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, C, parent);
+	if (w.djointCount > 0)
+		k_uf_joints<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p, parent);
+	k_uf_roots<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, n, w.u32a.p);
+	scan_exclusive(w.u32a.p, w.u32b.p, n, w.counters.p + CT_ISLANDS, w.primScratch.p, s);
+	if (!sync_counters(w))
+		return false;
+	int32_t I = (int32_t)w.hostCounters[CT_ISLANDS];
+	w.lastHadIslands = true;
+	w.lastIslandCount = I;
+	if (I == 0)
+		return true;
+	if (!w.islandAwakeD.ensure(I + 1) || !w.u32d.ensure(2 * (size_t)(I + 1)))
+		return false;
+	uint32_t* islandAwake = w.islandAwakeD.p;
+	launch_fill_u32(islandAwake, 0u, I, s);
+	k_island_of<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, w.u32b.p, n, w.islandOf.p, islandAwake);
+	k_island_wake<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), n, w.islandOf.p, islandAwake, w.counters.p);
+
+	// island body lists (ascending) and contact lists (ascending)
+	uint32_t* islandBodies = w.u32d.p;          // counts, then exclusive starts
+	uint32_t* islandContacts = w.u32d.p + I + 1;
+	launch_fill_u32(islandBodies, 0u, 2 * (int64_t)(I + 1), s);
+	int bitsI = 1;
+	while ((1ll << bitsI) <= I)
+		++bitsI;
+	if (!w.u64a.ensure(m) || !w.u64b.ensure(m) || !w.u64c.ensure(m) || !w.sortValsAlt.ensure(m) ||
+		!w.i32b.ensure(n) || !w.i32c.ensure(C + 1) || !w.i32d.ensure(m) ||
+		!w.i32e.ensure(std::max<int32_t>(w.djointCount, 1)) || !w.contactColor.ensure(std::max<int32_t>(C, 1)))
+		return false;
+	uint32_t* flag = w.u32a.p;
+	uint32_t* offs = w.u32b.p;
+	uint32_t* bodyListVals = (uint32_t*)w.i32b.p;
+	uint32_t* contactListVals = (uint32_t*)w.i32c.p;
+	// bodies
+	k_body_island_keys<<<grids(n), kBlockS, 0, s>>>(w.islandOf.p, islandAwake, n, w.u64a.p, (uint32_t*)w.i32d.p,
+													 flag, islandBodies);
+	scan_exclusive(flag, offs, n, w.counters.p + CT_INTEGRATED, w.primScratch.p, s);
+	k_compact_pairs<<<grids(n), kBlockS, 0, s>>>(w.u64a.p, (uint32_t*)w.i32d.p, flag, offs, n, w.u64b.p,
+												  bodyListVals);
+	// contacts
+	launch_fill_i32(w.contactColor.p, -1, C, s);
+	if (C > 0)
+	{
+		k_contact_island<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.cc.normal.p, w.bc.sb.p,
+													   w.islandOf.p, islandAwake, C, w.u64a.p, (uint32_t*)w.i32d.p,
+													   flag, islandContacts, w.counters.p);
+		scan_exclusive(flag, offs, C, w.counters.p + CT_SEG_COUNT, w.primScratch.p, s);
+		k_compact_pairs<<<grids(C), kBlockS, 0, s>>>(w.u64a.p, (uint32_t*)w.i32d.p, flag, offs, C, w.u64c.p,
+													  contactListVals);
+	}
+	if (!sync_counters(w))
+		return false;
+	int32_t NB = (int32_t)w.hostCounters[CT_INTEGRATED];
+	int32_t NC = C > 0 ? (int32_t)w.hostCounters[CT_SEG_COUNT] : 0;
+	radix_sort_pairs(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p, s);
+	if (NC > 0)
+		radix_sort_pairs(w.u64c.p, contactListVals, w.u64a.p, w.sortValsAlt.p, NC, bitsI, w.primScratch.p, s);
+	scan_exclusive(islandBodies, islandBodies, I + 1, nullptr, w.primScratch.p, s);
+	scan_exclusive(islandContacts, islandContacts, I + 1, nullptr, w.primScratch.p, s);
+
+	// awake island list
+	if (!w.u32a.ensure(I + 1))
+		return false;
+	scan_exclusive(islandAwake, w.u32a.p, I, w.counters.p + CT_AWAKE_ISLANDS, w.primScratch.p, s);
+	int32_t* awakeList = w.i32a.p; // parent no longer needed
+	k_mark_flag_list<<<grids(I), kBlockS, 0, s>>>(islandAwake, I, w.u32a.p, awakeList);
+	if (w.djointCount > 0)
+		k_joint_island<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p,
+																 w.islandOf.p, islandAwake, w.i32e.p);
+	if (!sync_counters(w))
+		return false;
+	int32_t NA = (int32_t)w.hostCounters[CT_AWAKE_ISLANDS];
+	w.stats.awakeDynamicBodies = NB;
+	w.stats.solvedContacts = NC;
+
+	if (NA > 0)
+	{
+		if (!w.u32a.ensure(n + 1))
+			return false;
+		SolveCtx X;
+		X.B = make_body_view(w);
+		X.C = make_contact_view(w);
+		X.J = w.djoints.p;
+		X.nJ = w.djointCount;
+		X.jointIsland = w.i32e.p;
+		X.bodyList = bodyListVals;
+		X.bodyStart = islandBodies;
+		X.contactList = contactListVals;
+		X.contactStart = islandContacts;
+		X.colorMask = w.u32a.p;
+		X.contactColor = w.contactColor.p;
+		X.gravity = w.def.gravity;
+		X.h = dt / (float)substeps;
+		X.invH = X.h > 0.0f ? 1.0f / X.h : 0.0f;
+		X.gravityLen = m3d_length(w.def.gravity);
+		X.maxCorrection = kMaxDepenetrationSpeed * X.h;
+		X.iterations = w.def.positionIterations > 0 ? w.def.positionIterations : 2;
+		X.substeps = substeps;
+		k_solve_islands_serial<<<(NA + 63) / 64, 64, 0, s>>>(X, awakeList, NA);
+	}
+
+	// sleeping
+	if (!w.i32b.ensure(I) || !w.u32b.ensure(I))
+		return false;
+	launch_fill_i32(w.i32b.p, 0x7FFFFFFF, I, s);
+	launch_fill_u32(w.u32b.p, f2u(FLT_MAX), I, s);
+	k_sleep_prep<<<grids(n), kBlockS, 0, s>>>(w.bc.flags.p, w.islandOf.p, islandAwake, n, w.i32b.p);
+	k_sleep_update<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), w.islandOf.p, islandAwake, n, dt, w.i32b.p,
+												w.u32b.p);
+	k_sleep_apply<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), w.islandOf.p, islandAwake, n, w.i32b.p, w.u32b.p,
+											   w.counters.p);
+	return true;
+}
+
+void finish_step(World& w, float dt)
+{
+	int32_t n = w.deviceBodyCount;
+	if (n > 0)
+		k_finish<<<grids(n), kBlockS, 0, w.stream>>>(make_body_view(w), n, dt, w.anyForcesApplied ? 1 : 0,
+													 w.counters.p);
+}
+
+} // namespace mx

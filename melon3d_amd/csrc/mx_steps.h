@@ -1,0 +1,73 @@
+// Launch wrappers of the step stages (one per pipeline stage), called by the
+// host orchestrator in step.cpp.
+#pragma once
+
+#include "mx_world.h"
+
+namespace mx
+{
+
+// device counters block layout
+enum Counter : int
+{
+	CT_ENTRIES = 0,
+	CT_OVERSIZED,
+	CT_UX_MIN,
+	CT_UX_MAX,
+	CT_UY_MIN,
+	CT_UY_MAX,
+	CT_UZ_MIN,
+	CT_UZ_MAX,
+	CT_CANDIDATES,
+	CT_NEW_CONTACTS,
+	CT_REMOVED,
+	CT_HOLES,
+	CT_TAILS,
+	CT_END_EVENTS,
+	CT_TRANSITIONS,
+	CT_REGEN,
+	CT_CHANGED,
+	CT_ANY_AWAKE_DYN,
+	CT_ISLANDS,
+	CT_TOUCHING,
+	CT_STABLE_DIRTY,
+	CT_REFAT,
+	CT_AWAKE_BODIES,
+	CT_BODY_COUNT,
+	CT_SOLVED_CONTACTS,
+	CT_SOLVED_POINTS,
+	CT_BB_POINTS,
+	CT_AWAKE_ISLANDS,
+	CT_INTEGRATED,
+	CT_SEG_COUNT,
+	CT_COUNT_
+};
+
+constexpr int kCountersSize = 64;
+
+// ---- k_broad.hip ----
+void launch_refresh(World& w, float dt, bool rebuildStable);
+// inStableTier := isStable for every slot (after a stable-tier build)
+void launch_stable_migrate(World& w);
+// Builds one tier (0 active, 1 stable); returns false on allocation failure.
+bool build_tier(World& w, int tier);
+bool enumerate_and_create(World& w);
+bool remove_separated(World& w);
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+bool apply_moves(World& w, const int2* dMoves, int32_t count);
+void launch_material_refresh(World& w, const uint8_t* dFlags);
+void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const float4* stage, const uint32_t* stageFlags,
+						   const uint4* stageFilter, const uint8_t* stageBp);
+
+// ---- k_narrow.hip ----
+bool narrowphase(World& w, float dt);
+
+// ---- k_solve.hip ----
+bool islands_and_solve(World& w, float dt, int substeps);
+void finish_step(World& w, float dt);
+
+bool sync_counters(World& w);
+bool push_host_changes(World& w);
+
+} // namespace mx

@@ -1,0 +1,231 @@
+// Host-side world of the device backend: the host mirror of bodies and
+// joints that serves the m3d_* API, the device-resident SoA state the step
+// kernels run on, and the bookkeeping that keeps the two consistent.
+//
+// Ownership: the device copy of body state is authoritative after a step;
+// the host mirror is refreshed lazily (one bulk download on the first API
+// access after a step). API writes edit the host mirror and mark the body
+// dirty; dirty bodies are scattered to the device at the start of the next
+// step. Contacts live only on the device; destroys between steps are
+// resolved on the host against a downloaded contact sidecar and applied to
+// the device contact arrays as one permutation at the next step.
+#pragma once
+
+#include "mx_solver.h"
+#include "mx_prims.h"
+#include "melon3d_ext.h"
+
+#include <string>
+#include <vector>
+
+namespace mx
+{
+
+void set_error(const char* fmt, ...);
+bool hip_ok(hipError_t e, const char* what);
+
+// Device buffer with capacity-doubling growth.
+template <class T>
+struct DBuf
+{
+	T* p = nullptr;
+	size_t cap = 0;
+	// Ensures room for n elements; keeps the first `keep` elements.
+	bool ensure(size_t n, size_t keep = 0)
+	{
+		if (n <= cap)
+		{
+			return true;
+		}
+		size_t nc = cap ? cap : 64;
+		while (nc < n)
+		{
+			nc = nc * 3 / 2 + 64;
+		}
+		T* np = nullptr;
+		if (hipMalloc((void**)&np, nc * sizeof(T)) != hipSuccess)
+		{
+			set_error("hipMalloc of %zu bytes failed", nc * sizeof(T));
+			return false;
+		}
+		if (p != nullptr)
+		{
+			// queued kernels may still read the old buffer
+			(void)hipDeviceSynchronize();
+			if (keep > 0)
+			{
+				(void)hipMemcpy(np, p, keep * sizeof(T), hipMemcpyDeviceToDevice);
+			}
+			(void)hipFree(p);
+		}
+		p = np;
+		cap = nc;
+		return true;
+	}
+	void release()
+	{
+		if (p != nullptr)
+		{
+			hipFree(p);
+		}
+		p = nullptr;
+		cap = 0;
+	}
+};
+
+struct HostBody
+{
+	m3d_vec3 position;
+	m3d_quat rotation;
+	m3d_vec3 linearVelocity;
+	m3d_vec3 angularVelocity;
+	m3d_vec3 prevPosition;
+	m3d_quat prevRotation;
+	m3d_vec3 prevLinearVelocity;
+	m3d_vec3 prevAngularVelocity;
+	float invMass;
+	m3d_vec3 invInertiaLocal;
+	m3d_mat3 invInertiaWorld;
+	Shape shape;
+	float friction;
+	float restitution;
+	float rollingResistance;
+	float mass;
+	m3d_vec3 force;
+	m3d_vec3 torque;
+	m3d_filter filter;
+	bool isSensor;
+	m3d_body_type type;
+	float linearDamping;
+	float angularDamping;
+	float gravityScale;
+	float sleepTime;
+	float smoothedEnergy;
+	bool enableSleep;
+	bool isAwake;
+	bool inUse;
+	uint32_t generation;
+	uint64_t userData;
+	m3d_aabb fat;
+	uint8_t bp; // BP_HAS_FAT | BP_IN_STABLE_TIER | BP_IN_STABLE_CELLS persist across steps
+};
+
+// Per-body column sets used for bulk transfers.
+struct BodyColumns
+{
+	DBuf<float4> pos, rot, vel, angVel, prevPos, prevRot, prevVel, prevAngVel, invIL, iw0, iw1, iw2, shape, mat,
+		force, torque, fatLo, fatHi;
+	DBuf<uint32_t> flags;
+	DBuf<uint4> filter;
+	DBuf<uint8_t> bp, sb;
+};
+
+struct ContactColumns
+{
+	DBuf<int2> bodies;
+	DBuf<float4> mat, normal, cachePA, cachePB, cacheQA, cacheQB;
+	DBuf<float4> pA, pB, pJA, pJB, pRA, pRB, pFA, pFB, pSlip; // 4 * cap each
+};
+
+struct StepCounters; // device-side counters block (k_step.hip)
+
+struct World
+{
+	m3d_world_def def;
+	int device = 0;
+	hipStream_t stream = nullptr;
+
+	// ---- host mirror ----
+	std::vector<HostBody> bodies;
+	std::vector<int32_t> freeBodies;
+	std::vector<DJoint> joints;
+	std::vector<uint32_t> jointGen;
+	std::vector<int32_t> freeJoints;
+	bool gridDirty = true;
+	bool stableDirty = true;
+	bool stableForce = false;
+	int32_t stepsSinceStableBuild = 1000;
+	bool anyForcesApplied = false;
+	bool jointedPairsDirty = true;
+	std::vector<m3d_contact_event> beginEvents, endEvents;
+	m3d_profile profile{};
+
+	// ---- sync state ----
+	bool hostStale = false;          // device bodies newer than the host mirror
+	std::vector<int32_t> dirtyList;  // bodies to upload at the next step
+	std::vector<uint8_t> dirtyFlag;
+	bool jointsDirty = true;         // joint table to upload
+	std::vector<uint8_t> materialRefresh; // bodies whose contacts recombine materials
+	bool anyMaterialRefresh = false;
+
+	// contact sidecar mirror for between-step destroys
+	bool sidecarValid = false;
+	std::vector<int2> hcBodies;
+	std::vector<uint32_t> hcFlags;
+	std::vector<int32_t> hcOrigin;   // original device slot currently at each index
+	bool pendingPermutation = false;
+
+	// ---- device state ----
+	int32_t deviceBodyCap = 0;  // capacity of the body columns
+	int32_t deviceBodyCount = 0; // bodies.size() as known by the device
+	BodyColumns bc;
+	ContactColumns cc;
+	int32_t contactCap = 0;
+	int32_t contactCount = 0;
+	DBuf<DJoint> djoints;
+	int32_t djointCount = 0;
+	DBuf<uint64_t> jointedKeys;
+	int32_t jointedCount = 0;
+
+	// grid tiers
+	DBuf<uint64_t> activeKeys, stableKeys, sortKeysAlt;
+	DBuf<uint32_t> activeBodies, stableBodies, sortValsAlt;
+	int64_t activeCount = 0, stableCount = 0;
+	std::vector<int32_t> activeOversized, stableOversized; // host copies (small)
+	DBuf<int32_t> dActiveOversized, dStableOversized;
+
+	// scratch
+	DBuf<uint32_t> u32a, u32b, u32c, u32d;
+	DBuf<uint64_t> u64a, u64b, u64c;
+	DBuf<int32_t> i32a, i32b, i32c, i32d, i32e;
+	DBuf<uint8_t> u8a;
+	DBuf<char> primScratch;
+	DBuf<uint64_t> hashKeys;
+	DBuf<uint32_t> hashVals;
+	DBuf<int32_t> islandOf;       // per body: island id (-1)
+	DBuf<uint32_t> islandAwakeD;  // per island: any body awake
+	DBuf<int32_t> contactColor;   // per contact: -1 / 0..8
+	int32_t lastIslandCount = 0;
+	bool lastHadIslands = false;
+	DBuf<uint32_t> counters;      // device counters block
+	uint32_t* hostCounters = nullptr; // pinned mirror of the counters block
+	DBuf<int2> moves;             // permutation moves (dst, src)
+	DBuf<int32_t> eventIdx;
+
+	m3dx_step_stats stats{};
+
+	World();
+	~World();
+};
+
+// ---- host <-> device transfer (world_sync.cpp) ----
+bool ensure_device_bodies(World& w, int32_t n);
+bool ensure_device_contacts(World& w, int32_t n);
+bool upload_dirty(World& w);
+bool download_bodies(World& w);
+void ensure_host(World& w);
+void mark_dirty(World& w, int32_t i);
+bool ensure_sidecar(World& w);
+bool apply_pending_permutation(World& w);
+
+// ---- step (k_step.hip) ----
+void world_step(World& w, float dt, int substeps);
+void refresh_broadphase(World& w, float dt);
+void wake_body_host(World& w, int32_t i);
+
+} // namespace mx
+
+struct m3d_world
+{
+	mx::World w;
+};

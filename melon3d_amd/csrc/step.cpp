@@ -1,0 +1,177 @@
+// m3d_world_step orchestration on the host: the order of stages follows
+// src/world.cpp:1437-2167 (broadphase refresh, joint exclusion list, pair
+// enumeration and creation, separated-pair removal, narrowphase and touch
+// events, islands and solve, sleeping, kinematic bodies, force clear,
+// profile). Every stage runs on the device; the host reads a small counter
+// block between stages to size the next one.
+#include "mx_steps.h"
+
+#include <algorithm>
+#include <chrono>
+#include <string.h>
+
+namespace mx
+{
+
+using Clock = std::chrono::steady_clock;
+
+static float ms_between(Clock::time_point a, Clock::time_point b)
+{
+	return std::chrono::duration<float, std::milli>(b - a).count();
+}
+
+static bool upload_joints(World& w)
+{
+	if (!w.jointsDirty)
+		return true;
+	int32_t nj = (int32_t)w.joints.size();
+	w.djointCount = nj;
+	if (nj > 0)
+	{
+		if (!w.djoints.ensure(nj))
+			return false;
+		if (!hip_ok(hipMemcpyAsync(w.djoints.p, w.joints.data(), nj * sizeof(DJoint), hipMemcpyHostToDevice, w.stream),
+					"joint upload"))
+			return false;
+	}
+	w.jointsDirty = false;
+	return true;
+}
+
+// Joint lambdas/compliance are transient device state; bring them back so
+// the host table (the upload source) stays current.
+static bool download_joints(World& w)
+{
+	if (w.djointCount == 0)
+		return true;
+	return hip_ok(hipMemcpyAsync(w.joints.data(), w.djoints.p, w.djointCount * sizeof(DJoint), hipMemcpyDeviceToHost,
+								 w.stream),
+				  "joint download");
+}
+
+static bool upload_jointed_pairs(World& w)
+{
+	if (!w.jointedPairsDirty)
+		return true;
+	std::vector<uint64_t> keys;
+	for (const DJoint& j : w.joints)
+	{
+		if (j.inUse)
+		{
+			int32_t lo = j.bodyA < j.bodyB ? j.bodyA : j.bodyB;
+			int32_t hi = j.bodyA < j.bodyB ? j.bodyB : j.bodyA;
+			keys.push_back(contact_key(lo, hi));
+		}
+	}
+	std::sort(keys.begin(), keys.end());
+	w.jointedCount = (int32_t)keys.size();
+	if (!keys.empty())
+	{
+		if (!w.jointedKeys.ensure(keys.size()))
+			return false;
+		if (!hip_ok(hipMemcpy(w.jointedKeys.p, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice),
+					"jointed pairs"))
+			return false;
+	}
+	w.jointedPairsDirty = false;
+	return true;
+}
+
+static bool apply_material_refresh(World& w)
+{
+	if (!w.anyMaterialRefresh)
+		return true;
+	int32_t n = w.deviceBodyCount;
+	if (!w.u8a.ensure(n))
+		return false;
+	w.materialRefresh.resize(n, 0);
+	hipMemcpyAsync(w.u8a.p, w.materialRefresh.data(), n, hipMemcpyHostToDevice, w.stream);
+	launch_material_refresh(w, w.u8a.p);
+	std::fill(w.materialRefresh.begin(), w.materialRefresh.end(), 0);
+	w.anyMaterialRefresh = false;
+	return hip_ok(hipStreamSynchronize(w.stream), "material refresh");
+}
+
+// Host changes made since the last step reach the device here.
+bool push_host_changes(World& w)
+{
+	return apply_pending_permutation(w) && upload_dirty(w) && upload_joints(w) && apply_material_refresh(w);
+}
+
+// RefreshBroadphase; src/world.cpp:156-229
+void refresh_broadphase(World& w, float dt)
+{
+	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
+	launch_refresh(w, dt, rebuildStable);
+	build_tier(w, 0);
+	if (rebuildStable)
+	{
+		build_tier(w, 1);
+		// inStableTier = isStable for every slot
+		w.stableDirty = false;
+		w.stableForce = false;
+		w.stepsSinceStableBuild = 0;
+		launch_stable_migrate(w);
+	}
+	else
+	{
+		++w.stepsSinceStableBuild;
+	}
+	w.gridDirty = false;
+}
+
+void world_step(World& w, float dt, int substeps)
+{
+	if (dt <= 0.0f)
+		return;
+	if (substeps < 1)
+		substeps = 1;
+	auto t0 = Clock::now();
+	w.beginEvents.clear();
+	w.endEvents.clear();
+	memset(&w.stats, 0, sizeof(w.stats));
+	if (!push_host_changes(w))
+		return;
+	hipMemsetAsync(w.counters.p, 0, kCountersSize * sizeof(uint32_t), w.stream);
+
+	refresh_broadphase(w, dt);
+	if (!upload_jointed_pairs(w) || !enumerate_and_create(w) || !remove_separated(w))
+		return;
+	auto tBroad = Clock::now();
+	if (!narrowphase(w, dt))
+		return;
+	auto tNarrow = Clock::now();
+	if (!islands_and_solve(w, dt, substeps))
+		return;
+	finish_step(w, dt);
+	if (!download_joints(w) || !sync_counters(w))
+		return;
+	const uint32_t* hc = w.hostCounters;
+	if (hc[CT_STABLE_DIRTY])
+		w.stableDirty = true;
+	w.anyForcesApplied = false;
+	auto tEnd = Clock::now();
+
+	w.profile.broadphaseMs = ms_between(t0, tBroad);
+	w.profile.narrowphaseMs = ms_between(tBroad, tNarrow);
+	w.profile.solverMs = ms_between(tNarrow, tEnd);
+	w.profile.stepMs = ms_between(t0, tEnd);
+	w.profile.bodyCount = (int32_t)hc[CT_BODY_COUNT];
+	w.profile.awakeBodyCount = (int32_t)hc[CT_AWAKE_BODIES];
+	w.profile.contactCount = w.contactCount;
+	w.profile.touchingContactCount = w.lastHadIslands ? (int32_t)hc[CT_TOUCHING] : 0;
+	w.profile.islandCount = w.lastHadIslands ? w.lastIslandCount : 0;
+
+	w.stats.bodyCapacity = w.deviceBodyCount;
+	w.stats.contacts = w.contactCount;
+	w.stats.regeneratedManifolds = (int32_t)hc[CT_REGEN];
+	w.stats.activeEntries = (int32_t)w.activeCount;
+	w.stats.stableEntries = (int32_t)w.stableCount;
+	w.stats.refattedBodies = (int32_t)hc[CT_REFAT];
+	w.stats.islands = w.profile.islandCount;
+
+	w.hostStale = true;
+	w.sidecarValid = false;
+}
+
+} // namespace mx

@@ -1,0 +1,75 @@
+"""Helpers comparing the device backend with the reference build."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def bodies_array(snap):
+    """(position[n,3], rotation[n,4], linVel[n,3], angVel[n,3], inUse[n]) as float64."""
+    n = snap.header.bodyCapacity
+    pos = np.zeros((n, 3))
+    rot = np.zeros((n, 4))
+    lv = np.zeros((n, 3))
+    av = np.zeros((n, 3))
+    use = np.zeros(n, dtype=bool)
+    awake = np.zeros(n, dtype=bool)
+    for i in range(n):
+        b = snap.bodies[i]
+        pos[i] = b.position[:]
+        rot[i] = b.rotation[:]
+        lv[i] = b.linearVelocity[:]
+        av[i] = b.angularVelocity[:]
+        use[i] = bool(b.inUse)
+        awake[i] = bool(b.isAwake)
+    return pos, rot, lv, av, use, awake
+
+
+def contact_pairs(snap):
+    return [(snap.contacts[c].bodyA, snap.contacts[c].bodyB) for c in range(snap.header.contactCount)]
+
+
+def contact_touching(snap):
+    return [bool(snap.contacts[c].touching) for c in range(snap.header.contactCount)]
+
+
+def rot_angle(q1, q2):
+    """Angle (rad) of the relative rotation between unit quaternion rows."""
+    d = np.abs(np.sum(q1 * q2, axis=1)).clip(0.0, 1.0)
+    return 2.0 * np.arccos(d)
+
+
+def state_diff(a, b):
+    pa, ra, va, wa, ua, aa = bodies_array(a)
+    pb, rb, vb, wb, ub, ab = bodies_array(b)
+    assert (ua == ub).all(), "in-use sets differ"
+    m = ua
+    dp = float(np.abs(pa[m] - pb[m]).max()) if m.any() else 0.0
+    dr = float(rot_angle(ra[m], rb[m]).max()) if m.any() else 0.0
+    dv = float(np.abs(va[m] - vb[m]).max()) if m.any() else 0.0
+    dw = float(np.abs(wa[m] - wb[m]).max()) if m.any() else 0.0
+    return {"pos": dp, "rot": dr, "vel": dv, "angvel": dw, "awake_equal": bool((aa[m] == ab[m]).all())}
+
+
+def events(lib, world):
+    ev = lib.m3d_world_contact_events(world)
+    b = [(ev.beginEvents[k].bodyA.index, ev.beginEvents[k].bodyB.index, bool(ev.beginEvents[k].isSensor))
+         for k in range(ev.beginCount)]
+    e = [(ev.endEvents[k].bodyA.index, ev.endEvents[k].bodyB.index, bool(ev.endEvents[k].isSensor))
+         for k in range(ev.endCount)]
+    return b, e
+
+
+def one_step_from(ref, dev, ref_world, dt=1.0 / 60.0, sub=4):
+    """Import the reference world's full state into a fresh device world,
+    step both once and return (ref snapshot, device snapshot, device world)."""
+    snap = ref.export_state(ref_world)
+    dw = dev.import_state(snap)
+    rw2 = ref.import_state(snap)
+    ref.m3d_world_step(rw2, dt, sub)
+    dev.m3d_world_step(dw, dt, sub)
+    a = ref.export_state(rw2)
+    b = dev.export_state(dw)
+    ref.m3d_world_destroy(rw2)
+    return a, b, dw
