@@ -40,6 +40,10 @@ enum Counter : int
 	CT_AWAKE_ISLANDS,
 	CT_INTEGRATED,
 	CT_SEG_COUNT,
+	CT_COLORED_ISLANDS,
+	CT_SLOTS,
+	CT_PAR_ENTRIES,
+	CT_JOINT_ISLANDS,
 	CT_COUNT_
 };
 

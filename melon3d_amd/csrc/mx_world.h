@@ -194,6 +194,12 @@ struct World
 	DBuf<uint32_t> hashVals;
 	DBuf<int32_t> islandOf;       // per body: island id (-1)
 	DBuf<uint32_t> islandAwakeD;  // per island: any body awake
+	DBuf<uint32_t> islandMode;    // per island: awake / colored / has joints
+	// parallel solve scratch
+	DBuf<uint64_t> s64a, s64b;
+	DBuf<uint32_t> s32a, s32b, s32c, sSlotStart;
+	DBuf<int32_t> sColor, sLevel, sJointList;
+	DBuf<int2> sPacket;
 	DBuf<int32_t> contactColor;   // per contact: -1 / 0..8
 	int32_t lastIslandCount = 0;
 	bool lastHadIslands = false;

@@ -79,6 +79,17 @@ World::~World()
 	hashVals.release();
 	islandOf.release();
 	islandAwakeD.release();
+	islandMode.release();
+	s64a.release();
+	s64b.release();
+	s32a.release();
+	s32b.release();
+	s32c.release();
+	sSlotStart.release();
+	sColor.release();
+	sLevel.release();
+	sJointList.release();
+	sPacket.release();
 	contactColor.release();
 	counters.release();
 	moves.release();
