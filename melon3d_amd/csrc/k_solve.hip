@@ -170,6 +170,10 @@ __global__ void k_contact_island(const int2* bodies, const float4* mat, const fl
 				f = 1;
 				atomicAdd(&islandContacts[isl], 1u);
 				atomicAdd(&counters[CT_SOLVED_CONTACTS], 1u);
+				uint32_t pc = (uint32_t)f2i(normal[c].w);
+				atomicAdd(&counters[CT_SOLVED_POINTS], pc);
+				if (!(cf & CF_CURVED))
+					atomicAdd(&counters[CT_BB_POINTS], pc);
 			}
 		}
 	}
@@ -1304,6 +1308,8 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	int32_t NA = (int32_t)w.hostCounters[CT_AWAKE_ISLANDS];
 	w.stats.awakeDynamicBodies = NB;
 	w.stats.solvedContacts = NC;
+	w.stats.solvedPoints = (int32_t)w.hostCounters[CT_SOLVED_POINTS];
+	w.stats.boxBoxPoints = (int32_t)w.hostCounters[CT_BB_POINTS];
 
 	if (NA > 0 && !solve_awake_islands(w, dt, substeps, I, NB, NC, bodyListVals, contactListVals, islandBodies,
 										islandContacts, islandAwake, bitsI))

@@ -15,6 +15,14 @@
 namespace mx
 {
 
+// Single-precision transcendentals evaluated in double precision and rounded
+// once: the correctly rounded float result (up to double-rounding ties), the
+// same on host and device, so parity with the reference does not hinge on a
+// library's last-ulp choices.
+MX_HD float sin_cr(float x) { return (float)sin((double)x); }
+MX_HD float cos_cr(float x) { return (float)cos((double)x); }
+MX_HD float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 // Working state of one body for the solver stages.
 struct WB
 {
@@ -280,8 +288,8 @@ MX_HD void integrate_body(const DBodies& B, int32_t i, float h, m3d_vec3 gravity
 		}
 		else
 		{
-			sh = sinf(half);
-			ch = cosf(half);
+			sh = sin_cr(half);
+			ch = cos_cr(half);
 		}
 		float s = sh / wlen;
 		m3d_quat dq = { w.x * s, w.y * s, w.z * s, ch };
@@ -320,7 +328,7 @@ MX_HD void reconstruct_velocity(const DBodies& B, int32_t i, float invH)
 	if (v2 > 4e-4f)
 	{
 		float vlen = sqrtf(v2);
-		float angle = 2.0f * atan2f(vlen, fabsf(dq.w));
+		float angle = 2.0f * atan2_cr(vlen, fabsf(dq.w));
 		w = m3d_scale(sign * invH * angle / vlen, m3d_v3(dq.x, dq.y, dq.z));
 	}
 	else
@@ -827,7 +835,7 @@ MX_HD float hinge_angle(m3d_quat qa, m3d_quat qb, const DJoint& j)
 	m3d_vec3 refB = m3d_rotate(qb, j.localRefB);
 	float s = m3d_dot(axis, m3d_cross(refA, refB));
 	float c = m3d_dot(refA, refB);
-	return atan2f(s, c);
+	return atan2_cr(s, c);
 }
 
 // src/xpbd.cpp:413-433

@@ -18,6 +18,23 @@
 
 using namespace m3d;
 
+#ifdef M3D_ORACLE_CR_LIBM
+// Strict parity build: the reference's single-precision transcendentals
+// (src/xpbd.cpp:693-694, :882; src/m3d_internal.h:361) resolve to correctly
+// rounded versions (double evaluation, one rounding) inside this library, the
+// same functions the device backend evaluates. glibc's own sinf/cosf are
+// within 0.56 ulp of these; the release-flag build keeps the system libm.
+#include <math.h>
+extern "C" float sinf(float x) { return (float)sin((double)x); }
+extern "C" float cosf(float x) { return (float)cos((double)x); }
+extern "C" float atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
+extern "C" void sincosf(float x, float* s, float* c)
+{
+	*s = (float)sin((double)x);
+	*c = (float)cos((double)x);
+}
+#endif
+
 namespace
 {
 

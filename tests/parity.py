@@ -35,9 +35,16 @@ def contact_touching(snap):
 
 
 def rot_angle(q1, q2):
-    """Angle (rad) of the relative rotation between unit quaternion rows."""
-    d = np.abs(np.sum(q1 * q2, axis=1)).clip(0.0, 1.0)
-    return 2.0 * np.arccos(d)
+    """Angle (rad) of the relative rotation q1 * conj(q2) between quaternion
+    rows, from 2 atan2(|vec|, |w|); float32 products are exact in float64, so
+    identical inputs give exactly 0 even when |q| != 1 by rounding."""
+    x1, y1, z1, w1 = q1.T
+    x2, y2, z2, w2 = (-q2[:, 0], -q2[:, 1], -q2[:, 2], q2[:, 3])
+    rx = w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2
+    ry = w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2
+    rz = w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2
+    rw = w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2
+    return 2.0 * np.arctan2(np.sqrt(rx * rx + ry * ry + rz * rz), np.abs(rw))
 
 
 def state_diff(a, b):

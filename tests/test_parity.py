@@ -106,7 +106,9 @@ def test_stream_churn_lockstep(ref, dev):
         dev.m3d_world_step(dw, DT, 4)
         a, b = ref.export_state(rw), dev.export_state(dw)
         assert contact_pairs(a) == contact_pairs(b), f"step {step}: contact order"
-        assert events(ref, rw) == events(dev, dw), f"step {step}: events"
+        (rb, re), (db, de) = events(ref, rw), events(dev, dw)
+        assert rb == db, f"step {step}: begin events ref-only {sorted(set(rb) - set(db))} dev-only {sorted(set(db) - set(rb))}"
+        assert re == de, f"step {step}: end events ref {re[:8]} dev {de[:8]} ref-only {sorted(set(re) - set(de))} dev-only {sorted(set(de) - set(re))}"
         d = state_diff(a, b)
         assert d["pos"] <= 1e-3, f"step {step}: {d}"
     ref.m3d_world_destroy(rw)
