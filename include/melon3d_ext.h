@@ -227,8 +227,8 @@ typedef struct m3dx_step_stats
 	int32_t refattedBodies;
 	int32_t islands;
 	int32_t coloredIslands;
-	int32_t sweeps;               /* Gauss-Seidel sweeps per iteration */
-	int32_t colorRounds;          /* rounds of the parallel coloring fixpoint */
+	int32_t sweeps;               /* grid-wide Gauss-Seidel sweeps per iteration (giant islands) */
+	int32_t colorRounds;          /* rounds of the giant-island coloring and level fixpoints */
 } m3dx_step_stats;
 
 M3D_API void m3dx_world_stats(m3d_world* world, m3dx_step_stats* out);

@@ -111,6 +111,7 @@ static DContacts contact_view(World& w)
 	C.pFA = w.cc.pFA.p;
 	C.pFB = w.cc.pFB.p;
 	C.pSlip = w.cc.pSlip.p;
+	C.hint = w.cc.hint.p;
 	return C;
 }
 
@@ -777,6 +778,7 @@ __global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t c
 	if (bf_shape(fl) != M3D_SHAPE_BOX || bf_shape(fh) != M3D_SHAPE_BOX)
 		cf |= CF_CURVED;
 	C.bodies[c] = make_int2(lo, hi);
+	C.hint[c] = make_int2(0, 0);
 	C.mat[c] = make_float4(friction, restitution, rolling, u2f(cf));
 	C.normal[c] = make_float4(0.0f, 1.0f, 0.0f, i2f(0));
 	float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -979,6 +981,7 @@ __global__ void k_apply_moves(DContacts C, const int2* moves, int32_t count)
 		return;
 	int32_t d = moves[k].x, s = moves[k].y;
 	C.bodies[d] = C.bodies[s];
+	C.hint[d] = C.hint[s];
 	C.mat[d] = C.mat[s];
 	C.normal[d] = C.normal[s];
 	C.cachePA[d] = C.cachePA[s];

@@ -1,0 +1,1332 @@
+// Island solve kernels. Every awake island takes one of three paths, chosen
+// by island shape only (so results never depend on the launch geometry):
+//
+//   joint islands   one lane runs the reference's serial island schedule,
+//                   joints included (src/xpbd.cpp:898-974, :1167-1369);
+//   local islands   (< kGiantContacts contacts) one workgroup per island:
+//                   lane 0 computes the exact schedule in LDS (greedy
+//                   first-fit colors for islands with >= 300 contacts, then
+//                   serial-order levels of the overflow pass; serial-order
+//                   levels for the rest), then all 256 lanes run the substep
+//                   loop with workgroup barriers between Gauss-Seidel slots;
+//   giant islands   every workgroup of a cooperative grid: the greedy
+//                   coloring and overflow levels are least fixpoints of their
+//                   recurrences, iterated in place until a round changes
+//                   nothing, then the substep loop with grid barriers.
+//
+// A slot never writes one body twice, so each slot is a plain parallel
+// pass, and every body sees its constraints in the reference's order:
+// colors 0..7 as the colored path's buckets (packet arithmetic, 8
+// consecutive bucket members per packet), the overflow and the small
+// islands in serial contact order (serial arithmetic).
+#include "mx_steps.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+
+namespace mx
+{
+
+DBodies make_body_view(World& w);
+DContacts make_contact_view(World& w);
+
+static inline unsigned gridk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// --------------------------------------------------------- contact passes
+
+struct PassCtx
+{
+	DBodies B;
+	DContacts C;
+	float h, gravityLen, maxCorrection;
+};
+
+__device__ __forceinline__ void load_wb(const DBodies& B, int32_t i, WB& b)
+{
+	float4 P = B.pos[i];
+	b.p = f4_v3(P);
+	b.invMass = P.w;
+	b.q = f4_q(B.rot[i]);
+	b.v = f4_v3(B.vel[i]);
+	b.w = f4_v3(B.angVel[i]);
+	b.iw = load_iw(B, i);
+}
+
+__device__ __forceinline__ void store_pose(const DBodies& B, int32_t i, const WB& b)
+{
+	if (b.invMass == 0.0f)
+		return;
+	B.pos[i] = v3_f4(b.p, b.invMass);
+	B.rot[i] = q_f4(b.q);
+}
+
+__device__ __forceinline__ void store_vel(const DBodies& B, int32_t i, const WB& b)
+{
+	if (b.invMass == 0.0f)
+		return;
+	float4 V = B.vel[i], W = B.angVel[i];
+	B.vel[i] = v3_f4(b.v, V.w);
+	B.angVel[i] = v3_f4(b.w, W.w);
+}
+
+// Solve record of one scheduled contact: contact, body A, body B, and its
+// form: bits 0-7 the packet's maxPoints (kRecSerial: serial form), bit 8 set
+// for box-box pairs (the only pairs with stiction friction). Building it
+// once per substep leaves one dependent load in front of each pass.
+constexpr int32_t kRecSerial = 0xFF;
+constexpr int32_t kRecBoxBox = 0x100;
+
+__device__ __forceinline__ int4 make_rec(const PassCtx& X, int32_t c, int32_t maxPoints)
+{
+	int2 ab = X.C.bodies[c];
+	bool bb = bf_shape(X.B.flags[ab.x]) == M3D_SHAPE_BOX && bf_shape(X.B.flags[ab.y]) == M3D_SHAPE_BOX;
+	return make_int4(c, ab.x, ab.y, (maxPoints < 0 ? kRecSerial : maxPoints) | (bb ? kRecBoxBox : 0));
+}
+
+// The three Gauss-Seidel passes over scheduled contacts. load() fetches the
+// contact-side inputs, which only the contact itself writes, so a sweep may
+// issue it a slot early; run() reads the bodies and solves.
+struct NormalsPass
+{
+	using In = NormalsIn;
+	__device__ static void load(const PassCtx& X, int4 r, In& d) { load_normals_in(X.C, r.x, d); }
+	__device__ static void run(const PassCtx& X, int4 r, const In& d)
+	{
+		float4 PA = X.B.pos[r.y], PB = X.B.pos[r.z];
+		m3d_vec3 pa = f4_v3(PA), pb = f4_v3(PB);
+		m3d_quat qa = f4_q(X.B.rot[r.y]), qb = f4_q(X.B.rot[r.z]);
+		int32_t mp = r.w & 0xFF;
+		bool wa, wb;
+		if (mp == kRecSerial)
+		{
+			solve_normals_serial_in(X.C, r.x, d, pa, qa, PA.w, pb, qb, PB.w, X.maxCorrection);
+			wa = PA.w != 0.0f;
+			wb = PB.w != 0.0f;
+		}
+		else
+		{
+			solve_normals_packet_in(X.C, r.x, d, mp, pa, qa, PA.w, pb, qb, PB.w, X.maxCorrection);
+			wa = PA.w > 0.0f;
+			wb = PB.w > 0.0f;
+		}
+		if (wa)
+		{
+			X.B.pos[r.y] = v3_f4(pa, PA.w);
+			X.B.rot[r.y] = q_f4(qa);
+		}
+		if (wb)
+		{
+			X.B.pos[r.z] = v3_f4(pb, PB.w);
+			X.B.rot[r.z] = q_f4(qb);
+		}
+	}
+};
+
+struct FrictionPass
+{
+	using In = FrictionIn;
+	__device__ static void load(const PassCtx& X, int4 r, In& d)
+	{
+		if (r.w & kRecBoxBox)
+			load_friction_in(X.C, r.x, d);
+	}
+	__device__ static void run(const PassCtx& X, int4 r, const In& d)
+	{
+		if (!(r.w & kRecBoxBox))
+			return;
+		WB a, b;
+		load_wb(X.B, r.y, a);
+		load_wb(X.B, r.z, b);
+		apply_friction_in(X.C, r.x, d, a, b);
+		store_pose(X.B, r.y, a);
+		store_pose(X.B, r.z, b);
+	}
+};
+
+struct VelocityPass
+{
+	using In = VelocityIn;
+	__device__ static void load(const PassCtx& X, int4 r, In& d) { load_velocity_in(X.C, r.x, d); }
+	__device__ static void run(const PassCtx& X, int4 r, const In& d)
+	{
+		WB a, b;
+		load_wb(X.B, r.y, a);
+		load_wb(X.B, r.z, b);
+		m3d_vec3 pvA = f4_v3(X.B.prevVel[r.y]), pwA = f4_v3(X.B.prevAngVel[r.y]);
+		m3d_vec3 pvB = f4_v3(X.B.prevVel[r.z]), pwB = f4_v3(X.B.prevAngVel[r.z]);
+		VelShape sa, sbv;
+		sa.type = bf_shape(X.B.flags[r.y]);
+		sa.radius = X.B.shape[r.y].w;
+		sa.halfHeight = X.B.mat[r.y].x;
+		sbv.type = bf_shape(X.B.flags[r.z]);
+		sbv.radius = X.B.shape[r.z].w;
+		sbv.halfHeight = X.B.mat[r.z].x;
+		solve_velocity_in(X.C, r.x, d, X.h, X.gravityLen, a, b, pvA, pwA, pvB, pwB, sa, sbv);
+		store_vel(X.B, r.y, a);
+		store_vel(X.B, r.z, b);
+	}
+};
+
+template <class Pass>
+__device__ __forceinline__ void run_rec(const PassCtx& X, int4 r)
+{
+	typename Pass::In d;
+	Pass::load(X, r, d);
+	Pass::run(X, r, d);
+}
+
+__device__ void pass_normals_serial(const PassCtx& X, int32_t c) { run_rec<NormalsPass>(X, make_rec(X, c, -1)); }
+__device__ void pass_normals_packet(const PassCtx& X, int32_t c, int32_t maxPoints)
+{
+	run_rec<NormalsPass>(X, make_rec(X, c, maxPoints));
+}
+__device__ void pass_friction(const PassCtx& X, int32_t c) { run_rec<FrictionPass>(X, make_rec(X, c, -1)); }
+__device__ void pass_velocity(const PassCtx& X, int32_t c) { run_rec<VelocityPass>(X, make_rec(X, c, -1)); }
+
+__device__ void pass_joint(const PassCtx& X, DJoint& jt, int op)
+{
+	WB a, b;
+	load_wb(X.B, jt.bodyA, a);
+	load_wb(X.B, jt.bodyB, b);
+	if (op == 0)
+	{
+		prepare_joint(jt, a, b);
+	}
+	else if (op == 1)
+	{
+		solve_joint_position(jt, a, b, X.h);
+		store_pose(X.B, jt.bodyA, a);
+		store_pose(X.B, jt.bodyB, b);
+	}
+	else
+	{
+		solve_joint_velocity(jt, a, b, X.h);
+		store_vel(X.B, jt.bodyA, a);
+		store_vel(X.B, jt.bodyB, b);
+	}
+}
+
+struct IslandCtx
+{
+	PassCtx P;
+	const uint32_t* bodyList;
+	const uint32_t* bodyStart;
+	const uint32_t* contactList;
+	const uint32_t* contactStart;
+	const uint64_t* entryIsland;
+	const uint32_t* mode;
+	const int32_t* islandOf;
+	DJoint* J;
+	int32_t nJ;
+	const int32_t* jointIsland;
+	uint32_t* colorMask; // per body (joint-island lanes)
+	int32_t* contactColor;
+	int4* rec; // per contact-list entry: solve records of the local islands
+	m3d_vec3 gravity;
+	float invH;
+	int32_t iterations, substeps, I, NB, NC;
+};
+
+// ------------------------------------------------------ joint islands
+
+__global__ void k_solve_joint_islands(IslandCtx X)
+{
+	int32_t isl = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (isl >= X.I)
+		return;
+	uint32_t m = X.mode[isl];
+	if ((m & (IM_AWAKE | IM_JOINT)) != (IM_AWAKE | IM_JOINT))
+		return;
+	const PassCtx& P = X.P;
+	uint32_t b0 = X.bodyStart[isl], b1 = X.bodyStart[isl + 1];
+	uint32_t c0 = X.contactStart[isl], c1 = X.contactStart[isl + 1];
+	bool colored = (int32_t)(c1 - c0) >= kBigIslandContacts;
+
+	for (uint32_t k = b0; k < b1; ++k)
+	{
+		int32_t bi = (int32_t)X.bodyList[k];
+		store_iw(P.B, bi, inv_inertia_world(f4_q(P.B.rot[bi]), f4_v3(P.B.invIL[bi])));
+	}
+	for (int32_t j = 0; j < X.nJ; ++j)
+		if (X.jointIsland[j] == isl)
+			pass_joint(P, X.J[j], 0);
+
+	int32_t colorCount = 0;
+	if (colored)
+	{
+		for (uint32_t k = b0; k < b1; ++k)
+			X.colorMask[X.bodyList[k]] = 0;
+		for (uint32_t k = c0; k < c1; ++k)
+		{
+			int32_t c = (int32_t)X.contactList[k];
+			int2 ab = P.C.bodies[c];
+			bool dA = P.B.pos[ab.x].w > 0.0f, dB = P.B.pos[ab.y].w > 0.0f;
+			uint32_t used = (dA ? X.colorMask[ab.x] : 0u) | (dB ? X.colorMask[ab.y] : 0u);
+			int32_t color = (int32_t)__builtin_ctz(~used);
+			if (color >= kMaxGraphColors)
+			{
+				X.contactColor[c] = kMaxGraphColors;
+				continue;
+			}
+			X.contactColor[c] = color;
+			if (dA)
+				X.colorMask[ab.x] |= 1u << color;
+			if (dB)
+				X.colorMask[ab.y] |= 1u << color;
+			colorCount = color + 1 > colorCount ? color + 1 : colorCount;
+		}
+	}
+
+	for (int sub = 0; sub < X.substeps; ++sub)
+	{
+		for (uint32_t k = b0; k < b1; ++k)
+			integrate_body(P.B, (int32_t)X.bodyList[k], P.h, X.gravity);
+		for (uint32_t k = c0; k < c1; ++k)
+			prepare_contact(P.B, P.C, (int32_t)X.contactList[k], P.h);
+		for (int32_t j = 0; j < X.nJ; ++j)
+		{
+			if (X.jointIsland[j] == isl)
+			{
+				X.J[j].lambda = 0.0f;
+				X.J[j].motorLambda = 0.0f;
+			}
+		}
+		for (int it = 0; it < X.iterations; ++it)
+		{
+			for (int32_t j = 0; j < X.nJ; ++j)
+				if (X.jointIsland[j] == isl)
+					pass_joint(P, X.J[j], 1);
+			if (!colored)
+			{
+				for (uint32_t k = c0; k < c1; ++k)
+					pass_normals_serial(P, (int32_t)X.contactList[k]);
+				continue;
+			}
+			for (int32_t color = 0; color < colorCount; ++color)
+			{
+				int32_t lane[8];
+				int32_t nl = 0;
+				for (uint32_t k = c0; k <= c1; ++k)
+				{
+					bool end = k == c1;
+					if (!end)
+					{
+						int32_t c = (int32_t)X.contactList[k];
+						if (X.contactColor[c] != color)
+							continue;
+						lane[nl++] = c;
+					}
+					if (nl == 8 || (end && nl > 0))
+					{
+						int32_t mp = 0;
+						for (int l = 0; l < nl; ++l)
+						{
+							int32_t pc = f2i(P.C.normal[lane[l]].w);
+							mp = pc > mp ? pc : mp;
+						}
+						for (int l = 0; l < nl; ++l)
+							pass_normals_packet(P, lane[l], mp);
+						nl = 0;
+					}
+				}
+			}
+			for (uint32_t k = c0; k < c1; ++k)
+			{
+				int32_t c = (int32_t)X.contactList[k];
+				if (X.contactColor[c] == kMaxGraphColors)
+					pass_normals_serial(P, c);
+			}
+		}
+		for (int32_t color = 0; color <= kMaxGraphColors; ++color)
+		{
+			if (colored && color == colorCount && color < kMaxGraphColors)
+				color = kMaxGraphColors;
+			for (uint32_t k = c0; k < c1; ++k)
+			{
+				int32_t c = (int32_t)X.contactList[k];
+				if (!colored || X.contactColor[c] == color)
+					pass_friction(P, c);
+			}
+			if (!colored)
+				break;
+		}
+		for (uint32_t k = b0; k < b1; ++k)
+			reconstruct_velocity(P.B, (int32_t)X.bodyList[k], X.invH);
+		for (int32_t color = 0; color <= kMaxGraphColors; ++color)
+		{
+			if (colored && color == colorCount && color < kMaxGraphColors)
+				color = kMaxGraphColors;
+			for (uint32_t k = c0; k < c1; ++k)
+			{
+				int32_t c = (int32_t)X.contactList[k];
+				if (!colored || X.contactColor[c] == color)
+					pass_velocity(P, c);
+			}
+			if (!colored)
+				break;
+		}
+		for (int32_t j = 0; j < X.nJ; ++j)
+			if (X.jointIsland[j] == isl)
+				pass_joint(P, X.J[j], 2);
+	}
+}
+
+// ------------------------------------------------------ local islands
+
+constexpr int kLocalThreads = 256;
+constexpr int kSlotCap = kGiantContacts + 16;
+
+// One pass over a local island's slots by its workgroup, with a workgroup
+// barrier after every non-empty slot. Each lane's first entry of the next
+// slot is loaded before that barrier.
+template <class Pass>
+__device__ void local_sweep(const PassCtx& P, const int4* rec, const uint16_t* sStart, int32_t nSlots)
+{
+	const int32_t tid = (int32_t)threadIdx.x;
+	int32_t e = -1;
+	int4 r = make_int4(0, 0, 0, 0);
+	typename Pass::In d;
+	if (nSlots > 0 && (int32_t)sStart[0] + tid < (int32_t)sStart[1])
+	{
+		e = (int32_t)sStart[0] + tid;
+		r = rec[e];
+		Pass::load(P, r, d);
+	}
+	for (int32_t s = 0; s < nSlots; ++s)
+	{
+		int32_t st = sStart[s], en = sStart[s + 1];
+		if (e >= 0)
+		{
+			Pass::run(P, r, d);
+			for (int32_t e2 = e + kLocalThreads; e2 < en; e2 += kLocalThreads)
+				run_rec<Pass>(P, rec[e2]);
+		}
+		e = -1;
+		if (s + 1 < nSlots && (int32_t)sStart[s + 1] + tid < (int32_t)sStart[s + 2])
+		{
+			e = (int32_t)sStart[s + 1] + tid;
+			r = rec[e];
+			Pass::load(P, r, d);
+		}
+		if (st != en)
+			__syncthreads();
+	}
+}
+
+__device__ __forceinline__ uint16_t local_body(const uint32_t* list, int32_t n, int32_t body)
+{
+	int32_t l = 0, r = n;
+	while (l < r)
+	{
+		int32_t m = (l + r) >> 1;
+		if ((int32_t)list[m] < body)
+			l = m + 1;
+		else
+			r = m;
+	}
+	return (uint16_t)l;
+}
+
+__global__ void __launch_bounds__(kLocalThreads) k_solve_local(IslandCtx X)
+{
+	const int32_t isl = (int32_t)blockIdx.x;
+	const uint32_t m = X.mode[isl];
+	if (!(m & IM_AWAKE) || (m & (IM_JOINT | IM_GIANT)))
+		return;
+	const int tid = (int)threadIdx.x;
+	const PassCtx& P = X.P;
+	const int32_t b0 = (int32_t)X.bodyStart[isl];
+	const int32_t nb = (int32_t)X.bodyStart[isl + 1] - b0;
+	const int32_t c0 = (int32_t)X.contactStart[isl];
+	const int32_t K = (int32_t)X.contactStart[isl + 1] - c0;
+	// a non-joint island with K contacts has at most K + 1 bodies; giant
+	// islands never reach this kernel, so both fit the LDS arrays
+	if (K >= kGiantContacts || nb > kGiantContacts)
+		return;
+	const bool colored = K >= kBigIslandContacts;
+	const uint32_t* bl = X.bodyList + b0;
+	const uint32_t* cl = X.contactList + c0;
+	int4* rec = X.rec + c0;
+
+	__shared__ uint16_t sLa[kGiantContacts], sLb[kGiantContacts], sSlot[kGiantContacts], sOrder[kGiantContacts];
+	__shared__ uint16_t sBody[kGiantContacts + 1];
+	__shared__ uint16_t sStart[kSlotCap + 1], sCursor[kSlotCap + 1];
+	__shared__ int32_t sNSlots;
+
+	// local index of every written (invMass > 0) endpoint; 0xFFFF otherwise
+	for (int32_t k = tid; k < K; k += kLocalThreads)
+	{
+		int2 ab = P.C.bodies[cl[k]];
+		sLa[k] = P.B.pos[ab.x].w > 0.0f ? local_body(bl, nb, ab.x) : (uint16_t)0xFFFF;
+		sLb[k] = P.B.pos[ab.y].w > 0.0f ? local_body(bl, nb, ab.y) : (uint16_t)0xFFFF;
+	}
+	__syncthreads();
+
+	// exact schedule, one lane (src/xpbd.cpp:1179-1235 coloring; serial order)
+	if (tid == 0)
+	{
+		int32_t maxSlot = -1;
+		if (colored)
+		{
+			for (int32_t x = 0; x < nb; ++x)
+				sBody[x] = 0;
+			for (int32_t k = 0; k < K; ++k)
+			{
+				uint16_t la = sLa[k], lb = sLb[k];
+				uint32_t used = (la != 0xFFFF ? sBody[la] : 0u) | (lb != 0xFFFF ? sBody[lb] : 0u);
+				int32_t color = (int32_t)__builtin_ctz(~used);
+				if (color >= kMaxGraphColors)
+				{
+					sSlot[k] = 0xFFFF;
+					continue;
+				}
+				sSlot[k] = (uint16_t)color;
+				if (la != 0xFFFF)
+					sBody[la] = (uint16_t)(sBody[la] | (1u << color));
+				if (lb != 0xFFFF)
+					sBody[lb] = (uint16_t)(sBody[lb] | (1u << color));
+				maxSlot = color > maxSlot ? color : maxSlot;
+			}
+		}
+		// serial-order levels (all contacts, or the colored path's overflow)
+		for (int32_t x = 0; x < nb; ++x)
+			sBody[x] = 0;
+		for (int32_t k = 0; k < K; ++k)
+		{
+			if (colored && sSlot[k] != 0xFFFF)
+				continue;
+			uint16_t la = sLa[k], lb = sLb[k];
+			int32_t lv = 0;
+			if (la != 0xFFFF)
+				lv = sBody[la];
+			if (lb != 0xFFFF && sBody[lb] > lv)
+				lv = sBody[lb];
+			if (la != 0xFFFF)
+				sBody[la] = (uint16_t)(lv + 1);
+			if (lb != 0xFFFF)
+				sBody[lb] = (uint16_t)(lv + 1);
+			int32_t slot = colored ? kMaxGraphColors + lv : lv;
+			sSlot[k] = (uint16_t)slot;
+			maxSlot = slot > maxSlot ? slot : maxSlot;
+		}
+		int32_t ns = maxSlot + 1;
+		for (int32_t s = 0; s <= ns; ++s)
+			sStart[s] = 0;
+		for (int32_t k = 0; k < K; ++k)
+			sStart[sSlot[k] + 1]++;
+		for (int32_t s = 0; s < ns; ++s)
+		{
+			sStart[s + 1] = (uint16_t)(sStart[s + 1] + sStart[s]);
+			sCursor[s] = sStart[s];
+		}
+		for (int32_t k = 0; k < K; ++k)
+			sOrder[sCursor[sSlot[k]]++] = (uint16_t)k;
+		sNSlots = ns;
+	}
+	__syncthreads();
+	const int32_t nSlots = sNSlots;
+	if (colored)
+	{
+		for (int32_t k = tid; k < K; k += kLocalThreads)
+			X.contactColor[cl[k]] = sSlot[k] < kMaxGraphColors ? (int32_t)sSlot[k] : kMaxGraphColors;
+	}
+
+	for (int sub = 0; sub < X.substeps; ++sub)
+	{
+		for (int32_t k = tid; k < nb; k += kLocalThreads)
+			integrate_body(P.B, (int32_t)bl[k], P.h, X.gravity);
+		__syncthreads();
+		for (int32_t k = tid; k < K; k += kLocalThreads)
+			prepare_contact(P.B, P.C, (int32_t)cl[k], P.h);
+		__syncthreads();
+		// solve records in slot order (packet maxPoints from this substep's
+		// manifolds)
+		for (int32_t r = tid; r < K; r += kLocalThreads)
+		{
+			int32_t k = sOrder[r];
+			int32_t slot = sSlot[k];
+			int32_t mp = -1;
+			if (colored && slot < kMaxGraphColors)
+			{
+				int32_t st = sStart[slot], en = sStart[slot + 1];
+				int32_t ps = st + ((r - st) & ~7);
+				int32_t pe = ps + 8 < en ? ps + 8 : en;
+				mp = 0;
+				for (int32_t q = ps; q < pe; ++q)
+				{
+					int32_t pc = f2i(P.C.normal[cl[sOrder[q]]].w);
+					mp = pc > mp ? pc : mp;
+				}
+			}
+			rec[r] = make_rec(P, (int32_t)cl[k], mp);
+		}
+		__syncthreads();
+		for (int it = 0; it < X.iterations; ++it)
+			local_sweep<NormalsPass>(P, rec, sStart, nSlots);
+		local_sweep<FrictionPass>(P, rec, sStart, nSlots);
+		for (int32_t k = tid; k < nb; k += kLocalThreads)
+			reconstruct_velocity(P.B, (int32_t)bl[k], X.invH);
+		__syncthreads();
+		local_sweep<VelocityPass>(P, rec, sStart, nSlots);
+	}
+}
+
+// ------------------------------------------------------ giant islands
+
+constexpr int kGiantThreads = 256;
+
+struct GiantCtx
+{
+	IslandCtx X;
+	const uint32_t* adjStart; // per body (n + 1)
+	const uint32_t* adj;      // contact-list entries sorted by (body, entry)
+	const int4* pred;         // per entry: predecessor ranges in adj (x < 0: not giant)
+	int32_t* color;           // per entry
+	int32_t* level;           // per entry
+	uint32_t* ctr;            // 6 convergence counters (zeroed before launch)
+	uint32_t* counters;
+	uint64_t* slotKeys;       // per entry, sorted later
+	uint32_t* slotVals;
+	int bitsI;
+};
+
+__device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
+{
+	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+}
+
+// Barrier of a co-resident grid: one monotonic arrival counter (zeroed
+// before the launch); barrier e completes when it reaches (e + 1) * blocks.
+// The release fence publishes this workgroup's writes before it arrives,
+// the acquire fence drops stale cache lines before it reads others'.
+struct GridSync
+{
+	uint32_t* count;
+	uint32_t blocks;
+	uint32_t epoch;
+
+	// Barrier; then returns *word as read by one lane after it (the same
+	// value for every lane of the grid as long as nobody writes *word
+	// before the next barrier).
+	__device__ uint32_t sync_read(const uint32_t* word)
+	{
+		__shared__ uint32_t value;
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+			uint32_t target = (epoch + 1u) * blocks;
+			__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+				__builtin_amdgcn_s_sleep(1);
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+			if (word)
+				value = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		++epoch;
+		__syncthreads();
+		return word ? value : 0u;
+	}
+
+	__device__ void sync() { (void)sync_read(nullptr); }
+
+	// The same barrier in two halves: loads that do not depend on other
+	// workgroups' writes of this phase may be issued in between.
+	__device__ void arrive()
+	{
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+			__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+	}
+
+	__device__ void wait()
+	{
+		if (threadIdx.x == 0)
+		{
+			uint32_t target = (epoch + 1u) * blocks;
+			while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+				__builtin_amdgcn_s_sleep(1);
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+		}
+		++epoch;
+		__syncthreads();
+	}
+};
+
+// Plain loads and stores of values that other lanes update within a
+// fixpoint round: a round may see either value, and the grid barrier
+// between rounds publishes every update.
+__device__ __forceinline__ int32_t ld_round(const int32_t* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void st_round(int32_t* p, int32_t v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// OR of the colors (< 8) of the entries adj[s..e): the entry's predecessors
+// on one body. Loads are issued 8 at a time.
+__device__ __forceinline__ uint32_t pred_colors(const GiantCtx& G, int32_t s, int32_t e)
+{
+	uint32_t used = 0;
+	for (int32_t base = s; base < e; base += 8)
+	{
+		int32_t k2[8];
+#pragma unroll
+		for (int i = 0; i < 8; ++i)
+			k2[i] = base + i < e ? (int32_t)G.adj[base + i] : -1;
+#pragma unroll
+		for (int i = 0; i < 8; ++i)
+		{
+			if (k2[i] >= 0)
+			{
+				int32_t col = ld_round(&G.color[k2[i]]);
+				if (col < kMaxGraphColors)
+					used |= 1u << col;
+			}
+		}
+	}
+	return used;
+}
+
+// Deepest overflow level + 1 among the predecessors adj[s..e).
+__device__ __forceinline__ int32_t pred_level(const GiantCtx& G, int32_t s, int32_t e)
+{
+	int32_t lv = 0;
+	for (int32_t base = s; base < e; base += 8)
+	{
+		int32_t k2[8];
+#pragma unroll
+		for (int i = 0; i < 8; ++i)
+			k2[i] = base + i < e ? (int32_t)G.adj[base + i] : -1;
+#pragma unroll
+		for (int i = 0; i < 8; ++i)
+		{
+			if (k2[i] >= 0 && G.color[k2[i]] == kMaxGraphColors)
+			{
+				int32_t l2 = ld_round(&G.level[k2[i]]) + 1;
+				lv = l2 > lv ? l2 : lv;
+			}
+		}
+	}
+	return lv;
+}
+
+// One coloring round for entry k: first fit over its predecessors (earlier
+// entries sharing a written body, the ranges pr); true when it changed.
+__device__ bool color_round(const GiantCtx& G, int32_t k, int4 pr)
+{
+	uint32_t used = pred_colors(G, pr.x, pr.y) | pred_colors(G, pr.z, pr.w);
+	int32_t nc = (int32_t)__builtin_ctz(~used);
+	nc = nc < kMaxGraphColors ? nc : kMaxGraphColors;
+	if (nc != ld_round(&G.color[k]))
+	{
+		st_round(&G.color[k], nc);
+		return true;
+	}
+	return false;
+}
+
+// One level round for an overflow entry (serial order among the overflow).
+__device__ bool level_round(const GiantCtx& G, int32_t k, int4 pr)
+{
+	int32_t la = pred_level(G, pr.x, pr.y), lb = pred_level(G, pr.z, pr.w);
+	int32_t lv = la > lb ? la : lb;
+	if (lv != ld_round(&G.level[k]))
+	{
+		st_round(&G.level[k], lv);
+		return true;
+	}
+	return false;
+}
+
+// Rounds until one changes nothing. Round r flags changes in ctr[r % 3]
+// and clears ctr[(r + 1) % 3], which no lane reads again before it is
+// reused two rounds later.
+template <int WHICH>
+__device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* ctr, int32_t tid, int32_t nt)
+{
+	for (int32_t r = 0; r < (1 << 24); ++r)
+	{
+		if (tid == 0)
+			__hip_atomic_store(&ctr[(r + 1) % 3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t changes = 0;
+		for (int32_t k = tid; k < G.X.NC; k += nt)
+		{
+			int4 pr = G.pred[k];
+			if (pr.x < 0)
+				continue;
+			if (WHICH == 0)
+				changes += color_round(G, k, pr) ? 1u : 0u;
+			else if (G.color[k] == kMaxGraphColors)
+				changes += level_round(G, k, pr) ? 1u : 0u;
+		}
+		// one flag store per workgroup that changed something (same-address
+		// atomics from every wave would serialize)
+		if (__syncthreads_or(changes != 0u) && threadIdx.x == 0)
+			__hip_atomic_store(&ctr[r % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (grid.sync_read(&ctr[r % 3]) == 0u)
+			return r + 1;
+	}
+	return -1;
+}
+
+__global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
+{
+	GridSync grid{G.ctr + 8, gridDim.x, 0u};
+	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
+	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt);
+	int32_t rl = fixpoint_rounds<1>(G, grid, G.ctr + 3, tid, nt);
+	if (tid == 0)
+		G.counters[CT_COLOR_ROUNDS] = (uint32_t)(rc + rl);
+	// slot keys: (slot, island) for giant entries, a sentinel above all others
+	for (int32_t k = tid; k < G.X.NC; k += nt)
+	{
+		uint64_t key = ~0ull;
+		if (giant_entry(G, k))
+		{
+			int32_t col = G.color[k];
+			int32_t slot = col < kMaxGraphColors ? col : kMaxGraphColors + G.level[k];
+			G.X.contactColor[G.X.contactList[k]] = col;
+			G.X.P.C.hint[G.X.contactList[k]] = make_int2(col, G.level[k]);
+			key = ((uint64_t)slot << G.bitsI) | G.X.entryIsland[k];
+			atomicMax(&G.counters[CT_SLOTS], (uint32_t)slot + 1u);
+		}
+		G.slotKeys[k] = key;
+		G.slotVals[k] = G.X.contactList[k];
+	}
+}
+
+// Adjacency pairs (body, entry) of the giant entries, and the fixpoint's
+// starting point: the contact's color and level of its last solve (any start
+// converges to the same least fixpoint; a close one in fewer rounds).
+__global__ void k_giant_adj(IslandCtx X, int4* pred, int32_t* color, int32_t* level, int32_t n, uint64_t* keys,
+							uint32_t* vals, uint32_t* bodyCount)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= X.NC)
+		return;
+	bool g = (X.mode[(int32_t)X.entryIsland[k]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+	int2 ab = X.P.C.bodies[X.contactList[k]];
+	bool da = g && X.P.B.pos[ab.x].w > 0.0f;
+	bool db = g && X.P.B.pos[ab.y].w > 0.0f;
+	keys[2 * k] = da ? (uint64_t)ab.x : (uint64_t)n;
+	vals[2 * k] = (uint32_t)k;
+	keys[2 * k + 1] = db ? (uint64_t)ab.y : (uint64_t)n;
+	vals[2 * k + 1] = (uint32_t)k;
+	if (da)
+		atomicAdd(&bodyCount[ab.x], 1u);
+	if (db)
+		atomicAdd(&bodyCount[ab.y], 1u);
+	pred[k] = g ? make_int4(0, 0, 0, 0) : make_int4(-1, -1, -1, -1);
+	int2 h = g ? X.P.C.hint[X.contactList[k]] : make_int2(0, 0);
+	color[k] = h.x < 0 ? 0 : (h.x > kMaxGraphColors ? kMaxGraphColors : h.x);
+	level[k] = h.y < 0 ? 0 : (h.y > (1 << 24) ? (1 << 24) : h.y);
+}
+
+// Predecessor range of every (body, entry) pair: the entries before it in
+// the body's sorted list.
+__global__ void k_giant_pred(IslandCtx X, const uint64_t* keys, const uint32_t* adj, const uint32_t* adjStart,
+							 int64_t pairs, int32_t n, int4* pred)
+{
+	int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= pairs || keys[e] >= (uint64_t)n)
+		return;
+	int32_t body = (int32_t)keys[e];
+	int32_t k = (int32_t)adj[e];
+	int32_t st = (int32_t)adjStart[body];
+	if (X.P.C.bodies[X.contactList[k]].x == body)
+	{
+		pred[k].x = st;
+		pred[k].y = (int32_t)e;
+	}
+	else
+	{
+		pred[k].z = st;
+		pred[k].w = (int32_t)e;
+	}
+}
+
+// slotStart[s] = first sorted entry with slot >= s, for s in [0, maxS]
+__global__ void k_giant_slot_starts(const uint64_t* keys, int32_t NC, int bitsI, int32_t maxS, uint32_t* slotStart)
+{
+	int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (s > maxS)
+		return;
+	uint64_t target = (uint64_t)s << bitsI;
+	int32_t l = 0, r = NC;
+	while (l < r)
+	{
+		int32_t mid = (l + r) >> 1;
+		if (keys[mid] < target)
+			l = mid + 1;
+		else
+			r = mid;
+	}
+	slotStart[s] = (uint32_t)l;
+}
+
+// packet = 8 consecutive members of one (color, island) bucket; -1 outside
+// the colored slots
+__global__ void k_giant_packets(const uint64_t* keys, int32_t NC, int bitsI, int2* packet)
+{
+	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= NC)
+		return;
+	uint64_t key = keys[e];
+	if (key == ~0ull || (int32_t)(key >> bitsI) >= kMaxGraphColors)
+	{
+		packet[e] = make_int2(-1, -1);
+		return;
+	}
+	int32_t l = 0, r = e;
+	while (l < r)
+	{
+		int32_t mid = (l + r) >> 1;
+		if (keys[mid] < key)
+			l = mid + 1;
+		else
+			r = mid;
+	}
+	int32_t gs = l;
+	l = e;
+	r = NC;
+	while (l < r)
+	{
+		int32_t mid = (l + r) >> 1;
+		if (keys[mid] <= key)
+			l = mid + 1;
+		else
+			r = mid;
+	}
+	int32_t ps = gs + ((e - gs) & ~7);
+	int32_t pe = ps + 8 < l ? ps + 8 : l;
+	packet[e] = make_int2(ps, pe);
+}
+
+struct GiantSolveCtx
+{
+	IslandCtx X;
+	const uint32_t* entries;   // contact per sorted entry
+	const int2* packet;
+	const uint32_t* slotStart; // [0, nSlots]
+	const uint32_t* counters;
+	uint32_t* barrier;         // zeroed arrival counter
+	int4* rec;                 // solve records, grouped within each slot
+	const uint64_t* keys;      // sorted (slot, island) keys
+	int bitsI;
+	uint32_t* bucketCount;     // per (slot, group); zero between substeps
+	uint32_t* bucketCursor;    // per (slot, group)
+};
+
+// Slots with at most this many entries are run by workgroup 0 alone: a run
+// of consecutive small slots then costs one grid barrier instead of one per
+// slot (the overflow levels are many and small).
+constexpr int32_t kSmallSlot = 256;
+
+// First entry of slot s for this lane, or -1: every lane of the grid for
+// large slots, the lanes of workgroup 0 for small ones. ss: slot starts.
+__device__ __forceinline__ int32_t giant_lane_entry(const uint32_t* ss, int32_t s, int32_t tid, int32_t& stride)
+{
+	int32_t st = (int32_t)ss[s], en = (int32_t)ss[s + 1];
+	int32_t e;
+	if (en - st <= kSmallSlot)
+	{
+		if (blockIdx.x != 0)
+			return -1;
+		e = st + (int32_t)threadIdx.x;
+		stride = (int32_t)blockDim.x;
+	}
+	else
+	{
+		e = st + tid;
+		stride = (int32_t)(gridDim.x * blockDim.x);
+	}
+	return e < en ? e : -1;
+}
+
+// One Gauss-Seidel pass over all slots. Each lane's first entry of the next
+// slot (its record and contact-side inputs, which no other contact writes)
+// is loaded while the grid barrier that ends the current slot is pending.
+template <class Pass>
+__device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync& grid, int32_t nSlots)
+{
+	const PassCtx& P = S.X.P;
+	const int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	int32_t stride = 1;
+	int4 r = make_int4(0, 0, 0, 0);
+	typename Pass::In d;
+	int32_t e = nSlots > 0 ? giant_lane_entry(ss, 0, tid, stride) : -1;
+	if (e >= 0)
+	{
+		r = S.rec[e];
+		Pass::load(P, r, d);
+	}
+	bool pendingSmall = false;
+	for (int32_t s = 0; s < nSlots; ++s)
+	{
+		int32_t st = (int32_t)ss[s], en = (int32_t)ss[s + 1];
+		bool small = en - st <= kSmallSlot;
+		if (st != en && !small && pendingSmall)
+		{
+			grid.sync();
+			pendingSmall = false;
+		}
+		if (e >= 0)
+		{
+			Pass::run(P, r, d);
+			for (int32_t e2 = e + stride; e2 < en; e2 += stride)
+				run_rec<Pass>(P, S.rec[e2]);
+		}
+		int32_t strideNext = stride;
+		int32_t eNext = s + 1 < nSlots ? giant_lane_entry(ss, s + 1, tid, strideNext) : -1;
+		bool big = st != en && !small;
+		if (big)
+			grid.arrive();
+		else if (st != en && blockIdx.x == 0)
+			__syncthreads();
+		e = eNext;
+		stride = strideNext;
+		if (e >= 0)
+		{
+			r = S.rec[e];
+			Pass::load(P, r, d);
+		}
+		if (big)
+			grid.wait();
+		else if (st != en)
+			pendingSmall = true;
+	}
+	if (pendingSmall)
+		grid.sync();
+}
+
+// Substep stages of the giant islands: plain launches for the per-body and
+// per-contact stages, one cooperative launch per Gauss-Seidel pass (so each
+// holds only that pass's registers).
+__device__ __forceinline__ bool giant_body(const IslandCtx& X, int32_t b)
+{
+	return (X.mode[X.islandOf[b]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+}
+
+__global__ void k_giant_bodies(GiantSolveCtx S, int op)
+{
+	const IslandCtx& X = S.X;
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= X.NB)
+		return;
+	int32_t b = (int32_t)X.bodyList[k];
+	if (!giant_body(X, b))
+		return;
+	if (op == 0)
+		integrate_body(X.P.B, b, X.P.h, X.gravity);
+	else
+		reconstruct_velocity(X.P.B, b, X.invH);
+}
+
+// Within a slot the contacts are independent, so their order is free: the
+// sweeps run them grouped by point count and curvature (kBuckets groups per
+// slot), which keeps the lanes of a wave on the same path.
+constexpr int kBuckets = 8;
+
+__device__ __forceinline__ uint32_t giant_bucket(const PassCtx& P, const GiantSolveCtx& S, int32_t e, int32_t c)
+{
+	int32_t pc = f2i(P.C.normal[c].w);
+	pc = pc < 1 ? 1 : (pc > 4 ? 4 : pc);
+	bool curved = (f2u(P.C.mat[c].w) & CF_CURVED) != 0;
+	uint32_t slot = (uint32_t)(S.keys[e] >> S.bitsI);
+	return slot * kBuckets + (uint32_t)(pc - 1) * 2u + (curved ? 1u : 0u);
+}
+
+// atomicAdd(&ctr[key], 1) for every active lane, one atomic per distinct key
+// of the wave; returns this lane's old value.
+__device__ uint32_t wave_atomic_inc(uint32_t* ctr, uint32_t key)
+{
+	const uint32_t lane = __lane_id();
+	uint64_t active = __ballot(1);
+	uint32_t result = 0;
+	while (active)
+	{
+		int leader = __ffsll((unsigned long long)active) - 1;
+		uint32_t lk = __shfl(key, leader);
+		uint64_t mask = __ballot(key == lk) & active;
+		uint32_t base = 0;
+		if ((int)lane == leader)
+			base = atomicAdd(&ctr[lk], (uint32_t)__popcll(mask));
+		base = __shfl(base, leader);
+		if (key == lk)
+			result = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+		active &= ~mask;
+	}
+	return result;
+}
+
+__global__ void k_giant_prepare(GiantSolveCtx S)
+{
+	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	int32_t c = (int32_t)S.entries[e];
+	prepare_contact(S.X.P.B, S.X.P.C, c, S.X.P.h);
+	(void)wave_atomic_inc(S.bucketCount, giant_bucket(S.X.P, S, e, c));
+}
+
+// Start of every (slot, group) run; clears the counts for the next substep.
+__global__ void k_giant_bucket_starts(GiantSolveCtx S)
+{
+	int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (s >= (int32_t)S.counters[CT_SLOTS])
+		return;
+	uint32_t at = S.slotStart[s];
+	for (int b = 0; b < kBuckets; ++b)
+	{
+		uint32_t n = S.bucketCount[s * kBuckets + b];
+		S.bucketCount[s * kBuckets + b] = 0u;
+		S.bucketCursor[s * kBuckets + b] = at;
+		at += n;
+	}
+}
+
+__global__ void k_giant_records(GiantSolveCtx S)
+{
+	const PassCtx& P = S.X.P;
+	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	int2 pk = S.packet[e];
+	int32_t mp = -1;
+	if (pk.x >= 0)
+	{
+		mp = 0;
+		for (int32_t q = pk.x; q < pk.y; ++q)
+		{
+			int32_t pc = f2i(P.C.normal[S.entries[q]].w);
+			mp = pc > mp ? pc : mp;
+		}
+	}
+	int32_t c = (int32_t)S.entries[e];
+	uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
+	S.rec[at] = make_rec(P, c, mp);
+}
+
+// `passes` sweeps of one pass kind. The barrier counter is zeroed again by
+// the last workgroup to leave, for the next launch.
+template <class Pass>
+__global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, int passes)
+{
+	GridSync grid{S.barrier, gridDim.x, 0u};
+	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
+	// slot starts in LDS when they fit
+	constexpr int32_t kSlotLds = 2048;
+	__shared__ uint32_t sSlots[kSlotLds];
+	const uint32_t* ss = S.slotStart;
+	if (nSlots + 1 <= kSlotLds)
+	{
+		for (int32_t i = (int32_t)threadIdx.x; i <= nSlots; i += (int32_t)blockDim.x)
+			sSlots[i] = S.slotStart[i];
+		__syncthreads();
+		ss = sSlots;
+	}
+	for (int it = 0; it < passes; ++it)
+		giant_sweep<Pass>(S, ss, grid, nSlots);
+	if (threadIdx.x == 0)
+	{
+		uint32_t left = __hip_atomic_fetch_add(S.barrier + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (left + 1 == gridDim.x)
+		{
+			__hip_atomic_store(S.barrier, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			__hip_atomic_store(S.barrier + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+	}
+}
+
+// Co-resident grid size of a cooperative kernel (blocks of kGiantThreads),
+// cached per device and kernel.
+template <class K>
+static int coop_blocks(World& w, K kernel)
+{
+	constexpr int kMaxDevices = 64;
+	static std::atomic<int> cache[kMaxDevices];
+	bool cacheable = w.device >= 0 && w.device < kMaxDevices;
+	if (cacheable)
+	{
+		int v = cache[w.device].load(std::memory_order_relaxed);
+		if (v > 0)
+			return v;
+	}
+	int cus = 0, perCU = 0;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w.device) != hipSuccess || cus < 1)
+		cus = 1;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kGiantThreads, 0) != hipSuccess || perCU < 1)
+		perCU = 1;
+	int v = perCU * cus;
+	if (cacheable)
+		cache[w.device].store(v, std::memory_order_relaxed);
+	return v;
+}
+
+// Workgroups of a giant-island launch: enough lanes for every entry, at
+// most kGiantMaxBlocks (a grid barrier costs about 0.04 us more with every
+// workgroup that arrives, and a slot's lanes solve one contact each in
+// parallel, so more lanes than the largest slot only add barrier time) and
+// never more than can be co-resident. M3D_GIANT_BLOCKS overrides the cap.
+constexpr int kGiantMaxBlocks = 64;
+
+// The coloring fixpoint does several loads per entry and round, so it
+// takes one lane per entry up to kOrderMaxBlocks workgroups.
+constexpr int kOrderMaxBlocks = 128;
+
+static int env_blocks(const char* name, int dflt)
+{
+	const char* e = getenv(name);
+	int v = e ? atoi(e) : 0;
+	return v > 0 ? v : dflt;
+}
+
+template <class K>
+static int giant_blocks(World& w, K kernel, int32_t entries, int cap)
+{
+	int b = (int)std::min<int64_t>(((int64_t)entries + kGiantThreads - 1) / kGiantThreads, cap);
+	return std::max(1, std::min(b, coop_blocks(w, kernel)));
+}
+
+bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
+{
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	IslandCtx X;
+	X.P.B = make_body_view(w);
+	X.P.C = make_contact_view(w);
+	X.P.h = dt / (float)substeps;
+	X.P.gravityLen = m3d_length(w.def.gravity);
+	X.P.maxCorrection = kMaxDepenetrationSpeed * X.P.h;
+	X.bodyList = L.bodyList;
+	X.bodyStart = L.bodyStart;
+	X.contactList = L.contactList;
+	X.contactStart = L.contactStart;
+	X.entryIsland = L.entryIsland;
+	X.mode = L.mode;
+	X.islandOf = L.islandOf;
+	X.J = w.djoints.p;
+	X.nJ = w.djointCount;
+	X.jointIsland = L.jointIsland;
+	X.colorMask = nullptr;
+	X.contactColor = w.contactColor.p;
+	X.gravity = w.def.gravity;
+	X.invH = X.P.h > 0.0f ? 1.0f / X.P.h : 0.0f;
+	X.iterations = w.def.positionIterations > 0 ? w.def.positionIterations : 2;
+	X.substeps = substeps;
+	X.I = L.I;
+	X.NB = L.NB;
+	X.NC = L.NC;
+
+	if (!w.sRec.ensure(std::max<int32_t>(L.NC, 1)))
+		return false;
+	X.rec = w.sRec.p;
+	if (w.djointCount > 0)
+	{
+		if (!w.u32a.ensure((size_t)n + 1))
+			return false;
+		X.colorMask = w.u32a.p;
+		k_solve_joint_islands<<<gridk(L.I), 256, 0, s>>>(X);
+	}
+	k_solve_local<<<(unsigned)L.I, kLocalThreads, 0, s>>>(X);
+
+	if (L.giantIslands == 0 || L.NC == 0)
+		return true;
+
+	// ---- giant islands ----
+	int32_t NC = L.NC;
+	size_t two = 2 * (size_t)NC;
+	int bitsN = 1;
+	while ((1ll << bitsN) <= (int64_t)n)
+		++bitsN;
+	int32_t maxS = NC + kMaxGraphColors + 1; // slot bound: 8 colors + overflow levels
+	int bitsS = 1;
+	while ((1ll << bitsS) <= (int64_t)maxS)
+		++bitsS;
+	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
+		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
+		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) ||
+		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) ||
+		!w.primScratch.ensure(std::max(scan_scratch_bytes((int64_t)two + n + 2), sort_scratch_bytes((int64_t)two)) +
+							  1024))
+		return false;
+	uint32_t* bodyCount = w.s32c.p;
+	launch_fill_u32(bodyCount, 0u, (int64_t)n + 1, s);
+	k_giant_adj<<<gridk(NC), 256, 0, s>>>(X, w.sPred.p, w.sColor.p, w.sLevel.p, n, w.s64a.p, w.s32a.p, bodyCount);
+	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.primScratch.p, s);
+	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, s);
+	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
+	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6), barriers 8 and 9
+
+	GiantCtx G;
+	G.X = X;
+	G.adjStart = bodyCount;
+	G.adj = w.s32a.p;
+	G.pred = w.sPred.p;
+	G.color = w.sColor.p;
+	G.level = w.sLevel.p;
+	G.ctr = w.sCtr.p;
+	G.counters = w.counters.p;
+	G.slotKeys = w.s64b.p;
+	G.slotVals = w.s32b.p;
+	G.bitsI = L.bitsI;
+	{
+		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);
+		int blocks = giant_blocks(w, k_giant_order, NC, cap);
+		void* args[] = { &G };
+		if (!hip_ok(hipLaunchCooperativeKernel((const void*)k_giant_order, dim3(blocks), dim3(kGiantThreads), args, 0, s),
+					"giant order"))
+			return false;
+	}
+	radix_sort_pairs(w.s64b.p, w.s32b.p, w.s64a.p, w.s32a.p, NC, L.bitsI + bitsS + 1, w.primScratch.p, s);
+	k_giant_slot_starts<<<gridk(maxS + 1), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, maxS, w.sSlotStart.p);
+	k_giant_packets<<<gridk(NC), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, w.sPacket.p);
+
+	GiantSolveCtx S;
+	S.X = X;
+	S.entries = w.s32b.p;
+	S.packet = w.sPacket.p;
+	S.slotStart = w.sSlotStart.p;
+	S.counters = w.counters.p;
+	S.barrier = w.sCtr.p + 9;
+	S.rec = w.sRecG.p;
+	S.keys = w.s64b.p;
+	S.bitsI = L.bitsI;
+	S.bucketCount = w.sBucket.p;
+	S.bucketCursor = w.sBucket.p + (size_t)(maxS + 1) * kBuckets;
+	launch_fill_u32(S.bucketCount, 0u, (int64_t)(maxS + 1) * kBuckets, s);
+	static const int cap = env_blocks("M3D_GIANT_BLOCKS", kGiantMaxBlocks);
+	const int blocks = std::min(giant_blocks(w, k_giant_pass<NormalsPass>, NC, cap),
+								std::min(giant_blocks(w, k_giant_pass<FrictionPass>, NC, cap),
+										 giant_blocks(w, k_giant_pass<VelocityPass>, NC, cap)));
+	auto coop = [&](const void* kernel, int passes) {
+		void* args[] = { &S, &passes };
+		return hip_ok(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s),
+					  "giant pass");
+	};
+	for (int sub = 0; sub < substeps; ++sub)
+	{
+		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
+		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
+		k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
+		k_giant_records<<<gridk(NC), 256, 0, s>>>(S);
+		if (!coop((const void*)k_giant_pass<NormalsPass>, X.iterations) ||
+			!coop((const void*)k_giant_pass<FrictionPass>, 1))
+			return false;
+		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
+		if (!coop((const void*)k_giant_pass<VelocityPass>, 1))
+			return false;
+	}
+	return true;
+}
+
+} // namespace mx

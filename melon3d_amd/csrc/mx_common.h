@@ -184,6 +184,7 @@ struct DContacts
 	float4* pFA; // frictionAnchorA xyz
 	float4* pFB; // frictionAnchorB xyz
 	float4* pSlip; // slipDir xyz
+	int2* hint;    // schedule warm start: last color, last overflow level
 };
 
 MX_HD Shape load_shape(const DBodies& B, int32_t i)

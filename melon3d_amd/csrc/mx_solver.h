@@ -488,25 +488,57 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 
 // ------------------------------------------------------ normal projection
 
+// Contact-side inputs of a normal projection. Every point slot is allocated
+// (4 per contact), so all of them are loaded at once, before the point count
+// is known: one memory round trip, which a caller may also issue early (only
+// the contact itself writes these values).
+struct NormalsIn
+{
+	float4 nc;
+	float4 A4[4], B4[4], JA[4], JB[4];
+};
+
+MX_HD void load_normals_in(const DContacts& C, int32_t c, NormalsIn& d)
+{
+	d.nc = C.normal[c];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = pidx(C, c, j);
+		d.A4[j] = C.pA[p];
+		d.B4[j] = C.pB[p];
+		d.JA[j] = C.pJA[p];
+		d.JB[j] = C.pJB[p];
+	}
+}
+
 // Serial-order form (SolveContactNormals, src/xpbd.cpp:122-163): divides by
 // wN and skips points with wN <= 0 or sep >= 0. Updates the body states in
 // place and accumulates lambdaN.
-MX_HD void solve_normals_serial(const DContacts& C, int32_t c, m3d_vec3& pa, m3d_quat& qa, float imA, m3d_vec3& pb,
-								m3d_quat& qb, float imB, float maxCorrection)
+MX_HD void solve_normals_serial_in(const DContacts& C, int32_t c, const NormalsIn& d, m3d_vec3& pa, m3d_quat& qa,
+								   float imA, m3d_vec3& pb, m3d_quat& qb, float imB, float maxCorrection)
 {
-	float4 nc = C.normal[c];
+	const float4 nc = d.nc;
+	const float4* A4 = d.A4;
+	const float4* B4 = d.B4;
+	const float4* JA = d.JA;
+	const float4* JB = d.JB;
 	m3d_vec3 n = f4_v3(nc);
 	int32_t count = f2i(nc.w);
-	for (int j = 0; j < count; ++j)
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = pidx(C, c, j);
-		float4 b4 = C.pB[p];
+		if (j >= count)
+		{
+			break;
+		}
+		float4 b4 = B4[j];
 		float wN = b4.w;
 		if (wN <= 0.0f)
 		{
 			continue;
 		}
-		float4 a4 = C.pA[p];
+		float4 a4 = A4[j];
 		m3d_vec3 rA = m3d_rotate(qa, f4_v3(a4));
 		m3d_vec3 rB = m3d_rotate(qb, f4_v3(b4));
 		float sep = a4.w + m3d_dot(m3d_sub(m3d_add(pb, rB), m3d_add(pa, rA)), n);
@@ -519,31 +551,50 @@ MX_HD void solve_normals_serial(const DContacts& C, int32_t c, m3d_vec3& pa, m3d
 			sep = -maxCorrection;
 		}
 		float dl = -sep / wN;
-		float4 ja = C.pJA[p], jb = C.pJB[p];
-		C.pJA[p].w = ja.w + dl;
+		float4 ja = JA[j], jb = JB[j];
+		C.pJA[pidx(C, c, j)].w = ja.w + dl;
 		apply_axis_correction(pb, qb, imB, dl, n, f4_v3(jb));
 		apply_axis_correction(pa, qa, imA, -dl, n, f4_v3(ja));
 	}
+}
+
+MX_HD void solve_normals_serial(const DContacts& C, int32_t c, m3d_vec3& pa, m3d_quat& qa, float imA, m3d_vec3& pb,
+								m3d_quat& qb, float imB, float maxCorrection)
+{
+	NormalsIn d;
+	load_normals_in(C, c, d);
+	solve_normals_serial_in(C, c, d, pa, qa, imA, pb, qb, imB, maxCorrection);
 }
 
 // 8-lane packet form (SolvePacketNormals, src/xpbd.cpp:1047-1141) applied to
 // one lane: points iterate to the packet's maxPoints; padding points use a
 // zero point with invW = 1 and a zero gate; both bodies' lane copies are
 // updated every point (the caller writes back dynamic bodies only).
-MX_HD void solve_normals_packet(const DContacts& C, int32_t c, int32_t maxPoints, m3d_vec3& pa, m3d_quat& qa,
-								float imA, m3d_vec3& pb, m3d_quat& qb, float imB, float maxCorrection)
+MX_HD void solve_normals_packet_in(const DContacts& C, int32_t c, const NormalsIn& d, int32_t maxPoints,
+								   m3d_vec3& pa, m3d_quat& qa, float imA, m3d_vec3& pb, m3d_quat& qb, float imB,
+								   float maxCorrection)
 {
-	float4 nc = C.normal[c];
+	const float4 nc = d.nc;
+	const float4* A4 = d.A4;
+	const float4* B4 = d.B4;
+	const float4* JA = d.JA;
+	const float4* JB = d.JB;
 	float nx = nc.x, ny = nc.y, nz = nc.z;
 	int32_t count = f2i(nc.w);
-	for (int j = 0; j < maxPoints; ++j)
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
+		if (j >= maxPoints)
+		{
+			break;
+		}
 		bool on = j < count;
 		int32_t p = pidx(C, c, on ? j : 0);
-		float4 a4 = on ? C.pA[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		float4 b4 = on ? C.pB[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		float4 ja = on ? C.pJA[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		float4 jb = on ? C.pJB[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		float4 a4 = on ? A4[j] : zero;
+		float4 b4 = on ? B4[j] : zero;
+		float4 ja = on ? JA[j] : zero;
+		float4 jb = on ? JB[j] : zero;
 		float active = on ? 1.0f : 0.0f;
 		float invW = (on && b4.w > 0.0f) ? 1.0f / b4.w : 1.0f;
 
@@ -612,28 +663,65 @@ MX_HD void solve_normals_packet(const DContacts& C, int32_t c, int32_t maxPoints
 	}
 }
 
+MX_HD void solve_normals_packet(const DContacts& C, int32_t c, int32_t maxPoints, m3d_vec3& pa, m3d_quat& qa,
+								float imA, m3d_vec3& pb, m3d_quat& qb, float imB, float maxCorrection)
+{
+	NormalsIn d;
+	load_normals_in(C, c, d);
+	solve_normals_packet_in(C, c, d, maxPoints, pa, qa, imA, pb, qb, imB, maxCorrection);
+}
+
 // -------------------------------------------------------------- friction
 
 // Positional stiction toward persistent anchors, box-box only;
 // src/xpbd.cpp:172-269
-MX_HD void apply_friction(const DContacts& C, int32_t c, bool boxBox, WB& a, WB& b)
+// Contact-side inputs of the stiction pass, loaded at once (every point
+// slot is allocated).
+struct FrictionIn
 {
-	if (!boxBox)
+	float4 nc;
+	float mu;
+	float4 RA0, RB0;
+	float LN[4];
+	float4 FA[4], FB[4], JB[4];
+};
+
+MX_HD void load_friction_in(const DContacts& C, int32_t c, FrictionIn& d)
+{
+	d.nc = C.normal[c];
+	d.mu = C.mat[c].x;
+	int32_t p0 = pidx(C, c, 0);
+	d.RA0 = C.pRA[p0];
+	d.RB0 = C.pRB[p0];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
-		return;
+		int32_t p = pidx(C, c, j);
+		d.LN[j] = C.pJA[p].w;
+		d.FA[j] = C.pFA[p];
+		d.FB[j] = C.pFB[p];
+		d.JB[j] = C.pJB[p];
 	}
-	float4 nc = C.normal[c];
+}
+
+MX_HD void apply_friction_in(const DContacts& C, int32_t c, const FrictionIn& d, WB& a, WB& b)
+{
+	const float4 nc = d.nc;
+	const float mu = d.mu;
+	const float4 RA0 = d.RA0, RB0 = d.RB0;
+	const float* LN = d.LN;
+	const float4* FA = d.FA;
+	const float4* FB = d.FB;
+	const float4* JB = d.JB;
 	int32_t count = f2i(nc.w);
 	if (count == 0)
 	{
 		return;
 	}
 	m3d_vec3 n = f4_v3(nc);
-	float mu = C.mat[c].x;
 	{
 		const float maxSpeed2 = 0.1f * 0.1f;
-		int32_t p0 = pidx(C, c, 0);
-		m3d_vec3 rA0 = f4_v3(C.pRA[p0]), rB0 = f4_v3(C.pRB[p0]);
+		m3d_vec3 rA0 = f4_v3(RA0), rB0 = f4_v3(RB0);
 		m3d_vec3 vrel = m3d_sub(m3d_add(b.v, m3d_cross(b.w, rB0)), m3d_add(a.v, m3d_cross(a.w, rA0)));
 		m3d_vec3 vt = m3d_sub(vrel, m3d_scale(m3d_dot(vrel, n), n));
 		if (m3d_length_sq(vt) > maxSpeed2)
@@ -647,16 +735,21 @@ MX_HD void apply_friction(const DContacts& C, int32_t c, bool boxBox, WB& a, WB&
 			return;
 		}
 	}
-	for (int j = 0; j < count; ++j)
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
+		if (j >= count)
+		{
+			break;
+		}
 		int32_t p = pidx(C, c, j);
-		float lambdaN = C.pJA[p].w;
+		float lambdaN = LN[j];
 		if (lambdaN <= 0.0f)
 		{
 			continue;
 		}
-		m3d_vec3 rA = m3d_rotate(a.q, f4_v3(C.pFA[p]));
-		m3d_vec3 rB = m3d_rotate(b.q, f4_v3(C.pFB[p]));
+		m3d_vec3 rA = m3d_rotate(a.q, f4_v3(FA[j]));
+		m3d_vec3 rB = m3d_rotate(b.q, f4_v3(FB[j]));
 		m3d_vec3 gap = m3d_sub(m3d_add(b.p, rB), m3d_add(a.p, rA));
 		m3d_vec3 dpt = m3d_sub(gap, m3d_scale(m3d_dot(gap, n), n));
 		float lt = m3d_length(dpt);
@@ -671,7 +764,7 @@ MX_HD void apply_friction(const DContacts& C, int32_t c, bool boxBox, WB& a, WB&
 			continue;
 		}
 		float needed = kFrictionRelaxation * (lt / wt);
-		float4 jb = C.pJB[p];
+		float4 jb = JB[j];
 		float lambdaT = jb.w;
 		float allowed = mu * lambdaN - lambdaT;
 		if (allowed <= 0.0f)
@@ -691,6 +784,17 @@ MX_HD void apply_friction(const DContacts& C, int32_t c, bool boxBox, WB& a, WB&
 	}
 }
 
+MX_HD void apply_friction(const DContacts& C, int32_t c, bool boxBox, WB& a, WB& b)
+{
+	if (!boxBox)
+	{
+		return;
+	}
+	FrictionIn d;
+	load_friction_in(C, c, d);
+	apply_friction_in(C, c, d, a, b);
+}
+
 // ------------------------------------------------------------ velocity
 
 struct VelShape
@@ -702,39 +806,83 @@ struct VelShape
 
 // Dynamic friction, restitution (applied unconditionally, XPBD eq. 29) and
 // per-contact rolling resistance; src/xpbd.cpp:275-407.
-MX_HD void solve_velocity(const DContacts& C, int32_t c, float h, float gravityLen, WB& a, WB& b,
-						  const m3d_vec3& pvA, const m3d_vec3& pwA, const m3d_vec3& pvB, const m3d_vec3& pwB,
-						  const VelShape& shA, const VelShape& shB)
+// Contact-side inputs of the velocity pass, loaded at once: manifold,
+// per-point impulses and slip, and the lever-arm sources (anchors for curved
+// pairs, stored arms otherwise).
+struct VelocityIn
 {
-	float4 nc = C.normal[c];
+	float4 nc, mat;
+	float4 JA[4], B4[4], SL[4], JBv[4], AR[4], BR[4];
+};
+
+MX_HD void load_velocity_in(const DContacts& C, int32_t c, VelocityIn& d)
+{
+	d.nc = C.normal[c];
+	d.mat = C.mat[c];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = pidx(C, c, j);
+		d.JA[j] = C.pJA[p];
+		d.B4[j] = C.pB[p];
+		d.SL[j] = C.pSlip[p];
+		d.JBv[j] = C.pJB[p];
+	}
+	bool curved = (f2u(d.mat.w) & CF_CURVED) != 0;
+	const float4* srcA = curved ? C.pA : C.pRA;
+	const float4* srcB = curved ? C.pB : C.pRB;
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = pidx(C, c, j);
+		d.AR[j] = srcA[p];
+		d.BR[j] = srcB[p];
+	}
+}
+
+MX_HD void solve_velocity_in(const DContacts& C, int32_t c, const VelocityIn& d, float h, float gravityLen, WB& a,
+							 WB& b, const m3d_vec3& pvA, const m3d_vec3& pwA, const m3d_vec3& pvB,
+							 const m3d_vec3& pwB, const VelShape& shA, const VelShape& shB)
+{
+	const float4 nc = d.nc, mat = d.mat;
+	const float4* JA = d.JA;
+	const float4* B4 = d.B4;
+	const float4* SL = d.SL;
+	const float4* JBv = d.JBv;
+	const float4* AR = d.AR;
+	const float4* BR = d.BR;
 	m3d_vec3 n = f4_v3(nc);
 	int32_t count = f2i(nc.w);
-	float4 mat = C.mat[c];
 	float mu = mat.x, e0 = mat.y, rr = mat.z;
 	bool curved = (f2u(mat.w) & CF_CURVED) != 0;
 	float invH = h > 0.0f ? 1.0f / h : 0.0f;
 	float lambdaNTotal = 0.0f;
 
-	for (int j = 0; j < count; ++j)
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
+		if (j >= count)
+		{
+			break;
+		}
 		int32_t p = pidx(C, c, j);
-		float4 ja = C.pJA[p];
+		float4 ja = JA[j];
 		float lambdaN = ja.w;
 		if (lambdaN <= 0.0f)
 		{
 			continue;
 		}
-		float4 a4 = C.pA[p], b4 = C.pB[p];
+		float4 b4 = B4[j];
 		m3d_vec3 rAf, rBf;
 		if (curved)
 		{
-			rAf = contact_arm(shA.type, shA.radius, shA.halfHeight, a.q, f4_v3(a4), n, 1.0f);
-			rBf = contact_arm(shB.type, shB.radius, shB.halfHeight, b.q, f4_v3(b4), n, -1.0f);
+			rAf = contact_arm(shA.type, shA.radius, shA.halfHeight, a.q, f4_v3(AR[j]), n, 1.0f);
+			rBf = contact_arm(shB.type, shB.radius, shB.halfHeight, b.q, f4_v3(BR[j]), n, -1.0f);
 		}
 		else
 		{
-			rAf = f4_v3(C.pRA[p]);
-			rBf = f4_v3(C.pRB[p]);
+			rAf = f4_v3(AR[j]);
+			rBf = f4_v3(BR[j]);
 		}
 		m3d_vec3 vrel = m3d_sub(m3d_add(b.v, m3d_cross(b.w, rBf)), m3d_add(a.v, m3d_cross(a.w, rAf)));
 		float vn = m3d_dot(vrel, n);
@@ -747,7 +895,7 @@ MX_HD void solve_velocity(const DContacts& C, int32_t c, float h, float gravityL
 			if (wt > 0.0f)
 			{
 				const float staticSlip = 0.08f;
-				m3d_vec3 slip = f4_v3(C.pSlip[p]);
+				m3d_vec3 slip = f4_v3(SL[j]);
 				bool coherent = m3d_dot(t, slip) > 0.7f;
 				float pStop = vtLen / wt;
 				float pMax = mu * lambdaN * invH;
@@ -769,7 +917,7 @@ MX_HD void solve_velocity(const DContacts& C, int32_t c, float h, float gravityL
 		if (wN > 0.0f)
 		{
 			float jj = (target - vn) / wN;
-			apply_axis_velocity(b, jj, n, f4_v3(C.pJB[p]));
+			apply_axis_velocity(b, jj, n, f4_v3(JBv[j]));
 			apply_axis_velocity(a, -jj, n, f4_v3(ja));
 		}
 		lambdaNTotal += lambdaN > 0.0f ? lambdaN : 0.0f;
@@ -794,6 +942,15 @@ MX_HD void solve_velocity(const DContacts& C, int32_t c, float h, float gravityL
 			}
 		}
 	}
+}
+
+MX_HD void solve_velocity(const DContacts& C, int32_t c, float h, float gravityLen, WB& a, WB& b,
+						  const m3d_vec3& pvA, const m3d_vec3& pwA, const m3d_vec3& pvB, const m3d_vec3& pwB,
+						  const VelShape& shA, const VelShape& shB)
+{
+	VelocityIn d;
+	load_velocity_in(C, c, d);
+	solve_velocity_in(C, c, d, h, gravityLen, a, b, pvA, pwA, pvB, pwB, shA, shB);
 }
 
 // ---------------------------------------------------------------- joints

@@ -44,10 +44,36 @@ enum Counter : int
 	CT_SLOTS,
 	CT_PAR_ENTRIES,
 	CT_JOINT_ISLANDS,
+	CT_GIANT_ISLANDS,
+	CT_COLOR_ROUNDS,
 	CT_COUNT_
 };
 
 constexpr int kCountersSize = 64;
+
+// island mode bits
+constexpr uint32_t IM_AWAKE = 1u;
+constexpr uint32_t IM_COLORED = 2u;
+constexpr uint32_t IM_JOINT = 4u;
+constexpr uint32_t IM_GIANT = 8u;
+// islands with at least this many solver contacts are solved by the whole
+// device (cooperative grid); smaller ones by one workgroup each
+constexpr int32_t kGiantContacts = 4096;
+
+// Device island lists handed from island building to the island solve.
+struct IslandLists
+{
+	const uint32_t* bodyList;     // awake island bodies, by island, ascending
+	const uint32_t* bodyStart;    // per island, exclusive (I + 1)
+	const uint32_t* contactList;  // solver contacts, by island, ascending
+	const uint32_t* contactStart; // per island, exclusive (I + 1)
+	const uint64_t* entryIsland;  // island of each contactList entry
+	const uint32_t* mode;         // per island mode bits
+	const int32_t* islandOf;      // per body
+	const int32_t* jointIsland;   // per joint: awake island or -1
+	int32_t I, NB, NC, giantIslands;
+	int bitsI; // bits of an island id
+};
 
 // ---- k_broad.hip ----
 void launch_refresh(World& w, float dt, bool rebuildStable);
@@ -70,6 +96,9 @@ bool narrowphase(World& w, float dt);
 // ---- k_solve.hip ----
 bool islands_and_solve(World& w, float dt, int substeps);
 void finish_step(World& w, float dt);
+
+// ---- k_isolve.hip ----
+bool solve_islands(World& w, const IslandLists& L, float dt, int substeps);
 
 bool sync_counters(World& w);
 bool push_host_changes(World& w);

@@ -123,6 +123,7 @@ struct BodyColumns
 struct ContactColumns
 {
 	DBuf<int2> bodies;
+	DBuf<int2> hint; // last greedy color and overflow level (giant islands), a warm start
 	DBuf<float4> mat, normal, cachePA, cachePB, cacheQA, cacheQB;
 	DBuf<float4> pA, pB, pJA, pJB, pRA, pRB, pFA, pFB, pSlip; // 4 * cap each
 };
@@ -199,7 +200,11 @@ struct World
 	DBuf<uint64_t> s64a, s64b;
 	DBuf<uint32_t> s32a, s32b, s32c, sSlotStart;
 	DBuf<int32_t> sColor, sLevel, sJointList;
+	DBuf<uint32_t> sCtr;          // convergence counters of the giant-island fixpoints
 	DBuf<int2> sPacket;
+	DBuf<int4> sRec, sRecG;       // solve records (local islands by list entry; giant by sorted entry)
+	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
+	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry
 	DBuf<int32_t> contactColor;   // per contact: -1 / 0..8
 	int32_t lastIslandCount = 0;
 	bool lastHadIslands = false;

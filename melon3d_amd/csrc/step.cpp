@@ -169,6 +169,8 @@ void world_step(World& w, float dt, int substeps)
 	w.stats.stableEntries = (int32_t)w.stableCount;
 	w.stats.refattedBodies = (int32_t)hc[CT_REFAT];
 	w.stats.islands = w.profile.islandCount;
+	w.stats.sweeps = (int32_t)hc[CT_SLOTS];
+	w.stats.colorRounds = (int32_t)hc[CT_COLOR_ROUNDS];
 
 	w.hostStale = true;
 	w.sidecarValid = false;

@@ -51,6 +51,7 @@ World::~World()
 	b.bp.release();
 	b.sb.release();
 	cc.bodies.release();
+	cc.hint.release();
 	djoints.release();
 	jointedKeys.release();
 	activeKeys.release();
@@ -89,7 +90,12 @@ World::~World()
 	sColor.release();
 	sLevel.release();
 	sJointList.release();
+	sCtr.release();
 	sPacket.release();
+	sRec.release();
+	sRecG.release();
+	sBucket.release();
+	sPred.release();
 	contactColor.release();
 	counters.release();
 	moves.release();
@@ -174,7 +180,7 @@ bool ensure_device_contacts(World& w, int32_t n)
 	for (auto* p : cols)
 		if (!realloc_exact(*p, cap, keep))
 			return false;
-	if (!realloc_exact(c.bodies, cap, keep))
+	if (!realloc_exact(c.bodies, cap, keep) || !realloc_exact(c.hint, cap, keep))
 		return false;
 	DBuf<float4>* pts[] = { &c.pA, &c.pB, &c.pJA, &c.pJB, &c.pRA, &c.pRB, &c.pFA, &c.pFB, &c.pSlip };
 	for (auto* p : pts)

@@ -93,6 +93,47 @@ def test_rain_one_step(ref, dev, at):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("at", [1, 40])
+def test_giant_island_one_step(ref, dev, at):
+    """Box pyramid, 90 levels (4095 boxes): one island with far more
+    contacts than a workgroup schedules, solved by the whole-device path
+    (greedy colors and overflow levels as device fixpoints)."""
+    rw = ref.create_world()
+    scenes.pyramid(ref, rw, base=90)
+    for _ in range(at):
+        ref.m3d_world_step(rw, DT, 4)
+    a, b, dw = one_step_from(ref, dev, rw)
+    st = dev.stats(dw)
+    assert st.solvedContacts >= 4096 and st.sweeps > 0, "giant path not taken"
+    assert contact_pairs(a) == contact_pairs(b)
+    d = state_diff(a, b)
+    assert d["pos"] <= TOL_POS and d["rot"] <= TOL_ROT, d
+    assert d["awake_equal"]
+    dev.m3d_world_destroy(dw)
+    ref.m3d_world_destroy(rw)
+
+
+@pytest.mark.gpu
+def test_giant_island_lockstep_colors(ref, dev):
+    """Both engines run the 90-level pyramid from scratch; every step the
+    giant island's greedy colors must match the reference's buckets."""
+    rw, dw = ref.create_world(), dev.create_world()
+    scenes.pyramid(ref, rw, base=90)
+    scenes.pyramid(dev, dw, base=90)
+    for step in range(12):
+        ref.m3d_world_step(rw, DT, 4)
+        dev.m3d_world_step(dw, DT, 4)
+        a, b = ref.export_state(rw), dev.export_state(dw)
+        assert contact_pairs(a) == contact_pairs(b), f"step {step}: contact order"
+        C_ = a.header.contactCount
+        assert ref.contact_colors(rw, C_) == dev.contact_colors(dw, C_), f"step {step}: colors"
+        d = state_diff(a, b)
+        assert d["pos"] <= 1e-4, f"step {step}: {d}"
+    ref.m3d_world_destroy(rw)
+    dev.m3d_world_destroy(dw)
+
+
+@pytest.mark.gpu
 def test_stream_churn_lockstep(ref, dev):
     """Streaming churn (destroy oldest + respawn every step) driven through
     the API on both engines; contact order and events must match."""
