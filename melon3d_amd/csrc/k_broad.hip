@@ -593,66 +593,125 @@ __device__ __forceinline__ bool dedupe_ok(const PairCtx& P, int32_t a, int32_t b
 	return cell_key_of(lo2, P.cellSize) == runKey;
 }
 
-// One thread per run start; emit=0 counts, emit=1 writes keys at offs[i].
-__global__ void k_pairs(PairCtx P, int emit, uint32_t* cnt, const uint32_t* offs, uint64_t* out)
+// Candidate pairs in the reference's order (src/world.cpp:1600-1637): cell
+// runs of the sorted active tier in key order; per run, active x active
+// pairs (i, j > i), then for each stable entry of the same cell (in stable
+// order) the pairs with every active entry of the run. That order is cut
+// into items, one lane each:
+//   A item  an active entry i: its pairs (i, j) for j after i in the run;
+//   B item  a stable entry s:  its pairs (i, s) for every i of the run.
+// Item t of run r sits at runBase[r] + [A items of r, then B items of r], so
+// an exclusive scan of per-item counts gives every item its output range.
+struct RunInfo
+{
+	const uint32_t* flag;   // active entry starts a run
+	const uint32_t* runEx;  // exclusive scan of flag: run index at a start
+	uint32_t* runStart;
+	uint32_t* runEnd;
+	uint32_t* runStabLo;    // first stable entry of the run's cell
+	uint32_t* runItems;     // A + B items per run (zero beyond the last run)
+	const uint32_t* runBase;
+};
+
+__global__ void k_run_flags(const uint64_t* act, int64_t nAct, uint32_t* flag)
 {
 	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= P.nAct)
-		return;
-	uint64_t key = P.act[i];
-	if (i > 0 && P.act[i - 1] == key)
-	{
-		if (!emit)
-			cnt[i] = 0;
-		return;
-	}
-	int64_t e = i + 1;
-	while (e < P.nAct && P.act[e] == key)
-		++e;
-	uint32_t k = 0;
-	uint32_t o = emit ? offs[i] : 0;
-	for (int64_t x = i; x < e; ++x)
-	{
-		int32_t a = (int32_t)P.actBody[x];
-		for (int64_t y = x + 1; y < e; ++y)
-		{
-			int32_t b = (int32_t)P.actBody[y];
-			if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
-			{
-				if (emit)
-					out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
-				++k;
-			}
-		}
-	}
-	// stable entries of the same cell: lower_bound
-	int64_t l = 0, r = P.nStab;
+	if (i < nAct)
+		flag[i] = (i == 0 || act[i] != act[i - 1]) ? 1u : 0u;
+}
+
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t n, uint64_t key)
+{
+	int64_t l = 0, r = n;
 	while (l < r)
 	{
 		int64_t m = (l + r) >> 1;
-		if (P.stab[m] < key)
+		if (a[m] < key)
 			l = m + 1;
 		else
 			r = m;
 	}
-	for (int64_t s = l; s < P.nStab && P.stab[s] == key; ++s)
+	return l;
+}
+
+__global__ void k_run_info(PairCtx P, RunInfo R)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.nAct || !R.flag[i])
+		return;
+	uint64_t key = P.act[i];
+	int64_t e = i + 1;
+	while (e < P.nAct && P.act[e] == key)
+		++e;
+	int64_t l = lower_bound_u64(P.stab, P.nStab, key);
+	int64_t u = l;
+	while (u < P.nStab && P.stab[u] == key)
+		++u;
+	uint32_t r = R.runEx[i];
+	R.runStart[r] = (uint32_t)i;
+	R.runEnd[r] = (uint32_t)e;
+	R.runStabLo[r] = (uint32_t)l;
+	R.runItems[r] = (uint32_t)((e - i) + (u - l));
+}
+
+// A items: one lane per active entry. emit=0 writes the count at cnt[t],
+// emit=1 the keys from out[offs[t]].
+__global__ void k_pairs_a(PairCtx P, RunInfo R, int emit, uint32_t* cnt, const uint32_t* offs, uint64_t* out)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.nAct)
+		return;
+	uint32_t r = R.runEx[i] + R.flag[i] - 1u;
+	int64_t st = R.runStart[r], en = R.runEnd[r];
+	int64_t t = (int64_t)R.runBase[r] + (i - st);
+	uint64_t key = P.act[i];
+	int32_t a = (int32_t)P.actBody[i];
+	uint32_t k = 0;
+	uint64_t o = emit ? offs[t] : 0;
+	for (int64_t y = i + 1; y < en; ++y)
 	{
-		int32_t b = (int32_t)P.stabBody[s];
-		for (int64_t x = i; x < e; ++x)
+		int32_t b = (int32_t)P.actBody[y];
+		if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
 		{
-			int32_t a = (int32_t)P.actBody[x];
-			if (a == b)
-				continue;
-			if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
-			{
-				if (emit)
-					out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
-				++k;
-			}
+			if (emit)
+				out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
+			++k;
 		}
 	}
 	if (!emit)
-		cnt[i] = k;
+		cnt[t] = k;
+}
+
+// B items: one lane per stable entry whose cell has an active run.
+__global__ void k_pairs_b(PairCtx P, RunInfo R, int emit, uint32_t* cnt, const uint32_t* offs, uint64_t* out)
+{
+	int64_t sidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (sidx >= P.nStab)
+		return;
+	uint64_t key = P.stab[sidx];
+	int64_t h = lower_bound_u64(P.act, P.nAct, key);
+	if (h >= P.nAct || P.act[h] != key)
+		return;
+	uint32_t r = R.runEx[h];
+	int64_t st = R.runStart[r], en = R.runEnd[r];
+	int64_t t = (int64_t)R.runBase[r] + (en - st) + (sidx - (int64_t)R.runStabLo[r]);
+	int32_t b = (int32_t)P.stabBody[sidx];
+	uint32_t k = 0;
+	uint64_t o = emit ? offs[t] : 0;
+	for (int64_t x = st; x < en; ++x)
+	{
+		int32_t a = (int32_t)P.actBody[x];
+		if (a == b)
+			continue; // stale stable entry of a body that woke
+		if (dedupe_ok(P, a, b, key) && pair_admissible(P, a, b))
+		{
+			if (emit)
+				out[o + k] = contact_key(a < b ? a : b, a < b ? b : a);
+			++k;
+		}
+	}
+	if (!emit)
+		cnt[t] = k;
 }
 
 // Coarse-list candidate segments, laid out back to back in one index space.
@@ -830,13 +889,40 @@ bool enumerate_and_create(World& w)
 	P.nJointed = w.jointedCount;
 
 	int64_t E = w.activeCount;
+	int64_t SB = w.stableCount;
 	int64_t gridTotal = 0;
+	RunInfo R;
+	uint32_t* itemCnt = nullptr;
+	uint32_t* itemOffs = nullptr;
 	if (E > 0)
 	{
-		if (!w.u32a.ensure(E) || !w.u32b.ensure(E) || !ensure_prim_scratch(w, E))
+		// items: at most one per active entry plus one per stable entry
+		int64_t T = E + SB;
+		if (!w.pairRuns.ensure(7 * (size_t)E + 2 * (size_t)T) || !ensure_prim_scratch(w, T))
 			return false;
-		k_pairs<<<grid_for(E), kBlock, 0, s>>>(P, 0, w.u32a.p, nullptr, nullptr);
-		scan_exclusive(w.u32a.p, w.u32b.p, E, w.counters.p + CT_CANDIDATES, w.primScratch.p, s);
+		uint32_t* base = w.pairRuns.p;
+		uint32_t* flag = base;
+		uint32_t* runEx = base + E;
+		R.flag = flag;
+		R.runEx = runEx;
+		R.runStart = base + 2 * E;
+		R.runEnd = base + 3 * E;
+		R.runStabLo = base + 4 * E;
+		R.runItems = base + 5 * E;
+		uint32_t* runBase = base + 6 * E;
+		R.runBase = runBase;
+		itemCnt = base + 7 * E;
+		itemOffs = itemCnt + T;
+		k_run_flags<<<grid_for(E), kBlock, 0, s>>>(w.activeKeys.p, E, flag);
+		scan_exclusive(flag, runEx, E, nullptr, w.primScratch.p, s);
+		launch_fill_u32(R.runItems, 0u, E, s);
+		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
+		scan_exclusive(R.runItems, runBase, E, nullptr, w.primScratch.p, s);
+		launch_fill_u32(itemCnt, 0u, T, s);
+		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
+		if (SB > 0)
+			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
+		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.primScratch.p, s);
 		if (!sync_counters(w))
 			return false;
 		gridTotal = w.hostCounters[CT_CANDIDATES];
@@ -905,7 +991,11 @@ bool enumerate_and_create(World& w)
 	if (!w.u64a.ensure(total) || !w.u64b.ensure(total))
 		return false;
 	if (gridTotal > 0)
-		k_pairs<<<grid_for(E), kBlock, 0, s>>>(P, 1, nullptr, w.u32b.p, w.u64a.p);
+	{
+		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 1, nullptr, itemOffs, w.u64a.p);
+		if (SB > 0)
+			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 1, nullptr, itemOffs, w.u64a.p);
+	}
 	if (ovAdmitted > 0)
 		k_over_emit<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags, ovOffs, w.u64a.p + gridTotal);
 

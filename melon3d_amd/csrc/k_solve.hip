@@ -14,6 +14,7 @@
 //   sleep     low-pass energy, timers and whole-island sleep
 //             (src/world.cpp:2060-2108)
 #include "mx_steps.h"
+#include "mx_wave.h"
 
 #include <algorithm>
 
@@ -151,14 +152,12 @@ __global__ void k_contact_island(const int2* bodies, const float4* mat, const fl
 								 uint32_t* vals, uint32_t* flag, uint32_t* islandContacts, uint32_t* counters)
 {
 	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (c >= C)
-		return;
-	uint32_t cf = f2u(mat[c].w);
-	uint32_t f = 0;
+	bool valid = c < C;
+	uint32_t cf = valid ? f2u(mat[c].w) : 0u;
+	uint32_t f = 0, pc = 0;
 	int32_t isl = -1;
 	if (cf & CF_TOUCHING)
 	{
-		atomicAdd(&counters[CT_TOUCHING], 1u);
 		if (!(cf & CF_SENSOR))
 		{
 			int2 ab = bodies[c];
@@ -167,15 +166,18 @@ __global__ void k_contact_island(const int2* bodies, const float4* mat, const fl
 			if (isl >= 0 && islandAwake[isl])
 			{
 				f = 1;
-				atomicAdd(&islandContacts[isl], 1u);
-				atomicAdd(&counters[CT_SOLVED_CONTACTS], 1u);
-				uint32_t pc = (uint32_t)f2i(normal[c].w);
-				atomicAdd(&counters[CT_SOLVED_POINTS], pc);
-				if (!(cf & CF_CURVED))
-					atomicAdd(&counters[CT_BB_POINTS], pc);
+				pc = (uint32_t)f2i(normal[c].w);
 			}
 		}
 	}
+	// counts aggregated per wave (all lanes reach here)
+	wave_add(&counters[CT_TOUCHING], (cf & CF_TOUCHING) ? 1u : 0u);
+	wave_add(&counters[CT_SOLVED_CONTACTS], f);
+	wave_add(&counters[CT_SOLVED_POINTS], pc);
+	wave_add(&counters[CT_BB_POINTS], (cf & CF_CURVED) ? 0u : pc);
+	wave_add_keyed(islandContacts, f ? (uint32_t)isl : ~0u, 1u);
+	if (!valid)
+		return;
 	flag[c] = f;
 	keys[c] = f ? (uint64_t)isl : 0ull;
 	vals[c] = (uint32_t)c;
@@ -185,15 +187,15 @@ __global__ void k_body_island_keys(const int32_t* islandOf, const uint32_t* isla
 								   uint32_t* vals, uint32_t* flag, uint32_t* islandBodies)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i >= n)
-		return;
-	int32_t isl = islandOf[i];
+	bool valid = i < n;
+	int32_t isl = valid ? islandOf[i] : -1;
 	uint32_t f = (isl >= 0 && islandAwake[isl]) ? 1u : 0u;
+	wave_add_keyed(islandBodies, f ? (uint32_t)isl : ~0u, 1u);
+	if (!valid)
+		return;
 	flag[i] = f;
 	keys[i] = f ? (uint64_t)isl : 0ull;
 	vals[i] = (uint32_t)i;
-	if (f)
-		atomicAdd(&islandBodies[isl], 1u);
 }
 
 __global__ void k_compact_pairs(const uint64_t* keys, const uint32_t* vals, const uint32_t* flag, const uint32_t* offs,
@@ -235,40 +237,39 @@ __global__ void k_sleep_prep(const uint32_t* flags, const int32_t* islandOf, con
 							 int32_t* firstNoSleep)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i >= n)
-		return;
-	int32_t isl = islandOf[i];
-	if (isl < 0 || !islandAwake[isl])
-		return;
-	if (!(flags[i] & BF_ENABLE_SLEEP))
-		atomicMin(&firstNoSleep[isl], i);
+	int32_t isl = i < n ? islandOf[i] : -1;
+	bool hit = isl >= 0 && islandAwake[isl] && !(flags[i] & BF_ENABLE_SLEEP);
+	wave_min_keyed((uint32_t*)firstNoSleep, hit ? (uint32_t)isl : ~0u, (uint32_t)i);
 }
 
 __global__ void k_sleep_update(DBodies B, const int32_t* islandOf, const uint32_t* islandAwake, int32_t n, float dt,
 							   const int32_t* firstNoSleep, uint32_t* minSleepBits)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i >= n)
-		return;
-	int32_t isl = islandOf[i];
-	if (isl < 0 || !islandAwake[isl] || i >= firstNoSleep[isl])
-		return;
-	m3d_vec3 v = f4_v3(B.vel[i]), w = f4_v3(B.angVel[i]);
-	float4 T = B.iw1[i], E = B.iw2[i];
-	float sleepTime = T.w, se = E.w;
-	float energy = m3d_length_sq(v) + m3d_length_sq(w);
-	se += kSleepEnergyAlpha * (energy - se);
-	if (se > kSleepWakeFactor * kSleepEnergyThresh)
-		sleepTime = 0.0f;
-	else if (se < kSleepEnergyThresh)
-		sleepTime += dt;
-	else
-		sleepTime += 0.5f * dt;
-	T.w = sleepTime;
-	E.w = se;
-	B.iw1[i] = T;
-	B.iw2[i] = E;
-	atomicMin(&minSleepBits[isl], f2u(sleepTime));
+	int32_t isl = i < n ? islandOf[i] : -1;
+	bool live = isl >= 0 && islandAwake[isl] && i < firstNoSleep[isl];
+	uint32_t key = ~0u, bits = 0u;
+	if (live)
+	{
+		m3d_vec3 v = f4_v3(B.vel[i]), w = f4_v3(B.angVel[i]);
+		float4 T = B.iw1[i], E = B.iw2[i];
+		float sleepTime = T.w, se = E.w;
+		float energy = m3d_length_sq(v) + m3d_length_sq(w);
+		se += kSleepEnergyAlpha * (energy - se);
+		if (se > kSleepWakeFactor * kSleepEnergyThresh)
+			sleepTime = 0.0f;
+		else if (se < kSleepEnergyThresh)
+			sleepTime += dt;
+		else
+			sleepTime += 0.5f * dt;
+		T.w = sleepTime;
+		E.w = se;
+		B.iw1[i] = T;
+		B.iw2[i] = E;
+		key = (uint32_t)isl;
+		bits = f2u(sleepTime); // non-negative: orders like the float
+	}
+	wave_min_keyed(minSleepBits, key, bits);
 }
 
 __global__ void k_sleep_apply(DBodies B, const int32_t* islandOf, const uint32_t* islandAwake, int32_t n,
@@ -297,9 +298,11 @@ __global__ void k_sleep_apply(DBodies B, const int32_t* islandOf, const uint32_t
 __global__ void k_finish(DBodies B, int32_t n, float dt, int clearForces, uint32_t* counters)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	uint8_t f = i < n ? B.sb[i] : (uint8_t)0;
+	wave_add(&counters[CT_BODY_COUNT], (f & SB_IN_USE) ? 1u : 0u);
+	wave_add(&counters[CT_AWAKE_BODIES], ((f & SB_IN_USE) && (f & SB_AWAKE)) ? 1u : 0u);
 	if (i >= n)
 		return;
-	uint8_t f = B.sb[i];
 	if ((f & SB_KINEMATIC) && (f & SB_AWAKE))
 	{
 		float4 P = B.pos[i];
@@ -312,12 +315,6 @@ __global__ void k_finish(DBodies B, int32_t n, float dt, int clearForces, uint32
 	{
 		B.force[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		B.torque[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	}
-	if (f & SB_IN_USE)
-	{
-		atomicAdd(&counters[CT_BODY_COUNT], 1u);
-		if (f & SB_AWAKE)
-			atomicAdd(&counters[CT_AWAKE_BODIES], 1u);
 	}
 }
 

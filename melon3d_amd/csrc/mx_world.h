@@ -191,6 +191,7 @@ struct World
 	DBuf<int32_t> i32a, i32b, i32c, i32d, i32e;
 	DBuf<uint8_t> u8a;
 	DBuf<char> primScratch;
+	DBuf<uint32_t> pairRuns;      // pair enumeration: run tables, item counts and offsets
 	DBuf<uint64_t> hashKeys;
 	DBuf<uint32_t> hashVals;
 	DBuf<int32_t> islandOf;       // per body: island id (-1)

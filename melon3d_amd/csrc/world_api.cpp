@@ -76,6 +76,7 @@ World::~World()
 	i32e.release();
 	u8a.release();
 	primScratch.release();
+	pairRuns.release();
 	hashKeys.release();
 	hashVals.release();
 	islandOf.release();
