@@ -393,22 +393,31 @@ extern "C"
 {
 #endif
 
+/* Each declaration replaces the reference's declaration of the same name and
+ * signature; reference lines are given per group (include/melon3d.h:N and the
+ * implementation in src/world.cpp). */
+
+/* include/melon3d.h:452-456; src/world.cpp:396-466 */
 M3D_API m3d_world_def m3d_world_def_default(void);
 M3D_API m3d_body_def m3d_body_def_default(void);
 M3D_API m3d_shape_def m3d_shape_def_default(void);
 M3D_API m3d_hinge_joint_def m3d_hinge_joint_def_default(void);
 M3D_API m3d_weld_joint_def m3d_weld_joint_def_default(void);
 
+/* include/melon3d.h:458-459; src/world.cpp:468-506 */
 M3D_API m3d_world* m3d_world_create(const m3d_world_def* def);
 M3D_API void m3d_world_destroy(m3d_world* world);
 
-/* Advance the simulation by dt with substepCount XPBD substeps (4 is typical). */
+/* Advance the simulation by dt with substepCount XPBD substeps (4 is typical).
+ * include/melon3d.h:462; src/world.cpp:1437-2167 (the accelerated path). */
 M3D_API void m3d_world_step(m3d_world* world, float dt, int substepCount);
 
+/* include/melon3d.h:464-466; src/world.cpp:508-615 */
 M3D_API m3d_body_id m3d_body_create(m3d_world* world, const m3d_body_def* bodyDef, const m3d_shape_def* shapeDef);
 M3D_API void m3d_body_destroy(m3d_world* world, m3d_body_id id);
 M3D_API bool m3d_body_is_valid(m3d_world* world, m3d_body_id id);
 
+/* include/melon3d.h:468-493; src/world.cpp:617-852 */
 M3D_API m3d_vec3 m3d_body_position(m3d_world* world, m3d_body_id id);
 M3D_API m3d_quat m3d_body_rotation(m3d_world* world, m3d_body_id id);
 M3D_API m3d_transform m3d_body_transform(m3d_world* world, m3d_body_id id);
@@ -435,9 +444,11 @@ M3D_API void m3d_body_set_friction(m3d_world* world, m3d_body_id id, float frict
 M3D_API float m3d_body_restitution(m3d_world* world, m3d_body_id id);
 M3D_API void m3d_body_set_restitution(m3d_world* world, m3d_body_id id, float restitution);
 
+/* include/melon3d.h:495-497; src/world.cpp:854-864 */
 M3D_API m3d_vec3 m3d_world_gravity(m3d_world* world);
 M3D_API void m3d_world_set_gravity(m3d_world* world, m3d_vec3 gravity);
 
+/* include/melon3d.h:499-507; src/world.cpp:866-1079 */
 M3D_API m3d_joint_id m3d_joint_create_distance(m3d_world* world, const m3d_distance_joint_def* def);
 M3D_API m3d_joint_id m3d_joint_create_ball(m3d_world* world, const m3d_ball_joint_def* def);
 M3D_API m3d_joint_id m3d_joint_create_hinge(m3d_world* world, const m3d_hinge_joint_def* def);
@@ -447,7 +458,8 @@ M3D_API float m3d_joint_hinge_angle(m3d_world* world, m3d_joint_id id);
 M3D_API void m3d_joint_hinge_set_motor(m3d_world* world, m3d_joint_id id, bool enable, float motorSpeed,
 									   float maxMotorTorque);
 
-/* Closest hit along origin + t*translation, t in [0,1]. */
+/* Closest hit along origin + t*translation, t in [0,1].
+ * include/melon3d.h:510-522; src/world.cpp:1081-1416 */
 M3D_API m3d_ray_result m3d_world_ray_cast(m3d_world* world, m3d_vec3 origin, m3d_vec3 translation);
 M3D_API m3d_ray_result m3d_world_ray_cast_filtered(m3d_world* world, m3d_vec3 origin, m3d_vec3 translation,
 												   uint32_t maskBits);
@@ -458,7 +470,8 @@ M3D_API m3d_ray_result m3d_world_sphere_cast(m3d_world* world, m3d_vec3 origin, 
 typedef bool (*m3d_overlap_fn)(m3d_body_id body, void* context);
 M3D_API void m3d_world_overlap_aabb(m3d_world* world, m3d_aabb aabb, m3d_overlap_fn fn, void* context);
 
-/* Begin/end touch events of the last step; valid until the next step. */
+/* Begin/end touch events of the last step; valid until the next step.
+ * include/melon3d.h:525-527; src/world.cpp:1418-1431 */
 M3D_API m3d_contact_events m3d_world_contact_events(m3d_world* world);
 
 M3D_API m3d_profile m3d_world_profile(m3d_world* world);
