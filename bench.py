@@ -2,19 +2,29 @@
 """Melon3D world-step throughput on one MI355X.
 
 A "step" is one m3d_world_step (dt = 1/60 s, 4 substeps) of the whole world.
-Default workload: the 20 000-body rain pile (1/3 spheres r~U[0.2,0.5],
-1/3 capsules r~U[0.15,0.3] hh~U[0.2,0.6], 1/3 boxes he~U[0.2,0.6], uniform
-orientations, friction U[0.3,0.8], dropped from a 40x40x60 m column);
-W untimed warmup steps, then K timed steps. Inputs are resident on the
-device when timing starts; m3d_world_step returns after the device
-finished the step, so the wall clock around the K calls is the step time.
 
-CPU baseline: the reference engine built from its own sources with its
-release flags (-O3 -mavx2 -mfma), multithreaded on this host's cores,
-stepping a bounded sample from the exact state the GPU had when its timed
-region started.
+Default workload: the streaming scene, 200 000 bodies in a 200 x 200 m arena
+(1/3 spheres r~U[0.2,0.5], 1/3 capsules r~U[0.15,0.3] hh~U[0.2,0.6], 1/3
+boxes he~U[0.2,0.6], uniform orientations, friction U[0.3,0.8], seed 777);
+every step despawns the 1000 oldest bodies and spawns 1000 at y~U[30,60]
+with downward speed U[5,20] m/s, in native code, inside the timed region
+(as the reference's bench does, bench/bench.cpp:243-253). `--scene rain`
+runs the 20 000-body rain pile instead.
 
-Prints one JSON line (see keys below).
+W untimed warmup steps, then K timed steps. State is resident on the device
+when timing starts; m3d_world_step returns after the device finished the
+step, so the wall clock around the K steps is the step time.
+
+roofline: the dominant timed kernel (largest summed HIP-event time on the
+library's stream over the timed steps): algorithmic bytes per launch (from
+the step counters, byte model in DESIGN.md) / average launch duration.
+
+cpu_baseline: the reference engine built from its own sources with its
+release flags (-O3 -mavx2 -mfma), workerCount = this host's cores, stepping
+a bounded sample from the exact state the GPU had when its timed region
+started (same churn sequence).
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -28,15 +38,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 60.0
 
 
 def traffic_bytes(st, substeps: int) -> float:
-    """Compulsory HBM bytes of one step (fp32 SoA, every live field read or
-    written once per pass): per awake body per substep 184 B, per manifold
-    point 52 B (+24 B friction anchors for box-box points), per solved
-    contact 40 B; per body broadphase 88 B (+24 B on re-fat); 32 B per grid
-    entry; 640 B per regenerated manifold, 120 B per cached contact; islands
-    and coloring 16 B per solved contact + 12 B per body."""
+    """Compulsory HBM bytes of one whole step (fp32 SoA, every live field
+    read or written once per pass): per awake body per substep 184 B, per
+    manifold point 52 B (+24 B friction anchors for box-box points), per
+    solved contact 40 B; per body broadphase 88 B (+24 B on re-fat); 32 B per
+    grid entry; 640 B per regenerated manifold, 120 B per cached contact;
+    islands and coloring 16 B per solved contact + 12 B per body."""
     S = substeps
     N = st.bodyCapacity
     per_sub = 184 * st.awakeDynamicBodies + 52 * st.solvedPoints + 24 * st.boxBoxPoints + 40 * st.solvedContacts
@@ -49,94 +60,112 @@ def build_scene(lib, w, scene: str, bodies: int):
     from melon3d_amd import scenes
 
     if scene == "rain":
-        scenes.rain_pile(lib, w, count=bodies, seed=12345, column=(40.0, 40.0, 60.0))
+        scenes.native_rain_pile(lib, w, count=bodies, seed=12345, column=(40.0, 40.0, 60.0))
         return None, bodies
     if scene == "pyramid":
         ids = scenes.pyramid(lib, w, base=20)
         return None, len(ids)
     if scene == "stream":
-        st = scenes.Stream(lib, w, count=bodies, churn=max(1, bodies // 200), seed=777)
+        st = scenes.NativeStream(lib, w, count=bodies, churn=max(1, bodies // 200), seed=777, arena=200.0)
         return st, bodies
     raise SystemExit(f"unknown scene {scene}")
 
 
-def cpu_baseline(snap, steps_max: int, budget_s: float, substeps: int, dyn: int):
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, dyn: int):
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libm3dref_fast.so")
     if not os.path.exists(ref_path):
         return None
     from melon3d_amd import api
 
     ref = api.load_library(ref_path)
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
+    cores = host_cores()
     snap.header.workerCount = cores
     rw = ref.import_state(snap)
+    rs = stream.clone_onto(ref, rw) if stream is not None else None
     t0 = time.perf_counter()
     k = 0
     while k < steps_max:
-        ref.m3d_world_step(rw, 1.0 / 60.0, substeps)
+        if rs is not None:
+            rs.churn_step()
+        ref.m3d_world_step(rw, DT, substeps)
         k += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     ref.m3d_world_destroy(rw)
     return {"value": dyn * k / dt, "unit": "body-steps/s", "cores": cores, "kind": "reference",
-            "sample": f"{k} steps of the same world from the GPU's timed-region start state, "
+            "sample": f"{k} steps ({'with the same churn, ' if rs else ''}from the GPU's timed-region start state), "
                       f"reference built -O3 -mavx2 -mfma, workerCount={cores}"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--scene", default="rain", choices=["rain", "stream", "pyramid"])
-    ap.add_argument("--bodies", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid"])
+    ap.add_argument("--bodies", type=int, default=None)
     ap.add_argument("--substeps", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(batched worlds shard across GPUs by world range; see DESIGN.md)"}))
+                                   "(batched worlds would shard across GPUs by world range)"}))
         sys.exit(2)
+    if args.bodies is None:
+        args.bodies = {"stream": 200000, "rain": 20000, "pyramid": 210}[args.scene]
 
     from melon3d_amd import api
 
     dev = api.load()
     w = dev.create_world()
+    t_build = time.perf_counter()
     stream, dyn = build_scene(dev, w, args.scene, args.bodies)
+    print(f"# scene built in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         if stream:
             stream.churn_step()
-        dev.m3d_world_step(w, 1.0 / 60.0, args.substeps)
-        if (i + 1) % 100 == 0:
+        dev.m3d_world_step(w, DT, args.substeps)
+        if (i + 1) % 50 == 0:
             print(f"# warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
     snap = None if args.no_cpu else dev.export_state(w)
+    cpu_stream = stream.clone_onto(dev, w) if (stream and snap is not None) else None  # churn state at t0
 
-    traffic = []
+    dev.m3dx_set_kernel_timing(w, 1)
+    traffic = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
         if stream:
             stream.churn_step()
-        dev.m3d_world_step(w, 1.0 / 60.0, args.substeps)
-        traffic.append(traffic_bytes(dev.stats(w), args.substeps))
+        dev.m3d_world_step(w, DT, args.substeps)
+        traffic += traffic_bytes(dev.stats(w), args.substeps)
     elapsed = time.perf_counter() - t0
+    kt = dev.kernel_timing(w)
+    dev.m3dx_set_kernel_timing(w, 0)
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
     ms = 1000.0 * elapsed / args.steps
     value = dyn * args.steps / elapsed
-    bytes_per_step = sum(traffic) / len(traffic)
-    achieved = bytes_per_step / (elapsed / args.steps) / 1e9
+
+    # dominant timed kernel with a byte model
+    name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[2] > 0), key=lambda x: x[1][1],
+                                        default=("none", (0, 0.0, 0.0)))
+    avg_s = (kms / 1000.0) / launches if launches else 0.0
+    achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
 
     cpu = None
     if snap is not None:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(snap, args.cpu_steps, args.cpu_budget, args.substeps, dyn)
+        cpu = cpu_baseline(snap, cpu_stream, args.cpu_steps, args.cpu_budget, args.substeps, dyn)
 
     out = {
         "metric": "body-steps/sec",
@@ -151,19 +180,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.scene} {dyn} bodies, dt 1/60 s, {args.substeps} substeps",
+        "config": {"workload": f"{args.scene} {dyn} bodies" + (f", {stream and max(1, dyn // 200)} spawned and "
+                                                                   f"despawned per step" if stream else "")
+                               + f", dt 1/60 s, {args.substeps} substeps",
                    "bodies": dyn, "contacts": prof.contactCount, "touching": prof.touchingContactCount,
-                   "islands": prof.islandCount, "awake": prof.awakeBodyCount, "sweeps": st.sweeps,
+                   "islands": prof.islandCount, "awake": prof.awakeBodyCount, "giant_sweeps": st.sweeps,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "scope": "whole step (algorithmic bytes / step wall time)"},
+                     "launches": launches, "avg_launch_us": avg_s * 1e6,
+                     "bytes_per_launch": (kbytes / launches) if launches else 0.0},
+        "step_traffic_model": {"bytes_per_step": traffic / args.steps,
+                               "achieved_GBps": traffic / elapsed / 1e9},
+        "kernel_ms_per_step": {n: v[1] / args.steps for n, v in kt.items()},
         "cpu_baseline": cpu,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     dev.m3d_world_destroy(w)
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

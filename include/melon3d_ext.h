@@ -233,6 +233,33 @@ typedef struct m3dx_step_stats
 
 M3D_API void m3dx_world_stats(m3d_world* world, m3dx_step_stats* out);
 
+/* ------------------------------------------------------- kernel timing */
+
+/* Kernels timed with HIP events on the world's stream when timing is on. */
+typedef enum m3dx_kernel
+{
+	M3DX_K_GIANT_NORMALS = 0, /* giant islands: normal projection passes */
+	M3DX_K_GIANT_FRICTION,    /* giant islands: stiction pass */
+	M3DX_K_GIANT_VELOCITY,    /* giant islands: velocity pass */
+	M3DX_K_GIANT_ORDER,       /* giant islands: coloring and level fixpoints */
+	M3DX_K_JOINT_ISLANDS,     /* islands with joints (one lane each) */
+	M3DX_K_NARROW,            /* narrowphase */
+	M3DX_K_PAIRS,             /* pair counting and emission */
+	M3DX_K_COUNT
+} m3dx_kernel;
+
+typedef struct m3dx_kernel_time
+{
+	int32_t launches;   /* launches since timing was (re)enabled */
+	float ms;           /* summed event time */
+	double bytes;       /* summed algorithmic bytes (0: no byte model) */
+} m3dx_kernel_time;
+
+/* enable != 0 starts (and resets) timing, 0 stops it. Returns 0 or -1. */
+M3D_API int m3dx_set_kernel_timing(m3d_world* world, int enable);
+/* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
+M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

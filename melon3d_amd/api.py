@@ -262,6 +262,12 @@ class m3dx_step_stats(C.Structure):
         "sweeps", "colorRounds")]
 
 
+class m3dx_kernel_time(C.Structure):
+    _fields_ = [("launches", C.c_int32), ("ms", C.c_float), ("bytes", C.c_double)]
+
+
+M3DX_KERNELS = ["grid_normals", "grid_friction", "grid_velocity", "grid_order", "joint_islands", "narrow", "pairs"]
+
 W = C.c_void_p  # m3d_world*
 overlap_fn = C.CFUNCTYPE(C.c_bool, m3d_body_id, C.c_void_p)
 
@@ -323,6 +329,8 @@ _SIGS = {
     "m3dx_debug_islands": (C.c_int, [W, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_uint8), C.c_int]),
     "m3dx_debug_contact_colors": (C.c_int, [W, C.POINTER(C.c_int32), C.c_int]),
     "m3dx_world_stats": (None, [W, C.POINTER(m3dx_step_stats)]),
+    "m3dx_set_kernel_timing": (C.c_int, [W, C.c_int]),
+    "m3dx_kernel_timing": (C.c_int, [W, C.POINTER(m3dx_kernel_time)]),
     "m3dx_last_error": (C.c_char_p, []),
     "m3dx_set_device": (C.c_int, []),
 }
@@ -426,6 +434,12 @@ class Library:
         buf = (C.c_int32 * max(count, 1))()
         n = self.m3dx_debug_contact_colors(world, buf, count)
         return list(buf[: min(n, count)])
+
+    def kernel_timing(self, world):
+        """{kernel name: (launches, ms, algorithmic bytes)} since timing was enabled."""
+        arr = (m3dx_kernel_time * len(M3DX_KERNELS))()
+        self.m3dx_kernel_timing(world, arr)
+        return {n: (arr[i].launches, arr[i].ms, arr[i].bytes) for i, n in enumerate(M3DX_KERNELS)}
 
     def stats(self, world):
         s = m3dx_step_stats()

@@ -62,6 +62,21 @@ def build_library(jobs: int = 6) -> str:
     return LIB
 
 
+SCENES_LIB = os.path.join(HERE, "libm3d_scenes.so")
+
+
+def build_scene_driver() -> str:
+    """Native scene driver for the benchmark (plain C, host only)."""
+    tmp = SCENES_LIB + ".tmp"
+    cmd = [os.environ.get("CC", "gcc"), "-O2", "-std=c11", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "scene_driver.c"), "-o", tmp, "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("scene driver build failed:\n" + r.stderr)
+    os.replace(tmp, SCENES_LIB)
+    return SCENES_LIB
+
+
 def build_oracle(reference: str = "/root/reference") -> None:
     """C restatement always; the reference build only where its tree exists."""
     oracle = os.path.join(ROOT, "oracle")
@@ -75,5 +90,6 @@ def build_oracle(reference: str = "/root/reference") -> None:
 
 if __name__ == "__main__":
     print(build_library())
+    build_scene_driver()
     build_oracle()
     sys.exit(0)

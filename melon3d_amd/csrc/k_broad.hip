@@ -919,9 +919,11 @@ bool enumerate_and_create(World& w)
 		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
 		scan_exclusive(R.runItems, runBase, E, nullptr, w.primScratch.p, s);
 		launch_fill_u32(itemCnt, 0u, T, s);
+		w.mark_begin(M3DX_K_PAIRS);
 		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
 		if (SB > 0)
 			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
+		w.mark_end();
 		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.primScratch.p, s);
 		if (!sync_counters(w))
 			return false;
@@ -992,9 +994,11 @@ bool enumerate_and_create(World& w)
 		return false;
 	if (gridTotal > 0)
 	{
+		w.mark_begin(M3DX_K_PAIRS);
 		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 1, nullptr, itemOffs, w.u64a.p);
 		if (SB > 0)
 			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 1, nullptr, itemOffs, w.u64a.p);
+		w.mark_end();
 	}
 	if (ovAdmitted > 0)
 		k_over_emit<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags, ovOffs, w.u64a.p + gridTotal);
@@ -1202,6 +1206,42 @@ __global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const 
 	m.y = ma.z > mb.z ? ma.z : mb.z;
 	m.z = sqrtf(ma.w * mb.w);
 	C.mat[c] = m;
+}
+
+// WakeBodyInternal (src/world.cpp:121-140) for listed slots on the device.
+__global__ void k_wake_list(DBodies B, const int32_t* idx, int32_t count, int32_t n)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	int32_t i = idx[k];
+	if (i < 0 || i >= n || !(B.flags[i] & BF_IN_USE) || bf_type(B.flags[i]) == M3D_BODY_STATIC)
+		return;
+	B.flags[i] |= BF_AWAKE;
+	B.sb[i] |= SB_AWAKE;
+	float4 a = B.iw1[i];
+	a.w = 0.0f;
+	B.iw1[i] = a;
+	float4 e = B.iw2[i];
+	e.w = 4.0f * kSleepEnergyThresh;
+	B.iw2[i] = e;
+}
+
+bool launch_wake_list(World& w)
+{
+	if (w.deviceWake.empty())
+		return true;
+	w.wakeUpload.swap(w.deviceWake);
+	w.deviceWake.clear();
+	int32_t count = (int32_t)w.wakeUpload.size();
+	if (!w.dWake.ensure(count))
+		return false;
+	if (!hip_ok(hipMemcpyAsync(w.dWake.p, w.wakeUpload.data(), count * sizeof(int32_t), hipMemcpyHostToDevice,
+							   w.stream),
+				"wake list"))
+		return false;
+	k_wake_list<<<grid_for(count), kBlock, 0, w.stream>>>(body_view(w), w.dWake.p, count, w.deviceBodyCount);
+	return true;
 }
 
 void launch_material_refresh(World& w, const uint8_t* dFlags)

@@ -1,25 +1,26 @@
-// Island solve kernels. Every awake island takes one of three paths, chosen
-// by island shape only (so results never depend on the launch geometry):
+// Island solve kernels. Awake islands take one of two paths:
 //
 //   joint islands   one lane runs the reference's serial island schedule,
 //                   joints included (src/xpbd.cpp:898-974, :1167-1369);
-//   local islands   (< kGiantContacts contacts) one workgroup per island:
-//                   lane 0 computes the exact schedule in LDS (greedy
-//                   first-fit colors for islands with >= 300 contacts, then
-//                   serial-order levels of the overflow pass; serial-order
-//                   levels for the rest), then all 256 lanes run the substep
-//                   loop with workgroup barriers between Gauss-Seidel slots;
-//   giant islands   every workgroup of a cooperative grid: the greedy
-//                   coloring and overflow levels are least fixpoints of their
+//   all others      together, on a cooperative grid. Each contact gets a
+//                   slot: colored islands (>= 300 contacts) their greedy
+//                   first-fit color 0..7 (src/xpbd.cpp:1179-1235), then
+//                   8 + the serial-order level of their overflow pass; other
+//                   islands 8 + the serial-order level of their contacts.
+//                   Colors and levels are least fixpoints of their
 //                   recurrences, iterated in place until a round changes
-//                   nothing, then the substep loop with grid barriers.
+//                   nothing; then the substep loop sweeps the slots with a
+//                   grid barrier between slots.
 //
 // A slot never writes one body twice, so each slot is a plain parallel
 // pass, and every body sees its constraints in the reference's order:
 // colors 0..7 as the colored path's buckets (packet arithmetic, 8
 // consecutive bucket members per packet), the overflow and the small
-// islands in serial contact order (serial arithmetic).
+// islands in serial contact order (serial arithmetic). Islands are
+// body-disjoint, so sweeping all of them slot by slot together changes
+// nothing for any one of them.
 #include "mx_steps.h"
+#include "mx_wave.h"
 
 #include <algorithm>
 #include <atomic>
@@ -222,7 +223,6 @@ struct IslandCtx
 	const int32_t* jointIsland;
 	uint32_t* colorMask; // per body (joint-island lanes)
 	int32_t* contactColor;
-	int4* rec; // per contact-list entry: solve records of the local islands
 	m3d_vec3 gravity;
 	float invH;
 	int32_t iterations, substeps, I, NB, NC;
@@ -372,206 +372,6 @@ __global__ void k_solve_joint_islands(IslandCtx X)
 	}
 }
 
-// ------------------------------------------------------ local islands
-
-constexpr int kLocalThreads = 256;
-constexpr int kSlotCap = kGiantContacts + 16;
-
-// One pass over a local island's slots by its workgroup, with a workgroup
-// barrier after every non-empty slot. Each lane's first entry of the next
-// slot is loaded before that barrier.
-template <class Pass>
-__device__ void local_sweep(const PassCtx& P, const int4* rec, const uint16_t* sStart, int32_t nSlots)
-{
-	const int32_t tid = (int32_t)threadIdx.x;
-	int32_t e = -1;
-	int4 r = make_int4(0, 0, 0, 0);
-	typename Pass::In d;
-	if (nSlots > 0 && (int32_t)sStart[0] + tid < (int32_t)sStart[1])
-	{
-		e = (int32_t)sStart[0] + tid;
-		r = rec[e];
-		Pass::load(P, r, d);
-	}
-	for (int32_t s = 0; s < nSlots; ++s)
-	{
-		int32_t st = sStart[s], en = sStart[s + 1];
-		if (e >= 0)
-		{
-			Pass::run(P, r, d);
-			for (int32_t e2 = e + kLocalThreads; e2 < en; e2 += kLocalThreads)
-				run_rec<Pass>(P, rec[e2]);
-		}
-		e = -1;
-		if (s + 1 < nSlots && (int32_t)sStart[s + 1] + tid < (int32_t)sStart[s + 2])
-		{
-			e = (int32_t)sStart[s + 1] + tid;
-			r = rec[e];
-			Pass::load(P, r, d);
-		}
-		if (st != en)
-			__syncthreads();
-	}
-}
-
-__device__ __forceinline__ uint16_t local_body(const uint32_t* list, int32_t n, int32_t body)
-{
-	int32_t l = 0, r = n;
-	while (l < r)
-	{
-		int32_t m = (l + r) >> 1;
-		if ((int32_t)list[m] < body)
-			l = m + 1;
-		else
-			r = m;
-	}
-	return (uint16_t)l;
-}
-
-__global__ void __launch_bounds__(kLocalThreads) k_solve_local(IslandCtx X)
-{
-	const int32_t isl = (int32_t)blockIdx.x;
-	const uint32_t m = X.mode[isl];
-	if (!(m & IM_AWAKE) || (m & (IM_JOINT | IM_GIANT)))
-		return;
-	const int tid = (int)threadIdx.x;
-	const PassCtx& P = X.P;
-	const int32_t b0 = (int32_t)X.bodyStart[isl];
-	const int32_t nb = (int32_t)X.bodyStart[isl + 1] - b0;
-	const int32_t c0 = (int32_t)X.contactStart[isl];
-	const int32_t K = (int32_t)X.contactStart[isl + 1] - c0;
-	// a non-joint island with K contacts has at most K + 1 bodies; giant
-	// islands never reach this kernel, so both fit the LDS arrays
-	if (K >= kGiantContacts || nb > kGiantContacts)
-		return;
-	const bool colored = K >= kBigIslandContacts;
-	const uint32_t* bl = X.bodyList + b0;
-	const uint32_t* cl = X.contactList + c0;
-	int4* rec = X.rec + c0;
-
-	__shared__ uint16_t sLa[kGiantContacts], sLb[kGiantContacts], sSlot[kGiantContacts], sOrder[kGiantContacts];
-	__shared__ uint16_t sBody[kGiantContacts + 1];
-	__shared__ uint16_t sStart[kSlotCap + 1], sCursor[kSlotCap + 1];
-	__shared__ int32_t sNSlots;
-
-	// local index of every written (invMass > 0) endpoint; 0xFFFF otherwise
-	for (int32_t k = tid; k < K; k += kLocalThreads)
-	{
-		int2 ab = P.C.bodies[cl[k]];
-		sLa[k] = P.B.pos[ab.x].w > 0.0f ? local_body(bl, nb, ab.x) : (uint16_t)0xFFFF;
-		sLb[k] = P.B.pos[ab.y].w > 0.0f ? local_body(bl, nb, ab.y) : (uint16_t)0xFFFF;
-	}
-	__syncthreads();
-
-	// exact schedule, one lane (src/xpbd.cpp:1179-1235 coloring; serial order)
-	if (tid == 0)
-	{
-		int32_t maxSlot = -1;
-		if (colored)
-		{
-			for (int32_t x = 0; x < nb; ++x)
-				sBody[x] = 0;
-			for (int32_t k = 0; k < K; ++k)
-			{
-				uint16_t la = sLa[k], lb = sLb[k];
-				uint32_t used = (la != 0xFFFF ? sBody[la] : 0u) | (lb != 0xFFFF ? sBody[lb] : 0u);
-				int32_t color = (int32_t)__builtin_ctz(~used);
-				if (color >= kMaxGraphColors)
-				{
-					sSlot[k] = 0xFFFF;
-					continue;
-				}
-				sSlot[k] = (uint16_t)color;
-				if (la != 0xFFFF)
-					sBody[la] = (uint16_t)(sBody[la] | (1u << color));
-				if (lb != 0xFFFF)
-					sBody[lb] = (uint16_t)(sBody[lb] | (1u << color));
-				maxSlot = color > maxSlot ? color : maxSlot;
-			}
-		}
-		// serial-order levels (all contacts, or the colored path's overflow)
-		for (int32_t x = 0; x < nb; ++x)
-			sBody[x] = 0;
-		for (int32_t k = 0; k < K; ++k)
-		{
-			if (colored && sSlot[k] != 0xFFFF)
-				continue;
-			uint16_t la = sLa[k], lb = sLb[k];
-			int32_t lv = 0;
-			if (la != 0xFFFF)
-				lv = sBody[la];
-			if (lb != 0xFFFF && sBody[lb] > lv)
-				lv = sBody[lb];
-			if (la != 0xFFFF)
-				sBody[la] = (uint16_t)(lv + 1);
-			if (lb != 0xFFFF)
-				sBody[lb] = (uint16_t)(lv + 1);
-			int32_t slot = colored ? kMaxGraphColors + lv : lv;
-			sSlot[k] = (uint16_t)slot;
-			maxSlot = slot > maxSlot ? slot : maxSlot;
-		}
-		int32_t ns = maxSlot + 1;
-		for (int32_t s = 0; s <= ns; ++s)
-			sStart[s] = 0;
-		for (int32_t k = 0; k < K; ++k)
-			sStart[sSlot[k] + 1]++;
-		for (int32_t s = 0; s < ns; ++s)
-		{
-			sStart[s + 1] = (uint16_t)(sStart[s + 1] + sStart[s]);
-			sCursor[s] = sStart[s];
-		}
-		for (int32_t k = 0; k < K; ++k)
-			sOrder[sCursor[sSlot[k]]++] = (uint16_t)k;
-		sNSlots = ns;
-	}
-	__syncthreads();
-	const int32_t nSlots = sNSlots;
-	if (colored)
-	{
-		for (int32_t k = tid; k < K; k += kLocalThreads)
-			X.contactColor[cl[k]] = sSlot[k] < kMaxGraphColors ? (int32_t)sSlot[k] : kMaxGraphColors;
-	}
-
-	for (int sub = 0; sub < X.substeps; ++sub)
-	{
-		for (int32_t k = tid; k < nb; k += kLocalThreads)
-			integrate_body(P.B, (int32_t)bl[k], P.h, X.gravity);
-		__syncthreads();
-		for (int32_t k = tid; k < K; k += kLocalThreads)
-			prepare_contact(P.B, P.C, (int32_t)cl[k], P.h);
-		__syncthreads();
-		// solve records in slot order (packet maxPoints from this substep's
-		// manifolds)
-		for (int32_t r = tid; r < K; r += kLocalThreads)
-		{
-			int32_t k = sOrder[r];
-			int32_t slot = sSlot[k];
-			int32_t mp = -1;
-			if (colored && slot < kMaxGraphColors)
-			{
-				int32_t st = sStart[slot], en = sStart[slot + 1];
-				int32_t ps = st + ((r - st) & ~7);
-				int32_t pe = ps + 8 < en ? ps + 8 : en;
-				mp = 0;
-				for (int32_t q = ps; q < pe; ++q)
-				{
-					int32_t pc = f2i(P.C.normal[cl[sOrder[q]]].w);
-					mp = pc > mp ? pc : mp;
-				}
-			}
-			rec[r] = make_rec(P, (int32_t)cl[k], mp);
-		}
-		__syncthreads();
-		for (int it = 0; it < X.iterations; ++it)
-			local_sweep<NormalsPass>(P, rec, sStart, nSlots);
-		local_sweep<FrictionPass>(P, rec, sStart, nSlots);
-		for (int32_t k = tid; k < nb; k += kLocalThreads)
-			reconstruct_velocity(P.B, (int32_t)bl[k], X.invH);
-		__syncthreads();
-		local_sweep<VelocityPass>(P, rec, sStart, nSlots);
-	}
-}
-
 // ------------------------------------------------------ giant islands
 
 constexpr int kGiantThreads = 256;
@@ -581,7 +381,8 @@ struct GiantCtx
 	IslandCtx X;
 	const uint32_t* adjStart; // per body (n + 1)
 	const uint32_t* adj;      // contact-list entries sorted by (body, entry)
-	const int4* pred;         // per entry: predecessor ranges in adj (x < 0: not giant)
+	const int4* pred;         // per entry: predecessor ranges in adj (x < 0: not on the grid path)
+	const uint8_t* colored;   // per entry: island takes greedy colors
 	int32_t* color;           // per entry
 	int32_t* level;           // per entry
 	uint32_t* ctr;            // 6 convergence counters (zeroed before launch)
@@ -593,7 +394,7 @@ struct GiantCtx
 
 __device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
 {
-	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & (IM_GRID | IM_JOINT)) == IM_GRID;
 }
 
 // Barrier of a co-resident grid: one monotonic arrival counter (zeroed
@@ -763,7 +564,10 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 			if (pr.x < 0)
 				continue;
 			if (WHICH == 0)
-				changes += color_round(G, k, pr) ? 1u : 0u;
+			{
+				if (G.colored[k])
+					changes += color_round(G, k, pr) ? 1u : 0u;
+			}
 			else if (G.color[k] == kMaxGraphColors)
 				changes += level_round(G, k, pr) ? 1u : 0u;
 		}
@@ -787,6 +591,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 	if (tid == 0)
 		G.counters[CT_COLOR_ROUNDS] = (uint32_t)(rc + rl);
 	// slot keys: (slot, island) for giant entries, a sentinel above all others
+	uint32_t maxSlot = 0;
 	for (int32_t k = tid; k < G.X.NC; k += nt)
 	{
 		uint64_t key = ~0ull;
@@ -794,26 +599,37 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 		{
 			int32_t col = G.color[k];
 			int32_t slot = col < kMaxGraphColors ? col : kMaxGraphColors + G.level[k];
-			G.X.contactColor[G.X.contactList[k]] = col;
+			if (G.colored[k])
+				G.X.contactColor[G.X.contactList[k]] = col;
 			G.X.P.C.hint[G.X.contactList[k]] = make_int2(col, G.level[k]);
 			key = ((uint64_t)slot << G.bitsI) | G.X.entryIsland[k];
-			atomicMax(&G.counters[CT_SLOTS], (uint32_t)slot + 1u);
+			maxSlot = (uint32_t)slot + 1u > maxSlot ? (uint32_t)slot + 1u : maxSlot;
 		}
 		G.slotKeys[k] = key;
 		G.slotVals[k] = G.X.contactList[k];
 	}
+	// one atomic per wave (all lanes reach here)
+	for (int off = 32; off > 0; off >>= 1)
+	{
+		uint32_t o = __shfl_xor(maxSlot, off);
+		maxSlot = o > maxSlot ? o : maxSlot;
+	}
+	if (__lane_id() == 0 && maxSlot > 0u)
+		atomicMax(&G.counters[CT_SLOTS], maxSlot);
 }
 
 // Adjacency pairs (body, entry) of the giant entries, and the fixpoint's
 // starting point: the contact's color and level of its last solve (any start
 // converges to the same least fixpoint; a close one in fewer rounds).
-__global__ void k_giant_adj(IslandCtx X, int4* pred, int32_t* color, int32_t* level, int32_t n, uint64_t* keys,
-							uint32_t* vals, uint32_t* bodyCount)
+__global__ void k_giant_adj(IslandCtx X, int4* pred, uint8_t* colored, int32_t* color, int32_t* level, int32_t n,
+							uint64_t* keys, uint32_t* vals, uint32_t* bodyCount)
 {
 	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (k >= X.NC)
 		return;
-	bool g = (X.mode[(int32_t)X.entryIsland[k]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+	uint32_t m = X.mode[(int32_t)X.entryIsland[k]];
+	bool g = (m & (IM_GRID | IM_JOINT)) == IM_GRID;
+	bool col = g && (m & IM_COLORED);
 	int2 ab = X.P.C.bodies[X.contactList[k]];
 	bool da = g && X.P.B.pos[ab.x].w > 0.0f;
 	bool db = g && X.P.B.pos[ab.y].w > 0.0f;
@@ -826,8 +642,10 @@ __global__ void k_giant_adj(IslandCtx X, int4* pred, int32_t* color, int32_t* le
 	if (db)
 		atomicAdd(&bodyCount[ab.y], 1u);
 	pred[k] = g ? make_int4(0, 0, 0, 0) : make_int4(-1, -1, -1, -1);
+	colored[k] = col ? 1u : 0u;
 	int2 h = g ? X.P.C.hint[X.contactList[k]] : make_int2(0, 0);
-	color[k] = h.x < 0 ? 0 : (h.x > kMaxGraphColors ? kMaxGraphColors : h.x);
+	// islands below the colored threshold: every contact in the serial pass
+	color[k] = !col ? kMaxGraphColors : (h.x < 0 ? 0 : (h.x > kMaxGraphColors ? kMaxGraphColors : h.x));
 	level[k] = h.y < 0 ? 0 : (h.y > (1 << 24) ? (1 << 24) : h.y);
 }
 
@@ -918,6 +736,7 @@ struct GiantSolveCtx
 	const int2* packet;
 	const uint32_t* slotStart; // [0, nSlots]
 	const uint32_t* counters;
+	uint32_t* counters2;       // the same block, for the record counters
 	uint32_t* barrier;         // zeroed arrival counter
 	int4* rec;                 // solve records, grouped within each slot
 	const uint64_t* keys;      // sorted (slot, island) keys
@@ -1013,7 +832,7 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 // holds only that pass's registers).
 __device__ __forceinline__ bool giant_body(const IslandCtx& X, int32_t b)
 {
-	return (X.mode[X.islandOf[b]] & (IM_GIANT | IM_JOINT)) == IM_GIANT;
+	return (X.mode[X.islandOf[b]] & (IM_GRID | IM_JOINT)) == IM_GRID;
 }
 
 __global__ void k_giant_bodies(GiantSolveCtx S, int op)
@@ -1098,22 +917,29 @@ __global__ void k_giant_records(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
 	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
-		return;
-	int2 pk = S.packet[e];
-	int32_t mp = -1;
-	if (pk.x >= 0)
+	uint32_t points = 0;
+	bool valid = e < (int32_t)S.slotStart[S.counters[CT_SLOTS]];
+	if (valid)
 	{
-		mp = 0;
-		for (int32_t q = pk.x; q < pk.y; ++q)
+		int2 pk = S.packet[e];
+		int32_t mp = -1;
+		if (pk.x >= 0)
 		{
-			int32_t pc = f2i(P.C.normal[S.entries[q]].w);
-			mp = pc > mp ? pc : mp;
+			mp = 0;
+			for (int32_t q = pk.x; q < pk.y; ++q)
+			{
+				int32_t pc = f2i(P.C.normal[S.entries[q]].w);
+				mp = pc > mp ? pc : mp;
+			}
 		}
+		int32_t c = (int32_t)S.entries[e];
+		uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
+		S.rec[at] = make_rec(P, c, mp);
+		points = (uint32_t)f2i(P.C.normal[c].w);
 	}
-	int32_t c = (int32_t)S.entries[e];
-	uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
-	S.rec[at] = make_rec(P, c, mp);
+	// entries and points of the substep, for the byte model of the passes
+	wave_add(&S.counters2[CT_GIANT_ENTRIES], valid ? 1u : 0u);
+	wave_add(&S.counters2[CT_GIANT_POINTS], points);
 }
 
 // `passes` sweeps of one pass kind. The barrier counter is zeroed again by
@@ -1227,22 +1053,32 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	X.NB = L.NB;
 	X.NC = L.NC;
 
-	if (!w.sRec.ensure(std::max<int32_t>(L.NC, 1)))
-		return false;
-	X.rec = w.sRec.p;
 	if (w.djointCount > 0)
 	{
 		if (!w.u32a.ensure((size_t)n + 1))
 			return false;
 		X.colorMask = w.u32a.p;
+		w.mark_begin(M3DX_K_JOINT_ISLANDS);
 		k_solve_joint_islands<<<gridk(L.I), 256, 0, s>>>(X);
+		w.mark_end();
 	}
-	k_solve_local<<<(unsigned)L.I, kLocalThreads, 0, s>>>(X);
-
-	if (L.giantIslands == 0 || L.NC == 0)
+	if (L.NB == 0)
 		return true;
 
-	// ---- giant islands ----
+	GiantSolveCtx S;
+	S.X = X;
+	if (L.NC == 0)
+	{
+		// no contacts: bodies only (integrate, reconstruct)
+		for (int sub = 0; sub < substeps; ++sub)
+		{
+			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
+			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
+		}
+		return true;
+	}
+
+	// ---- the grid path ----
 	int32_t NC = L.NC;
 	size_t two = 2 * (size_t)NC;
 	int bitsN = 1;
@@ -1254,14 +1090,15 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		++bitsS;
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
-		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) ||
+		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) ||
 		!w.primScratch.ensure(std::max(scan_scratch_bytes((int64_t)two + n + 2), sort_scratch_bytes((int64_t)two)) +
 							  1024))
 		return false;
 	uint32_t* bodyCount = w.s32c.p;
 	launch_fill_u32(bodyCount, 0u, (int64_t)n + 1, s);
-	k_giant_adj<<<gridk(NC), 256, 0, s>>>(X, w.sPred.p, w.sColor.p, w.sLevel.p, n, w.s64a.p, w.s32a.p, bodyCount);
+	k_giant_adj<<<gridk(NC), 256, 0, s>>>(X, w.sPred.p, w.sColored.p, w.sColor.p, w.sLevel.p, n, w.s64a.p, w.s32a.p,
+										  bodyCount);
 	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.primScratch.p, s);
 	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, s);
 	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
@@ -1272,6 +1109,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.adjStart = bodyCount;
 	G.adj = w.s32a.p;
 	G.pred = w.sPred.p;
+	G.colored = w.sColored.p;
 	G.color = w.sColor.p;
 	G.level = w.sLevel.p;
 	G.ctr = w.sCtr.p;
@@ -1283,7 +1121,10 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);
 		int blocks = giant_blocks(w, k_giant_order, NC, cap);
 		void* args[] = { &G };
-		if (!hip_ok(hipLaunchCooperativeKernel((const void*)k_giant_order, dim3(blocks), dim3(kGiantThreads), args, 0, s),
+		w.mark_begin(M3DX_K_GIANT_ORDER);
+		hipError_t err = hipLaunchCooperativeKernel((const void*)k_giant_order, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+		w.mark_end();
+		if (!hip_ok(err,
 					"giant order"))
 			return false;
 	}
@@ -1291,12 +1132,11 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	k_giant_slot_starts<<<gridk(maxS + 1), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, maxS, w.sSlotStart.p);
 	k_giant_packets<<<gridk(NC), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, w.sPacket.p);
 
-	GiantSolveCtx S;
-	S.X = X;
 	S.entries = w.s32b.p;
 	S.packet = w.sPacket.p;
 	S.slotStart = w.sSlotStart.p;
 	S.counters = w.counters.p;
+	S.counters2 = w.counters.p;
 	S.barrier = w.sCtr.p + 9;
 	S.rec = w.sRecG.p;
 	S.keys = w.s64b.p;
@@ -1319,11 +1159,19 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
 		k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
 		k_giant_records<<<gridk(NC), 256, 0, s>>>(S);
-		if (!coop((const void*)k_giant_pass<NormalsPass>, X.iterations) ||
-			!coop((const void*)k_giant_pass<FrictionPass>, 1))
+		w.mark_begin(M3DX_K_GIANT_NORMALS);
+		bool ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
+		w.mark_end();
+		w.mark_begin(M3DX_K_GIANT_FRICTION);
+		ok = ok && coop((const void*)k_giant_pass<FrictionPass>, 1);
+		w.mark_end();
+		if (!ok)
 			return false;
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
-		if (!coop((const void*)k_giant_pass<VelocityPass>, 1))
+		w.mark_begin(M3DX_K_GIANT_VELOCITY);
+		ok = coop((const void*)k_giant_pass<VelocityPass>, 1);
+		w.mark_end();
+		if (!ok)
 			return false;
 	}
 	return true;

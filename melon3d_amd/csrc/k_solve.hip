@@ -330,17 +330,12 @@ __global__ void k_island_modes(const uint32_t* islandAwake, const uint32_t* isla
 	uint32_t m = 0;
 	if (islandAwake[i])
 	{
-		m = IM_AWAKE;
-		int32_t K = (int32_t)islandContacts[i];
-		if (K >= kBigIslandContacts)
+		// joint islands get IM_JOINT afterwards and leave the grid path
+		m = IM_AWAKE | IM_GRID;
+		if ((int32_t)islandContacts[i] >= kBigIslandContacts)
 		{
 			m |= IM_COLORED;
 			atomicAdd(&counters[CT_COLORED_ISLANDS], 1u);
-		}
-		if (K >= kGiantContacts)
-		{
-			m |= IM_GIANT;
-			atomicAdd(&counters[CT_GIANT_ISLANDS], 1u);
 		}
 	}
 	mode[i] = m;
@@ -464,7 +459,6 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		L.I = I;
 		L.NB = NB;
 		L.NC = NC;
-		L.giantIslands = (int32_t)w.hostCounters[CT_GIANT_ISLANDS];
 		L.bitsI = bitsI;
 		if (!solve_islands(w, L, dt, substeps))
 			return false;

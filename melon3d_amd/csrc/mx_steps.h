@@ -46,6 +46,8 @@ enum Counter : int
 	CT_JOINT_ISLANDS,
 	CT_GIANT_ISLANDS,
 	CT_COLOR_ROUNDS,
+	CT_GIANT_ENTRIES, // summed over substeps
+	CT_GIANT_POINTS,  // summed over substeps
 	CT_COUNT_
 };
 
@@ -55,10 +57,8 @@ constexpr int kCountersSize = 64;
 constexpr uint32_t IM_AWAKE = 1u;
 constexpr uint32_t IM_COLORED = 2u;
 constexpr uint32_t IM_JOINT = 4u;
-constexpr uint32_t IM_GIANT = 8u;
-// islands with at least this many solver contacts are solved by the whole
-// device (cooperative grid); smaller ones by one workgroup each
-constexpr int32_t kGiantContacts = 4096;
+// awake islands without joints: solved together by the cooperative grid
+constexpr uint32_t IM_GRID = 8u;
 
 // Device island lists handed from island building to the island solve.
 struct IslandLists
@@ -71,7 +71,7 @@ struct IslandLists
 	const uint32_t* mode;         // per island mode bits
 	const int32_t* islandOf;      // per body
 	const int32_t* jointIsland;   // per joint: awake island or -1
-	int32_t I, NB, NC, giantIslands;
+	int32_t I, NB, NC;
 	int bitsI; // bits of an island id
 };
 
@@ -87,6 +87,8 @@ void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 bool apply_moves(World& w, const int2* dMoves, int32_t count);
 void launch_material_refresh(World& w, const uint8_t* dFlags);
+// applies and clears the host's list of bodies to wake
+bool launch_wake_list(World& w);
 void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const float4* stage, const uint32_t* stageFlags,
 						   const uint4* stageFilter, const uint8_t* stageBp);
 

@@ -164,6 +164,16 @@ struct World
 	std::vector<int2> hcBodies;
 	std::vector<uint32_t> hcFlags;
 	std::vector<int32_t> hcOrigin;   // original device slot currently at each index
+	// contacts per body for destroys: hcList[hcStart[b] .. hcStart[b+1]) holds
+	// the current positions of body b's contacts (-1: removed); hcSlot[c]
+	// locates contact c's two entries
+	std::vector<int32_t> hcStart, hcList;
+	std::vector<int2> hcSlot;
+	// bodies to wake on the device at the next step (partners of destroyed
+	// bodies whose host record is not current)
+	std::vector<int32_t> deviceWake;
+	std::vector<int32_t> wakeUpload; // the list being uploaded (kept until the next push)
+	DBuf<int32_t> dWake;
 	bool pendingPermutation = false;
 
 	// ---- device state ----
@@ -203,9 +213,10 @@ struct World
 	DBuf<int32_t> sColor, sLevel, sJointList;
 	DBuf<uint32_t> sCtr;          // convergence counters of the giant-island fixpoints
 	DBuf<int2> sPacket;
-	DBuf<int4> sRec, sRecG;       // solve records (local islands by list entry; giant by sorted entry)
+	DBuf<int4> sRecG;             // grid path: solve records by sorted entry
 	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
 	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry
+	DBuf<uint8_t> sColored;       // grid path: entry's island is colored
 	DBuf<int32_t> contactColor;   // per contact: -1 / 0..8
 	int32_t lastIslandCount = 0;
 	bool lastHadIslands = false;
@@ -215,6 +226,23 @@ struct World
 	DBuf<int32_t> eventIdx;
 
 	m3dx_step_stats stats{};
+
+	// ---- kernel timing (off unless enabled) ----
+	struct KernelMark
+	{
+		int kernel;
+		hipEvent_t begin, end;
+		double bytes;
+	};
+	bool timing = false;
+	std::vector<KernelMark> marks;     // events of the current step
+	std::vector<hipEvent_t> eventPool; // reused across steps
+	size_t eventsUsed = 0;
+	m3dx_kernel_time ktime[M3DX_K_COUNT]{};
+	// records the begin event of kernel k (no-op unless timing)
+	void mark_begin(int k, double bytes = 0.0);
+	void mark_end();
+	void collect_marks(); // after the step's final synchronization
 
 	World();
 	~World();

@@ -1,0 +1,205 @@
+/*
+ * scene_driver.c - synthetic benchmark scenes driven through the m3d_* API,
+ * in native code so that a streaming scene's per-step spawns and despawns
+ * cost what they cost a C caller (the reference's bench does them in C++,
+ * bench/bench.cpp:209-253).
+ *
+ * The driver calls the engine through a table of function pointers, so the
+ * same code drives any library exporting the API (this backend, or the
+ * reference build used as the CPU baseline).
+ *
+ * Random draws: the LCG of bench/bench.cpp:121-125 (s = s*1664525 +
+ * 1013904223, u = (s >> 8) / 0xFFFFFF), in double precision; per-body draw
+ * order as in melon3d_amd/scenes.py (position, size draws for the kind,
+ * quaternion, friction), so a scene built here equals the Python builder's.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "melon3d.h"
+
+typedef struct m3ds_api
+{
+	m3d_body_def (*body_def_default)(void);
+	m3d_shape_def (*shape_def_default)(void);
+	m3d_body_id (*body_create)(m3d_world*, const m3d_body_def*, const m3d_shape_def*);
+	void (*body_destroy)(m3d_world*, m3d_body_id);
+} m3ds_api;
+
+typedef struct m3ds_stream
+{
+	m3ds_api api;
+	m3d_world* world;
+	uint32_t rng;
+	int32_t count;
+	int32_t churn;
+	int32_t cursor;
+	float arena;
+	m3d_body_id* ring;
+} m3ds_stream;
+
+static double lcg_u(uint32_t* s)
+{
+	*s = *s * 1664525u + 1013904223u;
+	return (double)(*s >> 8) / (double)0x00FFFFFF;
+}
+
+static double lcg_uniform(uint32_t* s, double a, double b) { return a + (b - a) * lcg_u(s); }
+
+static const double kPi = 3.14159265358979323846;
+
+static m3d_body_id mixed_body(const m3ds_api* api, m3d_world* w, uint32_t* rng, int kind, double x, double y, double z,
+							  double vy)
+{
+	m3d_body_def bd = api->body_def_default();
+	bd.type = M3D_BODY_DYNAMIC;
+	bd.position.x = (float)x;
+	bd.position.y = (float)y;
+	bd.position.z = (float)z;
+	bd.linearVelocity.x = 0.0f;
+	bd.linearVelocity.y = (float)vy;
+	bd.linearVelocity.z = 0.0f;
+	m3d_shape_def sd = api->shape_def_default();
+	if (kind == 0)
+	{
+		sd.type = M3D_SHAPE_SPHERE;
+		sd.radius = (float)lcg_uniform(rng, 0.2, 0.5);
+	}
+	else if (kind == 1)
+	{
+		sd.type = M3D_SHAPE_CAPSULE;
+		sd.radius = (float)lcg_uniform(rng, 0.15, 0.3);
+		sd.halfHeight = (float)lcg_uniform(rng, 0.2, 0.6);
+	}
+	else
+	{
+		sd.type = M3D_SHAPE_BOX;
+		double hx = lcg_uniform(rng, 0.2, 0.6);
+		double hy = lcg_uniform(rng, 0.2, 0.6);
+		double hz = lcg_uniform(rng, 0.2, 0.6);
+		sd.halfExtents.x = (float)hx;
+		sd.halfExtents.y = (float)hy;
+		sd.halfExtents.z = (float)hz;
+	}
+	/* Shoemake's uniform rotation from three uniforms */
+	double u1 = lcg_u(rng), u2 = lcg_u(rng), u3 = lcg_u(rng);
+	double a = sqrt(1.0 - u1), b = sqrt(u1);
+	bd.orientation.x = (float)(a * sin(2.0 * kPi * u2));
+	bd.orientation.y = (float)(a * cos(2.0 * kPi * u2));
+	bd.orientation.z = (float)(b * sin(2.0 * kPi * u3));
+	bd.orientation.w = (float)(b * cos(2.0 * kPi * u3));
+	sd.friction = (float)lcg_uniform(rng, 0.3, 0.8);
+	return api->body_create(w, &bd, &sd);
+}
+
+static void add_ground(const m3ds_api* api, m3d_world* w, float hx, float hy, float hz, float y)
+{
+	m3d_body_def bd = api->body_def_default();
+	bd.type = M3D_BODY_STATIC;
+	bd.position.x = 0.0f;
+	bd.position.y = y;
+	bd.position.z = 0.0f;
+	m3d_shape_def sd = api->shape_def_default();
+	sd.type = M3D_SHAPE_BOX;
+	sd.halfExtents.x = hx;
+	sd.halfExtents.y = hy;
+	sd.halfExtents.z = hz;
+	sd.friction = 0.6f;
+	(void)api->body_create(w, &bd, &sd);
+}
+
+/* Rain pile: kind = i % 3, positions uniform in a gx * gy * gz column starting
+ * 1 m above the ground (draws x, y, z, then the body's draws). */
+int m3ds_rain(const m3ds_api* api, m3d_world* w, int count, uint32_t seed, float gx, float gz, float gy)
+{
+	add_ground(api, w, gx > 50.0f ? gx : 50.0f, 1.0f, gz > 50.0f ? gz : 50.0f, -1.0f);
+	uint32_t rng = seed;
+	for (int i = 0; i < count; ++i)
+	{
+		double x = lcg_uniform(&rng, -0.5 * gx, 0.5 * gx);
+		double y = 1.0 + lcg_uniform(&rng, 0.0, gy);
+		double z = lcg_uniform(&rng, -0.5 * gz, 0.5 * gz);
+		(void)mixed_body(api, w, &rng, i % 3, x, y, z, 0.0);
+	}
+	return count;
+}
+
+/* Streaming scene: count bodies uniform in an arena x 50 x arena volume; each
+ * churn step destroys the churn oldest (FIFO ring) and spawns as many at
+ * y in [30, 60] with downward speed U[5, 20] m/s. */
+m3ds_stream* m3ds_stream_create(const m3ds_api* api, m3d_world* w, int count, int churn, uint32_t seed, float arena)
+{
+	m3ds_stream* st = (m3ds_stream*)calloc(1, sizeof(m3ds_stream));
+	if (!st)
+		return NULL;
+	st->ring = (m3d_body_id*)malloc(sizeof(m3d_body_id) * (size_t)(count > 0 ? count : 1));
+	if (!st->ring)
+	{
+		free(st);
+		return NULL;
+	}
+	st->api = *api;
+	st->world = w;
+	st->rng = seed;
+	st->count = count;
+	st->churn = churn;
+	st->arena = arena;
+	add_ground(api, w, 0.5f * arena + 10.0f, 1.0f, 0.5f * arena + 10.0f, -1.0f);
+	for (int i = 0; i < count; ++i)
+	{
+		double x = lcg_uniform(&st->rng, -0.5 * arena, 0.5 * arena);
+		double y = 1.0 + lcg_uniform(&st->rng, 0.0, 50.0);
+		double z = lcg_uniform(&st->rng, -0.5 * arena, 0.5 * arena);
+		st->ring[i] = mixed_body(api, w, &st->rng, i % 3, x, y, z, 0.0);
+	}
+	return st;
+}
+
+void m3ds_stream_churn(m3ds_stream* st)
+{
+	for (int k = 0; k < st->churn && st->count > 0; ++k)
+	{
+		st->api.body_destroy(st->world, st->ring[st->cursor]);
+		double x = lcg_uniform(&st->rng, -0.5 * st->arena, 0.5 * st->arena);
+		double y = lcg_uniform(&st->rng, 30.0, 60.0);
+		double z = lcg_uniform(&st->rng, -0.5 * st->arena, 0.5 * st->arena);
+		double vy = -lcg_uniform(&st->rng, 5.0, 20.0);
+		st->ring[st->cursor] = mixed_body(&st->api, st->world, &st->rng, st->cursor % 3, x, y, z, vy);
+		st->cursor = (st->cursor + 1) % st->count;
+	}
+}
+
+/* The same stream state continued on another world holding the same bodies
+ * (a world imported from this one's snapshot), through another library. */
+m3ds_stream* m3ds_stream_clone(const m3ds_stream* src, const m3ds_api* api, m3d_world* w)
+{
+	m3ds_stream* st = (m3ds_stream*)calloc(1, sizeof(m3ds_stream));
+	if (!st)
+		return NULL;
+	st->ring = (m3d_body_id*)malloc(sizeof(m3d_body_id) * (size_t)(src->count > 0 ? src->count : 1));
+	if (!st->ring)
+	{
+		free(st);
+		return NULL;
+	}
+	memcpy(st->ring, src->ring, sizeof(m3d_body_id) * (size_t)src->count);
+	st->api = *api;
+	st->world = w;
+	st->rng = src->rng;
+	st->count = src->count;
+	st->churn = src->churn;
+	st->cursor = src->cursor;
+	st->arena = src->arena;
+	return st;
+}
+
+void m3ds_stream_free(m3ds_stream* st)
+{
+	if (st)
+	{
+		free(st->ring);
+		free(st);
+	}
+}

@@ -95,7 +95,9 @@ static bool apply_material_refresh(World& w)
 // Host changes made since the last step reach the device here.
 bool push_host_changes(World& w)
 {
-	return apply_pending_permutation(w) && upload_dirty(w) && upload_joints(w) && apply_material_refresh(w);
+	// wakes first: uploaded dirty records are authoritative and come after
+	return apply_pending_permutation(w) && launch_wake_list(w) && upload_dirty(w) && upload_joints(w) &&
+		   apply_material_refresh(w);
 }
 
 // RefreshBroadphase; src/world.cpp:156-229
@@ -147,6 +149,23 @@ void world_step(World& w, float dt, int substeps)
 	if (!download_joints(w) || !sync_counters(w))
 		return;
 	const uint32_t* hc = w.hostCounters;
+	if (w.timing)
+	{
+		// algorithmic bytes of the timed kernels this step:
+		// normals per contact and pass 160 B + 68 B per point; velocity per
+		// contact 472 B + 112 B per point; narrowphase 640 B per regenerated
+		// manifold, 120 B per cached contact
+		double ge = (double)hc[CT_GIANT_ENTRIES], gp = (double)hc[CT_GIANT_POINTS];
+		int iters = w.def.positionIterations > 0 ? w.def.positionIterations : 2;
+		double regen = (double)hc[CT_REGEN];
+		double stepBytes[M3DX_K_COUNT] = {};
+		stepBytes[M3DX_K_GIANT_NORMALS] = iters * (160.0 * ge + 68.0 * gp);
+		stepBytes[M3DX_K_GIANT_VELOCITY] = 472.0 * ge + 112.0 * gp;
+		stepBytes[M3DX_K_NARROW] = 640.0 * regen + 120.0 * std::max(0.0, (double)w.contactCount - regen);
+		w.collect_marks();
+		for (int k = 0; k < M3DX_K_COUNT; ++k)
+			w.ktime[k].bytes += stepBytes[k];
+	}
 	if (hc[CT_STABLE_DIRTY])
 		w.stableDirty = true;
 	w.anyForcesApplied = false;

@@ -93,18 +93,67 @@ World::~World()
 	sJointList.release();
 	sCtr.release();
 	sPacket.release();
-	sRec.release();
 	sRecG.release();
 	sBucket.release();
 	sPred.release();
+	sColored.release();
+	dWake.release();
 	contactColor.release();
 	counters.release();
 	moves.release();
 	eventIdx.release();
 	if (hostCounters != nullptr)
 		(void)hipHostFree(hostCounters);
+	for (hipEvent_t e : eventPool)
+		(void)hipEventDestroy(e);
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------- kernel timing
+
+void World::mark_begin(int k, double bytes)
+{
+	if (!timing)
+		return;
+	while (eventPool.size() < eventsUsed + 2)
+	{
+		hipEvent_t e = nullptr;
+		if (hipEventCreate(&e) != hipSuccess)
+			return;
+		eventPool.push_back(e);
+	}
+	KernelMark m;
+	m.kernel = k;
+	m.begin = eventPool[eventsUsed++];
+	m.end = eventPool[eventsUsed++];
+	m.bytes = bytes;
+	(void)hipEventRecord(m.begin, stream);
+	marks.push_back(m);
+}
+
+void World::mark_end()
+{
+	if (!timing || marks.empty())
+		return;
+	(void)hipEventRecord(marks.back().end, stream);
+}
+
+void World::collect_marks()
+{
+	for (const KernelMark& m : marks)
+	{
+		float ms = 0.0f;
+		if (hipEventElapsedTime(&ms, m.begin, m.end) == hipSuccess)
+		{
+			m3dx_kernel_time& t = ktime[m.kernel];
+			t.launches += 1;
+			t.ms += ms;
+			t.bytes += m.bytes;
+		}
+	}
+	marks.clear();
+	eventsUsed = 0;
 }
 
 // ------------------------------------------------------------- transfers
@@ -341,7 +390,10 @@ bool download_bodies(World& w)
 void ensure_host(World& w)
 {
 	if (w.hostStale)
+	{
+		launch_wake_list(w); // pending wakes belong to the state being read
 		download_bodies(w);
+	}
 }
 
 void mark_dirty(World& w, int32_t i)
@@ -374,6 +426,38 @@ bool ensure_sidecar(World& w)
 	}
 	for (int32_t c = 0; c < C; ++c)
 		w.hcOrigin[c] = c;
+	// per-body index of contact positions (counting sort by body)
+	int32_t n = (int32_t)w.bodies.size();
+	w.hcStart.assign((size_t)n + 1, 0);
+	for (int32_t c = 0; c < C; ++c)
+	{
+		int2 b = w.hcBodies[c];
+		if (b.x >= 0 && b.x < n)
+			w.hcStart[b.x + 1]++;
+		if (b.y >= 0 && b.y < n)
+			w.hcStart[b.y + 1]++;
+	}
+	for (int32_t i = 0; i < n; ++i)
+		w.hcStart[i + 1] += w.hcStart[i];
+	w.hcList.assign((size_t)w.hcStart[n], -1);
+	w.hcSlot.assign(C, make_int2(-1, -1));
+	std::vector<int32_t> fill(w.hcStart.begin(), w.hcStart.end() - 1);
+	for (int32_t c = 0; c < C; ++c)
+	{
+		int2 b = w.hcBodies[c];
+		int2 sl = make_int2(-1, -1);
+		if (b.x >= 0 && b.x < n)
+		{
+			sl.x = fill[b.x]++;
+			w.hcList[sl.x] = c;
+		}
+		if (b.y >= 0 && b.y < n)
+		{
+			sl.y = fill[b.y]++;
+			w.hcList[sl.y] = c;
+		}
+		w.hcSlot[c] = sl;
+	}
 	w.sidecarValid = true;
 	w.pendingPermutation = false;
 	return true;
@@ -700,10 +784,25 @@ m3d_body_id m3d_body_create(m3d_world* world, const m3d_body_def* bd, const m3d_
 }
 
 // src/world.cpp:569-615
+// Wakes a destroyed body's partner: on the host record when that record is
+// current (the mirror is fresh, or the body is dirty and its record will be
+// uploaded), else through the device wake list (only the wake fields change).
+static void wake_partner(World& w, int32_t i)
+{
+	if (!w.hostStale || (w.dirtyFlag.size() > (size_t)i && w.dirtyFlag[i]))
+	{
+		wake_body_host(w, i);
+		return;
+	}
+	if (w.bodies[i].type == M3D_BODY_STATIC)
+		return;
+	w.stableDirty = true;
+	w.deviceWake.push_back(i);
+}
+
 void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 {
 	World& w = world->w;
-	ensure_host(w);
 	HostBody* b = body_checked(w, id);
 	if (b == nullptr)
 		return;
@@ -711,34 +810,54 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 	if (ensure_sidecar(w))
 	{
 		// contacts of this body, removed in descending index order with
-		// swap-with-last (src/world.cpp:100-119)
-		for (int32_t i = (int32_t)w.hcBodies.size() - 1; i >= 0; --i)
+		// swap-with-last (src/world.cpp:576-593, :100-119). A descending scan
+		// only ever swaps in a contact it has already passed, so the body's
+		// contacts go in descending order of their current positions.
+		std::vector<int32_t> mine;
+		if (index + 1 < (int32_t)w.hcStart.size())
+		{
+			for (int32_t s = w.hcStart[index]; s < w.hcStart[index + 1]; ++s)
+				if (w.hcList[s] >= 0)
+					mine.push_back(w.hcList[s]);
+		}
+		std::sort(mine.begin(), mine.end(), [](int32_t a, int32_t b) { return a > b; });
+		for (int32_t i : mine)
 		{
 			int2 c = w.hcBodies[i];
-			if (c.x != index && c.y != index)
-				continue;
 			int32_t other = c.x == index ? c.y : c.x;
 			uint32_t cf = w.hcFlags[i];
 			if (cf & CF_TOUCHING)
 			{
 				if (!(cf & CF_SENSOR))
-					wake_body_host(w, other);
+					wake_partner(w, other);
 				m3d_contact_event ev;
 				ev.bodyA = make_id(w, c.x);
 				ev.bodyB = make_id(w, c.y);
 				ev.isSensor = (cf & CF_SENSOR) != 0;
 				w.endEvents.push_back(ev);
 			}
+			int2 gone = w.hcSlot[i];
+			if (gone.x >= 0)
+				w.hcList[gone.x] = -1;
+			if (gone.y >= 0)
+				w.hcList[gone.y] = -1;
 			int32_t last = (int32_t)w.hcBodies.size() - 1;
 			if (i != last)
 			{
 				w.hcBodies[i] = w.hcBodies[last];
 				w.hcFlags[i] = w.hcFlags[last];
 				w.hcOrigin[i] = w.hcOrigin[last];
+				int2 sl = w.hcSlot[last];
+				w.hcSlot[i] = sl;
+				if (sl.x >= 0)
+					w.hcList[sl.x] = i;
+				if (sl.y >= 0)
+					w.hcList[sl.y] = i;
 			}
 			w.hcBodies.pop_back();
 			w.hcFlags.pop_back();
 			w.hcOrigin.pop_back();
+			w.hcSlot.pop_back();
 			w.pendingPermutation = true;
 		}
 	}
@@ -1959,5 +2078,27 @@ int m3dx_debug_contact_colors(m3d_world* world, int32_t* colorOfContact, int cap
 }
 
 void m3dx_world_stats(m3d_world* world, m3dx_step_stats* out) { *out = world->w.stats; }
+
+int m3dx_set_kernel_timing(m3d_world* world, int enable)
+{
+	if (world == nullptr)
+		return -1;
+	World& w = world->w;
+	w.timing = enable != 0;
+	w.marks.clear();
+	w.eventsUsed = 0;
+	for (auto& t : w.ktime)
+		t = m3dx_kernel_time{};
+	return 0;
+}
+
+int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out)
+{
+	if (world == nullptr || out == nullptr)
+		return -1;
+	for (int k = 0; k < M3DX_K_COUNT; ++k)
+		out[k] = world->w.ktime[k];
+	return M3DX_K_COUNT;
+}
 
 } // extern "C"

@@ -146,3 +146,84 @@ class Stream:
             vy = -rng.uniform(5.0, 20.0)
             self.ring[self.cursor] = mixed_body(L, w, rng, self.cursor % 3, (x, y, z), (0.0, vy, 0.0))
             self.cursor = (self.cursor + 1) % n
+
+
+# ------------------------------------------------------------ native driver
+
+class _SceneApi(C.Structure):
+    _fields_ = [("body_def_default", C.c_void_p), ("shape_def_default", C.c_void_p),
+                ("body_create", C.c_void_p), ("body_destroy", C.c_void_p)]
+
+
+_driver = None
+
+
+def _scene_driver():
+    """libm3d_scenes.so (built by melon3d_amd/build.py)."""
+    global _driver
+    if _driver is None:
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libm3d_scenes.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing; run python melon3d_amd/build.py")
+        d = C.CDLL(path)
+        d.m3ds_rain.argtypes = [C.POINTER(_SceneApi), C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float]
+        d.m3ds_rain.restype = C.c_int
+        d.m3ds_stream_create.argtypes = [C.POINTER(_SceneApi), C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_float]
+        d.m3ds_stream_create.restype = C.c_void_p
+        d.m3ds_stream_churn.argtypes = [C.c_void_p]
+        d.m3ds_stream_churn.restype = None
+        d.m3ds_stream_clone.argtypes = [C.c_void_p, C.POINTER(_SceneApi), C.c_void_p]
+        d.m3ds_stream_clone.restype = C.c_void_p
+        d.m3ds_stream_free.argtypes = [C.c_void_p]
+        d.m3ds_stream_free.restype = None
+        _driver = d
+    return _driver
+
+
+def _api_table(L: Library) -> _SceneApi:
+    t = _SceneApi()
+    for name, fn in (("body_def_default", "m3d_body_def_default"), ("shape_def_default", "m3d_shape_def_default"),
+                     ("body_create", "m3d_body_create"), ("body_destroy", "m3d_body_destroy")):
+        setattr(t, name, C.cast(getattr(L.lib, fn), C.c_void_p).value)
+    return t
+
+
+def _world_ptr(w) -> int:
+    return C.cast(w, C.c_void_p).value
+
+
+def native_rain_pile(L: Library, w, count=20000, seed=12345, column=(40.0, 40.0, 60.0)) -> int:
+    """rain_pile built in native code (same draws, same bodies)."""
+    gx, gz, gy = column
+    t = _api_table(L)
+    return _scene_driver().m3ds_rain(C.byref(t), _world_ptr(w), count, seed, gx, gz, gy)
+
+
+class NativeStream:
+    """Stream with its per-step churn in native code (as the reference's own
+    bench does it in C++)."""
+
+    def __init__(self, L: Library, w, count=200000, churn=1000, seed=777, arena=200.0, _handle=None):
+        self._table = _api_table(L)
+        self._h = _handle if _handle is not None else _scene_driver().m3ds_stream_create(
+            C.byref(self._table), _world_ptr(w), count, churn, seed, arena)
+        if not self._h:
+            raise RuntimeError("stream driver allocation failed")
+
+    def churn_step(self):
+        _scene_driver().m3ds_stream_churn(self._h)
+
+    def clone_onto(self, L: Library, w) -> "NativeStream":
+        """Continue this stream on a world imported from this one's snapshot."""
+        t = _api_table(L)
+        h = _scene_driver().m3ds_stream_clone(self._h, C.byref(t), _world_ptr(w))
+        s = NativeStream.__new__(NativeStream)
+        s._table, s._h = t, h
+        return s
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _driver is not None:
+            _driver.m3ds_stream_free(h)
+            self._h = None
