@@ -531,4 +531,14 @@ void m3dx_world_stats(m3d_world* w, m3dx_step_stats* out)
 	out->stableEntries = (int32_t)w->grid.m_stable.size();
 }
 
+/* no device kernels on the CPU engine: timing is unavailable */
+int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
+
+int m3dx_kernel_timing(m3d_world*, m3dx_kernel_time* out)
+{
+	if (out != nullptr)
+		memset(out, 0, sizeof(m3dx_kernel_time) * M3DX_K_COUNT);
+	return M3DX_K_COUNT;
+}
+
 } // extern "C"
