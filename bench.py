@@ -16,8 +16,11 @@ when timing starts; m3d_world_step returns after the device finished the
 step, so the wall clock around the K steps is the step time.
 
 roofline: the dominant timed kernel (largest summed HIP-event time on the
-library's stream over the timed steps): algorithmic bytes per launch (from
-the step counters, byte model in DESIGN.md) / average launch duration.
+library's stream, over --timing-steps extra steps after the timed region):
+algorithmic bytes per launch (from the step counters: normals pass 160 B
+per contact + 68 B per point, velocity pass 472 B per contact + 112 B per
+point, narrowphase 640 B per regenerated manifold + 120 B per cached contact)
+/ average launch duration.
 
 cpu_baseline: the reference engine built from its own sources with its
 release flags (-O3 -mavx2 -mfma), workerCount = this host's cores, stepping
@@ -113,10 +116,23 @@ def main():
     ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid"])
     ap.add_argument("--bodies", type=int, default=None)
     ap.add_argument("--substeps", type=int, default=4)
+    ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 256 << 20}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(batched worlds would shard across GPUs by world range)"}))
@@ -140,7 +156,6 @@ def main():
     snap = None if args.no_cpu else dev.export_state(w)
     cpu_stream = stream.clone_onto(dev, w) if (stream and snap is not None) else None  # churn state at t0
 
-    dev.m3dx_set_kernel_timing(w, 1)
     traffic = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -149,12 +164,20 @@ def main():
         dev.m3d_world_step(w, DT, args.substeps)
         traffic += traffic_bytes(dev.stats(w), args.substeps)
     elapsed = time.perf_counter() - t0
-    kt = dev.kernel_timing(w)
-    dev.m3dx_set_kernel_timing(w, 0)
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
     ms = 1000.0 * elapsed / args.steps
     value = dyn * args.steps / elapsed
+
+    # kernel timing in extra steps after the timed region (events add host
+    # work, so they stay out of the timed steps)
+    dev.m3dx_set_kernel_timing(w, 1)
+    for i in range(args.timing_steps):
+        if stream:
+            stream.churn_step()
+        dev.m3d_world_step(w, DT, args.substeps)
+    kt = dev.kernel_timing(w)
+    dev.m3dx_set_kernel_timing(w, 0)
 
     # dominant timed kernel with a byte model
     name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[2] > 0), key=lambda x: x[1][1],
@@ -192,7 +215,7 @@ def main():
                      "bytes_per_launch": (kbytes / launches) if launches else 0.0},
         "step_traffic_model": {"bytes_per_step": traffic / args.steps,
                                "achieved_GBps": traffic / elapsed / 1e9},
-        "kernel_ms_per_step": {n: v[1] / args.steps for n, v in kt.items()},
+        "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
         "cpu_baseline": cpu,
     }
     if cpu:

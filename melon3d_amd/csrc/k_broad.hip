@@ -377,10 +377,10 @@ bool sync_counters(World& w)
 	return hip_ok(hipStreamSynchronize(w.stream), "stream sync");
 }
 
-static bool ensure_prim_scratch(World& w, int64_t n)
+bool ensure_prim_scratch(World& w, int64_t n)
 {
-	size_t need = std::max(scan_scratch_bytes(n), sort_scratch_bytes(n)) + 1024;
-	return w.primScratch.ensure(need);
+	return w.primScratch.ensure(sort_scratch_bytes(n) + 1024) &&
+		   w.scanStatus.ensure((size_t)std::max(scan_status_words(n), sort_status_words(n)));
 }
 
 bool build_tier(World& w, int tier)
@@ -407,12 +407,12 @@ bool build_tier(World& w, int tier)
 	uint32_t* offs = w.u32b.p;
 	uint32_t* ov = w.u32c.p;
 	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, w.def.gridCellSize, cnt, ov);
-	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.primScratch.p, s);
+	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
 	// oversized list in body order
 	uint32_t* ovOff = w.u32d.ensure(n) ? w.u32d.p : nullptr;
 	if (ovOff == nullptr)
 		return false;
-	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.primScratch.p, s);
+	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
 	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
 	if (!sync_counters(w))
 		return false;
@@ -453,7 +453,7 @@ bool build_tier(World& w, int tier)
 	while (bits < 64 && (range - 1) >> bits)
 		++bits;
 	k_key_compress<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
-	radix_sort_pairs(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p, s);
+	radix_sort_pairs(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p, w.scanStatus.p, s);
 	k_key_expand<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
 	return true;
 }
@@ -914,17 +914,17 @@ bool enumerate_and_create(World& w)
 		itemCnt = base + 7 * E;
 		itemOffs = itemCnt + T;
 		k_run_flags<<<grid_for(E), kBlock, 0, s>>>(w.activeKeys.p, E, flag);
-		scan_exclusive(flag, runEx, E, nullptr, w.primScratch.p, s);
+		scan_exclusive(flag, runEx, E, nullptr, w.scanStatus.p, s);
 		launch_fill_u32(R.runItems, 0u, E, s);
 		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
-		scan_exclusive(R.runItems, runBase, E, nullptr, w.primScratch.p, s);
+		scan_exclusive(R.runItems, runBase, E, nullptr, w.scanStatus.p, s);
 		launch_fill_u32(itemCnt, 0u, T, s);
 		w.mark_begin(M3DX_K_PAIRS);
 		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
 		if (SB > 0)
 			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
 		w.mark_end();
-		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.primScratch.p, s);
+		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.scanStatus.p, s);
 		if (!sync_counters(w))
 			return false;
 		gridTotal = w.hostCounters[CT_CANDIDATES];
@@ -980,7 +980,7 @@ bool enumerate_and_create(World& w)
 		ovFlags = w.u32c.p;
 		ovOffs = w.u32d.p;
 		k_over_flags<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags);
-		scan_exclusive(ovFlags, ovOffs, ovTotal, w.counters.p + CT_SEG_COUNT, w.primScratch.p, s);
+		scan_exclusive(ovFlags, ovOffs, ovTotal, w.counters.p + CT_SEG_COUNT, w.scanStatus.p, s);
 		if (!sync_counters(w))
 			return false;
 		ovAdmitted = w.hostCounters[CT_SEG_COUNT];
@@ -1012,7 +1012,7 @@ bool enumerate_and_create(World& w)
 	launch_fill_u32(w.u32a.p, 0xFFFFFFFFu, csz, s);
 	k_first_insert<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1);
 	k_first_flag<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1, w.u32b.p);
-	scan_exclusive(w.u32b.p, w.u32c.p, total, w.counters.p + CT_NEW_CONTACTS, w.primScratch.p, s);
+	scan_exclusive(w.u32b.p, w.u32c.p, total, w.counters.p + CT_NEW_CONTACTS, w.scanStatus.p, s);
 	k_compact_u64<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, w.u32b.p, w.u32c.p, total, w.u64b.p);
 	if (!sync_counters(w))
 		return false;
@@ -1126,7 +1126,7 @@ bool remove_separated(World& w)
 		return false;
 	uint32_t* rm = w.u32a.p;
 	k_remove_flags<<<grid_for(C), kBlock, 0, s>>>(body_view(w), w.cc.bodies.p, C, rm);
-	scan_exclusive(rm, w.u32b.p, C, w.counters.p + CT_REMOVED, w.primScratch.p, s);
+	scan_exclusive(rm, w.u32b.p, C, w.counters.p + CT_REMOVED, w.scanStatus.p, s);
 	if (!sync_counters(w))
 		return false;
 	int32_t R = (int32_t)w.hostCounters[CT_REMOVED];
@@ -1140,9 +1140,9 @@ bool remove_separated(World& w)
 	uint32_t* holeOff = (uint32_t*)w.i32a.p;
 	uint32_t* tailOff = (uint32_t*)w.i32b.p;
 	uint32_t* evOff = (uint32_t*)w.i32c.p;
-	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.primScratch.p, s);
-	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.primScratch.p, s);
-	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.primScratch.p, s);
+	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.scanStatus.p, s);
+	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.scanStatus.p, s);
+	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.scanStatus.p, s);
 	if (!sync_counters(w))
 		return false;
 	int32_t H = (int32_t)w.hostCounters[CT_HOLES];
@@ -1206,6 +1206,49 @@ __global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const 
 	m.y = ma.z > mb.z ? ma.z : mb.z;
 	m.z = sqrtf(ma.w * mb.w);
 	C.mat[c] = m;
+}
+
+__global__ void k_pack_sidecar(const int2* bodies, const float4* mat, int32_t C, int4* out)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c < C)
+	{
+		int2 b = bodies[c];
+		out[c] = make_int4(b.x, b.y, (int32_t)f2u(mat[c].w), 0);
+	}
+}
+
+bool prefetch_sidecar(World& w)
+{
+	int32_t C = w.contactCount;
+	w.sidecarPrefetched = false;
+	if (C == 0)
+		return true;
+	if (w.pinnedSidecarCap < (size_t)C)
+	{
+		if (w.pinnedSidecar != nullptr)
+			(void)hipHostFree(w.pinnedSidecar);
+		w.pinnedSidecar = nullptr;
+		size_t cap = (size_t)C + (size_t)C / 2 + 1024;
+		if (hipHostMalloc((void**)&w.pinnedSidecar, cap * sizeof(int4), 0) != hipSuccess)
+		{
+			w.pinnedSidecarCap = 0;
+			return false;
+		}
+		w.pinnedSidecarCap = cap;
+	}
+	if (w.sidecarReady == nullptr && hipEventCreateWithFlags(&w.sidecarReady, hipEventDisableTiming) != hipSuccess)
+		return false;
+	if (!w.dSidecar.ensure(C))
+		return false;
+	k_pack_sidecar<<<grid_for(C), kBlock, 0, w.stream>>>(w.cc.bodies.p, w.cc.mat.p, C, w.dSidecar.p);
+	if (hipMemcpyAsync(w.pinnedSidecar, w.dSidecar.p, (size_t)C * sizeof(int4), hipMemcpyDeviceToHost, w.stream) !=
+			hipSuccess ||
+		hipEventRecord(w.sidecarReady, w.stream) != hipSuccess)
+		return false;
+	w.sidecarPrefetched = true;
+	w.sidecarPrefetchCount = C;
+	return true;
 }
 
 // WakeBodyInternal (src/world.cpp:121-140) for listed slots on the device.

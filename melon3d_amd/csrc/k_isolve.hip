@@ -1092,15 +1092,14 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
 		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) ||
-		!w.primScratch.ensure(std::max(scan_scratch_bytes((int64_t)two + n + 2), sort_scratch_bytes((int64_t)two)) +
-							  1024))
+		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
 		return false;
 	uint32_t* bodyCount = w.s32c.p;
 	launch_fill_u32(bodyCount, 0u, (int64_t)n + 1, s);
 	k_giant_adj<<<gridk(NC), 256, 0, s>>>(X, w.sPred.p, w.sColored.p, w.sColor.p, w.sLevel.p, n, w.s64a.p, w.s32a.p,
 										  bodyCount);
-	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.primScratch.p, s);
-	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, s);
+	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
+	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, w.scanStatus.p, s);
 	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
 	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6), barriers 8 and 9
 
@@ -1128,7 +1127,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 					"giant order"))
 			return false;
 	}
-	radix_sort_pairs(w.s64b.p, w.s32b.p, w.s64a.p, w.s32a.p, NC, L.bitsI + bitsS + 1, w.primScratch.p, s);
+	radix_sort_pairs(w.s64b.p, w.s32b.p, w.s64a.p, w.s32a.p, NC, L.bitsI + bitsS + 1, w.primScratch.p, w.scanStatus.p, s);
 	k_giant_slot_starts<<<gridk(maxS + 1), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, maxS, w.sSlotStart.p);
 	k_giant_packets<<<gridk(NC), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, w.sPacket.p);
 

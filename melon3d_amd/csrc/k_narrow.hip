@@ -169,14 +169,14 @@ bool narrowphase(World& w, float dt)
 	if (C == 0)
 		return true;
 	if (!w.u8a.ensure(C) || !w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.i32a.ensure(C) ||
-		!w.primScratch.ensure(scan_scratch_bytes(C) + 1024))
+		!ensure_prim_scratch(w, C))
 		return false;
 	int8_t* trans = (int8_t*)w.u8a.p;
 	w.mark_begin(M3DX_K_NARROW);
 	k_narrow<<<gridn(C), kBlockN, 0, s>>>(make_body_view(w), make_contact_view(w), C, dt, trans, w.counters.p);
 	w.mark_end();
 	k_trans_flags<<<gridn(C), kBlockN, 0, s>>>(trans, C, w.u32a.p);
-	scan_exclusive(w.u32a.p, w.u32b.p, C, w.counters.p + CT_TRANSITIONS, w.primScratch.p, s);
+	scan_exclusive(w.u32a.p, w.u32b.p, C, w.counters.p + CT_TRANSITIONS, w.scanStatus.p, s);
 	if (!sync_counters(w))
 		return false;
 	int32_t T = (int32_t)w.hostCounters[CT_TRANSITIONS];

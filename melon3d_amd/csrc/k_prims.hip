@@ -6,6 +6,8 @@
 
 #include <stdio.h>
 
+#include <atomic>
+
 namespace mx
 {
 
@@ -33,80 +35,172 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* ld
 	return incl - v;
 }
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* in, int64_t n, uint32_t* blockSums)
+// ---------------------------------------------------------------- scan
+//
+// Single-pass exclusive scan with decoupled look-back: each workgroup scans
+// its 2048-element tile, publishes the tile sum, then adds the prefix of the
+// tiles before it, read from their published words (an inclusive prefix ends
+// the look-back early). Words are tagged with the call's epoch, so the
+// status array never needs clearing between calls: it is zeroed once when
+// allocated and holds nothing but status words (any other data left in it
+// could carry the current tag and be read as a published sum). A workgroup only waits on lower-numbered ones,
+// which publish their sums without waiting, so the look-back always ends.
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = 256 * kScanItems;
+constexpr uint64_t kFlagAgg = 1ull, kFlagPrefix = 2ull;
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint64_t flag, uint32_t v)
 {
-	__shared__ uint32_t lds[256];
-	int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * 4;
-	uint32_t s = 0;
-	for (int k = 0; k < 4; ++k)
+	return ((uint64_t)epoch << 34) | (flag << 32) | v;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+	const int lane = __lane_id();
+	for (int off = 1; off < 64; off <<= 1)
 	{
-		if (base + k < n)
+		uint32_t o = __shfl_up(v, off);
+		if (lane >= off)
+			v += o;
+	}
+	return v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_onepass(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total,
+													  uint64_t* status, uint32_t epoch)
+{
+	__shared__ uint32_t waveSum[4];
+	__shared__ uint32_t tilePrefix;
+	const int t = threadIdx.x;
+	const int lane = t & 63, wave = t >> 6;
+	const int64_t tile = blockIdx.x;
+	const int64_t base = tile * kScanTile + (int64_t)t * kScanItems;
+	uint32_t v[kScanItems];
+	uint32_t sum = 0;
+	const bool vecIn = ((uintptr_t)in & 15u) == 0u;
+	const bool vecOut = ((uintptr_t)out & 15u) == 0u;
+	if (vecIn && base + kScanItems <= n)
+	{
+		const uint4* p = (const uint4*)(in + base);
+		uint4 a = p[0], b = p[1];
+		v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+		v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
+			v[k] = base + k < n ? in[base + k] : 0u;
+	}
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k)
+		sum += v[k];
+	uint32_t incl = wave_inclusive_sum(sum);
+	if (lane == 63)
+		waveSum[wave] = incl;
+	__syncthreads();
+	uint32_t waveOff = 0, agg = 0;
+	for (int w = 0; w < 4; ++w)
+	{
+		waveOff += w < wave ? waveSum[w] : 0u;
+		agg += waveSum[w];
+	}
+	// publish, then look back (wave 0)
+	if (wave == 0)
+	{
+		uint32_t prefix = 0;
+		if (tile == 0)
 		{
-			s += in[base + k];
+			if (lane == 0)
+				__hip_atomic_store(&status[0], pack_status(epoch, kFlagPrefix, agg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else
+		{
+			if (lane == 0)
+				__hip_atomic_store(&status[tile], pack_status(epoch, kFlagAgg, agg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+			int64_t j = tile - 1; // nearest tile not yet consumed
+			for (;;)
+			{
+				// lane l reads tile j - l; before tile 0 a zero prefix stands in
+				const int64_t idx = j - lane;
+				uint64_t word;
+				uint64_t preMask;
+				int first;
+				for (;;)
+				{
+					word = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+									: pack_status(epoch, kFlagPrefix, 0u);
+					uint64_t flag = (word >> 32) & 3ull;
+					bool ready = (uint32_t)(word >> 34) == epoch && flag != 0ull;
+					uint64_t readyMask = __ballot(ready);
+					preMask = __ballot(ready && flag == kFlagPrefix);
+					first = preMask ? __ffsll((unsigned long long)preMask) - 1 : 64;
+					// lanes 0..first (all 64 without a prefix) must be ready
+					uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+					if ((readyMask & need) == need)
+						break;
+				}
+				uint32_t mine = lane <= first ? (uint32_t)word : 0u;
+				for (int off = 32; off > 0; off >>= 1)
+					mine += __shfl_xor(mine, off);
+				prefix += mine;
+				if (first < 64)
+					break;
+				j -= 64;
+			}
+			if (lane == 0)
+				__hip_atomic_store(&status[tile], pack_status(epoch, kFlagPrefix, prefix + agg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		if (lane == 0)
+		{
+			tilePrefix = prefix;
+			if (total != nullptr && (tile + 1) * kScanTile >= n)
+				*total = prefix + agg;
 		}
 	}
-	uint32_t tot;
-	block_exclusive_sum(s, lds, &tot);
-	if (threadIdx.x == 0)
+	__syncthreads();
+	uint32_t run = tilePrefix + waveOff + (incl - sum);
+	if (vecOut && base + kScanItems <= n)
 	{
-		blockSums[blockIdx.x] = tot;
-	}
-}
-
-// Scans up to 256*1024 block sums in one workgroup (serially in chunks).
-__global__ void __launch_bounds__(256) k_scan_blocksums(uint32_t* sums, int64_t nb, uint32_t* total)
-{
-	__shared__ uint32_t lds[256];
-	uint32_t carry = 0;
-	for (int64_t base = 0; base < nb; base += 256)
-	{
-		int64_t i = base + threadIdx.x;
-		uint32_t v = i < nb ? sums[i] : 0u;
-		uint32_t tot;
-		uint32_t ex = block_exclusive_sum(v, lds, &tot);
-		if (i < nb)
+		uint32_t o[kScanItems];
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
 		{
-			sums[i] = ex + carry;
+			o[k] = run;
+			run += v[k];
 		}
-		carry += tot;
+		uint4* q = (uint4*)(out + base);
+		q[0] = make_uint4(o[0], o[1], o[2], o[3]);
+		q[1] = make_uint4(o[4], o[5], o[6], o[7]);
 	}
-	if (threadIdx.x == 0 && total != nullptr)
+	else
 	{
-		*total = carry;
-	}
-}
-
-__global__ void __launch_bounds__(256) k_scan_final(const uint32_t* in, uint32_t* out, int64_t n,
-													const uint32_t* blockOffsets)
-{
-	__shared__ uint32_t lds[256];
-	int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * 4;
-	uint32_t v[4];
-	uint32_t s = 0;
-	for (int k = 0; k < 4; ++k)
-	{
-		v[k] = base + k < n ? in[base + k] : 0u;
-		s += v[k];
-	}
-	uint32_t tot;
-	uint32_t ex = block_exclusive_sum(s, lds, &tot) + blockOffsets[blockIdx.x];
-	for (int k = 0; k < 4; ++k)
-	{
-		if (base + k < n)
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
 		{
-			out[base + k] = ex;
+			if (base + k < n)
+				out[base + k] = run;
+			run += v[k];
 		}
-		ex += v[k];
 	}
 }
 
-size_t scan_scratch_bytes(int64_t n)
+static uint32_t next_scan_epoch()
 {
-	int64_t nb = (n + kScanBlock - 1) / kScanBlock;
-	return (size_t)(nb + 1) * sizeof(uint32_t) + 256;
+	static std::atomic<uint32_t> epoch{ 0 };
+	return (epoch.fetch_add(1, std::memory_order_relaxed) + 1u) & 0x3FFFFFFFu;
 }
 
-void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* scratch, hipStream_t s)
+int64_t scan_status_words(int64_t n)
+{
+	return (n + kScanTile - 1) / kScanTile + 1;
+}
+
+void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint64_t* status, hipStream_t s)
 {
 	if (n <= 0)
 	{
@@ -116,11 +210,10 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 		}
 		return;
 	}
-	int64_t nb = (n + kScanBlock - 1) / kScanBlock;
-	uint32_t* sums = (uint32_t*)scratch;
-	k_scan_reduce<<<(unsigned)nb, 256, 0, s>>>(in, n, sums);
-	k_scan_blocksums<<<1, 256, 0, s>>>(sums, nb, total);
-	k_scan_final<<<(unsigned)nb, 256, 0, s>>>(in, out, n, sums);
+	int64_t tiles = (n + kScanTile - 1) / kScanTile;
+	// in and out may alias: each workgroup reads its whole tile before it
+	// writes it, and reads no other tile's input
+	k_scan_onepass<<<(unsigned)tiles, 256, 0, s>>>(in, out, n, total, status, next_scan_epoch());
 }
 
 // ------------------------------------------------------------ radix sort
@@ -196,11 +289,17 @@ size_t sort_scratch_bytes(int64_t n)
 {
 	int64_t nb = (n + kSortBlock - 1) / kSortBlock;
 	int64_t h = nb * kRadix;
-	return (size_t)h * sizeof(uint32_t) + scan_scratch_bytes(h) + 256;
+	return (size_t)h * sizeof(uint32_t);
+}
+
+int64_t sort_status_words(int64_t n)
+{
+	int64_t nb = (n + kSortBlock - 1) / kSortBlock;
+	return scan_status_words(nb * kRadix);
 }
 
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
-					  void* scratch, hipStream_t s)
+					  void* scratch, uint64_t* status, hipStream_t s)
 {
 	if (n <= 1 || bits <= 0)
 	{
@@ -209,7 +308,6 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 	int nb = (int)((n + kSortBlock - 1) / kSortBlock);
 	int64_t h = (int64_t)nb * kRadix;
 	uint32_t* hist = (uint32_t*)scratch;
-	void* scanScratch = (void*)((char*)scratch + ((h * sizeof(uint32_t) + 255) & ~(size_t)255));
 	int passes = (bits + kRadixBits - 1) / kRadixBits;
 	uint64_t* kin = keys;
 	uint32_t* vin = vals;
@@ -219,7 +317,7 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 	{
 		int shift = p * kRadixBits;
 		k_radix_hist<<<nb, 256, 0, s>>>(kin, n, shift, hist, nb);
-		scan_exclusive(hist, hist, h, nullptr, scanScratch, s);
+		scan_exclusive(hist, hist, h, nullptr, status, s);
 		k_radix_scatter<<<nb, 256, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
 		uint64_t* tk = kin;
 		kin = kout;

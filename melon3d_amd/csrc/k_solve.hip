@@ -364,7 +364,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 
 	int64_t m = std::max<int64_t>(n, C);
 	if (!w.i32a.ensure(n) || !w.u32a.ensure(m + 1) || !w.u32b.ensure(m + 1) || !w.islandOf.ensure(n) ||
-		!w.primScratch.ensure(std::max(scan_scratch_bytes(m + 1), sort_scratch_bytes(m + 1)) + 1024))
+		!ensure_prim_scratch(w, m + 1))
 		return false;
 	int32_t* parent = w.i32a.p;
 	k_any_awake_dyn<<<grids(n), kBlockS, 0, s>>>(w.bc.sb.p, n, w.counters.p);
@@ -374,7 +374,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	if (w.djointCount > 0)
 		k_uf_joints<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p, parent);
 	k_uf_roots<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, n, w.u32a.p);
-	scan_exclusive(w.u32a.p, w.u32b.p, n, w.counters.p + CT_ISLANDS, w.primScratch.p, s);
+	scan_exclusive(w.u32a.p, w.u32b.p, n, w.counters.p + CT_ISLANDS, w.scanStatus.p, s);
 	if (!sync_counters(w))
 		return false;
 	// no awake dynamic body: the reference skips island building entirely
@@ -409,7 +409,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	uint32_t* contactListVals = (uint32_t*)w.i32c.p;
 	k_body_island_keys<<<grids(n), kBlockS, 0, s>>>(w.islandOf.p, islandAwake, n, w.u64a.p, (uint32_t*)w.i32d.p,
 													 flag, islandBodies);
-	scan_exclusive(flag, offs, n, w.counters.p + CT_INTEGRATED, w.primScratch.p, s);
+	scan_exclusive(flag, offs, n, w.counters.p + CT_INTEGRATED, w.scanStatus.p, s);
 	k_compact_pairs<<<grids(n), kBlockS, 0, s>>>(w.u64a.p, (uint32_t*)w.i32d.p, flag, offs, n, w.u64b.p,
 												  bodyListVals);
 	launch_fill_i32(w.contactColor.p, -1, C, s);
@@ -418,7 +418,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		k_contact_island<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.cc.normal.p, w.bc.sb.p,
 													   w.islandOf.p, islandAwake, C, w.u64a.p, (uint32_t*)w.i32d.p,
 													   flag, islandContacts, w.counters.p);
-		scan_exclusive(flag, offs, C, w.counters.p + CT_SEG_COUNT, w.primScratch.p, s);
+		scan_exclusive(flag, offs, C, w.counters.p + CT_SEG_COUNT, w.scanStatus.p, s);
 		k_compact_pairs<<<grids(C), kBlockS, 0, s>>>(w.u64a.p, (uint32_t*)w.i32d.p, flag, offs, C, w.u64c.p,
 													  contactListVals);
 	}
@@ -439,11 +439,11 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	w.stats.solvedPoints = (int32_t)w.hostCounters[CT_SOLVED_POINTS];
 	w.stats.boxBoxPoints = (int32_t)w.hostCounters[CT_BB_POINTS];
 	w.stats.coloredIslands = (int32_t)w.hostCounters[CT_COLORED_ISLANDS];
-	radix_sort_pairs(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p, s);
+	radix_sort_pairs(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p, w.scanStatus.p, s);
 	if (NC > 0)
-		radix_sort_pairs(w.u64c.p, contactListVals, w.u64a.p, w.sortValsAlt.p, NC, bitsI, w.primScratch.p, s);
-	scan_exclusive(islandBodies, islandBodies, I + 1, nullptr, w.primScratch.p, s);
-	scan_exclusive(islandContacts, islandContacts, I + 1, nullptr, w.primScratch.p, s);
+		radix_sort_pairs(w.u64c.p, contactListVals, w.u64a.p, w.sortValsAlt.p, NC, bitsI, w.primScratch.p, w.scanStatus.p, s);
+	scan_exclusive(islandBodies, islandBodies, I + 1, nullptr, w.scanStatus.p, s);
+	scan_exclusive(islandContacts, islandContacts, I + 1, nullptr, w.scanStatus.p, s);
 
 	if (NB > 0)
 	{

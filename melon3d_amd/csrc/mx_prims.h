@@ -8,24 +8,22 @@
 namespace mx
 {
 
-// Scratch arena handed to the primitives (grown by the caller).
-struct PrimScratch
-{
-	void* buf = nullptr;
-	size_t bytes = 0;
-};
-
-// Bytes of scratch needed by scan_exclusive for n elements.
-size_t scan_scratch_bytes(int64_t n);
+// Look-back status words scan_exclusive needs for n elements. The status
+// storage belongs to the scans alone: zeroed once when allocated and never
+// written by anything else (its words carry a per-call epoch tag, so a
+// scan tells the words of earlier scans apart, but not arbitrary data).
+int64_t scan_status_words(int64_t n);
 // out[i] = sum(in[0..i)); *total (device pointer, may be null) = sum(in).
 // in and out may alias.
-void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, void* scratch, hipStream_t s);
+void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint64_t* status, hipStream_t s);
 
-// Bytes of scratch needed by radix_sort_pairs for n elements.
+// Bytes of histogram scratch and status words radix_sort_pairs needs for n
+// elements.
 size_t sort_scratch_bytes(int64_t n);
+int64_t sort_status_words(int64_t n);
 // Stable sort of (keys, vals) by the low `bits` bits of keys, ascending.
 // Results end in keys/vals; keysAlt/valsAlt are ping-pong buffers of n.
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
-					  void* scratch, hipStream_t s);
+					  void* scratch, uint64_t* status, hipStream_t s);
 
 } // namespace mx

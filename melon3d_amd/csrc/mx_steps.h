@@ -89,6 +89,8 @@ bool apply_moves(World& w, const int2* dMoves, int32_t count);
 void launch_material_refresh(World& w, const uint8_t* dFlags);
 // applies and clears the host's list of bodies to wake
 bool launch_wake_list(World& w);
+// starts copying (body a, body b, flags) of every contact to pinned memory
+bool prefetch_sidecar(World& w);
 void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const float4* stage, const uint32_t* stageFlags,
 						   const uint4* stageFilter, const uint8_t* stageBp);
 
@@ -103,6 +105,9 @@ void finish_step(World& w, float dt);
 bool solve_islands(World& w, const IslandLists& L, float dt, int substeps);
 
 bool sync_counters(World& w);
+// Sort histogram scratch and scan status words for scans and sorts of up
+// to n elements.
+bool ensure_prim_scratch(World& w, int64_t n);
 bool push_host_changes(World& w);
 
 } // namespace mx

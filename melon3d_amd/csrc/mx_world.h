@@ -48,10 +48,12 @@ struct DBuf
 			set_error("hipMalloc of %zu bytes failed", nc * sizeof(T));
 			return false;
 		}
+		// new memory starts zeroed (the scan's tagged status words rely on
+		// never meeting a stale tag); allocations are rare
+		(void)hipMemset(np, 0, nc * sizeof(T));
+		(void)hipDeviceSynchronize();
 		if (p != nullptr)
 		{
-			// queued kernels may still read the old buffer
-			(void)hipDeviceSynchronize();
 			if (keep > 0)
 			{
 				(void)hipMemcpy(np, p, keep * sizeof(T), hipMemcpyDeviceToDevice);
@@ -171,6 +173,15 @@ struct World
 	std::vector<int2> hcSlot;
 	// bodies to wake on the device at the next step (partners of destroyed
 	// bodies whose host record is not current)
+	// packed sidecar (body a, body b, flags) copied to pinned memory at the
+	// end of a step when the caller destroyed bodies in the last window
+	bool destroyedSinceStep = false;
+	bool sidecarPrefetched = false;
+	int32_t sidecarPrefetchCount = 0;
+	int4* pinnedSidecar = nullptr;
+	size_t pinnedSidecarCap = 0;
+	hipEvent_t sidecarReady = nullptr;
+	DBuf<int4> dSidecar;
 	std::vector<int32_t> deviceWake;
 	std::vector<int32_t> wakeUpload; // the list being uploaded (kept until the next push)
 	DBuf<int32_t> dWake;
@@ -200,7 +211,8 @@ struct World
 	DBuf<uint64_t> u64a, u64b, u64c;
 	DBuf<int32_t> i32a, i32b, i32c, i32d, i32e;
 	DBuf<uint8_t> u8a;
-	DBuf<char> primScratch;
+	DBuf<char> primScratch;       // radix sort histograms
+	DBuf<uint64_t> scanStatus;    // scan look-back words (nothing else writes them)
 	DBuf<uint32_t> pairRuns;      // pair enumeration: run tables, item counts and offsets
 	DBuf<uint64_t> hashKeys;
 	DBuf<uint32_t> hashVals;

@@ -151,7 +151,8 @@ void world_step(World& w, float dt, int substeps)
 	const uint32_t* hc = w.hostCounters;
 	if (w.timing)
 	{
-		// algorithmic bytes of the timed kernels this step:
+		// algorithmic bytes of the timed kernels this step (each field read or
+		// written once per pass):
 		// normals per contact and pass 160 B + 68 B per point; velocity per
 		// contact 472 B + 112 B per point; narrowphase 640 B per regenerated
 		// manifold, 120 B per cached contact
@@ -193,6 +194,11 @@ void world_step(World& w, float dt, int substeps)
 
 	w.hostStale = true;
 	w.sidecarValid = false;
+	// a caller that destroyed bodies before this step likely does again
+	// before the next: start the contact sidecar download now
+	if (w.destroyedSinceStep)
+		(void)prefetch_sidecar(w);
+	w.destroyedSinceStep = false;
 }
 
 } // namespace mx

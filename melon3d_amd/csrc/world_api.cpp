@@ -78,6 +78,7 @@ World::~World()
 	primScratch.release();
 	pairRuns.release();
 	hashKeys.release();
+	scanStatus.release();
 	hashVals.release();
 	islandOf.release();
 	islandAwakeD.release();
@@ -106,6 +107,11 @@ World::~World()
 		(void)hipHostFree(hostCounters);
 	for (hipEvent_t e : eventPool)
 		(void)hipEventDestroy(e);
+	if (sidecarReady != nullptr)
+		(void)hipEventDestroy(sidecarReady);
+	if (pinnedSidecar != nullptr)
+		(void)hipHostFree(pinnedSidecar);
+	dSidecar.release();
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
 }
@@ -417,13 +423,18 @@ bool ensure_sidecar(World& w)
 	w.hcOrigin.resize(C);
 	if (C > 0)
 	{
-		std::vector<float4> mat(C);
-		if (!hip_ok(hipMemcpy(w.hcBodies.data(), w.cc.bodies.p, C * sizeof(int2), hipMemcpyDeviceToHost), "sidecar") ||
-			!hip_ok(hipMemcpy(mat.data(), w.cc.mat.p, C * sizeof(float4), hipMemcpyDeviceToHost), "sidecar"))
+		if (!(w.sidecarPrefetched && w.sidecarPrefetchCount == C) && !prefetch_sidecar(w))
 			return false;
+		if (!hip_ok(hipEventSynchronize(w.sidecarReady), "sidecar"))
+			return false;
+		const int4* p = w.pinnedSidecar;
 		for (int32_t c = 0; c < C; ++c)
-			w.hcFlags[c] = f2u(mat[c].w);
+		{
+			w.hcBodies[c] = make_int2(p[c].x, p[c].y);
+			w.hcFlags[c] = (uint32_t)p[c].z;
+		}
 	}
+	w.sidecarPrefetched = false;
 	for (int32_t c = 0; c < C; ++c)
 		w.hcOrigin[c] = c;
 	// per-body index of contact positions (counting sort by body)
@@ -807,6 +818,7 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 	if (b == nullptr)
 		return;
 	int32_t index = id.index;
+	w.destroyedSinceStep = true;
 	if (ensure_sidecar(w))
 	{
 		// contacts of this body, removed in descending index order with
