@@ -16,7 +16,9 @@ when timing starts; m3d_world_step returns after the device finished the
 step, so the wall clock around the K steps is the step time.
 
 roofline: the dominant timed kernel (largest summed HIP-event time on the
-library's stream, over --timing-steps extra steps after the timed region):
+library's stream, over --timing-steps extra steps after the timed region);
+`traffic` is that kernel's measured HBM bytes per launch (rocprofv3 PMC, from
+the committed profile summary of this workload):
 algorithmic bytes per launch (from the step counters: normals pass 160 B
 per contact + 68 B per point, velocity pass 472 B per contact + 112 B per
 point, narrowphase 640 B per regenerated manifold + 120 B per cached contact)
@@ -57,6 +59,24 @@ def traffic_bytes(st, substeps: int) -> float:
     once = (88 * N + 24 * st.refattedBodies + 32 * st.activeEntries + 640 * st.regeneratedManifolds +
             120 * max(st.contacts - st.regeneratedManifolds, 0) + 16 * st.solvedContacts + 12 * N)
     return float(S * per_sub + once)
+
+
+def measured_traffic(scene: str, bodies: int, kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on this
+    workload, from the committed profile summary; None when not measured."""
+    if scene != "stream" or bodies != 200000:
+        return None
+    path = os.path.join(ROOT, "profiles", "stream200k_grid_normals.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def build_scene(lib, w, scene: str, bodies: int):
@@ -156,13 +176,13 @@ def main():
     snap = None if args.no_cpu else dev.export_state(w)
     cpu_stream = stream.clone_onto(dev, w) if (stream and snap is not None) else None  # churn state at t0
 
-    traffic = 0.0
+    model_bytes = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
         if stream:
             stream.churn_step()
         dev.m3d_world_step(w, DT, args.substeps)
-        traffic += traffic_bytes(dev.stats(w), args.substeps)
+        model_bytes += traffic_bytes(dev.stats(w), args.substeps)
     elapsed = time.perf_counter() - t0
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
@@ -184,6 +204,7 @@ def main():
                                         default=("none", (0, 0.0, 0.0)))
     avg_s = (kms / 1000.0) / launches if launches else 0.0
     achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
+    traffic = measured_traffic(args.scene, args.bodies, name)
 
     cpu = None
     if snap is not None:
@@ -210,11 +231,11 @@ def main():
                    "islands": prof.islandCount, "awake": prof.awakeBodyCount, "giant_sweeps": st.sweeps,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "launches": launches, "avg_launch_us": avg_s * 1e6,
                      "bytes_per_launch": (kbytes / launches) if launches else 0.0},
-        "step_traffic_model": {"bytes_per_step": traffic / args.steps,
-                               "achieved_GBps": traffic / elapsed / 1e9},
+        "step_traffic_model": {"bytes_per_step": model_bytes / args.steps,
+                               "achieved_GBps": model_bytes / elapsed / 1e9},
         "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
         "cpu_baseline": cpu,
     }
