@@ -94,6 +94,146 @@ def build_scene(lib, w, scene: str, bodies: int):
     raise SystemExit(f"unknown scene {scene}")
 
 
+def batch_cpu_baseline(worlds: int, warmup: int, steps_max: int, budget_s: float, substeps: int):
+    """Reference engine on a sample of the batched workload: worlds 0 ..
+    worlds-1, each its own world with workerCount = 1, spread over a pool of
+    host threads (one world per task), advanced `warmup` untimed steps, then
+    timed steps until the budget is spent."""
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libm3dref_fast.so")
+    if not os.path.exists(ref_path):
+        return None
+    from concurrent.futures import ThreadPoolExecutor
+
+    from melon3d_amd import api, scenes
+
+    ref = api.load_library(ref_path)
+    cores = host_cores()
+    wd = ref.m3d_world_def_default()
+    wd.workerCount = 1
+    ws, dyn = [], 0
+    for i in range(worlds):
+        w = ref.create_world(wd)
+        n, _ = scenes.batch_world(ref, w, i)
+        ws.append(w)
+        dyn += n
+
+    def run(w, k):
+        for _ in range(k):
+            ref.m3d_world_step(w, DT, substeps)
+
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda w: run(w, warmup), ws))
+        t0 = time.perf_counter()
+        k = 0
+        while k < steps_max:
+            list(ex.map(lambda w: run(w, 1), ws))
+            k += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+    for w in ws:
+        ref.m3d_world_destroy(w)
+    return {"value": dyn * k / dt, "unit": "body-steps/s", "cores": cores, "kind": "reference",
+            "sample": f"batched-workload worlds 0..{worlds - 1} ({dyn} dynamic bodies), {k} steps after {warmup} "
+                      f"untimed, one world per task on {cores} threads, workerCount=1, reference built "
+                      f"-O3 -mavx2 -mfma"}
+
+
+def batch_loop_plain(dev, worlds: int, warmup: int, steps: int, substeps: int):
+    """The same sample stepped as plain device worlds, one m3d_world_step per
+    world (the loop a batch replaces)."""
+    from melon3d_amd import scenes
+
+    ws, dyn = [], 0
+    for i in range(worlds):
+        w = dev.create_world()
+        n, _ = scenes.batch_world(dev, w, i)
+        ws.append(w)
+        dyn += n
+    for _ in range(warmup):
+        for w in ws:
+            dev.m3d_world_step(w, DT, substeps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for w in ws:
+            dev.m3d_world_step(w, DT, substeps)
+    dt = time.perf_counter() - t0
+    for w in ws:
+        dev.m3d_world_destroy(w)
+    return {"value": dyn * steps / dt, "unit": "body-steps/s", "sample": f"worlds 0..{worlds - 1} ({dyn} dynamic "
+                                                                           f"bodies), {steps} steps after {warmup}"}
+
+
+def run_batch(args, dev):
+    from melon3d_amd import scenes
+
+    t_build = time.perf_counter()
+    w, info = scenes.batch(dev, args.worlds)
+    dyn = sum(n for n, _ in info)
+    print(f"# batch of {args.worlds} worlds ({dyn} dynamic bodies) built in {time.perf_counter() - t_build:.1f} s",
+          file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        dev.m3d_world_step(w, DT, args.substeps)
+        if (i + 1) % 20 == 0:
+            print(f"# warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dev.m3d_world_step(w, DT, args.substeps)
+    elapsed = time.perf_counter() - t0
+    prof = dev.m3d_world_profile(w)
+    st = dev.stats(w)
+    value = dyn * args.steps / elapsed
+    dev.m3dx_set_kernel_timing(w, 1)
+    for i in range(args.timing_steps):
+        dev.m3d_world_step(w, DT, args.substeps)
+    kt = dev.kernel_timing(w)
+    dev.m3dx_set_kernel_timing(w, 0)
+    dev.m3d_world_destroy(w)
+    name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[2] > 0), key=lambda x: x[1][1],
+                                        default=("none", (0, 0.0, 0.0)))
+    avg_s = (kms / 1000.0) / launches if launches else 0.0
+    achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
+    loop = None
+    if not args.no_loop:
+        print("# looping plain worlds ...", file=sys.stderr, flush=True)
+        loop = batch_loop_plain(dev, args.loop_worlds, args.warmup, 2, args.substeps)
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = batch_cpu_baseline(args.cpu_worlds, args.warmup, args.cpu_steps, args.cpu_budget, args.substeps)
+    out = {
+        "metric": "body-steps/sec",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {args.worlds} worlds (world w: seed 1000003(w+1), "
+                               f"floor(exp U[ln 16, ln 2049)) bodies, pile or pyramid), dt 1/60 s, "
+                               f"{args.substeps} substeps",
+                   "worlds": args.worlds, "bodies": dyn, "mean_bodies": dyn / args.worlds,
+                   "contacts": prof.contactCount, "islands": prof.islandCount, "awake": prof.awakeBodyCount,
+                   "parallelism": "single GPU, one launch sequence for all worlds"},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": launches,
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": (kbytes / launches) if launches else 0.0},
+        "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
+        "loop_plain_gpu": loop,
+        "cpu_baseline": cpu,
+    }
+    if loop:
+        out["batched_over_loop"] = value / loop["value"]
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def host_cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -133,7 +273,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid"])
+    ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid", "batch"])
+    ap.add_argument("--worlds", type=int, default=4096, help="batch: number of worlds")
+    ap.add_argument("--loop-worlds", type=int, default=64, help="batch: worlds stepped one by one for comparison")
+    ap.add_argument("--cpu-worlds", type=int, default=256, help="batch: worlds in the CPU baseline sample")
+    ap.add_argument("--no-loop", action="store_true")
     ap.add_argument("--bodies", type=int, default=None)
     ap.add_argument("--substeps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=20)
@@ -157,12 +301,14 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(batched worlds would shard across GPUs by world range)"}))
         sys.exit(2)
-    if args.bodies is None:
-        args.bodies = {"stream": 200000, "rain": 20000, "pyramid": 210}[args.scene]
-
     from melon3d_amd import api
 
     dev = api.load()
+    if args.scene == "batch":
+        run_batch(args, dev)
+        return
+    if args.bodies is None:
+        args.bodies = {"stream": 200000, "rain": 20000, "pyramid": 210}[args.scene]
     w = dev.create_world()
     t_build = time.perf_counter()
     stream, dyn = build_scene(dev, w, args.scene, args.bodies)

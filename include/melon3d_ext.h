@@ -260,6 +260,34 @@ M3D_API int m3dx_set_kernel_timing(m3d_world* world, int enable);
 /* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
 M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
 
+/* ------------------------------------------------------ batched worlds */
+
+/* A batch: one device world holding `count` independent sub-worlds that share
+   the world definition (no counterpart in the reference, which steps one world
+   per call). m3d_world_step steps every sub-world; each steps exactly as a
+   world of its own with the same bodies would (same pairs, contact order,
+   islands and results). Bodies are created with m3d_body_create into the
+   selected sub-world, sub-worlds filled in ascending order; body ids and
+   every m3d_body_* getter and setter work as for a plain world. Not
+   supported on a batch: m3d_body_destroy (an error, no-op), joints between
+   sub-worlds (creation returns the null id), and per-sub-world contact
+   events (end events of removed contacts are not reported). A batch exports
+   its full state (contacts grouped by sub-world, see
+   m3dx_batch_contact_counts) for inspection; importing it gives a plain
+   world. Queries see the bodies of all sub-worlds and use the broadphase of
+   the last step.
+   count in [1, 262144]; NULL on error. */
+M3D_API m3d_world* m3dx_batch_create(const m3d_world_def* def, int32_t count);
+/* Sub-world that m3d_body_create adds to; never below the current one.
+   Returns 0, or -1 (see m3dx_last_error). */
+M3D_API int m3dx_batch_select(m3d_world* world, int32_t sub);
+/* Number of sub-worlds (0 for a plain world). */
+M3D_API int32_t m3dx_batch_size(m3d_world* world);
+/* Body index range [first, first + count) of a sub-world. Returns 0 or -1. */
+M3D_API int m3dx_batch_body_range(m3d_world* world, int32_t sub, int32_t* first, int32_t* count);
+/* Live contacts of each sub-world (up to cap); returns the sub-world count. */
+M3D_API int m3dx_batch_contact_counts(m3d_world* world, int32_t* out, int32_t cap);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

@@ -331,6 +331,11 @@ _SIGS = {
     "m3dx_world_stats": (None, [W, C.POINTER(m3dx_step_stats)]),
     "m3dx_set_kernel_timing": (C.c_int, [W, C.c_int]),
     "m3dx_kernel_timing": (C.c_int, [W, C.POINTER(m3dx_kernel_time)]),
+    "m3dx_batch_create": (W, [C.POINTER(m3d_world_def), C.c_int32]),
+    "m3dx_batch_select": (C.c_int, [W, C.c_int32]),
+    "m3dx_batch_size": (C.c_int32, [W]),
+    "m3dx_batch_body_range": (C.c_int, [W, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "m3dx_batch_contact_counts": (C.c_int, [W, C.POINTER(C.c_int32), C.c_int32]),
     "m3dx_last_error": (C.c_char_p, []),
     "m3dx_set_device": (C.c_int, []),
 }

@@ -20,6 +20,7 @@
 #include "mx_steps.h"
 
 #include <algorithm>
+#include <string.h>
 
 namespace mx
 {
@@ -88,32 +89,36 @@ static DBodies body_view(World& w)
 	B.fatHi = w.bc.fatHi.p;
 	B.bp = w.bc.bp.p;
 	B.sb = w.bc.sb.p;
+	B.sub = w.subCount > 0 ? w.bodySub.p : nullptr;
+	B.subStable = w.subCount > 0 ? w.subStable.p : nullptr;
 	return B;
 }
 
-static DContacts contact_view(World& w)
+DContacts columns_view(ContactColumns& cc, int32_t cap)
 {
 	DContacts C;
-	C.cap = w.contactCap;
-	C.bodies = w.cc.bodies.p;
-	C.mat = w.cc.mat.p;
-	C.normal = w.cc.normal.p;
-	C.cachePA = w.cc.cachePA.p;
-	C.cachePB = w.cc.cachePB.p;
-	C.cacheQA = w.cc.cacheQA.p;
-	C.cacheQB = w.cc.cacheQB.p;
-	C.pA = w.cc.pA.p;
-	C.pB = w.cc.pB.p;
-	C.pJA = w.cc.pJA.p;
-	C.pJB = w.cc.pJB.p;
-	C.pRA = w.cc.pRA.p;
-	C.pRB = w.cc.pRB.p;
-	C.pFA = w.cc.pFA.p;
-	C.pFB = w.cc.pFB.p;
-	C.pSlip = w.cc.pSlip.p;
-	C.hint = w.cc.hint.p;
+	C.cap = cap;
+	C.bodies = cc.bodies.p;
+	C.mat = cc.mat.p;
+	C.normal = cc.normal.p;
+	C.cachePA = cc.cachePA.p;
+	C.cachePB = cc.cachePB.p;
+	C.cacheQA = cc.cacheQA.p;
+	C.cacheQB = cc.cacheQB.p;
+	C.pA = cc.pA.p;
+	C.pB = cc.pB.p;
+	C.pJA = cc.pJA.p;
+	C.pJB = cc.pJB.p;
+	C.pRA = cc.pRA.p;
+	C.pRB = cc.pRB.p;
+	C.pFA = cc.pFA.p;
+	C.pFB = cc.pFB.p;
+	C.pSlip = cc.pSlip.p;
+	C.hint = cc.hint.p;
 	return C;
 }
+
+static DContacts contact_view(World& w) { return columns_view(w.cc, w.contactCap); }
 
 DBodies make_body_view(World& w) { return body_view(w); }
 DContacts make_contact_view(World& w) { return contact_view(w); }
@@ -144,6 +149,8 @@ __global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uin
 		bp |= BP_ACTIVE;
 	if (stable)
 		bp |= BP_STABLE;
+	if (B.sub != nullptr)
+		rebuildStable = (B.subStable[B.sub[i]] & SUB_REBUILD) != 0;
 	uint8_t sb = (uint8_t)((inUse ? SB_IN_USE : 0) | ((inUse && type == M3D_BODY_DYNAMIC) ? SB_DYNAMIC : 0) |
 						   ((inUse && type == M3D_BODY_KINEMATIC) ? SB_KINEMATIC : 0) | (awake ? SB_AWAKE : 0));
 	if (active)
@@ -202,20 +209,24 @@ void launch_refresh(World& w, float dt, bool rebuildStable)
 	}
 }
 
-__global__ void k_stable_migrate(uint8_t* bp, int32_t n)
+// inStableTier := isStable (src/world.cpp:215-218), for the sub-worlds that
+// rebuild their stable tier when worlds are batched
+__global__ void k_stable_migrate(DBodies B, int32_t n)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (i >= n)
 		return;
-	uint8_t b = bp[i];
-	bp[i] = (b & BP_STABLE) ? (uint8_t)(b | BP_IN_STABLE_TIER) : (uint8_t)(b & ~BP_IN_STABLE_TIER);
+	if (B.sub != nullptr && !(B.subStable[B.sub[i]] & SUB_REBUILD))
+		return;
+	uint8_t b = B.bp[i];
+	B.bp[i] = (b & BP_STABLE) ? (uint8_t)(b | BP_IN_STABLE_TIER) : (uint8_t)(b & ~BP_IN_STABLE_TIER);
 }
 
 void launch_stable_migrate(World& w)
 {
 	int32_t n = w.deviceBodyCount;
 	if (n > 0)
-		k_stable_migrate<<<grid_for(n), kBlock, 0, w.stream>>>(w.bc.bp.p, n);
+		k_stable_migrate<<<grid_for(n), kBlock, 0, w.stream>>>(body_view(w), n);
 }
 
 // -------------------------------------------------------------- tiers
@@ -237,13 +248,65 @@ MX_HD uint64_t cell_key_of(m3d_vec3 p, float cellSize)
 	return pack_cell(cell_coord(p.x, inv), cell_coord(p.y, inv), cell_coord(p.z, inv));
 }
 
-__global__ void k_tier_count(DBodies B, int32_t n, uint8_t includeBit, float cellSize, uint32_t* cnt, uint32_t* ov)
+// Batched worlds: the sub-world index above three 15-bit cell fields, so
+// cells of different sub-worlds never share a key and each sub-world's cells
+// keep the reference's x, y, z order (exact within 2^14 cells of the origin,
+// where the reference's 21-bit fields are exact too).
+MX_HD uint64_t pack_cell_sub(uint32_t s, int64_t x, int64_t y, int64_t z)
+{
+	const int64_t off = (int64_t)1 << 14;
+	uint64_t ux = (uint64_t)(x + off) & 0x7FFFu;
+	uint64_t uy = (uint64_t)(y + off) & 0x7FFFu;
+	uint64_t uz = (uint64_t)(z + off) & 0x7FFFu;
+	return ((uint64_t)s << 45) | (ux << 30) | (uy << 15) | uz;
+}
+
+__device__ __forceinline__ uint64_t body_cell_key(const DBodies& B, int32_t body, int64_t x, int64_t y, int64_t z)
+{
+	return B.sub != nullptr ? pack_cell_sub((uint32_t)B.sub[body], x, y, z) : pack_cell(x, y, z);
+}
+
+// cell fields of a stored key: sub-world, x, y, z
+struct KeyFields
+{
+	uint32_t s, x, y, z;
+};
+
+MX_HD KeyFields key_fields(uint64_t k, int batched)
+{
+	KeyFields f;
+	if (batched)
+	{
+		f.s = (uint32_t)(k >> 45);
+		f.x = (uint32_t)(k >> 30) & 0x7FFFu;
+		f.y = (uint32_t)(k >> 15) & 0x7FFFu;
+		f.z = (uint32_t)k & 0x7FFFu;
+	}
+	else
+	{
+		f.s = 0;
+		f.x = (uint32_t)(k >> 42) & 0x1FFFFFu;
+		f.y = (uint32_t)(k >> 21) & 0x1FFFFFu;
+		f.z = (uint32_t)k & 0x1FFFFFu;
+	}
+	return f;
+}
+
+// body included in a tier build: its tier bit, and (batched stable builds)
+// its sub-world rebuilds this step
+__device__ __forceinline__ bool tier_included(const DBodies& B, int32_t i, uint8_t includeBit, int gated)
+{
+	return (B.bp[i] & includeBit) && (!gated || (B.subStable[B.sub[i]] & SUB_REBUILD));
+}
+
+__global__ void k_tier_count(DBodies B, int32_t n, uint8_t includeBit, int gated, float cellSize, uint32_t* cnt,
+							 uint32_t* ov)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (i >= n)
 		return;
 	uint32_t c = 0, o = 0;
-	if (B.bp[i] & includeBit)
+	if (tier_included(B, i, includeBit, gated))
 	{
 		float inv = 1.0f / cellSize;
 		float4 lo = B.fatLo[i], hi = B.fatHi[i];
@@ -289,13 +352,14 @@ __device__ __forceinline__ void block_minmax(uint32_t v, bool valid, uint32_t* g
 	__syncthreads();
 }
 
-__global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, float cellSize, const uint32_t* cnt,
+__global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated, float cellSize, const uint32_t* cnt,
 							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, uint32_t* counters)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
 	bool any = false;
-	if (i < n && cnt[i] > 0 && (B.bp[i] & includeBit))
+	const int batched = B.sub != nullptr;
+	if (i < n && cnt[i] > 0 && tier_included(B, i, includeBit, gated))
 	{
 		float inv = 1.0f / cellSize;
 		float4 lo = B.fatLo[i], hi = B.fatHi[i];
@@ -306,12 +370,12 @@ __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, float cell
 			for (int64_t y = y0; y <= y1; ++y)
 				for (int64_t z = z0; z <= z1; ++z)
 				{
-					uint64_t k = pack_cell(x, y, z);
+					uint64_t k = body_cell_key(B, i, x, y, z);
 					keys[o] = k;
 					bodies[o] = (uint32_t)i;
 					++o;
-					uint32_t ux = (uint32_t)(k >> 42) & 0x1FFFFFu, uy = (uint32_t)(k >> 21) & 0x1FFFFFu,
-							 uz = (uint32_t)k & 0x1FFFFFu;
+					KeyFields f = key_fields(k, batched);
+					uint32_t ux = f.x, uy = f.y, uz = f.z;
 					mnx = min(mnx, ux);
 					mxx = max(mxx, ux);
 					mny = min(mny, uy);
@@ -329,10 +393,14 @@ __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, float cell
 	block_minmax(mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
 }
 
+// Sort keys: the occupied box of cell fields (and the sub-world, most
+// significant) mapped to a dense integer range, so the radix sort only
+// passes over the bits that vary; order is that of the stored keys.
 struct KeyCompress
 {
 	uint32_t ux0, uy0, uz0;
-	uint64_t ny, nz;
+	uint64_t nx, ny, nz;
+	int batched;
 };
 
 __global__ void k_key_compress(uint64_t* keys, int64_t n, KeyCompress kc)
@@ -340,9 +408,8 @@ __global__ void k_key_compress(uint64_t* keys, int64_t n, KeyCompress kc)
 	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	uint64_t k = keys[i];
-	uint64_t ux = (k >> 42) & 0x1FFFFFu, uy = (k >> 21) & 0x1FFFFFu, uz = k & 0x1FFFFFu;
-	keys[i] = ((ux - kc.ux0) * kc.ny + (uy - kc.uy0)) * kc.nz + (uz - kc.uz0);
+	KeyFields f = key_fields(keys[i], kc.batched);
+	keys[i] = (((uint64_t)f.s * kc.nx + (f.x - kc.ux0)) * kc.ny + (f.y - kc.uy0)) * kc.nz + (f.z - kc.uz0);
 }
 
 __global__ void k_key_expand(uint64_t* keys, int64_t n, KeyCompress kc)
@@ -354,18 +421,54 @@ __global__ void k_key_expand(uint64_t* keys, int64_t n, KeyCompress kc)
 	uint64_t cz = c % kc.nz;
 	uint64_t r = c / kc.nz;
 	uint64_t cy = r % kc.ny;
-	uint64_t cx = r / kc.ny;
-	keys[i] = ((cx + kc.ux0) << 42) | ((cy + kc.uy0) << 21) | (cz + kc.uz0);
+	r /= kc.ny;
+	uint64_t cx = r % kc.nx;
+	uint64_t cs = r / kc.nx;
+	keys[i] = kc.batched ? ((cs << 45) | ((cx + kc.ux0) << 30) | ((cy + kc.uy0) << 15) | (cz + kc.uz0))
+						 : (((cx + kc.ux0) << 42) | ((cy + kc.uy0) << 21) | (cz + kc.uz0));
 }
 
-__global__ void k_mark_cells(DBodies B, int32_t n, uint8_t includeBit, uint8_t cellBit, const uint32_t* cnt)
+// cell field bounds of existing stored keys (kept stable entries)
+__global__ void k_key_bounds(const uint64_t* keys, int64_t n, int batched, uint32_t* counters)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	bool any = i < n;
+	KeyFields f = key_fields(any ? keys[i] : 0ull, batched);
+	block_minmax(f.x, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
+	block_minmax(f.y, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
+	block_minmax(f.z, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+}
+
+__global__ void k_mark_cells(DBodies B, int32_t n, uint8_t includeBit, int gated, uint8_t cellBit, const uint32_t* cnt)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (i >= n)
 		return;
+	if (gated && !(B.subStable[B.sub[i]] & SUB_REBUILD))
+		return; // this sub-world keeps its stable tier
 	uint8_t bp = B.bp[i];
 	bool in = (bp & includeBit) && cnt[i] > 0;
 	B.bp[i] = in ? (uint8_t)(bp | cellBit) : (uint8_t)(bp & ~cellBit);
+}
+
+// compression of keys whose field bounds sit in the counters block
+static KeyCompress key_compress_from(const uint32_t* hc, int32_t subCount, int* bitsOut)
+{
+	KeyCompress kc;
+	kc.batched = subCount > 0 ? 1 : 0;
+	kc.ux0 = hc[CT_UX_MIN];
+	kc.uy0 = hc[CT_UY_MIN];
+	kc.uz0 = hc[CT_UZ_MIN];
+	kc.nx = (uint64_t)(hc[CT_UX_MAX] - kc.ux0) + 1;
+	kc.ny = (uint64_t)(hc[CT_UY_MAX] - kc.uy0) + 1;
+	kc.nz = (uint64_t)(hc[CT_UZ_MAX] - kc.uz0) + 1;
+	uint64_t ns = subCount > 0 ? (uint64_t)subCount : 1;
+	uint64_t range = ns * kc.nx * kc.ny * kc.nz; // <= 2^63 (plain) or 2^18 * 2^45 (batched)
+	int bits = 0;
+	while (bits < 64 && (range - 1) >> bits)
+		++bits;
+	*bitsOut = bits;
+	return kc;
 }
 
 bool sync_counters(World& w)
@@ -406,7 +509,7 @@ bool build_tier(World& w, int tier)
 	uint32_t* cnt = w.u32a.p;
 	uint32_t* offs = w.u32b.p;
 	uint32_t* ov = w.u32c.p;
-	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, w.def.gridCellSize, cnt, ov);
+	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, ov);
 	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
 	// oversized list in body order
 	uint32_t* ovOff = w.u32d.ensure(n) ? w.u32d.p : nullptr;
@@ -426,7 +529,7 @@ bool build_tier(World& w, int tier)
 		hipMemcpyAsync(dOver.p, w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
 		hipMemcpyAsync(over.data(), w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToHost, s);
 	}
-	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, cellBit, cnt);
+	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, cellBit, cnt);
 	count = E;
 	if (E == 0)
 	{
@@ -437,21 +540,12 @@ bool build_tier(World& w, int tier)
 		return false;
 	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
 	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
-	k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, w.def.gridCellSize, cnt, offs, keys.p, vals.p,
+	k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p, vals.p,
 											   w.counters.p);
 	if (!sync_counters(w))
 		return false;
-	KeyCompress kc;
-	kc.ux0 = w.hostCounters[CT_UX_MIN];
-	kc.uy0 = w.hostCounters[CT_UY_MIN];
-	kc.uz0 = w.hostCounters[CT_UZ_MIN];
-	uint64_t nx = (uint64_t)(w.hostCounters[CT_UX_MAX] - kc.ux0) + 1;
-	kc.ny = (uint64_t)(w.hostCounters[CT_UY_MAX] - kc.uy0) + 1;
-	kc.nz = (uint64_t)(w.hostCounters[CT_UZ_MAX] - kc.uz0) + 1;
-	uint64_t range = nx * kc.ny * kc.nz; // <= 2^63
 	int bits = 0;
-	while (bits < 64 && (range - 1) >> bits)
-		++bits;
+	KeyCompress kc = key_compress_from(w.hostCounters, w.subCount, &bits);
 	k_key_compress<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
 	radix_sort_pairs(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p, w.scanStatus.p, s);
 	k_key_expand<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
@@ -590,7 +684,8 @@ __device__ __forceinline__ bool dedupe_ok(const PairCtx& P, int32_t a, int32_t b
 {
 	float4 la = P.B.fatLo[a], lb = P.B.fatLo[b];
 	m3d_vec3 lo2 = m3d_max(f4_v3(la), f4_v3(lb));
-	return cell_key_of(lo2, P.cellSize) == runKey;
+	float inv = 1.0f / P.cellSize;
+	return body_cell_key(P.B, a, cell_coord(lo2.x, inv), cell_coord(lo2.y, inv), cell_coord(lo2.z, inv)) == runKey;
 }
 
 // Candidate pairs in the reference's order (src/world.cpp:1600-1637): cell
@@ -720,16 +815,14 @@ struct OverSeg
 	int32_t a;
 	int32_t kind;  // 0: all bodies in either tier's cells, 1: list, 2: bodies in active cells
 	int32_t list;  // 0 active oversized list, 1 stable oversized list
-	int32_t first; // first list index (kind 1)
+	int32_t first; // first list index (kind 1) or first body (kinds 0, 2)
 	int64_t start; // first global candidate index
 	int64_t len;
 };
 
-constexpr int kMaxSegs = 64;
-
 struct OverCtx
 {
-	OverSeg seg[kMaxSegs];
+	const OverSeg* seg; // ascending start
 	int32_t nseg;
 	const int32_t* actOver;
 	const int32_t* stabOver;
@@ -737,10 +830,16 @@ struct OverCtx
 
 __device__ bool over_candidate(const PairCtx& P, const OverCtx& O, int64_t g, uint64_t* keyOut)
 {
-	int s = 0;
-	while (s + 1 < O.nseg && O.seg[s + 1].start <= g)
-		++s;
-	const OverSeg& S = O.seg[s];
+	int32_t lo = 0, hi = O.nseg - 1; // last segment starting at or before g
+	while (lo < hi)
+	{
+		int32_t m = (lo + hi + 1) >> 1;
+		if (O.seg[m].start <= g)
+			lo = m;
+		else
+			hi = m - 1;
+	}
+	const OverSeg S = O.seg[lo];
 	int64_t k = g - S.start;
 	int32_t a = S.a, b;
 	if (S.kind == 1)
@@ -751,7 +850,7 @@ __device__ bool over_candidate(const PairCtx& P, const OverCtx& O, int64_t g, ui
 	}
 	else
 	{
-		b = (int32_t)k;
+		b = S.first + (int32_t)k;
 		uint8_t bp = P.B.bp[b];
 		if (S.kind == 0)
 		{
@@ -818,14 +917,15 @@ __global__ void k_compact_u64(const uint64_t* in, const uint32_t* flags, const u
 
 // Contact initialization; src/world.cpp:1540-1568 (value-initialized
 // manifold, combined materials, curved flag).
-__global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t count, int32_t first)
+__global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t count, int32_t first,
+						 const int32_t* dst)
 {
 	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (k >= count)
 		return;
 	uint64_t key = keys[k];
 	int32_t lo = (int32_t)(key >> 32), hi = (int32_t)(uint32_t)key;
-	int32_t c = first + k;
+	int32_t c = dst != nullptr ? dst[k] : first + k;
 	float4 ml = B.mat[lo], mh = B.mat[hi];
 	uint32_t fl = B.flags[lo], fh = B.flags[hi];
 	float friction = sqrtf(ml.y * mh.y);
@@ -862,6 +962,7 @@ __global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t c
 
 bool enumerate_and_create(World& w)
 {
+	w.newCount = 0;
 	hipStream_t s = w.stream;
 	int32_t n = w.deviceBodyCount;
 	int32_t C0 = w.contactCount;
@@ -930,45 +1031,82 @@ bool enumerate_and_create(World& w)
 		gridTotal = w.hostCounters[CT_CANDIDATES];
 	}
 
-	// coarse-list segments in the reference's serial order
+	// Coarse-list segments in the reference's serial order
+	// (src/world.cpp:1643-1702). Batched: each oversized body meets only its
+	// own sub-world's bodies and oversized lists, which keeps every
+	// sub-world's candidate order that of a world of its own.
 	OverCtx O;
-	O.nseg = 0;
 	O.actOver = w.dActiveOversized.p;
 	O.stabOver = w.dStableOversized.p;
 	int64_t ovTotal = 0;
+	std::vector<OverSeg> segs;
 	auto addSeg = [&](int32_t a, int kind, int list, int first, int64_t len) {
 		if (len <= 0)
-			return true;
-		if (O.nseg >= kMaxSegs)
-		{
-			set_error("too many oversized bodies (%d segments)", O.nseg);
-			return false;
-		}
-		OverSeg& S = O.seg[O.nseg++];
+			return;
+		OverSeg S;
 		S.a = a;
 		S.kind = kind;
 		S.list = list;
 		S.first = first;
 		S.start = ovTotal;
 		S.len = len;
+		segs.push_back(S);
 		ovTotal += len;
-		return true;
 	};
 	const std::vector<int32_t>& ao = w.activeOversized;
 	const std::vector<int32_t>& so = w.stableOversized;
-	bool segOk = true;
-	for (size_t oi = 0; oi < ao.size(); ++oi)
+	if (w.subCount == 0)
 	{
-		segOk &= addSeg(ao[oi], 0, 0, 0, n);
-		segOk &= addSeg(ao[oi], 1, 0, (int)oi + 1, (int64_t)ao.size() - (int64_t)oi - 1);
-		segOk &= addSeg(ao[oi], 1, 1, 0, (int64_t)so.size());
+		for (size_t oi = 0; oi < ao.size(); ++oi)
+		{
+			addSeg(ao[oi], 0, 0, 0, n);
+			addSeg(ao[oi], 1, 0, (int)oi + 1, (int64_t)ao.size() - (int64_t)oi - 1);
+			addSeg(ao[oi], 1, 1, 0, (int64_t)so.size());
+		}
+		for (size_t si = 0; si < so.size(); ++si)
+			addSeg(so[si], 2, 0, 0, n);
 	}
-	for (size_t si = 0; si < so.size(); ++si)
+	else
 	{
-		segOk &= addSeg(so[si], 2, 0, 0, n);
+		// both lists ascend by body, so a sub-world's entries are one range
+		auto subRange = [&](const std::vector<int32_t>& l, int32_t sw, size_t* b, size_t* e) {
+			int32_t lo = w.subBodyStart[sw], hi = w.subBodyStart[sw + 1];
+			*b = (size_t)(std::lower_bound(l.begin(), l.end(), lo) - l.begin());
+			*e = (size_t)(std::lower_bound(l.begin(), l.end(), hi) - l.begin());
+		};
+		for (size_t oi = 0; oi < ao.size(); ++oi)
+		{
+			int32_t sw = w.bodySubHost[ao[oi]];
+			int32_t b0 = w.subBodyStart[sw], nb = w.subBodyStart[sw + 1] - b0;
+			size_t ab, ae, sb, se;
+			subRange(ao, sw, &ab, &ae);
+			subRange(so, sw, &sb, &se);
+			addSeg(ao[oi], 0, 0, b0, nb);
+			addSeg(ao[oi], 1, 0, (int)oi + 1, (int64_t)ae - (int64_t)oi - 1);
+			addSeg(ao[oi], 1, 1, (int)sb, (int64_t)se - (int64_t)sb);
+		}
+		for (size_t si = 0; si < so.size(); ++si)
+		{
+			int32_t sw = w.bodySubHost[so[si]];
+			int32_t b0 = w.subBodyStart[sw], nb = w.subBodyStart[sw + 1] - b0;
+			addSeg(so[si], 2, 0, b0, nb);
+		}
 	}
-	if (!segOk)
-		return false;
+	O.nseg = (int32_t)segs.size();
+	if (!segs.empty())
+	{
+		size_t words = (segs.size() * sizeof(OverSeg) + 7) / 8;
+		if (!w.overSegs.ensure(words))
+			return false;
+		w.overSegHost.resize(words);
+		memcpy(w.overSegHost.data(), segs.data(), segs.size() * sizeof(OverSeg));
+		hipMemcpyAsync(w.overSegs.p, w.overSegHost.data(), words * 8, hipMemcpyHostToDevice, s);
+		O.seg = (const OverSeg*)w.overSegs.p;
+	}
+	else
+	{
+		O.seg = nullptr;
+	}
 	int64_t total = gridTotal;
 	uint32_t* ovFlags = nullptr;
 	uint32_t* ovOffs = nullptr;
@@ -1017,11 +1155,13 @@ bool enumerate_and_create(World& w)
 	if (!sync_counters(w))
 		return false;
 	int32_t K = (int32_t)w.hostCounters[CT_NEW_CONTACTS];
+	if (w.subCount > 0)
+		return stage_new_contacts(w, w.u64b.p, K); // placed with the removals
 	if (K == 0)
 		return true;
 	if (!ensure_device_contacts(w, C0 + K))
 		return false;
-	k_create<<<grid_for(K), kBlock, 0, s>>>(body_view(w), contact_view(w), w.u64b.p, K, C0);
+	k_create<<<grid_for(K), kBlock, 0, s>>>(body_view(w), contact_view(w), w.u64b.p, K, C0, nullptr);
 	w.contactCount = C0 + K;
 	return true;
 }
@@ -1319,6 +1459,475 @@ void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const f
 	if (count > 0)
 		k_upload_scatter<<<grid_for(count), kBlock, 0, w.stream>>>(body_view(w), count, dIdx, stage, stageFlags,
 																	stageFilter, stageBp);
+}
+
+// ------------------------------------------------------------ batched sub-worlds
+//
+// A batch is one device world whose bodies are partitioned into sub-worlds
+// (index ranges). Cell keys carry the sub-world, coarse lists stay inside it
+// and contacts are kept contiguous per sub-world, each range in the order a
+// world of its own would hold; every later stage works per contact, body or
+// island, none of which ever spans two sub-worlds.
+
+static int bits_for(uint64_t range)
+{
+	int b = 0;
+	while (b < 64 && (range - 1) >> b)
+		++b;
+	return b;
+}
+
+__global__ void k_sub_or_dirty(uint32_t* subStable, const uint8_t* flags, int32_t S)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t < S && flags[t])
+		subStable[t] |= SUB_DIRTY;
+}
+
+bool upload_batch_state(World& w)
+{
+	if (w.subCount == 0)
+		return true;
+	hipStream_t s = w.stream;
+	int32_t n = (int32_t)w.bodySubHost.size();
+	if (w.bodySubDirty && n > 0)
+	{
+		if (!w.bodySub.ensure(n))
+			return false;
+		if (!hip_ok(hipMemcpyAsync(w.bodySub.p, w.bodySubHost.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s),
+					"sub-world upload"))
+			return false;
+		w.bodySubDirty = false;
+	}
+	if (w.anySubDirtyHost)
+	{
+		int32_t S = w.subCount;
+		if (!hip_ok(hipMemcpyAsync(w.subDirtyUpload.p, w.subDirtyHost.data(), S, hipMemcpyHostToDevice, s),
+					"sub-world dirt upload"))
+			return false;
+		k_sub_or_dirty<<<grid_for(S), kBlock, 0, s>>>(w.subStable.p, w.subDirtyUpload.p, S);
+		if (!hip_ok(hipStreamSynchronize(s), "sub-world dirt upload"))
+			return false;
+		std::fill(w.subDirtyHost.begin(), w.subDirtyHost.end(), 0);
+		w.anySubDirtyHost = false;
+	}
+	return true;
+}
+
+// rebuild = force || (dirty && steps since the last build >= 4); a rebuild
+// clears dirty and force and restarts the count, else the count advances
+__global__ void k_stable_plan(uint32_t* subStable, int32_t S, uint32_t* counters)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= S)
+		return;
+	uint32_t u = subStable[t];
+	uint32_t since = u >> 8;
+	bool rebuild = (u & SUB_FORCE) || ((u & SUB_DIRTY) && since >= 4u);
+	if (rebuild)
+	{
+		subStable[t] = SUB_REBUILD;
+		atomicAdd(&counters[CT_SUB_REBUILDS], 1u);
+	}
+	else
+	{
+		since = since < (1u << 20) ? since + 1u : since;
+		subStable[t] = (u & (SUB_DIRTY | SUB_FORCE)) | (since << 8);
+	}
+}
+
+void launch_stable_plan(World& w)
+{
+	if (w.subCount > 0)
+		k_stable_plan<<<grid_for(w.subCount), kBlock, 0, w.stream>>>(w.subStable.p, w.subCount, w.counters.p);
+}
+
+__global__ void k_keep_flags(const uint32_t* bodies, int64_t n, const int32_t* bodySub, const uint32_t* subStable,
+							 uint32_t* keep)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		keep[i] = (subStable[bodySub[bodies[i]]] & SUB_REBUILD) ? 0u : 1u;
+}
+
+__global__ void k_compact_entries(const uint64_t* keys, const uint32_t* bodies, const uint32_t* keep,
+								  const uint32_t* offs, int64_t n, uint64_t* keysOut, uint32_t* bodiesOut)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && keep[i])
+	{
+		keysOut[offs[i]] = keys[i];
+		bodiesOut[offs[i]] = bodies[i];
+	}
+}
+
+bool build_stable_batched(World& w)
+{
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	int32_t S = w.subCount;
+	DBodies B = body_view(w);
+	int64_t oldE = w.stableCount;
+	if (n == 0)
+		return true;
+	if (!w.u32a.ensure(n) || !w.u32b.ensure(n) || !w.u32c.ensure(n) || !w.u32d.ensure(n) || !w.i32a.ensure(n) ||
+		!w.i32d.ensure(std::max<int64_t>(oldE, 1)) || !w.i32e.ensure(std::max<int64_t>(oldE, 1)) ||
+		!ensure_prim_scratch(w, std::max<int64_t>(n, oldE)))
+		return false;
+	uint32_t* cnt = w.u32a.p;
+	uint32_t* offs = w.u32b.p;
+	uint32_t* ov = w.u32c.p;
+	uint32_t* ovOff = w.u32d.p;
+	uint32_t* keep = (uint32_t*)w.i32d.p;
+	uint32_t* keepOff = (uint32_t*)w.i32e.p;
+	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, ov);
+	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
+	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
+	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
+	if (oldE > 0)
+	{
+		k_keep_flags<<<grid_for(oldE), kBlock, 0, s>>>(w.stableBodies.p, oldE, w.bodySub.p, w.subStable.p, keep);
+		scan_exclusive(keep, keepOff, oldE, w.counters.p + CT_KEPT, w.scanStatus.p, s);
+	}
+	else
+	{
+		hipMemsetAsync(w.counters.p + CT_KEPT, 0, sizeof(uint32_t), s);
+	}
+	w.subStableHost.resize(S);
+	hipMemcpyAsync(w.subStableHost.data(), w.subStable.p, S * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+	if (!sync_counters(w))
+		return false;
+	int64_t eNew = w.hostCounters[CT_ENTRIES];
+	int32_t nov = (int32_t)w.hostCounters[CT_OVERSIZED];
+	int64_t eKeep = w.hostCounters[CT_KEPT];
+	int64_t E = eKeep + eNew;
+
+	// oversized list: kept entries of the other sub-worlds, new ones of the
+	// rebuilding sub-worlds, ascending by body
+	std::vector<int32_t> merged;
+	merged.reserve(w.stableOversized.size() + nov);
+	for (int32_t b : w.stableOversized)
+		if (!(w.subStableHost[w.bodySubHost[b]] & SUB_REBUILD))
+			merged.push_back(b);
+	if (nov > 0)
+	{
+		size_t k0 = merged.size();
+		merged.resize(k0 + nov);
+		if (!hip_ok(hipMemcpy(merged.data() + k0, w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToHost),
+					"stable oversized"))
+			return false;
+	}
+	std::sort(merged.begin(), merged.end());
+	w.stableOversized = merged;
+	if (!merged.empty())
+	{
+		if (!w.dStableOversized.ensure(merged.size()))
+			return false;
+		hipMemcpyAsync(w.dStableOversized.p, w.stableOversized.data(), merged.size() * sizeof(int32_t),
+					   hipMemcpyHostToDevice, s);
+	}
+	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, BP_STABLE, 1, BP_IN_STABLE_CELLS, cnt);
+	if (E == 0)
+	{
+		w.stableCount = 0;
+		return hip_ok(hipStreamSynchronize(s), "stable tier");
+	}
+	if (!w.stableKeysNext.ensure(E) || !w.stableBodiesNext.ensure(E) || !w.sortKeysAlt.ensure(E) ||
+		!w.sortValsAlt.ensure(E) || !ensure_prim_scratch(w, E))
+		return false;
+	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
+	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
+	if (eKeep > 0)
+	{
+		k_compact_entries<<<grid_for(oldE), kBlock, 0, s>>>(w.stableKeys.p, w.stableBodies.p, keep, keepOff, oldE,
+															  w.stableKeysNext.p, w.stableBodiesNext.p);
+		k_key_bounds<<<grid_for(eKeep), kBlock, 0, s>>>(w.stableKeysNext.p, eKeep, 1, w.counters.p);
+	}
+	if (eNew > 0)
+		k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, offs,
+												   w.stableKeysNext.p + eKeep, w.stableBodiesNext.p + eKeep,
+												   w.counters.p);
+	if (!sync_counters(w))
+		return false;
+	int bits = 0;
+	KeyCompress kc = key_compress_from(w.hostCounters, S, &bits);
+	// kept entries precede new ones, and each sub-world's entries all come
+	// from one of the two, so the stable sort keeps (key, body) order
+	k_key_compress<<<grid_for(E), kBlock, 0, s>>>(w.stableKeysNext.p, E, kc);
+	radix_sort_pairs(w.stableKeysNext.p, w.stableBodiesNext.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits,
+					 w.primScratch.p, w.scanStatus.p, s);
+	k_key_expand<<<grid_for(E), kBlock, 0, s>>>(w.stableKeysNext.p, E, kc);
+	std::swap(w.stableKeys, w.stableKeysNext);
+	std::swap(w.stableBodies, w.stableBodiesNext);
+	w.stableCount = E;
+	return true;
+}
+
+__global__ void k_new_sub(const int32_t* bodySub, const uint64_t* keys, int32_t K, uint64_t* subKey, uint32_t* idx)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k < K)
+	{
+		subKey[k] = (uint64_t)(uint32_t)bodySub[(int32_t)(keys[k] >> 32)];
+		idx[k] = (uint32_t)k;
+	}
+}
+
+__global__ void k_gather_keys(const uint64_t* keys, const uint32_t* idx, int32_t K, uint64_t* out)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k < K)
+		out[k] = keys[idx[k]];
+}
+
+bool stage_new_contacts(World& w, const uint64_t* keys, int32_t K)
+{
+	hipStream_t s = w.stream;
+	w.newCount = K;
+	if (K == 0)
+		return true;
+	if (!w.newKeys.ensure(K) || !w.s64a.ensure(K) || !w.s64b.ensure(K) || !w.s32a.ensure(K) || !w.s32b.ensure(K) ||
+		!ensure_prim_scratch(w, K))
+		return false;
+	k_new_sub<<<grid_for(K), kBlock, 0, s>>>(w.bodySub.p, keys, K, w.s64a.p, w.s32a.p);
+	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, K, bits_for((uint64_t)w.subCount), w.primScratch.p,
+					 w.scanStatus.p, s);
+	k_gather_keys<<<grid_for(K), kBlock, 0, s>>>(keys, w.s32a.p, K, w.newKeys.p);
+	return true;
+}
+
+__device__ __forceinline__ int32_t contact_sub(const int32_t* bodySub, const int2* bodies, int32_t c)
+{
+	return bodySub[bodies[c].x];
+}
+
+__global__ void k_count_removed(const uint32_t* rm, const int2* bodies, const int32_t* bodySub, int32_t C,
+								uint32_t* removed, uint32_t* counters)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c < C && rm[c])
+	{
+		atomicAdd(&removed[contact_sub(bodySub, bodies, c)], 1u);
+		atomicAdd(&counters[CT_REMOVED], 1u);
+	}
+}
+
+__global__ void k_count_new(const uint64_t* keys, int32_t K, const int32_t* bodySub, uint32_t* added)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k < K)
+		atomicAdd(&added[bodySub[(int32_t)(keys[k] >> 32)]], 1u);
+}
+
+// contacts per sub-world after the step's removals and additions (entry S
+// stays zero, so exclusive scans over S + 1 entries end with the total)
+__global__ void k_sub_sizes(const int32_t* start, const uint32_t* removed, const uint32_t* added, int32_t S,
+							uint32_t* newC)
+{
+	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t < S)
+		newC[t] = (uint32_t)(start[t + 1] - start[t]) + added[t] - removed[t];
+	else if (t == S)
+		newC[t] = 0u;
+}
+
+// removal bookkeeping per sub-world range: a hole is a removed contact that
+// stays inside the shrunk range, a tail a survivor beyond it
+__global__ void k_hole_tail_sub(const uint32_t* rm, const int2* bodies, const int32_t* bodySub, const int32_t* start,
+								const uint32_t* newC, int32_t C, uint32_t* hole, uint32_t* tail)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int32_t sw = contact_sub(bodySub, bodies, c);
+	uint32_t l = (uint32_t)(c - start[sw]);
+	bool r = rm[c] != 0;
+	hole[c] = (r && l < newC[sw]) ? 1u : 0u;
+	tail[c] = (!r && l >= newC[sw]) ? 1u : 0u;
+}
+
+struct PlaceCtx
+{
+	const int32_t* bodySub;
+	const int2* bodies;      // old contacts
+	const int32_t* start;    // old ranges (S + 1)
+	const uint32_t* newC;
+	const uint32_t* newStart; // S + 1
+	const uint32_t* newKeyStart;
+	const uint32_t* rm;
+	const uint32_t* holeOff;
+	const uint32_t* tailOff;
+	const int32_t* holeList;  // removed positions inside the shrunk ranges, ascending
+	const uint32_t* counters;
+	int32_t C0;
+};
+
+// The swap-with-last removal of src/world.cpp:1725-1733 on one sub-world's
+// range, in closed form: the k-th tail (ascending) fills the k-th hole.
+__device__ __forceinline__ int32_t place_tail(const PlaceCtx& X, int32_t sw, uint32_t t)
+{
+	int32_t st0 = X.start[sw];
+	int32_t h = X.holeList[X.holeOff[st0] + t];
+	return (int32_t)X.newStart[sw] + (h - st0);
+}
+
+__device__ __forceinline__ void copy_contact(const DContacts& S, int32_t s, const DContacts& D, int32_t d)
+{
+	D.bodies[d] = S.bodies[s];
+	D.hint[d] = S.hint[s];
+	D.mat[d] = S.mat[s];
+	D.normal[d] = S.normal[s];
+	D.cachePA[d] = S.cachePA[s];
+	D.cachePB[d] = S.cachePB[s];
+	D.cacheQA[d] = S.cacheQA[s];
+	D.cacheQB[d] = S.cacheQB[s];
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t pd = pidx(D, d, j), ps = pidx(S, s, j);
+		D.pA[pd] = S.pA[ps];
+		D.pB[pd] = S.pB[ps];
+		D.pJA[pd] = S.pJA[ps];
+		D.pJB[pd] = S.pJB[ps];
+		D.pRA[pd] = S.pRA[ps];
+		D.pRB[pd] = S.pRB[ps];
+		D.pFA[pd] = S.pFA[ps];
+		D.pFB[pd] = S.pFB[ps];
+		D.pSlip[pd] = S.pSlip[ps];
+	}
+}
+
+__global__ void k_place_old(PlaceCtx X, DContacts src, DContacts dstC)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= X.C0 || X.rm[c])
+		return;
+	int32_t sw = contact_sub(X.bodySub, X.bodies, c);
+	int32_t st0 = X.start[sw];
+	uint32_t l = (uint32_t)(c - st0);
+	int32_t d;
+	if (l < X.newC[sw])
+		d = (int32_t)X.newStart[sw] + (int32_t)l;
+	else
+		d = place_tail(X, sw, X.tailOff[c] - X.tailOff[st0]);
+	copy_contact(src, c, dstC, d);
+}
+
+__global__ void k_place_new(PlaceCtx X, const uint64_t* keys, int32_t K, int32_t* dst)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= K)
+		return;
+	int32_t sw = X.bodySub[(int32_t)(keys[k] >> 32)];
+	int32_t st0 = X.start[sw], st1 = X.start[sw + 1];
+	uint32_t c0 = (uint32_t)(st1 - st0);
+	uint32_t l = c0 + ((uint32_t)k - X.newKeyStart[sw]);
+	uint32_t nc = X.newC[sw];
+	if (l < nc)
+	{
+		dst[k] = (int32_t)X.newStart[sw] + (int32_t)l;
+		return;
+	}
+	// tails ascend: the old survivors beyond the shrunk range, then new ones
+	uint32_t oldTails = 0;
+	if (c0 > nc)
+	{
+		uint32_t endOff = st1 < X.C0 ? X.tailOff[st1] : X.counters[CT_TAILS];
+		oldTails = endOff - X.tailOff[st0];
+	}
+	uint32_t first = nc > c0 ? nc : c0;
+	dst[k] = place_tail(X, sw, oldTails + (l - first));
+}
+
+bool remove_and_place(World& w)
+{
+	hipStream_t s = w.stream;
+	int32_t S = w.subCount;
+	int32_t C0 = w.contactCount;
+	int32_t K = w.newCount;
+	if (C0 == 0 && K == 0)
+		return true;
+	if (!w.subWork.ensure(4 * (size_t)S + 8) || !w.u32a.ensure(std::max(C0, 1)) || !w.u32b.ensure(std::max(C0, 1)) ||
+		!w.u32c.ensure(std::max(C0, 1)) || !w.i32a.ensure(std::max(C0, 1)) || !w.i32b.ensure(std::max(C0, 1)) ||
+		!w.i32c.ensure(std::max(K, 1)) || !ensure_prim_scratch(w, std::max<int64_t>(C0, S + 1)))
+		return false;
+	uint32_t* removed = (uint32_t*)w.subWork.p;
+	uint32_t* added = removed + S;
+	uint32_t* newC = added + S;            // S + 1
+	uint32_t* newStart = newC + S + 1;     // S + 1
+	if (!w.i32d.ensure((size_t)S + 1))
+		return false;
+	uint32_t* newKeyStart = (uint32_t*)w.i32d.p; // S + 1
+	launch_fill_u32(removed, 0u, 2 * (int64_t)S, s);
+	const int2* oldBodies = w.cc.bodies.p;
+	uint32_t* rm = w.u32a.p;
+	if (C0 > 0)
+	{
+		k_remove_flags<<<grid_for(C0), kBlock, 0, s>>>(body_view(w), oldBodies, C0, rm);
+		k_count_removed<<<grid_for(C0), kBlock, 0, s>>>(rm, oldBodies, w.bodySub.p, C0, removed, w.counters.p);
+	}
+	if (K > 0)
+		k_count_new<<<grid_for(K), kBlock, 0, s>>>(w.newKeys.p, K, w.bodySub.p, added);
+	if (!sync_counters(w))
+		return false;
+	if (w.hostCounters[CT_REMOVED] == 0 && K == 0)
+		return true;
+	k_sub_sizes<<<grid_for(S + 1), kBlock, 0, s>>>(w.subContactStart.p, removed, added, S, newC);
+	scan_exclusive(newC, newStart, S + 1, w.counters.p + CT_PLACED, w.scanStatus.p, s);
+	// new keys are grouped by sub-world: exclusive scan of the counts
+	// (added has S entries; the scan's total lands in the last slot)
+	scan_exclusive(added, newKeyStart, S, newKeyStart + S, w.scanStatus.p, s);
+	uint32_t* hole = w.u32b.p;
+	uint32_t* tail = w.u32c.p;
+	uint32_t* holeOff = (uint32_t*)w.i32a.p;
+	uint32_t* tailOff = (uint32_t*)w.i32b.p;
+	int32_t H = 0;
+	if (C0 > 0)
+	{
+		k_hole_tail_sub<<<grid_for(C0), kBlock, 0, s>>>(rm, oldBodies, w.bodySub.p, w.subContactStart.p, newC, C0,
+														hole, tail);
+		scan_exclusive(hole, holeOff, C0, w.counters.p + CT_HOLES, w.scanStatus.p, s);
+		scan_exclusive(tail, tailOff, C0, w.counters.p + CT_TAILS, w.scanStatus.p, s);
+	}
+	if (!sync_counters(w))
+		return false;
+	int32_t total = (int32_t)w.hostCounters[CT_PLACED];
+	H = C0 > 0 ? (int32_t)w.hostCounters[CT_HOLES] : 0;
+	if (H > 0)
+	{
+		if (!w.eventIdx.ensure(H))
+			return false;
+		k_compact_indices<<<grid_for(C0), kBlock, 0, s>>>(hole, holeOff, C0, w.eventIdx.p);
+	}
+	if (!ensure_contact_columns(w, w.cc2, w.contactCap2, std::max(total, 1), 0))
+		return false;
+	PlaceCtx X;
+	X.bodySub = w.bodySub.p;
+	X.bodies = oldBodies;
+	X.start = w.subContactStart.p;
+	X.newC = newC;
+	X.newStart = newStart;
+	X.newKeyStart = newKeyStart;
+	X.rm = rm;
+	X.holeOff = holeOff;
+	X.tailOff = tailOff;
+	X.holeList = w.eventIdx.p;
+	X.counters = w.counters.p;
+	X.C0 = C0;
+	DContacts src = contact_view(w);
+	DContacts dstC = columns_view(w.cc2, w.contactCap2);
+	if (C0 > 0)
+		k_place_old<<<grid_for(C0), kBlock, 0, s>>>(X, src, dstC);
+	if (K > 0)
+	{
+		int32_t* dst = w.i32c.p;
+		k_place_new<<<grid_for(K), kBlock, 0, s>>>(X, w.newKeys.p, K, dst);
+		k_create<<<grid_for(K), kBlock, 0, s>>>(body_view(w), dstC, w.newKeys.p, K, 0, dst);
+	}
+	hipMemcpyAsync(w.subContactStart.p, newStart, ((size_t)S + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+	std::swap(w.cc, w.cc2);
+	std::swap(w.contactCap, w.contactCap2);
+	w.contactCount = total;
+	return hip_ok(hipStreamSynchronize(s), "contact placement");
 }
 
 } // namespace mx

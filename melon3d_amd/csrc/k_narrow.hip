@@ -159,6 +159,7 @@ __global__ void k_wake_apply(DBodies B, int32_t n, const int32_t* wakeAt, uint32
 	b.w = 4.0f * kSleepEnergyThresh; // smoothedEnergy
 	B.iw2[i] = b;
 	counters[CT_STABLE_DIRTY] = 1u;
+	mark_sub_stable_dirty(B, i);
 }
 
 bool narrowphase(World& w, float dt)

@@ -144,6 +144,7 @@ __global__ void k_island_wake(DBodies B, int32_t n, const int32_t* islandOf, con
 	b.w = 4.0f * kSleepEnergyThresh;
 	B.iw2[i] = b;
 	counters[CT_STABLE_DIRTY] = 1u;
+	mark_sub_stable_dirty(B, i);
 }
 
 // solver contacts: touching, non-sensor, dynamic body's island awake
@@ -291,6 +292,7 @@ __global__ void k_sleep_apply(DBodies B, const int32_t* islandOf, const uint32_t
 	B.vel[i] = make_float4(0.0f, 0.0f, 0.0f, V.w);
 	B.angVel[i] = make_float4(0.0f, 0.0f, 0.0f, W.w);
 	counters[CT_STABLE_DIRTY] = 1u;
+	mark_sub_stable_dirty(B, i);
 }
 
 // ---------------------------------------------------------- end of step

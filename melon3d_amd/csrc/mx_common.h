@@ -160,7 +160,26 @@ struct DBodies
 	float4* fatHi;      // fat AABB upper xyz
 	uint8_t* bp;        // BP_* bits
 	uint8_t* sb;        // reference sidecar bits (SB_*)
+	// batched worlds (null for a plain world): sub-world of each body, and
+	// per sub-world stable-tier state (SUB_* bits, steps since the last
+	// stable build from bit 8)
+	const int32_t* sub;
+	uint32_t* subStable;
 };
+
+constexpr uint32_t SUB_DIRTY = 1u;   // stable tier membership changed
+constexpr uint32_t SUB_FORCE = 2u;   // stable tier must be rebuilt now
+constexpr uint32_t SUB_REBUILD = 4u; // stable tier is rebuilt this step
+
+#if defined(__HIP__)
+// A sub-world's stable tier is dirty (the per-world stableDirty of
+// src/world.cpp:2106, 130, 565 when worlds are batched).
+__device__ __forceinline__ void mark_sub_stable_dirty(const DBodies& B, int32_t i)
+{
+	if (B.sub != nullptr)
+		atomicOr(&B.subStable[B.sub[i]], SUB_DIRTY);
+}
+#endif
 
 // Contacts: contact columns indexed by contact slot; manifold point columns
 // indexed [pointSlot * cap + contact] so a wave reading point j of 64

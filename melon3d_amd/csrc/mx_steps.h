@@ -48,6 +48,9 @@ enum Counter : int
 	CT_COLOR_ROUNDS,
 	CT_GIANT_ENTRIES, // summed over substeps
 	CT_GIANT_POINTS,  // summed over substeps
+	CT_SUB_REBUILDS,  // batched: sub-worlds rebuilding their stable tier
+	CT_KEPT,          // batched: stable entries kept from the last build
+	CT_PLACED,        // batched: contacts after placement
 	CT_COUNT_
 };
 
@@ -93,6 +96,19 @@ bool launch_wake_list(World& w);
 bool prefetch_sidecar(World& w);
 void launch_upload_scatter(World& w, int32_t count, const int32_t* dIdx, const float4* stage, const uint32_t* stageFlags,
 						   const uint4* stageFilter, const uint8_t* stageBp);
+
+// ---- batched sub-worlds (k_broad.hip) ----
+// host-side sub-world state to the device (body sub-worlds, API dirt)
+bool upload_batch_state(World& w);
+// per sub-world stable-tier decision of this step (src/world.cpp:170, 212-227)
+void launch_stable_plan(World& w);
+// stable tier of the sub-worlds that rebuild it; the others keep theirs
+bool build_stable_batched(World& w);
+// groups the step's new pairs (candidate order) by sub-world
+bool stage_new_contacts(World& w, const uint64_t* keys, int32_t K);
+// separated-pair removal and new-contact placement per sub-world, each sub-world
+// as src/world.cpp:1705-1735 and :1540-1568 would lay out a world of its own
+bool remove_and_place(World& w);
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);

@@ -200,8 +200,8 @@ struct World
 	int32_t jointedCount = 0;
 
 	// grid tiers
-	DBuf<uint64_t> activeKeys, stableKeys, sortKeysAlt;
-	DBuf<uint32_t> activeBodies, stableBodies, sortValsAlt;
+	DBuf<uint64_t> activeKeys, stableKeys, sortKeysAlt, stableKeysNext;
+	DBuf<uint32_t> activeBodies, stableBodies, sortValsAlt, stableBodiesNext;
 	int64_t activeCount = 0, stableCount = 0;
 	std::vector<int32_t> activeOversized, stableOversized; // host copies (small)
 	DBuf<int32_t> dActiveOversized, dStableOversized;
@@ -214,6 +214,8 @@ struct World
 	DBuf<char> primScratch;       // radix sort histograms
 	DBuf<uint64_t> scanStatus;    // scan look-back words (nothing else writes them)
 	DBuf<uint32_t> pairRuns;      // pair enumeration: run tables, item counts and offsets
+	DBuf<uint64_t> overSegs;      // coarse-list candidate segments
+	std::vector<uint64_t> overSegHost;
 	DBuf<uint64_t> hashKeys;
 	DBuf<uint32_t> hashVals;
 	DBuf<int32_t> islandOf;       // per body: island id (-1)
@@ -239,6 +241,29 @@ struct World
 
 	m3dx_step_stats stats{};
 
+	// ---- batched sub-worlds (subCount 0: a plain world) ----
+	// Bodies of sub-world s occupy [subBodyStart[s], subBodyStart[s + 1]);
+	// contacts stay contiguous by sub-world, each range in the order a world
+	// of its own would hold them, so every sub-world steps exactly as it
+	// would alone.
+	int32_t subCount = 0;
+	int32_t subCurrent = 0;
+	std::vector<int32_t> subBodyStart;  // subCount + 1
+	std::vector<int32_t> bodySubHost;   // per body
+	bool bodySubDirty = false;
+	std::vector<uint8_t> subDirtyHost;  // stable-tier dirt raised by API calls
+	bool anySubDirtyHost = false;
+	DBuf<int32_t> bodySub;              // device copy of bodySubHost
+	DBuf<uint32_t> subStable;           // per sub-world SUB_* bits | steps since build << 8
+	DBuf<uint8_t> subDirtyUpload;
+	DBuf<int32_t> subContactStart;      // subCount + 1, contacts by sub-world
+	DBuf<int32_t> subWork;              // per sub-world counters of contact placement
+	DBuf<uint64_t> newKeys;             // created pairs of this step, grouped by sub-world
+	int32_t newCount = 0;
+	std::vector<uint32_t> subStableHost;
+	ContactColumns cc2;                 // placement target, swapped with cc
+	int32_t contactCap2 = 0;
+
 	// ---- kernel timing (off unless enabled) ----
 	struct KernelMark
 	{
@@ -263,6 +288,9 @@ struct World
 // ---- host <-> device transfer (world_sync.cpp) ----
 bool ensure_device_bodies(World& w, int32_t n);
 bool ensure_device_contacts(World& w, int32_t n);
+// grows one contact column set to at least n contacts, keeping the first keep
+bool ensure_contact_columns(World& w, ContactColumns& c, int32_t& cap, int32_t n, size_t keep);
+DContacts columns_view(ContactColumns& cc, int32_t cap);
 bool upload_dirty(World& w);
 bool download_bodies(World& w);
 void ensure_host(World& w);
@@ -274,6 +302,7 @@ bool apply_pending_permutation(World& w);
 void world_step(World& w, float dt, int substeps);
 void refresh_broadphase(World& w, float dt);
 void wake_body_host(World& w, int32_t i);
+void set_stable_dirty(World& w, int32_t i);
 
 } // namespace mx
 

@@ -126,6 +126,78 @@ int m3ds_rain(const m3ds_api* api, m3d_world* w, int count, uint32_t seed, float
 	return count;
 }
 
+/* Box pyramid rows of the reference bench (bench/bench.cpp:48-72): row r holds
+ * base - r boxes of half-extent 0.5 centred on x = 0 at y = 0.5 + r; the
+ * smallest base that holds `count` boxes, filled bottom-up and left to right
+ * until `count` are placed. */
+static void pyramid_rows(const m3ds_api* api, m3d_world* w, int count)
+{
+	int base = 1;
+	while (base * (base + 1) / 2 < count)
+		++base;
+	int made = 0;
+	for (int row = 0; row < base && made < count; ++row)
+	{
+		int n = base - row;
+		double x0 = -0.5 * (double)(n - 1);
+		for (int i = 0; i < n && made < count; ++i, ++made)
+		{
+			m3d_body_def bd = api->body_def_default();
+			bd.type = M3D_BODY_DYNAMIC;
+			bd.position.x = (float)(x0 + (double)i);
+			bd.position.y = (float)(0.5 + (double)row);
+			bd.position.z = 0.0f;
+			m3d_shape_def sd = api->shape_def_default();
+			sd.type = M3D_SHAPE_BOX;
+			sd.halfExtents.x = 0.5f;
+			sd.halfExtents.y = 0.5f;
+			sd.halfExtents.z = 0.5f;
+			sd.friction = 0.6f;
+			(void)api->body_create(w, &bd, &sd);
+		}
+	}
+}
+
+/* One world of the batched workload, index w (seed 1000003 * (w + 1)):
+ * dynamic body count floor(exp(U[ln 16, ln 2049))), then a pile (u < 1/2) or
+ * a pyramid. The pile follows the rain rules in a column scaled to the rain
+ * pile's density (40 x 60 x 40 m per 20 000 bodies, times (count/20000)^(1/3))
+ * on the rain ground; the pyramid stands on the pyramid ground (half-extents
+ * 50 x 0.5 x 50, top face at y = 0). Returns the dynamic body count; *isPile
+ * receives the kind. */
+int m3ds_batch_world(const m3ds_api* api, m3d_world* w, int index, int* isPile)
+{
+	uint32_t rng = 1000003u * (uint32_t)(index + 1);
+	double lo = log(16.0), hi = log(2049.0);
+	int count = (int)floor(exp(lcg_uniform(&rng, lo, hi)));
+	if (count < 16)
+		count = 16;
+	if (count > 2048)
+		count = 2048;
+	int pile = lcg_u(&rng) < 0.5;
+	if (isPile)
+		*isPile = pile;
+	if (pile)
+	{
+		double sc = cbrt((double)count / 20000.0);
+		double gx = 40.0 * sc, gz = 40.0 * sc, gy = 60.0 * sc;
+		add_ground(api, w, gx > 50.0 ? (float)gx : 50.0f, 1.0f, gz > 50.0 ? (float)gz : 50.0f, -1.0f);
+		for (int i = 0; i < count; ++i)
+		{
+			double x = lcg_uniform(&rng, -0.5 * gx, 0.5 * gx);
+			double y = 1.0 + lcg_uniform(&rng, 0.0, gy);
+			double z = lcg_uniform(&rng, -0.5 * gz, 0.5 * gz);
+			(void)mixed_body(api, w, &rng, i % 3, x, y, z, 0.0);
+		}
+	}
+	else
+	{
+		add_ground(api, w, 50.0f, 0.5f, 50.0f, -0.5f);
+		pyramid_rows(api, w, count);
+	}
+	return count;
+}
+
 /* Streaming scene: count bodies uniform in an arena x 50 x arena volume; each
  * churn step destroys the churn oldest (FIFO ring) and spawns as many at
  * y in [30, 60] with downward speed U[5, 20] m/s. */

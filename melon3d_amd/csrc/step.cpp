@@ -96,13 +96,29 @@ static bool apply_material_refresh(World& w)
 bool push_host_changes(World& w)
 {
 	// wakes first: uploaded dirty records are authoritative and come after
-	return apply_pending_permutation(w) && launch_wake_list(w) && upload_dirty(w) && upload_joints(w) &&
-		   apply_material_refresh(w);
+	return apply_pending_permutation(w) && launch_wake_list(w) && upload_dirty(w) && upload_batch_state(w) &&
+		   upload_joints(w) && apply_material_refresh(w);
 }
 
 // RefreshBroadphase; src/world.cpp:156-229
 void refresh_broadphase(World& w, float dt)
 {
+	if (w.subCount > 0)
+	{
+		// per sub-world stable-tier timing; the active tier is rebuilt for all
+		launch_stable_plan(w);
+		launch_refresh(w, dt, false);
+		build_tier(w, 0);
+		if (!sync_counters(w))
+			return;
+		if (w.hostCounters[CT_SUB_REBUILDS] > 0)
+		{
+			build_stable_batched(w);
+			launch_stable_migrate(w);
+		}
+		w.gridDirty = false;
+		return;
+	}
 	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
 	launch_refresh(w, dt, rebuildStable);
 	build_tier(w, 0);
@@ -137,7 +153,9 @@ void world_step(World& w, float dt, int substeps)
 	hipMemsetAsync(w.counters.p, 0, kCountersSize * sizeof(uint32_t), w.stream);
 
 	refresh_broadphase(w, dt);
-	if (!upload_jointed_pairs(w) || !enumerate_and_create(w) || !remove_separated(w))
+	if (!upload_jointed_pairs(w) || !enumerate_and_create(w))
+		return;
+	if (!(w.subCount > 0 ? remove_and_place(w) : remove_separated(w)))
 		return;
 	auto tBroad = Clock::now();
 	if (!narrowphase(w, dt))

@@ -52,10 +52,27 @@ World::~World()
 	b.sb.release();
 	cc.bodies.release();
 	cc.hint.release();
+	{
+		DBuf<float4>* c2[] = { &cc2.mat, &cc2.normal, &cc2.cachePA, &cc2.cachePB, &cc2.cacheQA, &cc2.cacheQB,
+							   &cc2.pA, &cc2.pB, &cc2.pJA, &cc2.pJB, &cc2.pRA, &cc2.pRB, &cc2.pFA, &cc2.pFB,
+							   &cc2.pSlip };
+		for (auto* p : c2)
+			p->release();
+		cc2.bodies.release();
+		cc2.hint.release();
+	}
+	bodySub.release();
+	subStable.release();
+	subDirtyUpload.release();
+	subContactStart.release();
+	subWork.release();
+	newKeys.release();
 	djoints.release();
 	jointedKeys.release();
 	activeKeys.release();
 	stableKeys.release();
+	stableKeysNext.release();
+	stableBodiesNext.release();
 	sortKeysAlt.release();
 	activeBodies.release();
 	stableBodies.release();
@@ -77,6 +94,7 @@ World::~World()
 	u8a.release();
 	primScratch.release();
 	pairRuns.release();
+	overSegs.release();
 	hashKeys.release();
 	scanStatus.release();
 	hashVals.release();
@@ -225,13 +243,16 @@ static bool grow_points(DBuf<float4>& col, size_t oldCap, size_t newCap, size_t 
 
 bool ensure_device_contacts(World& w, int32_t n)
 {
-	if (n <= w.contactCap)
+	return ensure_contact_columns(w, w.cc, w.contactCap, n, (size_t)w.contactCount);
+}
+
+bool ensure_contact_columns(World& w, ContactColumns& c, int32_t& capRef, int32_t n, size_t keep)
+{
+	if (n <= capRef)
 		return true;
 	(void)hipStreamSynchronize(w.stream);
-	size_t keep = (size_t)w.contactCount;
-	size_t oldCap = (size_t)w.contactCap;
+	size_t oldCap = (size_t)capRef;
 	size_t cap = std::max<size_t>((size_t)n, oldCap * 3 / 2 + 256);
-	ContactColumns& c = w.cc;
 	DBuf<float4>* cols[] = { &c.mat, &c.normal, &c.cachePA, &c.cachePB, &c.cacheQA, &c.cacheQB };
 	for (auto* p : cols)
 		if (!realloc_exact(*p, cap, keep))
@@ -242,7 +263,7 @@ bool ensure_device_contacts(World& w, int32_t n)
 	for (auto* p : pts)
 		if (!grow_points(*p, oldCap, cap, keep))
 			return false;
-	w.contactCap = (int32_t)cap;
+	capRef = (int32_t)cap;
 	return true;
 }
 
@@ -499,6 +520,20 @@ bool apply_pending_permutation(World& w)
 	return hip_ok(hipStreamSynchronize(w.stream), "permutation");
 }
 
+// stableDirty of the world holding body i (per sub-world when batched)
+void set_stable_dirty(World& w, int32_t i)
+{
+	if (w.subCount > 0)
+	{
+		w.subDirtyHost[w.bodySubHost[i]] = 1;
+		w.anySubDirtyHost = true;
+	}
+	else
+	{
+		w.stableDirty = true;
+	}
+}
+
 // WakeBodyInternal on the host mirror; src/world.cpp:121-140
 void wake_body_host(World& w, int32_t i)
 {
@@ -506,7 +541,7 @@ void wake_body_host(World& w, int32_t i)
 	if (b.type == M3D_BODY_STATIC)
 		return;
 	if (!b.isAwake)
-		w.stableDirty = true;
+		set_stable_dirty(w, i);
 	b.isAwake = true;
 	b.sleepTime = 0.0f;
 	b.smoothedEnergy = 4.0f * kSleepEnergyThresh;
@@ -724,6 +759,89 @@ m3d_world* m3d_world_create(const m3d_world_def* def)
 	return wp;
 }
 
+// Batched worlds: one device world holding `count` independent sub-worlds
+// that share the world definition. Bodies go to the selected sub-world;
+// sub-worlds never interact and each steps exactly as a world of its own.
+m3d_world* m3dx_batch_create(const m3d_world_def* def, int32_t count)
+{
+	if (count < 1 || count > (1 << 18))
+	{
+		set_error("batch size %d outside [1, 262144]", count);
+		return nullptr;
+	}
+	m3d_world* wp = m3d_world_create(def);
+	if (wp == nullptr)
+		return nullptr;
+	World& w = wp->w;
+	w.subCount = count;
+	w.subCurrent = 0;
+	w.subBodyStart.assign((size_t)count + 1, 0);
+	w.subDirtyHost.assign((size_t)count, 0);
+	w.subStableHost.assign((size_t)count, 0);
+	bool ok = w.subStable.ensure(count) && w.subContactStart.ensure((size_t)count + 1) &&
+			  w.subWork.ensure(4 * (size_t)count + 4) && w.subDirtyUpload.ensure(count);
+	if (ok)
+	{
+		// every sub-world starts dirty, 1000 steps after its last stable build
+		// (src/m3d_internal.h:624-626)
+		std::vector<uint32_t> init((size_t)count, SUB_DIRTY | (1000u << 8));
+		ok = hip_ok(hipMemcpy(w.subStable.p, init.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice),
+					"batch init") &&
+			 hip_ok(hipMemset(w.subContactStart.p, 0, ((size_t)count + 1) * sizeof(int32_t)), "batch init");
+	}
+	if (!ok)
+	{
+		m3d_world_destroy(wp);
+		return nullptr;
+	}
+	return wp;
+}
+
+// Selects the sub-world that m3d_body_create adds bodies to; sub-worlds are
+// filled in ascending order. Returns 0, or -1 (with m3dx_last_error) for a
+// plain world or an index below the current one.
+int m3dx_batch_select(m3d_world* world, int32_t sub)
+{
+	World& w = world->w;
+	if (w.subCount == 0 || sub < w.subCurrent || sub >= w.subCount)
+	{
+		set_error("sub-world %d cannot be selected (batch of %d, current %d)", sub, w.subCount, w.subCurrent);
+		return -1;
+	}
+	w.subCurrent = sub;
+	return 0;
+}
+
+int32_t m3dx_batch_size(m3d_world* world) { return world->w.subCount; }
+
+// Body index range of a sub-world; returns -1 for a bad index.
+int m3dx_batch_body_range(m3d_world* world, int32_t sub, int32_t* first, int32_t* count)
+{
+	World& w = world->w;
+	if (sub < 0 || sub >= w.subCount)
+		return -1;
+	*first = w.subBodyStart[sub];
+	*count = w.subBodyStart[sub + 1] - w.subBodyStart[sub];
+	return 0;
+}
+
+// Contacts per sub-world after the last step (cap entries); returns the
+// number of sub-worlds.
+int m3dx_batch_contact_counts(m3d_world* world, int32_t* out, int32_t cap)
+{
+	World& w = world->w;
+	if (w.subCount == 0)
+		return 0;
+	std::vector<int32_t> st((size_t)w.subCount + 1);
+	(void)hipStreamSynchronize(w.stream);
+	if (!hip_ok(hipMemcpy(st.data(), w.subContactStart.p, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost),
+				"batch contact counts"))
+		return -1;
+	for (int32_t s = 0; s < w.subCount && s < cap; ++s)
+		out[s] = st[s + 1] - st[s];
+	return w.subCount;
+}
+
 void m3d_world_destroy(m3d_world* world)
 {
 	if (world == nullptr)
@@ -789,7 +907,15 @@ m3d_body_id m3d_body_create(m3d_world* world, const m3d_body_def* bd, const m3d_
 	update_inv_inertia(b);
 	b.bp = 0; // recycled slot: stale fat AABB and tier membership reset
 	w.gridDirty = true;
-	w.stableDirty = true;
+	if (w.subCount > 0)
+	{
+		// bodies of a sub-world are one index range, in sub-world order
+		w.bodySubHost.push_back(w.subCurrent);
+		for (int32_t s = w.subCurrent + 1; s <= w.subCount; ++s)
+			w.subBodyStart[s] = index + 1;
+		w.bodySubDirty = true;
+	}
+	set_stable_dirty(w, index);
 	mark_dirty(w, index);
 	return make_id(w, index);
 }
@@ -807,7 +933,7 @@ static void wake_partner(World& w, int32_t i)
 	}
 	if (w.bodies[i].type == M3D_BODY_STATIC)
 		return;
-	w.stableDirty = true;
+	set_stable_dirty(w, i);
 	w.deviceWake.push_back(i);
 }
 
@@ -817,6 +943,12 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 	HostBody* b = body_checked(w, id);
 	if (b == nullptr)
 		return;
+	if (w.subCount > 0)
+	{
+		// a recycled slot would break the per-sub-world body ranges
+		set_error("bodies of batched worlds cannot be destroyed");
+		return;
+	}
 	int32_t index = id.index;
 	w.destroyedSinceStep = true;
 	if (ensure_sidecar(w))
@@ -937,7 +1069,7 @@ void m3d_body_set_transform(m3d_world* world, m3d_body_id id, m3d_vec3 position,
 	update_inv_inertia(*b);
 	b->bp &= (uint8_t)~BP_HAS_FAT;
 	w.gridDirty = true;
-	w.stableDirty = true;
+	set_stable_dirty(w, id.index);
 	mark_dirty(w, id.index);
 	wake_body_host(w, id.index);
 }
@@ -1143,6 +1275,12 @@ void m3d_world_set_gravity(m3d_world* world, m3d_vec3 gravity) { world->w.def.gr
 // ------------------------------------------------------------ joints
 
 // src/world.cpp:868-890
+// joints of batched worlds stay inside one sub-world
+static bool same_sub_world(const World& w, int32_t a, int32_t b)
+{
+	return w.subCount == 0 || w.bodySubHost[a] == w.bodySubHost[b];
+}
+
 static m3d_joint_id allocate_joint(World& w, DJoint** out)
 {
 	int32_t index;
@@ -1173,7 +1311,8 @@ m3d_joint_id m3d_joint_create_distance(m3d_world* world, const m3d_distance_join
 	ensure_host(w);
 	HostBody* a = body_checked(w, def->bodyA);
 	HostBody* b = body_checked(w, def->bodyB);
-	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index)
+	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index ||
+		!same_sub_world(w, def->bodyA.index, def->bodyB.index))
 		return { -1, 0 };
 	DJoint* j;
 	m3d_joint_id id = allocate_joint(w, &j);
@@ -1207,7 +1346,8 @@ m3d_joint_id m3d_joint_create_ball(m3d_world* world, const m3d_ball_joint_def* d
 	ensure_host(w);
 	HostBody* a = body_checked(w, def->bodyA);
 	HostBody* b = body_checked(w, def->bodyB);
-	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index)
+	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index ||
+		!same_sub_world(w, def->bodyA.index, def->bodyB.index))
 		return { -1, 0 };
 	DJoint* j;
 	m3d_joint_id id = allocate_joint(w, &j);
@@ -1229,7 +1369,8 @@ m3d_joint_id m3d_joint_create_hinge(m3d_world* world, const m3d_hinge_joint_def*
 	ensure_host(w);
 	HostBody* a = body_checked(w, def->bodyA);
 	HostBody* b = body_checked(w, def->bodyB);
-	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index)
+	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index ||
+		!same_sub_world(w, def->bodyA.index, def->bodyB.index))
 		return { -1, 0 };
 	DJoint* j;
 	m3d_joint_id id = allocate_joint(w, &j);
@@ -1276,7 +1417,8 @@ m3d_joint_id m3d_joint_create_weld(m3d_world* world, const m3d_weld_joint_def* d
 	ensure_host(w);
 	HostBody* a = body_checked(w, def->bodyA);
 	HostBody* b = body_checked(w, def->bodyB);
-	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index)
+	if (a == nullptr || b == nullptr || def->bodyA.index == def->bodyB.index ||
+		!same_sub_world(w, def->bodyA.index, def->bodyB.index))
 		return { -1, 0 };
 	DJoint* j;
 	m3d_joint_id id = allocate_joint(w, &j);
@@ -1352,7 +1494,9 @@ static m3d_ray_result ray_cast_internal(World& w, m3d_vec3 origin, m3d_vec3 tran
 	r.normal = m3d_v3_zero();
 	r.fraction = 2.0f;
 	r.hit = false;
-	if (w.gridDirty || w.stableDirty)
+	// (batched worlds: the refresh state is per sub-world, so queries read
+	// the broadphase of the last step)
+	if (w.subCount == 0 && (w.gridDirty || w.stableDirty))
 	{
 		if (push_host_changes(w))
 		{

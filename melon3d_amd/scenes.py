@@ -175,6 +175,8 @@ def _scene_driver():
         d.m3ds_stream_churn.restype = None
         d.m3ds_stream_clone.argtypes = [C.c_void_p, C.POINTER(_SceneApi), C.c_void_p]
         d.m3ds_stream_clone.restype = C.c_void_p
+        d.m3ds_batch_world.argtypes = [C.POINTER(_SceneApi), C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        d.m3ds_batch_world.restype = C.c_int
         d.m3ds_stream_free.argtypes = [C.c_void_p]
         d.m3ds_stream_free.restype = None
         _driver = d
@@ -227,3 +229,35 @@ class NativeStream:
         if h and _driver is not None:
             _driver.m3ds_stream_free(h)
             self._h = None
+
+
+# ------------------------------------------------------------ batched worlds
+
+def batch_world(L: Library, w, index: int):
+    """World `index` of the batched workload built into world `w` (a plain
+    world, or a batch with that sub-world selected). Returns (dynamic body
+    count, is_pile)."""
+    t = _api_table(L)
+    kind = C.c_int(0)
+    n = _scene_driver().m3ds_batch_world(C.byref(t), _world_ptr(w), index, C.byref(kind))
+    return n, bool(kind.value)
+
+
+def batch(L: Library, worlds: int, first: int = 0, wd=None):
+    """A batch world holding workload worlds first .. first + worlds - 1 as
+    sub-worlds 0 .. worlds - 1. Returns (world, [(count, is_pile)])."""
+    if wd is None:
+        wd = L.m3d_world_def_default()
+    w = L.m3dx_batch_create(C.byref(wd), worlds)
+    if not w:
+        raise RuntimeError("m3dx_batch_create failed: " + L.last_error())
+    t = _api_table(L)
+    d = _scene_driver()
+    info = []
+    kind = C.c_int(0)
+    for s in range(worlds):
+        if L.m3dx_batch_select(w, s) != 0:
+            raise RuntimeError(L.last_error())
+        n = d.m3ds_batch_world(C.byref(t), _world_ptr(w), first + s, C.byref(kind))
+        info.append((n, bool(kind.value)))
+    return w, info

@@ -531,6 +531,14 @@ void m3dx_world_stats(m3d_world* w, m3dx_step_stats* out)
 	out->stableEntries = (int32_t)w->grid.m_stable.size();
 }
 
+/* batched worlds are a device-backend extension: the CPU engine steps one
+   world per call, so a batch is a loop over plain worlds there */
+m3d_world* m3dx_batch_create(const m3d_world_def*, int32_t) { return nullptr; }
+int m3dx_batch_select(m3d_world*, int32_t) { return -1; }
+int32_t m3dx_batch_size(m3d_world*) { return 0; }
+int m3dx_batch_body_range(m3d_world*, int32_t, int32_t*, int32_t*) { return -1; }
+int m3dx_batch_contact_counts(m3d_world*, int32_t*, int32_t) { return 0; }
+
 /* no device kernels on the CPU engine: timing is unavailable */
 int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
 

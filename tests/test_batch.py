@@ -1,0 +1,171 @@
+"""Batched sub-worlds: every sub-world of a batch steps exactly as a world of
+its own holding the same bodies.
+
+The batch is new (the reference steps one world per call), so its oracle is
+the single-world step: the plain device world, which matches the reference
+engine step for step (test_parity.py), and the reference engine itself for a
+few steps. A batch sub-world must equal its plain twin bit for bit: same
+bodies, same contacts in the same order with the same manifolds, through
+sleeping and waking (per-sub-world stable-tier timing) and contact removal.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from melon3d_amd import scenes
+from parity import bodies_array, rot_angle
+
+DT = 1.0 / 60.0
+
+
+def _counts(dev, w, S):
+    out = (C.c_int32 * S)()
+    assert dev.m3dx_batch_contact_counts(w, out, S) == S
+    return list(out)
+
+
+def _split(dev, w, S):
+    """Per sub-world (bodies, contact rows) of a batch, body indices local."""
+    snap = dev.export_state(w)
+    pos, rot, lv, av, use, awake = bodies_array(snap)
+    counts = _counts(dev, w, S)
+    out = []
+    c0 = 0
+    for s in range(S):
+        first, n = C.c_int32(0), C.c_int32(0)
+        assert dev.m3dx_batch_body_range(w, s, C.byref(first), C.byref(n)) == 0
+        b0, nb = first.value, n.value
+        sl = slice(b0, b0 + nb)
+        bodies = (pos[sl], rot[sl], lv[sl], av[sl], use[sl], awake[sl])
+        rows = [_contact_row(snap.contacts[c], b0) for c in range(c0, c0 + counts[s])]
+        out.append((bodies, rows))
+        c0 += counts[s]
+    assert c0 == snap.header.contactCount
+    return out
+
+
+def _contact_row(c, b0):
+    pts = tuple((tuple(p.localAnchorA), tuple(p.localAnchorB), p.separation, p.id, p.lambdaN, p.lambdaT,
+                 tuple(p.slipDir)) for p in list(c.points)[:c.pointCount])
+    return (c.bodyA - b0, c.bodyB - b0, c.touching, c.pointCount, tuple(c.normal), pts)
+
+
+def _plain(dev, w):
+    snap = dev.export_state(w)
+    rows = [_contact_row(snap.contacts[c], 0) for c in range(snap.header.contactCount)]
+    return bodies_array(snap), rows
+
+
+def _assert_same(batch_part, plain_part, s, step):
+    (bb, brows), (pb, prows) = batch_part, plain_part
+    for k, name in enumerate(("position", "rotation", "linear velocity", "angular velocity", "in use", "awake")):
+        assert np.array_equal(bb[k], pb[k]), f"sub-world {s} step {step}: {name} differs"
+    assert len(brows) == len(prows), f"sub-world {s} step {step}: {len(brows)} vs {len(prows)} contacts"
+    for i, (a, b) in enumerate(zip(brows, prows)):
+        assert a == b, f"sub-world {s} step {step}: contact {i} differs"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("first,S,steps,every", [(0, 12, 60, 10), (40, 6, 240, 40)])
+def test_batch_equals_plain_worlds(dev, first, S, steps, every):
+    """Bitwise equality with S plain worlds, including sleep (the second case
+    runs 4 s, long enough for piles and pyramids to sleep, which rebuilds the
+    stable tiers of sub-worlds at different steps)."""
+    bw, info = scenes.batch(dev, S, first=first)
+    plain = []
+    for s in range(S):
+        w = dev.create_world()
+        n, pile = scenes.batch_world(dev, w, first + s)
+        assert (n, pile) == info[s]
+        plain.append(w)
+    asleep = 0
+    try:
+        for step in range(1, steps + 1):
+            dev.m3d_world_step(bw, DT, 4)
+            for w in plain:
+                dev.m3d_world_step(w, DT, 4)
+            if step % every == 0 or step == steps:
+                parts = _split(dev, bw, S)
+                for s in range(S):
+                    _assert_same(parts[s], _plain(dev, plain[s]), s, step)
+                asleep = sum(int((~p[0][5] & p[0][4]).sum()) for p in parts)
+    finally:
+        dev.m3d_world_destroy(bw)
+        for w in plain:
+            dev.m3d_world_destroy(w)
+    if steps >= 240:
+        assert asleep > 0, "no body fell asleep: the stable-tier path went untested"
+
+
+@pytest.mark.gpu
+def test_batch_against_reference(dev, ref):
+    """Four sub-worlds against the reference engine, 3 steps, 1e-5."""
+    S, first = 4, 7
+    bw, info = scenes.batch(dev, S, first=first)
+    refs = []
+    for s in range(S):
+        w = ref.create_world()
+        scenes.batch_world(ref, w, first + s)
+        refs.append(w)
+    try:
+        for step in range(3):
+            dev.m3d_world_step(bw, DT, 4)
+            for w in refs:
+                ref.m3d_world_step(w, DT, 4)
+        parts = _split(dev, bw, S)
+        for s in range(S):
+            rs = ref.export_state(refs[s])
+            pb, rb, vb, wb, ub, ab = bodies_array(rs)
+            pa, ra, va, wa, ua, aa = parts[s][0]
+            assert (ua == ub).all()
+            assert float(np.abs(pa - pb).max()) <= 1e-5, s
+            assert float(rot_angle(ra, rb).max()) <= 1e-5, s
+            assert (aa == ab).all(), s
+            assert len(parts[s][1]) == rs.header.contactCount
+    finally:
+        dev.m3d_world_destroy(bw)
+        for w in refs:
+            ref.m3d_world_destroy(w)
+
+
+@pytest.mark.gpu
+def test_batch_rejects_destroy_and_cross_joints(dev):
+    bw, info = scenes.batch(dev, 2, first=0)
+    try:
+        first, n = C.c_int32(0), C.c_int32(0)
+        dev.m3dx_batch_body_range(bw, 1, C.byref(first), C.byref(n))
+        from melon3d_amd.api import m3d_ball_joint_def, m3d_body_id
+        a = m3d_body_id(1, 1)
+        b = m3d_body_id(first.value + 1, 1)
+        dev.m3d_body_destroy(bw, a)
+        assert dev.m3d_body_is_valid(bw, a)
+        jd = m3d_ball_joint_def()
+        jd.bodyA, jd.bodyB = a, b
+        assert dev.m3d_joint_create_ball(bw, C.byref(jd)).index == -1
+        jd.bodyB = m3d_body_id(2, 1)  # same sub-world
+        assert dev.m3d_joint_create_ball(bw, C.byref(jd)).index >= 0
+        assert dev.m3dx_batch_select(bw, 0) == -1  # below the current sub-world
+    finally:
+        dev.m3d_world_destroy(bw)
+
+
+def test_batch_workload_generator(ref):
+    """The batched workload: log-uniform counts in [16, 2048], piles and
+    pyramids about half each, deterministic per world index."""
+    counts, kinds = [], []
+    for i in range(256):
+        w = ref.create_world()
+        n, p = scenes.batch_world(ref, w, i)
+        ref.m3d_world_destroy(w)
+        counts.append(n)
+        kinds.append(p)
+    assert min(counts) >= 16 and max(counts) <= 2048
+    assert 90 <= sum(kinds) <= 166
+    # log-uniform: about a third of the worlds below 64 bodies (ln 4 / ln 128)
+    assert 55 <= sum(c < 64 for c in counts) <= 115
+    w = ref.create_world()
+    assert scenes.batch_world(ref, w, 5) == (counts[5], kinds[5])
+    ref.m3d_world_destroy(w)
+    # the reference engine has no batches
+    assert not ref.m3dx_batch_create(None, 4)
