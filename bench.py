@@ -164,6 +164,93 @@ def batch_loop_plain(dev, worlds: int, warmup: int, steps: int, substeps: int):
                                                                            f"bodies), {steps} steps after {warmup}"}
 
 
+def run_diff(args, dev):
+    """Differentiable worlds: forward (the engine's steps, recording each
+    step's solve inputs) and backward (the adjoint of all recorded steps) of
+    the final-COM loss; plus the gradient check on 16 worlds over 3 steps."""
+    import numpy as np
+
+    from melon3d_amd import diff
+
+    scene = diff.DiffScene(worlds=args.worlds)
+    w = scene.build(dev)
+    if dev.m3dx_diff_record(w, 1) != 0:
+        raise SystemExit(dev.last_error())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.m3d_world_step(w, DT, args.substeps)
+    fwd = time.perf_counter() - t0
+    dev.m3dx_diff_record(w, 0)
+    engine = diff.final_positions(dev, w, scene)
+    m = scene.mass[..., None]
+    com = (m * engine).sum(1) / scene.mass.sum(1)[:, None]
+    g_pos = (2.0 * (com - scene.target))[:, None, :] * m / scene.mass.sum(1)[:, None, None]
+    print(f"# forward {fwd:.2f} s, backward ...", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    gv, gf, gm = diff.backward(dev, w, scene, g_pos)
+    bwd = time.perf_counter() - t0
+    dev.m3d_world_destroy(w)
+    # gradient check: per-world directional derivatives vs FP64 central differences
+    cs = diff.DiffScene(worlds=16)
+    cw = cs.build(dev)
+    dev.m3dx_diff_record(cw, 1)
+    for _ in range(3):
+        dev.m3d_world_step(cw, DT, args.substeps)
+    dev.m3dx_diff_record(cw, 0)
+    e = diff.final_positions(dev, cw, cs)
+    cm = cs.mass[..., None]
+    msum = cs.mass.sum(1)[:, None, None]
+    ccom = (cm * e).sum(1) / cs.mass.sum(1)[:, None]
+    g_com = 2.0 * (ccom - cs.target)
+    cgv, cgf, cgm = diff.backward(dev, cw, cs, g_com[:, None, :] * cm / msum)
+    cgm = cgm + (g_com[:, None, :] * ((e - ccom[:, None, :]) / msum)).sum(2)
+    rng = np.random.default_rng(11)
+    dv, df, dm = rng.normal(size=cs.v0.shape), rng.normal(size=cs.friction.shape) * 0.05, rng.normal(
+        size=cs.mass.shape) * 10.0
+    adj = (cgv * dv).sum((1, 2)) + (cgf * df).sum(1) + (cgm * dm).sum(1)
+
+    def losses(eps):
+        st = diff.replay(dev, cw, cs, 3, precision=1, v0=cs.v0 + eps * dv, friction=cs.friction + eps * df,
+                         mass=cs.mass + eps * dm)
+        pos = st[cs.dynamic_index(), 0:3]
+        mm = (cs.mass + eps * dm)[..., None]
+        c = (mm * pos).sum(1) / mm.sum(1)
+        return ((c - cs.target) ** 2).sum(1)
+
+    fd = (losses(1e-4) - losses(-1e-4)) / 2e-4
+    rel = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-12)
+    dev.m3d_world_destroy(cw)
+    dyn = args.worlds * diff.BODIES
+    out = {
+        "metric": "body-steps/sec",
+        "value": dyn * args.steps / fwd,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * fwd / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"differentiable: {args.worlds} worlds x 64 bodies (spheres r 0.5, cubes he 0.5), "
+                               f"v0 ~ N(0, 2), friction ~ U[0.2, 0.9], loss |COM - target|^2 after {args.steps} "
+                               f"steps, dt 1/60 s, {args.substeps} substeps",
+                   "worlds": args.worlds, "bodies": dyn, "parallelism": "single GPU"},
+        "forward_s": fwd,
+        "backward_s": bwd,
+        "backward_over_forward": bwd / fwd,
+        "grad_check": {"worlds": 16, "steps": 3, "h": 1e-4, "max_rel_err": float(rel.max()),
+                       "median_rel_err": float(np.median(rel))},
+        "grad_norms": {"v0": float(np.linalg.norm(gv)), "friction": float(np.linalg.norm(gf)),
+                       "mass": float(np.linalg.norm(gm))},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args, dev):
     from melon3d_amd import scenes
 
@@ -273,7 +360,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid", "batch"])
+    ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid", "batch", "diff"])
     ap.add_argument("--worlds", type=int, default=4096, help="batch: number of worlds")
     ap.add_argument("--loop-worlds", type=int, default=64, help="batch: worlds stepped one by one for comparison")
     ap.add_argument("--cpu-worlds", type=int, default=256, help="batch: worlds in the CPU baseline sample")
@@ -306,6 +393,13 @@ def main():
     dev = api.load()
     if args.scene == "batch":
         run_batch(args, dev)
+        return
+    if args.scene == "diff":
+        if args.worlds == 4096:
+            args.worlds = 256
+        if args.steps == 200:
+            args.steps = 120
+        run_diff(args, dev)
         return
     if args.bodies is None:
         args.bodies = {"stream": 200000, "rain": 20000, "pyramid": 210}[args.scene]

@@ -288,6 +288,41 @@ M3D_API int m3dx_batch_body_range(m3d_world* world, int32_t sub, int32_t* first,
 /* Live contacts of each sub-world (up to cap); returns the sub-world count. */
 M3D_API int m3dx_batch_contact_counts(m3d_world* world, int32_t* out, int32_t cap);
 
+/* ---------------------------------------------- differentiable worlds */
+
+/* Adjoint of the XPBD solve (no counterpart in the reference). While
+   recording, every step stores its solve inputs: body state, mass and
+   friction, and the solver contacts with their manifolds (the fixed contact
+   set of that step). The adjoint is that of the step's solve (the four
+   substeps of integrate, prepare, normal projection, stiction, velocity
+   reconstruction and velocity pass, in the serial-island order) with the
+   contact set and manifolds held fixed; a recorded step's output state is
+   the next step's input. Requires a world created with enableSleep = false;
+   bodies must not be created or destroyed while recording. Plain worlds and
+   batches (one lane per sub-world) both work. */
+/* enable != 0 starts recording (dropping earlier records), 0 stops it.
+   Returns 0 or -1. */
+M3D_API int m3dx_diff_record(m3d_world* world, int enable);
+/* Number of recorded steps. */
+M3D_API int32_t m3dx_diff_steps(m3d_world* world);
+/* State recorded at the start of a step (13 floats per body: x q v w).
+   Returns the step's solver contact count, or -1. */
+M3D_API int m3dx_diff_state(m3d_world* world, int step, float* out);
+/* Replays recorded steps first .. first + count - 1 in sequence from the
+   state recorded at `first` (or stateIn: 13 values per body, x q v w),
+   with per-body mass and friction (null: as recorded). precision 0 is the
+   engine's float arithmetic, 1 double; arrays are host memory of that type.
+   stateOut receives the final state (13 per body). Returns 0 or -1. */
+M3D_API int m3dx_diff_replay(m3d_world* world, int first, int count, int precision, const void* stateIn,
+                             const void* mass, const void* friction, void* stateOut);
+/* Vector-Jacobian product of all recorded steps: seed is the adjoint of the
+   final state (13 floats per body: x q v w); returns the adjoint of the first
+   step's input state (13 per body; its v part is the gradient with respect to
+   the initial linear velocities) and the gradients with respect to each
+   body's mass and friction. Output pointers may be null. Returns 0 or -1. */
+M3D_API int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad,
+                               float* frictionGrad);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

@@ -336,6 +336,11 @@ _SIGS = {
     "m3dx_batch_size": (C.c_int32, [W]),
     "m3dx_batch_body_range": (C.c_int, [W, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "m3dx_batch_contact_counts": (C.c_int, [W, C.POINTER(C.c_int32), C.c_int32]),
+    "m3dx_diff_record": (C.c_int, [W, C.c_int]),
+    "m3dx_diff_steps": (C.c_int32, [W]),
+    "m3dx_diff_state": (C.c_int, [W, C.c_int, C.c_void_p]),
+    "m3dx_diff_replay": (C.c_int, [W, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "m3dx_diff_backward": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "m3dx_last_error": (C.c_char_p, []),
     "m3dx_set_device": (C.c_int, []),
 }

@@ -1156,6 +1156,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	{
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
+		if (!diff_record_substep(w, sub))
+			return false;
 		k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
 		k_giant_records<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_NORMALS);

@@ -462,7 +462,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		L.NB = NB;
 		L.NC = NC;
 		L.bitsI = bitsI;
-		if (!solve_islands(w, L, dt, substeps))
+		if (!diff_record_step(w, L, dt, substeps) || !solve_islands(w, L, dt, substeps))
 			return false;
 	}
 

@@ -1,0 +1,959 @@
+// Differentiable replay of the XPBD solve of one step (the four substeps of
+// src/xpbd.cpp:898-974 for serial islands: integrate, prepare, normal
+// projection iterations, stiction, velocity reconstruction, velocity pass)
+// over a fixed contact set: the solver contacts and manifolds the engine's
+// narrowphase produced for that step.
+//
+// Everything is written once over a generic scalar T:
+//   float      the same expressions in the same order as mx_solver.h, so a
+//              replay reproduces the engine's solve bit for bit;
+//   double     the FP64 replay used for finite differences;
+//   Ad<F>      a reverse-mode tape: every arithmetic result is a node with
+//              its two partial derivatives; walking the tape backwards
+//              gives the adjoint of the whole step.
+// Branch decisions (sep < 0, caps, friction cone, slip coherence) are taken
+// on values, so the adjoint is that of the piecewise-smooth step along the
+// branches the forward pass took.
+#pragma once
+
+#include "mx_common.h"
+
+#include <math.h>
+
+namespace mx
+{
+namespace dd
+{
+
+// ------------------------------------------------------------ tape
+
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+
+// Node k of one lane at index k * stride (lanes interleaved for coalescing).
+template <class F>
+struct Tape
+{
+	uint32_t* a;
+	uint32_t* b;
+	F* da;
+	F* db;
+	uint64_t stride;
+	uint32_t n;
+	uint32_t cap;
+	int overflow;
+};
+
+template <class F>
+struct Ad
+{
+	F v;
+	uint32_t id;
+	Tape<F>* t;
+};
+
+template <class F>
+MX_HD uint32_t push_node(Tape<F>* t, uint32_t a, F da, uint32_t b, F db)
+{
+	if (t == nullptr)
+		return kNoNode;
+	if (t->n >= t->cap)
+	{
+		t->overflow = 1;
+		return kNoNode;
+	}
+	uint64_t k = (uint64_t)t->n * t->stride;
+	t->a[k] = a;
+	t->b[k] = b;
+	t->da[k] = da;
+	t->db[k] = db;
+	return t->n++;
+}
+
+template <class F>
+MX_HD Ad<F> ad_make(F v, uint32_t a, F da, uint32_t b, F db, Tape<F>* t)
+{
+	Ad<F> r;
+	r.v = v;
+	r.t = t;
+	r.id = (a == kNoNode && b == kNoNode) ? kNoNode : push_node(t, a, da, b, db);
+	return r;
+}
+
+template <class F>
+MX_HD Tape<F>* tape_of(const Ad<F>& x, const Ad<F>& y)
+{
+	return x.t != nullptr ? x.t : y.t;
+}
+
+template <class F>
+MX_HD Ad<F> ad_const(F v)
+{
+	Ad<F> r;
+	r.v = v;
+	r.id = kNoNode;
+	r.t = nullptr;
+	return r;
+}
+
+// a leaf (parameter or state input): a node without inputs
+template <class F>
+MX_HD Ad<F> ad_leaf(F v, Tape<F>* t)
+{
+	Ad<F> r;
+	r.v = v;
+	r.t = t;
+	r.id = push_node(t, kNoNode, (F)0, kNoNode, (F)0);
+	return r;
+}
+
+template <class F>
+MX_HD Ad<F> operator+(Ad<F> x, Ad<F> y)
+{
+	return ad_make<F>(x.v + y.v, x.id, (F)1, y.id, (F)1, tape_of(x, y));
+}
+template <class F>
+MX_HD Ad<F> operator-(Ad<F> x, Ad<F> y)
+{
+	return ad_make<F>(x.v - y.v, x.id, (F)1, y.id, (F)-1, tape_of(x, y));
+}
+template <class F>
+MX_HD Ad<F> operator*(Ad<F> x, Ad<F> y)
+{
+	return ad_make<F>(x.v * y.v, x.id, y.v, y.id, x.v, tape_of(x, y));
+}
+template <class F>
+MX_HD Ad<F> operator/(Ad<F> x, Ad<F> y)
+{
+	F r = x.v / y.v;
+	return ad_make<F>(r, x.id, (F)1 / y.v, y.id, -r / y.v, tape_of(x, y));
+}
+template <class F>
+MX_HD Ad<F> operator-(Ad<F> x)
+{
+	return ad_make<F>(-x.v, x.id, (F)-1, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator+(Ad<F> x, F c)
+{
+	return ad_make<F>(x.v + c, x.id, (F)1, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator+(F c, Ad<F> x)
+{
+	return ad_make<F>(c + x.v, x.id, (F)1, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator-(Ad<F> x, F c)
+{
+	return ad_make<F>(x.v - c, x.id, (F)1, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator-(F c, Ad<F> x)
+{
+	return ad_make<F>(c - x.v, x.id, (F)-1, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator*(Ad<F> x, F c)
+{
+	return ad_make<F>(x.v * c, x.id, c, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator*(F c, Ad<F> x)
+{
+	return ad_make<F>(c * x.v, x.id, c, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator/(Ad<F> x, F c)
+{
+	return ad_make<F>(x.v / c, x.id, (F)1 / c, kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> operator/(F c, Ad<F> x)
+{
+	F r = c / x.v;
+	return ad_make<F>(r, x.id, -r / x.v, kNoNode, (F)0, x.t);
+}
+
+// value of a scalar of any kind
+MX_HD float val(float x) { return x; }
+MX_HD double val(double x) { return x; }
+template <class F>
+MX_HD F val(const Ad<F>& x)
+{
+	return x.v;
+}
+
+// constants of the scalar's precision
+template <class T>
+struct Num;
+template <>
+struct Num<float>
+{
+	typedef float F;
+	static MX_HD float c(double x) { return (float)x; }
+};
+template <>
+struct Num<double>
+{
+	typedef double F;
+	static MX_HD double c(double x) { return x; }
+};
+template <class G>
+struct Num<Ad<G>>
+{
+	typedef G F;
+	static MX_HD Ad<G> c(double x) { return ad_const<G>((G)x); }
+};
+
+// elementary functions; float keeps the engine's correctly rounded forms
+MX_HD float sqrt_(float x) { return sqrtf(x); }
+MX_HD double sqrt_(double x) { return sqrt(x); }
+template <class F>
+MX_HD Ad<F> sqrt_(Ad<F> x)
+{
+	F r = sqrt_(x.v);
+	return ad_make<F>(r, x.id, r > (F)0 ? (F)0.5 / r : (F)0, kNoNode, (F)0, x.t);
+}
+MX_HD float sin_(float x) { return (float)sin((double)x); }
+MX_HD double sin_(double x) { return sin(x); }
+MX_HD float cos_(float x) { return (float)cos((double)x); }
+MX_HD double cos_(double x) { return cos(x); }
+template <class F>
+MX_HD Ad<F> sin_(Ad<F> x)
+{
+	return ad_make<F>(sin_(x.v), x.id, cos_(x.v), kNoNode, (F)0, x.t);
+}
+template <class F>
+MX_HD Ad<F> cos_(Ad<F> x)
+{
+	return ad_make<F>(cos_(x.v), x.id, -sin_(x.v), kNoNode, (F)0, x.t);
+}
+MX_HD float atan2_(float y, float x) { return (float)atan2((double)y, (double)x); }
+MX_HD double atan2_(double y, double x) { return atan2(y, x); }
+template <class F>
+MX_HD Ad<F> atan2_(Ad<F> y, Ad<F> x)
+{
+	F r2 = x.v * x.v + y.v * y.v;
+	F dy = r2 > (F)0 ? x.v / r2 : (F)0;
+	F dx = r2 > (F)0 ? -y.v / r2 : (F)0;
+	return ad_make<F>(atan2_(y.v, x.v), y.id, dy, x.id, dx, tape_of(y, x));
+}
+MX_HD float fabs_(float x) { return fabsf(x); }
+MX_HD double fabs_(double x) { return fabs(x); }
+template <class F>
+MX_HD Ad<F> fabs_(Ad<F> x)
+{
+	return ad_make<F>(x.v < (F)0 ? -x.v : x.v, x.id, x.v < (F)0 ? (F)-1 : (F)1, kNoNode, (F)0, x.t);
+}
+
+// ------------------------------------------------------------ vectors
+
+template <class T>
+struct V3
+{
+	T x, y, z;
+};
+template <class T>
+struct Q4
+{
+	T x, y, z, w;
+};
+template <class T>
+struct M3
+{
+	V3<T> cx, cy, cz;
+};
+
+template <class T>
+MX_HD V3<T> v3(T x, T y, T z)
+{
+	V3<T> r = { x, y, z };
+	return r;
+}
+template <class T>
+MX_HD V3<T> add(V3<T> a, V3<T> b)
+{
+	return v3<T>(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <class T>
+MX_HD V3<T> sub(V3<T> a, V3<T> b)
+{
+	return v3<T>(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <class T>
+MX_HD V3<T> neg(V3<T> a)
+{
+	return v3<T>(-a.x, -a.y, -a.z);
+}
+template <class T, class S>
+MX_HD V3<T> scale(S s, V3<T> v)
+{
+	return v3<T>(s * v.x, s * v.y, s * v.z);
+}
+template <class T, class S>
+MX_HD V3<T> mul_add(V3<T> a, S s, V3<T> b)
+{
+	return v3<T>(a.x + s * b.x, a.y + s * b.y, a.z + s * b.z);
+}
+template <class T>
+MX_HD T dot(V3<T> a, V3<T> b)
+{
+	return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class T>
+MX_HD V3<T> cross(V3<T> a, V3<T> b)
+{
+	return v3<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class T>
+MX_HD T length_sq(V3<T> v)
+{
+	return dot(v, v);
+}
+template <class T>
+MX_HD T length(V3<T> v)
+{
+	return sqrt_(dot(v, v));
+}
+template <class T>
+MX_HD Q4<T> quat_mul(Q4<T> a, Q4<T> b)
+{
+	Q4<T> r;
+	r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+	r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+	r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+	r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+	return r;
+}
+template <class T>
+MX_HD Q4<T> quat_conj(Q4<T> q)
+{
+	Q4<T> r = { -q.x, -q.y, -q.z, q.w };
+	return r;
+}
+template <class T>
+MX_HD V3<T> rotate(Q4<T> q, V3<T> v)
+{
+	typedef typename Num<T>::F F;
+	V3<T> u = v3<T>(q.x, q.y, q.z);
+	V3<T> t = scale((F)2, cross(u, v));
+	return add(add(v, scale(q.w, t)), cross(u, t));
+}
+template <class T>
+MX_HD M3<T> quat_to_mat3(Q4<T> q)
+{
+	typedef typename Num<T>::F F;
+	T x = q.x, y = q.y, z = q.z, w = q.w;
+	T x2 = x + x, y2 = y + y, z2 = z + z;
+	T xx = x * x2, xy = x * y2, xz = x * z2;
+	T yy = y * y2, yz = y * z2, zz = z * z2;
+	T wx = w * x2, wy = w * y2, wz = w * z2;
+	M3<T> m;
+	m.cx = v3<T>((F)1 - (yy + zz), xy + wz, xz - wy);
+	m.cy = v3<T>(xy - wz, (F)1 - (xx + zz), yz + wx);
+	m.cz = v3<T>(xz + wy, yz - wx, (F)1 - (xx + yy));
+	return m;
+}
+template <class T>
+MX_HD V3<T> mat3_mulv(const M3<T>& m, V3<T> v)
+{
+	return add(add(scale(v.x, m.cx), scale(v.y, m.cy)), scale(v.z, m.cz));
+}
+
+// src/xpbd.cpp:21-25
+template <class T>
+MX_HD Q4<T> renorm_fast(Q4<T> q)
+{
+	typedef typename Num<T>::F F;
+	T s = (F)1.5 - (F)0.5 * (q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+	Q4<T> r = { q.x * s, q.y * s, q.z * s, q.w * s };
+	return r;
+}
+
+template <class T>
+MX_HD Q4<T> rotate_by(Q4<T> q, V3<T> dw)
+{
+	typedef typename Num<T>::F F;
+	Q4<T> wq = { dw.x, dw.y, dw.z, Num<T>::c(0.0) };
+	Q4<T> dq = quat_mul(wq, q);
+	Q4<T> r = { q.x + (F)0.5 * dq.x, q.y + (F)0.5 * dq.y, q.z + (F)0.5 * dq.z, q.w + (F)0.5 * dq.w };
+	return renorm_fast(r);
+}
+
+// I^-1_world = R diag(d) R^T (src/m3d_internal.h:126-146)
+template <class T>
+MX_HD M3<T> inv_inertia_world(Q4<T> rot, V3<T> d)
+{
+	M3<T> r = quat_to_mat3(rot);
+	M3<T> rd;
+	rd.cx = scale(d.x, r.cx);
+	rd.cy = scale(d.y, r.cy);
+	rd.cz = scale(d.z, r.cz);
+	M3<T> m;
+	m.cx = v3<T>(rd.cx.x * r.cx.x + rd.cy.x * r.cy.x + rd.cz.x * r.cz.x,
+				 rd.cx.y * r.cx.x + rd.cy.y * r.cy.x + rd.cz.y * r.cz.x,
+				 rd.cx.z * r.cx.x + rd.cy.z * r.cy.x + rd.cz.z * r.cz.x);
+	m.cy = v3<T>(rd.cx.x * r.cx.y + rd.cy.x * r.cy.y + rd.cz.x * r.cz.y,
+				 rd.cx.y * r.cx.y + rd.cy.y * r.cy.y + rd.cz.y * r.cz.y,
+				 rd.cx.z * r.cx.y + rd.cy.z * r.cy.y + rd.cz.z * r.cz.y);
+	m.cz = v3<T>(rd.cx.x * r.cx.z + rd.cy.x * r.cy.z + rd.cz.x * r.cz.z,
+				 rd.cx.y * r.cx.z + rd.cy.y * r.cy.z + rd.cz.y * r.cz.z,
+				 rd.cx.z * r.cx.z + rd.cy.z * r.cy.z + rd.cz.z * r.cz.z);
+	return m;
+}
+
+// ------------------------------------------------------------ step data
+
+// Inputs of one world's step (plain floats, as recorded; device or host
+// memory). Bodies: the world's range; contacts: its solver contacts in
+// ascending engine order (the serial order of src/world.cpp:2012-2058 per
+// island; islands touch disjoint dynamic bodies, so one ascending pass
+// equals island-by-island).
+struct StepIn
+{
+	int32_t nb;           // bodies
+	const float4* pos;    // xyz, invMass
+	const float4* rot;
+	const float4* vel;    // xyz, gravityScale
+	const float4* angVel; // xyz, mass
+	const float4* invIL;  // xyz, linearDamping
+	const float* angDamp;
+	const float4* shape;  // halfExtents xyz, radius
+	const float4* bmat;   // halfHeight, friction, restitution, rollingResistance
+	const uint32_t* flags; // BF_* bits
+	int32_t nc;           // solver contacts
+	const int2* bodies;   // local body indices, a < b
+	const float4* normal; // xyz, pointCount bits
+	const float4* cmat;   // friction, restitution, rolling, CF_* bits
+	// per contact, 4 point slots each, [c * 4 + j]
+	const float4* pA;     // localAnchorA, separationOffset
+	const float4* pB;     // localAnchorB
+	const float4* pFA;    // friction anchors at step start
+	const float4* pFB;
+	const float4* pSlip;  // slip direction
+	// the manifold fields above hold one block per substep (the manifold
+	// each substep's prepare left, re-collided or not): substep s reads the
+	// block at s * cStride contacts
+	int64_t cStride;
+	float dt;
+	int substeps;
+	m3d_vec3 gravity;
+};
+
+// Mutable replay state of one world (scratch of the replaying lane).
+template <class T>
+struct StepState
+{
+	V3<T>* x;
+	Q4<T>* q;
+	V3<T>* v;
+	V3<T>* w;
+	V3<T>* px; // previous substep
+	Q4<T>* pq;
+	V3<T>* pv;
+	V3<T>* pw;
+	M3<T>* iw;
+	T* invMass;
+	V3<T>* il;   // inverse local inertia
+	T* mu;       // per contact: combined friction
+	// per point [c * 4 + j]
+	T* lambdaN;
+	T* lambdaT;
+	V3<T>* rA;
+	V3<T>* rB;
+	V3<T>* jA;
+	V3<T>* jB;
+	T* wN;
+	float4* fA; // friction anchors (no derivative: re-anchoring copies constants)
+	float4* fB;
+	float4* slip;
+};
+
+MX_HD V3<float> f3(float4 v) { return v3<float>(v.x, v.y, v.z); }
+
+template <class T>
+MX_HD V3<T> lift(float4 v)
+{
+	return v3<T>(Num<T>::c(v.x), Num<T>::c(v.y), Num<T>::c(v.z));
+}
+
+MX_HD bool is_dynamic(const StepIn& in, int32_t i) { return bf_type(in.flags[i]) == M3D_BODY_DYNAMIC; }
+
+// ------------------------------------------------------------ stages
+
+// IntegrateBodySubstep (src/xpbd.cpp:587-705) for a body without forces,
+// torques or gyroscopic term (isotropic inverse inertia); mx_solver.h
+// integrate_body.
+template <class T>
+MX_HD void integrate(const StepIn& in, StepState<T>& S, int32_t i, float h)
+{
+	typedef typename Num<T>::F F;
+	if (!is_dynamic(in, i))
+		return;
+	float gravityScale = in.vel[i].w;
+	float linDamp = in.invIL[i].w;
+	float angDamp = in.angDamp[i];
+	S.pv[i] = S.v[i];
+	S.pw[i] = S.w[i];
+	V3<T> v = mul_add(S.v[i], (F)(h * gravityScale), v3<T>(Num<T>::c(in.gravity.x), Num<T>::c(in.gravity.y),
+															 Num<T>::c(in.gravity.z)));
+	v = scale((F)(1.0f / (1.0f + h * linDamp)), v);
+	V3<T> w = scale((F)(1.0f / (1.0f + h * angDamp)), S.w[i]);
+	S.px[i] = S.x[i];
+	S.pq[i] = S.q[i];
+	V3<T> x = mul_add(S.x[i], (F)h, v);
+	Q4<T> rot = S.q[i];
+	T wlen = length(w);
+	if (val(wlen) > (F)1e-9)
+	{
+		T half = (F)(0.5f * h) * wlen;
+		T sh, ch;
+		if (val(half) < (F)0.02)
+		{
+			T h2 = half * half;
+			sh = half * ((F)1 - h2 * (F)(1.0f / 6.0f));
+			ch = (F)1 - h2 * (F)0.5;
+		}
+		else
+		{
+			sh = sin_(half);
+			ch = cos_(half);
+		}
+		T s = sh / wlen;
+		Q4<T> dq = { w.x * s, w.y * s, w.z * s, ch };
+		rot = renorm_fast(quat_mul(dq, rot));
+	}
+	S.iw[i] = inv_inertia_world(rot, S.il[i]);
+	S.x[i] = x;
+	S.q[i] = rot;
+	S.v[i] = v;
+	S.w[i] = w;
+}
+
+// contact arm of src/xpbd.cpp:100-116 (mx_solver.h contact_arm)
+template <class T>
+MX_HD V3<T> contact_arm(int shapeType, float radius, float halfHeight, Q4<T> rot, float4 local, V3<T> n, float sign)
+{
+	typedef typename Num<T>::F F;
+	if (shapeType == M3D_SHAPE_SPHERE)
+		return scale((F)(sign * radius), n);
+	if (shapeType == M3D_SHAPE_CAPSULE)
+	{
+		float ay = local.y < -halfHeight ? -halfHeight : (local.y > halfHeight ? halfHeight : local.y);
+		V3<T> axis = rotate(rot, v3<T>(Num<T>::c(0.0), Num<T>::c(1.0), Num<T>::c(0.0)));
+		return add(scale((F)ay, axis), scale((F)(sign * radius), n));
+	}
+	return rotate(rot, lift<T>(local));
+}
+
+template <class T>
+struct Body
+{
+	V3<T> p;
+	Q4<T> q;
+	V3<T> v;
+	V3<T> w;
+	T invMass;
+	M3<T> iw;
+};
+
+template <class T>
+MX_HD Body<T> load_body(const StepState<T>& S, int32_t i)
+{
+	Body<T> b;
+	b.p = S.x[i];
+	b.q = S.q[i];
+	b.v = S.v[i];
+	b.w = S.w[i];
+	b.invMass = S.invMass[i];
+	b.iw = S.iw[i];
+	return b;
+}
+
+template <class T>
+MX_HD void store_body(StepState<T>& S, int32_t i, const Body<T>& b, const StepIn& in)
+{
+	if (!is_dynamic(in, i))
+		return; // static and kinematic bodies keep their state
+	S.x[i] = b.p;
+	S.q[i] = b.q;
+	S.v[i] = b.v;
+	S.w[i] = b.w;
+}
+
+// freeze arms, Jacobians and normal mass (src/xpbd.cpp:845-861); the
+// manifold is the recorded one (no re-collision)
+template <class T>
+MX_HD void prepare(const StepIn& in, StepState<T>& S, int32_t c)
+{
+	int2 ab = in.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	float4 nc = in.normal[c];
+	V3<T> n = lift<T>(nc);
+	int32_t count = f2i(nc.w);
+	T imA = S.invMass[ia], imB = S.invMass[ib];
+	float4 SA = in.shape[ia], SB = in.shape[ib];
+	int tA = bf_shape(in.flags[ia]), tB = bf_shape(in.flags[ib]);
+	float hhA = in.bmat[ia].x, hhB = in.bmat[ib].x;
+	for (int j = 0; j < count; ++j)
+	{
+		int32_t p = c * 4 + j;
+		V3<T> rA = contact_arm(tA, SA.w, hhA, S.q[ia], in.pA[p], n, 1.0f);
+		V3<T> rB = contact_arm(tB, SB.w, hhB, S.q[ib], in.pB[p], n, -1.0f);
+		V3<T> crA = cross(rA, n);
+		V3<T> crB = cross(rB, n);
+		V3<T> jA = mat3_mulv(S.iw[ia], crA);
+		V3<T> jB = mat3_mulv(S.iw[ib], crB);
+		S.wN[p] = imA + imB + dot(crA, jA) + dot(crB, jB);
+		S.jA[p] = jA;
+		S.jB[p] = jB;
+		S.rA[p] = rA;
+		S.rB[p] = rB;
+		S.lambdaN[p] = Num<T>::c(0.0);
+		S.lambdaT[p] = Num<T>::c(0.0);
+	}
+}
+
+// src/xpbd.cpp:45-58
+template <class T>
+MX_HD void axis_correction(V3<T>& pos, Q4<T>& rot, T invMass, T dl, V3<T> n, V3<T> angJ)
+{
+	typedef typename Num<T>::F F;
+	if (val(invMass) == (F)0)
+		return;
+	pos = mul_add(pos, dl * invMass, n);
+	rot = rotate_by(rot, scale(dl, angJ));
+}
+
+// SolveContactNormals, serial form (src/xpbd.cpp:122-163)
+template <class T>
+MX_HD void normals(const StepIn& in, StepState<T>& S, int32_t c, float maxCorrection)
+{
+	typedef typename Num<T>::F F;
+	int2 ab = in.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	float4 nc = in.normal[c];
+	V3<T> n = lift<T>(nc);
+	int32_t count = f2i(nc.w);
+	V3<T> pa = S.x[ia], pb = S.x[ib];
+	Q4<T> qa = S.q[ia], qb = S.q[ib];
+	T imA = S.invMass[ia], imB = S.invMass[ib];
+	for (int j = 0; j < count; ++j)
+	{
+		int32_t p = c * 4 + j;
+		T wN = S.wN[p];
+		if (!(val(wN) > (F)0))
+			continue;
+		float4 a4 = in.pA[p];
+		V3<T> rA = rotate(qa, lift<T>(a4));
+		V3<T> rB = rotate(qb, lift<T>(in.pB[p]));
+		T sep = Num<T>::c(a4.w) + dot(sub(add(pb, rB), add(pa, rA)), n);
+		if (!(val(sep) < (F)0))
+			continue;
+		if (val(sep) < (F)(-maxCorrection))
+			sep = Num<T>::c(-maxCorrection);
+		T dl = -sep / wN;
+		S.lambdaN[p] = S.lambdaN[p] + dl;
+		axis_correction(pb, qb, imB, dl, n, S.jB[p]);
+		axis_correction(pa, qa, imA, -dl, n, S.jA[p]);
+	}
+	if (is_dynamic(in, ia))
+	{
+		S.x[ia] = pa;
+		S.q[ia] = qa;
+	}
+	if (is_dynamic(in, ib))
+	{
+		S.x[ib] = pb;
+		S.q[ib] = qb;
+	}
+}
+
+// src/xpbd.cpp:85-89
+template <class T>
+MX_HD T generalized_inv_mass(const Body<T>& b, V3<T> r, V3<T> axis)
+{
+	V3<T> rn = cross(r, axis);
+	return b.invMass + dot(rn, mat3_mulv(b.iw, rn));
+}
+
+// src/xpbd.cpp:28-41
+template <class T>
+MX_HD void position_correction(Body<T>& b, V3<T> p, V3<T> r)
+{
+	typedef typename Num<T>::F F;
+	if (val(b.invMass) == (F)0)
+		return;
+	b.p = mul_add(b.p, b.invMass, p);
+	V3<T> dw = mat3_mulv(b.iw, cross(r, p));
+	b.q = rotate_by(b.q, dw);
+}
+
+// ApplyContactFriction (src/xpbd.cpp:172-269), box-box pairs only
+template <class T>
+MX_HD void friction(const StepIn& in, StepState<T>& S, int32_t c)
+{
+	typedef typename Num<T>::F F;
+	int2 ab = in.bodies[c];
+	float4 nc = in.normal[c];
+	int32_t count = f2i(nc.w);
+	if (count == 0)
+		return;
+	Body<T> a = load_body(S, ab.x), b = load_body(S, ab.y);
+	V3<T> n = lift<T>(nc);
+	T mu = S.mu[c];
+	{
+		const F maxSpeed2 = (F)(0.1f * 0.1f);
+		V3<T> rA0 = S.rA[c * 4], rB0 = S.rB[c * 4];
+		V3<T> vrel = sub(add(b.v, cross(b.w, rB0)), add(a.v, cross(a.w, rA0)));
+		V3<T> vt = sub(vrel, scale(dot(vrel, n), n));
+		if (val(length_sq(vt)) > maxSpeed2)
+		{
+			for (int j = 0; j < count; ++j)
+			{
+				S.fA[c * 4 + j] = in.pA[c * 4 + j];
+				S.fB[c * 4 + j] = in.pB[c * 4 + j];
+			}
+			return;
+		}
+	}
+	for (int j = 0; j < count; ++j)
+	{
+		int32_t p = c * 4 + j;
+		T lambdaN = S.lambdaN[p];
+		if (val(lambdaN) <= (F)0)
+			continue;
+		V3<T> rA = rotate(a.q, lift<T>(S.fA[p]));
+		V3<T> rB = rotate(b.q, lift<T>(S.fB[p]));
+		V3<T> gap = sub(add(b.p, rB), add(a.p, rA));
+		V3<T> dpt = sub(gap, scale(dot(gap, n), n));
+		T lt = length(dpt);
+		if (val(lt) <= (F)1e-9)
+			continue;
+		V3<T> t = scale((F)1 / lt, dpt);
+		T wt = generalized_inv_mass(a, rA, t) + generalized_inv_mass(b, rB, t);
+		if (val(wt) <= (F)0)
+			continue;
+		T needed = (F)kFrictionRelaxation * (lt / wt);
+		T lambdaT = S.lambdaT[p];
+		T allowed = mu * lambdaN - lambdaT;
+		if (val(allowed) <= (F)0)
+			continue;
+		T dL = val(needed) < val(allowed) ? needed : allowed;
+		S.lambdaT[p] = lambdaT + dL;
+		V3<T> pt = scale(dL, t);
+		position_correction(b, neg(pt), rB);
+		position_correction(a, pt, rA);
+		if (val(needed) > (F)2 * val(allowed))
+		{
+			S.fA[p] = in.pA[p];
+			S.fB[p] = in.pB[p];
+		}
+	}
+	store_body(S, ab.x, a, in);
+	store_body(S, ab.y, b, in);
+}
+
+// ReconstructBodyVelocity (src/xpbd.cpp:865-892)
+template <class T>
+MX_HD void reconstruct(const StepIn& in, StepState<T>& S, int32_t i, float invH)
+{
+	typedef typename Num<T>::F F;
+	if (!is_dynamic(in, i))
+		return;
+	V3<T> v = scale((F)invH, sub(S.x[i], S.px[i]));
+	Q4<T> dq = quat_mul(S.q[i], quat_conj(S.pq[i]));
+	F sign = val(dq.w) >= (F)0 ? (F)1 : (F)-1;
+	T v2 = dq.x * dq.x + dq.y * dq.y + dq.z * dq.z;
+	V3<T> w;
+	if (val(v2) > (F)4e-4)
+	{
+		T vlen = sqrt_(v2);
+		T angle = (F)2 * atan2_(vlen, fabs_(dq.w));
+		w = scale(sign * (F)invH * angle / vlen, v3<T>(dq.x, dq.y, dq.z));
+	}
+	else
+	{
+		T corr = (F)1 + v2 * (F)(1.0f / 6.0f);
+		w = scale((F)2 * (F)invH * sign * corr, v3<T>(dq.x, dq.y, dq.z));
+	}
+	S.v[i] = v;
+	S.w[i] = w;
+}
+
+// src/xpbd.cpp:61-82
+template <class T>
+MX_HD void axis_velocity(Body<T>& b, T j, V3<T> n, V3<T> angJ)
+{
+	typedef typename Num<T>::F F;
+	if (val(b.invMass) == (F)0)
+		return;
+	b.v = mul_add(b.v, j * b.invMass, n);
+	b.w = mul_add(b.w, j, angJ);
+}
+
+template <class T>
+MX_HD void velocity_impulse(Body<T>& b, V3<T> p, V3<T> r)
+{
+	typedef typename Num<T>::F F;
+	if (val(b.invMass) == (F)0)
+		return;
+	b.v = mul_add(b.v, b.invMass, p);
+	b.w = add(b.w, mat3_mulv(b.iw, cross(r, p)));
+}
+
+// SolveContactVelocity (src/xpbd.cpp:275-407): dynamic friction, restitution
+// (applied unconditionally), rolling resistance
+template <class T>
+MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float gravityLen)
+{
+	typedef typename Num<T>::F F;
+	int2 ab = in.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	float4 nc = in.normal[c], mat = in.cmat[c];
+	V3<T> n = lift<T>(nc);
+	int32_t count = f2i(nc.w);
+	T mu = S.mu[c];
+	float e0 = mat.y, rr = mat.z;
+	bool curved = (f2u(mat.w) & CF_CURVED) != 0;
+	float invH = h > 0.0f ? 1.0f / h : 0.0f;
+	Body<T> a = load_body(S, ia), b = load_body(S, ib);
+	V3<T> pvA = S.pv[ia], pwA = S.pw[ia], pvB = S.pv[ib], pwB = S.pw[ib];
+	float4 SA = in.shape[ia], SB = in.shape[ib];
+	int tA = bf_shape(in.flags[ia]), tB = bf_shape(in.flags[ib]);
+	float hhA = in.bmat[ia].x, hhB = in.bmat[ib].x;
+	T lambdaNTotal = Num<T>::c(0.0);
+	for (int j = 0; j < count; ++j)
+	{
+		int32_t p = c * 4 + j;
+		T lambdaN = S.lambdaN[p];
+		if (val(lambdaN) <= (F)0)
+			continue;
+		V3<T> rAf, rBf;
+		if (curved)
+		{
+			rAf = contact_arm(tA, SA.w, hhA, a.q, in.pA[p], n, 1.0f);
+			rBf = contact_arm(tB, SB.w, hhB, b.q, in.pB[p], n, -1.0f);
+		}
+		else
+		{
+			rAf = S.rA[p];
+			rBf = S.rB[p];
+		}
+		V3<T> vrel = sub(add(b.v, cross(b.w, rBf)), add(a.v, cross(a.w, rAf)));
+		T vn = dot(vrel, n);
+		V3<T> vt = sub(vrel, scale(vn, n));
+		T vtLen = length(vt);
+		if (val(vtLen) > (F)1e-9)
+		{
+			V3<T> t = scale((F)1 / vtLen, vt);
+			T wt = generalized_inv_mass(a, rAf, t) + generalized_inv_mass(b, rBf, t);
+			if (val(wt) > (F)0)
+			{
+				const F staticSlip = (F)0.08;
+				float4 sl = S.slip[p];
+				bool coherent = val(t.x) * (F)sl.x + val(t.y) * (F)sl.y + val(t.z) * (F)sl.z > (F)0.7;
+				T pStop = vtLen / wt;
+				T pMax = mu * lambdaN * (F)invH;
+				T P = !coherent && val(vtLen) < staticSlip ? pStop : (val(pStop) < val(pMax) ? pStop : pMax);
+				S.slip[p] = make_float4((float)val(t.x), (float)val(t.y), (float)val(t.z), 0.0f);
+				V3<T> imp = scale(-P, t);
+				velocity_impulse(b, imp, rBf);
+				velocity_impulse(a, neg(imp), rAf);
+			}
+		}
+		V3<T> vrelPrev = sub(add(pvB, cross(pwB, rBf)), add(pvA, cross(pwA, rAf)));
+		T vnPrev = dot(vrelPrev, n);
+		F e = val(fabs_(vnPrev)) > (F)(2.0f * gravityLen * h) ? (F)e0 : (F)0;
+		vrel = sub(add(b.v, cross(b.w, rBf)), add(a.v, cross(a.w, rAf)));
+		vn = dot(vrel, n);
+		T target = -e * vnPrev;
+		target = val(target) > (F)0 ? target : Num<T>::c(0.0);
+		T wN = S.wN[p];
+		if (val(wN) > (F)0)
+		{
+			T jj = (target - vn) / wN;
+			axis_velocity(b, jj, n, S.jB[p]);
+			axis_velocity(a, -jj, n, S.jA[p]);
+		}
+		lambdaNTotal = lambdaNTotal + (val(lambdaN) > (F)0 ? lambdaN : Num<T>::c(0.0));
+	}
+	if (rr > 0.0f && val(lambdaNTotal) > (F)0)
+	{
+		V3<T> wrel = sub(b.w, a.w);
+		V3<T> wroll = sub(wrel, scale(dot(wrel, n), n));
+		T wl = length(wroll);
+		if (val(wl) > (F)1e-6)
+		{
+			V3<T> axis = scale((F)1 / wl, wroll);
+			T k = dot(axis, mat3_mulv(a.iw, axis)) + dot(axis, mat3_mulv(b.iw, axis));
+			if (val(k) > (F)1e-12)
+			{
+				T maxDw = (F)rr * lambdaNTotal * (F)invH * k;
+				T dw = val(maxDw) < val(wl) ? maxDw : wl;
+				V3<T> angImp = scale(dw / k, axis);
+				b.w = sub(b.w, mat3_mulv(b.iw, angImp));
+				a.w = add(a.w, mat3_mulv(a.iw, angImp));
+			}
+		}
+	}
+	store_body(S, ia, a, in);
+	store_body(S, ib, b, in);
+}
+
+// One substep (src/xpbd.cpp:898-974) with the manifold block of `in`:
+// integrate, prepare, iterations x normals, stiction, reconstruct, velocity.
+template <class T>
+MX_HD void solve_substep(const StepIn& in, StepState<T>& S, int iterations, float h, float invH, float maxCorrection,
+						 float gravityLen)
+{
+	for (int32_t c = 0; c < in.nc; ++c)
+		for (int j = 0; j < 4; ++j)
+		{
+			S.fA[c * 4 + j] = in.pFA[c * 4 + j];
+			S.fB[c * 4 + j] = in.pFB[c * 4 + j];
+			S.slip[c * 4 + j] = in.pSlip[c * 4 + j];
+		}
+	for (int32_t i = 0; i < in.nb; ++i)
+		integrate(in, S, i, h);
+	for (int32_t c = 0; c < in.nc; ++c)
+		prepare(in, S, c);
+	for (int it = 0; it < iterations; ++it)
+		for (int32_t c = 0; c < in.nc; ++c)
+			normals(in, S, c, maxCorrection);
+	for (int32_t c = 0; c < in.nc; ++c)
+	{
+		int2 ab = in.bodies[c];
+		if (bf_shape(in.flags[ab.x]) == M3D_SHAPE_BOX && bf_shape(in.flags[ab.y]) == M3D_SHAPE_BOX)
+			friction(in, S, c);
+	}
+	for (int32_t i = 0; i < in.nb; ++i)
+		reconstruct(in, S, i, invH);
+	for (int32_t c = 0; c < in.nc; ++c)
+		velocity(in, S, c, h, gravityLen);
+}
+
+// The step's solve: its substeps, each with its recorded manifold block.
+template <class T>
+MX_HD void solve_step(const StepIn& in, StepState<T>& S, int iterations)
+{
+	float h = in.dt / (float)in.substeps;
+	float invH = 1.0f / h;
+	float maxCorrection = kMaxDepenetrationSpeed * h;
+	float gravityLen = m3d_length(in.gravity);
+	for (int s = 0; s < in.substeps; ++s)
+	{
+		StepIn ins = in;
+		ins.normal = in.normal + s * in.cStride;
+		ins.cmat = in.cmat + s * in.cStride;
+		ins.pA = in.pA + 4 * s * in.cStride;
+		ins.pB = in.pB + 4 * s * in.cStride;
+		ins.pFA = in.pFA + 4 * s * in.cStride;
+		ins.pFB = in.pFB + 4 * s * in.cStride;
+		ins.pSlip = in.pSlip + 4 * s * in.cStride;
+		solve_substep(ins, S, iterations, h, invH, maxCorrection, gravityLen);
+	}
+}
+
+} // namespace dd
+} // namespace mx

@@ -51,6 +51,7 @@ enum Counter : int
 	CT_SUB_REBUILDS,  // batched: sub-worlds rebuilding their stable tier
 	CT_KEPT,          // batched: stable entries kept from the last build
 	CT_PLACED,        // batched: contacts after placement
+	CT_DIFF_CONTACTS, // recorded solver contacts
 	CT_COUNT_
 };
 
@@ -109,6 +110,12 @@ bool stage_new_contacts(World& w, const uint64_t* keys, int32_t K);
 // separated-pair removal and new-contact placement per sub-world, each sub-world
 // as src/world.cpp:1705-1735 and :1540-1568 would lay out a world of its own
 bool remove_and_place(World& w);
+
+// ---- k_diff.hip ----
+// records the step's solve inputs when differentiable recording is on
+bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps);
+// records the manifold each substep's prepare left
+bool diff_record_substep(World& w, int sub);
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);

@@ -131,6 +131,8 @@ struct ContactColumns
 };
 
 struct StepCounters; // device-side counters block (k_step.hip)
+struct DiffState;    // recorded steps of a differentiable world (k_diff.hip)
+void diff_release(struct World& w);
 
 struct World
 {
@@ -263,6 +265,8 @@ struct World
 	std::vector<uint32_t> subStableHost;
 	ContactColumns cc2;                 // placement target, swapped with cc
 	int32_t contactCap2 = 0;
+
+	DiffState* diff = nullptr;          // recorded steps for the adjoint
 
 	// ---- kernel timing (off unless enabled) ----
 	struct KernelMark

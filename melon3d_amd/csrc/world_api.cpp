@@ -39,6 +39,7 @@ World::World() {}
 
 World::~World()
 {
+	diff_release(*this);
 	BodyColumns& b = bc;
 	DBuf<float4>* f4[] = { &b.pos, &b.rot, &b.vel, &b.angVel, &b.prevPos, &b.prevRot, &b.prevVel, &b.prevAngVel,
 						   &b.invIL, &b.iw0, &b.iw1, &b.iw2, &b.shape, &b.mat, &b.force, &b.torque, &b.fatLo, &b.fatHi,

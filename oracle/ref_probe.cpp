@@ -539,6 +539,13 @@ int32_t m3dx_batch_size(m3d_world*) { return 0; }
 int m3dx_batch_body_range(m3d_world*, int32_t, int32_t*, int32_t*) { return -1; }
 int m3dx_batch_contact_counts(m3d_world*, int32_t*, int32_t) { return 0; }
 
+/* the adjoint replays recorded device steps: not available on the CPU engine */
+int m3dx_diff_record(m3d_world*, int) { return -1; }
+int32_t m3dx_diff_steps(m3d_world*) { return 0; }
+int m3dx_diff_state(m3d_world*, int, float*) { return -1; }
+int m3dx_diff_replay(m3d_world*, int, int, int, const void*, const void*, const void*, void*) { return -1; }
+int m3dx_diff_backward(m3d_world*, const float*, float*, float*, float*) { return -1; }
+
 /* no device kernels on the CPU engine: timing is unavailable */
 int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
 
