@@ -1348,14 +1348,32 @@ __global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const 
 	C.mat[c] = m;
 }
 
-__global__ void k_pack_sidecar(const int2* bodies, const float4* mat, int32_t C, int4* out)
+__global__ void k_pack_sidecar(const int2* bodies, const float4* mat, int32_t C, int4* out, uint32_t* perBody)
 {
 	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (c < C)
 	{
 		int2 b = bodies[c];
-		out[c] = make_int4(b.x, b.y, (int32_t)f2u(mat[c].w), 0);
+		out[c] = make_int4(b.x, b.y, (int32_t)f2u(mat[c].w), c);
+		atomicAdd(&perBody[b.x], 1u);
+		atomicAdd(&perBody[b.y], 1u);
 	}
+}
+
+// per-body contact index: list[start[b] ..] = positions of body b's
+// contacts (any order), slot[c] = the two entries of contact c
+__global__ void k_contact_csr(const int2* bodies, int32_t C, const uint32_t* start, uint32_t* cursor, int32_t* list,
+							  int2* slot)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int2 b = bodies[c];
+	int32_t sa = (int32_t)(start[b.x] + atomicAdd(&cursor[b.x], 1u));
+	int32_t sb = (int32_t)(start[b.y] + atomicAdd(&cursor[b.y], 1u));
+	list[sa] = c;
+	list[sb] = c;
+	slot[c] = make_int2(sa, sb);
 }
 
 bool prefetch_sidecar(World& w)
@@ -1377,17 +1395,44 @@ bool prefetch_sidecar(World& w)
 		}
 		w.pinnedSidecarCap = cap;
 	}
+	int32_t n = w.deviceBodyCount;
+	size_t csrInts = (size_t)n + 1 + 4 * (size_t)C; // start, list (2C), slot (2C)
+	if (w.pinnedCsrCap < csrInts)
+	{
+		if (w.pinnedCsr != nullptr)
+			(void)hipHostFree(w.pinnedCsr);
+		w.pinnedCsr = nullptr;
+		size_t cap = csrInts + csrInts / 2 + 1024;
+		if (hipHostMalloc((void**)&w.pinnedCsr, cap * sizeof(int32_t), 0) != hipSuccess)
+		{
+			w.pinnedCsrCap = 0;
+			return false;
+		}
+		w.pinnedCsrCap = cap;
+	}
 	if (w.sidecarReady == nullptr && hipEventCreateWithFlags(&w.sidecarReady, hipEventDisableTiming) != hipSuccess)
 		return false;
-	if (!w.dSidecar.ensure(C))
+	// device layout: start (n + 1), list, slot, then per-body counts (n + 1)
+	if (!w.dSidecar.ensure(C) || !w.dCsr.ensure(csrInts + (size_t)n + 1) || !ensure_prim_scratch(w, (int64_t)n + 1))
 		return false;
-	k_pack_sidecar<<<grid_for(C), kBlock, 0, w.stream>>>(w.cc.bodies.p, w.cc.mat.p, C, w.dSidecar.p);
-	if (hipMemcpyAsync(w.pinnedSidecar, w.dSidecar.p, (size_t)C * sizeof(int4), hipMemcpyDeviceToHost, w.stream) !=
+	hipStream_t s = w.stream;
+	uint32_t* start = (uint32_t*)w.dCsr.p;
+	int32_t* list = w.dCsr.p + n + 1;
+	int2* slot = (int2*)(list + 2 * (size_t)C);
+	uint32_t* counts = (uint32_t*)(w.dCsr.p + csrInts);
+	launch_fill_u32(counts, 0u, (int64_t)n + 1, s);
+	k_pack_sidecar<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, C, w.dSidecar.p, counts);
+	scan_exclusive(counts, start, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
+	launch_fill_u32(counts, 0u, (int64_t)n + 1, s);
+	k_contact_csr<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, C, start, counts, list, slot);
+	if (hipMemcpyAsync(w.pinnedSidecar, w.dSidecar.p, (size_t)C * sizeof(int4), hipMemcpyDeviceToHost, s) !=
 			hipSuccess ||
-		hipEventRecord(w.sidecarReady, w.stream) != hipSuccess)
+		hipMemcpyAsync(w.pinnedCsr, w.dCsr.p, csrInts * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		hipEventRecord(w.sidecarReady, s) != hipSuccess)
 		return false;
 	w.sidecarPrefetched = true;
 	w.sidecarPrefetchCount = C;
+	w.sidecarPrefetchN = n;
 	return true;
 }
 

@@ -163,16 +163,23 @@ struct World
 	std::vector<uint8_t> materialRefresh; // bodies whose contacts recombine materials
 	bool anyMaterialRefresh = false;
 
-	// contact sidecar mirror for between-step destroys
+	// Contact sidecar mirror for between-step destroys, in pinned memory the
+	// device fills at the end of a step: per contact (body a, body b, flags,
+	// original slot); per body the current positions of its contacts
+	// (hcList[hcStart[b] .. hcStart[b + 1]), -1 once removed); per contact its
+	// two entries in hcList (hcSlot).
 	bool sidecarValid = false;
-	std::vector<int2> hcBodies;
-	std::vector<uint32_t> hcFlags;
-	std::vector<int32_t> hcOrigin;   // original device slot currently at each index
-	// contacts per body for destroys: hcList[hcStart[b] .. hcStart[b+1]) holds
-	// the current positions of body b's contacts (-1: removed); hcSlot[c]
-	// locates contact c's two entries
-	std::vector<int32_t> hcStart, hcList;
-	std::vector<int2> hcSlot;
+	int4* hcSide = nullptr;
+	int32_t hcCount = 0;        // contacts in the mirror
+	int32_t hcN = 0;            // bodies the per-body index covers
+	int32_t* hcStart = nullptr;
+	int32_t* hcList = nullptr;
+	int2* hcSlot = nullptr;
+	std::vector<int32_t> hcMoved; // positions that received another contact
+	int32_t* pinnedCsr = nullptr;
+	size_t pinnedCsrCap = 0;
+	int32_t sidecarPrefetchN = 0;
+	DBuf<int32_t> dCsr;
 	// bodies to wake on the device at the next step (partners of destroyed
 	// bodies whose host record is not current)
 	// packed sidecar (body a, body b, flags) copied to pinned memory at the

@@ -130,6 +130,9 @@ World::~World()
 		(void)hipEventDestroy(sidecarReady);
 	if (pinnedSidecar != nullptr)
 		(void)hipHostFree(pinnedSidecar);
+	if (pinnedCsr != nullptr)
+		(void)hipHostFree(pinnedCsr);
+	dCsr.release();
 	dSidecar.release();
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
@@ -440,57 +443,23 @@ bool ensure_sidecar(World& w)
 	if (w.sidecarValid)
 		return true;
 	int32_t C = w.contactCount;
-	w.hcBodies.resize(C);
-	w.hcFlags.resize(C);
-	w.hcOrigin.resize(C);
+	int32_t n = w.deviceBodyCount;
+	w.hcCount = C;
+	w.hcN = 0;
+	w.hcMoved.clear();
 	if (C > 0)
 	{
-		if (!(w.sidecarPrefetched && w.sidecarPrefetchCount == C) && !prefetch_sidecar(w))
+		if (!(w.sidecarPrefetched && w.sidecarPrefetchCount == C && w.sidecarPrefetchN == n) && !prefetch_sidecar(w))
 			return false;
 		if (!hip_ok(hipEventSynchronize(w.sidecarReady), "sidecar"))
 			return false;
-		const int4* p = w.pinnedSidecar;
-		for (int32_t c = 0; c < C; ++c)
-		{
-			w.hcBodies[c] = make_int2(p[c].x, p[c].y);
-			w.hcFlags[c] = (uint32_t)p[c].z;
-		}
+		w.hcSide = w.pinnedSidecar;
+		w.hcN = n;
+		w.hcStart = w.pinnedCsr;
+		w.hcList = w.pinnedCsr + n + 1;
+		w.hcSlot = (int2*)(w.hcList + 2 * (size_t)C);
 	}
 	w.sidecarPrefetched = false;
-	for (int32_t c = 0; c < C; ++c)
-		w.hcOrigin[c] = c;
-	// per-body index of contact positions (counting sort by body)
-	int32_t n = (int32_t)w.bodies.size();
-	w.hcStart.assign((size_t)n + 1, 0);
-	for (int32_t c = 0; c < C; ++c)
-	{
-		int2 b = w.hcBodies[c];
-		if (b.x >= 0 && b.x < n)
-			w.hcStart[b.x + 1]++;
-		if (b.y >= 0 && b.y < n)
-			w.hcStart[b.y + 1]++;
-	}
-	for (int32_t i = 0; i < n; ++i)
-		w.hcStart[i + 1] += w.hcStart[i];
-	w.hcList.assign((size_t)w.hcStart[n], -1);
-	w.hcSlot.assign(C, make_int2(-1, -1));
-	std::vector<int32_t> fill(w.hcStart.begin(), w.hcStart.end() - 1);
-	for (int32_t c = 0; c < C; ++c)
-	{
-		int2 b = w.hcBodies[c];
-		int2 sl = make_int2(-1, -1);
-		if (b.x >= 0 && b.x < n)
-		{
-			sl.x = fill[b.x]++;
-			w.hcList[sl.x] = c;
-		}
-		if (b.y >= 0 && b.y < n)
-		{
-			sl.y = fill[b.y]++;
-			w.hcList[sl.y] = c;
-		}
-		w.hcSlot[c] = sl;
-	}
 	w.sidecarValid = true;
 	w.pendingPermutation = false;
 	return true;
@@ -500,11 +469,18 @@ bool apply_pending_permutation(World& w)
 {
 	if (!w.pendingPermutation)
 		return true;
-	int32_t newC = (int32_t)w.hcOrigin.size();
+	int32_t newC = w.hcCount;
 	std::vector<int2> mv;
-	for (int32_t i = 0; i < newC; ++i)
-		if (w.hcOrigin[i] != i)
-			mv.push_back(make_int2(i, w.hcOrigin[i]));
+	for (int32_t i : w.hcMoved)
+	{
+		// the contact now at i came from its original slot (always at or
+		// beyond newC, so sources and destinations are disjoint)
+		if (i < newC && w.hcSide[i].w != i)
+		{
+			mv.push_back(make_int2(i, w.hcSide[i].w));
+			w.hcSide[i].w = i;
+		}
+	}
 	if (!mv.empty())
 	{
 		if (!w.moves.ensure(mv.size()))
@@ -515,8 +491,7 @@ bool apply_pending_permutation(World& w)
 		apply_moves(w, w.moves.p, (int32_t)mv.size());
 	}
 	w.contactCount = newC;
-	for (int32_t i = 0; i < newC; ++i)
-		w.hcOrigin[i] = i;
+	w.hcMoved.clear();
 	w.pendingPermutation = false;
 	return hip_ok(hipStreamSynchronize(w.stream), "permutation");
 }
@@ -959,7 +934,7 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 		// only ever swaps in a contact it has already passed, so the body's
 		// contacts go in descending order of their current positions.
 		std::vector<int32_t> mine;
-		if (index + 1 < (int32_t)w.hcStart.size())
+		if (index < w.hcN)
 		{
 			for (int32_t s = w.hcStart[index]; s < w.hcStart[index + 1]; ++s)
 				if (w.hcList[s] >= 0)
@@ -968,9 +943,10 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 		std::sort(mine.begin(), mine.end(), [](int32_t a, int32_t b) { return a > b; });
 		for (int32_t i : mine)
 		{
-			int2 c = w.hcBodies[i];
+			int4 e = w.hcSide[i];
+			int2 c = make_int2(e.x, e.y);
 			int32_t other = c.x == index ? c.y : c.x;
-			uint32_t cf = w.hcFlags[i];
+			uint32_t cf = (uint32_t)e.z;
 			if (cf & CF_TOUCHING)
 			{
 				if (!(cf & CF_SENSOR))
@@ -986,23 +962,19 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 				w.hcList[gone.x] = -1;
 			if (gone.y >= 0)
 				w.hcList[gone.y] = -1;
-			int32_t last = (int32_t)w.hcBodies.size() - 1;
+			int32_t last = w.hcCount - 1;
 			if (i != last)
 			{
-				w.hcBodies[i] = w.hcBodies[last];
-				w.hcFlags[i] = w.hcFlags[last];
-				w.hcOrigin[i] = w.hcOrigin[last];
+				w.hcSide[i] = w.hcSide[last];
 				int2 sl = w.hcSlot[last];
 				w.hcSlot[i] = sl;
 				if (sl.x >= 0)
 					w.hcList[sl.x] = i;
 				if (sl.y >= 0)
 					w.hcList[sl.y] = i;
+				w.hcMoved.push_back(i);
 			}
-			w.hcBodies.pop_back();
-			w.hcFlags.pop_back();
-			w.hcOrigin.pop_back();
-			w.hcSlot.pop_back();
+			--w.hcCount;
 			w.pendingPermutation = true;
 		}
 	}
