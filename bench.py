@@ -417,12 +417,20 @@ def main():
     cpu_stream = stream.clone_onto(dev, w) if (stream and snap is not None) else None  # churn state at t0
 
     model_bytes = 0.0
+    churn_s = 0.0
+    prof_ms = [0.0, 0.0, 0.0]
     t0 = time.perf_counter()
     for i in range(args.steps):
         if stream:
+            tc = time.perf_counter()
             stream.churn_step()
+            churn_s += time.perf_counter() - tc
         dev.m3d_world_step(w, DT, args.substeps)
         model_bytes += traffic_bytes(dev.stats(w), args.substeps)
+        p = dev.m3d_world_profile(w)
+        prof_ms[0] += p.broadphaseMs
+        prof_ms[1] += p.narrowphaseMs
+        prof_ms[2] += p.solverMs
     elapsed = time.perf_counter() - t0
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
@@ -477,6 +485,10 @@ def main():
         "step_traffic_model": {"bytes_per_step": model_bytes / args.steps,
                                "achieved_GBps": model_bytes / elapsed / 1e9},
         "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
+        "host_split_ms_per_step": {"churn_api_calls": 1000.0 * churn_s / args.steps,
+                                   "step_broadphase": prof_ms[0] / args.steps,
+                                   "step_narrowphase": prof_ms[1] / args.steps,
+                                   "step_solver": prof_ms[2] / args.steps},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -70,8 +70,8 @@ struct DiffState
 	std::vector<DiffStepRec*> steps;
 	// reverse pass scratch
 	DBuf<char> state;   // StepState arrays
-	DBuf<uint32_t> ta, tb;
-	DBuf<float> tda, tdb, adj;
+	DBuf<dd::Node<float>> nodes;
+	DBuf<float> adj;
 	DBuf<Tape<float>> tapes;
 	DBuf<uint32_t> outIds;
 	DBuf<float> seed, leafAdj;
@@ -88,10 +88,7 @@ struct DiffState
 		}
 		bodyStart.release();
 		state.release();
-		ta.release();
-		tb.release();
-		tda.release();
-		tdb.release();
+		nodes.release();
 		adj.release();
 		tapes.release();
 		outIds.release();
@@ -532,10 +529,12 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 							Ad<float>* muScratch)
 {
 	typedef Ad<float> A;
+	__shared__ Tape<float> tl; // the tape header in LDS: every node bumps it
 	int32_t wi = (int32_t)blockIdx.x;
 	if (wi >= V.worlds || threadIdx.x != 0)
 		return;
-	Tape<float>* t = &tapes[wi];
+	tl = tapes[wi];
+	Tape<float>* t = &tl;
 	StepIn in = world_in(V, wi);
 	int32_t b0 = V.bodyStart[wi];
 	StepState<A> S = world_state(G, b0, V.contactStart[wi]);
@@ -594,6 +593,8 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 		o[11] = S.w[i].y.id;
 		o[12] = S.w[i].z.id;
 	}
+	tapes[wi].n = tl.n;
+	tapes[wi].overflow = tl.overflow;
 }
 
 // Reverse sweep of each world's tape: adj(out) = seed, then every node
@@ -607,7 +608,7 @@ __global__ void k_diff_backprop(const Tape<float>* tapes, const int32_t* bodySta
 	const Tape<float>& T = tapes[wi];
 	int32_t b0 = bodyStart[wi], b1 = bodyStart[wi + 1];
 	uint64_t st = T.stride;
-	float* A = adj + (T.a - tapes[0].a); // laid out like the tape columns
+	float* A = adj + (T.nodes - tapes[0].nodes); // laid out like the tape
 	for (uint32_t k = 0; k < T.n; ++k)
 		A[(uint64_t)k * st] = 0.0f;
 	for (int32_t g = b0; g < b1; ++g)
@@ -623,11 +624,11 @@ __global__ void k_diff_backprop(const Tape<float>* tapes, const int32_t* bodySta
 		float a = A[e];
 		if (a == 0.0f)
 			continue;
-		uint32_t ia = T.a[e], ib = T.b[e];
-		if (ia != dd::kNoNode)
-			A[(uint64_t)ia * st] += a * T.da[e];
-		if (ib != dd::kNoNode)
-			A[(uint64_t)ib * st] += a * T.db[e];
+		dd::Node<float> nd = T.nodes[e];
+		if (nd.a != dd::kNoNode)
+			A[(uint64_t)nd.a * st] += a * nd.da;
+		if (nd.b != dd::kNoNode)
+			A[(uint64_t)nd.b * st] += a * nd.db;
 	}
 	for (int32_t g = b0; g < b1; ++g)
 		for (int k = 0; k < kLeaves; ++k)
@@ -637,19 +638,14 @@ __global__ void k_diff_backprop(const Tape<float>* tapes, const int32_t* bodySta
 		}
 }
 
-__global__ void k_diff_tapes_init(Tape<float>* tapes, int32_t worlds, uint32_t* a, uint32_t* b, float* da, float* db,
-								  uint32_t cap)
+__global__ void k_diff_tapes_init(Tape<float>* tapes, int32_t worlds, dd::Node<float>* nodes, uint32_t cap)
 {
 	int32_t wi = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (wi >= worlds)
 		return;
 	// each world's nodes contiguous: the reverse sweep reads them in order
 	Tape<float> t;
-	uint64_t base = (uint64_t)wi * cap;
-	t.a = a + base;
-	t.b = b + base;
-	t.da = da + base;
-	t.db = db + base;
+	t.nodes = nodes + (uint64_t)wi * cap;
 	t.stride = 1;
 	t.n = 0;
 	t.cap = cap;
@@ -860,10 +856,9 @@ int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, flo
 		if (cap > 0xFFFFFFF0ull)
 			cap = 0xFFFFFFF0ull;
 		size_t entries = (size_t)cap * W;
-		if (!D->ta.ensure(entries) || !D->tb.ensure(entries) || !D->tda.ensure(entries) || !D->tdb.ensure(entries) ||
-			!D->adj.ensure(entries))
+		if (!D->nodes.ensure(entries) || !D->adj.ensure(entries))
 			return -1;
-		k_diff_tapes_init<<<gridk(W), 256, 0, s>>>(D->tapes.p, W, D->ta.p, D->tb.p, D->tda.p, D->tdb.p, (uint32_t)cap);
+		k_diff_tapes_init<<<gridk(W), 256, 0, s>>>(D->tapes.p, W, D->nodes.p, (uint32_t)cap);
 		StepState<Ad<float>> G;
 		if (!ensure_state(D, R, &G) || !D->scratchI.ensure((size_t)n * sizeof(Ad<float>) / sizeof(int32_t) + 4))
 			return -1;

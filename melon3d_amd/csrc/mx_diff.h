@@ -29,14 +29,22 @@ namespace dd
 
 constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
-// Node k of one lane at index k * stride (lanes interleaved for coalescing).
+// One tape node: its two inputs and the partial derivatives (one 16-byte
+// store for F = float).
+template <class F>
+struct Node
+{
+	uint32_t a, b;
+	F da, db;
+};
+
+// Node k of one lane at nodes[k * stride]. The tape header lives where the
+// replaying lane reaches it fastest (LDS on the device): every operation
+// reads and bumps its count.
 template <class F>
 struct Tape
 {
-	uint32_t* a;
-	uint32_t* b;
-	F* da;
-	F* db;
+	Node<F>* nodes;
 	uint64_t stride;
 	uint32_t n;
 	uint32_t cap;
@@ -61,12 +69,15 @@ MX_HD uint32_t push_node(Tape<F>* t, uint32_t a, F da, uint32_t b, F db)
 		t->overflow = 1;
 		return kNoNode;
 	}
-	uint64_t k = (uint64_t)t->n * t->stride;
-	t->a[k] = a;
-	t->b[k] = b;
-	t->da[k] = da;
-	t->db[k] = db;
-	return t->n++;
+	uint32_t n = t->n;
+	Node<F> e;
+	e.a = a;
+	e.b = b;
+	e.da = da;
+	e.db = db;
+	t->nodes[(uint64_t)n * t->stride] = e;
+	t->n = n + 1;
+	return n;
 }
 
 template <class F>
