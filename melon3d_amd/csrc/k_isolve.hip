@@ -389,6 +389,7 @@ struct GiantCtx
 	uint32_t* counters;
 	uint64_t* slotKeys;       // per entry, sorted later
 	uint32_t* slotVals;
+	uint8_t* dirty;           // 2 x NC: entries to re-evaluate in the next round (zeroed)
 	int bitsI;
 };
 
@@ -547,29 +548,66 @@ __device__ bool level_round(const GiantCtx& G, int32_t k, int4 pr)
 	return false;
 }
 
-// Rounds until one changes nothing. Round r flags changes in ctr[r % 3]
-// and clears ctr[(r + 1) % 3], which no lane reads again before it is
-// reused two rounds later.
+// An entry changed: every later entry on its dynamic bodies (its successors,
+// which read it) is re-evaluated in the next round.
+__device__ void mark_successors(const GiantCtx& G, int32_t k, int4 pr, uint8_t* next)
+{
+	int2 ab = G.X.P.C.bodies[G.X.contactList[k]];
+	if (G.X.P.B.pos[ab.x].w > 0.0f)
+	{
+		int32_t end = (int32_t)G.adjStart[ab.x + 1];
+		for (int32_t e = pr.y + 1; e < end; ++e)
+			next[G.adj[e]] = 1u;
+	}
+	if (G.X.P.B.pos[ab.y].w > 0.0f)
+	{
+		int32_t end = (int32_t)G.adjStart[ab.y + 1];
+		for (int32_t e = pr.w + 1; e < end; ++e)
+			next[G.adj[e]] = 1u;
+	}
+}
+
+// Rounds until one changes nothing. Round 0 evaluates every entry; round
+// r > 0 only the entries one of whose predecessors changed in round r - 1
+// (dirty[r % 2], cleared as they are taken; marks for the next round go to
+// the other half). Round r flags changes in ctr[r % 3] and clears
+// ctr[(r + 1) % 3], which no lane reads again before it is reused two
+// rounds later.
 template <int WHICH>
 __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* ctr, int32_t tid, int32_t nt)
 {
+	const int32_t NC = G.X.NC;
 	for (int32_t r = 0; r < (1 << 24); ++r)
 	{
 		if (tid == 0)
 			__hip_atomic_store(&ctr[(r + 1) % 3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint8_t* cur = G.dirty + (size_t)(r & 1) * NC;
+		uint8_t* next = G.dirty + (size_t)((r + 1) & 1) * NC;
 		uint32_t changes = 0;
-		for (int32_t k = tid; k < G.X.NC; k += nt)
+		for (int32_t k = tid; k < NC; k += nt)
 		{
+			if (r > 0)
+			{
+				if (!cur[k])
+					continue;
+				cur[k] = 0u;
+			}
 			int4 pr = G.pred[k];
 			if (pr.x < 0)
 				continue;
+			bool changed = false;
 			if (WHICH == 0)
 			{
 				if (G.colored[k])
-					changes += color_round(G, k, pr) ? 1u : 0u;
+					changed = color_round(G, k, pr);
 			}
 			else if (G.color[k] == kMaxGraphColors)
-				changes += level_round(G, k, pr) ? 1u : 0u;
+				changed = level_round(G, k, pr);
+			if (changed)
+			{
+				++changes;
+				mark_successors(G, k, pr, next);
+			}
 		}
 		// one flag store per workgroup that changed something (same-address
 		// atomics from every wave would serialize)
@@ -1091,7 +1129,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
 		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
-		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) ||
+		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
 		return false;
 	uint32_t* bodyCount = w.s32c.p;
@@ -1102,6 +1140,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, w.scanStatus.p, s);
 	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
 	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6), barriers 8 and 9
+	launch_fill_u32((uint32_t*)w.sDirty.p, 0u, (int64_t)(two + 3) / 4, s);
 
 	GiantCtx G;
 	G.X = X;
@@ -1115,6 +1154,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.counters = w.counters.p;
 	G.slotKeys = w.s64b.p;
 	G.slotVals = w.s32b.p;
+	G.dirty = w.sDirty.p;
 	G.bitsI = L.bitsI;
 	{
 		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);

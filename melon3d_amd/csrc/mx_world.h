@@ -240,6 +240,7 @@ struct World
 	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
 	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry
 	DBuf<uint8_t> sColored;       // grid path: entry's island is colored
+	DBuf<uint8_t> sDirty;         // grid path: fixpoint worklist flags (2 x entries)
 	DBuf<int32_t> contactColor;   // per contact: -1 / 0..8
 	int32_t lastIslandCount = 0;
 	bool lastHadIslands = false;

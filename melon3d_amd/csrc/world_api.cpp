@@ -117,6 +117,7 @@ World::~World()
 	sBucket.release();
 	sPred.release();
 	sColored.release();
+	sDirty.release();
 	dWake.release();
 	contactColor.release();
 	counters.release();
