@@ -1037,10 +1037,11 @@ static int coop_blocks(World& w, K kernel)
 }
 
 // Workgroups of a giant-island launch: enough lanes for every entry, at
-// most kGiantMaxBlocks (a grid barrier costs about 0.04 us more with every
-// workgroup that arrives, and a slot's lanes solve one contact each in
-// parallel, so more lanes than the largest slot only add barrier time) and
-// never more than can be co-resident. M3D_GIANT_BLOCKS overrides the cap.
+// most a cap (a grid barrier costs about 0.04 us more with every workgroup
+// that arrives, and a slot's lanes solve one contact each in parallel, so
+// lanes beyond the larger slots only add barrier time) and never more than
+// can be co-resident. The passes' cap grows from kGiantMaxBlocks with the
+// step's entries; M3D_GIANT_BLOCKS overrides it.
 constexpr int kGiantMaxBlocks = 64;
 
 // The coloring fixpoint does several loads per entry and round, so it
@@ -1183,7 +1184,11 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.bucketCount = w.sBucket.p;
 	S.bucketCursor = w.sBucket.p + (size_t)(maxS + 1) * kBuckets;
 	launch_fill_u32(S.bucketCount, 0u, (int64_t)(maxS + 1) * kBuckets, s);
-	static const int cap = env_blocks("M3D_GIANT_BLOCKS", kGiantMaxBlocks);
+	// more entries per slot pay for the dearer barrier of a wider grid: about
+	// one workgroup per 1536 entries, 64 to 256 (measured on a 190 k-entry
+	// stream step and a 4 M-entry batch step)
+	static const int capEnv = env_blocks("M3D_GIANT_BLOCKS", 0);
+	const int cap = capEnv > 0 ? capEnv : std::max(kGiantMaxBlocks, std::min(4 * kGiantMaxBlocks, NC / 1536));
 	const int blocks = std::min(giant_blocks(w, k_giant_pass<NormalsPass>, NC, cap),
 								std::min(giant_blocks(w, k_giant_pass<FrictionPass>, NC, cap),
 										 giant_blocks(w, k_giant_pass<VelocityPass>, NC, cap)));
