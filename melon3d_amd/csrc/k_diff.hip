@@ -529,12 +529,11 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 							Ad<float>* muScratch)
 {
 	typedef Ad<float> A;
-	__shared__ Tape<float> tl; // the tape header in LDS: every node bumps it
 	int32_t wi = (int32_t)blockIdx.x;
 	if (wi >= V.worlds || threadIdx.x != 0)
 		return;
+	Tape<float>& tl = dd::ad_tape(); // this lane's tape header, in LDS
 	tl = tapes[wi];
-	Tape<float>* t = &tl;
 	StepIn in = world_in(V, wi);
 	int32_t b0 = V.bodyStart[wi];
 	StepState<A> S = world_state(G, b0, V.contactStart[wi]);
@@ -543,16 +542,16 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 	for (int32_t i = 0; i < in.nb; ++i)
 	{
 		float4 P = in.pos[i], Q = in.rot[i], Vv = in.vel[i], Ww = in.angVel[i];
-		A x = dd::ad_leaf(P.x, t), y = dd::ad_leaf(P.y, t), z = dd::ad_leaf(P.z, t);
-		A qx = dd::ad_leaf(Q.x, t), qy = dd::ad_leaf(Q.y, t), qz = dd::ad_leaf(Q.z, t), qw = dd::ad_leaf(Q.w, t);
-		A vx = dd::ad_leaf(Vv.x, t), vy = dd::ad_leaf(Vv.y, t), vz = dd::ad_leaf(Vv.z, t);
-		A wx = dd::ad_leaf(Ww.x, t), wy = dd::ad_leaf(Ww.y, t), wz = dd::ad_leaf(Ww.z, t);
+		A x = dd::ad_leaf(P.x), y = dd::ad_leaf(P.y), z = dd::ad_leaf(P.z);
+		A qx = dd::ad_leaf(Q.x), qy = dd::ad_leaf(Q.y), qz = dd::ad_leaf(Q.z), qw = dd::ad_leaf(Q.w);
+		A vx = dd::ad_leaf(Vv.x), vy = dd::ad_leaf(Vv.y), vz = dd::ad_leaf(Vv.z);
+		A wx = dd::ad_leaf(Ww.x), wy = dd::ad_leaf(Ww.y), wz = dd::ad_leaf(Ww.z);
 		S.x[i] = dd::v3<A>(x, y, z);
 		S.q[i] = { qx, qy, qz, qw };
 		S.v[i] = dd::v3<A>(vx, vy, vz);
 		S.w[i] = dd::v3<A>(wx, wy, wz);
-		S.invMass[i] = dd::ad_leaf(Ww.w, t); // the mass leaf, turned into 1 / m below
-		bmu[i] = dd::ad_leaf(in.bmat[i].y, t);
+		S.invMass[i] = dd::ad_leaf(Ww.w); // the mass leaf, turned into 1 / m below
+		bmu[i] = dd::ad_leaf(in.bmat[i].y);
 	}
 	for (int32_t i = 0; i < in.nb; ++i)
 	{
@@ -562,10 +561,10 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 		float mv = Ww.w;
 		if (P.w != 0.0f && mv > 0.0f)
 		{
-			S.invMass[i] = dd::ad_make<float>(P.w, m.id, -P.w / mv, dd::kNoNode, 0.0f, t);
-			S.il[i] = dd::v3<A>(dd::ad_make<float>(IL.x, m.id, -IL.x / mv, dd::kNoNode, 0.0f, t),
-								dd::ad_make<float>(IL.y, m.id, -IL.y / mv, dd::kNoNode, 0.0f, t),
-								dd::ad_make<float>(IL.z, m.id, -IL.z / mv, dd::kNoNode, 0.0f, t));
+			S.invMass[i] = dd::ad_make<float>(P.w, m.id, -P.w / mv, dd::kNoNode, 0.0f);
+			S.il[i] = dd::v3<A>(dd::ad_make<float>(IL.x, m.id, -IL.x / mv, dd::kNoNode, 0.0f),
+								dd::ad_make<float>(IL.y, m.id, -IL.y / mv, dd::kNoNode, 0.0f),
+								dd::ad_make<float>(IL.z, m.id, -IL.z / mv, dd::kNoNode, 0.0f));
 		}
 		else
 		{
@@ -599,43 +598,75 @@ __global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tape
 
 // Reverse sweep of each world's tape: adj(out) = seed, then every node
 // passes its adjoint to its inputs; the leaves' adjoints are the gradient.
+// Inputs are mostly recent nodes, so the adjoints of the kWin nodes below
+// the current one live in an LDS ring (slot k % kWin) and only older
+// inputs go to the global array.
+constexpr uint32_t kWin = 8192;
+
 __global__ void k_diff_backprop(const Tape<float>* tapes, const int32_t* bodyStart, int32_t worlds, float* adj,
 								const uint32_t* outIds, const float* seed, float* leafAdj)
 {
+	__shared__ float win[kWin];
 	int32_t wi = (int32_t)blockIdx.x;
-	if (wi >= worlds || threadIdx.x != 0)
+	if (wi >= worlds)
+		return;
+	for (uint32_t q = threadIdx.x; q < kWin; q += blockDim.x)
+		win[q] = 0.0f;
+	__syncthreads();
+	if (threadIdx.x != 0)
 		return;
 	const Tape<float>& T = tapes[wi];
-	int32_t b0 = bodyStart[wi], b1 = bodyStart[wi + 1];
-	uint64_t st = T.stride;
-	float* A = adj + (T.nodes - tapes[0].nodes); // laid out like the tape
-	for (uint32_t k = 0; k < T.n; ++k)
-		A[(uint64_t)k * st] = 0.0f;
+	const int32_t b0 = bodyStart[wi], b1 = bodyStart[wi + 1];
+	const uint32_t n = (uint32_t)T.n;
+	const dd::Node<float>* nodes = T.nodes;
+	float* glob = adj + (T.nodes - tapes[0].nodes); // this world's part
+	// nodes below the first window get contributions in glob
+	const uint32_t low = n > kWin ? n - kWin : 0u;
+	for (uint32_t k = 0; k < low; ++k)
+		glob[k] = 0.0f;
 	for (int32_t g = b0; g < b1; ++g)
 		for (int k = 0; k < kOuts; ++k)
 		{
 			uint32_t id = outIds[(int64_t)g * kOuts + k];
-			if (id != dd::kNoNode && id < T.n)
-				A[(uint64_t)id * st] += seed[(int64_t)g * kOuts + k];
+			if (id == dd::kNoNode || id >= n)
+				continue;
+			float sd = seed[(int64_t)g * kOuts + k];
+			if (id >= low)
+				win[id % kWin] += sd;
+			else
+				glob[id] += sd;
 		}
-	for (uint32_t k = T.n; k-- > 0;)
+	const uint32_t nLeaves = (uint32_t)(b1 - b0) * kLeaves;
+	for (uint32_t k = n; k-- > 0;)
 	{
-		uint64_t e = (uint64_t)k * st;
-		float a = A[e];
+		// node k's adjoint: its ring slot, plus what reached it while it was
+		// below the ring (only nodes below `low` ever were)
+		float a = win[k % kWin];
+		win[k % kWin] = 0.0f; // from here on the slot is node k - kWin's
+		if (k < low)
+			a += glob[k];
+		if (k < nLeaves)
+			leafAdj[(int64_t)(b0 + k / kLeaves) * kLeaves + k % kLeaves] = a;
 		if (a == 0.0f)
 			continue;
-		dd::Node<float> nd = T.nodes[e];
+		dd::Node<float> nd = nodes[k];
+		// after node k the ring holds nodes [k - kWin, k - 1]
+		uint32_t edge = k >= kWin ? k - kWin : 0u;
 		if (nd.a != dd::kNoNode)
-			A[(uint64_t)nd.a * st] += a * nd.da;
-		if (nd.b != dd::kNoNode)
-			A[(uint64_t)nd.b * st] += a * nd.db;
-	}
-	for (int32_t g = b0; g < b1; ++g)
-		for (int k = 0; k < kLeaves; ++k)
 		{
-			uint32_t id = (uint32_t)((g - b0) * kLeaves + k);
-			leafAdj[(int64_t)g * kLeaves + k] = id < T.n ? A[(uint64_t)id * st] : 0.0f;
+			if (nd.a >= edge)
+				win[nd.a % kWin] += a * nd.da;
+			else
+				glob[nd.a] += a * nd.da;
 		}
+		if (nd.b != dd::kNoNode)
+		{
+			if (nd.b >= edge)
+				win[nd.b % kWin] += a * nd.db;
+			else
+				glob[nd.b] += a * nd.db;
+		}
+	}
 }
 
 __global__ void k_diff_tapes_init(Tape<float>* tapes, int32_t worlds, dd::Node<float>* nodes, uint32_t cap)

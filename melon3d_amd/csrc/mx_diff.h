@@ -38,62 +38,61 @@ struct Node
 	F da, db;
 };
 
-// Node k of one lane at nodes[k * stride]. The tape header lives where the
-// replaying lane reaches it fastest (LDS on the device): every operation
-// reads and bumps its count.
+// Node k of one lane at nodes[k * stride]. The count is 64-bit so that, by
+// type, no node store can alias it: with the header in LDS at one known
+// address, the compiler keeps the count in a register across operations.
 template <class F>
 struct Tape
 {
 	Node<F>* nodes;
 	uint64_t stride;
-	uint32_t n;
-	uint32_t cap;
+	uint64_t n;
+	uint64_t cap;
 	int overflow;
 };
 
+// Tape scalar: a value and the node that produced it (kNoNode: a constant).
 template <class F>
 struct Ad
 {
 	F v;
 	uint32_t id;
-	Tape<F>* t;
 };
 
-template <class F>
-MX_HD uint32_t push_node(Tape<F>* t, uint32_t a, F da, uint32_t b, F db)
+// The replaying lane's tape header (one active lane per workgroup).
+__device__ __forceinline__ Tape<float>& ad_tape()
 {
-	if (t == nullptr)
-		return kNoNode;
-	if (t->n >= t->cap)
+	__shared__ Tape<float> tape;
+	return tape;
+}
+
+template <class F>
+__device__ __forceinline__ uint32_t push_node(uint32_t a, F da, uint32_t b, F db)
+{
+	Tape<F>& t = ad_tape();
+	uint64_t n = t.n;
+	if (n >= t.cap)
 	{
-		t->overflow = 1;
+		t.overflow = 1;
 		return kNoNode;
 	}
-	uint32_t n = t->n;
 	Node<F> e;
 	e.a = a;
 	e.b = b;
 	e.da = da;
 	e.db = db;
-	t->nodes[(uint64_t)n * t->stride] = e;
-	t->n = n + 1;
-	return n;
+	t.nodes[n * t.stride] = e;
+	t.n = n + 1;
+	return (uint32_t)n;
 }
 
 template <class F>
-MX_HD Ad<F> ad_make(F v, uint32_t a, F da, uint32_t b, F db, Tape<F>* t)
+__device__ __forceinline__ Ad<F> ad_make(F v, uint32_t a, F da, uint32_t b, F db)
 {
 	Ad<F> r;
 	r.v = v;
-	r.t = t;
-	r.id = (a == kNoNode && b == kNoNode) ? kNoNode : push_node(t, a, da, b, db);
+	r.id = (a == kNoNode && b == kNoNode) ? kNoNode : push_node<F>(a, da, b, db);
 	return r;
-}
-
-template <class F>
-MX_HD Tape<F>* tape_of(const Ad<F>& x, const Ad<F>& y)
-{
-	return x.t != nullptr ? x.t : y.t;
 }
 
 template <class F>
@@ -102,87 +101,85 @@ MX_HD Ad<F> ad_const(F v)
 	Ad<F> r;
 	r.v = v;
 	r.id = kNoNode;
-	r.t = nullptr;
 	return r;
 }
 
 // a leaf (parameter or state input): a node without inputs
 template <class F>
-MX_HD Ad<F> ad_leaf(F v, Tape<F>* t)
+__device__ __forceinline__ Ad<F> ad_leaf(F v)
 {
 	Ad<F> r;
 	r.v = v;
-	r.t = t;
-	r.id = push_node(t, kNoNode, (F)0, kNoNode, (F)0);
+	r.id = push_node<F>(kNoNode, (F)0, kNoNode, (F)0);
 	return r;
 }
 
 template <class F>
-MX_HD Ad<F> operator+(Ad<F> x, Ad<F> y)
+__device__ __forceinline__ Ad<F> operator+(Ad<F> x, Ad<F> y)
 {
-	return ad_make<F>(x.v + y.v, x.id, (F)1, y.id, (F)1, tape_of(x, y));
+	return ad_make<F>(x.v + y.v, x.id, (F)1, y.id, (F)1);
 }
 template <class F>
-MX_HD Ad<F> operator-(Ad<F> x, Ad<F> y)
+__device__ __forceinline__ Ad<F> operator-(Ad<F> x, Ad<F> y)
 {
-	return ad_make<F>(x.v - y.v, x.id, (F)1, y.id, (F)-1, tape_of(x, y));
+	return ad_make<F>(x.v - y.v, x.id, (F)1, y.id, (F)-1);
 }
 template <class F>
-MX_HD Ad<F> operator*(Ad<F> x, Ad<F> y)
+__device__ __forceinline__ Ad<F> operator*(Ad<F> x, Ad<F> y)
 {
-	return ad_make<F>(x.v * y.v, x.id, y.v, y.id, x.v, tape_of(x, y));
+	return ad_make<F>(x.v * y.v, x.id, y.v, y.id, x.v);
 }
 template <class F>
-MX_HD Ad<F> operator/(Ad<F> x, Ad<F> y)
+__device__ __forceinline__ Ad<F> operator/(Ad<F> x, Ad<F> y)
 {
 	F r = x.v / y.v;
-	return ad_make<F>(r, x.id, (F)1 / y.v, y.id, -r / y.v, tape_of(x, y));
+	return ad_make<F>(r, x.id, (F)1 / y.v, y.id, -r / y.v);
 }
 template <class F>
-MX_HD Ad<F> operator-(Ad<F> x)
+__device__ __forceinline__ Ad<F> operator-(Ad<F> x)
 {
-	return ad_make<F>(-x.v, x.id, (F)-1, kNoNode, (F)0, x.t);
+	return ad_make<F>(-x.v, x.id, (F)-1, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator+(Ad<F> x, F c)
+__device__ __forceinline__ Ad<F> operator+(Ad<F> x, F c)
 {
-	return ad_make<F>(x.v + c, x.id, (F)1, kNoNode, (F)0, x.t);
+	return ad_make<F>(x.v + c, x.id, (F)1, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator+(F c, Ad<F> x)
+__device__ __forceinline__ Ad<F> operator+(F c, Ad<F> x)
 {
-	return ad_make<F>(c + x.v, x.id, (F)1, kNoNode, (F)0, x.t);
+	return ad_make<F>(c + x.v, x.id, (F)1, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator-(Ad<F> x, F c)
+__device__ __forceinline__ Ad<F> operator-(Ad<F> x, F c)
 {
-	return ad_make<F>(x.v - c, x.id, (F)1, kNoNode, (F)0, x.t);
+	return ad_make<F>(x.v - c, x.id, (F)1, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator-(F c, Ad<F> x)
+__device__ __forceinline__ Ad<F> operator-(F c, Ad<F> x)
 {
-	return ad_make<F>(c - x.v, x.id, (F)-1, kNoNode, (F)0, x.t);
+	return ad_make<F>(c - x.v, x.id, (F)-1, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator*(Ad<F> x, F c)
+__device__ __forceinline__ Ad<F> operator*(Ad<F> x, F c)
 {
-	return ad_make<F>(x.v * c, x.id, c, kNoNode, (F)0, x.t);
+	return ad_make<F>(x.v * c, x.id, c, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator*(F c, Ad<F> x)
+__device__ __forceinline__ Ad<F> operator*(F c, Ad<F> x)
 {
-	return ad_make<F>(c * x.v, x.id, c, kNoNode, (F)0, x.t);
+	return ad_make<F>(c * x.v, x.id, c, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator/(Ad<F> x, F c)
+__device__ __forceinline__ Ad<F> operator/(Ad<F> x, F c)
 {
-	return ad_make<F>(x.v / c, x.id, (F)1 / c, kNoNode, (F)0, x.t);
+	return ad_make<F>(x.v / c, x.id, (F)1 / c, kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> operator/(F c, Ad<F> x)
+__device__ __forceinline__ Ad<F> operator/(F c, Ad<F> x)
 {
 	F r = c / x.v;
-	return ad_make<F>(r, x.id, -r / x.v, kNoNode, (F)0, x.t);
+	return ad_make<F>(r, x.id, -r / x.v, kNoNode, (F)0);
 }
 
 // value of a scalar of any kind
@@ -220,41 +217,41 @@ struct Num<Ad<G>>
 MX_HD float sqrt_(float x) { return sqrtf(x); }
 MX_HD double sqrt_(double x) { return sqrt(x); }
 template <class F>
-MX_HD Ad<F> sqrt_(Ad<F> x)
+__device__ __forceinline__ Ad<F> sqrt_(Ad<F> x)
 {
 	F r = sqrt_(x.v);
-	return ad_make<F>(r, x.id, r > (F)0 ? (F)0.5 / r : (F)0, kNoNode, (F)0, x.t);
+	return ad_make<F>(r, x.id, r > (F)0 ? (F)0.5 / r : (F)0, kNoNode, (F)0);
 }
 MX_HD float sin_(float x) { return (float)sin((double)x); }
 MX_HD double sin_(double x) { return sin(x); }
 MX_HD float cos_(float x) { return (float)cos((double)x); }
 MX_HD double cos_(double x) { return cos(x); }
 template <class F>
-MX_HD Ad<F> sin_(Ad<F> x)
+__device__ __forceinline__ Ad<F> sin_(Ad<F> x)
 {
-	return ad_make<F>(sin_(x.v), x.id, cos_(x.v), kNoNode, (F)0, x.t);
+	return ad_make<F>(sin_(x.v), x.id, cos_(x.v), kNoNode, (F)0);
 }
 template <class F>
-MX_HD Ad<F> cos_(Ad<F> x)
+__device__ __forceinline__ Ad<F> cos_(Ad<F> x)
 {
-	return ad_make<F>(cos_(x.v), x.id, -sin_(x.v), kNoNode, (F)0, x.t);
+	return ad_make<F>(cos_(x.v), x.id, -sin_(x.v), kNoNode, (F)0);
 }
 MX_HD float atan2_(float y, float x) { return (float)atan2((double)y, (double)x); }
 MX_HD double atan2_(double y, double x) { return atan2(y, x); }
 template <class F>
-MX_HD Ad<F> atan2_(Ad<F> y, Ad<F> x)
+__device__ __forceinline__ Ad<F> atan2_(Ad<F> y, Ad<F> x)
 {
 	F r2 = x.v * x.v + y.v * y.v;
 	F dy = r2 > (F)0 ? x.v / r2 : (F)0;
 	F dx = r2 > (F)0 ? -y.v / r2 : (F)0;
-	return ad_make<F>(atan2_(y.v, x.v), y.id, dy, x.id, dx, tape_of(y, x));
+	return ad_make<F>(atan2_(y.v, x.v), y.id, dy, x.id, dx);
 }
 MX_HD float fabs_(float x) { return fabsf(x); }
 MX_HD double fabs_(double x) { return fabs(x); }
 template <class F>
-MX_HD Ad<F> fabs_(Ad<F> x)
+__device__ __forceinline__ Ad<F> fabs_(Ad<F> x)
 {
-	return ad_make<F>(x.v < (F)0 ? -x.v : x.v, x.id, x.v < (F)0 ? (F)-1 : (F)1, kNoNode, (F)0, x.t);
+	return ad_make<F>(x.v < (F)0 ? -x.v : x.v, x.id, x.v < (F)0 ? (F)-1 : (F)1, kNoNode, (F)0);
 }
 
 // ------------------------------------------------------------ vectors
