@@ -319,9 +319,35 @@ M3D_API int m3dx_diff_replay(m3d_world* world, int first, int count, int precisi
    final state (13 floats per body: x q v w); returns the adjoint of the first
    step's input state (13 per body; its v part is the gradient with respect to
    the initial linear velocities) and the gradients with respect to each
-   body's mass and friction. Output pointers may be null. Returns 0 or -1. */
+   body's mass and friction. Output pointers may be null. Returns 0 or -1.
+   The parallel adjoint: per world and step, the solve's ops in reverse level
+   order, each op's vector-Jacobian product from forward tangents; states are
+   recomputed from a checkpoint every m3dx_diff_set_checkpoint steps. Worlds
+   of up to 3072 solver contacts and 2048 bodies. */
 M3D_API int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad,
                                float* frictionGrad);
+/* The same product by a reverse-mode tape walked by one lane per world (a
+   second, independent evaluation, for checking). */
+M3D_API int m3dx_diff_backward_tape(m3d_world* world, const float* seed, float* stateAdj, float* massGrad,
+                                    float* frictionGrad);
+/* Steps between body-state checkpoints of the backward pass (default 10).
+   Returns 0 or -1. */
+M3D_API int m3dx_diff_set_checkpoint(m3d_world* world, int every);
+/* Recorded steps first .. first + count - 1 replayed by the parallel
+   adjoint's forward kernels (float, the engine's arithmetic) from the state
+   recorded at `first`; stateOut receives the final state (13 floats per
+   body). Returns 0 or -1. */
+M3D_API int m3dx_diff_forward(m3d_world* world, int first, int count, float* stateOut);
+/* Forward tangents of recorded steps first .. first + count - 1 on the host
+   CPU along one direction (dStateIn, dMass, dFriction) from stateIn with
+   per-body mass and friction (null: the recorded values); derivatives in
+   double. primal 0: values in float, the engine's own path; 1: values in
+   double (the FP64 replay). stateOut and dStateOut receive the final state
+   and its directional derivative (13 doubles per body). Host arrays.
+   Returns 0 or -1. */
+M3D_API int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal, const double* stateIn,
+                              const double* dStateIn, const double* mass, const double* dMass,
+                              const double* friction, const double* dFriction, double* stateOut, double* dStateOut);
 
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);

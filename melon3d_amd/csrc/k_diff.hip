@@ -11,8 +11,7 @@
 // backwards from the adjoint of the step's output state, giving the adjoint
 // of its input state (the seed of the step before) and the step's share of
 // the mass and friction gradients.
-#include "mx_diff.h"
-#include "mx_steps.h"
+#include "mx_diffrec.h"
 
 #include <algorithm>
 #include <string.h>
@@ -23,85 +22,6 @@ namespace mx
 
 DBodies make_body_view(World& w);
 DContacts make_contact_view(World& w);
-
-using dd::Ad;
-using dd::StepIn;
-using dd::StepState;
-using dd::Tape;
-
-constexpr int kLeaves = 15; // per body: x(3) q(4) v(3) w(3) mass friction
-constexpr int kOuts = 13;   // per body: x(3) q(4) v(3) w(3)
-
-struct DiffStepRec
-{
-	int32_t nb = 0, nc = 0;
-	int32_t maxWorldBodies = 0, maxWorldPoints = 0;
-	float dt = 0.0f;
-	int substeps = 1;
-	m3d_vec3 gravity{};
-	DBuf<float4> pos, rot, vel, angVel, invIL, shape, bmat;
-	DBuf<float> angDamp;
-	DBuf<uint32_t> flags;
-	DBuf<int32_t> contactStart; // worlds + 1
-	DBuf<int2> bodies;          // world-local
-	DBuf<int32_t> idx;          // engine contact slot of each recorded contact
-	// one block per substep: the manifold the substep's prepare left
-	DBuf<float4> normal, cmat;            // [s * nc + c]
-	DBuf<float4> pA, pB, pFA, pFB, pSlip; // [(s * nc + c) * 4 + j]
-	void release()
-	{
-		DBuf<float4>* f[] = { &pos, &rot, &vel, &angVel, &invIL, &shape, &bmat, &normal, &cmat, &pA, &pB, &pFA, &pFB, &pSlip };
-		for (auto* p : f)
-			p->release();
-		angDamp.release();
-		flags.release();
-		contactStart.release();
-		bodies.release();
-		idx.release();
-	}
-};
-
-struct DiffState
-{
-	bool recording = false;
-	int32_t worlds = 1;
-	std::vector<int32_t> bodyStartHost; // worlds + 1
-	DBuf<int32_t> bodyStart;
-	std::vector<DiffStepRec*> steps;
-	// reverse pass scratch
-	DBuf<char> state;   // StepState arrays
-	DBuf<dd::Node<float>> nodes;
-	DBuf<float> adj;
-	DBuf<Tape<float>> tapes;
-	DBuf<uint32_t> outIds;
-	DBuf<float> seed, leafAdj;
-	DBuf<double> dstate, dparam;
-	DBuf<float> fstate, fparam;
-	DBuf<int32_t> scratchI;
-	DBuf<uint32_t> counters;
-	~DiffState()
-	{
-		for (DiffStepRec* r : steps)
-		{
-			r->release();
-			delete r;
-		}
-		bodyStart.release();
-		state.release();
-		nodes.release();
-		adj.release();
-		tapes.release();
-		outIds.release();
-		seed.release();
-		leafAdj.release();
-		dstate.release();
-		dparam.release();
-		fstate.release();
-		fparam.release();
-		scratchI.release();
-		counters.release();
-	}
-};
 
 void diff_release(World& w)
 {
@@ -273,132 +193,6 @@ bool diff_record_substep(World& w, int sub)
 }
 
 // ------------------------------------------------------------ replay
-
-// replay scratch of one precision, carved from one byte buffer
-template <class T>
-struct StateBufs
-{
-	StepState<T> S;
-	static size_t bytes(int64_t nb, int64_t nc)
-	{
-		int64_t np = 4 * nc;
-		return (size_t)nb * (sizeof(dd::V3<T>) * 6 + sizeof(dd::Q4<T>) * 2 + sizeof(dd::M3<T>) + sizeof(T) +
-							 sizeof(dd::V3<T>)) +
-			   (size_t)nc * sizeof(T) + (size_t)np * (sizeof(T) * 3 + sizeof(dd::V3<T>) * 4 + sizeof(float4) * 3) +
-			   64 * 32;
-	}
-	void carve(char* base, int64_t nb, int64_t nc)
-	{
-		int64_t np = 4 * nc;
-		char* p = base;
-		auto take = [&](size_t n) {
-			char* r = p;
-			p += (n + 63) & ~(size_t)63;
-			return r;
-		};
-		S.x = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.q = (dd::Q4<T>*)take(nb * sizeof(dd::Q4<T>));
-		S.v = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.w = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.px = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.pq = (dd::Q4<T>*)take(nb * sizeof(dd::Q4<T>));
-		S.pv = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.pw = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.iw = (dd::M3<T>*)take(nb * sizeof(dd::M3<T>));
-		S.invMass = (T*)take(nb * sizeof(T));
-		S.il = (dd::V3<T>*)take(nb * sizeof(dd::V3<T>));
-		S.mu = (T*)take(std::max<int64_t>(nc, 1) * sizeof(T));
-		S.lambdaN = (T*)take(std::max<int64_t>(np, 1) * sizeof(T));
-		S.lambdaT = (T*)take(std::max<int64_t>(np, 1) * sizeof(T));
-		S.rA = (dd::V3<T>*)take(std::max<int64_t>(np, 1) * sizeof(dd::V3<T>));
-		S.rB = (dd::V3<T>*)take(std::max<int64_t>(np, 1) * sizeof(dd::V3<T>));
-		S.jA = (dd::V3<T>*)take(std::max<int64_t>(np, 1) * sizeof(dd::V3<T>));
-		S.jB = (dd::V3<T>*)take(std::max<int64_t>(np, 1) * sizeof(dd::V3<T>));
-		S.wN = (T*)take(std::max<int64_t>(np, 1) * sizeof(T));
-		S.fA = (float4*)take(std::max<int64_t>(np, 1) * sizeof(float4));
-		S.fB = (float4*)take(std::max<int64_t>(np, 1) * sizeof(float4));
-		S.slip = (float4*)take(std::max<int64_t>(np, 1) * sizeof(float4));
-	}
-};
-
-struct RecView
-{
-	const float4 *pos, *rot, *vel, *angVel, *invIL, *shape, *bmat;
-	const float* angDamp;
-	const uint32_t* flags;
-	const int32_t* bodyStart;
-	const int32_t* contactStart;
-	const int2* bodies;
-	const float4 *normal, *cmat, *pA, *pB, *pFA, *pFB, *pSlip;
-	int64_t cStride;
-	int32_t worlds;
-	float dt;
-	int substeps;
-	m3d_vec3 gravity;
-	int iterations;
-};
-
-static RecView rec_view(const DiffState* D, const DiffStepRec* R, int iterations)
-{
-	RecView V;
-	V.pos = R->pos.p;
-	V.rot = R->rot.p;
-	V.vel = R->vel.p;
-	V.angVel = R->angVel.p;
-	V.invIL = R->invIL.p;
-	V.shape = R->shape.p;
-	V.bmat = R->bmat.p;
-	V.angDamp = R->angDamp.p;
-	V.flags = R->flags.p;
-	V.bodyStart = D->bodyStart.p;
-	V.contactStart = R->contactStart.p;
-	V.bodies = R->bodies.p;
-	V.normal = R->normal.p;
-	V.cmat = R->cmat.p;
-	V.pA = R->pA.p;
-	V.pB = R->pB.p;
-	V.pFA = R->pFA.p;
-	V.pFB = R->pFB.p;
-	V.pSlip = R->pSlip.p;
-	V.cStride = R->nc;
-	V.worlds = D->worlds;
-	V.dt = R->dt;
-	V.substeps = R->substeps;
-	V.gravity = R->gravity;
-	V.iterations = iterations;
-	return V;
-}
-
-__device__ StepIn world_in(const RecView& V, int32_t wi)
-{
-	int32_t b0 = V.bodyStart[wi], b1 = V.bodyStart[wi + 1];
-	int32_t c0 = V.contactStart[wi], c1 = V.contactStart[wi + 1];
-	StepIn in;
-	in.nb = b1 - b0;
-	in.pos = V.pos + b0;
-	in.rot = V.rot + b0;
-	in.vel = V.vel + b0;
-	in.angVel = V.angVel + b0;
-	in.invIL = V.invIL + b0;
-	in.angDamp = V.angDamp + b0;
-	in.shape = V.shape + b0;
-	in.bmat = V.bmat + b0;
-	in.flags = V.flags + b0;
-	in.nc = c1 - c0;
-	in.bodies = V.bodies + c0;
-	in.normal = V.normal + c0;
-	in.cmat = V.cmat + c0;
-	in.pA = V.pA + 4 * (int64_t)c0;
-	in.pB = V.pB + 4 * (int64_t)c0;
-	in.pFA = V.pFA + 4 * (int64_t)c0;
-	in.pFB = V.pFB + 4 * (int64_t)c0;
-	in.pSlip = V.pSlip + 4 * (int64_t)c0;
-	in.cStride = V.cStride;
-	in.dt = V.dt;
-	in.substeps = V.substeps;
-	in.gravity = V.gravity;
-	return in;
-}
 
 template <class T>
 __device__ StepState<T> world_state(const StepState<T>& G, int32_t b0, int32_t c0)
@@ -686,7 +480,6 @@ __global__ void k_diff_tapes_init(Tape<float>* tapes, int32_t worlds, dd::Node<f
 
 // ------------------------------------------------------------ host
 
-static int iterations_of(const World& w) { return w.def.positionIterations > 0 ? w.def.positionIterations : 2; }
 
 template <class T>
 static bool ensure_state(DiffState* D, const DiffStepRec* R, StepState<T>* out)
@@ -698,6 +491,149 @@ static bool ensure_state(DiffState* D, const DiffStepRec* R, StepState<T>* out)
 	sb.carve(D->state.p, R->nb, std::max(R->nc, 1));
 	*out = sb.S;
 	return true;
+}
+
+
+// Forward tangents of the recorded steps on the host (see m3dx_diff_jvp_cpu):
+// values in P (float: the engine's own path, double: the FP64 replay),
+// derivatives in double.
+template <class P>
+static int jvp_host(World& w, int first, int count, const double* stateIn, const double* dStateIn, const double* mass,
+					const double* dMass, const double* friction, const double* dFriction, double* stateOut,
+					double* dStateOut)
+{
+	typedef dd::Dual<P, double> D;
+	DiffState* Ds = w.diff;
+	const int32_t n = Ds->steps[first]->nb;
+	const int32_t W = Ds->worlds;
+	const int iterations = iterations_of(w);
+	std::vector<D> cur((size_t)n * kOuts);
+	for (size_t k = 0; k < cur.size(); ++k)
+		cur[k] = dd::dual<P, double>((P)stateIn[k], dStateIn[k]);
+	auto down = [](auto& v, const auto& buf, size_t cnt) {
+		v.resize(cnt);
+		return cnt == 0 || hipMemcpy(v.data(), buf.p, cnt * sizeof(v[0]), hipMemcpyDeviceToHost) == hipSuccess;
+	};
+	std::vector<float4> pos, rot, vel, angVel, invIL, shape, bmat, normal, cmat, pA, pB, pFA, pFB, pSlip;
+	std::vector<float> angDamp;
+	std::vector<uint32_t> flags;
+	std::vector<int32_t> cstart;
+	std::vector<int2> bodies;
+	std::vector<char> mem;
+	for (int k = 0; k < count; ++k)
+	{
+		const DiffStepRec* R = Ds->steps[first + k];
+		if (R->nb != n)
+		{
+			set_error("body count changed while recording");
+			return -1;
+		}
+		size_t nb = (size_t)n, ncs = (size_t)R->nc * R->substeps;
+		bool ok = down(pos, R->pos, nb) && down(rot, R->rot, nb) && down(vel, R->vel, nb) && down(angVel, R->angVel, nb) &&
+				  down(invIL, R->invIL, nb) && down(shape, R->shape, nb) && down(bmat, R->bmat, nb) &&
+				  down(angDamp, R->angDamp, nb) && down(flags, R->flags, nb) &&
+				  down(cstart, R->contactStart, (size_t)W + 1) && down(bodies, R->bodies, (size_t)R->nc) &&
+				  down(normal, R->normal, ncs) && down(cmat, R->cmat, ncs) && down(pA, R->pA, 4 * ncs) &&
+				  down(pB, R->pB, 4 * ncs) && down(pFA, R->pFA, 4 * ncs) && down(pFB, R->pFB, 4 * ncs) &&
+				  down(pSlip, R->pSlip, 4 * ncs);
+		if (!ok)
+		{
+			set_error("jvp: record download failed");
+			return -1;
+		}
+		for (int32_t wi = 0; wi < W; ++wi)
+		{
+			int32_t b0 = Ds->bodyStartHost[wi], b1 = Ds->bodyStartHost[wi + 1];
+			int32_t c0 = cstart[wi], c1 = cstart[wi + 1];
+			StepIn in;
+			in.nb = b1 - b0;
+			in.pos = pos.data() + b0;
+			in.rot = rot.data() + b0;
+			in.vel = vel.data() + b0;
+			in.angVel = angVel.data() + b0;
+			in.invIL = invIL.data() + b0;
+			in.angDamp = angDamp.data() + b0;
+			in.shape = shape.data() + b0;
+			in.bmat = bmat.data() + b0;
+			in.flags = flags.data() + b0;
+			in.nc = c1 - c0;
+			in.bodies = bodies.data() + c0;
+			in.normal = normal.data() + c0;
+			in.cmat = cmat.data() + c0;
+			in.pA = pA.data() + 4 * (int64_t)c0;
+			in.pB = pB.data() + 4 * (int64_t)c0;
+			in.pFA = pFA.data() + 4 * (int64_t)c0;
+			in.pFB = pFB.data() + 4 * (int64_t)c0;
+			in.pSlip = pSlip.data() + 4 * (int64_t)c0;
+			in.cStride = R->nc;
+			in.dt = R->dt;
+			in.substeps = R->substeps;
+			in.gravity = R->gravity;
+			StateBufs<D> sb;
+			mem.assign(StateBufs<D>::bytes(in.nb, std::max(in.nc, 1)), 0);
+			sb.carve(mem.data(), in.nb, std::max(in.nc, 1));
+			StepState<D>& S = sb.S;
+			std::vector<D> bmu((size_t)in.nb);
+			for (int32_t i = 0; i < in.nb; ++i)
+			{
+				const D* x = cur.data() + (size_t)(b0 + i) * kOuts;
+				S.x[i] = dd::v3<D>(x[0], x[1], x[2]);
+				S.q[i] = { x[3], x[4], x[5], x[6] };
+				S.v[i] = dd::v3<D>(x[7], x[8], x[9]);
+				S.w[i] = dd::v3<D>(x[10], x[11], x[12]);
+				float4 Pp = in.pos[i], IL = in.invIL[i];
+				D invMass = dd::dual<P, double>((P)Pp.w, 0.0);
+				dd::V3<D> il = dd::v3<D>(dd::dual<P, double>((P)IL.x, 0.0), dd::dual<P, double>((P)IL.y, 0.0),
+										 dd::dual<P, double>((P)IL.z, 0.0));
+				if (Pp.w != 0.0f && mass != nullptr)
+				{
+					// inverse mass and inertia scale as 1 / mass
+					D m = dd::dual<P, double>((P)mass[b0 + i], dMass[b0 + i]);
+					D kk = (P)in.angVel[i].w / m;
+					invMass = (P)1 / m;
+					il = dd::v3<D>(il.x * kk, il.y * kk, il.z * kk);
+				}
+				else if (Pp.w != 0.0f && in.angVel[i].w > 0.0f)
+				{
+					// the recorded values, with d/dm = -value / m
+					double m = in.angVel[i].w, dm = dMass[b0 + i];
+					invMass.d = -(double)invMass.v / m * dm;
+					il.x.d = -(double)il.x.v / m * dm;
+					il.y.d = -(double)il.y.v / m * dm;
+					il.z.d = -(double)il.z.v / m * dm;
+				}
+				S.invMass[i] = invMass;
+				S.il[i] = il;
+				bmu[i] = dd::dual<P, double>(friction != nullptr ? (P)friction[b0 + i] : (P)in.bmat[i].y,
+											 dFriction[b0 + i]);
+				S.iw[i] = dd::inv_inertia_world(S.q[i], S.il[i]);
+				S.px[i] = S.x[i];
+				S.pq[i] = S.q[i];
+				S.pv[i] = S.v[i];
+				S.pw[i] = S.w[i];
+			}
+			for (int32_t c = 0; c < in.nc; ++c)
+			{
+				int2 ab = in.bodies[c];
+				S.mu[c] = dd::sqrt_(bmu[ab.x] * bmu[ab.y]);
+			}
+			dd::solve_step(in, S, iterations);
+			for (int32_t i = 0; i < in.nb; ++i)
+			{
+				D* o = cur.data() + (size_t)(b0 + i) * kOuts;
+				D vals[kOuts] = { S.x[i].x, S.x[i].y, S.x[i].z, S.q[i].x, S.q[i].y, S.q[i].z, S.q[i].w,
+								  S.v[i].x, S.v[i].y, S.v[i].z, S.w[i].x, S.w[i].y, S.w[i].z };
+				for (int q = 0; q < kOuts; ++q)
+					o[q] = vals[q];
+			}
+		}
+	}
+	for (size_t k = 0; k < cur.size(); ++k)
+	{
+		stateOut[k] = (double)cur[k].v;
+		dStateOut[k] = cur[k].d;
+	}
+	return 0;
 }
 
 } // namespace mx
@@ -856,7 +792,7 @@ int m3dx_diff_replay(m3d_world* world, int first, int count, int precision, cons
 // respect to the initial velocities) and the gradients with respect to each
 // body's mass and friction (summed over the steps). Any pointer but seed may
 // be null. Returns 0, or -1 (m3dx_last_error) on tape overflow or error.
-int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
+int m3dx_diff_backward_tape(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
 {
 	World& w = world->w;
 	DiffState* D = w.diff;
@@ -926,6 +862,33 @@ int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, flo
 	if (frictionGrad != nullptr)
 		memcpy(frictionGrad, fg.data(), fg.size() * sizeof(float));
 	return 0;
+}
+
+// Forward tangents of the recorded steps first .. first + count - 1 on the
+// host CPU (the same solve templates as the device replays) from stateIn with
+// per-body mass and friction (null: the recorded values), along one
+// direction (dStateIn, dMass, dFriction); derivatives in double. primal 0:
+// values in float, the engine's own path (the float steps' derivative in
+// double precision); 1: values in double (the FP64 replay, whose path may
+// leave the engine's where a branch is close). stateOut and dStateOut
+// receive the final state and its directional derivative (13 per body). Host
+// arrays of doubles; returns 0 or -1.
+int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal, const double* stateIn,
+					  const double* dStateIn, const double* mass, const double* dMass, const double* friction,
+					  const double* dFriction, double* stateOut, double* dStateOut)
+{
+	World& w = world->w;
+	DiffState* Ds = w.diff;
+	if (Ds == nullptr || first < 0 || count < 1 || first + count > (int)Ds->steps.size() || stateIn == nullptr ||
+		dStateIn == nullptr || dMass == nullptr || dFriction == nullptr || stateOut == nullptr || dStateOut == nullptr)
+	{
+		set_error("jvp: no recorded steps %d..%d or a null array", first, first + count - 1);
+		return -1;
+	}
+	if (primal)
+		return jvp_host<double>(w, first, count, stateIn, dStateIn, mass, dMass, friction, dFriction, stateOut,
+								dStateOut);
+	return jvp_host<float>(w, first, count, stateIn, dStateIn, mass, dMass, friction, dFriction, stateOut, dStateOut);
 }
 
 } // extern "C"

@@ -20,6 +20,13 @@
 
 #include <math.h>
 
+// Loops over a manifold's (at most four) points; unrolling them would let
+// fixed-size point arrays live in registers, at a large compile-time cost for
+// the taped instantiation.
+#ifndef MX_UNROLL_POINTS
+#define MX_UNROLL_POINTS
+#endif
+
 namespace mx
 {
 namespace dd
@@ -252,6 +259,140 @@ template <class F>
 __device__ __forceinline__ Ad<F> fabs_(Ad<F> x)
 {
 	return ad_make<F>(x.v < (F)0 ? -x.v : x.v, x.id, x.v < (F)0 ? (F)-1 : (F)1, kNoNode, (F)0);
+}
+
+// ------------------------------------------------------------ forward tangents
+
+// Forward-mode scalar: a value and its derivative along one input
+// direction. The value part is computed by exactly the plain expressions of
+// its type F, so a Dual replay takes the same branches as the plain one; the
+// derivative may be carried in a wider type G (float values with double
+// derivatives differentiate the float path in double precision).
+template <class F, class G = F>
+struct Dual
+{
+	F v;
+	G d;
+};
+
+template <class F, class G = F>
+MX_HD Dual<F, G> dual(F v, G d)
+{
+	Dual<F, G> r;
+	r.v = v;
+	r.d = d;
+	return r;
+}
+
+template <class F, class G>
+MX_HD Dual<F, G> operator+(Dual<F, G> x, Dual<F, G> y)
+{
+	return dual<F, G>(x.v + y.v, x.d + y.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator-(Dual<F, G> x, Dual<F, G> y)
+{
+	return dual<F, G>(x.v - y.v, x.d - y.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator*(Dual<F, G> x, Dual<F, G> y)
+{
+	return dual<F, G>(x.v * y.v, x.d * (G)y.v + (G)x.v * y.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator/(Dual<F, G> x, Dual<F, G> y)
+{
+	F r = x.v / y.v;
+	return dual<F, G>(r, (x.d - (G)r * y.d) / (G)y.v);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator-(Dual<F, G> x)
+{
+	return dual<F, G>(-x.v, -x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator+(Dual<F, G> x, F c)
+{
+	return dual<F, G>(x.v + c, x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator+(F c, Dual<F, G> x)
+{
+	return dual<F, G>(c + x.v, x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator-(Dual<F, G> x, F c)
+{
+	return dual<F, G>(x.v - c, x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator-(F c, Dual<F, G> x)
+{
+	return dual<F, G>(c - x.v, -x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator*(Dual<F, G> x, F c)
+{
+	return dual<F, G>(x.v * c, x.d * (G)c);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator*(F c, Dual<F, G> x)
+{
+	return dual<F, G>(c * x.v, (G)c * x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator/(Dual<F, G> x, F c)
+{
+	return dual<F, G>(x.v / c, x.d / (G)c);
+}
+template <class F, class G>
+MX_HD Dual<F, G> operator/(F c, Dual<F, G> x)
+{
+	F r = c / x.v;
+	return dual<F, G>(r, -(G)r * x.d / (G)x.v);
+}
+
+template <class F, class G>
+MX_HD F val(const Dual<F, G>& x)
+{
+	return x.v;
+}
+
+template <class P, class G>
+struct Num<Dual<P, G>>
+{
+	typedef P F;
+	static MX_HD Dual<P, G> c(double x) { return dual<P, G>((P)x, (G)0); }
+};
+
+template <class F, class G>
+MX_HD Dual<F, G> sqrt_(Dual<F, G> x)
+{
+	F r = sqrt_(x.v);
+	return dual<F, G>(r, r > (F)0 ? x.d * ((G)0.5 / (G)r) : (G)0);
+}
+template <class F, class G>
+MX_HD Dual<F, G> sin_(Dual<F, G> x)
+{
+	return dual<F, G>(sin_(x.v), cos_((G)x.v) * x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> cos_(Dual<F, G> x)
+{
+	return dual<F, G>(cos_(x.v), -sin_((G)x.v) * x.d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> atan2_(Dual<F, G> y, Dual<F, G> x)
+{
+	G xv = (G)x.v, yv = (G)y.v;
+	G r2 = xv * xv + yv * yv;
+	G d = r2 > (G)0 ? (xv * y.d - yv * x.d) / r2 : (G)0;
+	return dual<F, G>(atan2_(y.v, x.v), d);
+}
+template <class F, class G>
+MX_HD Dual<F, G> fabs_(Dual<F, G> x)
+{
+	return x.v < (F)0 ? dual<F, G>(-x.v, -x.d) : x;
 }
 
 // ------------------------------------------------------------ vectors
@@ -488,30 +629,27 @@ MX_HD V3<T> lift(float4 v)
 MX_HD bool is_dynamic(const StepIn& in, int32_t i) { return bf_type(in.flags[i]) == M3D_BODY_DYNAMIC; }
 
 // ------------------------------------------------------------ stages
+//
+// Each stage is written once on values: one body's or one contact's inputs
+// and outputs in registers. The array-based replay below (solve_step) loads
+// them from a world's state and stores them back; the per-contact adjoint
+// evaluates the same functions on forward tangents.
 
 // IntegrateBodySubstep (src/xpbd.cpp:587-705) for a body without forces,
 // torques or gyroscopic term (isotropic inverse inertia); mx_solver.h
-// integrate_body.
+// integrate_body. The previous-substep copies are the inputs themselves.
 template <class T>
-MX_HD void integrate(const StepIn& in, StepState<T>& S, int32_t i, float h)
+MX_HD void integrate_op(V3<T>& x, Q4<T>& q, V3<T>& v, V3<T>& w, V3<T> il, M3<T>& iw, float h, float gravityScale,
+						float linDamp, float angDamp, m3d_vec3 gravity)
 {
 	typedef typename Num<T>::F F;
-	if (!is_dynamic(in, i))
-		return;
-	float gravityScale = in.vel[i].w;
-	float linDamp = in.invIL[i].w;
-	float angDamp = in.angDamp[i];
-	S.pv[i] = S.v[i];
-	S.pw[i] = S.w[i];
-	V3<T> v = mul_add(S.v[i], (F)(h * gravityScale), v3<T>(Num<T>::c(in.gravity.x), Num<T>::c(in.gravity.y),
-															 Num<T>::c(in.gravity.z)));
-	v = scale((F)(1.0f / (1.0f + h * linDamp)), v);
-	V3<T> w = scale((F)(1.0f / (1.0f + h * angDamp)), S.w[i]);
-	S.px[i] = S.x[i];
-	S.pq[i] = S.q[i];
-	V3<T> x = mul_add(S.x[i], (F)h, v);
-	Q4<T> rot = S.q[i];
-	T wlen = length(w);
+	V3<T> vn = mul_add(v, (F)(h * gravityScale), v3<T>(Num<T>::c(gravity.x), Num<T>::c(gravity.y),
+														Num<T>::c(gravity.z)));
+	vn = scale((F)(1.0f / (1.0f + h * linDamp)), vn);
+	V3<T> wn = scale((F)(1.0f / (1.0f + h * angDamp)), w);
+	V3<T> xn = mul_add(x, (F)h, vn);
+	Q4<T> rot = q;
+	T wlen = length(wn);
 	if (val(wlen) > (F)1e-9)
 	{
 		T half = (F)(0.5f * h) * wlen;
@@ -528,14 +666,14 @@ MX_HD void integrate(const StepIn& in, StepState<T>& S, int32_t i, float h)
 			ch = cos_(half);
 		}
 		T s = sh / wlen;
-		Q4<T> dq = { w.x * s, w.y * s, w.z * s, ch };
+		Q4<T> dq = { wn.x * s, wn.y * s, wn.z * s, ch };
 		rot = renorm_fast(quat_mul(dq, rot));
 	}
-	S.iw[i] = inv_inertia_world(rot, S.il[i]);
-	S.x[i] = x;
-	S.q[i] = rot;
-	S.v[i] = v;
-	S.w[i] = w;
+	iw = inv_inertia_world(rot, il);
+	x = xn;
+	q = rot;
+	v = vn;
+	w = wn;
 }
 
 // contact arm of src/xpbd.cpp:100-116 (mx_solver.h contact_arm)
@@ -565,61 +703,64 @@ struct Body
 	M3<T> iw;
 };
 
-template <class T>
-MX_HD Body<T> load_body(const StepState<T>& S, int32_t i)
+// Shape and manifold data of one contact in one substep (plain floats).
+struct ContactGeo
 {
-	Body<T> b;
-	b.p = S.x[i];
-	b.q = S.q[i];
-	b.v = S.v[i];
-	b.w = S.w[i];
-	b.invMass = S.invMass[i];
-	b.iw = S.iw[i];
-	return b;
-}
+	float4 n;     // normal xyz, point count bits
+	float4 mat;   // friction, restitution, rolling, CF_* bits
+	float4 pA[4]; // localAnchorA, separationOffset
+	float4 pB[4]; // localAnchorB
+	float radiusA, radiusB, hhA, hhB;
+	int tA, tB;
+	int count;
+	bool dynA, dynB;
+};
 
-template <class T>
-MX_HD void store_body(StepState<T>& S, int32_t i, const Body<T>& b, const StepIn& in)
+MX_HD ContactGeo contact_geo(const StepIn& in, int32_t c)
 {
-	if (!is_dynamic(in, i))
-		return; // static and kinematic bodies keep their state
-	S.x[i] = b.p;
-	S.q[i] = b.q;
-	S.v[i] = b.v;
-	S.w[i] = b.w;
-}
-
-// freeze arms, Jacobians and normal mass (src/xpbd.cpp:845-861); the
-// manifold is the recorded one (no re-collision)
-template <class T>
-MX_HD void prepare(const StepIn& in, StepState<T>& S, int32_t c)
-{
+	ContactGeo g;
 	int2 ab = in.bodies[c];
-	int32_t ia = ab.x, ib = ab.y;
-	float4 nc = in.normal[c];
-	V3<T> n = lift<T>(nc);
-	int32_t count = f2i(nc.w);
-	T imA = S.invMass[ia], imB = S.invMass[ib];
-	float4 SA = in.shape[ia], SB = in.shape[ib];
-	int tA = bf_shape(in.flags[ia]), tB = bf_shape(in.flags[ib]);
-	float hhA = in.bmat[ia].x, hhB = in.bmat[ib].x;
-	for (int j = 0; j < count; ++j)
+	g.n = in.normal[c];
+	g.mat = in.cmat[c];
+	g.count = f2i(g.n.w);
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = c * 4 + j;
-		V3<T> rA = contact_arm(tA, SA.w, hhA, S.q[ia], in.pA[p], n, 1.0f);
-		V3<T> rB = contact_arm(tB, SB.w, hhB, S.q[ib], in.pB[p], n, -1.0f);
-		V3<T> crA = cross(rA, n);
-		V3<T> crB = cross(rB, n);
-		V3<T> jA = mat3_mulv(S.iw[ia], crA);
-		V3<T> jB = mat3_mulv(S.iw[ib], crB);
-		S.wN[p] = imA + imB + dot(crA, jA) + dot(crB, jB);
-		S.jA[p] = jA;
-		S.jB[p] = jB;
-		S.rA[p] = rA;
-		S.rB[p] = rB;
-		S.lambdaN[p] = Num<T>::c(0.0);
-		S.lambdaT[p] = Num<T>::c(0.0);
+		g.pA[j] = in.pA[c * 4 + j];
+		g.pB[j] = in.pB[c * 4 + j];
 	}
+	g.radiusA = in.shape[ab.x].w;
+	g.radiusB = in.shape[ab.y].w;
+	g.hhA = in.bmat[ab.x].x;
+	g.hhB = in.bmat[ab.y].x;
+	g.tA = bf_shape(in.flags[ab.x]);
+	g.tB = bf_shape(in.flags[ab.y]);
+	g.dynA = is_dynamic(in, ab.x);
+	g.dynB = is_dynamic(in, ab.y);
+	return g;
+}
+
+// Frozen arms, Jacobians and normal mass of one point (src/xpbd.cpp:845-861)
+template <class T>
+struct PointPrep
+{
+	V3<T> rA, rB, jA, jB;
+	T wN;
+};
+
+template <class T>
+MX_HD PointPrep<T> prepare_point(const ContactGeo& g, int j, Q4<T> qa, Q4<T> qb, const M3<T>& iwa, const M3<T>& iwb,
+								 T imA, T imB)
+{
+	V3<T> n = lift<T>(g.n);
+	PointPrep<T> r;
+	r.rA = contact_arm(g.tA, g.radiusA, g.hhA, qa, g.pA[j], n, 1.0f);
+	r.rB = contact_arm(g.tB, g.radiusB, g.hhB, qb, g.pB[j], n, -1.0f);
+	V3<T> crA = cross(r.rA, n);
+	V3<T> crB = cross(r.rB, n);
+	r.jA = mat3_mulv(iwa, crA);
+	r.jB = mat3_mulv(iwb, crB);
+	r.wN = imA + imB + dot(crA, r.jA) + dot(crB, r.jB);
+	return r;
 }
 
 // src/xpbd.cpp:45-58
@@ -633,47 +774,34 @@ MX_HD void axis_correction(V3<T>& pos, Q4<T>& rot, T invMass, T dl, V3<T> n, V3<
 	rot = rotate_by(rot, scale(dl, angJ));
 }
 
-// SolveContactNormals, serial form (src/xpbd.cpp:122-163)
+// SolveContactNormals, serial form (src/xpbd.cpp:122-163): the contact's
+// points in order against the two bodies' positions and rotations.
 template <class T>
-MX_HD void normals(const StepIn& in, StepState<T>& S, int32_t c, float maxCorrection)
+MX_HD void normals_op(const ContactGeo& g, V3<T>& pa, Q4<T>& qa, V3<T>& pb, Q4<T>& qb, T imA, T imB, T* lambdaN,
+					  const T* wN, const V3<T>* jA, const V3<T>* jB, float maxCorrection)
 {
 	typedef typename Num<T>::F F;
-	int2 ab = in.bodies[c];
-	int32_t ia = ab.x, ib = ab.y;
-	float4 nc = in.normal[c];
-	V3<T> n = lift<T>(nc);
-	int32_t count = f2i(nc.w);
-	V3<T> pa = S.x[ia], pb = S.x[ib];
-	Q4<T> qa = S.q[ia], qb = S.q[ib];
-	T imA = S.invMass[ia], imB = S.invMass[ib];
-	for (int j = 0; j < count; ++j)
+	V3<T> n = lift<T>(g.n);
+MX_UNROLL_POINTS
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = c * 4 + j;
-		T wN = S.wN[p];
-		if (!(val(wN) > (F)0))
+		if (j >= g.count)
+			break;
+		T w = wN[j];
+		if (!(val(w) > (F)0))
 			continue;
-		float4 a4 = in.pA[p];
+		float4 a4 = g.pA[j];
 		V3<T> rA = rotate(qa, lift<T>(a4));
-		V3<T> rB = rotate(qb, lift<T>(in.pB[p]));
+		V3<T> rB = rotate(qb, lift<T>(g.pB[j]));
 		T sep = Num<T>::c(a4.w) + dot(sub(add(pb, rB), add(pa, rA)), n);
 		if (!(val(sep) < (F)0))
 			continue;
 		if (val(sep) < (F)(-maxCorrection))
 			sep = Num<T>::c(-maxCorrection);
-		T dl = -sep / wN;
-		S.lambdaN[p] = S.lambdaN[p] + dl;
-		axis_correction(pb, qb, imB, dl, n, S.jB[p]);
-		axis_correction(pa, qa, imA, -dl, n, S.jA[p]);
-	}
-	if (is_dynamic(in, ia))
-	{
-		S.x[ia] = pa;
-		S.q[ia] = qa;
-	}
-	if (is_dynamic(in, ib))
-	{
-		S.x[ib] = pb;
-		S.q[ib] = qb;
+		T dl = -sep / w;
+		lambdaN[j] = lambdaN[j] + dl;
+		axis_correction(pb, qb, imB, dl, n, jB[j]);
+		axis_correction(pa, qa, imA, -dl, n, jA[j]);
 	}
 }
 
@@ -697,42 +825,44 @@ MX_HD void position_correction(Body<T>& b, V3<T> p, V3<T> r)
 	b.q = rotate_by(b.q, dw);
 }
 
-// ApplyContactFriction (src/xpbd.cpp:172-269), box-box pairs only
+// ApplyContactFriction (src/xpbd.cpp:172-269), box-box pairs only; rA0 and
+// rB0 are the frozen arms of point 0 (the speed gate), fA/fB the friction
+// anchors (re-anchoring replaces them by the contact anchors).
 template <class T>
-MX_HD void friction(const StepIn& in, StepState<T>& S, int32_t c)
+MX_HD void friction_op(const ContactGeo& g, Body<T>& a, Body<T>& b, T mu, const T* lambdaN, T* lambdaT, V3<T> rA0,
+					   V3<T> rB0, float4* fA, float4* fB)
 {
 	typedef typename Num<T>::F F;
-	int2 ab = in.bodies[c];
-	float4 nc = in.normal[c];
-	int32_t count = f2i(nc.w);
-	if (count == 0)
+	if (g.count == 0)
 		return;
-	Body<T> a = load_body(S, ab.x), b = load_body(S, ab.y);
-	V3<T> n = lift<T>(nc);
-	T mu = S.mu[c];
+	V3<T> n = lift<T>(g.n);
 	{
 		const F maxSpeed2 = (F)(0.1f * 0.1f);
-		V3<T> rA0 = S.rA[c * 4], rB0 = S.rB[c * 4];
 		V3<T> vrel = sub(add(b.v, cross(b.w, rB0)), add(a.v, cross(a.w, rA0)));
 		V3<T> vt = sub(vrel, scale(dot(vrel, n), n));
 		if (val(length_sq(vt)) > maxSpeed2)
 		{
-			for (int j = 0; j < count; ++j)
+MX_UNROLL_POINTS
+			for (int j = 0; j < 4; ++j)
 			{
-				S.fA[c * 4 + j] = in.pA[c * 4 + j];
-				S.fB[c * 4 + j] = in.pB[c * 4 + j];
+				if (j >= g.count)
+					break;
+				fA[j] = g.pA[j];
+				fB[j] = g.pB[j];
 			}
 			return;
 		}
 	}
-	for (int j = 0; j < count; ++j)
+MX_UNROLL_POINTS
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = c * 4 + j;
-		T lambdaN = S.lambdaN[p];
-		if (val(lambdaN) <= (F)0)
+		if (j >= g.count)
+			break;
+		T lN = lambdaN[j];
+		if (val(lN) <= (F)0)
 			continue;
-		V3<T> rA = rotate(a.q, lift<T>(S.fA[p]));
-		V3<T> rB = rotate(b.q, lift<T>(S.fB[p]));
+		V3<T> rA = rotate(a.q, lift<T>(fA[j]));
+		V3<T> rB = rotate(b.q, lift<T>(fB[j]));
 		V3<T> gap = sub(add(b.p, rB), add(a.p, rA));
 		V3<T> dpt = sub(gap, scale(dot(gap, n), n));
 		T lt = length(dpt);
@@ -743,37 +873,32 @@ MX_HD void friction(const StepIn& in, StepState<T>& S, int32_t c)
 		if (val(wt) <= (F)0)
 			continue;
 		T needed = (F)kFrictionRelaxation * (lt / wt);
-		T lambdaT = S.lambdaT[p];
-		T allowed = mu * lambdaN - lambdaT;
+		T lT = lambdaT[j];
+		T allowed = mu * lN - lT;
 		if (val(allowed) <= (F)0)
 			continue;
 		T dL = val(needed) < val(allowed) ? needed : allowed;
-		S.lambdaT[p] = lambdaT + dL;
+		lambdaT[j] = lT + dL;
 		V3<T> pt = scale(dL, t);
 		position_correction(b, neg(pt), rB);
 		position_correction(a, pt, rA);
 		if (val(needed) > (F)2 * val(allowed))
 		{
-			S.fA[p] = in.pA[p];
-			S.fB[p] = in.pB[p];
+			fA[j] = g.pA[j];
+			fB[j] = g.pB[j];
 		}
 	}
-	store_body(S, ab.x, a, in);
-	store_body(S, ab.y, b, in);
 }
 
 // ReconstructBodyVelocity (src/xpbd.cpp:865-892)
 template <class T>
-MX_HD void reconstruct(const StepIn& in, StepState<T>& S, int32_t i, float invH)
+MX_HD void reconstruct_op(V3<T> x, Q4<T> q, V3<T> px, Q4<T> pq, float invH, V3<T>& v, V3<T>& w)
 {
 	typedef typename Num<T>::F F;
-	if (!is_dynamic(in, i))
-		return;
-	V3<T> v = scale((F)invH, sub(S.x[i], S.px[i]));
-	Q4<T> dq = quat_mul(S.q[i], quat_conj(S.pq[i]));
+	v = scale((F)invH, sub(x, px));
+	Q4<T> dq = quat_mul(q, quat_conj(pq));
 	F sign = val(dq.w) >= (F)0 ? (F)1 : (F)-1;
 	T v2 = dq.x * dq.x + dq.y * dq.y + dq.z * dq.z;
-	V3<T> w;
 	if (val(v2) > (F)4e-4)
 	{
 		T vlen = sqrt_(v2);
@@ -785,8 +910,6 @@ MX_HD void reconstruct(const StepIn& in, StepState<T>& S, int32_t i, float invH)
 		T corr = (F)1 + v2 * (F)(1.0f / 6.0f);
 		w = scale((F)2 * (F)invH * sign * corr, v3<T>(dq.x, dq.y, dq.z));
 	}
-	S.v[i] = v;
-	S.w[i] = w;
 }
 
 // src/xpbd.cpp:61-82
@@ -811,42 +934,36 @@ MX_HD void velocity_impulse(Body<T>& b, V3<T> p, V3<T> r)
 }
 
 // SolveContactVelocity (src/xpbd.cpp:275-407): dynamic friction, restitution
-// (applied unconditionally), rolling resistance
+// (applied unconditionally), rolling resistance. pv*/pw* are the bodies'
+// velocities at the substep start, slip the points' slip directions.
 template <class T>
-MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float gravityLen)
+MX_HD void velocity_op(const ContactGeo& g, Body<T>& a, Body<T>& b, V3<T> pvA, V3<T> pwA, V3<T> pvB, V3<T> pwB, T mu,
+					   const T* lambdaNs, const PointPrep<T>* pp, float4* slip, float h, float gravityLen)
 {
 	typedef typename Num<T>::F F;
-	int2 ab = in.bodies[c];
-	int32_t ia = ab.x, ib = ab.y;
-	float4 nc = in.normal[c], mat = in.cmat[c];
-	V3<T> n = lift<T>(nc);
-	int32_t count = f2i(nc.w);
-	T mu = S.mu[c];
-	float e0 = mat.y, rr = mat.z;
-	bool curved = (f2u(mat.w) & CF_CURVED) != 0;
+	V3<T> n = lift<T>(g.n);
+	float e0 = g.mat.y, rr = g.mat.z;
+	bool curved = (f2u(g.mat.w) & CF_CURVED) != 0;
 	float invH = h > 0.0f ? 1.0f / h : 0.0f;
-	Body<T> a = load_body(S, ia), b = load_body(S, ib);
-	V3<T> pvA = S.pv[ia], pwA = S.pw[ia], pvB = S.pv[ib], pwB = S.pw[ib];
-	float4 SA = in.shape[ia], SB = in.shape[ib];
-	int tA = bf_shape(in.flags[ia]), tB = bf_shape(in.flags[ib]);
-	float hhA = in.bmat[ia].x, hhB = in.bmat[ib].x;
 	T lambdaNTotal = Num<T>::c(0.0);
-	for (int j = 0; j < count; ++j)
+MX_UNROLL_POINTS
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = c * 4 + j;
-		T lambdaN = S.lambdaN[p];
+		if (j >= g.count)
+			break;
+		T lambdaN = lambdaNs[j];
 		if (val(lambdaN) <= (F)0)
 			continue;
 		V3<T> rAf, rBf;
 		if (curved)
 		{
-			rAf = contact_arm(tA, SA.w, hhA, a.q, in.pA[p], n, 1.0f);
-			rBf = contact_arm(tB, SB.w, hhB, b.q, in.pB[p], n, -1.0f);
+			rAf = contact_arm(g.tA, g.radiusA, g.hhA, a.q, g.pA[j], n, 1.0f);
+			rBf = contact_arm(g.tB, g.radiusB, g.hhB, b.q, g.pB[j], n, -1.0f);
 		}
 		else
 		{
-			rAf = S.rA[p];
-			rBf = S.rB[p];
+			rAf = pp[j].rA;
+			rBf = pp[j].rB;
 		}
 		V3<T> vrel = sub(add(b.v, cross(b.w, rBf)), add(a.v, cross(a.w, rAf)));
 		T vn = dot(vrel, n);
@@ -859,12 +976,12 @@ MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float
 			if (val(wt) > (F)0)
 			{
 				const F staticSlip = (F)0.08;
-				float4 sl = S.slip[p];
+				float4 sl = slip[j];
 				bool coherent = val(t.x) * (F)sl.x + val(t.y) * (F)sl.y + val(t.z) * (F)sl.z > (F)0.7;
 				T pStop = vtLen / wt;
 				T pMax = mu * lambdaN * (F)invH;
 				T P = !coherent && val(vtLen) < staticSlip ? pStop : (val(pStop) < val(pMax) ? pStop : pMax);
-				S.slip[p] = make_float4((float)val(t.x), (float)val(t.y), (float)val(t.z), 0.0f);
+				slip[j] = make_float4((float)val(t.x), (float)val(t.y), (float)val(t.z), 0.0f);
 				V3<T> imp = scale(-P, t);
 				velocity_impulse(b, imp, rBf);
 				velocity_impulse(a, neg(imp), rAf);
@@ -877,12 +994,12 @@ MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float
 		vn = dot(vrel, n);
 		T target = -e * vnPrev;
 		target = val(target) > (F)0 ? target : Num<T>::c(0.0);
-		T wN = S.wN[p];
+		T wN = pp[j].wN;
 		if (val(wN) > (F)0)
 		{
 			T jj = (target - vn) / wN;
-			axis_velocity(b, jj, n, S.jB[p]);
-			axis_velocity(a, -jj, n, S.jA[p]);
+			axis_velocity(b, jj, n, pp[j].jB);
+			axis_velocity(a, -jj, n, pp[j].jA);
 		}
 		lambdaNTotal = lambdaNTotal + (val(lambdaN) > (F)0 ? lambdaN : Num<T>::c(0.0));
 	}
@@ -905,6 +1022,173 @@ MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float
 			}
 		}
 	}
+}
+
+// ------------------------------------------------------------ world replay
+
+template <class T>
+MX_HD void integrate(const StepIn& in, StepState<T>& S, int32_t i, float h)
+{
+	if (!is_dynamic(in, i))
+		return;
+	S.px[i] = S.x[i];
+	S.pq[i] = S.q[i];
+	S.pv[i] = S.v[i];
+	S.pw[i] = S.w[i];
+	V3<T> x = S.x[i], v = S.v[i], w = S.w[i];
+	Q4<T> q = S.q[i];
+	M3<T> iw;
+	integrate_op(x, q, v, w, S.il[i], iw, h, in.vel[i].w, in.invIL[i].w, in.angDamp[i], in.gravity);
+	S.iw[i] = iw;
+	S.x[i] = x;
+	S.q[i] = q;
+	S.v[i] = v;
+	S.w[i] = w;
+}
+
+template <class T>
+MX_HD Body<T> load_body(const StepState<T>& S, int32_t i)
+{
+	Body<T> b;
+	b.p = S.x[i];
+	b.q = S.q[i];
+	b.v = S.v[i];
+	b.w = S.w[i];
+	b.invMass = S.invMass[i];
+	b.iw = S.iw[i];
+	return b;
+}
+
+template <class T>
+MX_HD void store_body(StepState<T>& S, int32_t i, const Body<T>& b, const StepIn& in)
+{
+	if (!is_dynamic(in, i))
+		return; // static and kinematic bodies keep their state
+	S.x[i] = b.p;
+	S.q[i] = b.q;
+	S.v[i] = b.v;
+	S.w[i] = b.w;
+}
+
+// freeze arms, Jacobians and normal mass; the manifold is the recorded one
+// (no re-collision)
+template <class T>
+MX_HD void prepare(const StepIn& in, StepState<T>& S, int32_t c)
+{
+	int2 ab = in.bodies[c];
+	ContactGeo g = contact_geo(in, c);
+	for (int j = 0; j < g.count; ++j)
+	{
+		int32_t p = c * 4 + j;
+		PointPrep<T> r = prepare_point(g, j, S.q[ab.x], S.q[ab.y], S.iw[ab.x], S.iw[ab.y], S.invMass[ab.x],
+									   S.invMass[ab.y]);
+		S.wN[p] = r.wN;
+		S.jA[p] = r.jA;
+		S.jB[p] = r.jB;
+		S.rA[p] = r.rA;
+		S.rB[p] = r.rB;
+		S.lambdaN[p] = Num<T>::c(0.0);
+		S.lambdaT[p] = Num<T>::c(0.0);
+	}
+}
+
+template <class T>
+MX_HD void normals(const StepIn& in, StepState<T>& S, int32_t c, float maxCorrection)
+{
+	int2 ab = in.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	ContactGeo g = contact_geo(in, c);
+	V3<T> pa = S.x[ia], pb = S.x[ib];
+	Q4<T> qa = S.q[ia], qb = S.q[ib];
+	T lN[4], wN[4];
+	V3<T> jA[4], jB[4];
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = c * 4 + (j < g.count ? j : 0);
+		lN[j] = S.lambdaN[p];
+		wN[j] = S.wN[p];
+		jA[j] = S.jA[p];
+		jB[j] = S.jB[p];
+	}
+	normals_op(g, pa, qa, pb, qb, S.invMass[ia], S.invMass[ib], lN, wN, jA, jB, maxCorrection);
+	for (int j = 0; j < g.count; ++j)
+		S.lambdaN[c * 4 + j] = lN[j];
+	if (g.dynA)
+	{
+		S.x[ia] = pa;
+		S.q[ia] = qa;
+	}
+	if (g.dynB)
+	{
+		S.x[ib] = pb;
+		S.q[ib] = qb;
+	}
+}
+
+template <class T>
+MX_HD void friction(const StepIn& in, StepState<T>& S, int32_t c)
+{
+	int2 ab = in.bodies[c];
+	ContactGeo g = contact_geo(in, c);
+	if (g.count == 0)
+		return;
+	Body<T> a = load_body(S, ab.x), b = load_body(S, ab.y);
+	T lN[4], lT[4];
+	float4 fA[4], fB[4];
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = c * 4 + (j < g.count ? j : 0);
+		lN[j] = S.lambdaN[p];
+		lT[j] = S.lambdaT[p];
+		fA[j] = S.fA[p];
+		fB[j] = S.fB[p];
+	}
+	friction_op(g, a, b, S.mu[c], lN, lT, S.rA[c * 4], S.rB[c * 4], fA, fB);
+	for (int j = 0; j < g.count; ++j)
+	{
+		S.lambdaT[c * 4 + j] = lT[j];
+		S.fA[c * 4 + j] = fA[j];
+		S.fB[c * 4 + j] = fB[j];
+	}
+	store_body(S, ab.x, a, in);
+	store_body(S, ab.y, b, in);
+}
+
+template <class T>
+MX_HD void reconstruct(const StepIn& in, StepState<T>& S, int32_t i, float invH)
+{
+	if (!is_dynamic(in, i))
+		return;
+	V3<T> v, w;
+	reconstruct_op(S.x[i], S.q[i], S.px[i], S.pq[i], invH, v, w);
+	S.v[i] = v;
+	S.w[i] = w;
+}
+
+template <class T>
+MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float gravityLen)
+{
+	int2 ab = in.bodies[c];
+	int32_t ia = ab.x, ib = ab.y;
+	ContactGeo g = contact_geo(in, c);
+	Body<T> a = load_body(S, ia), b = load_body(S, ib);
+	T lN[4];
+	PointPrep<T> pp[4];
+	float4 slip[4];
+	for (int j = 0; j < 4; ++j)
+	{
+		int32_t p = c * 4 + (j < g.count ? j : 0);
+		lN[j] = S.lambdaN[p];
+		pp[j].rA = S.rA[p];
+		pp[j].rB = S.rB[p];
+		pp[j].jA = S.jA[p];
+		pp[j].jB = S.jB[p];
+		pp[j].wN = S.wN[p];
+		slip[j] = S.slip[p];
+	}
+	velocity_op(g, a, b, S.pv[ia], S.pw[ia], S.pv[ib], S.pw[ib], S.mu[c], lN, pp, slip, h, gravityLen);
+	for (int j = 0; j < g.count; ++j)
+		S.slip[c * 4 + j] = slip[j];
 	store_body(S, ia, a, in);
 	store_body(S, ib, b, in);
 }
@@ -941,6 +1225,20 @@ MX_HD void solve_substep(const StepIn& in, StepState<T>& S, int iterations, floa
 		velocity(in, S, c, h, gravityLen);
 }
 
+// The manifold block of substep s (blocks of cStride contacts).
+MX_HD StepIn substep_in(const StepIn& in, int s)
+{
+	StepIn ins = in;
+	ins.normal = in.normal + s * in.cStride;
+	ins.cmat = in.cmat + s * in.cStride;
+	ins.pA = in.pA + 4 * s * in.cStride;
+	ins.pB = in.pB + 4 * s * in.cStride;
+	ins.pFA = in.pFA + 4 * s * in.cStride;
+	ins.pFB = in.pFB + 4 * s * in.cStride;
+	ins.pSlip = in.pSlip + 4 * s * in.cStride;
+	return ins;
+}
+
 // The step's solve: its substeps, each with its recorded manifold block.
 template <class T>
 MX_HD void solve_step(const StepIn& in, StepState<T>& S, int iterations)
@@ -950,17 +1248,7 @@ MX_HD void solve_step(const StepIn& in, StepState<T>& S, int iterations)
 	float maxCorrection = kMaxDepenetrationSpeed * h;
 	float gravityLen = m3d_length(in.gravity);
 	for (int s = 0; s < in.substeps; ++s)
-	{
-		StepIn ins = in;
-		ins.normal = in.normal + s * in.cStride;
-		ins.cmat = in.cmat + s * in.cStride;
-		ins.pA = in.pA + 4 * s * in.cStride;
-		ins.pB = in.pB + 4 * s * in.cStride;
-		ins.pFA = in.pFA + 4 * s * in.cStride;
-		ins.pFB = in.pFB + 4 * s * in.cStride;
-		ins.pSlip = in.pSlip + 4 * s * in.cStride;
-		solve_substep(ins, S, iterations, h, invH, maxCorrection, gravityLen);
-	}
+		solve_substep(substep_in(in, s), S, iterations, h, invH, maxCorrection, gravityLen);
 }
 
 } // namespace dd

@@ -163,8 +163,9 @@ def replay(L: Library, w, scene: DiffScene, steps: int, precision: int = 1, v0=N
     return out
 
 
-def backward(L: Library, w, scene: DiffScene, grad_final_pos: np.ndarray):
-    """(d/dv0, d/dfriction, d/dmass) of <grad_final_pos, final positions>."""
+def backward(L: Library, w, scene: DiffScene, grad_final_pos: np.ndarray, tape: bool = False):
+    """(d/dv0, d/dfriction, d/dmass) of <grad_final_pos, final positions>, by
+    the parallel adjoint (or, tape=True, by the reverse-mode tape)."""
     n = scene.body_count()
     idx = scene.dynamic_index()
     seed = np.zeros((n, OUT), dtype=np.float32)
@@ -173,7 +174,8 @@ def backward(L: Library, w, scene: DiffScene, grad_final_pos: np.ndarray):
     mg = np.zeros(n, dtype=np.float32)
     fg = np.zeros(n, dtype=np.float32)
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
-    if L.m3dx_diff_backward(w, ptr(seed), ptr(adj), ptr(mg), ptr(fg)) != 0:
+    fn = L.m3dx_diff_backward_tape if tape else L.m3dx_diff_backward
+    if fn(w, ptr(seed), ptr(adj), ptr(mg), ptr(fg)) != 0:
         raise RuntimeError("adjoint failed: " + L.last_error())
     return adj[idx, 7:10].astype(np.float64), fg[idx].astype(np.float64), mg[idx].astype(np.float64)
 

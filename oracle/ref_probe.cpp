@@ -545,6 +545,14 @@ int32_t m3dx_diff_steps(m3d_world*) { return 0; }
 int m3dx_diff_state(m3d_world*, int, float*) { return -1; }
 int m3dx_diff_replay(m3d_world*, int, int, int, const void*, const void*, const void*, void*) { return -1; }
 int m3dx_diff_backward(m3d_world*, const float*, float*, float*, float*) { return -1; }
+int m3dx_diff_backward_tape(m3d_world*, const float*, float*, float*, float*) { return -1; }
+int m3dx_diff_set_checkpoint(m3d_world*, int) { return -1; }
+int m3dx_diff_forward(m3d_world*, int, int, float*) { return -1; }
+int m3dx_diff_jvp_cpu(m3d_world*, int, int, int, const double*, const double*, const double*, const double*, const double*,
+                      const double*, double*, double*)
+{
+    return -1;
+}
 
 /* no device kernels on the CPU engine: timing is unavailable */
 int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
