@@ -166,13 +166,26 @@ def batch_loop_plain(dev, worlds: int, warmup: int, steps: int, substeps: int):
 
 def run_diff(args, dev):
     """Differentiable worlds: forward (the engine's steps, recording each
-    step's solve inputs) and backward (the adjoint of all recorded steps) of
-    the final-COM loss; plus the gradient check on 16 worlds over 3 steps."""
+    step's solve inputs) and backward (the parallel adjoint of all recorded
+    steps, states recomputed from checkpoints every 10 steps) of the
+    final-COM loss; the plain forward (no recording) for comparison; the
+    gradient checked on a 16-world subset against FP64 central differences
+    (3 steps, double replay) and against FP64 forward tangents on the host
+    CPU along the engine's path (all steps)."""
+    import ctypes as C
+
     import numpy as np
 
     from melon3d_amd import diff
 
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
     scene = diff.DiffScene(worlds=args.worlds)
+    w = scene.build(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.m3d_world_step(w, DT, args.substeps)
+    fwd_plain = time.perf_counter() - t0
+    dev.m3d_world_destroy(w)
     w = scene.build(dev)
     if dev.m3dx_diff_record(w, 1) != 0:
         raise SystemExit(dev.last_error())
@@ -189,20 +202,35 @@ def run_diff(args, dev):
     t0 = time.perf_counter()
     gv, gf, gm = diff.backward(dev, w, scene, g_pos)
     bwd = time.perf_counter() - t0
+    # the same backward again with HIP events around every adjoint launch
+    dev.m3dx_set_kernel_timing(w, 1)
+    diff.backward(dev, w, scene, g_pos)
+    kt = dev.kernel_timing(w)
+    dev.m3dx_set_kernel_timing(w, 0)
     dev.m3d_world_destroy(w)
-    # gradient check: per-world directional derivatives vs FP64 central differences
+    launches, kms, kbytes = kt["adjoint"]
+    avg_s = (kms / 1000.0) / launches if launches else 0.0
+    achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
+
+    def loss_grads(L, cw, cs, steps):
+        e = diff.final_positions(L, cw, cs)
+        cm = cs.mass[..., None]
+        msum = cs.mass.sum(1)[:, None, None]
+        ccom = (cm * e).sum(1) / cs.mass.sum(1)[:, None]
+        g_com = 2.0 * (ccom - cs.target)
+        cgv, cgf, cgm = diff.backward(L, cw, cs, g_com[:, None, :] * cm / msum)
+        return e, ccom, g_com, cgv, cgf, cgm
+
+    # gradient check 1: per-world directional derivatives vs FP64 central
+    # differences of the double replay, 3 steps
     cs = diff.DiffScene(worlds=16)
     cw = cs.build(dev)
     dev.m3dx_diff_record(cw, 1)
     for _ in range(3):
         dev.m3d_world_step(cw, DT, args.substeps)
     dev.m3dx_diff_record(cw, 0)
-    e = diff.final_positions(dev, cw, cs)
-    cm = cs.mass[..., None]
+    e, ccom, g_com, cgv, cgf, cgm = loss_grads(dev, cw, cs, 3)
     msum = cs.mass.sum(1)[:, None, None]
-    ccom = (cm * e).sum(1) / cs.mass.sum(1)[:, None]
-    g_com = 2.0 * (ccom - cs.target)
-    cgv, cgf, cgm = diff.backward(dev, cw, cs, g_com[:, None, :] * cm / msum)
     cgm = cgm + (g_com[:, None, :] * ((e - ccom[:, None, :]) / msum)).sum(2)
     rng = np.random.default_rng(11)
     dv, df, dm = rng.normal(size=cs.v0.shape), rng.normal(size=cs.friction.shape) * 0.05, rng.normal(
@@ -218,8 +246,47 @@ def run_diff(args, dev):
         return ((c - cs.target) ** 2).sum(1)
 
     fd = (losses(1e-4) - losses(-1e-4)) / 2e-4
-    rel = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-12)
+    rel_fd = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-12)
     dev.m3d_world_destroy(cw)
+
+    # gradient check 2: the same subset over all steps against FP64 forward
+    # tangents on the host CPU along the engine's path, directions in v0,
+    # friction and mass (the recorded values' derivatives)
+    cw = cs.build(dev)
+    dev.m3dx_diff_record(cw, 1)
+    for _ in range(args.steps):
+        dev.m3d_world_step(cw, DT, args.substeps)
+    dev.m3dx_diff_record(cw, 0)
+    e, ccom, g_com, cgv, cgf, cgm = loss_grads(dev, cw, cs, args.steps)
+    n = cs.body_count()
+    idx = cs.dynamic_index()
+    st0 = np.zeros((n, 13), np.float32)
+    dev.m3dx_diff_state(cw, 0, ptr(st0))
+    u = np.zeros((n, 13))
+    u[idx, 7:10] = dv
+    dmass = np.zeros(n)
+    dmass[idx] = dm
+    dfric = np.zeros(n)
+    dfric[idx] = df
+    out = np.zeros((n, 13))
+    dout = np.zeros((n, 13))
+    t_cpu = time.perf_counter()
+    if dev.m3dx_diff_jvp_cpu(cw, 0, args.steps, 0, ptr(st0.astype(np.float64)), ptr(u), None, ptr(dmass), None,
+                             ptr(dfric), ptr(out), ptr(dout)) != 0:
+        raise SystemExit(dev.last_error())
+    t_cpu = time.perf_counter() - t_cpu
+    dx = dout[idx, 0:3]
+    mm = cs.mass[..., None]
+    # loss along the direction: d|COM - t|^2 with COM = sum(m x) / sum(m), the
+    # recorded masses' own perturbation entering the COM weights too
+    dcom = (mm * dx).sum(1) / cs.mass.sum(1)[:, None] + (dm[..., None] * (e - ccom[:, None, :])).sum(1) / \
+        cs.mass.sum(1)[:, None]
+    jvp = (g_com * dcom).sum(1)
+    cgm_total = cgm + (g_com[:, None, :] * ((e - ccom[:, None, :]) / msum)).sum(2)
+    vjp = (cgv * dv).sum((1, 2)) + (cgf * df).sum(1) + (cgm_total * dm).sum(1)
+    rel_cpu = np.abs(vjp - jvp) / np.maximum(np.abs(jvp), 1e-12)
+    dev.m3d_world_destroy(cw)
+
     dyn = args.worlds * diff.BODIES
     out = {
         "metric": "body-steps/sec",
@@ -236,16 +303,24 @@ def run_diff(args, dev):
         "data": "synthetic",
         "config": {"workload": f"differentiable: {args.worlds} worlds x 64 bodies (spheres r 0.5, cubes he 0.5), "
                                f"v0 ~ N(0, 2), friction ~ U[0.2, 0.9], loss |COM - target|^2 after {args.steps} "
-                               f"steps, dt 1/60 s, {args.substeps} substeps",
+                               f"steps, dt 1/60 s, {args.substeps} substeps, checkpoints every 10 steps",
                    "worlds": args.worlds, "bodies": dyn, "parallelism": "single GPU"},
         "forward_s": fwd,
+        "forward_plain_s": fwd_plain,
         "backward_s": bwd,
         "backward_over_forward": bwd / fwd,
-        "grad_check": {"worlds": 16, "steps": 3, "h": 1e-4, "max_rel_err": float(rel.max()),
-                       "median_rel_err": float(np.median(rel))},
+        "backward_over_plain_forward": bwd / fwd_plain,
+        "grad_check": {"subset_worlds": 16,
+                       "fp64_central_differences": {"steps": 3, "h": 1e-4, "max_rel_err": float(rel_fd.max()),
+                                                    "median_rel_err": float(np.median(rel_fd))},
+                       "fp64_cpu_tangents": {"steps": args.steps, "max_rel_err": float(rel_cpu.max()),
+                                             "median_rel_err": float(np.median(rel_cpu)), "cpu_s": t_cpu}},
         "grad_norms": {"v0": float(np.linalg.norm(gv)), "friction": float(np.linalg.norm(gf)),
                        "mass": float(np.linalg.norm(gm))},
-        "roofline": None,
+        "roofline": {"bound": "hbm", "kernel": "adjoint", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": launches,
+                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": (kbytes / launches) if launches else 0.0},
+        "kernel_ms_backward": {n: v[1] for n, v in kt.items() if v[0] > 0},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -245,6 +245,8 @@ typedef enum m3dx_kernel
 	M3DX_K_JOINT_ISLANDS,     /* islands with joints (one lane each) */
 	M3DX_K_NARROW,            /* narrowphase */
 	M3DX_K_PAIRS,             /* pair counting and emission */
+	M3DX_K_ADJOINT,           /* differentiable worlds: a step's forward replay and reverse */
+	M3DX_K_ADJ_FORWARD,       /* differentiable worlds: forward replays inside a checkpoint segment */
 	M3DX_K_COUNT
 } m3dx_kernel;
 
@@ -326,8 +328,13 @@ M3D_API int m3dx_diff_replay(m3d_world* world, int first, int count, int precisi
    of up to 3072 solver contacts and 2048 bodies. */
 M3D_API int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad,
                                float* frictionGrad);
-/* The same product by a reverse-mode tape walked by one lane per world (a
-   second, independent evaluation, for checking). */
+/* m3dx_diff_backward with double arrays: the adjoint is evaluated in double
+   precision along the engine's float path (float values, double
+   derivatives); the float entry point rounds its results. */
+M3D_API int m3dx_diff_backward_f64(m3d_world* world, const double* seed, double* stateAdj, double* massGrad,
+                                   double* frictionGrad);
+/* The same product by a reverse-mode tape in float walked by one lane per
+   world (a second, independent evaluation, for checking). */
 M3D_API int m3dx_diff_backward_tape(m3d_world* world, const float* seed, float* stateAdj, float* massGrad,
                                     float* frictionGrad);
 /* Steps between body-state checkpoints of the backward pass (default 10).

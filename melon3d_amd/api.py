@@ -266,7 +266,8 @@ class m3dx_kernel_time(C.Structure):
     _fields_ = [("launches", C.c_int32), ("ms", C.c_float), ("bytes", C.c_double)]
 
 
-M3DX_KERNELS = ["grid_normals", "grid_friction", "grid_velocity", "grid_order", "joint_islands", "narrow", "pairs"]
+M3DX_KERNELS = ["grid_normals", "grid_friction", "grid_velocity", "grid_order", "joint_islands", "narrow", "pairs",
+                "adjoint", "adjoint_forward"]
 
 W = C.c_void_p  # m3d_world*
 overlap_fn = C.CFUNCTYPE(C.c_bool, m3d_body_id, C.c_void_p)
@@ -342,6 +343,7 @@ _SIGS = {
     "m3dx_diff_replay": (C.c_int, [W, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "m3dx_diff_backward": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "m3dx_diff_backward_tape": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "m3dx_diff_backward_f64": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "m3dx_diff_set_checkpoint": (C.c_int, [W, C.c_int]),
     "m3dx_diff_forward": (C.c_int, [W, C.c_int, C.c_int, C.c_void_p]),
     "m3dx_diff_jvp_cpu": (C.c_int, [W, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 8),

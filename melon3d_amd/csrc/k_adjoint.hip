@@ -50,14 +50,16 @@ using dd::M3;
 using dd::PointPrep;
 using dd::Q4;
 using dd::V3;
-typedef Dual<float> DF;
+typedef Dual<float, double> DF;
 typedef dd::Body<DF> DBody;
 typedef dd::Body<float> FBody;
 
 constexpr int kAdjThreads = 512;
-constexpr int kAdjTasks = 4096;      // results of one chunk of (op, component) tasks
+constexpr int kAdjTasks = 2048;      // results of one chunk of (op, component) tasks
 constexpr int kAdjMaxContacts = 3072; // per world (level lists live in LDS)
 constexpr int kAdjMaxBodies = 2048;
+// a workgroup's static LDS stays within 64 KiB
+static_assert(kAdjTasks * 8 + (3 * kAdjMaxContacts + 2 + kAdjMaxBodies + 1) * 4 <= 65536, "adjoint LDS");
 
 // per-op input components
 constexpr int kDimIntegrate = 16;   // x q v w (13), inverse local inertia (3)
@@ -87,8 +89,9 @@ struct AdjLayout
 	int64_t st, im, il, mu, lamN;
 	int64_t rbPre, rbPost, rbIw, rbFric;
 	int64_t rcPrep, rcNrm, rcLamN, rcFric, rcVel;
-	int64_t aSt, aPrev, aIw, aIm, aIl, aMu, aLamN, aPrep;
-	int64_t total;
+	int64_t total;                                      // floats
+	int64_t aSt, aPrev, aIw, aIm, aIl, aMu, aLamN, aPrep; // in the double slab
+	int64_t dtotal;                                     // doubles
 };
 
 static AdjLayout adj_layout(int32_t MB, int32_t MC, int S, int I)
@@ -119,6 +122,8 @@ static AdjLayout adj_layout(int32_t MB, int32_t MC, int S, int I)
 	L.rcLamN = take(S * C * 4);
 	L.rcFric = take(S * C * 14);
 	L.rcVel = take(S * C * 20);
+	L.total = o;
+	o = 0;
 	L.aSt = take(B * 13);
 	L.aPrev = take(B * 13);
 	L.aIw = take(B * 9);
@@ -127,7 +132,7 @@ static AdjLayout adj_layout(int32_t MB, int32_t MC, int S, int I)
 	L.aMu = take(C);
 	L.aLamN = take(C * 4);
 	L.aPrep = take(C * 52);
-	L.total = o;
+	L.dtotal = o;
 	return L;
 }
 
@@ -168,8 +173,8 @@ MX_HD void stm(float* p, const M3<float>& m)
 }
 
 // component k of an op's inputs, with tangent 1 when k is the evaluated one
-MX_HD DF dv(float v, int k, int d) { return dd::dual<float>(v, k == d ? 1.0f : 0.0f); }
-MX_HD DF dc(float v) { return dd::dual<float>(v, 0.0f); }
+MX_HD DF dv(float v, int k, int d) { return dd::dual<float, double>(v, k == d ? 1.0 : 0.0); }
+MX_HD DF dc(float v) { return dd::dual<float, double>(v, 0.0); }
 MX_HD V3<DF> dv3(const float* p, int k, int d)
 {
 	return dd::v3<DF>(dv(p[0], k, d), dv(p[1], k + 1, d), dv(p[2], k + 2, d));
@@ -189,12 +194,12 @@ MX_HD M3<DF> dm3(const float* p, int k, int d)
 	return m;
 }
 // adjoint . tangent
-MX_HD float td3(const float* a, const V3<DF>& v) { return a[0] * v.x.d + a[1] * v.y.d + a[2] * v.z.d; }
-MX_HD float td4(const float* a, const Q4<DF>& q)
+MX_HD double td3(const double* a, const V3<DF>& v) { return a[0] * v.x.d + a[1] * v.y.d + a[2] * v.z.d; }
+MX_HD double td4(const double* a, const Q4<DF>& q)
 {
 	return a[0] * q.x.d + a[1] * q.y.d + a[2] * q.z.d + a[3] * q.w.d;
 }
-MX_HD float tdm(const float* a, const M3<DF>& m)
+MX_HD double tdm(const double* a, const M3<DF>& m)
 {
 	return td3(a, m.cx) + td3(a + 3, m.cy) + td3(a + 6, m.cz);
 }
@@ -203,7 +208,7 @@ MX_HD float tdm(const float* a, const M3<DF>& m)
 struct AW
 {
 	float *st, *im, *il, *mu, *lamN, *rbPre, *rbPost, *rbIw, *rbFric, *rcPrep, *rcNrm, *rcLamN, *rcFric, *rcVel;
-	float *aSt, *aPrev, *aIw, *aIm, *aIl, *aMu, *aLamN, *aPrep;
+	double *aSt, *aPrev, *aIw, *aIm, *aIl, *aMu, *aLamN, *aPrep;
 	const int32_t* order; // contacts by level (LDS)
 	const int32_t* start; // level starts (LDS)
 	int32_t nL;
@@ -282,6 +287,8 @@ MX_HD void fwd_substep(const AW& A, const StepIn& ins, const SubConst& K)
 			for (int32_t k = A.start[l] + tid; k < A.start[l + 1]; k += T)
 			{
 				int32_t c = A.order[k];
+				if ((uint32_t)c >= (uint32_t)nc)
+					continue;
 				ContactGeo g = dd::contact_geo(ins, c);
 				int2 ab = ins.bodies[c];
 				float* xa = A.st + ab.x * 13;
@@ -333,6 +340,8 @@ MX_HD void fwd_substep(const AW& A, const StepIn& ins, const SubConst& K)
 			for (int32_t k = A.start[l] + tid; k < A.start[l + 1]; k += T)
 			{
 				int32_t c = A.order[k];
+				if ((uint32_t)c >= (uint32_t)nc)
+					continue;
 				ContactGeo g = dd::contact_geo(ins, c);
 				if (g.tA != M3D_SHAPE_BOX || g.tB != M3D_SHAPE_BOX || g.count == 0)
 					continue;
@@ -404,6 +413,8 @@ MX_HD void fwd_substep(const AW& A, const StepIn& ins, const SubConst& K)
 			for (int32_t k = A.start[l] + tid; k < A.start[l + 1]; k += T)
 			{
 				int32_t c = A.order[k];
+				if ((uint32_t)c >= (uint32_t)nc)
+					continue;
 				ContactGeo g = dd::contact_geo(ins, c);
 				int2 ab = ins.bodies[c];
 				float* xa = A.st + ab.x * 13;
@@ -487,11 +498,11 @@ MX_HD bool vel_active(const ContactGeo& g, int d)
 	return true;
 }
 
-MX_HD float vel_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
+MX_HD double vel_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!vel_active(g, d))
-		return 0.0f;
+		return 0.0;
 	const int32_t s = K.s;
 	int2 ab = ins.bodies[c];
 	const float* rv = A.rcVel + ((int64_t)s * A.MC + c) * 20;
@@ -531,7 +542,7 @@ MX_HD float vel_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t 
 		slip[j] = ins.pSlip[c * 4 + j];
 	}
 	dd::velocity_op(g, a, b, pvA, pwA, pvB, pwB, mu, lN, pp, slip, K.h, K.gravityLen);
-	float r = 0.0f;
+	double r = 0.0;
 	if (g.dynA)
 		r += td3(A.aSt + ab.x * 13 + 7, a.v) + td3(A.aSt + ab.x * 13 + 10, a.w);
 	if (g.dynB)
@@ -539,7 +550,7 @@ MX_HD float vel_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t 
 	return r;
 }
 
-MX_HD void vel_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
+MX_HD void vel_apply(const AW& A, const StepIn& ins, int32_t c, int d, double r)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!vel_active(g, d))
@@ -585,10 +596,10 @@ MX_HD void vel_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
 
 // Velocity reconstruction of body i (kDimReconstruct: 0-6 x q, 7-13
 // previous x q).
-MX_HD float rec_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t i, int d)
+MX_HD double rec_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t i, int d)
 {
 	if (!dd::is_dynamic(ins, i))
-		return 0.0f;
+		return 0.0;
 	const float* fr = A.rbFric + ((int64_t)K.s * A.MB + i) * 7;
 	const float* pre = A.rbPre + ((int64_t)K.s * A.MB + i) * 13;
 	V3<DF> v, w;
@@ -596,7 +607,7 @@ MX_HD float rec_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t 
 	return td3(A.aSt + i * 13 + 7, v) + td3(A.aSt + i * 13 + 10, w);
 }
 
-MX_HD void rec_apply(const AW& A, const StepIn& ins, int32_t i, int d, float r)
+MX_HD void rec_apply(const AW& A, const StepIn& ins, int32_t i, int d, double r)
 {
 	if (!dd::is_dynamic(ins, i))
 		return;
@@ -606,7 +617,7 @@ MX_HD void rec_apply(const AW& A, const StepIn& ins, int32_t i, int d, float r)
 		A.aPrev[i * 13 + (d - 7)] += r;
 	if (d == 0)
 		for (int k = 7; k < 13; ++k)
-			A.aSt[i * 13 + k] = 0.0f; // overwritten by the reconstruction
+			A.aSt[i * 13 + k] = 0.0; // overwritten by the reconstruction
 }
 
 // Stiction op of contact c (kDimFriction: 0-6 xa qa, 7-13 xb qb, 14-22 iwA,
@@ -622,11 +633,11 @@ MX_HD bool fric_active(const ContactGeo& g, int d)
 	return true;
 }
 
-MX_HD float fric_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
+MX_HD double fric_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!fric_active(g, d))
-		return 0.0f;
+		return 0.0;
 	const int32_t s = K.s;
 	int2 ab = ins.bodies[c];
 	const float* rf = A.rcFric + ((int64_t)s * A.MC + c) * 14;
@@ -659,7 +670,7 @@ MX_HD float fric_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t
 		fB[j] = ins.pFB[c * 4 + j];
 	}
 	dd::friction_op(g, a, b, mu, lN, lT, dc3(pr), dc3(pr + 3), fA, fB);
-	float r = 0.0f;
+	double r = 0.0;
 	if (g.dynA)
 		r += td3(A.aSt + ab.x * 13, a.p) + td4(A.aSt + ab.x * 13 + 3, a.q);
 	if (g.dynB)
@@ -667,7 +678,7 @@ MX_HD float fric_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t
 	return r;
 }
 
-MX_HD void fric_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
+MX_HD void fric_apply(const AW& A, const StepIn& ins, int32_t c, int d, double r)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!fric_active(g, d))
@@ -706,11 +717,11 @@ MX_HD bool nrm_active(const ContactGeo& g, int d)
 	return true;
 }
 
-MX_HD float nrm_eval(const AW& A, const StepIn& ins, const SubConst& K, int it, int32_t c, int d)
+MX_HD double nrm_eval(const AW& A, const StepIn& ins, const SubConst& K, int it, int32_t c, int d)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!nrm_active(g, d))
-		return 0.0f;
+		return 0.0;
 	const int32_t s = K.s;
 	int2 ab = ins.bodies[c];
 	const float* rn = A.rcNrm + (((int64_t)s * A.I + it) * A.MC + c) * 18;
@@ -730,7 +741,7 @@ MX_HD float nrm_eval(const AW& A, const StepIn& ins, const SubConst& K, int it, 
 		jB[j] = dv3(pr + j * 13 + 9, k + 4, d);
 	}
 	dd::normals_op(g, pa, qa, pb, qb, imA, imB, lN, wN, jA, jB, K.maxCorrection);
-	float r = 0.0f;
+	double r = 0.0;
 	if (g.dynA)
 		r += td3(A.aSt + ab.x * 13, pa) + td4(A.aSt + ab.x * 13 + 3, qa);
 	if (g.dynB)
@@ -742,7 +753,7 @@ MX_HD float nrm_eval(const AW& A, const StepIn& ins, const SubConst& K, int it, 
 	return r;
 }
 
-MX_HD void nrm_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
+MX_HD void nrm_apply(const AW& A, const StepIn& ins, int32_t c, int d, double r)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (!nrm_active(g, d))
@@ -771,11 +782,11 @@ MX_HD bool prep_active(const ContactGeo& g, int d)
 	return onA ? g.dynA : g.dynB;
 }
 
-MX_HD float prep_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
+MX_HD double prep_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t c, int d)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (g.count == 0 || !prep_active(g, d))
-		return 0.0f;
+		return 0.0;
 	const int32_t s = K.s;
 	int2 ab = ins.bodies[c];
 	const float* post = A.rbPost + (int64_t)s * A.MB * 13;
@@ -783,20 +794,20 @@ MX_HD float prep_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t
 	Q4<DF> qa = dq4(post + ab.x * 13 + 3, 0, d), qb = dq4(post + ab.y * 13 + 3, 4, d);
 	M3<DF> iwa = dm3(iwr + ab.x * 9, 8, d), iwb = dm3(iwr + ab.y * 9, 17, d);
 	DF imA = dv(A.im[ab.x], 26, d), imB = dv(A.im[ab.y], 27, d);
-	float r = 0.0f;
+	double r = 0.0;
 #pragma unroll
 	for (int j = 0; j < 4; ++j)
 	{
 		if (j >= g.count)
 			break;
 		PointPrep<DF> p = dd::prepare_point(g, j, qa, qb, iwa, iwb, imA, imB);
-		const float* ap = A.aPrep + c * 52 + j * 13;
+		const double* ap = A.aPrep + c * 52 + j * 13;
 		r += td3(ap, p.rA) + td3(ap + 3, p.rB) + td3(ap + 6, p.jA) + td3(ap + 9, p.jB) + ap[12] * p.wN.d;
 	}
 	return r;
 }
 
-MX_HD void prep_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
+MX_HD void prep_apply(const AW& A, const StepIn& ins, int32_t c, int d, double r)
 {
 	ContactGeo g = dd::contact_geo(ins, c);
 	if (g.count == 0 || !prep_active(g, d))
@@ -819,24 +830,24 @@ MX_HD void prep_apply(const AW& A, const StepIn& ins, int32_t c, int d, float r)
 // Integration of body i (kDimIntegrate: 0-12 x q v w at the substep start,
 // 13-15 inverse local inertia). Outputs: the state, the previous-substep
 // copies (the inputs themselves) and the world inverse inertia.
-MX_HD float int_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t i, int d)
+MX_HD double int_eval(const AW& A, const StepIn& ins, const SubConst& K, int32_t i, int d)
 {
 	if (!dd::is_dynamic(ins, i))
-		return 0.0f;
+		return 0.0;
 	const float* pre = A.rbPre + ((int64_t)K.s * A.MB + i) * 13;
 	V3<DF> x = dv3(pre, 0, d), v = dv3(pre + 7, 7, d), w = dv3(pre + 10, 10, d);
 	Q4<DF> q = dq4(pre + 3, 3, d);
 	V3<DF> il = dv3(A.il + i * 3, 13, d);
 	M3<DF> iw;
 	dd::integrate_op(x, q, v, w, il, iw, K.h, ins.vel[i].w, ins.invIL[i].w, ins.angDamp[i], ins.gravity);
-	const float* as = A.aSt + i * 13;
-	float r = td3(as, x) + td4(as + 3, q) + td3(as + 7, v) + td3(as + 10, w) + tdm(A.aIw + i * 9, iw);
+	const double* as = A.aSt + i * 13;
+	double r = td3(as, x) + td4(as + 3, q) + td3(as + 7, v) + td3(as + 10, w) + tdm(A.aIw + i * 9, iw);
 	if (d < 13)
 		r += A.aPrev[i * 13 + d];
 	return r;
 }
 
-MX_HD void int_apply(const AW& A, const StepIn& ins, int32_t i, int d, float r)
+MX_HD void int_apply(const AW& A, const StepIn& ins, int32_t i, int d, double r)
 {
 	if (!dd::is_dynamic(ins, i))
 		return;
@@ -849,7 +860,8 @@ MX_HD void int_apply(const AW& A, const StepIn& ins, int32_t i, int d, float r)
 // Runs the (item, component) products of items list[k0 .. k1) (list null:
 // items k0 .. k1), reading everything before writing anything.
 template <class Eval, class Apply>
-MX_HD void run_products(float* res, const int32_t* list, int32_t k0, int32_t k1, int D, Eval ev, Apply ap)
+MX_HD void run_products(double* res, const int32_t* list, int32_t k0, int32_t k1, int32_t bound, int D, Eval ev,
+						Apply ap)
 {
 	const int32_t per = kAdjTasks / D;
 	for (int32_t base = k0; base < k1; base += per)
@@ -859,52 +871,55 @@ MX_HD void run_products(float* res, const int32_t* list, int32_t k0, int32_t k1,
 		for (int32_t t = ADJ_TID; t < n; t += ADJ_NT)
 		{
 			int32_t k = base + t / D;
-			res[t] = ev(list != nullptr ? list[k] : k, t % D);
+			int32_t item = list != nullptr ? list[k] : k;
+			res[t] = (uint32_t)item < (uint32_t)bound ? ev(item, t % D) : 0.0;
 		}
 		ADJ_SYNC();
 		for (int32_t t = ADJ_TID; t < n; t += ADJ_NT)
 		{
 			int32_t k = base + t / D;
-			ap(list != nullptr ? list[k] : k, t % D, res[t]);
+			int32_t item = list != nullptr ? list[k] : k;
+			if ((uint32_t)item < (uint32_t)bound)
+				ap(item, t % D, res[t]);
 		}
 		ADJ_SYNC();
 	}
 }
 
-MX_HD void rev_substep(const AW& A, const StepIn& ins, const SubConst& K, float* res)
+MX_HD void rev_substep(const AW& A, const StepIn& ins, const SubConst& K, double* res)
 {
 	const int tid = ADJ_TID, T = ADJ_NT;
 	const int32_t nb = ins.nb, nc = ins.nc;
 	for (int32_t k = tid; k < nb * 13; k += T)
-		A.aPrev[k] = 0.0f;
+		A.aPrev[k] = 0.0;
 	for (int32_t k = tid; k < nb * 9; k += T)
-		A.aIw[k] = 0.0f;
+		A.aIw[k] = 0.0;
 	for (int32_t k = tid; k < nc * 4; k += T)
-		A.aLamN[k] = 0.0f;
+		A.aLamN[k] = 0.0;
 	for (int32_t k = tid; k < nc * 52; k += T)
-		A.aPrep[k] = 0.0f;
+		A.aPrep[k] = 0.0;
 	ADJ_SYNC();
 	// velocity pass
 	for (int32_t l = A.nL - 1; l >= 0; --l)
 		run_products(
-			res, A.order, A.start[l], A.start[l + 1], kDimVelocity,
+			res, A.order, A.start[l], A.start[l + 1], nc, kDimVelocity,
 			[&](int32_t c, int d) { return vel_eval(A, ins, K, c, d); },
-			[&](int32_t c, int d, float r) { vel_apply(A, ins, c, d, r); });
+			[&](int32_t c, int d, double r) { vel_apply(A, ins, c, d, r); });
 	// velocity reconstruction
 	run_products(
-		res, nullptr, 0, nb, kDimReconstruct, [&](int32_t i, int d) { return rec_eval(A, ins, K, i, d); },
-		[&](int32_t i, int d, float r) { rec_apply(A, ins, i, d, r); });
+		res, nullptr, 0, nb, nb, kDimReconstruct, [&](int32_t i, int d) { return rec_eval(A, ins, K, i, d); },
+		[&](int32_t i, int d, double r) { rec_apply(A, ins, i, d, r); });
 	// stiction (box-box pairs; the others' products are zero and not applied)
 	for (int32_t l = A.nL - 1; l >= 0; --l)
 		run_products(
-			res, A.order, A.start[l], A.start[l + 1], kDimFriction,
+			res, A.order, A.start[l], A.start[l + 1], nc, kDimFriction,
 			[&](int32_t c, int d) {
 				ContactGeo g = dd::contact_geo(ins, c);
 				if (g.tA != M3D_SHAPE_BOX || g.tB != M3D_SHAPE_BOX || g.count == 0)
-					return 0.0f;
+					return 0.0;
 				return fric_eval(A, ins, K, c, d);
 			},
-			[&](int32_t c, int d, float r) {
+			[&](int32_t c, int d, double r) {
 				ContactGeo g = dd::contact_geo(ins, c);
 				if (g.tA != M3D_SHAPE_BOX || g.tB != M3D_SHAPE_BOX || g.count == 0)
 					return;
@@ -914,19 +929,19 @@ MX_HD void rev_substep(const AW& A, const StepIn& ins, const SubConst& K, float*
 	for (int it = A.I - 1; it >= 0; --it)
 		for (int32_t l = A.nL - 1; l >= 0; --l)
 			run_products(
-				res, A.order, A.start[l], A.start[l + 1], kDimNormals,
+				res, A.order, A.start[l], A.start[l + 1], nc, kDimNormals,
 				[&](int32_t c, int d) { return nrm_eval(A, ins, K, it, c, d); },
-				[&](int32_t c, int d, float r) { nrm_apply(A, ins, c, d, r); });
+				[&](int32_t c, int d, double r) { nrm_apply(A, ins, c, d, r); });
 	// prepare (levels keep contacts that share a body apart)
 	for (int32_t l = A.nL - 1; l >= 0; --l)
 		run_products(
-			res, A.order, A.start[l], A.start[l + 1], kDimPrepare,
+			res, A.order, A.start[l], A.start[l + 1], nc, kDimPrepare,
 			[&](int32_t c, int d) { return prep_eval(A, ins, K, c, d); },
-			[&](int32_t c, int d, float r) { prep_apply(A, ins, c, d, r); });
+			[&](int32_t c, int d, double r) { prep_apply(A, ins, c, d, r); });
 	// integration
 	run_products(
-		res, nullptr, 0, nb, kDimIntegrate, [&](int32_t i, int d) { return int_eval(A, ins, K, i, d); },
-		[&](int32_t i, int d, float r) { int_apply(A, ins, i, d, r); });
+		res, nullptr, 0, nb, nb, kDimIntegrate, [&](int32_t i, int d) { return int_eval(A, ins, K, i, d); },
+		[&](int32_t i, int d, double r) { int_apply(A, ins, i, d, r); });
 }
 
 // ------------------------------------------------------------ kernel
@@ -939,7 +954,7 @@ MX_HD void rev_substep(const AW& A, const StepIn& ins, const SubConst& K, float*
 // Workgroup-local lists of one world (LDS on the device).
 struct AdjLocal
 {
-	float* res;      // kAdjTasks
+	double* res;     // kAdjTasks
 	int32_t* order;  // kAdjMaxContacts
 	int32_t* level;  // kAdjMaxContacts
 	int32_t* start;  // kAdjMaxContacts + 2
@@ -947,11 +962,11 @@ struct AdjLocal
 	int32_t* nL;     // 1
 };
 
-MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const float* stateIn, float* stateOut,
-					 int reverse, const float* seedIn, float* seedOut, float* massGrad, float* fricGrad, int32_t wi,
-					 const AdjLocal& Lc)
+MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, double* dslab, const float* stateIn,
+					 float* stateOut, int reverse, const double* seedIn, double* seedOut, double* massGrad,
+					 double* fricGrad, int32_t wi, const AdjLocal& Lc)
 {
-	float* res = Lc.res;
+	double* res = Lc.res;
 	int32_t* sOrder = Lc.order;
 	int32_t* sLevel = Lc.level;
 	int32_t* sStart = Lc.start;
@@ -962,6 +977,7 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const fl
 	const int32_t b0 = V.bodyStart[wi];
 	const int32_t nb = in.nb, nc = in.nc;
 	float* W = slab + (int64_t)wi * L.total;
+	double* WD = dslab + (int64_t)wi * L.dtotal;
 	AW A;
 	A.st = W + L.st;
 	A.im = W + L.im;
@@ -977,14 +993,14 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const fl
 	A.rcLamN = W + L.rcLamN;
 	A.rcFric = W + L.rcFric;
 	A.rcVel = W + L.rcVel;
-	A.aSt = W + L.aSt;
-	A.aPrev = W + L.aPrev;
-	A.aIw = W + L.aIw;
-	A.aIm = W + L.aIm;
-	A.aIl = W + L.aIl;
-	A.aMu = W + L.aMu;
-	A.aLamN = W + L.aLamN;
-	A.aPrep = W + L.aPrep;
+	A.aSt = WD + L.aSt;
+	A.aPrev = WD + L.aPrev;
+	A.aIw = WD + L.aIw;
+	A.aIm = WD + L.aIm;
+	A.aIl = WD + L.aIl;
+	A.aMu = WD + L.aMu;
+	A.aLamN = WD + L.aLamN;
+	A.aPrep = WD + L.aPrep;
 	A.order = sOrder;
 	A.start = sStart;
 	A.MB = L.MB;
@@ -1098,11 +1114,11 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const fl
 		A.aSt[k] = seedIn[(int64_t)b0 * 13 + k];
 	for (int32_t i = tid; i < nb; i += T)
 	{
-		A.aIm[i] = 0.0f;
-		A.aIl[i * 3] = A.aIl[i * 3 + 1] = A.aIl[i * 3 + 2] = 0.0f;
+		A.aIm[i] = 0.0;
+		A.aIl[i * 3] = A.aIl[i * 3 + 1] = A.aIl[i * 3 + 2] = 0.0;
 	}
 	for (int32_t c = tid; c < nc; c += T)
-		A.aMu[c] = 0.0f;
+		A.aMu[c] = 0.0;
 	ADJ_SYNC();
 	for (int s = in.substeps - 1; s >= 0; --s)
 	{
@@ -1116,10 +1132,11 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const fl
 	for (int32_t i = tid; i < nb; i += T)
 	{
 		float4 P = in.pos[i], IL = in.invIL[i];
-		float m = in.angVel[i].w;
-		if (dd::is_dynamic(in, i) && P.w != 0.0f && m > 0.0f)
-			massGrad[b0 + i] += A.aIm[i] * (-P.w / m) + A.aIl[i * 3] * (-IL.x / m) + A.aIl[i * 3 + 1] * (-IL.y / m) +
-								A.aIl[i * 3 + 2] * (-IL.z / m);
+		double m = in.angVel[i].w;
+		if (dd::is_dynamic(in, i) && P.w != 0.0f && m > 0.0)
+			massGrad[b0 + i] += A.aIm[i] * (-(double)P.w / m) + A.aIl[i * 3] * (-(double)IL.x / m) +
+								A.aIl[i * 3 + 1] * (-(double)IL.y / m) +
+								A.aIl[i * 3 + 2] * (-(double)IL.z / m);
 	}
 	ADJ_SYNC();
 	// friction: mu_c = sqrt(mu_a mu_b), summed into the bodies in contact order
@@ -1129,17 +1146,18 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, const fl
 			int2 ab = in.bodies[c];
 			float ma = in.bmat[ab.x].y, mb = in.bmat[ab.y].y;
 			float r = dd::sqrt_(ma * mb);
-			float dp = A.aMu[c] * (r > 0.0f ? 0.5f / r : 0.0f);
-			fricGrad[b0 + ab.x] += dp * mb;
-			fricGrad[b0 + ab.y] += dp * ma;
+			double dp = A.aMu[c] * (r > 0.0f ? 0.5 / (double)r : 0.0);
+			fricGrad[b0 + ab.x] += dp * (double)mb;
+			fricGrad[b0 + ab.y] += dp * (double)ma;
 		}
 }
 
-__global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L, float* slab, const float* stateIn,
-														  float* stateOut, int reverse, const float* seedIn,
-														  float* seedOut, float* massGrad, float* fricGrad)
+__global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L, float* slab, double* dslab,
+														  const float* stateIn, float* stateOut, int reverse,
+														  const double* seedIn, double* seedOut, double* massGrad,
+														  double* fricGrad)
 {
-	__shared__ float res[kAdjTasks];
+	__shared__ double res[kAdjTasks];
 	__shared__ int32_t sOrder[kAdjMaxContacts];
 	__shared__ int32_t sLevel[kAdjMaxContacts];
 	__shared__ int32_t sStart[kAdjMaxContacts + 2];
@@ -1149,7 +1167,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L
 	if (wi >= V.worlds)
 		return;
 	AdjLocal Lc = { res, sOrder, sLevel, sStart, sLast, &sNL };
-	adj_world(V, L, slab, stateIn, stateOut, reverse, seedIn, seedOut, massGrad, fricGrad, wi, Lc);
+	adj_world(V, L, slab, dslab, stateIn, stateOut, reverse, seedIn, seedOut, massGrad, fricGrad, wi, Lc);
 }
 
 } // namespace mx
@@ -1158,8 +1176,19 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L
 
 using namespace mx;
 
-static bool adj_launch(DiffState* D, const DiffStepRec* R, int iterations, hipStream_t s, const float* stateIn,
-					   float* stateOut, int reverse, const float* seedIn, float* seedOut)
+// Compulsory bytes of one launch: the recorded step (per body 7 float4
+// columns, damping and flags; per contact its bodies and, per substep, its
+// normal, material and four points of five float4 columns), the state in and
+// out, and in reverse the seed in and out and the two gradients.
+static double adj_bytes(const DiffStepRec* R, int reverse)
+{
+	double body = 7 * 16 + 4 + 4 + 2 * 52 + (reverse ? 2 * 52 + 8 : 0);
+	double contact = 8 + (double)R->substeps * (2 * 16 + 4 * 5 * 16);
+	return body * R->nb + contact * R->nc;
+}
+
+static bool adj_launch(World& w, DiffState* D, const DiffStepRec* R, int iterations, hipStream_t s,
+					   const float* stateIn, float* stateOut, int reverse, const double* seedIn, double* seedOut)
 {
 	int32_t MC = std::max(R->maxWorldPoints / 4, 1);
 	int32_t MB = std::max(R->maxWorldBodies, 1);
@@ -1170,11 +1199,13 @@ static bool adj_launch(DiffState* D, const DiffStepRec* R, int iterations, hipSt
 		return false;
 	}
 	AdjLayout L = adj_layout(MB, MC, R->substeps, iterations);
-	if (!D->adjSlab.ensure((size_t)L.total * D->worlds))
+	if (!D->adjSlab.ensure((size_t)L.total * D->worlds) || !D->adjDSlab.ensure((size_t)L.dtotal * D->worlds))
 		return false;
 	RecView V = rec_view(D, R, iterations);
-	k_adj_step<<<D->worlds, kAdjThreads, 0, s>>>(V, L, D->adjSlab.p, stateIn, stateOut, reverse, seedIn, seedOut,
-												  D->massG.p, D->fricG.p);
+	w.mark_begin(reverse ? M3DX_K_ADJOINT : M3DX_K_ADJ_FORWARD, adj_bytes(R, reverse));
+	k_adj_step<<<D->worlds, kAdjThreads, 0, s>>>(V, L, D->adjSlab.p, D->adjDSlab.p, stateIn, stateOut, reverse,
+												  seedIn, seedOut, D->massG.p, D->fricG.p);
+	w.mark_end();
 	return hip_ok(hipGetLastError(), "adjoint launch");
 }
 
@@ -1195,14 +1226,15 @@ int m3dx_diff_set_checkpoint(m3d_world* world, int every)
 	return 0;
 }
 
-// Vector-Jacobian product of the recorded steps (the parallel adjoint):
-// given the adjoint of the final state (host, 13 per body: x, q, v, w),
-// returns the adjoint of the first recorded step's input state (13 per body;
-// its v is the gradient with respect to the initial velocities) and the
-// gradients with respect to each body's mass and friction (summed over the
-// steps). Any pointer but seed may be null. Returns 0, or -1
-// (m3dx_last_error).
-int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
+// Vector-Jacobian product of the recorded steps (the parallel adjoint), in
+// double precision along the engine's float path: given the adjoint of the
+// final state (host, 13 per body: x, q, v, w), returns the adjoint of the
+// first recorded step's input state (13 per body; its v is the gradient with
+// respect to the initial velocities) and the gradients with respect to each
+// body's mass and friction (summed over the steps). Any pointer but seed may
+// be null. Returns 0, or -1 (m3dx_last_error).
+int m3dx_diff_backward_f64(m3d_world* world, const double* seed, double* stateAdj, double* massGrad,
+						   double* frictionGrad)
 {
 	World& w = world->w;
 	DiffState* D = w.diff;
@@ -1226,39 +1258,66 @@ int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, flo
 	if (!D->seedA.ensure(st) || !D->seedB.ensure(st) || !D->massG.ensure(std::max(n, 1)) ||
 		!D->fricG.ensure(std::max(n, 1)) || !D->segStates.ensure(st * (size_t)K))
 		return -1;
-	float* sa = D->seedA.p;
-	float* sb = D->seedB.p;
-	hipMemcpyAsync(sa, seed, st * sizeof(float), hipMemcpyHostToDevice, s);
-	hipMemsetAsync(D->massG.p, 0, (size_t)n * sizeof(float), s);
-	hipMemsetAsync(D->fricG.p, 0, (size_t)n * sizeof(float), s);
+	double* sa = D->seedA.p;
+	double* sb = D->seedB.p;
+	hipMemcpyAsync(sa, seed, st * sizeof(double), hipMemcpyHostToDevice, s);
+	hipMemsetAsync(D->massG.p, 0, (size_t)n * sizeof(double), s);
+	hipMemsetAsync(D->fricG.p, 0, (size_t)n * sizeof(double), s);
 	for (int seg = ((N - 1) / K) * K; seg >= 0; seg -= K)
 	{
 		int end = std::min(seg + K, N);
 		// recompute the states inside the segment from its checkpoint
 		for (int k = seg; k < end - 1; ++k)
-			if (!adj_launch(D, D->steps[k], iterations, s, k == seg ? nullptr : D->segStates.p + st * (k - seg),
+			if (!adj_launch(w, D, D->steps[k], iterations, s, k == seg ? nullptr : D->segStates.p + st * (k - seg),
 							D->segStates.p + st * (k - seg + 1), 0, nullptr, nullptr))
 				return -1;
 		for (int k = end - 1; k >= seg; --k)
 		{
-			if (!adj_launch(D, D->steps[k], iterations, s, k == seg ? nullptr : D->segStates.p + st * (k - seg),
+			if (!adj_launch(w, D, D->steps[k], iterations, s, k == seg ? nullptr : D->segStates.p + st * (k - seg),
 							nullptr, 1, sa, sb))
 				return -1;
 			std::swap(sa, sb);
 		}
 	}
-	std::vector<float> ha(st), hm((size_t)n), hf((size_t)n);
-	hipMemcpyAsync(ha.data(), sa, st * sizeof(float), hipMemcpyDeviceToHost, s);
-	hipMemcpyAsync(hm.data(), D->massG.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s);
-	hipMemcpyAsync(hf.data(), D->fricG.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s);
+	std::vector<double> ha(st), hm((size_t)n), hf((size_t)n);
+	hipMemcpyAsync(ha.data(), sa, st * sizeof(double), hipMemcpyDeviceToHost, s);
+	hipMemcpyAsync(hm.data(), D->massG.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s);
+	hipMemcpyAsync(hf.data(), D->fricG.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s);
 	if (!hip_ok(hipStreamSynchronize(s), "adjoint"))
 		return -1;
+	w.collect_marks();
 	if (stateAdj != nullptr)
-		memcpy(stateAdj, ha.data(), st * sizeof(float));
+		memcpy(stateAdj, ha.data(), st * sizeof(double));
 	if (massGrad != nullptr)
-		memcpy(massGrad, hm.data(), (size_t)n * sizeof(float));
+		memcpy(massGrad, hm.data(), (size_t)n * sizeof(double));
 	if (frictionGrad != nullptr)
-		memcpy(frictionGrad, hf.data(), (size_t)n * sizeof(float));
+		memcpy(frictionGrad, hf.data(), (size_t)n * sizeof(double));
+	return 0;
+}
+
+// m3dx_diff_backward_f64 with float arrays.
+int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
+{
+	World& w = world->w;
+	if (w.diff == nullptr || w.diff->steps.empty())
+	{
+		set_error("no recorded steps");
+		return -1;
+	}
+	const size_t n = (size_t)w.diff->steps[0]->nb;
+	std::vector<double> sd(seed, seed + n * kOuts), a(n * kOuts), m(n), f(n);
+	if (m3dx_diff_backward_f64(world, sd.data(), a.data(), m.data(), f.data()) != 0)
+		return -1;
+	for (size_t k = 0; k < n * kOuts; ++k)
+		if (stateAdj != nullptr)
+			stateAdj[k] = (float)a[k];
+	for (size_t k = 0; k < n; ++k)
+	{
+		if (massGrad != nullptr)
+			massGrad[k] = (float)m[k];
+		if (frictionGrad != nullptr)
+			frictionGrad[k] = (float)f[k];
+	}
 	return 0;
 }
 
@@ -1283,13 +1342,14 @@ int m3dx_diff_forward(m3d_world* world, int first, int count, float* stateOut)
 	float* b = a + st;
 	for (int k = 0; k < count; ++k)
 	{
-		if (!adj_launch(D, D->steps[first + k], iterations_of(w), s, k == 0 ? nullptr : a, b, 0, nullptr, nullptr))
+		if (!adj_launch(w, D, D->steps[first + k], iterations_of(w), s, k == 0 ? nullptr : a, b, 0, nullptr, nullptr))
 			return -1;
 		std::swap(a, b);
 	}
 	if (!hip_ok(hipMemcpyAsync(stateOut, a, st * sizeof(float), hipMemcpyDeviceToHost, s), "forward output") ||
 		!hip_ok(hipStreamSynchronize(s), "forward"))
 		return -1;
+	w.collect_marks();
 	return 0;
 }
 

@@ -70,12 +70,13 @@ struct DiffState
 	// states of one checkpoint segment, seeds and gradients
 	int checkpointEvery = 10;
 	DBuf<float> adjSlab;
-	DBuf<int32_t> adjISlab;
-	DBuf<float> segStates, seedA, seedB, massG, fricG;
+	DBuf<double> adjDSlab;
+	DBuf<float> segStates;
+	DBuf<double> seedA, seedB, massG, fricG;
 	~DiffState()
 	{
 		adjSlab.release();
-		adjISlab.release();
+		adjDSlab.release();
 		segStates.release();
 		seedA.release();
 		seedB.release();

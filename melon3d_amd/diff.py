@@ -168,13 +168,14 @@ def backward(L: Library, w, scene: DiffScene, grad_final_pos: np.ndarray, tape: 
     the parallel adjoint (or, tape=True, by the reverse-mode tape)."""
     n = scene.body_count()
     idx = scene.dynamic_index()
-    seed = np.zeros((n, OUT), dtype=np.float32)
+    dt = np.float32 if tape else np.float64
+    seed = np.zeros((n, OUT), dtype=dt)
     seed[idx, 0:3] = grad_final_pos
-    adj = np.zeros((n, OUT), dtype=np.float32)
-    mg = np.zeros(n, dtype=np.float32)
-    fg = np.zeros(n, dtype=np.float32)
+    adj = np.zeros((n, OUT), dtype=dt)
+    mg = np.zeros(n, dtype=dt)
+    fg = np.zeros(n, dtype=dt)
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
-    fn = L.m3dx_diff_backward_tape if tape else L.m3dx_diff_backward
+    fn = L.m3dx_diff_backward_tape if tape else L.m3dx_diff_backward_f64
     if fn(w, ptr(seed), ptr(adj), ptr(mg), ptr(fg)) != 0:
         raise RuntimeError("adjoint failed: " + L.last_error())
     return adj[idx, 7:10].astype(np.float64), fg[idx].astype(np.float64), mg[idx].astype(np.float64)

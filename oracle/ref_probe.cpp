@@ -546,6 +546,7 @@ int m3dx_diff_state(m3d_world*, int, float*) { return -1; }
 int m3dx_diff_replay(m3d_world*, int, int, int, const void*, const void*, const void*, void*) { return -1; }
 int m3dx_diff_backward(m3d_world*, const float*, float*, float*, float*) { return -1; }
 int m3dx_diff_backward_tape(m3d_world*, const float*, float*, float*, float*) { return -1; }
+int m3dx_diff_backward_f64(m3d_world*, const double*, double*, double*, double*) { return -1; }
 int m3dx_diff_set_checkpoint(m3d_world*, int) { return -1; }
 int m3dx_diff_forward(m3d_world*, int, int, float*) { return -1; }
 int m3dx_diff_jvp_cpu(m3d_world*, int, int, int, const double*, const double*, const double*, const double*, const double*,
