@@ -185,6 +185,7 @@ def run_diff(args, dev):
     for _ in range(args.steps):
         dev.m3d_world_step(w, DT, args.substeps)
     fwd_plain = time.perf_counter() - t0
+    check_steps(dev, w)
     dev.m3d_world_destroy(w)
     w = scene.build(dev)
     if dev.m3dx_diff_record(w, 1) != 0:
@@ -193,6 +194,7 @@ def run_diff(args, dev):
     for _ in range(args.steps):
         dev.m3d_world_step(w, DT, args.substeps)
     fwd = time.perf_counter() - t0
+    check_steps(dev, w)
     dev.m3dx_diff_record(w, 0)
     engine = diff.final_positions(dev, w, scene)
     m = scene.mass[..., None]
@@ -342,6 +344,7 @@ def run_batch(args, dev):
     for i in range(args.steps):
         dev.m3d_world_step(w, DT, args.substeps)
     elapsed = time.perf_counter() - t0
+    check_steps(dev, w)
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
     value = dyn * args.steps / elapsed
@@ -394,6 +397,13 @@ def run_batch(args, dev):
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def check_steps(dev, w):
+    """A step that stopped on an error would make the timing meaningless."""
+    n = dev.m3dx_step_failures(w)
+    if n:
+        raise SystemExit(f"{n} steps failed: {dev.last_error()}")
 
 
 def host_cores() -> int:
@@ -507,6 +517,7 @@ def main():
         prof_ms[1] += p.narrowphaseMs
         prof_ms[2] += p.solverMs
     elapsed = time.perf_counter() - t0
+    check_steps(dev, w)
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
     ms = 1000.0 * elapsed / args.steps

@@ -259,6 +259,16 @@ typedef struct m3dx_kernel_time
 
 /* enable != 0 starts (and resets) timing, 0 stops it. Returns 0 or -1. */
 M3D_API int m3dx_set_kernel_timing(m3d_world* world, int enable);
+/* Workgroups of the giant-island solver passes: 0 (default) lets the
+   world's tuner choose them by measurement, per size class of the step's
+   solver entries; > 0 forces a count. Results do not depend on it.
+   Returns 0 or -1. */
+M3D_API int m3dx_set_pass_blocks(m3d_world* world, int blocks);
+/* Workgroups the last step's solver passes ran with (0: none ran). */
+M3D_API int m3dx_pass_blocks(m3d_world* world);
+/* Steps of this world that stopped on an error (m3dx_last_error holds the
+   last message; m3d_world_step has no return value). */
+M3D_API int32_t m3dx_step_failures(m3d_world* world);
 /* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
 M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
 

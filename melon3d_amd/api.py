@@ -331,6 +331,9 @@ _SIGS = {
     "m3dx_debug_contact_colors": (C.c_int, [W, C.POINTER(C.c_int32), C.c_int]),
     "m3dx_world_stats": (None, [W, C.POINTER(m3dx_step_stats)]),
     "m3dx_set_kernel_timing": (C.c_int, [W, C.c_int]),
+    "m3dx_set_pass_blocks": (C.c_int, [W, C.c_int]),
+    "m3dx_pass_blocks": (C.c_int, [W]),
+    "m3dx_step_failures": (C.c_int32, [W]),
     "m3dx_kernel_timing": (C.c_int, [W, C.POINTER(m3dx_kernel_time)]),
     "m3dx_batch_create": (W, [C.POINTER(m3d_world_def), C.c_int32]),
     "m3dx_batch_select": (C.c_int, [W, C.c_int32]),

@@ -1040,9 +1040,7 @@ static int coop_blocks(World& w, K kernel)
 // most a cap (a grid barrier costs about 0.04 us more with every workgroup
 // that arrives, and a slot's lanes solve one contact each in parallel, so
 // lanes beyond the larger slots only add barrier time) and never more than
-// can be co-resident. The passes' cap grows from kGiantMaxBlocks with the
-// step's entries; M3D_GIANT_BLOCKS overrides it.
-constexpr int kGiantMaxBlocks = 64;
+// can be co-resident. The passes' count comes from the world's tuner.
 
 // The coloring fixpoint does several loads per entry and round, so it
 // takes one lane per entry up to kOrderMaxBlocks workgroups.
@@ -1187,15 +1185,35 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	// more entries per slot pay for the dearer barrier of a wider grid: about
 	// one workgroup per 1536 entries, 64 to 256 (measured on a 190 k-entry
 	// stream step and a 4 M-entry batch step)
+	// workgroup count from the world's pass tuner (M3D_GIANT_BLOCKS or
+	// m3dx_set_pass_blocks force one)
 	static const int capEnv = env_blocks("M3D_GIANT_BLOCKS", 0);
-	const int cap = capEnv > 0 ? capEnv : std::max(kGiantMaxBlocks, std::min(4 * kGiantMaxBlocks, NC / 1536));
-	const int blocks = std::min(giant_blocks(w, k_giant_pass<NormalsPass>, NC, cap),
-								std::min(giant_blocks(w, k_giant_pass<FrictionPass>, NC, cap),
-										 giant_blocks(w, k_giant_pass<VelocityPass>, NC, cap)));
+	if (capEnv > 0 && w.tuner.forcedBlocks == 0)
+		w.tuner.forcedBlocks = capEnv;
+	const int coopMax = std::min(coop_blocks(w, k_giant_pass<NormalsPass>),
+								 std::min(coop_blocks(w, k_giant_pass<FrictionPass>),
+										  coop_blocks(w, k_giant_pass<VelocityPass>)));
+	bool measure = false;
+	int blocks = w.tuner_pick(NC, coopMax, measure);
+	w.tuner.lastBlocks = blocks;
+	if (measure)
+		(void)hipEventRecord(w.tuner.ev0, s);
+	// a count the runtime refuses for a cooperative launch (its limit can be
+	// below the occupancy estimate) is lowered until accepted, and the world
+	// keeps the accepted count as its limit
 	auto coop = [&](const void* kernel, int passes) {
 		void* args[] = { &S, &passes };
-		return hip_ok(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s),
-					  "giant pass");
+		for (;;)
+		{
+			hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+			if (e != hipErrorCooperativeLaunchTooLarge || blocks <= 1)
+				return hip_ok(e, "giant pass");
+			(void)hipGetLastError();
+			blocks = std::max(1, blocks * 3 / 4);
+			w.tuner.coopLimit = blocks;
+			w.tuner.lastBlocks = blocks;
+			w.tuner.pendingFailed = measure;
+		}
 	};
 	for (int sub = 0; sub < substeps; ++sub)
 	{
@@ -1220,6 +1238,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		if (!ok)
 			return false;
 	}
+	if (measure)
+		(void)hipEventRecord(w.tuner.ev1, s);
 	return true;
 }
 

@@ -293,6 +293,39 @@ struct World
 	void mark_end();
 	void collect_marks(); // after the step's final synchronization
 
+	// Workgroup count of the giant-island passes, chosen by measurement: per
+	// size class of the step's solver entries (floor log2), every candidate
+	// is timed on kTunerRounds steps (pass time per slot, HIP events around
+	// the substep loop) and the fastest is kept. The count changes no result
+	// (a slot's contacts touch disjoint bodies; the order inside a slot is
+	// free). forcedBlocks > 0 (API or M3D_GIANT_BLOCKS) bypasses it.
+	struct PassTuner
+	{
+		static constexpr int kClasses = 40;
+		static constexpr int kCands = 6;
+		static constexpr int kRounds = 2;
+		struct Class
+		{
+			int trials = 0;            // measurements taken
+			int chosen = 0;            // workgroups once tuned (0: tuning)
+			float perSlot[kCands] = {}; // best ms per slot of each candidate
+		};
+		Class cls[kClasses];
+		int forcedBlocks = 0;
+		int coopLimit = 0; // largest count a cooperative launch accepted after a refusal (0: none seen)
+		bool pendingFailed = false;
+		int pendingClass = -1, pendingCand = -1;
+		hipEvent_t ev0 = nullptr, ev1 = nullptr;
+		int lastBlocks = 0;
+	} tuner;
+	static constexpr int kTunerCands[PassTuner::kCands] = { 32, 64, 128, 192, 256, 384 };
+	// picks the passes' workgroup count for a step of `entries` solver entries
+	// (coopMax: co-resident limit); a pick being measured records ev0 / ev1
+	int tuner_pick(int32_t entries, int coopMax, bool& measure);
+	// after the step's final synchronization: the measurement of a pick
+	void tuner_step_end(int32_t slots);
+	int32_t stepFailures = 0; // steps that stopped on an error (m3dx_last_error)
+
 	World();
 	~World();
 };
@@ -312,7 +345,7 @@ bool apply_pending_permutation(World& w);
 
 // ---- step (k_step.hip) ----
 void world_step(World& w, float dt, int substeps);
-void refresh_broadphase(World& w, float dt);
+bool refresh_broadphase(World& w, float dt);
 void wake_body_host(World& w, int32_t i);
 void set_stable_dirty(World& w, int32_t i);
 

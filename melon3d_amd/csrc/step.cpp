@@ -101,7 +101,7 @@ bool push_host_changes(World& w)
 }
 
 // RefreshBroadphase; src/world.cpp:156-229
-void refresh_broadphase(World& w, float dt)
+bool refresh_broadphase(World& w, float dt)
 {
 	if (w.subCount > 0)
 	{
@@ -110,14 +110,14 @@ void refresh_broadphase(World& w, float dt)
 		launch_refresh(w, dt, false);
 		build_tier(w, 0);
 		if (!sync_counters(w))
-			return;
+			return false;
 		if (w.hostCounters[CT_SUB_REBUILDS] > 0)
 		{
 			build_stable_batched(w);
 			launch_stable_migrate(w);
 		}
 		w.gridDirty = false;
-		return;
+		return true;
 	}
 	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
 	launch_refresh(w, dt, rebuildStable);
@@ -136,37 +136,47 @@ void refresh_broadphase(World& w, float dt)
 		++w.stepsSinceStableBuild;
 	}
 	w.gridDirty = false;
+	return true;
 }
+
+static bool world_step_stages(World& w, float dt, int substeps);
 
 void world_step(World& w, float dt, int substeps)
 {
 	if (dt <= 0.0f)
 		return;
-	if (substeps < 1)
-		substeps = 1;
+	if (!world_step_stages(w, dt, substeps < 1 ? 1 : substeps))
+		++w.stepFailures;
+}
+
+// The step's stages; false when one stopped on an error.
+static bool world_step_stages(World& w, float dt, int substeps)
+{
 	auto t0 = Clock::now();
 	w.beginEvents.clear();
 	w.endEvents.clear();
 	memset(&w.stats, 0, sizeof(w.stats));
 	if (!push_host_changes(w))
-		return;
+		return false;
 	hipMemsetAsync(w.counters.p, 0, kCountersSize * sizeof(uint32_t), w.stream);
 
-	refresh_broadphase(w, dt);
+	if (!refresh_broadphase(w, dt))
+		return false;
 	if (!upload_jointed_pairs(w) || !enumerate_and_create(w))
-		return;
+		return false;
 	if (!(w.subCount > 0 ? remove_and_place(w) : remove_separated(w)))
-		return;
+		return false;
 	auto tBroad = Clock::now();
 	if (!narrowphase(w, dt))
-		return;
+		return false;
 	auto tNarrow = Clock::now();
 	if (!islands_and_solve(w, dt, substeps))
-		return;
+		return false;
 	finish_step(w, dt);
 	if (!download_joints(w) || !sync_counters(w))
-		return;
+		return false;
 	const uint32_t* hc = w.hostCounters;
+	w.tuner_step_end((int32_t)hc[CT_SLOTS]);
 	if (w.timing)
 	{
 		// algorithmic bytes of the timed kernels this step (each field read or
@@ -217,6 +227,7 @@ void world_step(World& w, float dt, int substeps)
 	if (w.destroyedSinceStep)
 		(void)prefetch_sidecar(w);
 	w.destroyedSinceStep = false;
+	return true;
 }
 
 } // namespace mx

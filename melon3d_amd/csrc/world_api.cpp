@@ -127,6 +127,10 @@ World::~World()
 		(void)hipHostFree(hostCounters);
 	for (hipEvent_t e : eventPool)
 		(void)hipEventDestroy(e);
+	if (tuner.ev0 != nullptr)
+		(void)hipEventDestroy(tuner.ev0);
+	if (tuner.ev1 != nullptr)
+		(void)hipEventDestroy(tuner.ev1);
 	if (sidecarReady != nullptr)
 		(void)hipEventDestroy(sidecarReady);
 	if (pinnedSidecar != nullptr)
@@ -183,6 +187,69 @@ void World::collect_marks()
 	}
 	marks.clear();
 	eventsUsed = 0;
+}
+
+// ------------------------------------------------------------- pass tuner
+
+constexpr int World::kTunerCands[World::PassTuner::kCands];
+
+int World::tuner_pick(int32_t entries, int coopMax, bool& measure)
+{
+	measure = false;
+	PassTuner& T = tuner;
+	// more workgroups than one lane per entry only add barrier arrivals
+	if (T.coopLimit > 0)
+		coopMax = std::min(coopMax, T.coopLimit);
+	int need = (int)std::min<int64_t>(((int64_t)entries + 255) / 256, (int64_t)std::max(coopMax, 1));
+	need = std::max(need, 1);
+	if (T.forcedBlocks > 0)
+		return std::min(T.forcedBlocks, need);
+	if (need <= kTunerCands[0])
+		return need;
+	int c = 0;
+	while (c + 1 < PassTuner::kClasses && ((int64_t)1 << (c + 1)) <= (int64_t)entries)
+		++c;
+	PassTuner::Class& K = T.cls[c];
+	if (K.chosen > 0)
+		return std::min(K.chosen, need);
+	if (T.ev0 == nullptr && hipEventCreate(&T.ev0) != hipSuccess)
+		T.ev0 = nullptr;
+	if (T.ev1 == nullptr && hipEventCreate(&T.ev1) != hipSuccess)
+		T.ev1 = nullptr;
+	if (T.ev0 == nullptr || T.ev1 == nullptr)
+		return std::min(kTunerCands[1], need);
+	int cand = K.trials % PassTuner::kCands;
+	T.pendingClass = c;
+	T.pendingCand = cand;
+	measure = true;
+	return std::min(kTunerCands[cand], need);
+}
+
+void World::tuner_step_end(int32_t slots)
+{
+	PassTuner& T = tuner;
+	if (T.pendingClass < 0)
+		return;
+	float ms = 0.0f;
+	if (hipEventElapsedTime(&ms, T.ev0, T.ev1) == hipSuccess)
+	{
+		PassTuner::Class& K = T.cls[T.pendingClass];
+		// a candidate the runtime refused (and ran with fewer) is never chosen
+		float per = T.pendingFailed ? 3.0e38f : ms / (float)std::max(slots, 1);
+		float& best = K.perSlot[T.pendingCand];
+		best = K.trials < PassTuner::kCands ? per : std::min(best, per);
+		if (++K.trials >= PassTuner::kCands * PassTuner::kRounds)
+		{
+			int arg = 0;
+			for (int k = 1; k < PassTuner::kCands; ++k)
+				if (K.perSlot[k] < K.perSlot[arg])
+					arg = k;
+			K.chosen = kTunerCands[arg];
+		}
+	}
+	T.pendingClass = -1;
+	T.pendingCand = -1;
+	T.pendingFailed = false;
 }
 
 // ------------------------------------------------------------- transfers
@@ -1474,7 +1541,7 @@ static m3d_ray_result ray_cast_internal(World& w, m3d_vec3 origin, m3d_vec3 tran
 	{
 		if (push_host_changes(w))
 		{
-			refresh_broadphase(w, 0.0f);
+			(void)refresh_broadphase(w, 0.0f);
 			(void)hipStreamSynchronize(w.stream);
 			w.hostStale = true;
 		}
@@ -2208,6 +2275,21 @@ int m3dx_debug_contact_colors(m3d_world* world, int32_t* colorOfContact, int cap
 }
 
 void m3dx_world_stats(m3d_world* world, m3dx_step_stats* out) { *out = world->w.stats; }
+
+int m3dx_set_pass_blocks(m3d_world* world, int blocks)
+{
+	if (blocks < 0)
+	{
+		set_error("pass workgroups must be >= 0");
+		return -1;
+	}
+	world->w.tuner.forcedBlocks = blocks;
+	return 0;
+}
+
+int m3dx_pass_blocks(m3d_world* world) { return world->w.tuner.lastBlocks; }
+
+int32_t m3dx_step_failures(m3d_world* world) { return world != nullptr ? world->w.stepFailures : -1; }
 
 int m3dx_set_kernel_timing(m3d_world* world, int enable)
 {

@@ -557,6 +557,9 @@ int m3dx_diff_jvp_cpu(m3d_world*, int, int, int, const double*, const double*, c
 
 /* no device kernels on the CPU engine: timing is unavailable */
 int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
+int m3dx_set_pass_blocks(m3d_world*, int) { return -1; }
+int m3dx_pass_blocks(m3d_world*) { return 0; }
+int32_t m3dx_step_failures(m3d_world*) { return 0; }
 
 int m3dx_kernel_timing(m3d_world*, m3dx_kernel_time* out)
 {

@@ -99,6 +99,35 @@ def test_batch_equals_plain_worlds(dev, first, S, steps, every):
 
 
 @pytest.mark.gpu
+def test_pass_workgroups_change_nothing(dev):
+    """The solver passes' workgroup count (tuned by measurement, or forced)
+    changes no result: a 48-world batch stepped with 32 and with 256 forced
+    workgroups and with the tuner choosing is bit-identical after 30 steps."""
+    S, steps = 48, 30
+    worlds = []
+    try:
+        for blocks in (32, 256, 0):
+            bw, _ = scenes.batch(dev, S, first=100)
+            assert dev.m3dx_set_pass_blocks(bw, blocks) == 0
+            worlds.append(bw)
+        used = [set() for _ in worlds]
+        for step in range(steps):
+            for k, bw in enumerate(worlds):
+                dev.m3d_world_step(bw, DT, 4)
+                used[k].add(dev.m3dx_pass_blocks(bw))
+        assert [dev.m3dx_step_failures(bw) for bw in worlds] == [0, 0, 0], dev.last_error()
+        assert used[0] != used[1], used
+        assert len(used[2]) > 1, used  # the tuner tried several counts
+        parts = [_split(dev, bw, S) for bw in worlds]
+        for k in (1, 2):
+            for s in range(S):
+                _assert_same(parts[k][s], parts[0][s], s, steps)
+    finally:
+        for bw in worlds:
+            dev.m3d_world_destroy(bw)
+
+
+@pytest.mark.gpu
 def test_batch_against_reference(dev, ref):
     """Four sub-worlds against the reference engine, 3 steps, 1e-5."""
     S, first = 4, 7

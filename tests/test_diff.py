@@ -21,6 +21,7 @@ def _record(dev, scene, steps):
         dev.m3d_world_step(w, DT, 4)
     dev.m3dx_diff_record(w, 0)
     assert dev.m3dx_diff_steps(w) == steps
+    assert dev.m3dx_step_failures(w) == 0, dev.last_error()
     return w
 
 
