@@ -1348,32 +1348,54 @@ __global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const 
 	C.mat[c] = m;
 }
 
-__global__ void k_pack_sidecar(const int2* bodies, const float4* mat, int32_t C, int4* out, uint32_t* perBody)
+// Sidecar record of every contact, and the (body, contact side) pairs the
+// per-body contact index is sorted from: entry 2c + side is contact c's
+// body A (side 0) or B (side 1). A sort replaces per-body atomic counters,
+// which serialize on bodies in many contacts (a ground).
+__global__ void k_pack_sidecar(const int2* bodies, const float4* mat, int32_t C, int4* out, uint64_t* keys,
+							   uint32_t* vals)
 {
 	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (c < C)
 	{
 		int2 b = bodies[c];
 		out[c] = make_int4(b.x, b.y, (int32_t)f2u(mat[c].w), c);
-		atomicAdd(&perBody[b.x], 1u);
-		atomicAdd(&perBody[b.y], 1u);
+		keys[2 * (int64_t)c] = (uint64_t)(uint32_t)b.x;
+		keys[2 * (int64_t)c + 1] = (uint64_t)(uint32_t)b.y;
+		vals[2 * (int64_t)c] = (uint32_t)c;
+		vals[2 * (int64_t)c + 1] = (uint32_t)c | 0x80000000u;
 	}
 }
 
-// per-body contact index: list[start[b] ..] = positions of body b's
-// contacts (any order), slot[c] = the two entries of contact c
-__global__ void k_contact_csr(const int2* bodies, int32_t C, const uint32_t* start, uint32_t* cursor, int32_t* list,
-							  int2* slot)
+// per-body contact index from the sorted pairs: list[start[b] ..] = body
+// b's contacts (ascending), slot[c] = the two entries of contact c
+__global__ void k_contact_csr(const uint32_t* vals, int64_t m, int32_t* list, int32_t* slot)
 {
-	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (c >= C)
+	int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= m)
 		return;
-	int2 b = bodies[c];
-	int32_t sa = (int32_t)(start[b.x] + atomicAdd(&cursor[b.x], 1u));
-	int32_t sb = (int32_t)(start[b.y] + atomicAdd(&cursor[b.y], 1u));
-	list[sa] = c;
-	list[sb] = c;
-	slot[c] = make_int2(sa, sb);
+	uint32_t v = vals[k];
+	int32_t c = (int32_t)(v & 0x7FFFFFFFu);
+	list[k] = c;
+	slot[2 * (int64_t)c + (v >> 31)] = (int32_t)k;
+}
+
+// start[b] = first sorted entry of body b (lower bound), b = 0 .. n
+__global__ void k_csr_start(const uint64_t* keys, int64_t m, int32_t n, uint32_t* start)
+{
+	int32_t b = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (b > n)
+		return;
+	int64_t lo = 0, hi = m;
+	while (lo < hi)
+	{
+		int64_t mid = (lo + hi) >> 1;
+		if (keys[mid] < (uint64_t)b)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	start[b] = (uint32_t)lo;
 }
 
 bool prefetch_sidecar(World& w)
@@ -1412,19 +1434,24 @@ bool prefetch_sidecar(World& w)
 	}
 	if (w.sidecarReady == nullptr && hipEventCreateWithFlags(&w.sidecarReady, hipEventDisableTiming) != hipSuccess)
 		return false;
-	// device layout: start (n + 1), list, slot, then per-body counts (n + 1)
-	if (!w.dSidecar.ensure(C) || !w.dCsr.ensure(csrInts + (size_t)n + 1) || !ensure_prim_scratch(w, (int64_t)n + 1))
+	// device layout: start (n + 1), list (2C), slot (2C)
+	const int64_t m = 2 * (int64_t)C;
+	if (!w.dSidecar.ensure(C) || !w.dCsr.ensure(csrInts) || !w.csrKeys.ensure(m) || !w.csrKeysAlt.ensure(m) ||
+		!w.csrVals.ensure(m) || !w.csrValsAlt.ensure(m) || !ensure_prim_scratch(w, m))
 		return false;
 	hipStream_t s = w.stream;
 	uint32_t* start = (uint32_t*)w.dCsr.p;
 	int32_t* list = w.dCsr.p + n + 1;
-	int2* slot = (int2*)(list + 2 * (size_t)C);
-	uint32_t* counts = (uint32_t*)(w.dCsr.p + csrInts);
-	launch_fill_u32(counts, 0u, (int64_t)n + 1, s);
-	k_pack_sidecar<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, C, w.dSidecar.p, counts);
-	scan_exclusive(counts, start, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
-	launch_fill_u32(counts, 0u, (int64_t)n + 1, s);
-	k_contact_csr<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, C, start, counts, list, slot);
+	int32_t* slot = list + m;
+	int bits = 1;
+	while (bits < 31 && ((int64_t)1 << bits) <= (int64_t)n)
+		++bits;
+	k_pack_sidecar<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, C, w.dSidecar.p, w.csrKeys.p,
+												  w.csrVals.p);
+	radix_sort_pairs(w.csrKeys.p, w.csrVals.p, w.csrKeysAlt.p, w.csrValsAlt.p, m, bits, w.primScratch.p,
+					 w.scanStatus.p, s);
+	k_csr_start<<<grid_for((int64_t)n + 1), kBlock, 0, s>>>(w.csrKeys.p, m, n, start);
+	k_contact_csr<<<grid_for(m), kBlock, 0, s>>>(w.csrVals.p, m, list, slot);
 	if (hipMemcpyAsync(w.pinnedSidecar, w.dSidecar.p, (size_t)C * sizeof(int4), hipMemcpyDeviceToHost, s) !=
 			hipSuccess ||
 		hipMemcpyAsync(w.pinnedCsr, w.dCsr.p, csrInts * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||

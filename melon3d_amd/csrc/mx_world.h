@@ -180,6 +180,8 @@ struct World
 	size_t pinnedCsrCap = 0;
 	int32_t sidecarPrefetchN = 0;
 	DBuf<int32_t> dCsr;
+	DBuf<uint64_t> csrKeys, csrKeysAlt; // (body, side) pairs sorted into the contact index
+	DBuf<uint32_t> csrVals, csrValsAlt;
 	// bodies to wake on the device at the next step (partners of destroyed
 	// bodies whose host record is not current)
 	// packed sidecar (body a, body b, flags) copied to pinned memory at the

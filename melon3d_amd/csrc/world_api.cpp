@@ -138,6 +138,10 @@ World::~World()
 	if (pinnedCsr != nullptr)
 		(void)hipHostFree(pinnedCsr);
 	dCsr.release();
+	csrKeys.release();
+	csrKeysAlt.release();
+	csrVals.release();
+	csrValsAlt.release();
 	dSidecar.release();
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
