@@ -348,6 +348,8 @@ def run_batch(args, dev):
     prof = dev.m3d_world_profile(w)
     st = dev.stats(w)
     value = dyn * args.steps / elapsed
+    pass_blocks = dev.m3dx_pass_blocks(w)
+    graph_steps = dev.m3dx_step_graph_launches(w)
     dev.m3dx_set_kernel_timing(w, 1)
     for i in range(args.timing_steps):
         dev.m3d_world_step(w, DT, args.substeps)
@@ -384,6 +386,7 @@ def run_batch(args, dev):
                                f"{args.substeps} substeps",
                    "worlds": args.worlds, "bodies": dyn, "mean_bodies": dyn / args.worlds,
                    "contacts": prof.contactCount, "islands": prof.islandCount, "awake": prof.awakeBodyCount,
+                   "pass_workgroups": pass_blocks, "graph_steps": graph_steps,
                    "parallelism": "single GPU, one launch sequence for all worlds"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": launches,

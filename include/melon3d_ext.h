@@ -269,6 +269,13 @@ M3D_API int m3dx_pass_blocks(m3d_world* world);
 /* Steps of this world that stopped on an error (m3dx_last_error holds the
    last message; m3d_world_step has no return value). */
 M3D_API int32_t m3dx_step_failures(m3d_world* world);
+/* A batch runs its island-solve substep launches (integration, prepare,
+   records, the cooperative passes) as one hipGraph once the passes'
+   workgroup count is settled; enable = 0 launches them directly. Results do
+   not depend on it. Returns 0 or -1. */
+M3D_API int m3dx_set_step_graphs(m3d_world* world, int enable);
+/* Steps whose substep loop ran as a graph. */
+M3D_API int64_t m3dx_step_graph_launches(m3d_world* world);
 /* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
 M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
 

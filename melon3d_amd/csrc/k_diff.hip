@@ -177,6 +177,8 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps)
 	return hip_ok(hipStreamSynchronize(s), "step record");
 }
 
+bool diff_is_recording(const World& w) { return w.diff != nullptr && w.diff->recording; }
+
 bool diff_record_substep(World& w, int sub)
 {
 	DiffState* D = w.diff;

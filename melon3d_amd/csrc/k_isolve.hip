@@ -1060,6 +1060,55 @@ static int giant_blocks(World& w, K kernel, int32_t entries, int cap)
 	return std::max(1, std::min(b, coop_blocks(w, kernel)));
 }
 
+// Captures `body`'s launches on the world's stream into a graph and launches
+// it: the world's executable graph is updated in place when the topology
+// matches (kernel arguments and grid sizes change from step to step) and
+// instantiated again otherwise. A capture the runtime refuses falls back to
+// direct launches (and turns graphs off for the world).
+template <class Body>
+static bool run_step_graph(World& w, hipStream_t s, Body&& body)
+{
+	if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		w.stepGraphs = false;
+		return body();
+	}
+	bool ok = body();
+	hipGraph_t g = nullptr;
+	hipError_t e = hipStreamEndCapture(s, &g);
+	if (!ok || e != hipSuccess || g == nullptr)
+	{
+		if (g != nullptr)
+			(void)hipGraphDestroy(g);
+		(void)hipGetLastError();
+		w.stepGraphs = false;
+		return body();
+	}
+	if (w.stepGraphExec != nullptr)
+	{
+		hipGraphNode_t errNode = nullptr;
+		hipGraphExecUpdateResult res;
+		if (hipGraphExecUpdate(w.stepGraphExec, g, &errNode, &res) != hipSuccess)
+		{
+			(void)hipGetLastError();
+			(void)hipGraphExecDestroy(w.stepGraphExec);
+			w.stepGraphExec = nullptr;
+		}
+	}
+	if (w.stepGraphExec == nullptr && hipGraphInstantiate(&w.stepGraphExec, g, nullptr, nullptr, 0) != hipSuccess)
+	{
+		(void)hipGetLastError();
+		w.stepGraphExec = nullptr;
+		(void)hipGraphDestroy(g);
+		w.stepGraphs = false;
+		return body();
+	}
+	(void)hipGraphDestroy(g);
+	++w.stepGraphLaunches;
+	return hip_ok(hipGraphLaunch(w.stepGraphExec, s), "step graph");
+}
+
 bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 {
 	hipStream_t s = w.stream;
@@ -1215,6 +1264,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 			w.tuner.pendingFailed = measure;
 		}
 	};
+	auto substepLoop = [&]() -> bool {
 	for (int sub = 0; sub < substeps; ++sub)
 	{
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
@@ -1238,9 +1288,19 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		if (!ok)
 			return false;
 	}
+	return true;
+	};
+	// A batch runs the loop as one graph once its workgroup count is settled
+	// and was accepted by a direct cooperative launch; never while the tuner
+	// measures, kernels are timed or steps are recorded.
+	bool graph = w.subCount > 0 && w.stepGraphs && !measure && !w.timing && blocks == w.validatedBlocks &&
+				 !diff_is_recording(w);
+	bool ok = graph ? run_step_graph(w, s, substepLoop) : substepLoop();
+	if (ok && !graph)
+		w.validatedBlocks = blocks;
 	if (measure)
 		(void)hipEventRecord(w.tuner.ev1, s);
-	return true;
+	return ok;
 }
 
 } // namespace mx

@@ -116,6 +116,7 @@ bool remove_and_place(World& w);
 bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps);
 // records the manifold each substep's prepare left
 bool diff_record_substep(World& w, int sub);
+bool diff_is_recording(const World& w);
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);

@@ -327,6 +327,12 @@ struct World
 	// after the step's final synchronization: the measurement of a pick
 	void tuner_step_end(int32_t slots);
 	int32_t stepFailures = 0; // steps that stopped on an error (m3dx_last_error)
+	// the giant substep loop of a batch as one hipGraph (captured each step,
+	// the executable graph updated in place)
+	bool stepGraphs = true;
+	int validatedBlocks = 0; // workgroups a direct cooperative launch accepted
+	hipGraphExec_t stepGraphExec = nullptr;
+	int64_t stepGraphLaunches = 0;
 
 	World();
 	~World();

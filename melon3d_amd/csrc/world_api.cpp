@@ -127,6 +127,8 @@ World::~World()
 		(void)hipHostFree(hostCounters);
 	for (hipEvent_t e : eventPool)
 		(void)hipEventDestroy(e);
+	if (stepGraphExec != nullptr)
+		(void)hipGraphExecDestroy(stepGraphExec);
 	if (tuner.ev0 != nullptr)
 		(void)hipEventDestroy(tuner.ev0);
 	if (tuner.ev1 != nullptr)
@@ -2294,6 +2296,16 @@ int m3dx_set_pass_blocks(m3d_world* world, int blocks)
 int m3dx_pass_blocks(m3d_world* world) { return world->w.tuner.lastBlocks; }
 
 int32_t m3dx_step_failures(m3d_world* world) { return world != nullptr ? world->w.stepFailures : -1; }
+
+int m3dx_set_step_graphs(m3d_world* world, int enable)
+{
+	if (world == nullptr)
+		return -1;
+	world->w.stepGraphs = enable != 0;
+	return 0;
+}
+
+int64_t m3dx_step_graph_launches(m3d_world* world) { return world != nullptr ? world->w.stepGraphLaunches : -1; }
 
 int m3dx_set_kernel_timing(m3d_world* world, int enable)
 {

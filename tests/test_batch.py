@@ -128,6 +128,33 @@ def test_pass_workgroups_change_nothing(dev):
 
 
 @pytest.mark.gpu
+def test_step_graph_changes_nothing(dev):
+    """The batch's substep loop run as a captured hipGraph (the default once
+    the workgroup count is settled) equals the directly launched loop bit for
+    bit over 30 steps."""
+    S, steps = 48, 30
+    worlds = []
+    try:
+        for graphs in (0, 1):
+            bw, _ = scenes.batch(dev, S, first=200)
+            assert dev.m3dx_set_pass_blocks(bw, 64) == 0
+            assert dev.m3dx_set_step_graphs(bw, graphs) == 0
+            worlds.append(bw)
+        for step in range(steps):
+            for bw in worlds:
+                dev.m3d_world_step(bw, DT, 4)
+        assert [dev.m3dx_step_failures(bw) for bw in worlds] == [0, 0], dev.last_error()
+        assert dev.m3dx_step_graph_launches(worlds[0]) == 0
+        assert dev.m3dx_step_graph_launches(worlds[1]) >= steps - 2
+        direct, graphed = (_split(dev, bw, S) for bw in worlds)
+        for s in range(S):
+            _assert_same(graphed[s], direct[s], s, steps)
+    finally:
+        for bw in worlds:
+            dev.m3d_world_destroy(bw)
+
+
+@pytest.mark.gpu
 def test_batch_against_reference(dev, ref):
     """Four sub-worlds against the reference engine, 3 steps, 1e-5."""
     S, first = 4, 7
