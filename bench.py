@@ -61,13 +61,14 @@ def traffic_bytes(st, substeps: int) -> float:
     return float(S * per_sub + once)
 
 
-def measured_traffic(scene: str, bodies: int, kernel: str):
+def measured_traffic(scene: str, size: int, kernel: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on this
     workload, from the committed profile summary; None when not measured."""
-    if scene != "stream" or bodies != 200000:
+    files = {("stream", 200000): "stream200k_grid_normals.md", ("batch", 4096): "batch4096_kernels.md"}
+    if (scene, size) not in files:
         return None
-    path = os.path.join(ROOT, "profiles", "stream200k_grid_normals.md")
+    path = os.path.join(ROOT, "profiles", files[(scene, size)])
     try:
         with open(path) as f:
             for line in f:
@@ -389,8 +390,9 @@ def run_batch(args, dev):
                    "pass_workgroups": pass_blocks, "graph_steps": graph_steps,
                    "parallelism": "single GPU, one launch sequence for all worlds"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": launches,
-                     "avg_launch_us": avg_s * 1e6, "bytes_per_launch": (kbytes / launches) if launches else 0.0},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch", args.worlds, name),
+                     "launches": launches, "avg_launch_us": avg_s * 1e6,
+                     "bytes_per_launch": (kbytes / launches) if launches else 0.0},
         "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
         "loop_plain_gpu": loop,
         "cpu_baseline": cpu,
