@@ -781,7 +781,54 @@ struct GiantSolveCtx
 	int bitsI;
 	uint32_t* bucketCount;     // per (slot, group); zero between substeps
 	uint32_t* bucketCursor;    // per (slot, group)
+	// normal-pass inputs of each scheduled contact at its record position:
+	// block 0 the normal, 1-4 anchors A (+ separation offset), 5-8 anchors B
+	// (+ wN), 9-12 angJA (+ lambdaN), 13-16 angJB; block b of record k at
+	// b * nrmStride + k. The pass reads them coalesced and accumulates
+	// lambdaN there; k_giant_lambda_back returns lambdaN to the contacts.
+	float4* nrm;
+	int64_t nrmStride;
 };
+
+// Contact-side inputs of a scheduled record e (its contact r.x) and the pass
+// itself; the normal pass reads the schedule-ordered copy.
+template <class Pass>
+__device__ __forceinline__ void sweep_load(const GiantSolveCtx& S, int32_t e, int4 r, typename Pass::In& d)
+{
+	Pass::load(S.X.P, r, d);
+}
+
+template <class Pass>
+__device__ __forceinline__ void sweep_run(const GiantSolveCtx& S, int32_t e, int4 r, const typename Pass::In& d)
+{
+	Pass::run(S.X.P, r, d);
+}
+
+template <>
+__device__ __forceinline__ void sweep_load<NormalsPass>(const GiantSolveCtx& S, int32_t e, int4 r, NormalsIn& d)
+{
+	const int64_t st = S.nrmStride;
+	d.nc = S.nrm[e];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		d.A4[j] = S.nrm[(1 + j) * st + e];
+		d.B4[j] = S.nrm[(5 + j) * st + e];
+		d.JA[j] = S.nrm[(9 + j) * st + e];
+		d.JB[j] = S.nrm[(13 + j) * st + e];
+	}
+}
+
+template <>
+__device__ __forceinline__ void sweep_run<NormalsPass>(const GiantSolveCtx& S, int32_t e, int4 r, const NormalsIn& d)
+{
+	// lambdaN accumulates in the copy: a contact view whose angJA column is
+	// the copy's block 9, indexed by record position
+	PassCtx P = S.X.P;
+	P.C.pJA = S.nrm + 9 * S.nrmStride;
+	P.C.cap = (int32_t)S.nrmStride;
+	NormalsPass::run(P, make_int4(e, r.y, r.z, r.w), d);
+}
 
 // Slots with at most this many entries are run by workgroup 0 alone: a run
 // of consecutive small slots then costs one grid barrier instead of one per
@@ -824,7 +871,7 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 	if (e >= 0)
 	{
 		r = S.rec[e];
-		Pass::load(P, r, d);
+		sweep_load<Pass>(S, e, r, d);
 	}
 	bool pendingSmall = false;
 	for (int32_t s = 0; s < nSlots; ++s)
@@ -838,9 +885,14 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 		}
 		if (e >= 0)
 		{
-			Pass::run(P, r, d);
+			sweep_run<Pass>(S, e, r, d);
 			for (int32_t e2 = e + stride; e2 < en; e2 += stride)
-				run_rec<Pass>(P, S.rec[e2]);
+			{
+				typename Pass::In d2;
+				int4 r2 = S.rec[e2];
+				sweep_load<Pass>(S, e2, r2, d2);
+				sweep_run<Pass>(S, e2, r2, d2);
+			}
 		}
 		int32_t strideNext = stride;
 		int32_t eNext = s + 1 < nSlots ? giant_lane_entry(ss, s + 1, tid, strideNext) : -1;
@@ -854,7 +906,7 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 		if (e >= 0)
 		{
 			r = S.rec[e];
-			Pass::load(P, r, d);
+			sweep_load<Pass>(S, e, r, d);
 		}
 		if (big)
 			grid.wait();
@@ -973,11 +1025,37 @@ __global__ void k_giant_records(GiantSolveCtx S)
 		int32_t c = (int32_t)S.entries[e];
 		uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
 		S.rec[at] = make_rec(P, c, mp);
-		points = (uint32_t)f2i(P.C.normal[c].w);
+		float4 nc = P.C.normal[c];
+		points = (uint32_t)f2i(nc.w);
+		const int64_t st = S.nrmStride;
+		S.nrm[at] = nc;
+		for (int j = 0; j < 4 && j < (int)points; ++j)
+		{
+			int32_t p = pidx(P.C, c, j);
+			S.nrm[(1 + j) * st + at] = P.C.pA[p];
+			S.nrm[(5 + j) * st + at] = P.C.pB[p];
+			S.nrm[(9 + j) * st + at] = P.C.pJA[p];
+			S.nrm[(13 + j) * st + at] = P.C.pJB[p];
+		}
 	}
 	// entries and points of the substep, for the byte model of the passes
 	wave_add(&S.counters2[CT_GIANT_ENTRIES], valid ? 1u : 0u);
 	wave_add(&S.counters2[CT_GIANT_POINTS], points);
+}
+
+// lambdaN of the normal pass back from the schedule-ordered copy to the
+// contacts (the stiction and velocity passes and the next steps read it there)
+__global__ void k_giant_lambda_back(GiantSolveCtx S)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	const DContacts& C = S.X.P.C;
+	int32_t c = S.rec[k].x;
+	const int64_t st = S.nrmStride;
+	int32_t count = f2i(S.nrm[k].w);
+	for (int j = 0; j < 4 && j < count; ++j)
+		C.pJA[pidx(C, c, j)].w = S.nrm[(9 + j) * st + k].w;
 }
 
 // `passes` sweeps of one pass kind. The barrier counter is zeroed again by
@@ -1176,7 +1254,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		++bitsS;
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
-		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
+		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) ||
+		!w.sNrm.ensure(17 * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
 		return false;
@@ -1226,6 +1305,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.counters2 = w.counters.p;
 	S.barrier = w.sCtr.p + 9;
 	S.rec = w.sRecG.p;
+	S.nrm = w.sNrm.p;
+	S.nrmStride = NC;
 	S.keys = w.s64b.p;
 	S.bitsI = L.bitsI;
 	S.bucketCount = w.sBucket.p;
@@ -1276,6 +1357,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		w.mark_begin(M3DX_K_GIANT_NORMALS);
 		bool ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
 		w.mark_end();
+		k_giant_lambda_back<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_FRICTION);
 		ok = ok && coop((const void*)k_giant_pass<FrictionPass>, 1);
 		w.mark_end();

@@ -239,6 +239,7 @@ struct World
 	DBuf<uint32_t> sCtr;          // convergence counters of the giant-island fixpoints
 	DBuf<int2> sPacket;
 	DBuf<int4> sRecG;             // grid path: solve records by sorted entry
+	DBuf<float4> sNrm;            // grid path: normal-pass inputs by schedule position (17 x entries)
 	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
 	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry
 	DBuf<uint8_t> sColored;       // grid path: entry's island is colored

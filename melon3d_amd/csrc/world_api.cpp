@@ -114,6 +114,7 @@ World::~World()
 	sCtr.release();
 	sPacket.release();
 	sRecG.release();
+	sNrm.release();
 	sBucket.release();
 	sPred.release();
 	sColored.release();
