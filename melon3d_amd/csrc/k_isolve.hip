@@ -781,14 +781,21 @@ struct GiantSolveCtx
 	int bitsI;
 	uint32_t* bucketCount;     // per (slot, group); zero between substeps
 	uint32_t* bucketCursor;    // per (slot, group)
-	// normal-pass inputs of each scheduled contact at its record position:
-	// block 0 the normal, 1-4 anchors A (+ separation offset), 5-8 anchors B
-	// (+ wN), 9-12 angJA (+ lambdaN), 13-16 angJB; block b of record k at
-	// b * nrmStride + k. The pass reads them coalesced and accumulates
-	// lambdaN there; k_giant_lambda_back returns lambdaN to the contacts.
+	// contact-side inputs of the three passes for each scheduled contact at
+	// its record position (block b of record k at b * nrmStride + k):
+	// 0 normal, 1-4 anchors A (+ separation offset), 5-8 anchors B (+ wN),
+	// 9-12 angJA (+ lambdaN), 13-16 angJB (+ lambdaT), 17 material, 18-21
+	// slip directions, 22-25 / 26-29 frozen arms A / B (flat contacts),
+	// 30-33 / 34-37 friction anchors A / B (box-box contacts). The passes
+	// read them coalesced; the normal pass accumulates lambdaN there and
+	// k_giant_lambda_back returns it to the contacts. Everything a pass
+	// writes besides lambdaN (lambdaT, re-anchoring, slip) goes to the
+	// contacts, which only the contact itself reads later in the substep.
 	float4* nrm;
 	int64_t nrmStride;
 };
+
+constexpr int kSchedBlocks = 38;
 
 // Contact-side inputs of a scheduled record e (its contact r.x) and the pass
 // itself; the normal pass reads the schedule-ordered copy.
@@ -816,6 +823,46 @@ __device__ __forceinline__ void sweep_load<NormalsPass>(const GiantSolveCtx& S, 
 		d.B4[j] = S.nrm[(5 + j) * st + e];
 		d.JA[j] = S.nrm[(9 + j) * st + e];
 		d.JB[j] = S.nrm[(13 + j) * st + e];
+	}
+}
+
+template <>
+__device__ __forceinline__ void sweep_load<FrictionPass>(const GiantSolveCtx& S, int32_t e, int4 r, FrictionIn& d)
+{
+	if (!(r.w & kRecBoxBox))
+		return;
+	const int64_t st = S.nrmStride;
+	d.nc = S.nrm[e];
+	d.mu = S.nrm[17 * st + e].x;
+	d.RA0 = S.nrm[22 * st + e];
+	d.RB0 = S.nrm[26 * st + e];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		d.LN[j] = S.nrm[(9 + j) * st + e].w;
+		d.FA[j] = S.nrm[(30 + j) * st + e];
+		d.FB[j] = S.nrm[(34 + j) * st + e];
+		d.JB[j] = S.nrm[(13 + j) * st + e];
+	}
+}
+
+template <>
+__device__ __forceinline__ void sweep_load<VelocityPass>(const GiantSolveCtx& S, int32_t e, int4 r, VelocityIn& d)
+{
+	const int64_t st = S.nrmStride;
+	d.nc = S.nrm[e];
+	d.mat = S.nrm[17 * st + e];
+	bool curved = (f2u(d.mat.w) & CF_CURVED) != 0;
+	const int64_t ra = curved ? 1 : 22, rb = curved ? 5 : 26;
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		d.JA[j] = S.nrm[(9 + j) * st + e];
+		d.B4[j] = S.nrm[(5 + j) * st + e];
+		d.SL[j] = S.nrm[(18 + j) * st + e];
+		d.JBv[j] = S.nrm[(13 + j) * st + e];
+		d.AR[j] = S.nrm[(ra + j) * st + e];
+		d.BR[j] = S.nrm[(rb + j) * st + e];
 	}
 }
 
@@ -1025,10 +1072,13 @@ __global__ void k_giant_records(GiantSolveCtx S)
 		int32_t c = (int32_t)S.entries[e];
 		uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
 		S.rec[at] = make_rec(P, c, mp);
-		float4 nc = P.C.normal[c];
+		float4 nc = P.C.normal[c], mat = P.C.mat[c];
 		points = (uint32_t)f2i(nc.w);
 		const int64_t st = S.nrmStride;
+		const bool flat = (f2u(mat.w) & CF_CURVED) == 0;
+		const bool boxBox = (S.rec[at].w & kRecBoxBox) != 0;
 		S.nrm[at] = nc;
+		S.nrm[17 * st + at] = mat;
 		for (int j = 0; j < 4 && j < (int)points; ++j)
 		{
 			int32_t p = pidx(P.C, c, j);
@@ -1036,6 +1086,17 @@ __global__ void k_giant_records(GiantSolveCtx S)
 			S.nrm[(5 + j) * st + at] = P.C.pB[p];
 			S.nrm[(9 + j) * st + at] = P.C.pJA[p];
 			S.nrm[(13 + j) * st + at] = P.C.pJB[p];
+			S.nrm[(18 + j) * st + at] = P.C.pSlip[p];
+			if (flat || (boxBox && j == 0))
+			{
+				S.nrm[(22 + j) * st + at] = P.C.pRA[p];
+				S.nrm[(26 + j) * st + at] = P.C.pRB[p];
+			}
+			if (boxBox)
+			{
+				S.nrm[(30 + j) * st + at] = P.C.pFA[p];
+				S.nrm[(34 + j) * st + at] = P.C.pFB[p];
+			}
 		}
 	}
 	// entries and points of the substep, for the byte model of the passes
@@ -1255,7 +1316,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
 		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) ||
-		!w.sNrm.ensure(17 * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
+		!w.sNrm.ensure(kSchedBlocks * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
 		return false;
