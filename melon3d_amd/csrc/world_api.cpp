@@ -11,6 +11,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <stdlib.h>
 
 namespace mx
 {
@@ -35,7 +36,13 @@ bool hip_ok(hipError_t e, const char* what)
 	return false;
 }
 
-World::World() {}
+World::World()
+{
+	// M3D_STEP_GRAPHS=0 launches a batch's substep loop directly (profilers
+	// that intercept graph launches)
+	const char* g = getenv("M3D_STEP_GRAPHS");
+	stepGraphs = !(g != nullptr && g[0] == '0');
+}
 
 World::~World()
 {
