@@ -418,7 +418,41 @@ def host_cores() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, dyn: int):
+def make_rays(n: int, seed: int, arena: float):
+    """n segments over the arena: origins x, z ~ U[-arena/2, arena/2], y ~
+    U[1, 60]; unit directions with a downward bias; length 60 m."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    o = np.column_stack([rng.uniform(-arena / 2, arena / 2, n), rng.uniform(1.0, 60.0, n),
+                         rng.uniform(-arena / 2, arena / 2, n)]).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 1] -= 1.5
+    d = (60.0 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return np.ascontiguousarray(o), np.ascontiguousarray(d)
+
+
+def ray_bench(dev, w, n: int, arena: float):
+    """m3dx_world_ray_cast_batch of n rays: end-to-end rays/s (host arrays in
+    and out) and the hit count."""
+    import ctypes as C
+
+    from melon3d_amd.api import m3d_ray_result
+
+    o, d = make_rays(n, 99, arena)
+    out = (m3d_ray_result * n)()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    dev.m3dx_world_ray_cast_batch(w, p(o[:1]), p(d[:1]), None, 1, out)  # first-call setup
+    t0 = time.perf_counter()
+    if dev.m3dx_world_ray_cast_batch(w, p(o), p(d), None, n, out) != 0:
+        raise SystemExit(dev.last_error())
+    dt = time.perf_counter() - t0
+    hits = sum(1 for k in range(0, n, 97) if out[k].hit)
+    return n / dt, hits * 97 / n
+
+
+def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, dyn: int, rays: int = 0,
+                 arena: float = 200.0):
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libm3dref_fast.so")
     if not os.path.exists(ref_path):
         return None
@@ -439,10 +473,19 @@ def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, d
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    ray_rate = None
+    if rays:
+        from melon3d_amd.api import m3d_vec3
+
+        o, d = make_rays(rays, 99, arena)
+        t1 = time.perf_counter()
+        for j in range(rays):
+            ref.m3d_world_ray_cast(rw, m3d_vec3(*map(float, o[j])), m3d_vec3(*map(float, d[j])))
+        ray_rate = rays / (time.perf_counter() - t1)
     ref.m3d_world_destroy(rw)
     return {"value": dyn * k / dt, "unit": "body-steps/s", "cores": cores, "kind": "reference",
             "sample": f"{k} steps ({'with the same churn, ' if rs else ''}from the GPU's timed-region start state), "
-                      f"reference built -O3 -mavx2 -mfma, workerCount={cores}"}
+                      f"reference built -O3 -mavx2 -mfma, workerCount={cores}", "ray_cast_rays_per_s": ray_rate}
 
 
 def main():
@@ -548,7 +591,16 @@ def main():
     cpu = None
     if snap is not None:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(snap, cpu_stream, args.cpu_steps, args.cpu_budget, args.substeps, dyn)
+        cpu = cpu_baseline(snap, cpu_stream, args.cpu_steps, args.cpu_budget, args.substeps, dyn, rays=2000,
+                           arena=200.0 if args.scene == "stream" else 40.0)
+    rays = None
+    if args.scene in ("stream", "rain"):
+        rate, frac = ray_bench(dev, w, 1 << 20, 200.0 if args.scene == "stream" else 40.0)
+        rays = {"rays": 1 << 20, "device_batch_rays_per_s": rate, "hit_fraction": frac,
+                "reference_cpu_rays_per_s": cpu.get("ray_cast_rays_per_s") if cpu else None,
+                "query": "m3dx_world_ray_cast_batch (host arrays in and out) vs the reference's m3d_world_ray_cast "
+                         "called once per ray through ctypes (2000 rays) on the same state; segments of 60 m over "
+                         "the arena"}
 
     out = {
         "metric": "body-steps/sec",
@@ -581,6 +633,7 @@ def main():
                                    "step_narrowphase": prof_ms[1] / args.steps,
                                    "step_solver": prof_ms[2] / args.steps},
         "cpu_baseline": cpu,
+        "ray_cast": rays,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]

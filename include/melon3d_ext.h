@@ -373,6 +373,18 @@ M3D_API int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal
                               const double* dStateIn, const double* mass, const double* dMass,
                               const double* friction, const double* dFriction, double* stateOut, double* dStateOut);
 
+/* ---------------------------------------------- scene queries */
+
+/* Batched ray casts on the device (no counterpart in the reference, which
+   casts one ray per call): out[k] = m3d_world_ray_cast_filtered(world,
+   origins[k], translations[k], maskBits ? maskBits[k] : 0xFFFFFFFF),
+   evaluated as the reference's ray cast (include/melon3d.h:510-514,
+   src/world.cpp:1081-1200: coarse lists, then the grid cells along the
+   segment) over the device grid, one lane per ray. Plain worlds only.
+   Returns 0 or -1. */
+M3D_API int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m3d_vec3* translations,
+                                      const uint32_t* maskBits, int32_t count, m3d_ray_result* out);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libmelon3d_amd.so")
 
-SOURCES = ["k_prims.hip", "k_broad.hip", "k_narrow.hip", "k_solve.hip", "k_isolve.hip", "k_diff.hip", "k_adjoint.hip", "step.cpp", "world_api.cpp"]
+SOURCES = ["k_prims.hip", "k_broad.hip", "k_narrow.hip", "k_solve.hip", "k_isolve.hip", "k_diff.hip", "k_adjoint.hip", "k_query.hip", "step.cpp", "world_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("M3D_OFFLOAD_ARCH", "gfx950")

@@ -231,16 +231,6 @@ void launch_stable_migrate(World& w)
 
 // -------------------------------------------------------------- tiers
 
-MX_HD int64_t cell_coord(float v, float inv) { return (int64_t)floorf(v * inv); }
-
-MX_HD uint64_t pack_cell(int64_t x, int64_t y, int64_t z)
-{
-	const int64_t off = (int64_t)1 << 20;
-	uint64_t ux = (uint64_t)(x + off) & 0x1FFFFFu;
-	uint64_t uy = (uint64_t)(y + off) & 0x1FFFFFu;
-	uint64_t uz = (uint64_t)(z + off) & 0x1FFFFFu;
-	return (ux << 42) | (uy << 21) | uz;
-}
 
 MX_HD uint64_t cell_key_of(m3d_vec3 p, float cellSize)
 {

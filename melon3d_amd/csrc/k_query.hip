@@ -1,0 +1,246 @@
+// Scene queries on the device: batched ray casts over the world's grid
+// tiers, one lane per ray, each the reference's RayCastInternal
+// (src/world.cpp:1081-1200): the coarse lists first (active, then stable),
+// then the cells along the segment by the Amanatides & Woo walk, each
+// cell's entries found by binary search in the sorted active tier, then the
+// stable tier; a body is taken when its hit fraction is strictly below the
+// best so far. The reference skips a body it has already tested in this
+// query; testing it again cannot change the result (same fraction, strict
+// comparison), so the lanes keep no per-query marks.
+#include "mx_steps.h"
+
+#include <float.h>
+#include <vector>
+
+namespace mx
+{
+
+DBodies make_body_view(World& w);
+
+struct RayCtx
+{
+	DBodies B;
+	const uint64_t* aKeys;
+	const uint32_t* aBodies;
+	int64_t aCount;
+	const uint64_t* sKeys;
+	const uint32_t* sBodies;
+	int64_t sCount;
+	const int32_t* aOver;
+	int32_t nAOver;
+	const int32_t* sOver;
+	int32_t nSOver;
+	float cellSize;
+};
+
+struct RayHit
+{
+	int32_t body;
+	float fraction;
+	m3d_vec3 normal;
+};
+
+__device__ __forceinline__ void ray_test(const RayCtx& R, int32_t bi, m3d_vec3 o, m3d_vec3 d, uint32_t mask,
+										 float& maxFraction, RayHit& h)
+{
+	uint32_t fl = R.B.flags[bi];
+	if (!(fl & BF_IN_USE) || (R.B.filter[bi].x & mask) == 0u)
+		return;
+	Shape s = load_shape(R.B, bi);
+	m3d_transform xf = load_xf(R.B, bi);
+	float f;
+	m3d_vec3 n;
+	if (ray_cast_shape(s, xf, o, d, &f, &n) && f < maxFraction)
+	{
+		maxFraction = f;
+		h.body = bi;
+		h.fraction = f;
+		h.normal = n;
+	}
+}
+
+// first index with keys[i] >= key
+__device__ __forceinline__ int64_t lower_bound_key(const uint64_t* keys, int64_t n, uint64_t key)
+{
+	int64_t lo = 0, hi = n;
+	while (lo < hi)
+	{
+		int64_t mid = (lo + hi) >> 1;
+		if (keys[mid] < key)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ void visit_cell(const RayCtx& R, int64_t x, int64_t y, int64_t z, m3d_vec3 o, m3d_vec3 d,
+										   uint32_t mask, float& maxFraction, RayHit& h)
+{
+	float cs = R.cellSize;
+	m3d_vec3 probe = m3d_v3(((float)x + 0.5f) * cs, ((float)y + 0.5f) * cs, ((float)z + 0.5f) * cs);
+	float inv = 1.0f / cs;
+	uint64_t key = pack_cell((int64_t)floorf(probe.x * inv), (int64_t)floorf(probe.y * inv),
+							 (int64_t)floorf(probe.z * inv));
+	for (int64_t i = lower_bound_key(R.aKeys, R.aCount, key); i < R.aCount && R.aKeys[i] == key; ++i)
+		ray_test(R, (int32_t)R.aBodies[i], o, d, mask, maxFraction, h);
+	for (int64_t i = lower_bound_key(R.sKeys, R.sCount, key); i < R.sCount && R.sKeys[i] == key; ++i)
+		ray_test(R, (int32_t)R.sBodies[i], o, d, mask, maxFraction, h);
+}
+
+// rayO: origin xyz, mask bits (w); rayD: translation xyz. outA: point xyz,
+// fraction (2 when nothing is hit); outB: normal xyz, body index bits (-1).
+__global__ void k_ray_cast(RayCtx R, const float4* rayO, const float4* rayD, int32_t count, float4* outA, float4* outB)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	float4 O = rayO[k], D = rayD[k];
+	m3d_vec3 o = f4_v3(O), d = f4_v3(D);
+	uint32_t mask = f2u(O.w);
+	RayHit h;
+	h.body = -1;
+	h.fraction = 2.0f;
+	h.normal = m3d_v3_zero();
+	float maxFraction = 1.0f;
+	for (int32_t i = 0; i < R.nAOver; ++i)
+		ray_test(R, R.aOver[i], o, d, mask, maxFraction, h);
+	for (int32_t i = 0; i < R.nSOver; ++i)
+		ray_test(R, R.sOver[i], o, d, mask, maxFraction, h);
+	if (R.aCount > 0 || R.sCount > 0)
+	{
+		float cs = R.cellSize;
+		float inv = 1.0f / cs;
+		int64_t cell[3] = { (int64_t)floorf(o.x * inv), (int64_t)floorf(o.y * inv), (int64_t)floorf(o.z * inv) };
+		const float dv[3] = { d.x, d.y, d.z };
+		const float ov[3] = { o.x, o.y, o.z };
+		int64_t stepDir[3];
+		float tMax[3], tDelta[3];
+#pragma unroll
+		for (int a = 0; a < 3; ++a)
+		{
+			if (fabsf(dv[a]) < 1e-12f)
+			{
+				stepDir[a] = 0;
+				tMax[a] = FLT_MAX;
+				tDelta[a] = FLT_MAX;
+			}
+			else
+			{
+				stepDir[a] = dv[a] > 0.0f ? 1 : -1;
+				float boundary = (float)(cell[a] + (stepDir[a] > 0 ? 1 : 0)) * cs;
+				tMax[a] = (boundary - ov[a]) / dv[a];
+				tDelta[a] = cs / fabsf(dv[a]);
+			}
+		}
+		float t = 0.0f;
+		for (int guard = 0; guard < 4096 && t <= maxFraction; ++guard)
+		{
+			visit_cell(R, cell[0], cell[1], cell[2], o, d, mask, maxFraction, h);
+			int axis = 0;
+			if (tMax[1] < tMax[axis])
+				axis = 1;
+			if (tMax[2] < tMax[axis])
+				axis = 2;
+			t = tMax[axis];
+			if (t > 1.0f || stepDir[axis] == 0)
+				break;
+			cell[axis] += stepDir[axis];
+			tMax[axis] += tDelta[axis];
+		}
+	}
+	m3d_vec3 point = h.body >= 0 ? m3d_mul_add(o, h.fraction, d) : m3d_add(o, d);
+	outA[k] = v3_f4(point, h.fraction);
+	outB[k] = v3_f4(h.normal, i2f(h.body));
+}
+
+} // namespace mx
+
+using namespace mx;
+
+extern "C"
+{
+
+// Batched ray casts on the device: out[k] is m3d_world_ray_cast_filtered(
+// world, origins[k], translations[k], maskBits ? maskBits[k] : all) for
+// every k, evaluated as the reference does (src/world.cpp:1081-1200) over
+// the device grid. Plain worlds; a dirty broadphase is refreshed with
+// dt = 0 first, as the reference does. Returns 0 or -1.
+int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m3d_vec3* translations,
+							  const uint32_t* maskBits, int32_t count, m3d_ray_result* out)
+{
+	if (world == nullptr || count < 0 || (count > 0 && (origins == nullptr || translations == nullptr || out == nullptr)))
+	{
+		set_error("ray cast batch: null arrays");
+		return -1;
+	}
+	World& w = world->w;
+	if (w.subCount > 0)
+	{
+		set_error("ray cast batch: batched worlds are not supported");
+		return -1;
+	}
+	if (count == 0)
+		return 0;
+	if (w.gridDirty || w.stableDirty)
+	{
+		if (!push_host_changes(w) || !refresh_broadphase(w, 0.0f))
+			return -1;
+		w.hostStale = true;
+	}
+	hipStream_t s = w.stream;
+	std::vector<float4> o((size_t)count), d((size_t)count);
+	for (int32_t k = 0; k < count; ++k)
+	{
+		uint32_t m = maskBits != nullptr ? maskBits[k] : 0xFFFFFFFFu;
+		o[k] = make_float4(origins[k].x, origins[k].y, origins[k].z, u2f(m));
+		d[k] = make_float4(translations[k].x, translations[k].y, translations[k].z, 0.0f);
+	}
+	DBuf<float4> buf;
+	if (!buf.ensure(4 * (size_t)count))
+		return -1;
+	float4* dO = buf.p;
+	float4* dD = dO + count;
+	float4* dA = dD + count;
+	float4* dB = dA + count;
+	RayCtx R;
+	R.B = make_body_view(w);
+	R.aKeys = w.activeKeys.p;
+	R.aBodies = w.activeBodies.p;
+	R.aCount = w.activeCount;
+	R.sKeys = w.stableKeys.p;
+	R.sBodies = w.stableBodies.p;
+	R.sCount = w.stableCount;
+	R.aOver = w.dActiveOversized.p;
+	R.nAOver = (int32_t)w.activeOversized.size();
+	R.sOver = w.dStableOversized.p;
+	R.nSOver = (int32_t)w.stableOversized.size();
+	R.cellSize = w.def.gridCellSize;
+	std::vector<float4> a((size_t)count), b((size_t)count);
+	bool ok = hip_ok(hipMemcpyAsync(dO, o.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "rays") &&
+			  hip_ok(hipMemcpyAsync(dD, d.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "rays");
+	if (ok)
+	{
+		k_ray_cast<<<(count + 127) / 128, 128, 0, s>>>(R, dO, dD, count, dA, dB);
+		ok = hip_ok(hipGetLastError(), "ray cast") &&
+			 hip_ok(hipMemcpyAsync(a.data(), dA, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
+			 hip_ok(hipMemcpyAsync(b.data(), dB, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
+			 hip_ok(hipStreamSynchronize(s), "ray cast");
+	}
+	buf.release();
+	if (!ok)
+		return -1;
+	for (int32_t k = 0; k < count; ++k)
+	{
+		m3d_ray_result& r = out[k];
+		int32_t bi = f2i(b[k].w);
+		r.hit = bi >= 0;
+		r.body = bi >= 0 ? m3d_body_id{ bi, w.bodies[bi].generation } : m3d_body_id_null();
+		r.point = m3d_v3(a[k].x, a[k].y, a[k].z);
+		r.fraction = a[k].w;
+		r.normal = m3d_v3(b[k].x, b[k].y, b[k].z);
+	}
+	return 0;
+}
+
+} // extern "C"

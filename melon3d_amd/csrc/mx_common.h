@@ -102,6 +102,19 @@ MX_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
 
 MX_HD uint64_t contact_key(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
 
+// Grid cell of a coordinate and the packed cell key (src/grid.cpp:19-37):
+// floor(v / cell) with an offset of 2^20, 21 bits per axis, x above y above z.
+MX_HD int64_t cell_coord(float v, float inv) { return (int64_t)floorf(v * inv); }
+
+MX_HD uint64_t pack_cell(int64_t x, int64_t y, int64_t z)
+{
+	const int64_t off = (int64_t)1 << 20;
+	uint64_t ux = (uint64_t)(x + off) & 0x1FFFFFu;
+	uint64_t uy = (uint64_t)(y + off) & 0x1FFFFFu;
+	uint64_t uz = (uint64_t)(z + off) & 0x1FFFFFu;
+	return (ux << 42) | (uy << 21) | uz;
+}
+
 // ------------------------------------------------------------- shapes
 struct Shape
 {

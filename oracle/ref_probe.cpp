@@ -558,6 +558,10 @@ int m3dx_diff_jvp_cpu(m3d_world*, int, int, int, const double*, const double*, c
 /* no device kernels on the CPU engine: timing is unavailable */
 int m3dx_set_kernel_timing(m3d_world*, int) { return -1; }
 int m3dx_set_pass_blocks(m3d_world*, int) { return -1; }
+int m3dx_world_ray_cast_batch(m3d_world*, const m3d_vec3*, const m3d_vec3*, const uint32_t*, int32_t, m3d_ray_result*)
+{
+    return -1;
+}
 int m3dx_pass_blocks(m3d_world*) { return 0; }
 int32_t m3dx_step_failures(m3d_world*) { return 0; }
 int m3dx_set_step_graphs(m3d_world*, int) { return -1; }
