@@ -384,6 +384,13 @@ M3D_API int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal
    Returns 0 or -1. */
 M3D_API int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m3d_vec3* translations,
                                       const uint32_t* maskBits, int32_t count, m3d_ray_result* out);
+/* Batched sphere casts on the device: out[k] = m3d_world_sphere_cast(world,
+   origins[k], radii[k], translations[k], maskBits ? maskBits[k] :
+   0xFFFFFFFF) (include/melon3d.h:515-518, src/world.cpp:1286-1392), with
+   candidates from the device grid. Plain worlds only. Returns 0 or -1. */
+M3D_API int m3dx_world_sphere_cast_batch(m3d_world* world, const m3d_vec3* origins, const float* radii,
+                                         const m3d_vec3* translations, const uint32_t* maskBits, int32_t count,
+                                         m3d_ray_result* out);
 
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);

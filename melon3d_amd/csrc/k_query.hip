@@ -154,9 +154,245 @@ __global__ void k_ray_cast(RayCtx R, const float4* rayO, const float4* rayD, int
 	outB[k] = v3_f4(h.normal, i2f(h.body));
 }
 
+// One body's part of a sphere cast (src/world.cpp:1286-1392) evaluated on
+// its own: kind 1 the swept sphere touches the body at the start, 2 a hit at
+// *fraction, 0 none. The reference walks bodies in index order, returns on
+// the first start overlap and otherwise keeps a hit only when its fraction
+// is strictly below the best so far (its box loop also stops once past the
+// best, which can only drop hits that would not be kept), so its result is:
+// the lowest-index start overlap, else the smallest fraction with ties to
+// the lowest index. Evaluating each body alone and applying that rule gives
+// the same result in any candidate order.
+struct CastHit
+{
+	int kind;
+	float fraction;
+	m3d_vec3 point, normal;
+};
+
+MX_HD CastHit sphere_cast_body(const Shape& sh, m3d_transform xf, m3d_vec3 origin, float radius, m3d_vec3 tr, float len)
+{
+	CastHit h;
+	h.kind = 0;
+	h.fraction = 2.0f;
+	h.point = m3d_v3_zero();
+	h.normal = m3d_v3_zero();
+	{
+		m3d_vec3 closest;
+		float raw = point_shape_closest(sh, xf, origin, &closest);
+		if (raw - radius < 1e-4f)
+		{
+			h.kind = 1;
+			h.fraction = 0.0f;
+			h.point = closest;
+			m3d_vec3 outward = raw >= 0.0f ? m3d_sub(origin, closest) : m3d_sub(closest, origin);
+			h.normal = m3d_normalize(outward);
+			return h;
+		}
+	}
+	if (sh.type == M3D_SHAPE_SPHERE || sh.type == M3D_SHAPE_CAPSULE)
+	{
+		Shape inflated = sh;
+		inflated.radius += radius;
+		float f;
+		m3d_vec3 n;
+		if (ray_cast_shape(inflated, xf, origin, tr, &f, &n))
+		{
+			m3d_vec3 center = m3d_mul_add(origin, f, tr);
+			h.kind = 2;
+			h.fraction = f;
+			h.normal = n;
+			h.point = m3d_mul_add(center, -radius, n);
+		}
+		return h;
+	}
+	float t = 0.0f;
+	for (int it = 0; it < 128; ++it)
+	{
+		m3d_vec3 p = m3d_mul_add(origin, t, tr);
+		m3d_vec3 closest;
+		float raw = point_shape_closest(sh, xf, p, &closest);
+		float d = raw - radius;
+		if (d < 1e-4f)
+		{
+			h.kind = 2;
+			h.fraction = t;
+			h.point = closest;
+			m3d_vec3 outward = raw >= 0.0f ? m3d_sub(p, closest) : m3d_sub(closest, p);
+			h.normal = m3d_normalize(outward);
+			return h;
+		}
+		if (len < 1e-9f)
+			break;
+		t += d / len;
+		if (t > 1.0f)
+			break;
+	}
+	return h;
+}
+
+struct CastBest
+{
+	int32_t startBody, hitBody;
+	CastHit start, hit;
+};
+
+__device__ __forceinline__ void cast_test(const RayCtx& R, int32_t bi, m3d_vec3 o, float radius, m3d_vec3 d, float len,
+										  uint32_t mask, const m3d_aabb& sweep, CastBest& B)
+{
+	uint32_t fl = R.B.flags[bi];
+	if (!(fl & BF_IN_USE) || (R.B.filter[bi].x & mask) == 0u)
+		return;
+	if (B.startBody >= 0 && bi >= B.startBody)
+		return; // a lower-index body already touches at the start
+	Shape sh = load_shape(R.B, bi);
+	m3d_transform xf = load_xf(R.B, bi);
+	if (!m3d_aabb_overlap(sweep, shape_aabb(sh, xf)))
+		return;
+	CastHit h = sphere_cast_body(sh, xf, o, radius, d, len);
+	if (h.kind == 1)
+	{
+		B.startBody = bi;
+		B.start = h;
+	}
+	else if (h.kind == 2 && (B.hitBody < 0 || h.fraction < B.hit.fraction ||
+							 (h.fraction == B.hit.fraction && bi < B.hitBody)))
+	{
+		B.hitBody = bi;
+		B.hit = h;
+	}
+}
+
+// Sphere casts: candidates are the coarse lists and the bodies of the grid
+// cells the swept sphere's box covers (a body whose tight box meets the
+// sweep has an entry in one of them); sweeps over more than kCastCells cells
+// test every body instead. castO: origin xyz, mask bits; castD: translation
+// xyz, radius. Outputs as k_ray_cast.
+constexpr int64_t kCastCells = 512;
+
+__global__ void k_sphere_cast(RayCtx R, int32_t nBodies, const float4* castO, const float4* castD, int32_t count,
+							  float4* outA, float4* outB)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	float4 O = castO[k], D = castD[k];
+	m3d_vec3 o = f4_v3(O), d = f4_v3(D);
+	uint32_t mask = f2u(O.w);
+	float radius = D.w > 0.0f ? D.w : 0.0f;
+	m3d_vec3 end = m3d_add(o, d);
+	m3d_vec3 pad = m3d_v3(radius + 1e-3f, radius + 1e-3f, radius + 1e-3f);
+	m3d_aabb sweep;
+	sweep.lowerBound = m3d_sub(m3d_min(o, end), pad);
+	sweep.upperBound = m3d_add(m3d_max(o, end), pad);
+	float len = m3d_length(d);
+	CastBest B;
+	B.startBody = -1;
+	B.hitBody = -1;
+	float inv = 1.0f / R.cellSize;
+	int64_t x0 = cell_coord(sweep.lowerBound.x, inv), x1 = cell_coord(sweep.upperBound.x, inv);
+	int64_t y0 = cell_coord(sweep.lowerBound.y, inv), y1 = cell_coord(sweep.upperBound.y, inv);
+	int64_t z0 = cell_coord(sweep.lowerBound.z, inv), z1 = cell_coord(sweep.upperBound.z, inv);
+	int64_t cells = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+	if (cells > kCastCells || cells <= 0)
+	{
+		for (int32_t bi = 0; bi < nBodies; ++bi)
+			cast_test(R, bi, o, radius, d, len, mask, sweep, B);
+	}
+	else
+	{
+		for (int32_t i = 0; i < R.nAOver; ++i)
+			cast_test(R, R.aOver[i], o, radius, d, len, mask, sweep, B);
+		for (int32_t i = 0; i < R.nSOver; ++i)
+			cast_test(R, R.sOver[i], o, radius, d, len, mask, sweep, B);
+		for (int64_t x = x0; x <= x1; ++x)
+			for (int64_t y = y0; y <= y1; ++y)
+				for (int64_t z = z0; z <= z1; ++z)
+				{
+					uint64_t key = pack_cell(x, y, z);
+					for (int64_t i = lower_bound_key(R.aKeys, R.aCount, key); i < R.aCount && R.aKeys[i] == key; ++i)
+						cast_test(R, (int32_t)R.aBodies[i], o, radius, d, len, mask, sweep, B);
+					for (int64_t i = lower_bound_key(R.sKeys, R.sCount, key); i < R.sCount && R.sKeys[i] == key; ++i)
+						cast_test(R, (int32_t)R.sBodies[i], o, radius, d, len, mask, sweep, B);
+				}
+	}
+	int32_t body = B.startBody >= 0 ? B.startBody : B.hitBody;
+	const CastHit* h = B.startBody >= 0 ? &B.start : &B.hit;
+	if (body >= 0)
+	{
+		outA[k] = v3_f4(h->point, h->fraction);
+		outB[k] = v3_f4(h->normal, i2f(body));
+	}
+	else
+	{
+		outA[k] = v3_f4(end, 2.0f);
+		outB[k] = v3_f4(m3d_v3_zero(), i2f(-1));
+	}
+}
+
 } // namespace mx
 
 using namespace mx;
+
+// host side of the batched queries: arrays up, kernel, results down
+template <class Launch>
+static int run_query(World& w, const std::vector<float4>& o, const std::vector<float4>& d, m3d_ray_result* out,
+					 Launch&& launch)
+{
+	int32_t count = (int32_t)o.size();
+	if (w.gridDirty || w.stableDirty)
+	{
+		if (!push_host_changes(w) || !refresh_broadphase(w, 0.0f))
+			return -1;
+		w.hostStale = true;
+	}
+	hipStream_t s = w.stream;
+	DBuf<float4> buf;
+	if (!buf.ensure(4 * (size_t)count))
+		return -1;
+	float4* dO = buf.p;
+	float4* dD = dO + count;
+	float4* dA = dD + count;
+	float4* dB = dA + count;
+	RayCtx R;
+	R.B = make_body_view(w);
+	R.aKeys = w.activeKeys.p;
+	R.aBodies = w.activeBodies.p;
+	R.aCount = w.activeCount;
+	R.sKeys = w.stableKeys.p;
+	R.sBodies = w.stableBodies.p;
+	R.sCount = w.stableCount;
+	R.aOver = w.dActiveOversized.p;
+	R.nAOver = (int32_t)w.activeOversized.size();
+	R.sOver = w.dStableOversized.p;
+	R.nSOver = (int32_t)w.stableOversized.size();
+	R.cellSize = w.def.gridCellSize;
+	std::vector<float4> a((size_t)count), b((size_t)count);
+	bool ok = hip_ok(hipMemcpyAsync(dO, o.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries") &&
+			  hip_ok(hipMemcpyAsync(dD, d.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries");
+	if (ok)
+	{
+		launch(R, dO, dD, count, dA, dB, s);
+		ok = hip_ok(hipGetLastError(), "query") &&
+			 hip_ok(hipMemcpyAsync(a.data(), dA, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
+			 hip_ok(hipMemcpyAsync(b.data(), dB, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
+			 hip_ok(hipStreamSynchronize(s), "query");
+	}
+	buf.release();
+	if (!ok)
+		return -1;
+	for (int32_t k = 0; k < count; ++k)
+	{
+		m3d_ray_result& r = out[k];
+		int32_t bi = f2i(b[k].w);
+		r.hit = bi >= 0;
+		r.body = bi >= 0 ? m3d_body_id{ bi, w.bodies[bi].generation } : m3d_body_id_null();
+		r.point = m3d_v3(a[k].x, a[k].y, a[k].z);
+		r.fraction = a[k].w;
+		r.normal = m3d_v3(b[k].x, b[k].y, b[k].z);
+	}
+	return 0;
+}
 
 extern "C"
 {
@@ -182,13 +418,6 @@ int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m
 	}
 	if (count == 0)
 		return 0;
-	if (w.gridDirty || w.stableDirty)
-	{
-		if (!push_host_changes(w) || !refresh_broadphase(w, 0.0f))
-			return -1;
-		w.hostStale = true;
-	}
-	hipStream_t s = w.stream;
 	std::vector<float4> o((size_t)count), d((size_t)count);
 	for (int32_t k = 0; k < count; ++k)
 	{
@@ -196,51 +425,46 @@ int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m
 		o[k] = make_float4(origins[k].x, origins[k].y, origins[k].z, u2f(m));
 		d[k] = make_float4(translations[k].x, translations[k].y, translations[k].z, 0.0f);
 	}
-	DBuf<float4> buf;
-	if (!buf.ensure(4 * (size_t)count))
-		return -1;
-	float4* dO = buf.p;
-	float4* dD = dO + count;
-	float4* dA = dD + count;
-	float4* dB = dA + count;
-	RayCtx R;
-	R.B = make_body_view(w);
-	R.aKeys = w.activeKeys.p;
-	R.aBodies = w.activeBodies.p;
-	R.aCount = w.activeCount;
-	R.sKeys = w.stableKeys.p;
-	R.sBodies = w.stableBodies.p;
-	R.sCount = w.stableCount;
-	R.aOver = w.dActiveOversized.p;
-	R.nAOver = (int32_t)w.activeOversized.size();
-	R.sOver = w.dStableOversized.p;
-	R.nSOver = (int32_t)w.stableOversized.size();
-	R.cellSize = w.def.gridCellSize;
-	std::vector<float4> a((size_t)count), b((size_t)count);
-	bool ok = hip_ok(hipMemcpyAsync(dO, o.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "rays") &&
-			  hip_ok(hipMemcpyAsync(dD, d.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "rays");
-	if (ok)
+	return run_query(w, o, d, out,
+					 [](const RayCtx& R, const float4* dO, const float4* dD, int32_t n, float4* dA, float4* dB,
+						hipStream_t s) { k_ray_cast<<<(n + 127) / 128, 128, 0, s>>>(R, dO, dD, n, dA, dB); });
+}
+
+// Batched sphere casts on the device: out[k] = m3d_world_sphere_cast(world,
+// origins[k], radii[k], translations[k], maskBits ? maskBits[k] : all),
+// the reference's result (src/world.cpp:1286-1392) with candidates taken
+// from the device grid. Plain worlds. Returns 0 or -1.
+int m3dx_world_sphere_cast_batch(m3d_world* world, const m3d_vec3* origins, const float* radii,
+								 const m3d_vec3* translations, const uint32_t* maskBits, int32_t count,
+								 m3d_ray_result* out)
+{
+	if (world == nullptr || count < 0 ||
+		(count > 0 && (origins == nullptr || radii == nullptr || translations == nullptr || out == nullptr)))
 	{
-		k_ray_cast<<<(count + 127) / 128, 128, 0, s>>>(R, dO, dD, count, dA, dB);
-		ok = hip_ok(hipGetLastError(), "ray cast") &&
-			 hip_ok(hipMemcpyAsync(a.data(), dA, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
-			 hip_ok(hipMemcpyAsync(b.data(), dB, count * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") &&
-			 hip_ok(hipStreamSynchronize(s), "ray cast");
-	}
-	buf.release();
-	if (!ok)
+		set_error("sphere cast batch: null arrays");
 		return -1;
+	}
+	World& w = world->w;
+	if (w.subCount > 0)
+	{
+		set_error("sphere cast batch: batched worlds are not supported");
+		return -1;
+	}
+	if (count == 0)
+		return 0;
+	std::vector<float4> o((size_t)count), d((size_t)count);
 	for (int32_t k = 0; k < count; ++k)
 	{
-		m3d_ray_result& r = out[k];
-		int32_t bi = f2i(b[k].w);
-		r.hit = bi >= 0;
-		r.body = bi >= 0 ? m3d_body_id{ bi, w.bodies[bi].generation } : m3d_body_id_null();
-		r.point = m3d_v3(a[k].x, a[k].y, a[k].z);
-		r.fraction = a[k].w;
-		r.normal = m3d_v3(b[k].x, b[k].y, b[k].z);
+		uint32_t m = maskBits != nullptr ? maskBits[k] : 0xFFFFFFFFu;
+		o[k] = make_float4(origins[k].x, origins[k].y, origins[k].z, u2f(m));
+		d[k] = make_float4(translations[k].x, translations[k].y, translations[k].z, radii[k]);
 	}
-	return 0;
+	int32_t nBodies = w.deviceBodyCount;
+	return run_query(w, o, d, out,
+					 [nBodies](const RayCtx& R, const float4* dO, const float4* dD, int32_t n, float4* dA, float4* dB,
+							   hipStream_t s) {
+						 k_sphere_cast<<<(n + 127) / 128, 128, 0, s>>>(R, nBodies, dO, dD, n, dA, dB);
+					 });
 }
 
 } // extern "C"

@@ -965,6 +965,66 @@ MX_HD bool ray_cast_shape(const Shape& s, m3d_transform xf, m3d_vec3 origin, m3d
 }
 
 // Tight shape AABB; src/grid.cpp:145-174
+// signed distance and closest surface point; src/world.cpp:1218-1284
+MX_HD float point_shape_closest(const Shape& s, m3d_transform xf, m3d_vec3 p, m3d_vec3* closest)
+{
+	if (s.type == M3D_SHAPE_SPHERE)
+	{
+		m3d_vec3 d = m3d_sub(p, xf.p);
+		float len = m3d_length(d);
+		m3d_vec3 dir = len > 1e-9f ? m3d_scale(1.0f / len, d) : m3d_v3(0.0f, 1.0f, 0.0f);
+		*closest = m3d_mul_add(xf.p, s.radius, dir);
+		return len - s.radius;
+	}
+	if (s.type == M3D_SHAPE_CAPSULE)
+	{
+		m3d_vec3 axis = m3d_rotate(xf.q, m3d_v3(0.0f, 1.0f, 0.0f));
+		m3d_vec3 pa = m3d_mul_add(xf.p, -s.halfHeight, axis);
+		m3d_vec3 ab = m3d_scale(2.0f * s.halfHeight, axis);
+		float den = m3d_dot(ab, ab);
+		float t = den > 1e-12f ? m3d_dot(m3d_sub(p, pa), ab) / den : 0.0f;
+		t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+		m3d_vec3 c = m3d_mul_add(pa, t, ab);
+		m3d_vec3 d = m3d_sub(p, c);
+		float len = m3d_length(d);
+		m3d_vec3 dir = len > 1e-9f ? m3d_scale(1.0f / len, d) : m3d_v3(0.0f, 1.0f, 0.0f);
+		*closest = m3d_mul_add(c, s.radius, dir);
+		return len - s.radius;
+	}
+	m3d_vec3 lp = m3d_inv_transform_point(xf, p);
+	m3d_vec3 he = s.halfExtents;
+	m3d_vec3 cl = clamp_to_box(lp, he);
+	m3d_vec3 diff = m3d_sub(lp, cl);
+	float d2 = m3d_length_sq(diff);
+	if (d2 > 0.0f)
+	{
+		*closest = m3d_transform_point(xf, cl);
+		return sqrtf(d2);
+	}
+	float dx = he.x - fabsf(lp.x);
+	float dy = he.y - fabsf(lp.y);
+	float dz = he.z - fabsf(lp.z);
+	m3d_vec3 face = lp;
+	float depth;
+	if (dx <= dy && dx <= dz)
+	{
+		face.x = lp.x >= 0.0f ? he.x : -he.x;
+		depth = dx;
+	}
+	else if (dy <= dz)
+	{
+		face.y = lp.y >= 0.0f ? he.y : -he.y;
+		depth = dy;
+	}
+	else
+	{
+		face.z = lp.z >= 0.0f ? he.z : -he.z;
+		depth = dz;
+	}
+	*closest = m3d_transform_point(xf, face);
+	return -depth;
+}
+
 MX_HD m3d_aabb shape_aabb(const Shape& s, m3d_transform xf)
 {
 	m3d_aabb r;

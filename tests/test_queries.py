@@ -77,6 +77,52 @@ def test_ray_cast_batch_matches_reference(ref, dev, at):
         dev.m3d_world_destroy(dw)
 
 
+def _sphere_batch(dev, w, o, r, d, masks=None):
+    n = len(o)
+    O = (m3d_vec3 * n)(*[m3d_vec3(*map(float, p)) for p in o])
+    D = (m3d_vec3 * n)(*[m3d_vec3(*map(float, p)) for p in d])
+    R = (C.c_float * n)(*map(float, r))
+    M = (C.c_uint32 * n)(*masks) if masks is not None else None
+    out = (m3d_ray_result * n)()
+    assert dev.m3dx_world_sphere_cast_batch(w, O, R, D, M, n, out) == 0, dev.last_error()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("at", [1, 60])
+def test_sphere_cast_batch_matches_reference(ref, dev, at):
+    """Sphere casts (short casts with grid candidates, and long ones that
+    test every body) equal the reference's m3d_world_sphere_cast bit for
+    bit, start overlaps included."""
+    rw = ref.create_world()
+    scenes.rain_pile(ref, rw, count=1500, column=(12.0, 12.0, 20.0))
+    for _ in range(at):
+        ref.m3d_world_step(rw, DT, 4)
+    dw = dev.import_state(ref.export_state(rw))
+    try:
+        ref.m3d_world_step(rw, DT, 4)
+        dev.m3d_world_step(dw, DT, 4)
+        rng = np.random.default_rng(100 + at)
+        n = 1500
+        o, d = _rays(rng, n)
+        d[: n // 2] *= 0.1  # short casts: grid candidates
+        r = rng.uniform(0.0, 0.8, n).astype(np.float32)
+        masks = [0xFFFFFFFF if k % 3 else int(rng.integers(1, 1 << 16)) for k in range(n)]
+        for mk in (None, masks):
+            out = _sphere_batch(dev, dw, o, r, d, mk)
+            hits = 0
+            for k in range(n):
+                q = ref.m3d_world_sphere_cast(rw, m3d_vec3(*map(float, o[k])), float(r[k]),
+                                              m3d_vec3(*map(float, d[k])), 0xFFFFFFFF if mk is None else mk[k])
+                assert _same(out[k], q), (k, out[k].hit, q.hit, out[k].body.index, q.body.index, out[k].fraction,
+                                          q.fraction)
+                hits += bool(q.hit)
+            assert 100 < hits < n, hits
+    finally:
+        ref.m3d_world_destroy(rw)
+        dev.m3d_world_destroy(dw)
+
+
 @pytest.mark.gpu
 def test_ray_cast_batch_refuses_batches(dev):
     bw, _ = scenes.batch(dev, 2)
