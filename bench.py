@@ -65,7 +65,8 @@ def measured_traffic(scene: str, size: int, kernel: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on this
     workload, from the committed profile summary; None when not measured."""
-    files = {("stream", 200000): "stream200k_grid_normals.md", ("batch", 4096): "batch4096_kernels.md"}
+    files = {("stream", 200000): "stream200k_grid_normals.md", ("batch", 4096): "batch4096_kernels.md",
+             ("diff", 256): "diff256_adjoint.md"}
     if (scene, size) not in files:
         return None
     path = os.path.join(ROOT, "profiles", files[(scene, size)])
@@ -321,7 +322,8 @@ def run_diff(args, dev):
         "grad_norms": {"v0": float(np.linalg.norm(gv)), "friction": float(np.linalg.norm(gf)),
                        "mass": float(np.linalg.norm(gm))},
         "roofline": {"bound": "hbm", "kernel": "adjoint", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launches": launches,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("diff", args.worlds, "adjoint"),
+                     "launches": launches,
                      "avg_launch_us": avg_s * 1e6, "bytes_per_launch": (kbytes / launches) if launches else 0.0},
         "kernel_ms_backward": {n: v[1] for n, v in kt.items() if v[0] > 0},
         "cpu_baseline": None,

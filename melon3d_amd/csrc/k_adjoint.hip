@@ -1152,8 +1152,13 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, double* 
 		}
 }
 
+// One workgroup per world and step. Reverse != 0: the reverse sweep of the
+// step (forward replay with records, then the VJPs); Reverse == 0: the
+// forward recomputation of a checkpoint segment's states. Two kernels so that
+// a kernel trace tells the two apart.
+template <int Reverse>
 __global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L, float* slab, double* dslab,
-														  const float* stateIn, float* stateOut, int reverse,
+														  const float* stateIn, float* stateOut,
 														  const double* seedIn, double* seedOut, double* massGrad,
 														  double* fricGrad)
 {
@@ -1167,7 +1172,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_step(RecView V, AdjLayout L
 	if (wi >= V.worlds)
 		return;
 	AdjLocal Lc = { res, sOrder, sLevel, sStart, sLast, &sNL };
-	adj_world(V, L, slab, dslab, stateIn, stateOut, reverse, seedIn, seedOut, massGrad, fricGrad, wi, Lc);
+	adj_world(V, L, slab, dslab, stateIn, stateOut, Reverse, seedIn, seedOut, massGrad, fricGrad, wi, Lc);
 }
 
 } // namespace mx
@@ -1203,8 +1208,12 @@ static bool adj_launch(World& w, DiffState* D, const DiffStepRec* R, int iterati
 		return false;
 	RecView V = rec_view(D, R, iterations);
 	w.mark_begin(reverse ? M3DX_K_ADJOINT : M3DX_K_ADJ_FORWARD, adj_bytes(R, reverse));
-	k_adj_step<<<D->worlds, kAdjThreads, 0, s>>>(V, L, D->adjSlab.p, D->adjDSlab.p, stateIn, stateOut, reverse,
-												  seedIn, seedOut, D->massG.p, D->fricG.p);
+	if (reverse)
+		k_adj_step<1><<<D->worlds, kAdjThreads, 0, s>>>(V, L, D->adjSlab.p, D->adjDSlab.p, stateIn, stateOut, seedIn,
+														 seedOut, D->massG.p, D->fricG.p);
+	else
+		k_adj_step<0><<<D->worlds, kAdjThreads, 0, s>>>(V, L, D->adjSlab.p, D->adjDSlab.p, stateIn, stateOut, seedIn,
+														 seedOut, D->massG.p, D->fricG.p);
 	w.mark_end();
 	return hip_ok(hipGetLastError(), "adjoint launch");
 }

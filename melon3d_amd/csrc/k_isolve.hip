@@ -73,10 +73,16 @@ __device__ __forceinline__ void store_vel(const DBodies& B, int32_t i, const WB&
 
 // Solve record of one scheduled contact: contact, body A, body B, and its
 // form: bits 0-7 the packet's maxPoints (kRecSerial: serial form), bit 8 set
-// for box-box pairs (the only pairs with stiction friction). Building it
-// once per substep leaves one dependent load in front of each pass.
+// for box-box pairs (the only pairs with stiction friction); the grid-wide
+// solve's records also hold the contact's point count in bits 12-14 and set
+// bit 15 for curved pairs. Building it once per substep leaves one dependent
+// load in front of each pass.
 constexpr int32_t kRecSerial = 0xFF;
 constexpr int32_t kRecBoxBox = 0x100;
+constexpr int kRecCountShift = 12;
+constexpr int32_t kRecCurved = 0x8000;
+
+__device__ __forceinline__ int rec_points(int4 r) { return (r.w >> kRecCountShift) & 7; }
 
 __device__ __forceinline__ int4 make_rec(const PassCtx& X, int32_t c, int32_t maxPoints)
 {
@@ -156,6 +162,8 @@ struct VelocityPass
 		load_wb(X.B, r.z, b);
 		m3d_vec3 pvA = f4_v3(X.B.prevVel[r.y]), pwA = f4_v3(X.B.prevAngVel[r.y]);
 		m3d_vec3 pvB = f4_v3(X.B.prevVel[r.z]), pwB = f4_v3(X.B.prevAngVel[r.z]);
+		// issued with the other body loads (one round trip), although only
+		// curved pairs use them
 		VelShape sa, sbv;
 		sa.type = bf_shape(X.B.flags[r.y]);
 		sa.radius = X.B.shape[r.y].w;
@@ -811,18 +819,24 @@ __device__ __forceinline__ void sweep_run(const GiantSolveCtx& S, int32_t e, int
 	Pass::run(S.X.P, r, d);
 }
 
+// Only the contact's own points are loaded (the copy holds nothing beyond
+// them, and the passes read nothing beyond them); records of one (slot, point
+// count) bucket are adjacent, so whole waves skip the same columns.
 template <>
 __device__ __forceinline__ void sweep_load<NormalsPass>(const GiantSolveCtx& S, int32_t e, int4 r, NormalsIn& d)
 {
 	const int64_t st = S.nrmStride;
+	const int n = rec_points(r);
+	const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	d.nc = S.nrm[e];
 #pragma unroll
 	for (int j = 0; j < 4; ++j)
 	{
-		d.A4[j] = S.nrm[(1 + j) * st + e];
-		d.B4[j] = S.nrm[(5 + j) * st + e];
-		d.JA[j] = S.nrm[(9 + j) * st + e];
-		d.JB[j] = S.nrm[(13 + j) * st + e];
+		bool on = j < n;
+		d.A4[j] = on ? S.nrm[(1 + j) * st + e] : zero;
+		d.B4[j] = on ? S.nrm[(5 + j) * st + e] : zero;
+		d.JA[j] = on ? S.nrm[(9 + j) * st + e] : zero;
+		d.JB[j] = on ? S.nrm[(13 + j) * st + e] : zero;
 	}
 }
 
@@ -832,6 +846,8 @@ __device__ __forceinline__ void sweep_load<FrictionPass>(const GiantSolveCtx& S,
 	if (!(r.w & kRecBoxBox))
 		return;
 	const int64_t st = S.nrmStride;
+	const int n = rec_points(r);
+	const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	d.nc = S.nrm[e];
 	d.mu = S.nrm[17 * st + e].x;
 	d.RA0 = S.nrm[22 * st + e];
@@ -839,20 +855,24 @@ __device__ __forceinline__ void sweep_load<FrictionPass>(const GiantSolveCtx& S,
 #pragma unroll
 	for (int j = 0; j < 4; ++j)
 	{
-		d.LN[j] = S.nrm[(9 + j) * st + e].w;
-		d.FA[j] = S.nrm[(30 + j) * st + e];
-		d.FB[j] = S.nrm[(34 + j) * st + e];
-		d.JB[j] = S.nrm[(13 + j) * st + e];
+		bool on = j < n;
+		d.LN[j] = on ? S.nrm[(9 + j) * st + e].w : 0.0f;
+		d.FA[j] = on ? S.nrm[(30 + j) * st + e] : zero;
+		d.FB[j] = on ? S.nrm[(34 + j) * st + e] : zero;
+		d.JB[j] = on ? S.nrm[(13 + j) * st + e] : zero;
 	}
 }
 
+// All four points' columns, unguarded: a point-count guard here made the
+// pass slower (it outweighed the skipped columns). The curved bit comes from
+// the record, so no load waits on another load of this contact.
 template <>
 __device__ __forceinline__ void sweep_load<VelocityPass>(const GiantSolveCtx& S, int32_t e, int4 r, VelocityIn& d)
 {
 	const int64_t st = S.nrmStride;
 	d.nc = S.nrm[e];
 	d.mat = S.nrm[17 * st + e];
-	bool curved = (f2u(d.mat.w) & CF_CURVED) != 0;
+	bool curved = (r.w & kRecCurved) != 0;
 	const int64_t ra = curved ? 1 : 22, rb = curved ? 5 : 26;
 #pragma unroll
 	for (int j = 0; j < 4; ++j)
@@ -1071,12 +1091,14 @@ __global__ void k_giant_records(GiantSolveCtx S)
 		}
 		int32_t c = (int32_t)S.entries[e];
 		uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
-		S.rec[at] = make_rec(P, c, mp);
 		float4 nc = P.C.normal[c], mat = P.C.mat[c];
 		points = (uint32_t)f2i(nc.w);
+		int4 rc = make_rec(P, c, mp);
+		rc.w |= ((int32_t)(points < 4 ? points : 4) << kRecCountShift) | ((f2u(mat.w) & CF_CURVED) ? kRecCurved : 0);
+		S.rec[at] = rc;
 		const int64_t st = S.nrmStride;
 		const bool flat = (f2u(mat.w) & CF_CURVED) == 0;
-		const bool boxBox = (S.rec[at].w & kRecBoxBox) != 0;
+		const bool boxBox = (rc.w & kRecBoxBox) != 0;
 		S.nrm[at] = nc;
 		S.nrm[17 * st + at] = mat;
 		for (int j = 0; j < 4 && j < (int)points; ++j)
