@@ -787,8 +787,11 @@ struct GiantSolveCtx
 	int4* rec;                 // solve records, grouped within each slot
 	const uint64_t* keys;      // sorted (slot, island) keys
 	int bitsI;
-	uint32_t* bucketCount;     // per (slot, group); zero between substeps
+	uint32_t* bucketCount;     // per (slot, group); zero between steps
 	uint32_t* bucketCursor;    // per (slot, group)
+	uint32_t* pos;             // record position of each sorted entry (per step)
+	uint32_t* cnt;             // point count of each sorted entry (per substep)
+	uint32_t* entryAt;         // sorted entry of each record position (per step)
 	// contact-side inputs of the three passes for each scheduled contact at
 	// its record position (block b of record k at b * nrmStride + k):
 	// 0 normal, 1-4 anchors A (+ separation offset), 5-8 anchors B (+ wN),
@@ -1044,17 +1047,18 @@ __device__ uint32_t wave_atomic_inc(uint32_t* ctr, uint32_t key)
 	return result;
 }
 
-__global__ void k_giant_prepare(GiantSolveCtx S)
+// Groups of the step's record positions, from the point counts at the step's
+// start (a manifold regenerated later in the step keeps its group: any order
+// within a slot is valid, the groups only keep waves on one path).
+__global__ void k_giant_bucket_count(GiantSolveCtx S)
 {
 	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
 		return;
-	int32_t c = (int32_t)S.entries[e];
-	prepare_contact(S.X.P.B, S.X.P.C, c, S.X.P.h);
-	(void)wave_atomic_inc(S.bucketCount, giant_bucket(S.X.P, S, e, c));
+	(void)wave_atomic_inc(S.bucketCount, giant_bucket(S.X.P, S, e, (int32_t)S.entries[e]));
 }
 
-// Start of every (slot, group) run; clears the counts for the next substep.
+// Start of every (slot, group) run; clears the counts for the next step.
 __global__ void k_giant_bucket_starts(GiantSolveCtx S)
 {
 	int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1070,60 +1074,103 @@ __global__ void k_giant_bucket_starts(GiantSolveCtx S)
 	}
 }
 
-__global__ void k_giant_records(GiantSolveCtx S)
+// Record position of every entry for the whole step, and the record's fixed
+// part: contact, bodies, box-box and curved bits.
+__global__ void k_giant_positions(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
 	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	int32_t c = (int32_t)S.entries[e];
+	uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
+	S.pos[e] = at;
+	S.entryAt[at] = (uint32_t)e;
+	int4 rc = make_rec(P, c, 0);
+	rc.w = (rc.w & kRecBoxBox) | ((f2u(P.C.mat[c].w) & CF_CURVED) ? kRecCurved : 0);
+	S.rec[at] = rc;
+}
+
+// Schedule-ordered copy of a contact's point rows at its record position,
+// taken from prepare_contact's registers as it stores them; the rows it does
+// not compute (slip, friction anchors) are loaded.
+struct CopySink
+{
+	float4* nrm;
+	const DContacts* C;
+	int64_t st;
+	uint32_t at;
+	bool flat, boxBox;
+	__device__ void point(int j, int32_t p, const float4& a4, const float4& b4, const float4& ja, const float4& jb,
+						  const float4& ra, const float4& rb) const
+	{
+		nrm[(1 + j) * st + at] = a4;
+		nrm[(5 + j) * st + at] = b4;
+		nrm[(9 + j) * st + at] = ja;
+		nrm[(13 + j) * st + at] = jb;
+		nrm[(18 + j) * st + at] = C->pSlip[p];
+		if (flat || (boxBox && j == 0))
+		{
+			nrm[(22 + j) * st + at] = ra;
+			nrm[(26 + j) * st + at] = rb;
+		}
+		if (boxBox)
+		{
+			nrm[(30 + j) * st + at] = C->pFA[p];
+			nrm[(34 + j) * st + at] = C->pFB[p];
+		}
+	}
+};
+
+// Every substep: the contact's constraint rows for the substep, written to
+// the contact and to the schedule-ordered copy the three passes read. Lanes
+// take record positions in order: the copy is written coalesced, and the
+// lanes of a wave share point count and curvature.
+__global__ void k_giant_prepare(GiantSolveCtx S)
+{
+	const PassCtx& P = S.X.P;
+	int32_t at = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	uint32_t points = 0;
-	bool valid = e < (int32_t)S.slotStart[S.counters[CT_SLOTS]];
+	bool valid = at < (int32_t)S.slotStart[S.counters[CT_SLOTS]];
 	if (valid)
 	{
-		int2 pk = S.packet[e];
-		int32_t mp = -1;
-		if (pk.x >= 0)
-		{
-			mp = 0;
-			for (int32_t q = pk.x; q < pk.y; ++q)
-			{
-				int32_t pc = f2i(P.C.normal[S.entries[q]].w);
-				mp = pc > mp ? pc : mp;
-			}
-		}
-		int32_t c = (int32_t)S.entries[e];
-		uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
-		float4 nc = P.C.normal[c], mat = P.C.mat[c];
+		int4 r = S.rec[at];
+		int32_t c = r.x;
+		CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0 };
+		prepare_contact(P.B, P.C, c, P.h, sink);
+		float4 nc = P.C.normal[c];
 		points = (uint32_t)f2i(nc.w);
-		int4 rc = make_rec(P, c, mp);
-		rc.w |= ((int32_t)(points < 4 ? points : 4) << kRecCountShift) | ((f2u(mat.w) & CF_CURVED) ? kRecCurved : 0);
-		S.rec[at] = rc;
-		const int64_t st = S.nrmStride;
-		const bool flat = (f2u(mat.w) & CF_CURVED) == 0;
-		const bool boxBox = (rc.w & kRecBoxBox) != 0;
+		S.cnt[S.entryAt[at]] = points;
 		S.nrm[at] = nc;
-		S.nrm[17 * st + at] = mat;
-		for (int j = 0; j < 4 && j < (int)points; ++j)
-		{
-			int32_t p = pidx(P.C, c, j);
-			S.nrm[(1 + j) * st + at] = P.C.pA[p];
-			S.nrm[(5 + j) * st + at] = P.C.pB[p];
-			S.nrm[(9 + j) * st + at] = P.C.pJA[p];
-			S.nrm[(13 + j) * st + at] = P.C.pJB[p];
-			S.nrm[(18 + j) * st + at] = P.C.pSlip[p];
-			if (flat || (boxBox && j == 0))
-			{
-				S.nrm[(22 + j) * st + at] = P.C.pRA[p];
-				S.nrm[(26 + j) * st + at] = P.C.pRB[p];
-			}
-			if (boxBox)
-			{
-				S.nrm[(30 + j) * st + at] = P.C.pFA[p];
-				S.nrm[(34 + j) * st + at] = P.C.pFB[p];
-			}
-		}
+		S.nrm[17 * S.nrmStride + at] = P.C.mat[c];
 	}
 	// entries and points of the substep, for the byte model of the passes
 	wave_add(&S.counters2[CT_GIANT_ENTRIES], valid ? 1u : 0u);
 	wave_add(&S.counters2[CT_GIANT_POINTS], points);
+}
+
+// Every substep: the record's form (the packet's largest point count, or the
+// serial form) and the contact's point count.
+__global__ void k_giant_rec_forms(GiantSolveCtx S)
+{
+	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	int2 pk = S.packet[e];
+	int32_t mp = kRecSerial;
+	if (pk.x >= 0)
+	{
+		mp = 0;
+		for (int32_t q = pk.x; q < pk.y; ++q)
+		{
+			int32_t pc = (int32_t)S.cnt[q];
+			mp = pc > mp ? pc : mp;
+		}
+	}
+	uint32_t at = S.pos[e];
+	int32_t points = (int32_t)S.cnt[e];
+	int32_t w = S.rec[at].w & (kRecBoxBox | kRecCurved);
+	S.rec[at].w = w | mp | ((points < 4 ? points : 4) << kRecCountShift);
 }
 
 // lambdaN of the normal pass back from the schedule-ordered copy to the
@@ -1337,7 +1384,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		++bitsS;
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
-		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) ||
+		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPosG.ensure(3 * (size_t)NC) ||
 		!w.sNrm.ensure(kSchedBlocks * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
@@ -1394,7 +1441,13 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.bitsI = L.bitsI;
 	S.bucketCount = w.sBucket.p;
 	S.bucketCursor = w.sBucket.p + (size_t)(maxS + 1) * kBuckets;
+	S.pos = w.sPosG.p;
+	S.cnt = w.sPosG.p + NC;
+	S.entryAt = w.sPosG.p + 2 * (size_t)NC;
 	launch_fill_u32(S.bucketCount, 0u, (int64_t)(maxS + 1) * kBuckets, s);
+	k_giant_bucket_count<<<gridk(NC), 256, 0, s>>>(S);
+	k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
+	k_giant_positions<<<gridk(NC), 256, 0, s>>>(S);
 	// more entries per slot pay for the dearer barrier of a wider grid: about
 	// one workgroup per 1536 entries, 64 to 256 (measured on a 190 k-entry
 	// stream step and a 4 M-entry batch step)
@@ -1435,8 +1488,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
 		if (!diff_record_substep(w, sub))
 			return false;
-		k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
-		k_giant_records<<<gridk(NC), 256, 0, s>>>(S);
+		k_giant_rec_forms<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_NORMALS);
 		bool ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
 		w.mark_end();

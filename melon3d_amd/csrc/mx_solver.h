@@ -361,7 +361,18 @@ MX_HD bool manifold_stale(m3d_vec3 pa, m3d_quat qa, m3d_vec3 pb, m3d_quat qb, m3
 MX_HD float min3(float a, float b, float c) { return fminf(a, fminf(b, c)); }
 
 // Per-substep contact preparation; src/xpbd.cpp:733-863.
-MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, float h)
+// Receives each point's prepared rows (anchor A, anchor B with wN, angular
+// Jacobians A and B, arms A and B) as they are stored; the default drops them.
+struct NoPrepSink
+{
+	MX_HD void point(int, int32_t, const float4&, const float4&, const float4&, const float4&, const float4&,
+					 const float4&) const
+	{
+	}
+};
+
+template <class Sink = NoPrepSink>
+MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, float h, const Sink& sink = Sink())
 {
 	int2 ab = C.bodies[c];
 	int32_t ia = ab.x, ib = ab.y;
@@ -477,12 +488,17 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 		m3d_vec3 jA = m3d_mat3_mulv(&iwA, crA);
 		m3d_vec3 jB = m3d_mat3_mulv(&iwB, crB);
 		float wN = imA + imB + m3d_dot(crA, jA) + m3d_dot(crB, jB);
-		C.pB[p] = make_float4(b4.x, b4.y, b4.z, wN);
-		C.pJA[p] = v3_f4(jA, 0.0f); // lambdaN = 0
-		C.pJB[p] = v3_f4(jB, 0.0f); // lambdaT = 0
+		const float4 bw = make_float4(b4.x, b4.y, b4.z, wN);
+		const float4 ja = v3_f4(jA, 0.0f); // lambdaN = 0
+		const float4 jb = v3_f4(jB, 0.0f); // lambdaT = 0
+		C.pB[p] = bw;
+		C.pJA[p] = ja;
+		C.pJB[p] = jb;
 		float4 ra = C.pRA[p], rb = C.pRB[p];
-		C.pRA[p] = v3_f4(rA, ra.w);
-		C.pRB[p] = v3_f4(rB, rb.w);
+		const float4 ra2 = v3_f4(rA, ra.w), rb2 = v3_f4(rB, rb.w);
+		C.pRA[p] = ra2;
+		C.pRB[p] = rb2;
+		sink.point(j, p, a4, bw, ja, jb, ra2, rb2);
 	}
 }
 

@@ -238,7 +238,8 @@ struct World
 	DBuf<int32_t> sColor, sLevel, sJointList;
 	DBuf<uint32_t> sCtr;          // convergence counters of the giant-island fixpoints
 	DBuf<int2> sPacket;
-	DBuf<int4> sRecG;             // grid path: solve records by sorted entry
+	DBuf<int4> sRecG;             // grid path: solve records by schedule position
+	DBuf<uint32_t> sPosG;         // grid path: record position and point count by sorted entry, entry by position
 	DBuf<float4> sNrm;            // grid path: normal-pass inputs by schedule position (17 x entries)
 	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
 	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry

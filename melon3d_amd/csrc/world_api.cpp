@@ -121,6 +121,7 @@ World::~World()
 	sCtr.release();
 	sPacket.release();
 	sRecG.release();
+	sPosG.release();
 	sNrm.release();
 	sBucket.release();
 	sPred.release();
