@@ -18,6 +18,7 @@
 //               receives the k-th largest surviving slot above it
 //               (src/world.cpp:1705-1735, :100-119)
 #include "mx_steps.h"
+#include "mx_wave.h"
 
 #include <algorithm>
 #include <string.h>
@@ -176,7 +177,7 @@ __global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uin
 			B.fatLo[i] = v3_f4(fat.lowerBound, 0.0f);
 			B.fatHi[i] = v3_f4(fat.upperBound, 0.0f);
 			bp |= BP_HAS_FAT | BP_MOVED;
-			atomicAdd(&counters[CT_REFAT], 1u);
+			wave_inc(&counters[CT_REFAT]);
 		}
 	}
 	else if (stable && rebuildStable && !(bp & BP_HAS_FAT))
@@ -1770,7 +1771,7 @@ __global__ void k_count_removed(const uint32_t* rm, const int2* bodies, const in
 	if (c < C && rm[c])
 	{
 		atomicAdd(&removed[contact_sub(bodySub, bodies, c)], 1u);
-		atomicAdd(&counters[CT_REMOVED], 1u);
+		wave_inc(&counters[CT_REMOVED]);
 	}
 }
 

@@ -1047,15 +1047,76 @@ __device__ uint32_t wave_atomic_inc(uint32_t* ctr, uint32_t key)
 	return result;
 }
 
-// Groups of the step's record positions, from the point counts at the step's
-// start (a manifold regenerated later in the step keeps its group: any order
-// within a slot is valid, the groups only keep waves on one path).
-__global__ void k_giant_bucket_count(GiantSolveCtx S)
+// Record positions of the step: within each slot, entries grouped by point
+// count and curvature, from the counts at the step's start (a manifold
+// regenerated later in the step keeps its group: any order within a slot is
+// valid, the groups only keep waves on one path). Place == false counts the
+// groups (bucketCount); Place == true takes positions from the group cursors
+// (bucketCursor, set to the group starts in between) and writes the record's
+// fixed part: contact, bodies, box-box and curved bits.
+//
+// Entries are sorted by slot, so a workgroup's entries fall in few slots: the
+// lanes of the first kGroupWin slots rank themselves in an LDS table and the
+// workgroup adds one global atomic per group it holds (atomics on one address
+// from every wave of a large grid would serialize); lanes beyond the window
+// fall back to one atomic per wave and group.
+constexpr int kGroupThreads = 1024;
+constexpr int kGroupWin = 4;
+
+template <bool Place>
+__global__ __launch_bounds__(kGroupThreads) void k_giant_group(GiantSolveCtx S)
 {
-	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+	__shared__ uint32_t tab[kGroupWin * kBuckets];
+	__shared__ uint32_t base[kGroupWin * kBuckets];
+	__shared__ uint32_t slot0;
+	const PassCtx& P = S.X.P;
+	const int32_t nValid = (int32_t)S.slotStart[S.counters[CT_SLOTS]];
+	const int32_t first = (int32_t)(blockIdx.x * blockDim.x);
+	const int32_t e = first + (int32_t)threadIdx.x;
+	uint32_t* ctr = Place ? S.bucketCursor : S.bucketCount;
+	if (threadIdx.x < (uint32_t)(kGroupWin * kBuckets))
+		tab[threadIdx.x] = 0u;
+	if (threadIdx.x == 0)
+		slot0 = first < nValid ? (uint32_t)(S.keys[first] >> S.bitsI) : 0u;
+	__syncthreads();
+	const bool valid = e < nValid;
+	int32_t c = 0;
+	uint32_t key = 0, local = 0;
+	int32_t cell = -1;
+	if (valid)
+	{
+		c = (int32_t)S.entries[e];
+		key = giant_bucket(P, S, e, c);
+		uint32_t off = key / kBuckets - slot0;
+		if (off < (uint32_t)kGroupWin)
+		{
+			cell = (int32_t)(off * kBuckets + key % kBuckets);
+			local = atomicAdd(&tab[cell], 1u);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < (uint32_t)(kGroupWin * kBuckets))
+	{
+		uint32_t n = tab[threadIdx.x];
+		uint32_t b = 0u;
+		if (n != 0u)
+			b = atomicAdd(&ctr[(slot0 + threadIdx.x / kBuckets) * kBuckets + threadIdx.x % kBuckets], n);
+		base[threadIdx.x] = b;
+	}
+	__syncthreads();
+	if (!valid)
 		return;
-	(void)wave_atomic_inc(S.bucketCount, giant_bucket(S.X.P, S, e, (int32_t)S.entries[e]));
+	uint32_t at = cell >= 0 ? base[cell] + local : 0u;
+	if (cell < 0)
+		at = wave_atomic_inc(ctr, key);
+	if (Place)
+	{
+		S.pos[e] = at;
+		S.entryAt[at] = (uint32_t)e;
+		int4 rc = make_rec(P, c, 0);
+		rc.w = (rc.w & kRecBoxBox) | ((f2u(P.C.mat[c].w) & CF_CURVED) ? kRecCurved : 0);
+		S.rec[at] = rc;
+	}
 }
 
 // Start of every (slot, group) run; clears the counts for the next step.
@@ -1072,23 +1133,6 @@ __global__ void k_giant_bucket_starts(GiantSolveCtx S)
 		S.bucketCursor[s * kBuckets + b] = at;
 		at += n;
 	}
-}
-
-// Record position of every entry for the whole step, and the record's fixed
-// part: contact, bodies, box-box and curved bits.
-__global__ void k_giant_positions(GiantSolveCtx S)
-{
-	const PassCtx& P = S.X.P;
-	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
-		return;
-	int32_t c = (int32_t)S.entries[e];
-	uint32_t at = wave_atomic_inc(S.bucketCursor, giant_bucket(P, S, e, c));
-	S.pos[e] = at;
-	S.entryAt[at] = (uint32_t)e;
-	int4 rc = make_rec(P, c, 0);
-	rc.w = (rc.w & kRecBoxBox) | ((f2u(P.C.mat[c].w) & CF_CURVED) ? kRecCurved : 0);
-	S.rec[at] = rc;
 }
 
 // Schedule-ordered copy of a contact's point rows at its record position,
@@ -1145,8 +1189,7 @@ __global__ void k_giant_prepare(GiantSolveCtx S)
 		S.nrm[17 * S.nrmStride + at] = P.C.mat[c];
 	}
 	// entries and points of the substep, for the byte model of the passes
-	wave_add(&S.counters2[CT_GIANT_ENTRIES], valid ? 1u : 0u);
-	wave_add(&S.counters2[CT_GIANT_POINTS], points);
+	block_add<2>(S.counters2, { CT_GIANT_ENTRIES, CT_GIANT_POINTS }, { valid ? 1u : 0u, points });
 }
 
 // Every substep: the record's form (the packet's largest point count, or the
@@ -1445,9 +1488,10 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.cnt = w.sPosG.p + NC;
 	S.entryAt = w.sPosG.p + 2 * (size_t)NC;
 	launch_fill_u32(S.bucketCount, 0u, (int64_t)(maxS + 1) * kBuckets, s);
-	k_giant_bucket_count<<<gridk(NC), 256, 0, s>>>(S);
+	const unsigned groupGrid = (unsigned)((NC + kGroupThreads - 1) / kGroupThreads);
+	k_giant_group<false><<<groupGrid, kGroupThreads, 0, s>>>(S);
 	k_giant_bucket_starts<<<gridk(maxS), 256, 0, s>>>(S);
-	k_giant_positions<<<gridk(NC), 256, 0, s>>>(S);
+	k_giant_group<true><<<groupGrid, kGroupThreads, 0, s>>>(S);
 	// more entries per slot pay for the dearer barrier of a wider grid: about
 	// one workgroup per 1536 entries, 64 to 256 (measured on a 190 k-entry
 	// stream step and a 4 M-entry batch step)

@@ -8,6 +8,7 @@
 // of wakeAt[x] = min(initial, t : transition t touches x and its partner was
 // awake before t), iterated until no value changes.
 #include "mx_steps.h"
+#include "mx_wave.h"
 
 namespace mx
 {
@@ -57,7 +58,7 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 	}
 	if (regen)
 	{
-		atomicAdd(&counters[CT_REGEN], 1u);
+		wave_inc(&counters[CT_REGEN]);
 		Manifold old;
 		load_manifold(C, c, old);
 		Manifold m = old;

@@ -171,11 +171,9 @@ __global__ void k_contact_island(const int2* bodies, const float4* mat, const fl
 			}
 		}
 	}
-	// counts aggregated per wave (all lanes reach here)
-	wave_add(&counters[CT_TOUCHING], (cf & CF_TOUCHING) ? 1u : 0u);
-	wave_add(&counters[CT_SOLVED_CONTACTS], f);
-	wave_add(&counters[CT_SOLVED_POINTS], pc);
-	wave_add(&counters[CT_BB_POINTS], (cf & CF_CURVED) ? 0u : pc);
+	// counts aggregated per workgroup (all lanes reach here)
+	block_add<4>(counters, { CT_TOUCHING, CT_SOLVED_CONTACTS, CT_SOLVED_POINTS, CT_BB_POINTS },
+				 { (cf & CF_TOUCHING) ? 1u : 0u, f, pc, (cf & CF_CURVED) ? 0u : pc });
 	wave_add_keyed(islandContacts, f ? (uint32_t)isl : ~0u, 1u);
 	if (!valid)
 		return;
@@ -301,8 +299,8 @@ __global__ void k_finish(DBodies B, int32_t n, float dt, int clearForces, uint32
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	uint8_t f = i < n ? B.sb[i] : (uint8_t)0;
-	wave_add(&counters[CT_BODY_COUNT], (f & SB_IN_USE) ? 1u : 0u);
-	wave_add(&counters[CT_AWAKE_BODIES], ((f & SB_IN_USE) && (f & SB_AWAKE)) ? 1u : 0u);
+	block_add<2>(counters, { CT_BODY_COUNT, CT_AWAKE_BODIES },
+				 { (f & SB_IN_USE) ? 1u : 0u, ((f & SB_IN_USE) && (f & SB_AWAKE)) ? 1u : 0u });
 	if (i >= n)
 		return;
 	if ((f & SB_KINEMATIC) && (f & SB_AWAKE))
@@ -337,7 +335,7 @@ __global__ void k_island_modes(const uint32_t* islandAwake, const uint32_t* isla
 		if ((int32_t)islandContacts[i] >= kBigIslandContacts)
 		{
 			m |= IM_COLORED;
-			atomicAdd(&counters[CT_COLORED_ISLANDS], 1u);
+			wave_inc(&counters[CT_COLORED_ISLANDS]);
 		}
 	}
 	mode[i] = m;

@@ -35,6 +35,48 @@ __device__ __forceinline__ void wave_add(uint32_t* p, uint32_t v)
 		atomicAdd(p, s);
 }
 
+// *p += number of calling lanes. Unlike the helpers above it may be called
+// from divergent code: only the wave's active lanes take part.
+__device__ __forceinline__ void wave_inc(uint32_t* p)
+{
+	uint64_t m = __ballot(1);
+	if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+		atomicAdd(p, (uint32_t)__popcll(m));
+}
+
+// ctr[idx[k]] += sum of v[k] over the workgroup, for k < N: one atomic per
+// counter and workgroup, for kernels in which every wave adds to the same few
+// counters. Every lane of the workgroup calls it (it holds a barrier); at most
+// 16 waves per workgroup.
+template <int N>
+__device__ __forceinline__ void block_add(uint32_t* ctr, const int (&idx)[N], const uint32_t (&v)[N])
+{
+	__shared__ uint32_t part[N][16];
+	const uint32_t wv = threadIdx.x >> 6;
+#pragma unroll
+	for (int k = 0; k < N; ++k)
+	{
+		uint32_t s = wave_sum(v[k]);
+		if (__lane_id() == 0)
+			part[k][wv] = s;
+	}
+	__syncthreads();
+	if (threadIdx.x < (uint32_t)N)
+	{
+		const uint32_t nw = (blockDim.x + 63) >> 6;
+		uint32_t t = 0;
+		for (uint32_t w = 0; w < nw; ++w)
+			t += part[threadIdx.x][w];
+		int k = (int)threadIdx.x;
+		int at = idx[0];
+#pragma unroll
+		for (int j = 1; j < N; ++j)
+			at = k == j ? idx[j] : at;
+		if (t != 0u)
+			atomicAdd(&ctr[at], t);
+	}
+}
+
 // ctr[key] += v for every lane whose key is not ~0u.
 __device__ __forceinline__ void wave_add_keyed(uint32_t* ctr, uint32_t key, uint32_t v)
 {
