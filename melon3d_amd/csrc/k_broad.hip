@@ -320,25 +320,35 @@ __global__ void k_compact_indices(const uint32_t* flags, const uint32_t* offs, i
 		out[offs[i]] = (int32_t)i;
 }
 
-__device__ __forceinline__ void block_minmax(uint32_t v, bool valid, uint32_t* gmin, uint32_t* gmax)
+// *gmin = min(*gmin, mn) and *gmax = max(*gmax, mx) over the workgroup's
+// lanes with valid set: wave shuffles, then one pair of global atomics per
+// workgroup (callers loop over their elements on reduce_grid workgroups).
+// Every lane of the workgroup calls it.
+__device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool valid, uint32_t* gmin, uint32_t* gmax)
 {
-	__shared__ uint32_t smin, smax;
+	__shared__ uint32_t smin[16], smax[16];
+	mn = wave_min(valid ? mn : 0xFFFFFFFFu);
+	mx = wave_max(valid ? mx : 0u);
+	const uint32_t wv = threadIdx.x >> 6;
+	if (__lane_id() == 0)
+	{
+		smin[wv] = mn;
+		smax[wv] = mx;
+	}
+	__syncthreads();
 	if (threadIdx.x == 0)
 	{
-		smin = 0xFFFFFFFFu;
-		smax = 0u;
-	}
-	__syncthreads();
-	if (valid)
-	{
-		atomicMin(&smin, v);
-		atomicMax(&smax, v);
-	}
-	__syncthreads();
-	if (threadIdx.x == 0 && smin <= smax)
-	{
-		atomicMin(gmin, smin);
-		atomicMax(gmax, smax);
+		const uint32_t nw = (blockDim.x + 63) >> 6;
+		for (uint32_t k = 1; k < nw; ++k)
+		{
+			mn = smin[k] < mn ? smin[k] : mn;
+			mx = smax[k] > mx ? smax[k] : mx;
+		}
+		if (mn <= mx)
+		{
+			atomicMin(gmin, mn);
+			atomicMax(gmax, mx);
+		}
 	}
 	__syncthreads();
 }
@@ -346,12 +356,13 @@ __device__ __forceinline__ void block_minmax(uint32_t v, bool valid, uint32_t* g
 __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated, float cellSize, const uint32_t* cnt,
 							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, uint32_t* counters)
 {
-	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
 	bool any = false;
 	const int batched = B.sub != nullptr;
-	if (i < n && cnt[i] > 0 && tier_included(B, i, includeBit, gated))
+	for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x))
 	{
+		if (!(cnt[i] > 0 && tier_included(B, i, includeBit, gated)))
+			continue;
 		float inv = 1.0f / cellSize;
 		float4 lo = B.fatLo[i], hi = B.fatHi[i];
 		int64_t x0 = cell_coord(lo.x, inv), y0 = cell_coord(lo.y, inv), z0 = cell_coord(lo.z, inv);
@@ -376,12 +387,9 @@ __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated,
 					any = true;
 				}
 	}
-	block_minmax(mnx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
-	block_minmax(mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
-	block_minmax(mny, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
-	block_minmax(mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
-	block_minmax(mnz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
-	block_minmax(mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+	block_minmax(mnx, mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
+	block_minmax(mny, mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
+	block_minmax(mnz, mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
 }
 
 // Sort keys: the occupied box of cell fields (and the sub-world, most
@@ -422,12 +430,22 @@ __global__ void k_key_expand(uint64_t* keys, int64_t n, KeyCompress kc)
 // cell field bounds of existing stored keys (kept stable entries)
 __global__ void k_key_bounds(const uint64_t* keys, int64_t n, int batched, uint32_t* counters)
 {
-	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	bool any = i < n;
-	KeyFields f = key_fields(any ? keys[i] : 0ull, batched);
-	block_minmax(f.x, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
-	block_minmax(f.y, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
-	block_minmax(f.z, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
+	bool any = false;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+	{
+		KeyFields f = key_fields(keys[i], batched);
+		mnx = min(mnx, f.x);
+		mxx = max(mxx, f.x);
+		mny = min(mny, f.y);
+		mxy = max(mxy, f.y);
+		mnz = min(mnz, f.z);
+		mxz = max(mxz, f.z);
+		any = true;
+	}
+	block_minmax(mnx, mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
+	block_minmax(mny, mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
+	block_minmax(mnz, mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
 }
 
 __global__ void k_mark_cells(DBodies B, int32_t n, uint8_t includeBit, int gated, uint8_t cellBit, const uint32_t* cnt)
@@ -531,8 +549,8 @@ bool build_tier(World& w, int tier)
 		return false;
 	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
 	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
-	k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p, vals.p,
-											   w.counters.p);
+	k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p,
+														   vals.p, w.counters.p);
 	if (!sync_counters(w))
 		return false;
 	int bits = 0;
@@ -1704,10 +1722,10 @@ bool build_stable_batched(World& w)
 	{
 		k_compact_entries<<<grid_for(oldE), kBlock, 0, s>>>(w.stableKeys.p, w.stableBodies.p, keep, keepOff, oldE,
 															  w.stableKeysNext.p, w.stableBodiesNext.p);
-		k_key_bounds<<<grid_for(eKeep), kBlock, 0, s>>>(w.stableKeysNext.p, eKeep, 1, w.counters.p);
+		k_key_bounds<<<reduce_grid(eKeep, kBlock), kBlock, 0, s>>>(w.stableKeysNext.p, eKeep, 1, w.counters.p);
 	}
 	if (eNew > 0)
-		k_tier_emit<<<grid_for(n), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, offs,
+		k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, offs,
 												   w.stableKeysNext.p + eKeep, w.stableBodiesNext.p + eKeep,
 												   w.counters.p);
 	if (!sync_counters(w))

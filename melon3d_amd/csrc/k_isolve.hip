@@ -1174,46 +1174,47 @@ __global__ void k_giant_prepare(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
 	int32_t at = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	uint32_t points = 0;
-	bool valid = at < (int32_t)S.slotStart[S.counters[CT_SLOTS]];
-	if (valid)
-	{
-		int4 r = S.rec[at];
-		int32_t c = r.x;
-		CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0 };
-		prepare_contact(P.B, P.C, c, P.h, sink);
-		float4 nc = P.C.normal[c];
-		points = (uint32_t)f2i(nc.w);
-		S.cnt[S.entryAt[at]] = points;
-		S.nrm[at] = nc;
-		S.nrm[17 * S.nrmStride + at] = P.C.mat[c];
-	}
-	// entries and points of the substep, for the byte model of the passes
-	block_add<2>(S.counters2, { CT_GIANT_ENTRIES, CT_GIANT_POINTS }, { valid ? 1u : 0u, points });
+	if (at >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+		return;
+	int4 r = S.rec[at];
+	int32_t c = r.x;
+	CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0 };
+	prepare_contact(P.B, P.C, c, P.h, sink);
+	float4 nc = P.C.normal[c];
+	S.cnt[S.entryAt[at]] = (uint32_t)f2i(nc.w);
+	S.nrm[at] = nc;
+	S.nrm[17 * S.nrmStride + at] = P.C.mat[c];
 }
 
 // Every substep: the record's form (the packet's largest point count, or the
 // serial form) and the contact's point count.
+// Also counts the substep's entries and points, for the byte model of the
+// passes. Runs on reduce_grid workgroups.
 __global__ void k_giant_rec_forms(GiantSolveCtx S)
 {
-	int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
-		return;
-	int2 pk = S.packet[e];
-	int32_t mp = kRecSerial;
-	if (pk.x >= 0)
+	const int32_t nValid = (int32_t)S.slotStart[S.counters[CT_SLOTS]];
+	uint32_t entries = 0, pointSum = 0;
+	for (int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); e < nValid; e += (int32_t)(gridDim.x * blockDim.x))
 	{
-		mp = 0;
-		for (int32_t q = pk.x; q < pk.y; ++q)
+		int2 pk = S.packet[e];
+		int32_t mp = kRecSerial;
+		if (pk.x >= 0)
 		{
-			int32_t pc = (int32_t)S.cnt[q];
-			mp = pc > mp ? pc : mp;
+			mp = 0;
+			for (int32_t q = pk.x; q < pk.y; ++q)
+			{
+				int32_t pc = (int32_t)S.cnt[q];
+				mp = pc > mp ? pc : mp;
+			}
 		}
+		uint32_t at = S.pos[e];
+		int32_t points = (int32_t)S.cnt[e];
+		int32_t w = S.rec[at].w & (kRecBoxBox | kRecCurved);
+		S.rec[at].w = w | mp | ((points < 4 ? points : 4) << kRecCountShift);
+		entries += 1u;
+		pointSum += (uint32_t)points;
 	}
-	uint32_t at = S.pos[e];
-	int32_t points = (int32_t)S.cnt[e];
-	int32_t w = S.rec[at].w & (kRecBoxBox | kRecCurved);
-	S.rec[at].w = w | mp | ((points < 4 ? points : 4) << kRecCountShift);
+	block_add<2>(S.counters2, { CT_GIANT_ENTRIES, CT_GIANT_POINTS }, { entries, pointSum });
 }
 
 // lambdaN of the normal pass back from the schedule-ordered copy to the
@@ -1532,7 +1533,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
 		if (!diff_record_substep(w, sub))
 			return false;
-		k_giant_rec_forms<<<gridk(NC), 256, 0, s>>>(S);
+		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_NORMALS);
 		bool ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
 		w.mark_end();

@@ -147,39 +147,49 @@ __global__ void k_island_wake(DBodies B, int32_t n, const int32_t* islandOf, con
 	mark_sub_stable_dirty(B, i);
 }
 
-// solver contacts: touching, non-sensor, dynamic body's island awake
+// solver contacts: touching, non-sensor, dynamic body's island awake. Runs
+// on reduce_grid workgroups; every lane takes the same number of turns (the
+// wave-wide helpers need the whole wave).
 __global__ void k_contact_island(const int2* bodies, const float4* mat, const float4* normal, const uint8_t* sb,
 								 const int32_t* islandOf, const uint32_t* islandAwake, int32_t C, uint64_t* keys,
 								 uint32_t* vals, uint32_t* flag, uint32_t* islandContacts, uint32_t* counters)
 {
-	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	bool valid = c < C;
-	uint32_t cf = valid ? f2u(mat[c].w) : 0u;
-	uint32_t f = 0, pc = 0;
-	int32_t isl = -1;
-	if (cf & CF_TOUCHING)
+	uint32_t nTouch = 0, nSolved = 0, nPoints = 0, nFlatPoints = 0;
+	for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < C; base += (int64_t)gridDim.x * blockDim.x)
 	{
-		if (!(cf & CF_SENSOR))
+		int32_t c = (int32_t)(base + threadIdx.x);
+		bool valid = c < C;
+		uint32_t cf = valid ? f2u(mat[c].w) : 0u;
+		uint32_t f = 0, pc = 0;
+		int32_t isl = -1;
+		if (cf & CF_TOUCHING)
 		{
-			int2 ab = bodies[c];
-			int32_t dyn = (sb[ab.x] & SB_DYNAMIC) ? ab.x : ab.y;
-			isl = islandOf[dyn];
-			if (isl >= 0 && islandAwake[isl])
+			if (!(cf & CF_SENSOR))
 			{
-				f = 1;
-				pc = (uint32_t)f2i(normal[c].w);
+				int2 ab = bodies[c];
+				int32_t dyn = (sb[ab.x] & SB_DYNAMIC) ? ab.x : ab.y;
+				isl = islandOf[dyn];
+				if (isl >= 0 && islandAwake[isl])
+				{
+					f = 1;
+					pc = (uint32_t)f2i(normal[c].w);
+				}
 			}
 		}
+		nTouch += (cf & CF_TOUCHING) ? 1u : 0u;
+		nSolved += f;
+		nPoints += pc;
+		nFlatPoints += (cf & CF_CURVED) ? 0u : pc;
+		wave_add_keyed(islandContacts, f ? (uint32_t)isl : ~0u, 1u);
+		if (valid)
+		{
+			flag[c] = f;
+			keys[c] = f ? (uint64_t)isl : 0ull;
+			vals[c] = (uint32_t)c;
+		}
 	}
-	// counts aggregated per workgroup (all lanes reach here)
 	block_add<4>(counters, { CT_TOUCHING, CT_SOLVED_CONTACTS, CT_SOLVED_POINTS, CT_BB_POINTS },
-				 { (cf & CF_TOUCHING) ? 1u : 0u, f, pc, (cf & CF_CURVED) ? 0u : pc });
-	wave_add_keyed(islandContacts, f ? (uint32_t)isl : ~0u, 1u);
-	if (!valid)
-		return;
-	flag[c] = f;
-	keys[c] = f ? (uint64_t)isl : 0ull;
-	vals[c] = (uint32_t)c;
+				 { nTouch, nSolved, nPoints, nFlatPoints });
 }
 
 __global__ void k_body_island_keys(const int32_t* islandOf, const uint32_t* islandAwake, int32_t n, uint64_t* keys,
@@ -295,27 +305,30 @@ __global__ void k_sleep_apply(DBodies B, const int32_t* islandOf, const uint32_t
 
 // ---------------------------------------------------------- end of step
 
+// Runs on reduce_grid workgroups.
 __global__ void k_finish(DBodies B, int32_t n, float dt, int clearForces, uint32_t* counters)
 {
-	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	uint8_t f = i < n ? B.sb[i] : (uint8_t)0;
-	block_add<2>(counters, { CT_BODY_COUNT, CT_AWAKE_BODIES },
-				 { (f & SB_IN_USE) ? 1u : 0u, ((f & SB_IN_USE) && (f & SB_AWAKE)) ? 1u : 0u });
-	if (i >= n)
-		return;
-	if ((f & SB_KINEMATIC) && (f & SB_AWAKE))
+	uint32_t nUse = 0, nAwake = 0;
+	for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x))
 	{
-		float4 P = B.pos[i];
-		m3d_vec3 p = m3d_mul_add(f4_v3(P), dt, f4_v3(B.vel[i]));
-		m3d_quat q = m3d_quat_integrate(f4_q(B.rot[i]), f4_v3(B.angVel[i]), dt);
-		B.pos[i] = v3_f4(p, P.w);
-		B.rot[i] = q_f4(q);
+		uint8_t f = B.sb[i];
+		nUse += (f & SB_IN_USE) ? 1u : 0u;
+		nAwake += ((f & SB_IN_USE) && (f & SB_AWAKE)) ? 1u : 0u;
+		if ((f & SB_KINEMATIC) && (f & SB_AWAKE))
+		{
+			float4 P = B.pos[i];
+			m3d_vec3 p = m3d_mul_add(f4_v3(P), dt, f4_v3(B.vel[i]));
+			m3d_quat q = m3d_quat_integrate(f4_q(B.rot[i]), f4_v3(B.angVel[i]), dt);
+			B.pos[i] = v3_f4(p, P.w);
+			B.rot[i] = q_f4(q);
+		}
+		if (clearForces && (f & SB_DYNAMIC))
+		{
+			B.force[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			B.torque[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
 	}
-	if (clearForces && (f & SB_DYNAMIC))
-	{
-		B.force[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		B.torque[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	}
+	block_add<2>(counters, { CT_BODY_COUNT, CT_AWAKE_BODIES }, { nUse, nAwake });
 }
 
 // ---------------------------------------------------------- island modes
@@ -415,7 +428,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	launch_fill_i32(w.contactColor.p, -1, C, s);
 	if (C > 0)
 	{
-		k_contact_island<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.cc.normal.p, w.bc.sb.p,
+		k_contact_island<<<reduce_grid(C, kBlockS), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.cc.normal.p, w.bc.sb.p,
 													   w.islandOf.p, islandAwake, C, w.u64a.p, (uint32_t*)w.i32d.p,
 													   flag, islandContacts, w.counters.p);
 		scan_exclusive(flag, offs, C, w.counters.p + CT_SEG_COUNT, w.scanStatus.p, s);
@@ -481,7 +494,7 @@ void finish_step(World& w, float dt)
 {
 	int32_t n = w.deviceBodyCount;
 	if (n > 0)
-		k_finish<<<grids(n), kBlockS, 0, w.stream>>>(make_body_view(w), n, dt, w.anyForcesApplied ? 1 : 0,
+		k_finish<<<reduce_grid(n, kBlockS), kBlockS, 0, w.stream>>>(make_body_view(w), n, dt, w.anyForcesApplied ? 1 : 0,
 													 w.counters.p);
 }
 

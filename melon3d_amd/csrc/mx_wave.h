@@ -27,6 +27,27 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 	return v;
 }
 
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+	for (int off = 32; off > 0; off >>= 1)
+	{
+		uint32_t o = __shfl_xor(v, off);
+		v = o > v ? o : v;
+	}
+	return v;
+}
+
+// Workgroups of a kernel that ends in per-workgroup global atomics on a few
+// counters: such atomics from every workgroup of a large grid serialize, so
+// the kernel loops over its elements on at most this many workgroups.
+constexpr unsigned kReduceGroups = 1024;
+
+static inline unsigned reduce_grid(int64_t n, int block)
+{
+	int64_t g = (n + block - 1) / block;
+	return (unsigned)(g < 1 ? 1 : (g > (int64_t)kReduceGroups ? (int64_t)kReduceGroups : g));
+}
+
 // *p += sum of v over the wave.
 __device__ __forceinline__ void wave_add(uint32_t* p, uint32_t v)
 {
