@@ -1782,22 +1782,27 @@ __device__ __forceinline__ int32_t contact_sub(const int32_t* bodySub, const int
 	return bodySub[bodies[c].x];
 }
 
+// Removed contacts per sub-world (a sub-world's contacts are one range, so a
+// wave's lanes share one or two keys) and in all; runs on reduce_grid
+// workgroups, every lane taking the same number of turns.
 __global__ void k_count_removed(const uint32_t* rm, const int2* bodies, const int32_t* bodySub, int32_t C,
 								uint32_t* removed, uint32_t* counters)
 {
-	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (c < C && rm[c])
+	uint32_t n = 0;
+	for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < C; base += (int64_t)gridDim.x * blockDim.x)
 	{
-		atomicAdd(&removed[contact_sub(bodySub, bodies, c)], 1u);
-		wave_inc(&counters[CT_REMOVED]);
+		int32_t c = (int32_t)(base + threadIdx.x);
+		bool r = c < C && rm[c] != 0u;
+		wave_add_keyed(removed, r ? (uint32_t)contact_sub(bodySub, bodies, c) : ~0u, 1u);
+		n += r ? 1u : 0u;
 	}
+	block_add<1>(counters, { CT_REMOVED }, { n });
 }
 
 __global__ void k_count_new(const uint64_t* keys, int32_t K, const int32_t* bodySub, uint32_t* added)
 {
 	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (k < K)
-		atomicAdd(&added[bodySub[(int32_t)(keys[k] >> 32)]], 1u);
+	wave_add_keyed(added, k < K ? (uint32_t)bodySub[(int32_t)(keys[k] >> 32)] : ~0u, 1u);
 }
 
 // contacts per sub-world after the step's removals and additions (entry S
@@ -1944,7 +1949,8 @@ bool remove_and_place(World& w)
 	if (C0 > 0)
 	{
 		k_remove_flags<<<grid_for(C0), kBlock, 0, s>>>(body_view(w), oldBodies, C0, rm);
-		k_count_removed<<<grid_for(C0), kBlock, 0, s>>>(rm, oldBodies, w.bodySub.p, C0, removed, w.counters.p);
+		k_count_removed<<<reduce_grid(C0, kBlock), kBlock, 0, s>>>(rm, oldBodies, w.bodySub.p, C0, removed,
+																  w.counters.p);
 	}
 	if (K > 0)
 		k_count_new<<<grid_for(K), kBlock, 0, s>>>(w.newKeys.p, K, w.bodySub.p, added);
