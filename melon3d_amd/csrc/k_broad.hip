@@ -99,6 +99,8 @@ DContacts columns_view(ContactColumns& cc, int32_t cap)
 {
 	DContacts C;
 	C.cap = cap;
+	C.cs = 4;
+	C.js = 1;
 	C.bodies = cc.bodies.p;
 	C.mat = cc.mat.p;
 	C.normal = cc.normal.p;
@@ -1217,39 +1219,57 @@ __global__ void k_make_moves(const uint32_t* hole, const uint32_t* holeOff, cons
 		moves[tailOff[c]].y = c;
 }
 
+// Point j of contact s, in every point column, to point j of contact d, and
+// a quarter of the contact's own columns: four consecutive lanes move one
+// contact, so a wave moves 16 contacts' points as contiguous runs (a
+// contact's four points are adjacent in each column).
+__device__ __forceinline__ void copy_contact_part(const DContacts& S, int32_t s, const DContacts& D, int32_t d, int j)
+{
+	int32_t pd = pidx(D, d, j), ps = pidx(S, s, j);
+	D.pA[pd] = S.pA[ps];
+	D.pB[pd] = S.pB[ps];
+	D.pJA[pd] = S.pJA[ps];
+	D.pJB[pd] = S.pJB[ps];
+	D.pRA[pd] = S.pRA[ps];
+	D.pRB[pd] = S.pRB[ps];
+	D.pFA[pd] = S.pFA[ps];
+	D.pFB[pd] = S.pFB[ps];
+	D.pSlip[pd] = S.pSlip[ps];
+	switch (j)
+	{
+	case 0:
+		D.bodies[d] = S.bodies[s];
+		D.hint[d] = S.hint[s];
+		D.mat[d] = S.mat[s];
+		break;
+	case 1:
+		D.normal[d] = S.normal[s];
+		D.cachePA[d] = S.cachePA[s];
+		break;
+	case 2:
+		D.cachePB[d] = S.cachePB[s];
+		D.cacheQA[d] = S.cacheQA[s];
+		break;
+	default:
+		D.cacheQB[d] = S.cacheQB[s];
+		break;
+	}
+}
+
+// One lane per (move, point).
 __global__ void k_apply_moves(DContacts C, const int2* moves, int32_t count)
 {
-	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	int32_t k = (int32_t)(t >> 2);
 	if (k >= count)
 		return;
-	int32_t d = moves[k].x, s = moves[k].y;
-	C.bodies[d] = C.bodies[s];
-	C.hint[d] = C.hint[s];
-	C.mat[d] = C.mat[s];
-	C.normal[d] = C.normal[s];
-	C.cachePA[d] = C.cachePA[s];
-	C.cachePB[d] = C.cachePB[s];
-	C.cacheQA[d] = C.cacheQA[s];
-	C.cacheQB[d] = C.cacheQB[s];
-	for (int j = 0; j < 4; ++j)
-	{
-		int32_t pd = pidx(C, d, j), ps = pidx(C, s, j);
-		C.pA[pd] = C.pA[ps];
-		C.pB[pd] = C.pB[ps];
-		C.pJA[pd] = C.pJA[ps];
-		C.pJB[pd] = C.pJB[ps];
-		C.pRA[pd] = C.pRA[ps];
-		C.pRB[pd] = C.pRB[ps];
-		C.pFA[pd] = C.pFA[ps];
-		C.pFB[pd] = C.pFB[ps];
-		C.pSlip[pd] = C.pSlip[ps];
-	}
+	copy_contact_part(C, moves[k].y, C, moves[k].x, (int)(t & 3));
 }
 
 bool apply_moves(World& w, const int2* dMoves, int32_t count)
 {
 	if (count > 0)
-		k_apply_moves<<<grid_for(count), kBlock, 0, w.stream>>>(contact_view(w), dMoves, count);
+		k_apply_moves<<<grid_for(4 * (int64_t)count), kBlock, 0, w.stream>>>(contact_view(w), dMoves, count);
 	return true;
 }
 
@@ -1322,7 +1342,7 @@ bool remove_separated(World& w)
 		if (!w.moves.ensure(H))
 			return false;
 		k_make_moves<<<grid_for(C), kBlock, 0, s>>>(hole, holeOff, tail, tailOff, C, w.moves.p);
-		k_apply_moves<<<grid_for(H), kBlock, 0, s>>>(contact_view(w), w.moves.p, H);
+		k_apply_moves<<<grid_for(4 * (int64_t)H), kBlock, 0, s>>>(contact_view(w), w.moves.p, H);
 	}
 	if (!hip_ok(hipStreamSynchronize(s), "contact removal"))
 		return false;
@@ -1857,34 +1877,11 @@ __device__ __forceinline__ int32_t place_tail(const PlaceCtx& X, int32_t sw, uin
 	return (int32_t)X.newStart[sw] + (h - st0);
 }
 
-__device__ __forceinline__ void copy_contact(const DContacts& S, int32_t s, const DContacts& D, int32_t d)
-{
-	D.bodies[d] = S.bodies[s];
-	D.hint[d] = S.hint[s];
-	D.mat[d] = S.mat[s];
-	D.normal[d] = S.normal[s];
-	D.cachePA[d] = S.cachePA[s];
-	D.cachePB[d] = S.cachePB[s];
-	D.cacheQA[d] = S.cacheQA[s];
-	D.cacheQB[d] = S.cacheQB[s];
-	for (int j = 0; j < 4; ++j)
-	{
-		int32_t pd = pidx(D, d, j), ps = pidx(S, s, j);
-		D.pA[pd] = S.pA[ps];
-		D.pB[pd] = S.pB[ps];
-		D.pJA[pd] = S.pJA[ps];
-		D.pJB[pd] = S.pJB[ps];
-		D.pRA[pd] = S.pRA[ps];
-		D.pRB[pd] = S.pRB[ps];
-		D.pFA[pd] = S.pFA[ps];
-		D.pFB[pd] = S.pFB[ps];
-		D.pSlip[pd] = S.pSlip[ps];
-	}
-}
-
+// One lane per (surviving contact, point).
 __global__ void k_place_old(PlaceCtx X, DContacts src, DContacts dstC)
 {
-	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	int32_t c = (int32_t)(t >> 2);
 	if (c >= X.C0 || X.rm[c])
 		return;
 	int32_t sw = contact_sub(X.bodySub, X.bodies, c);
@@ -1895,7 +1892,7 @@ __global__ void k_place_old(PlaceCtx X, DContacts src, DContacts dstC)
 		d = (int32_t)X.newStart[sw] + (int32_t)l;
 	else
 		d = place_tail(X, sw, X.tailOff[c] - X.tailOff[st0]);
-	copy_contact(src, c, dstC, d);
+	copy_contact_part(src, c, dstC, d, (int)(t & 3));
 }
 
 __global__ void k_place_new(PlaceCtx X, const uint64_t* keys, int32_t K, int32_t* dst)
@@ -2003,7 +2000,7 @@ bool remove_and_place(World& w)
 	DContacts src = contact_view(w);
 	DContacts dstC = columns_view(w.cc2, w.contactCap2);
 	if (C0 > 0)
-		k_place_old<<<grid_for(C0), kBlock, 0, s>>>(X, src, dstC);
+		k_place_old<<<grid_for(4 * (int64_t)C0), kBlock, 0, s>>>(X, src, dstC);
 	if (K > 0)
 	{
 		int32_t* dst = w.i32c.p;

@@ -897,6 +897,8 @@ __device__ __forceinline__ void sweep_run<NormalsPass>(const GiantSolveCtx& S, i
 	PassCtx P = S.X.P;
 	P.C.pJA = S.nrm + 9 * S.nrmStride;
 	P.C.cap = (int32_t)S.nrmStride;
+	P.C.cs = 1;
+	P.C.js = (int32_t)S.nrmStride;
 	NormalsPass::run(P, make_int4(e, r.y, r.z, r.w), d);
 }
 

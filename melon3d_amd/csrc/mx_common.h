@@ -200,6 +200,10 @@ __device__ __forceinline__ void mark_sub_stable_dirty(const DBodies& B, int32_t 
 struct DContacts
 {
 	int32_t cap;
+	// point j of contact c is at c * cs + j * js of the point columns: the
+	// world's columns keep a contact's four points together (cs 4, js 1), so
+	// a contact gathered at random costs one 64-byte run per column
+	int32_t cs, js;
 	int2* bodies;     // bodyA < bodyB
 	float4* mat;      // friction, restitution, rollingResistance, flags (CF_* bits)
 	float4* normal;   // normal xyz, pointCount (int bits)
@@ -255,7 +259,7 @@ MX_HD void store_iw(const DBodies& B, int32_t i, const m3d_mat3& m)
 	B.iw2[i] = v3_f4(m.cz, B.iw2[i].w);
 }
 
-MX_HD int32_t pidx(const DContacts& C, int32_t c, int32_t j) { return j * C.cap + c; }
+MX_HD int32_t pidx(const DContacts& C, int32_t c, int32_t j) { return c * C.cs + j * C.js; }
 
 // Load the persistent manifold of contact c (normal, count, points).
 MX_HD void load_manifold(const DContacts& C, int32_t c, Manifold& m)

@@ -311,7 +311,8 @@ static bool realloc_exact(DBuf<T>& b, size_t cap, size_t keep)
 	return true;
 }
 
-// Contact point columns are slot-major ([j * cap + c]); growing re-lays them.
+// Contact point columns keep a contact's four points together ([4c + j]); growing
+// copies the kept contacts' points.
 static bool grow_points(DBuf<float4>& col, size_t oldCap, size_t newCap, size_t keep)
 {
 	float4* np = nullptr;
@@ -319,8 +320,9 @@ static bool grow_points(DBuf<float4>& col, size_t oldCap, size_t newCap, size_t 
 		return false;
 	if (col.p != nullptr)
 	{
-		for (int j = 0; j < 4 && keep > 0; ++j)
-			(void)hipMemcpy(np + j * newCap, col.p + j * oldCap, keep * sizeof(float4), hipMemcpyDeviceToDevice);
+		// a contact's four points are contiguous (point j of contact c at 4c + j)
+		if (keep > 0)
+			(void)hipMemcpy(np, col.p, 4 * keep * sizeof(float4), hipMemcpyDeviceToDevice);
 		(void)hipFree(col.p);
 	}
 	col.p = np;
@@ -1923,7 +1925,7 @@ int m3dx_world_export(m3d_world* world, m3dx_snapshot* s)
 			o.pointCount = f2i(nrm[c].w);
 			for (int j = 0; j < 4; ++j)
 			{
-				size_t p = (size_t)j * cap + c;
+				size_t p = (size_t)c * 4 + j;
 				m3dx_point_state& op = o.points[j];
 				float4 a = pts[0][p], b = pts[1][p], ja = pts[2][p], jb = pts[3][p], ra = pts[4][p], rb = pts[5][p];
 				put3(op.localAnchorA, f4_v3(a));
@@ -2090,7 +2092,7 @@ m3d_world* m3dx_world_import(const m3dx_snapshot* s)
 			cqb[c] = q_f4(get4(o.cacheRotB));
 			for (int j = 0; j < 4; ++j)
 			{
-				size_t p = (size_t)j * cap + c;
+				size_t p = (size_t)c * 4 + j;
 				const m3dx_point_state& op = o.points[j];
 				pts[0][p] = v3_f4(get3(op.localAnchorA), op.separationOffset);
 				pts[1][p] = v3_f4(get3(op.localAnchorB), op.wN);
