@@ -104,10 +104,7 @@ DContacts columns_view(ContactColumns& cc, int32_t cap)
 	C.bodies = cc.bodies.p;
 	C.mat = cc.mat.p;
 	C.normal = cc.normal.p;
-	C.cachePA = cc.cachePA.p;
-	C.cachePB = cc.cachePB.p;
-	C.cacheQA = cc.cacheQA.p;
-	C.cacheQB = cc.cacheQB.p;
+	C.cache = cc.cache.p;
 	C.pA = cc.pA.p;
 	C.pB = cc.pB.p;
 	C.pJA = cc.pJA.p;
@@ -952,10 +949,11 @@ __global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t c
 	C.mat[c] = make_float4(friction, restitution, rolling, u2f(cf));
 	C.normal[c] = make_float4(0.0f, 1.0f, 0.0f, i2f(0));
 	float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	C.cachePA[c] = z;
-	C.cachePB[c] = z;
-	C.cacheQA[c] = z;
-	C.cacheQB[c] = z;
+	float4* kc = cache_of(C, c);
+	kc[0] = z;
+	kc[1] = z;
+	kc[2] = z;
+	kc[3] = z;
 	for (int j = 0; j < 4; ++j)
 	{
 		int32_t p = pidx(C, c, j);
@@ -1235,23 +1233,20 @@ __device__ __forceinline__ void copy_contact_part(const DContacts& S, int32_t s,
 	D.pFA[pd] = S.pFA[ps];
 	D.pFB[pd] = S.pFB[ps];
 	D.pSlip[pd] = S.pSlip[ps];
+	cache_of(D, d)[j] = cache_of(S, s)[j];
 	switch (j)
 	{
 	case 0:
 		D.bodies[d] = S.bodies[s];
 		D.hint[d] = S.hint[s];
-		D.mat[d] = S.mat[s];
 		break;
 	case 1:
-		D.normal[d] = S.normal[s];
-		D.cachePA[d] = S.cachePA[s];
+		D.mat[d] = S.mat[s];
 		break;
 	case 2:
-		D.cachePB[d] = S.cachePB[s];
-		D.cacheQA[d] = S.cacheQA[s];
+		D.normal[d] = S.normal[s];
 		break;
 	default:
-		D.cacheQB[d] = S.cacheQB[s];
 		break;
 	}
 }

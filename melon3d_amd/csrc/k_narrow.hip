@@ -53,8 +53,8 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 	bool regen = !(cf & CF_HAS_CACHE);
 	if (!regen)
 	{
-		regen = pose_changed(pa, qa, f4_v3(C.cachePA[c]), f4_q(C.cacheQA[c])) ||
-				pose_changed(pb, qb, f4_v3(C.cachePB[c]), f4_q(C.cacheQB[c]));
+		const float4* k = cache_of(C, c);
+		regen = pose_changed(pa, qa, f4_v3(k[0]), f4_q(k[1])) || pose_changed(pb, qb, f4_v3(k[2]), f4_q(k[3]));
 	}
 	if (regen)
 	{
@@ -69,10 +69,11 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 		collide_shapes(sA, xfA, sB, xfB, margin, &m);
 		inherit_friction_anchors(old, m, pa, qa, pb, qb, sA.type == M3D_SHAPE_BOX && sB.type == M3D_SHAPE_BOX);
 		cf |= CF_HAS_CACHE;
-		C.cachePA[c] = v3_f4(pa, 0.0f);
-		C.cacheQA[c] = q_f4(qa);
-		C.cachePB[c] = v3_f4(pb, 0.0f);
-		C.cacheQB[c] = q_f4(qb);
+		float4* k = cache_of(C, c);
+		k[0] = v3_f4(pa, 0.0f);
+		k[1] = q_f4(qa);
+		k[2] = v3_f4(pb, 0.0f);
+		k[3] = q_f4(qb);
 		set_separation_offsets(m, pa, qa, pb, qb);
 		store_manifold(C, c, m);
 		pointCount = m.pointCount;

@@ -207,10 +207,7 @@ struct DContacts
 	int2* bodies;     // bodyA < bodyB
 	float4* mat;      // friction, restitution, rollingResistance, flags (CF_* bits)
 	float4* normal;   // normal xyz, pointCount (int bits)
-	float4* cachePA;  // pose cache
-	float4* cachePB;
-	float4* cacheQA;
-	float4* cacheQB;
+	float4* cache;    // pose cache, 4 per contact (see cache_of)
 	float4* pA;  // localAnchorA xyz, separationOffset
 	float4* pB;  // localAnchorB xyz, wN
 	float4* pJA; // angJA xyz, lambdaN
@@ -260,6 +257,10 @@ MX_HD void store_iw(const DBodies& B, int32_t i, const m3d_mat3& m)
 }
 
 MX_HD int32_t pidx(const DContacts& C, int32_t c, int32_t j) { return c * C.cs + j * C.js; }
+
+// Pose cache of contact c as one 64-byte run: position A, rotation A,
+// position B, rotation B.
+MX_HD float4* cache_of(const DContacts& C, int32_t c) { return C.cache + 4 * (int64_t)c; }
 
 // Load the persistent manifold of contact c (normal, count, points).
 MX_HD void load_manifold(const DContacts& C, int32_t c, Manifold& m)

@@ -391,8 +391,8 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 	bool stale = !hasCache;
 	if (hasCache)
 	{
-		stale = manifold_stale(pa, qa, pb, qb, f4_v3(C.cachePA[c]), f4_q(C.cacheQA[c]), f4_v3(C.cachePB[c]),
-							   f4_q(C.cacheQB[c]));
+		const float4* k = cache_of(C, c);
+		stale = manifold_stale(pa, qa, pb, qb, f4_v3(k[0]), f4_q(k[1]), f4_v3(k[2]), f4_q(k[3]));
 	}
 	if (stale)
 	{
@@ -460,10 +460,11 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 			inherit_friction_anchors(old, m, pa, qa, pb, qb,
 									 sA.type == M3D_SHAPE_BOX && sB.type == M3D_SHAPE_BOX);
 			cf |= CF_HAS_CACHE;
-			C.cachePA[c] = v3_f4(pa, 0.0f);
-			C.cacheQA[c] = q_f4(qa);
-			C.cachePB[c] = v3_f4(pb, 0.0f);
-			C.cacheQB[c] = q_f4(qb);
+			float4* k = cache_of(C, c);
+			k[0] = v3_f4(pa, 0.0f);
+			k[1] = q_f4(qa);
+			k[2] = v3_f4(pb, 0.0f);
+			k[3] = q_f4(qb);
 			set_separation_offsets(m, pa, qa, pb, qb);
 		}
 		store_manifold(C, c, m);

@@ -126,7 +126,8 @@ struct ContactColumns
 {
 	DBuf<int2> bodies;
 	DBuf<int2> hint; // last greedy color and overflow level (giant islands), a warm start
-	DBuf<float4> mat, normal, cachePA, cachePB, cacheQA, cacheQB;
+	DBuf<float4> mat, normal;
+	DBuf<float4> cache; // pose cache, 4 per contact
 	DBuf<float4> pA, pB, pJA, pJB, pRA, pRB, pFA, pFB, pSlip; // 4 * cap each
 };
 

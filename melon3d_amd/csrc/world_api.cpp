@@ -50,7 +50,7 @@ World::~World()
 	BodyColumns& b = bc;
 	DBuf<float4>* f4[] = { &b.pos, &b.rot, &b.vel, &b.angVel, &b.prevPos, &b.prevRot, &b.prevVel, &b.prevAngVel,
 						   &b.invIL, &b.iw0, &b.iw1, &b.iw2, &b.shape, &b.mat, &b.force, &b.torque, &b.fatLo, &b.fatHi,
-						   &cc.mat, &cc.normal, &cc.cachePA, &cc.cachePB, &cc.cacheQA, &cc.cacheQB, &cc.pA, &cc.pB,
+						   &cc.mat, &cc.normal, &cc.cache, &cc.pA, &cc.pB,
 						   &cc.pJA, &cc.pJB, &cc.pRA, &cc.pRB, &cc.pFA, &cc.pFB, &cc.pSlip };
 	for (auto* p : f4)
 		p->release();
@@ -61,7 +61,7 @@ World::~World()
 	cc.bodies.release();
 	cc.hint.release();
 	{
-		DBuf<float4>* c2[] = { &cc2.mat, &cc2.normal, &cc2.cachePA, &cc2.cachePB, &cc2.cacheQA, &cc2.cacheQB,
+		DBuf<float4>* c2[] = { &cc2.mat, &cc2.normal, &cc2.cache,
 							   &cc2.pA, &cc2.pB, &cc2.pJA, &cc2.pJB, &cc2.pRA, &cc2.pRB, &cc2.pFA, &cc2.pFB,
 							   &cc2.pSlip };
 		for (auto* p : c2)
@@ -342,13 +342,13 @@ bool ensure_contact_columns(World& w, ContactColumns& c, int32_t& capRef, int32_
 	(void)hipStreamSynchronize(w.stream);
 	size_t oldCap = (size_t)capRef;
 	size_t cap = std::max<size_t>((size_t)n, oldCap * 3 / 2 + 256);
-	DBuf<float4>* cols[] = { &c.mat, &c.normal, &c.cachePA, &c.cachePB, &c.cacheQA, &c.cacheQB };
+	DBuf<float4>* cols[] = { &c.mat, &c.normal };
 	for (auto* p : cols)
 		if (!realloc_exact(*p, cap, keep))
 			return false;
 	if (!realloc_exact(c.bodies, cap, keep) || !realloc_exact(c.hint, cap, keep))
 		return false;
-	DBuf<float4>* pts[] = { &c.pA, &c.pB, &c.pJA, &c.pJB, &c.pRA, &c.pRB, &c.pFA, &c.pFB, &c.pSlip };
+	DBuf<float4>* pts[] = { &c.pA, &c.pB, &c.pJA, &c.pJB, &c.pRA, &c.pRB, &c.pFA, &c.pFB, &c.pSlip, &c.cache };
 	for (auto* p : pts)
 		if (!grow_points(*p, oldCap, cap, keep))
 			return false;
@@ -1883,17 +1883,22 @@ int m3dx_world_export(m3d_world* world, m3dx_snapshot* s)
 	{
 		int32_t cap = w.contactCap;
 		std::vector<int2> bodies(C);
-		std::vector<float4> mat(C), nrm(C), cpa(C), cpb(C), cqa(C), cqb(C);
+		std::vector<float4> mat(C), nrm(C), cpa(C), cpb(C), cqa(C), cqb(C), cch(4 * (size_t)C);
 		std::vector<float4> pts[9];
 		DBuf<float4>* pcols[9] = { &w.cc.pA, &w.cc.pB, &w.cc.pJA, &w.cc.pJB, &w.cc.pRA, &w.cc.pRB, &w.cc.pFA,
 								   &w.cc.pFB, &w.cc.pSlip };
 		bool ok = hip_ok(hipMemcpy(bodies.data(), w.cc.bodies.p, C * sizeof(int2), hipMemcpyDeviceToHost), "export") &&
 				  hip_ok(hipMemcpy(mat.data(), w.cc.mat.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export") &&
 				  hip_ok(hipMemcpy(nrm.data(), w.cc.normal.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export") &&
-				  hip_ok(hipMemcpy(cpa.data(), w.cc.cachePA.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export") &&
-				  hip_ok(hipMemcpy(cpb.data(), w.cc.cachePB.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export") &&
-				  hip_ok(hipMemcpy(cqa.data(), w.cc.cacheQA.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export") &&
-				  hip_ok(hipMemcpy(cqb.data(), w.cc.cacheQB.p, C * sizeof(float4), hipMemcpyDeviceToHost), "export");
+				  hip_ok(hipMemcpy(cch.data(), w.cc.cache.p, 4 * (size_t)C * sizeof(float4), hipMemcpyDeviceToHost),
+						 "export");
+		for (int32_t c = 0; c < C && ok; ++c)
+		{
+			cpa[c] = cch[4 * (size_t)c];
+			cqa[c] = cch[4 * (size_t)c + 1];
+			cpb[c] = cch[4 * (size_t)c + 2];
+			cqb[c] = cch[4 * (size_t)c + 3];
+		}
 		for (int k = 0; k < 9 && ok; ++k)
 		{
 			pts[k].resize((size_t)4 * cap);
@@ -2073,7 +2078,7 @@ m3d_world* m3dx_world_import(const m3dx_snapshot* s)
 		}
 		int32_t cap = w.contactCap;
 		std::vector<int2> bodies(C);
-		std::vector<float4> mat(C), nrm(C), cpa(C), cpb(C), cqa(C), cqb(C);
+		std::vector<float4> mat(C), nrm(C), cch(4 * (size_t)C);
 		std::vector<float4> pts[9];
 		for (int k = 0; k < 9; ++k)
 			pts[k].assign((size_t)4 * cap, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -2086,10 +2091,10 @@ m3d_world* m3dx_world_import(const m3dx_snapshot* s)
 			bodies[c] = make_int2(o.bodyA, o.bodyB);
 			mat[c] = make_float4(o.friction, o.restitution, o.rollingResistance, u2f(cf));
 			nrm[c] = v3_f4(get3(o.normal), i2f(o.pointCount));
-			cpa[c] = v3_f4(get3(o.cachePosA), 0.0f);
-			cpb[c] = v3_f4(get3(o.cachePosB), 0.0f);
-			cqa[c] = q_f4(get4(o.cacheRotA));
-			cqb[c] = q_f4(get4(o.cacheRotB));
+			cch[4 * (size_t)c] = v3_f4(get3(o.cachePosA), 0.0f);
+			cch[4 * (size_t)c + 1] = q_f4(get4(o.cacheRotA));
+			cch[4 * (size_t)c + 2] = v3_f4(get3(o.cachePosB), 0.0f);
+			cch[4 * (size_t)c + 3] = q_f4(get4(o.cacheRotB));
 			for (int j = 0; j < 4; ++j)
 			{
 				size_t p = (size_t)c * 4 + j;
@@ -2110,10 +2115,8 @@ m3d_world* m3dx_world_import(const m3dx_snapshot* s)
 		bool ok = hip_ok(hipMemcpy(w.cc.bodies.p, bodies.data(), C * sizeof(int2), hipMemcpyHostToDevice), "import") &&
 				  hip_ok(hipMemcpy(w.cc.mat.p, mat.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import") &&
 				  hip_ok(hipMemcpy(w.cc.normal.p, nrm.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import") &&
-				  hip_ok(hipMemcpy(w.cc.cachePA.p, cpa.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import") &&
-				  hip_ok(hipMemcpy(w.cc.cachePB.p, cpb.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import") &&
-				  hip_ok(hipMemcpy(w.cc.cacheQA.p, cqa.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import") &&
-				  hip_ok(hipMemcpy(w.cc.cacheQB.p, cqb.data(), C * sizeof(float4), hipMemcpyHostToDevice), "import");
+				  hip_ok(hipMemcpy(w.cc.cache.p, cch.data(), 4 * (size_t)C * sizeof(float4), hipMemcpyHostToDevice),
+						 "import");
 		for (int k = 0; k < 9 && ok; ++k)
 			ok = hip_ok(hipMemcpy(pcols[k]->p, pts[k].data(), (size_t)4 * cap * sizeof(float4), hipMemcpyHostToDevice),
 						"import");
