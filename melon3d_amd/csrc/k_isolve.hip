@@ -394,6 +394,7 @@ struct GiantCtx
 	int32_t* color;           // per entry
 	int32_t* level;           // per entry
 	uint32_t* ctr;            // 6 convergence counters (zeroed before launch)
+	uint32_t* bar;            // GridSync counters (kBarWords, zeroed before launch)
 	uint32_t* counters;
 	uint64_t* slotKeys;       // per entry, sorted later
 	uint32_t* slotVals;
@@ -406,15 +407,51 @@ __device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
 	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & (IM_GRID | IM_JOINT)) == IM_GRID;
 }
 
-// Barrier of a co-resident grid: one monotonic arrival counter (zeroed
-// before the launch); barrier e completes when it reaches (e + 1) * blocks.
-// The release fence publishes this workgroup's writes before it arrives,
-// the acquire fence drops stale cache lines before it reads others'.
+// Barrier of a co-resident grid, in two levels: workgroup b arrives on the
+// counter of group b % kBarGroups, the last arrival of a group on the top
+// counter, and every workgroup waits for the top counter. Each counter has
+// its own 128-byte line: device-scope atomics on one address from every
+// workgroup serialize (about 0.04 us each), and the two levels leave at most
+// blocks / kBarGroups + kBarGroups of them in a row. All counters are
+// monotonic (zeroed before the launch): barrier e completes when group g
+// reaches (e + 1) * size(g) and the top counter (e + 1) * groups. The
+// release fence publishes this workgroup's writes before it arrives, the
+// group's last arrival passes them on (acquire-release) to the top counter,
+// and the acquire fence drops stale cache lines before a workgroup reads
+// others' writes.
+constexpr uint32_t kBarGroups = 8;
+constexpr uint32_t kBarStride = 32;                         // u32 per counter
+constexpr uint32_t kBarWords = (kBarGroups + 2) * kBarStride; // groups, top, leave
+
 struct GridSync
 {
-	uint32_t* count;
+	uint32_t* base; // kBarWords words, zero at the launch
 	uint32_t blocks;
 	uint32_t epoch;
+
+	__device__ uint32_t* top() const { return base + kBarGroups * kBarStride; }
+
+	__device__ void arrive_lane0() const
+	{
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+		const uint32_t g = blockIdx.x % kBarGroups;
+		const uint32_t size = blocks / kBarGroups + (g < blocks % kBarGroups ? 1u : 0u);
+		uint32_t old = __hip_atomic_fetch_add(base + g * kBarStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (old + 1u == (epoch + 1u) * size)
+		{
+			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+			__hip_atomic_fetch_add(top(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+	}
+
+	__device__ void wait_lane0() const
+	{
+		const uint32_t groups = blocks < kBarGroups ? blocks : kBarGroups;
+		const uint32_t target = (epoch + 1u) * groups;
+		while (__hip_atomic_load(top(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+			__builtin_amdgcn_s_sleep(1);
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+	}
 
 	// Barrier; then returns *word as read by one lane after it (the same
 	// value for every lane of the grid as long as nobody writes *word
@@ -425,12 +462,8 @@ struct GridSync
 		__syncthreads();
 		if (threadIdx.x == 0)
 		{
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-			uint32_t target = (epoch + 1u) * blocks;
-			__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-				__builtin_amdgcn_s_sleep(1);
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+			arrive_lane0();
+			wait_lane0();
 			if (word)
 				value = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
@@ -447,21 +480,13 @@ struct GridSync
 	{
 		__syncthreads();
 		if (threadIdx.x == 0)
-		{
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-			__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		}
+			arrive_lane0();
 	}
 
 	__device__ void wait()
 	{
 		if (threadIdx.x == 0)
-		{
-			uint32_t target = (epoch + 1u) * blocks;
-			while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-				__builtin_amdgcn_s_sleep(1);
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-		}
+			wait_lane0();
 		++epoch;
 		__syncthreads();
 	}
@@ -629,7 +654,7 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 
 __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 {
-	GridSync grid{G.ctr + 8, gridDim.x, 0u};
+	GridSync grid{G.bar, gridDim.x, 0u};
 	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
 	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt);
@@ -783,7 +808,7 @@ struct GiantSolveCtx
 	const uint32_t* slotStart; // [0, nSlots]
 	const uint32_t* counters;
 	uint32_t* counters2;       // the same block, for the record counters
-	uint32_t* barrier;         // zeroed arrival counter
+	uint32_t* barrier;         // GridSync counters (kBarWords), zero between launches
 	int4* rec;                 // solve records, grouped within each slot
 	const uint64_t* keys;      // sorted (slot, island) keys
 	int bitsI;
@@ -1256,12 +1281,11 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 		giant_sweep<Pass>(S, ss, grid, nSlots);
 	if (threadIdx.x == 0)
 	{
-		uint32_t left = __hip_atomic_fetch_add(S.barrier + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t* leave = S.barrier + (kBarGroups + 1) * kBarStride;
+		uint32_t left = __hip_atomic_fetch_add(leave, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (left + 1 == gridDim.x)
-		{
-			__hip_atomic_store(S.barrier, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			__hip_atomic_store(S.barrier + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		}
+			for (uint32_t k = 0; k < kBarGroups + 2; ++k)
+				__hip_atomic_store(S.barrier + k * kBarStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
 }
 
@@ -1432,7 +1456,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
 		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPosG.ensure(3 * (size_t)NC) ||
 		!w.sNrm.ensure(kSchedBlocks * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
-		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.sDirty.ensure(two + 4) ||
+		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.gridBar.ensure(2 * (size_t)kBarWords) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
 		return false;
 	uint32_t* bodyCount = w.s32c.p;
@@ -1442,7 +1466,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
 	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, w.scanStatus.p, s);
 	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
-	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6), barriers 8 and 9
+	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6)
+	launch_fill_u32(w.gridBar.p, 0u, 2 * (int64_t)kBarWords, s); // the passes' and the order's barriers
 	launch_fill_u32((uint32_t*)w.sDirty.p, 0u, (int64_t)(two + 3) / 4, s);
 
 	GiantCtx G;
@@ -1454,6 +1479,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.color = w.sColor.p;
 	G.level = w.sLevel.p;
 	G.ctr = w.sCtr.p;
+	G.bar = w.gridBar.p + kBarWords;
 	G.counters = w.counters.p;
 	G.slotKeys = w.s64b.p;
 	G.slotVals = w.s32b.p;
@@ -1479,7 +1505,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.slotStart = w.sSlotStart.p;
 	S.counters = w.counters.p;
 	S.counters2 = w.counters.p;
-	S.barrier = w.sCtr.p + 9;
+	S.barrier = w.gridBar.p;
 	S.rec = w.sRecG.p;
 	S.nrm = w.sNrm.p;
 	S.nrmStride = NC;

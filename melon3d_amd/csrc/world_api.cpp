@@ -122,6 +122,7 @@ World::~World()
 	sPacket.release();
 	sRecG.release();
 	sPosG.release();
+	gridBar.release();
 	sNrm.release();
 	sBucket.release();
 	sPred.release();
