@@ -47,12 +47,14 @@ __device__ __forceinline__ int32_t uf_find(int32_t* parent, int32_t x)
 {
 	for (;;)
 	{
-		int32_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+		// agent scope, as the CAS in uf_union: the forest lives in device memory
+		// and only this kernel's lanes read it
+		int32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (p == x)
 			return x;
-		int32_t gp = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+		int32_t gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (gp != p)
-			__atomic_store_n(&parent[x], gp, __ATOMIC_RELAXED); // halving: still an ancestor
+			__hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); // halving: still an ancestor
 		x = p;
 	}
 }
