@@ -1217,6 +1217,13 @@ __global__ void k_make_moves(const uint32_t* hole, const uint32_t* holeOff, cons
 		moves[tailOff[c]].y = c;
 }
 
+__global__ void k_set_move_sources(int2* moves, const int32_t* src, int32_t H)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k < H)
+		moves[k].y = src[k];
+}
+
 // Point j of contact s, in every point column, to point j of contact d, and
 // a quarter of the contact's own columns: four consecutive lanes move one
 // contact, so a wave moves 16 contacts' points as contiguous runs (a
@@ -1334,9 +1341,40 @@ bool remove_separated(World& w)
 	}
 	if (H > 0)
 	{
-		if (!w.moves.ensure(H))
+		// The reference removes in descending order, moving the current last
+		// contact into each removed slot (src/world.cpp:1722-1734, 100-119).
+		// The removed slots at or beyond newC come first and may take in a
+		// contact from the end too, so the survivors of [newC, C) stand in the
+		// order a replay of that region (R slots) leaves them; the holes below
+		// newC then take them from the top down: the k-th hole (ascending)
+		// receives the k-th of them.
+		std::vector<uint32_t> flags(R);
+		std::vector<int32_t> order(R);
+		if (!w.moves.ensure(H) || !w.tailOrder.ensure(R) ||
+			!hip_ok(hipMemcpyAsync(flags.data(), rm + newC, (size_t)R * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+					"removal flags") ||
+			!hip_ok(hipStreamSynchronize(s), "removal flags"))
+			return false;
+		for (int32_t i = 0; i < R; ++i)
+			order[i] = newC + i;
+		int32_t top = R - 1;
+		for (int32_t r = R - 1; r >= 0; --r)
+			if (flags[r])
+			{
+				if (r != top)
+					order[r] = order[top];
+				--top;
+			}
+		if (top + 1 != H)
+		{
+			set_error("removal replay mismatch (%d survivors, %d holes)", top + 1, H);
+			return false;
+		}
+		if (!hip_ok(hipMemcpyAsync(w.tailOrder.p, order.data(), (size_t)H * sizeof(int32_t), hipMemcpyHostToDevice, s),
+					"removal order"))
 			return false;
 		k_make_moves<<<grid_for(C), kBlock, 0, s>>>(hole, holeOff, tail, tailOff, C, w.moves.p);
+		k_set_move_sources<<<grid_for(H), kBlock, 0, s>>>(w.moves.p, w.tailOrder.p, H);
 		k_apply_moves<<<grid_for(4 * (int64_t)H), kBlock, 0, s>>>(contact_view(w), w.moves.p, H);
 	}
 	if (!hip_ok(hipStreamSynchronize(s), "contact removal"))

@@ -240,6 +240,7 @@ struct World
 	DBuf<uint32_t> sCtr;          // convergence counters of the giant-island fixpoints
 	DBuf<int2> sPacket;
 	DBuf<int4> sRecG;             // grid path: solve records by schedule position
+	DBuf<int32_t> tailOrder;      // removal: sources of the holes, in the reference's order
 	DBuf<uint32_t> gridBar;       // grid path: barrier counters of the passes and of the ordering
 	DBuf<uint32_t> sPosG;         // grid path: record position and point count by sorted entry, entry by position
 	DBuf<float4> sNrm;            // grid path: normal-pass inputs by schedule position (17 x entries)

@@ -156,6 +156,9 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	w.beginEvents.clear();
 	w.endEvents.clear();
 	memset(&w.stats, 0, sizeof(w.stats));
+	// a sidecar downloaded before this step (and not used by a destroy since)
+	// no longer matches the device's contacts once the step runs
+	w.sidecarPrefetched = false;
 	if (!push_host_changes(w))
 		return false;
 	hipMemsetAsync(w.counters.p, 0, kCountersSize * sizeof(uint32_t), w.stream);

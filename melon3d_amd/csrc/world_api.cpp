@@ -123,6 +123,7 @@ World::~World()
 	sRecG.release();
 	sPosG.release();
 	gridBar.release();
+	tailOrder.release();
 	sNrm.release();
 	sBucket.release();
 	sPred.release();

@@ -134,6 +134,32 @@ def test_giant_island_lockstep_colors(ref, dev):
 
 
 @pytest.mark.gpu
+def test_destroy_after_idle_step(ref, dev):
+    """Churn with every third step run without destroys before it: the
+    contact index the device downloads at the end of a step that followed
+    destroys must not serve destroys after a later step (the contacts have
+    changed since); contact order and events must match the reference."""
+    rw, dw = ref.create_world(), dev.create_world()
+    sr = scenes.Stream(ref, rw, count=1200, churn=12, arena=16.0)
+    sd = scenes.Stream(dev, dw, count=1200, churn=12, arena=16.0)
+    for step in range(45):
+        if step % 3 != 1:
+            sr.churn_step()
+            sd.churn_step()
+        ref.m3d_world_step(rw, DT, 4)
+        dev.m3d_world_step(dw, DT, 4)
+        a, b = ref.export_state(rw), dev.export_state(dw)
+        assert contact_pairs(a) == contact_pairs(b), f"step {step}: contact order"
+        (rb, re), (db, de) = events(ref, rw), events(dev, dw)
+        assert rb == db, f"step {step}: begin events"
+        assert re == de, f"step {step}: end events"
+        d = state_diff(a, b)
+        assert d["pos"] <= 1e-3, f"step {step}: {d}"
+    ref.m3d_world_destroy(rw)
+    dev.m3d_world_destroy(dw)
+
+
+@pytest.mark.gpu
 def test_stream_churn_lockstep(ref, dev):
     """Streaming churn (destroy oldest + respawn every step) driven through
     the API on both engines; contact order and events must match."""
