@@ -1898,14 +1898,57 @@ struct PlaceCtx
 	const uint32_t* tailOff;
 	const int32_t* holeList;  // removed positions inside the shrunk ranges, ascending
 	const uint32_t* counters;
+	const uint32_t* regionOff; // per sub-world: start of its tail region in rankOf (its removals)
+	const int32_t* rankOf;     // per tail-region slot: rank of its element among the tails
 	int32_t C0;
 };
 
-// The swap-with-last removal of src/world.cpp:1725-1733 on one sub-world's
-// range, in closed form: the k-th tail (ascending) fills the k-th hole.
-__device__ __forceinline__ int32_t place_tail(const PlaceCtx& X, int32_t sw, uint32_t t)
+// The swap-with-last removal of src/world.cpp:1722-1734 on one sub-world's
+// range: the reference removes in descending order, moving the current last
+// contact into each removed slot. The removed slots at or beyond the shrunk
+// size newC come first and may take in a contact from the end too, so the
+// survivors of the tail region [newC, oldC + added) (old slots, then the
+// contacts created this step) stand in the order a replay of that region
+// leaves them, and the k-th hole (ascending) takes the k-th of them. The
+// region has as many slots as the sub-world has removals; one lane per
+// sub-world replays it and records each slot's rank (-1: removed).
+__global__ void k_tail_replay(const int32_t* start, const uint32_t* newC, const uint32_t* added, const uint32_t* rm,
+							  const uint32_t* regionOff, int32_t S, int32_t* order, int32_t* rankOf)
+{
+	int32_t sw = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (sw >= S)
+		return;
+	const int32_t st0 = start[sw];
+	const int32_t c0 = start[sw + 1] - st0;
+	const int32_t nc = (int32_t)newC[sw];
+	const int32_t n = c0 + (int32_t)added[sw] - nc;
+	int32_t* ord = order + regionOff[sw];
+	int32_t* rank = rankOf + regionOff[sw];
+	for (int32_t i = 0; i < n; ++i)
+	{
+		ord[i] = i;
+		rank[i] = -1;
+	}
+	int32_t top = n - 1;
+	for (int32_t r = n - 1; r >= 0; --r)
+	{
+		int32_t l = nc + r;
+		if (l < c0 && rm[st0 + l] != 0u)
+		{
+			if (r != top)
+				ord[r] = ord[top];
+			--top;
+		}
+	}
+	for (int32_t k = 0; k <= top; ++k)
+		rank[ord[k]] = k;
+}
+
+// Destination of the element at tail-region slot r of sub-world sw.
+__device__ __forceinline__ int32_t place_tail(const PlaceCtx& X, int32_t sw, uint32_t r)
 {
 	int32_t st0 = X.start[sw];
+	int32_t t = X.rankOf[X.regionOff[sw] + r];
 	int32_t h = X.holeList[X.holeOff[st0] + t];
 	return (int32_t)X.newStart[sw] + (h - st0);
 }
@@ -1924,7 +1967,7 @@ __global__ void k_place_old(PlaceCtx X, DContacts src, DContacts dstC)
 	if (l < X.newC[sw])
 		d = (int32_t)X.newStart[sw] + (int32_t)l;
 	else
-		d = place_tail(X, sw, X.tailOff[c] - X.tailOff[st0]);
+		d = place_tail(X, sw, l - X.newC[sw]);
 	copy_contact_part(src, c, dstC, d, (int)(t & 3));
 }
 
@@ -1943,15 +1986,7 @@ __global__ void k_place_new(PlaceCtx X, const uint64_t* keys, int32_t K, int32_t
 		dst[k] = (int32_t)X.newStart[sw] + (int32_t)l;
 		return;
 	}
-	// tails ascend: the old survivors beyond the shrunk range, then new ones
-	uint32_t oldTails = 0;
-	if (c0 > nc)
-	{
-		uint32_t endOff = st1 < X.C0 ? X.tailOff[st1] : X.counters[CT_TAILS];
-		oldTails = endOff - X.tailOff[st0];
-	}
-	uint32_t first = nc > c0 ? nc : c0;
-	dst[k] = place_tail(X, sw, oldTails + (l - first));
+	dst[k] = place_tail(X, sw, l - nc);
 }
 
 bool remove_and_place(World& w)
@@ -1962,7 +1997,7 @@ bool remove_and_place(World& w)
 	int32_t K = w.newCount;
 	if (C0 == 0 && K == 0)
 		return true;
-	if (!w.subWork.ensure(4 * (size_t)S + 8) || !w.u32a.ensure(std::max(C0, 1)) || !w.u32b.ensure(std::max(C0, 1)) ||
+	if (!w.subWork.ensure(5 * (size_t)S + 9) || !w.u32a.ensure(std::max(C0, 1)) || !w.u32b.ensure(std::max(C0, 1)) ||
 		!w.u32c.ensure(std::max(C0, 1)) || !w.i32a.ensure(std::max(C0, 1)) || !w.i32b.ensure(std::max(C0, 1)) ||
 		!w.i32c.ensure(std::max(K, 1)) || !ensure_prim_scratch(w, std::max<int64_t>(C0, S + 1)))
 		return false;
@@ -1970,6 +2005,7 @@ bool remove_and_place(World& w)
 	uint32_t* added = removed + S;
 	uint32_t* newC = added + S;            // S + 1
 	uint32_t* newStart = newC + S + 1;     // S + 1
+	uint32_t* regionOff = newStart + S + 1; // S + 1
 	if (!w.i32d.ensure((size_t)S + 1))
 		return false;
 	uint32_t* newKeyStart = (uint32_t*)w.i32d.p; // S + 1
@@ -1993,6 +2029,13 @@ bool remove_and_place(World& w)
 	// new keys are grouped by sub-world: exclusive scan of the counts
 	// (added has S entries; the scan's total lands in the last slot)
 	scan_exclusive(added, newKeyStart, S, newKeyStart + S, w.scanStatus.p, s);
+	// tail regions, one slot per removal of the sub-world
+	const int32_t R = (int32_t)w.hostCounters[CT_REMOVED];
+	scan_exclusive(removed, regionOff, S, regionOff + S, w.scanStatus.p, s);
+	if (!w.tailOrder.ensure(2 * (size_t)std::max(R, 1)))
+		return false;
+	k_tail_replay<<<grid_for(S), kBlock, 0, s>>>(w.subContactStart.p, newC, added, rm, regionOff, S, w.tailOrder.p,
+												 w.tailOrder.p + std::max(R, 1));
 	uint32_t* hole = w.u32b.p;
 	uint32_t* tail = w.u32c.p;
 	uint32_t* holeOff = (uint32_t*)w.i32a.p;
@@ -2029,6 +2072,8 @@ bool remove_and_place(World& w)
 	X.tailOff = tailOff;
 	X.holeList = w.eventIdx.p;
 	X.counters = w.counters.p;
+	X.regionOff = regionOff;
+	X.rankOf = w.tailOrder.p + std::max(R, 1);
 	X.C0 = C0;
 	DContacts src = contact_view(w);
 	DContacts dstC = columns_view(w.cc2, w.contactCap2);
