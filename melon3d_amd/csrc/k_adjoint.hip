@@ -1226,6 +1226,8 @@ extern "C"
 int m3dx_diff_set_checkpoint(m3d_world* world, int every)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	if (w.diff == nullptr || every < 1)
 	{
 		set_error("no differentiable recording, or checkpoint interval below 1");
@@ -1246,6 +1248,8 @@ int m3dx_diff_backward_f64(m3d_world* world, const double* seed, double* stateAd
 						   double* frictionGrad)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* D = w.diff;
 	if (D == nullptr || D->steps.empty())
 	{
@@ -1308,6 +1312,8 @@ int m3dx_diff_backward_f64(m3d_world* world, const double* seed, double* stateAd
 int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	if (w.diff == nullptr || w.diff->steps.empty())
 	{
 		set_error("no recorded steps");
@@ -1336,6 +1342,8 @@ int m3dx_diff_backward(m3d_world* world, const float* seed, float* stateAdj, flo
 int m3dx_diff_forward(m3d_world* world, int first, int count, float* stateOut)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* D = w.diff;
 	if (D == nullptr || first < 0 || count < 1 || first + count > (int)D->steps.size())
 	{

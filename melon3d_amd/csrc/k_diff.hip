@@ -104,10 +104,29 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps)
 	DiffState* D = w.diff;
 	if (D == nullptr || !D->recording)
 		return true;
-	if (w.djointCount > 0)
+	// The adjoint replays the serial island schedule (src/xpbd.cpp:898-974).
+	// An island of kBigIslandContacts or more solver contacts is solved by
+	// the colored schedule instead (src/xpbd.cpp:1167-1369, packets with
+	// invW = 1 / wN and the overflow pass), and joints are not replayed:
+	// such a step is not recorded, recording stops, and the records refuse
+	// every later use, so no gradient of a trajectory the engine did not run
+	// is ever returned. The step itself runs as usual.
+	if (w.djointCount > 0 || w.stats.coloredIslands > 0)
 	{
-		set_error("differentiable recording does not support joints");
-		return false;
+		char msg[256];
+		if (w.djointCount > 0)
+			snprintf(msg, sizeof msg, "differentiable recording refused at step %d: the world has joints",
+					 (int)D->steps.size());
+		else
+			snprintf(msg, sizeof msg,
+					 "differentiable recording refused at step %d: %d island(s) with %d or more solver contacts take "
+					 "the colored schedule, which the adjoint does not replay",
+					 (int)D->steps.size(), (int)w.stats.coloredIslands, (int)kBigIslandContacts);
+		D->recording = false;
+		D->refusedStep = (int32_t)D->steps.size();
+		D->refusal = msg;
+		set_error("%s", msg);
+		return true;
 	}
 	hipStream_t s = w.stream;
 	int32_t n = w.deviceBodyCount;
@@ -178,6 +197,14 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps)
 }
 
 bool diff_is_recording(const World& w) { return w.diff != nullptr && w.diff->recording; }
+
+bool diff_refused(const World& w)
+{
+	if (w.diff == nullptr || w.diff->refusedStep < 0)
+		return false;
+	set_error("%s", w.diff->refusal.c_str());
+	return true;
+}
 
 bool diff_record_substep(World& w, int sub)
 {
@@ -678,7 +705,7 @@ int m3dx_diff_record(m3d_world* world, int enable)
 	}
 	if (w.diff != nullptr)
 		w.diff->recording = false;
-	return 0;
+	return diff_refused(w) ? -1 : 0;
 }
 
 int32_t m3dx_diff_steps(m3d_world* world)
@@ -692,6 +719,8 @@ int32_t m3dx_diff_steps(m3d_world* world)
 int m3dx_diff_state(m3d_world* world, int step, float* out)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* D = w.diff;
 	if (D == nullptr || step < 0 || step >= (int)D->steps.size())
 		return -1;
@@ -731,6 +760,8 @@ int m3dx_diff_replay(m3d_world* world, int first, int count, int precision, cons
 					 const void* friction, void* stateOut)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* D = w.diff;
 	if (D == nullptr || first < 0 || count < 1 || first + count > (int)D->steps.size())
 	{
@@ -797,6 +828,8 @@ int m3dx_diff_replay(m3d_world* world, int first, int count, int precision, cons
 int m3dx_diff_backward_tape(m3d_world* world, const float* seed, float* stateAdj, float* massGrad, float* frictionGrad)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* D = w.diff;
 	if (D == nullptr || D->steps.empty())
 	{
@@ -880,6 +913,8 @@ int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal, const 
 					  const double* dFriction, double* stateOut, double* dStateOut)
 {
 	World& w = world->w;
+	if (diff_refused(w))
+		return -1;
 	DiffState* Ds = w.diff;
 	if (Ds == nullptr || first < 0 || count < 1 || first + count > (int)Ds->steps.size() || stateIn == nullptr ||
 		dStateIn == nullptr || dMass == nullptr || dFriction == nullptr || stateOut == nullptr || dStateOut == nullptr)

@@ -6,6 +6,7 @@
 #include "mx_diff.h"
 #include "mx_steps.h"
 
+#include <string>
 #include <vector>
 
 namespace mx
@@ -51,6 +52,12 @@ struct DiffStepRec
 struct DiffState
 {
 	bool recording = false;
+	// step at which recording was refused (-1: none): a step whose solve the
+	// adjoint does not cover (a colored island, or joints) stops the
+	// recording, the step itself runs as usual, and every later use of the
+	// records fails with `refusal`
+	int32_t refusedStep = -1;
+	std::string refusal;
 	int32_t worlds = 1;
 	std::vector<int32_t> bodyStartHost; // worlds + 1
 	DBuf<int32_t> bodyStart;

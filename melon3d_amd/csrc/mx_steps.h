@@ -117,6 +117,8 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps);
 // records the manifold each substep's prepare left
 bool diff_record_substep(World& w, int sub);
 bool diff_is_recording(const World& w);
+// true (and the error set) when the recording was refused
+bool diff_refused(const World& w);
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);

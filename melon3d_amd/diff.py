@@ -181,6 +181,43 @@ def backward(L: Library, w, scene: DiffScene, grad_final_pos: np.ndarray, tape: 
     return adj[idx, 7:10].astype(np.float64), fg[idx].astype(np.float64), mg[idx].astype(np.float64)
 
 
+class _RecordedWorld:
+    """Owns a recorded batch world for as long as the autograd graph that
+    needs it: freed when the owner is collected (weakref.finalize), so a
+    second backward (retain_graph=True) finds it alive and a forward whose
+    output never gets a backward does not leak it."""
+
+    def __init__(self, L: Library, w):
+        import weakref
+
+        self.L, self.w = L, w
+        self._finalizer = weakref.finalize(self, L.m3d_world_destroy, w)
+
+    def close(self):
+        self._finalizer()
+
+
+def record_rollout(L: Library, scene: DiffScene, v0, friction, mass, steps: int, substeps: int) -> _RecordedWorld:
+    """Builds the scene's batch with these parameters and steps it `steps`
+    times with recording on. Raises if a step failed or the recording was
+    refused (a colored island or joints: the adjoint would not describe the
+    steps the engine ran)."""
+    owner = _RecordedWorld(L, scene.build(L, v0, friction, mass))
+    w = owner.w
+    if L.m3dx_diff_record(w, 1) != 0:
+        owner.close()
+        raise RuntimeError("m3dx_diff_record failed: " + L.last_error())
+    for _ in range(steps):
+        L.m3d_world_step(w, DT, substeps)
+    failures = L.m3dx_step_failures(w)
+    rc = L.m3dx_diff_record(w, 0)
+    if failures != 0 or rc != 0:
+        msg = L.last_error()
+        owner.close()
+        raise RuntimeError(f"recorded rollout failed ({failures} failed steps): {msg}")
+    return owner
+
+
 def rollout_function():
     """torch.autograd.Function: (v0, friction, mass) -> final positions."""
     torch = _torch()
@@ -188,24 +225,17 @@ def rollout_function():
     class XpbdRollout(torch.autograd.Function):
         @staticmethod
         def forward(ctx, v0, friction, mass, L, scene, steps, substeps):
-            w = scene.build(L, v0.detach().cpu().double().numpy(), friction.detach().cpu().double().numpy(),
-                            mass.detach().cpu().double().numpy())
-            if L.m3dx_diff_record(w, 1) != 0:
-                L.m3d_world_destroy(w)
-                raise RuntimeError(L.last_error())
-            for _ in range(steps):
-                L.m3d_world_step(w, DT, substeps)
-            L.m3dx_diff_record(w, 0)
-            out = final_positions(L, w, scene)
-            ctx.L, ctx.w, ctx.scene = L, w, scene
+            owner = record_rollout(L, scene, v0.detach().cpu().double().numpy(),
+                                   friction.detach().cpu().double().numpy(), mass.detach().cpu().double().numpy(),
+                                   steps, substeps)
+            out = final_positions(L, owner.w, scene)
+            ctx.owner, ctx.scene = owner, scene
             return torch.as_tensor(out, dtype=v0.dtype, device=v0.device)
 
         @staticmethod
         def backward(ctx, g):
-            try:
-                gv, gf, gm = backward(ctx.L, ctx.w, ctx.scene, g.detach().cpu().numpy())
-            finally:
-                ctx.L.m3d_world_destroy(ctx.w)
+            owner = ctx.owner
+            gv, gf, gm = backward(owner.L, owner.w, ctx.scene, g.detach().cpu().numpy())
             dev, dt = g.device, g.dtype
             return (torch.as_tensor(gv, dtype=dt, device=dev), torch.as_tensor(gf, dtype=dt, device=dev),
                     torch.as_tensor(gm, dtype=dt, device=dev), None, None, None, None)

@@ -263,6 +263,75 @@ def test_torch_function_matches_c_abi(dev):
         gv, gf, gm = diff.backward(dev, w, scene, g_pos)
         assert np.allclose(v0.grad.numpy(), gv, rtol=1e-5, atol=1e-7)
         assert np.allclose(mu.grad.numpy(), gf, rtol=1e-5, atol=1e-7)
+        # mass enters twice: through the trajectory (the adjoint's gm) and
+        # through the centre-of-mass weights, d com / d m_k = (p_k - com) / M
+        M = scene.mass.sum(1)[:, None]
+        g_com = 2.0 * (com - scene.target)
+        g_weights = np.einsum("wc,wkc->wk", g_com, engine - com[:, None, :]) / M
+        assert np.allclose(mass.grad.numpy(), gm + g_weights, rtol=1e-5, atol=1e-9)
+    finally:
+        dev.m3d_world_destroy(w)
+
+
+@pytest.mark.gpu
+def test_torch_function_second_backward_and_release(dev):
+    """The recorded world lives as long as the graph: a second backward
+    (retain_graph) gives the same gradients, and dropping the graph frees
+    the world."""
+    torch = pytest.importorskip("torch")
+    import gc
+
+    scene = diff.DiffScene(worlds=2)
+    v0 = torch.tensor(scene.v0, dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(scene.friction, dtype=torch.float64, requires_grad=True)
+    mass = torch.tensor(scene.mass, dtype=torch.float64, requires_grad=True)
+    pos = diff.rollout(dev, scene, v0, mu, mass, steps=3)
+    owner = pos.grad_fn.owner
+    loss = scene.loss(pos, mass)
+    loss.backward(retain_graph=True)
+    g1 = v0.grad.clone()
+    v0.grad = None
+    loss.backward()
+    assert torch.equal(g1, v0.grad)
+    assert owner._finalizer.alive
+    del pos, loss
+    gc.collect()
+    assert not owner._finalizer.alive
+
+
+@pytest.mark.gpu
+def test_recording_refuses_colored_islands(dev):
+    """A 15 x 15 raft of boxes on the ground (test/test_main.cpp:960-985) is
+    one island of well over 300 contacts, which the engine solves with the
+    colored schedule: recording refuses it instead of returning the gradient
+    of the serial schedule. The steps still run as usual."""
+    import ctypes as C
+
+    from melon3d_amd import scenes
+
+    wd = dev.m3d_world_def_default()
+    wd.enableSleep = False
+    w = dev.create_world(wd)
+    try:
+        scenes.add_ground(dev, w)
+        raft = [scenes.add_box(dev, w, (-7.0 + i, 0.5, -7.0 + k), (0.501, 0.5, 0.501)) for i in range(15) for k in range(15)]
+        assert dev.m3dx_diff_record(w, 1) == 0
+        for _ in range(5):
+            dev.m3d_world_step(w, 1.0 / 60.0, 4)
+        assert dev.stats(w).coloredIslands >= 1
+        assert dev.m3dx_step_failures(w) == 0
+        assert dev.m3dx_diff_steps(w) == 0
+        assert dev.m3dx_diff_record(w, 0) == -1
+        assert "colored schedule" in dev.last_error()
+        n = 226
+        seed = np.zeros(n * 13)
+        out = np.zeros(n * 13)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        assert dev.m3dx_diff_backward_f64(w, ptr(seed), ptr(out), None, None) == -1
+        assert "colored schedule" in dev.last_error()
+        # the engine kept stepping: the raft rests on the ground
+        ys = [dev.m3d_body_position(w, b).y for b in raft]
+        assert max(abs(y - 0.5) for y in ys) < 0.01
     finally:
         dev.m3d_world_destroy(w)
 
