@@ -80,3 +80,39 @@ def one_step_from(ref, dev, ref_world, dt=1.0 / 60.0, sub=4):
     b = dev.export_state(dw)
     ref.m3d_world_destroy(rw2)
     return a, b, dw
+
+
+# ---- vectorized views for full-size states (hundreds of thousands of rows)
+
+def np_bodies(snap):
+    """Structured numpy view of a snapshot's body slots."""
+    return np.ctypeslib.as_array(snap._keep["bodies"])[: snap.header.bodyCapacity]
+
+
+def np_contacts(snap):
+    """Structured numpy view of a snapshot's contacts."""
+    return np.ctypeslib.as_array(snap._keep["contacts"])[: snap.header.contactCount]
+
+
+def state_diff_rows(A, B):
+    """state_diff over two structured arrays of body slots."""
+    assert (A["inUse"] == B["inUse"]).all(), "in-use sets differ"
+    m = A["inUse"] != 0
+    if not m.any():
+        return {"pos": 0.0, "rot": 0.0, "vel": 0.0, "angvel": 0.0, "awake_equal": True, "identical": True}
+    f = lambda k: float(np.abs(A[k][m].astype(np.float64) - B[k][m].astype(np.float64)).max())
+    rot = float(rot_angle(A["rotation"][m].astype(np.float64), B["rotation"][m].astype(np.float64)).max())
+    same = all(np.array_equal(A[k][m], B[k][m]) for k in ("position", "rotation", "linearVelocity", "angularVelocity"))
+    return {"pos": f("position"), "rot": rot, "vel": f("linearVelocity"), "angvel": f("angularVelocity"),
+            "awake_equal": bool((A["isAwake"][m] == B["isAwake"][m]).all()), "identical": same}
+
+
+def state_diff_np(a, b):
+    """state_diff over numpy views of two snapshots."""
+    return state_diff_rows(np_bodies(a), np_bodies(b))
+
+
+def pairs_np(snap, offset=0):
+    """(contacts, 2) body pairs in contact order, body indices minus offset."""
+    c = np_contacts(snap)
+    return np.stack([c["bodyA"] - offset, c["bodyB"] - offset], axis=1)

@@ -286,17 +286,17 @@ def test_torch_function_second_backward_and_release(dev):
     mu = torch.tensor(scene.friction, dtype=torch.float64, requires_grad=True)
     mass = torch.tensor(scene.mass, dtype=torch.float64, requires_grad=True)
     pos = diff.rollout(dev, scene, v0, mu, mass, steps=3)
-    owner = pos.grad_fn.owner
+    release = pos.grad_fn.owner._finalizer  # holds no reference to the owner
     loss = scene.loss(pos, mass)
     loss.backward(retain_graph=True)
     g1 = v0.grad.clone()
     v0.grad = None
     loss.backward()
     assert torch.equal(g1, v0.grad)
-    assert owner._finalizer.alive
+    assert release.alive
     del pos, loss
     gc.collect()
-    assert not owner._finalizer.alive
+    assert not release.alive
 
 
 @pytest.mark.gpu

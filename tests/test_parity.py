@@ -74,7 +74,7 @@ def test_pyramid_lockstep_stage_views(ref, dev):
         pr, pd = ref.m3d_world_profile(rw), dev.m3d_world_profile(dw)
         for f in ("bodyCount", "awakeBodyCount", "contactCount", "touchingContactCount", "islandCount"):
             assert getattr(pr, f) == getattr(pd, f), f"step {step}: profile {f}"
-        assert d["pos"] <= 1e-3, f"step {step}: trajectories diverged {d}"
+        assert d["pos"] <= TOL_POS and d["rot"] <= TOL_ROT, f"step {step}: trajectories diverged {d}"
     ref.m3d_world_destroy(rw)
     dev.m3d_world_destroy(dw)
 
@@ -128,7 +128,7 @@ def test_giant_island_lockstep_colors(ref, dev):
         C_ = a.header.contactCount
         assert ref.contact_colors(rw, C_) == dev.contact_colors(dw, C_), f"step {step}: colors"
         d = state_diff(a, b)
-        assert d["pos"] <= 1e-4, f"step {step}: {d}"
+        assert d["pos"] <= TOL_POS and d["rot"] <= TOL_ROT, f"step {step}: {d}"
     ref.m3d_world_destroy(rw)
     dev.m3d_world_destroy(dw)
 
@@ -154,7 +154,7 @@ def test_destroy_after_idle_step(ref, dev):
         assert rb == db, f"step {step}: begin events"
         assert re == de, f"step {step}: end events"
         d = state_diff(a, b)
-        assert d["pos"] <= 1e-3, f"step {step}: {d}"
+        assert d["pos"] <= TOL_POS and d["rot"] <= TOL_ROT, f"step {step}: {d}"
     ref.m3d_world_destroy(rw)
     dev.m3d_world_destroy(dw)
 
@@ -177,6 +177,6 @@ def test_stream_churn_lockstep(ref, dev):
         assert rb == db, f"step {step}: begin events ref-only {sorted(set(rb) - set(db))} dev-only {sorted(set(db) - set(rb))}"
         assert re == de, f"step {step}: end events ref {re[:8]} dev {de[:8]} ref-only {sorted(set(re) - set(de))} dev-only {sorted(set(de) - set(re))}"
         d = state_diff(a, b)
-        assert d["pos"] <= 1e-3, f"step {step}: {d}"
+        assert d["pos"] <= TOL_POS and d["rot"] <= TOL_ROT, f"step {step}: {d}"
     ref.m3d_world_destroy(rw)
     dev.m3d_world_destroy(dw)

@@ -1,0 +1,200 @@
+"""Joints and kinematic bodies against the reference engine.
+
+The jointed scene is the reference's own determinism scene (a chain of six
+hinged boxes from a static pivot with twelve boxes raining on it,
+test/test_main.cpp:732-782) plus one joint of every other kind and a hinge
+with a limit and a motor, so contacts and joints share islands. The
+kinematic scene moves and spins kinematic bodies through resting and falling
+dynamic ones (src/world.cpp:2113-2123). Bar: one step from identical state
+within 1e-5 m / 1e-5 rad with contact order and events exact, and the two
+engines in lockstep from creation within the same bound.
+"""
+import ctypes as C
+
+import pytest
+
+from melon3d_amd import scenes
+from melon3d_amd.api import (M3D_BODY_KINEMATIC, M3D_SHAPE_BOX, M3D_SHAPE_CAPSULE, M3D_SHAPE_SPHERE,
+                             m3d_ball_joint_def, m3d_distance_joint_def, m3d_vec3, m3d_weld_joint_def)
+from parity import contact_pairs, events, one_step_from, state_diff
+
+TOL = 1e-5
+DT = 1.0 / 60.0
+
+
+def _ground(L, w):
+    return scenes.add_ground(L, w, half=(50.0, 0.5, 50.0), y=-0.5)
+
+
+def _static_box(L, w, pos, half):
+    bd = L.m3d_body_def_default()
+    bd.position = m3d_vec3(*pos)
+    sd = L.m3d_shape_def_default()
+    sd.type = M3D_SHAPE_BOX
+    sd.halfExtents = m3d_vec3(*half)
+    return L.m3d_body_create(w, C.byref(bd), C.byref(sd))
+
+
+def joint_scene(L, w):
+    """Hinge chain with rain (test/test_main.cpp:742-771), a ball-jointed
+    pendulum, a welded pair, a distance spring and a limited, motorized
+    hinge. Returns the joint ids."""
+    _ground(L, w)
+    joints = []
+    prev = _static_box(L, w, (0.0, 8.0, 0.0), (0.1, 0.1, 0.1))
+    for i in range(6):
+        link = scenes.add_box(L, w, (0.9 * (i + 1), 8.0, 0.0), (0.4, 0.15, 0.15))
+        hd = L.m3d_hinge_joint_def_default()
+        hd.bodyA, hd.bodyB = prev, link
+        hd.localAnchorA = m3d_vec3(0.0, 0.0, 0.0) if i == 0 else m3d_vec3(0.45, 0.0, 0.0)
+        hd.localAnchorB = m3d_vec3(-0.45, 0.0, 0.0)
+        hd.localAxisA = m3d_vec3(0.0, 0.0, 1.0)
+        joints.append(L.m3d_joint_create_hinge(w, C.byref(hd)))
+        prev = link
+    for i in range(12):
+        scenes.add_box(L, w, (0.6 * i - 2.0, 10.0 + 0.3 * i, 0.05 * i), (0.25, 0.25, 0.25))
+    # ball-jointed pendulum swinging into a box stack
+    pivot = _static_box(L, w, (-6.0, 5.0, 0.0), (0.1, 0.1, 0.1))
+    bob = scenes.add_box(L, w, (-8.0, 5.0, 0.0), (0.3, 0.3, 0.3))
+    bj = m3d_ball_joint_def()
+    bj.bodyA, bj.bodyB = pivot, bob
+    bj.localAnchorA = m3d_vec3(0.0, 0.0, 0.0)
+    bj.localAnchorB = m3d_vec3(2.0, 0.0, 0.0)
+    joints.append(L.m3d_joint_create_ball(w, C.byref(bj)))
+    for k in range(4):
+        scenes.add_box(L, w, (-6.0, 0.5 + 1.0 * k, 0.0), (0.5, 0.5, 0.5))
+    # welded pair falling onto the ground
+    a = scenes.add_box(L, w, (4.0, 3.0, 4.0), (0.5, 0.2, 0.2))
+    b = scenes.add_box(L, w, (4.9, 3.0, 4.0), (0.2, 0.5, 0.2))
+    wj = m3d_weld_joint_def()
+    wj.bodyA, wj.bodyB = a, b
+    wj.localAnchorA = m3d_vec3(0.45, 0.0, 0.0)
+    wj.localAnchorB = m3d_vec3(0.0, 0.0, 0.0)
+    joints.append(L.m3d_joint_create_weld(w, C.byref(wj)))
+    # soft distance spring between a sphere and a capsule
+    bd = L.m3d_body_def_default()
+    bd.type = 2
+    bd.position = m3d_vec3(-3.0, 4.0, 5.0)
+    sd = L.m3d_shape_def_default()
+    sd.type = M3D_SHAPE_SPHERE
+    sd.radius = 0.4
+    s1 = L.m3d_body_create(w, C.byref(bd), C.byref(sd))
+    bd.position = m3d_vec3(-1.5, 4.5, 5.0)
+    sd.type = M3D_SHAPE_CAPSULE
+    sd.radius = 0.2
+    sd.halfHeight = 0.5
+    s2 = L.m3d_body_create(w, C.byref(bd), C.byref(sd))
+    dj = m3d_distance_joint_def()
+    dj.bodyA, dj.bodyB = s1, s2
+    dj.localAnchorA = m3d_vec3(0.0, 0.0, 0.0)
+    dj.localAnchorB = m3d_vec3(0.0, 0.3, 0.0)
+    dj.length = 1.2
+    dj.hertz = 3.0
+    dj.dampingRatio = 0.4
+    joints.append(L.m3d_joint_create_distance(w, C.byref(dj)))
+    # limited, motorized hinge: a door on a static post
+    post = _static_box(L, w, (6.0, 1.5, -5.0), (0.1, 1.5, 0.1))
+    door = scenes.add_box(L, w, (6.7, 1.5, -5.0), (0.6, 1.0, 0.05))
+    hd = L.m3d_hinge_joint_def_default()
+    hd.bodyA, hd.bodyB = post, door
+    hd.localAnchorA = m3d_vec3(0.1, 0.0, 0.0)
+    hd.localAnchorB = m3d_vec3(-0.6, 0.0, 0.0)
+    hd.localAxisA = m3d_vec3(0.0, 1.0, 0.0)
+    hd.localAxisB = m3d_vec3(0.0, 1.0, 0.0)
+    hd.enableLimit = True
+    hd.lowerAngle = -0.5
+    hd.upperAngle = 1.2
+    hd.enableMotor = True
+    hd.motorSpeed = 2.0
+    hd.maxMotorTorque = 50.0
+    joints.append(L.m3d_joint_create_hinge(w, C.byref(hd)))
+    # a box thrown at the door
+    scenes.add_box(L, w, (7.0, 1.0, -2.0), (0.3, 0.3, 0.3), linearVelocity=m3d_vec3(0.0, 1.0, -6.0))
+    return joints
+
+
+def kinematic_scene(L, w):
+    """A kinematic platform sliding under resting boxes, a spinning
+    kinematic paddle sweeping through falling spheres and capsules, and a
+    kinematic box rising through a stack."""
+    _ground(L, w)
+    kin = []
+    for pos, half, v, om in (((0.0, 0.5, 0.0), (3.0, 0.25, 3.0), (1.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+                             ((-8.0, 1.0, 0.0), (2.5, 0.2, 0.3), (0.0, 0.0, 0.0), (0.0, 3.0, 0.0)),
+                             ((8.0, 0.3, 0.0), (0.5, 0.3, 0.5), (0.0, 0.6, 0.0), (0.0, 0.0, 0.0))):
+        bd = L.m3d_body_def_default()
+        bd.type = M3D_BODY_KINEMATIC
+        bd.position = m3d_vec3(*pos)
+        bd.linearVelocity = m3d_vec3(*v)
+        bd.angularVelocity = m3d_vec3(*om)
+        sd = L.m3d_shape_def_default()
+        sd.type = M3D_SHAPE_BOX
+        sd.halfExtents = m3d_vec3(*half)
+        kin.append(L.m3d_body_create(w, C.byref(bd), C.byref(sd)))
+    for i in range(3):
+        for k in range(3):
+            scenes.add_box(L, w, (-1.5 + 1.5 * i, 1.26, -1.5 + 1.5 * k), (0.5, 0.5, 0.5))
+    rng = scenes.Lcg(99)
+    for i in range(40):
+        x, z = rng.uniform(-10.0, -6.0), rng.uniform(-2.0, 2.0)
+        scenes.mixed_body(L, w, rng, i % 2, (x, 2.0 + 0.6 * i, z))
+    for k in range(4):
+        scenes.add_box(L, w, (8.0, 1.11 + 1.0 * k, 0.0), (0.45, 0.5, 0.45))
+    return kin
+
+
+SCENES = {"joints": joint_scene, "kinematic": kinematic_scene}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["joints", "kinematic"])
+@pytest.mark.parametrize("at", [0, 30, 90, 180])
+def test_one_step(ref, dev, scene, at):
+    rw = ref.create_world()
+    try:
+        SCENES[scene](ref, rw)
+        for _ in range(at):
+            ref.m3d_world_step(rw, DT, 4)
+        a, b, dw = one_step_from(ref, dev, rw)
+        try:
+            assert contact_pairs(a) == contact_pairs(b), "contact order"
+            d = state_diff(a, b)
+            assert d["pos"] <= TOL and d["rot"] <= TOL, d
+            assert d["awake_equal"]
+            assert a.header.jointCapacity == b.header.jointCapacity
+            for j in range(a.header.jointCapacity):
+                ra, rb = a.joints[j], b.joints[j]
+                assert (ra.inUse, ra.type, ra.bodyA, ra.bodyB) == (rb.inUse, rb.type, rb.bodyA, rb.bodyB), j
+        finally:
+            dev.m3d_world_destroy(dw)
+    finally:
+        ref.m3d_world_destroy(rw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["joints", "kinematic"])
+def test_lockstep(ref, dev, scene):
+    rw, dw = ref.create_world(), dev.create_world()
+    try:
+        jr = SCENES[scene](ref, rw)
+        jd = SCENES[scene](dev, dw)
+        for step in range(180):
+            ref.m3d_world_step(rw, DT, 4)
+            dev.m3d_world_step(dw, DT, 4)
+            a, b = ref.export_state(rw), dev.export_state(dw)
+            assert contact_pairs(a) == contact_pairs(b), f"step {step}: contact order"
+            assert events(ref, rw) == events(dev, dw), f"step {step}: events"
+            d = state_diff(a, b)
+            assert d["pos"] <= TOL and d["rot"] <= TOL, f"step {step}: {d}"
+            assert d["awake_equal"], f"step {step}: sleep state"
+            if scene == "joints":
+                angles_r = [ref.m3d_joint_hinge_angle(rw, j) for j in jr]
+                angles_d = [dev.m3d_joint_hinge_angle(dw, j) for j in jd]
+                assert max(abs(x - y) for x, y in zip(angles_r, angles_d)) <= TOL, f"step {step}: hinge angles"
+            else:
+                for k in jd:
+                    assert dev.m3d_body_get_type(dw, k) == M3D_BODY_KINEMATIC
+        assert dev.m3dx_step_failures(dw) == 0, dev.last_error()
+    finally:
+        ref.m3d_world_destroy(rw)
+        dev.m3d_world_destroy(dw)
