@@ -389,6 +389,22 @@ M3D_API int m3dx_diff_jvp_cpu(m3d_world* world, int first, int count, int primal
    Returns 0 or -1. */
 M3D_API int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m3d_vec3* translations,
                                       const uint32_t* maskBits, int32_t count, m3d_ray_result* out);
+/* m3dx_world_ray_cast_batch that allocates nothing (no counterpart in the
+   reference; for callers that cannot allocate device memory: code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Device staging lives in a fixed per-GPU array of the library (4 MiB,
+   held until the process ends), host staging in fixed chunks on the stack.
+   At most 65536 rays per call (more is an error). It never refreshes the
+   broadphase (a refresh may grow device buffers), so it refuses (-1) where
+   the reference's ray cast would refresh it with an effect: after any body
+   or API change since the last step or query, and after a step that left
+   the stable tier dirty until a refreshing query (m3d_world_ray_cast,
+   m3dx_world_ray_cast_batch) has run. Calls that overlap on one GPU share
+   the array and race. Same results as m3dx_world_ray_cast_batch. Returns 0
+   or -1. */
+M3D_API int m3dx_world_ray_cast_batch_noalloc(m3d_world* world, const m3d_vec3* origins,
+                                              const m3d_vec3* translations, const uint32_t* maskBits,
+                                              int32_t count, m3d_ray_result* out);
 /* Batched sphere casts on the device: out[k] = m3d_world_sphere_cast(world,
    origins[k], radii[k], translations[k], maskBits ? maskBits[k] :
    0xFFFFFFFF) (include/melon3d.h:515-518, src/world.cpp:1286-1392), with

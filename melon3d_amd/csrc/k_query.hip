@@ -10,6 +10,7 @@
 #include "mx_steps.h"
 
 #include <float.h>
+#include <mutex>
 #include <vector>
 
 namespace mx
@@ -334,6 +335,64 @@ __global__ void k_sphere_cast(RayCtx R, int32_t nBodies, const float4* castO, co
 
 using namespace mx;
 
+// ---- allocation-free ray casts ------------------------------------------
+
+// Staging of m3dx_world_ray_cast_batch_noalloc, one array per GPU for the
+// life of the process: rays k of a call use float4 slots k (origin, mask),
+// kNoAllocMaxRays + k (translation), 2 kNoAllocMaxRays + k and
+// 3 kNoAllocMaxRays + k (the two result words). Every slot a call reads is
+// written by the same call first (its rays' inputs by the host copies, its
+// results by the kernel), so nothing needs clearing between calls; calls
+// that overlap on one GPU share the array and race.
+constexpr int32_t kNoAllocMaxRays = 65536; // 4 MiB per GPU
+static __device__ __align__(256) float4 scratch[4 * kNoAllocMaxRays];
+
+// Device address of `scratch` on a device, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+static float4* scratch_on(int device)
+{
+	constexpr int kMaxDevices = 64;
+	static float4* addr[kMaxDevices];
+	static std::mutex lock;
+	if (device < 0 || device >= kMaxDevices)
+	{
+		set_error("ray cast batch (no allocation): device %d out of range", device);
+		return nullptr;
+	}
+	std::lock_guard<std::mutex> g(lock);
+	if (addr[device] == nullptr)
+	{
+		int prev = 0;
+		if (!hip_ok(hipGetDevice(&prev), "hipGetDevice") || !hip_ok(hipSetDevice(device), "hipSetDevice"))
+			return nullptr;
+		void* p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		(void)hipSetDevice(prev);
+		if (!hip_ok(e, "hipGetSymbolAddress(scratch)") || p == nullptr)
+			return nullptr;
+		addr[device] = (float4*)p;
+	}
+	return addr[device];
+}
+
+static RayCtx ray_ctx(World& w)
+{
+	RayCtx R;
+	R.B = make_body_view(w);
+	R.aKeys = w.activeKeys.p;
+	R.aBodies = w.activeBodies.p;
+	R.aCount = w.activeCount;
+	R.sKeys = w.stableKeys.p;
+	R.sBodies = w.stableBodies.p;
+	R.sCount = w.stableCount;
+	R.aOver = w.dActiveOversized.p;
+	R.nAOver = (int32_t)w.activeOversized.size();
+	R.sOver = w.dStableOversized.p;
+	R.nSOver = (int32_t)w.stableOversized.size();
+	R.cellSize = w.def.gridCellSize;
+	return R;
+}
+
 // host side of the batched queries: arrays up, kernel, results down
 template <class Launch>
 static int run_query(World& w, const std::vector<float4>& o, const std::vector<float4>& d, m3d_ray_result* out,
@@ -354,19 +413,7 @@ static int run_query(World& w, const std::vector<float4>& o, const std::vector<f
 	float4* dD = dO + count;
 	float4* dA = dD + count;
 	float4* dB = dA + count;
-	RayCtx R;
-	R.B = make_body_view(w);
-	R.aKeys = w.activeKeys.p;
-	R.aBodies = w.activeBodies.p;
-	R.aCount = w.activeCount;
-	R.sKeys = w.stableKeys.p;
-	R.sBodies = w.stableBodies.p;
-	R.sCount = w.stableCount;
-	R.aOver = w.dActiveOversized.p;
-	R.nAOver = (int32_t)w.activeOversized.size();
-	R.sOver = w.dStableOversized.p;
-	R.nSOver = (int32_t)w.stableOversized.size();
-	R.cellSize = w.def.gridCellSize;
+	RayCtx R = ray_ctx(w);
 	std::vector<float4> a((size_t)count), b((size_t)count);
 	bool ok = hip_ok(hipMemcpyAsync(dO, o.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries") &&
 			  hip_ok(hipMemcpyAsync(dD, d.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries");
@@ -396,6 +443,98 @@ static int run_query(World& w, const std::vector<float4>& o, const std::vector<f
 
 extern "C"
 {
+
+// m3dx_world_ray_cast_batch without any allocation: device staging in the
+// library's per-GPU `scratch` array (at most kNoAllocMaxRays rays per
+// call), host staging in fixed chunks on the stack. The world's broadphase
+// must be current (the grid as the last step left it, with no API change
+// since), since refreshing it may grow device buffers. Same results as
+// m3dx_world_ray_cast_batch. Returns 0 or -1.
+int m3dx_world_ray_cast_batch_noalloc(m3d_world* world, const m3d_vec3* origins, const m3d_vec3* translations,
+									  const uint32_t* maskBits, int32_t count, m3d_ray_result* out)
+{
+	if (world == nullptr || count < 0 || (count > 0 && (origins == nullptr || translations == nullptr || out == nullptr)))
+	{
+		set_error("ray cast batch (no allocation): null arrays");
+		return -1;
+	}
+	if (count > kNoAllocMaxRays)
+	{
+		set_error("ray cast batch (no allocation): %d rays exceed the maximum of %d per call", count, kNoAllocMaxRays);
+		return -1;
+	}
+	World& w = world->w;
+	if (w.subCount > 0)
+	{
+		set_error("ray cast batch (no allocation): batched worlds are not supported");
+		return -1;
+	}
+	// The reference refreshes the broadphase before a query while the grid
+	// or the stable tier is dirty (src/world.cpp:1090-1093). That refresh is
+	// not done here (it may grow device buffers). A dirty grid is refused.
+	// A dirty stable tier alone is served when the grid's last refresh was a
+	// query's (nothing moved since) and the stable tier is not due for a
+	// rebuild: the refresh would then leave both tiers as they are and only
+	// count one more refresh since the last stable build, which is done here.
+	bool stableDue = w.stableForce || w.stepsSinceStableBuild >= 4;
+	if (w.gridDirty || !w.dirtyList.empty() || (w.stableDirty && (!w.queryRefreshed || stableDue)))
+	{
+		set_error("ray cast batch (no allocation): the broadphase is not current (a query that may refresh it, such "
+				  "as m3dx_world_ray_cast_batch, has to run first)");
+		return -1;
+	}
+	if (count == 0)
+		return 0;
+	float4* sc = scratch_on(w.device);
+	if (sc == nullptr)
+		return -1;
+	if (w.stableDirty)
+		++w.stepsSinceStableBuild;
+	float4* dO = sc;
+	float4* dD = sc + kNoAllocMaxRays;
+	float4* dA = sc + 2 * kNoAllocMaxRays;
+	float4* dB = sc + 3 * kNoAllocMaxRays;
+	hipStream_t s = w.stream;
+	constexpr int32_t kChunk = 1024;
+	float4 hO[kChunk], hD[kChunk];
+	for (int32_t c0 = 0; c0 < count; c0 += kChunk)
+	{
+		int32_t m = std::min(kChunk, count - c0);
+		for (int32_t k = 0; k < m; ++k)
+		{
+			const m3d_vec3 o = origins[c0 + k], d = translations[c0 + k];
+			hO[k] = make_float4(o.x, o.y, o.z, u2f(maskBits != nullptr ? maskBits[c0 + k] : 0xFFFFFFFFu));
+			hD[k] = make_float4(d.x, d.y, d.z, 0.0f);
+		}
+		// synchronous copies: the chunk buffers are reused by the next chunk
+		if (!hip_ok(hipMemcpyAsync(dO + c0, hO, m * sizeof(float4), hipMemcpyHostToDevice, s), "rays") ||
+			!hip_ok(hipMemcpyAsync(dD + c0, hD, m * sizeof(float4), hipMemcpyHostToDevice, s), "rays") ||
+			!hip_ok(hipStreamSynchronize(s), "rays"))
+			return -1;
+	}
+	k_ray_cast<<<(count + 127) / 128, 128, 0, s>>>(ray_ctx(w), dO, dD, count, dA, dB);
+	if (!hip_ok(hipGetLastError(), "ray cast"))
+		return -1;
+	for (int32_t c0 = 0; c0 < count; c0 += kChunk)
+	{
+		int32_t m = std::min(kChunk, count - c0);
+		if (!hip_ok(hipMemcpyAsync(hO, dA + c0, m * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") ||
+			!hip_ok(hipMemcpyAsync(hD, dB + c0, m * sizeof(float4), hipMemcpyDeviceToHost, s), "hits") ||
+			!hip_ok(hipStreamSynchronize(s), "hits"))
+			return -1;
+		for (int32_t k = 0; k < m; ++k)
+		{
+			m3d_ray_result& r = out[c0 + k];
+			int32_t bi = f2i(hD[k].w);
+			r.hit = bi >= 0;
+			r.body = bi >= 0 ? m3d_body_id{ bi, w.bodies[bi].generation } : m3d_body_id_null();
+			r.point = m3d_v3(hO[k].x, hO[k].y, hO[k].z);
+			r.fraction = hO[k].w;
+			r.normal = m3d_v3(hD[k].x, hD[k].y, hD[k].z);
+		}
+	}
+	return 0;
+}
 
 // Batched ray casts on the device: out[k] is m3d_world_ray_cast_filtered(
 // world, origins[k], translations[k], maskBits ? maskBits[k] : all) for

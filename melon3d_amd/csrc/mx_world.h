@@ -158,6 +158,7 @@ struct World
 
 	// ---- sync state ----
 	bool hostStale = false;          // device bodies newer than the host mirror
+	bool queryRefreshed = false;     // the grid's last refresh was a query's (dt = 0), after the last step moved bodies
 	std::vector<int32_t> dirtyList;  // bodies to upload at the next step
 	std::vector<uint8_t> dirtyFlag;
 	bool jointsDirty = true;         // joint table to upload

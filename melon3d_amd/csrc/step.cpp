@@ -117,6 +117,7 @@ bool refresh_broadphase(World& w, float dt)
 			launch_stable_migrate(w);
 		}
 		w.gridDirty = false;
+		w.queryRefreshed = dt == 0.0f;
 		return true;
 	}
 	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
@@ -136,6 +137,7 @@ bool refresh_broadphase(World& w, float dt)
 		++w.stepsSinceStableBuild;
 	}
 	w.gridDirty = false;
+	w.queryRefreshed = dt == 0.0f;
 	return true;
 }
 

@@ -562,6 +562,11 @@ int m3dx_world_ray_cast_batch(m3d_world*, const m3d_vec3*, const m3d_vec3*, cons
 {
     return -1;
 }
+int m3dx_world_ray_cast_batch_noalloc(m3d_world*, const m3d_vec3*, const m3d_vec3*, const uint32_t*, int32_t,
+                                      m3d_ray_result*)
+{
+    return -1;
+}
 int m3dx_world_sphere_cast_batch(m3d_world*, const m3d_vec3*, const float*, const m3d_vec3*, const uint32_t*, int32_t,
                                  m3d_ray_result*)
 {
