@@ -15,19 +15,29 @@ W untimed warmup steps, then K timed steps. State is resident on the device
 when timing starts; m3d_world_step returns after the device finished the
 step, so the wall clock around the K steps is the step time.
 
-roofline: the dominant timed kernel (largest summed HIP-event time on the
-library's stream, over --timing-steps extra steps after the timed region);
-`traffic` is that kernel's measured HBM bytes per launch (rocprofv3 PMC, from
-the committed profile summary of this workload):
-algorithmic bytes per launch (from the step counters: normals pass 160 B
-per contact + 68 B per point, velocity pass 472 B per contact + 112 B per
-point, narrowphase 640 B per regenerated manifold + 120 B per cached contact)
-/ average launch duration.
+roofline: the dominant kernel of the timed window: a device world imported
+from the state the timed region started from is stepped again over the same
+K steps, with the same churn, with HIP events around every timed kernel
+(solver passes, prepare, body integration, ordering, narrowphase, pairs and
+every radix sort); the kernel with the largest summed time is the dominant
+one. achieved = its algorithmic bytes per launch (the library's byte models,
+from the step counters) / its average launch duration. `traffic` is its
+measured HBM bytes per launch (rocprofv3 PMC FETCH_SIZE and WRITE_SIZE,
+scaled by a copy-kernel calibration) from the committed profile of this
+workload and window, or null when none matches.
 
 cpu_baseline: the reference engine built from its own sources with its
 release flags (-O3 -mavx2 -mfma), workerCount = this host's cores, stepping
 a bounded sample from the exact state the GPU had when its timed region
-started (same churn sequence).
+started (same churn sequence); cpu_baseline_1thread the same with
+workerCount = 1 (bench/bench.cpp:266-273 reports both).
+
+The stream line also carries the two other parts of the metric:
+`batch` (4096 worlds of the batched workload: body-steps/s over steps
+41-240, the same worlds looped as plain device worlds, and the reference
+on a 256-world sample, one world per host thread) and `diff` (the
+differentiable batch: forward and backward times, backward / forward, and
+gradient errors against FP64 central differences and FP64 CPU tangents).
 
 Prints one JSON line.
 """
@@ -61,11 +71,14 @@ def traffic_bytes(st, substeps: int) -> float:
     return float(S * per_sub + once)
 
 
-def measured_traffic(scene: str, size: int, kernel: str):
+def measured_traffic(scene: str, size: int, kernel: str, window=None):
     """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) on this
-    workload, from the committed profile summary; None when not measured."""
-    files = {("stream", 200000): "stream200k_grid_normals.md", ("batch", 4096): "batch4096_kernels.md",
+    workload, from the committed profile summary: lines
+    `traffic_bytes_per_launch <kernel> <bytes> <warmup> <steps>`, the profiled
+    run's window, which must be this run's `window` when one is given. None
+    when not measured."""
+    files = {("stream", 200000): "stream200k_window.md", ("batch", 4096): "batch4096_kernels.md",
              ("diff", 256): "diff256_adjoint.md"}
     if (scene, size) not in files:
         return None
@@ -74,7 +87,9 @@ def measured_traffic(scene: str, size: int, kernel: str):
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    if window is not None and (len(parts) < 5 or (int(parts[3]), int(parts[4])) != tuple(window)):
+                        continue
                     return float(parts[2])
     except OSError:
         return None
@@ -96,7 +111,7 @@ def build_scene(lib, w, scene: str, bodies: int):
     raise SystemExit(f"unknown scene {scene}")
 
 
-def batch_cpu_baseline(worlds: int, warmup: int, steps_max: int, budget_s: float, substeps: int):
+def batch_cpu_baseline(worlds: int, warmup: int, steps_max: int, budget_s: float, substeps: int, threads: int):
     """Reference engine on a sample of the batched workload: worlds 0 ..
     worlds-1, each its own world with workerCount = 1, spread over a pool of
     host threads (one world per task), advanced `warmup` untimed steps, then
@@ -109,7 +124,7 @@ def batch_cpu_baseline(worlds: int, warmup: int, steps_max: int, budget_s: float
     from melon3d_amd import api, scenes
 
     ref = api.load_library(ref_path)
-    cores = host_cores()
+    cores = threads
     wd = ref.m3d_world_def_default()
     wd.workerCount = 1
     ws, dyn = [], 0
@@ -328,7 +343,7 @@ def run_diff(args, dev):
         "kernel_ms_backward": {n: v[1] for n, v in kt.items() if v[0] > 0},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_batch(args, dev):
@@ -359,18 +374,16 @@ def run_batch(args, dev):
     kt = dev.kernel_timing(w)
     dev.m3dx_set_kernel_timing(w, 0)
     dev.m3d_world_destroy(w)
-    name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[2] > 0), key=lambda x: x[1][1],
-                                        default=("none", (0, 0.0, 0.0)))
-    avg_s = (kms / 1000.0) / launches if launches else 0.0
-    achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
+    roof = roofline_of(kt, "batch", args.worlds, None)
     loop = None
     if not args.no_loop:
         print("# looping plain worlds ...", file=sys.stderr, flush=True)
-        loop = batch_loop_plain(dev, args.loop_worlds, args.warmup, 2, args.substeps)
+        loop = batch_loop_plain(dev, args.loop_worlds, args.loop_warmup, args.loop_steps, args.substeps)
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = batch_cpu_baseline(args.cpu_worlds, args.warmup, args.cpu_steps, args.cpu_budget, args.substeps)
+        cpu = batch_cpu_baseline(args.cpu_worlds, args.warmup, args.cpu_steps, args.cpu_budget, args.substeps,
+                                 threads=host_cores())
     out = {
         "metric": "body-steps/sec",
         "value": value,
@@ -391,11 +404,9 @@ def run_batch(args, dev):
                    "contacts": prof.contactCount, "islands": prof.islandCount, "awake": prof.awakeBodyCount,
                    "pass_workgroups": pass_blocks, "graph_steps": graph_steps,
                    "parallelism": "single GPU, one launch sequence for all worlds"},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch", args.worlds, name),
-                     "launches": launches, "avg_launch_us": avg_s * 1e6,
-                     "bytes_per_launch": (kbytes / launches) if launches else 0.0},
-        "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
+        "roofline": roof,
+        "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items() if v[0] > 0},
+        "kernel_timing": f"HIP events over {args.timing_steps} steps after the timed region",
         "loop_plain_gpu": loop,
         "cpu_baseline": cpu,
     }
@@ -403,7 +414,7 @@ def run_batch(args, dev):
         out["batched_over_loop"] = value / loop["value"]
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def check_steps(dev, w):
@@ -453,16 +464,18 @@ def ray_bench(dev, w, n: int, arena: float):
     return n / dt, hits * 97 / n
 
 
-def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, dyn: int, rays: int = 0,
-                 arena: float = 200.0):
+def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, dyn: int, threads: int,
+                 rays: int = 0, arena: float = 200.0):
+    """The reference's release build (-O3 -mavx2 -mfma) stepping the GPU's
+    timed-region start state with the same churn, workerCount = threads, on
+    a bounded sample (steps until the budget is spent)."""
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libm3dref_fast.so")
     if not os.path.exists(ref_path):
         return None
     from melon3d_amd import api
 
     ref = api.load_library(ref_path)
-    cores = host_cores()
-    snap.header.workerCount = cores
+    snap.header.workerCount = threads
     rw = ref.import_state(snap)
     rs = stream.clone_onto(ref, rw) if stream is not None else None
     t0 = time.perf_counter()
@@ -485,26 +498,71 @@ def cpu_baseline(snap, stream, steps_max: int, budget_s: float, substeps: int, d
             ref.m3d_world_ray_cast(rw, m3d_vec3(*map(float, o[j])), m3d_vec3(*map(float, d[j])))
         ray_rate = rays / (time.perf_counter() - t1)
     ref.m3d_world_destroy(rw)
-    return {"value": dyn * k / dt, "unit": "body-steps/s", "cores": cores, "kind": "reference",
-            "sample": f"{k} steps ({'with the same churn, ' if rs else ''}from the GPU's timed-region start state), "
-                      f"reference built -O3 -mavx2 -mfma, workerCount={cores}", "ray_cast_rays_per_s": ray_rate}
+    out = {"value": dyn * k / dt, "unit": "body-steps/s", "cores": threads, "kind": "reference",
+           "sample": f"{k} steps ({'with the same churn, ' if rs else ''}from the GPU's timed-region start state), "
+                     f"reference -O3 -mavx2 -mfma, workerCount={threads}"}
+    if rays:
+        out["ray_cast_rays_per_s"] = ray_rate
+    return out
+
+
+def kernel_window(dev, snap, stream, steps: int, substeps: int):
+    """Per-kernel HIP-event times and algorithmic bytes of exactly the timed
+    window: a device world imported from the state the timed region started
+    from, stepped again with the same churn with kernel timing on (the events
+    cost host time, so they stay out of the timed region; the replay is the
+    same computation, the engine being deterministic)."""
+    w = dev.import_state(snap)
+    rs = stream.clone_onto(dev, w) if stream is not None else None
+    dev.m3dx_set_kernel_timing(w, 1)
+    for _ in range(steps):
+        if rs is not None:
+            rs.churn_step()
+        dev.m3d_world_step(w, DT, substeps)
+    check_steps(dev, w)
+    kt = dev.kernel_timing(w)
+    dev.m3dx_set_kernel_timing(w, 0)
+    dev.m3d_world_destroy(w)
+    return kt
+
+
+def roofline_of(kt, scene: str, size: int, window):
+    """The dominant kernel (largest summed event time over all timed kernels)
+    with its achieved algorithmic bandwidth and, when a committed profile of
+    this workload and window exists, its measured HBM traffic per launch."""
+    name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[0] > 0), key=lambda x: x[1][1],
+                                        default=("none", (0, 0.0, 0.0)))
+    avg_s = (kms / 1000.0) / launches if launches else 0.0
+    per_launch = kbytes / launches if launches else 0.0
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(scene, size, name, window),
+            "launches": launches, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": per_launch}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scene", default="stream", choices=["stream", "rain", "pyramid", "batch", "diff"])
     ap.add_argument("--worlds", type=int, default=4096, help="batch: number of worlds")
-    ap.add_argument("--loop-worlds", type=int, default=64, help="batch: worlds stepped one by one for comparison")
+    ap.add_argument("--batch-warmup", type=int, default=40)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--loop-worlds", type=int, default=256, help="batch: worlds stepped one by one for comparison")
+    ap.add_argument("--loop-warmup", type=int, default=10)
+    ap.add_argument("--loop-steps", type=int, default=5)
     ap.add_argument("--cpu-worlds", type=int, default=256, help="batch: worlds in the CPU baseline sample")
+    ap.add_argument("--diff-worlds", type=int, default=256)
+    ap.add_argument("--diff-steps", type=int, default=120)
     ap.add_argument("--no-loop", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="stream line without the batch sub-object")
+    ap.add_argument("--no-diff", action="store_true", help="stream line without the diff sub-object")
     ap.add_argument("--bodies", type=int, default=None)
     ap.add_argument("--substeps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -527,15 +585,29 @@ def main():
 
     dev = api.load()
     if args.scene == "batch":
-        run_batch(args, dev)
+        args.warmup, args.steps = args.batch_warmup, args.batch_steps
+        print(json.dumps(run_batch(args, dev)), flush=True)
         return
     if args.scene == "diff":
-        if args.worlds == 4096:
-            args.worlds = 256
-        if args.steps == 200:
-            args.steps = 120
-        run_diff(args, dev)
+        args.worlds, args.steps = args.diff_worlds, args.diff_steps
+        print(json.dumps(run_diff(args, dev)), flush=True)
         return
+    out = run_single(args, dev)
+    if args.scene == "stream" and not args.no_batch:
+        print("# batch ...", file=sys.stderr, flush=True)
+        b = argparse.Namespace(**vars(args))
+        b.warmup, b.steps = args.batch_warmup, args.batch_steps
+        out["batch"] = run_batch(b, dev)
+    if args.scene == "stream" and not args.no_diff:
+        print("# differentiable batch ...", file=sys.stderr, flush=True)
+        d = argparse.Namespace(**vars(args))
+        d.worlds, d.steps = args.diff_worlds, args.diff_steps
+        out["diff"] = run_diff(d, dev)
+    print(json.dumps(out), flush=True)
+
+
+def run_single(args, dev):
+    """One world (the stream by default): W untimed steps, K timed steps."""
     if args.bodies is None:
         args.bodies = {"stream": 200000, "rain": 20000, "pyramid": 210}[args.scene]
     w = dev.create_world()
@@ -548,8 +620,9 @@ def main():
         dev.m3d_world_step(w, DT, args.substeps)
         if (i + 1) % 50 == 0:
             print(f"# warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    snap = None if args.no_cpu else dev.export_state(w)
-    cpu_stream = stream.clone_onto(dev, w) if (stream and snap is not None) else None  # churn state at t0
+    # the state the timed region starts from, and the churn's state there
+    snap = dev.export_state(w)
+    stream0 = stream.clone_onto(dev, w) if stream else None
 
     model_bytes = 0.0
     churn_s = 0.0
@@ -573,28 +646,17 @@ def main():
     ms = 1000.0 * elapsed / args.steps
     value = dyn * args.steps / elapsed
 
-    # kernel timing in extra steps after the timed region (events add host
-    # work, so they stay out of the timed steps)
-    dev.m3dx_set_kernel_timing(w, 1)
-    for i in range(args.timing_steps):
-        if stream:
-            stream.churn_step()
-        dev.m3d_world_step(w, DT, args.substeps)
-    kt = dev.kernel_timing(w)
-    dev.m3dx_set_kernel_timing(w, 0)
+    print("# kernel timing of the timed window ...", file=sys.stderr, flush=True)
+    kt = kernel_window(dev, snap, stream0, args.steps, args.substeps)
+    roof = roofline_of(kt, args.scene, args.bodies, (args.warmup, args.steps))
 
-    # dominant timed kernel with a byte model
-    name, (launches, kms, kbytes) = max(((n, v) for n, v in kt.items() if v[2] > 0), key=lambda x: x[1][1],
-                                        default=("none", (0, 0.0, 0.0)))
-    avg_s = (kms / 1000.0) / launches if launches else 0.0
-    achieved = (kbytes / launches) / avg_s / 1e9 if launches and avg_s > 0 else 0.0
-    traffic = measured_traffic(args.scene, args.bodies, name)
-
-    cpu = None
-    if snap is not None:
+    cpu = cpu1 = None
+    if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(snap, cpu_stream, args.cpu_steps, args.cpu_budget, args.substeps, dyn, rays=2000,
-                           arena=200.0 if args.scene == "stream" else 40.0)
+        arena = 200.0 if args.scene == "stream" else 40.0
+        cpu = cpu_baseline(snap, stream0, args.cpu_steps, args.cpu_budget, args.substeps, dyn, host_cores(),
+                           rays=2000, arena=arena)
+        cpu1 = cpu_baseline(snap, stream0, args.cpu_steps, args.cpu_budget, args.substeps, dyn, 1)
     rays = None
     if args.scene in ("stream", "rain"):
         rate, frac = ray_bench(dev, w, 1 << 20, 200.0 if args.scene == "stream" else 40.0)
@@ -603,7 +665,7 @@ def main():
                 "query": "m3dx_world_ray_cast_batch (host arrays in and out) vs the reference's m3d_world_ray_cast "
                          "called once per ray through ctypes (2000 rays) on the same state; segments of 60 m over "
                          "the arena"}
-
+    kernel_total = sum(v[1] for v in kt.values()) / args.steps
     out = {
         "metric": "body-steps/sec",
         "value": value,
@@ -623,24 +685,25 @@ def main():
                    "bodies": dyn, "contacts": prof.contactCount, "touching": prof.touchingContactCount,
                    "islands": prof.islandCount, "awake": prof.awakeBodyCount, "giant_sweeps": st.sweeps,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "launches": launches, "avg_launch_us": avg_s * 1e6,
-                     "bytes_per_launch": (kbytes / launches) if launches else 0.0},
+        "roofline": roof,
         "step_traffic_model": {"bytes_per_step": model_bytes / args.steps,
                                "achieved_GBps": model_bytes / elapsed / 1e9},
-        "kernel_ms_per_step": {n: v[1] / max(args.timing_steps, 1) for n, v in kt.items()},
+        "kernel_ms_per_step": {n: v[1] / args.steps for n, v in kt.items() if v[0] > 0},
+        "timed_kernels_ms_per_step": kernel_total,
         "host_split_ms_per_step": {"churn_api_calls": 1000.0 * churn_s / args.steps,
                                    "step_broadphase": prof_ms[0] / args.steps,
                                    "step_narrowphase": prof_ms[1] / args.steps,
                                    "step_solver": prof_ms[2] / args.steps},
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "ray_cast": rays,
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    if cpu1:
+        out["gpu_over_cpu_1thread"] = value / cpu1["value"]
     dev.m3d_world_destroy(w)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -247,6 +247,9 @@ typedef enum m3dx_kernel
 	M3DX_K_PAIRS,             /* pair counting and emission */
 	M3DX_K_ADJOINT,           /* differentiable worlds: a step's forward replay and reverse */
 	M3DX_K_ADJ_FORWARD,       /* differentiable worlds: forward replays inside a checkpoint segment */
+	M3DX_K_GIANT_PREPARE,     /* solver: per-substep contact preparation */
+	M3DX_K_GIANT_BODIES,      /* solver: body integration and velocity reconstruction */
+	M3DX_K_SORT,              /* radix sorts of every stage (histogram, scan and scatter passes) */
 	M3DX_K_COUNT
 } m3dx_kernel;
 

@@ -267,7 +267,7 @@ class m3dx_kernel_time(C.Structure):
 
 
 M3DX_KERNELS = ["grid_normals", "grid_friction", "grid_velocity", "grid_order", "joint_islands", "narrow", "pairs",
-                "adjoint", "adjoint_forward"]
+                "adjoint", "adjoint_forward", "grid_prepare", "grid_bodies", "sort"]
 
 W = C.c_void_p  # m3d_world*
 overlap_fn = C.CFUNCTYPE(C.c_bool, m3d_body_id, C.c_void_p)

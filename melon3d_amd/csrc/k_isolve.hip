@@ -1220,7 +1220,7 @@ __global__ void k_giant_prepare(GiantSolveCtx S)
 __global__ void k_giant_rec_forms(GiantSolveCtx S)
 {
 	const int32_t nValid = (int32_t)S.slotStart[S.counters[CT_SLOTS]];
-	uint32_t entries = 0, pointSum = 0;
+	uint32_t entries = 0, pointSum = 0, bb = 0, bbPoints = 0;
 	for (int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); e < nValid; e += (int32_t)(gridDim.x * blockDim.x))
 	{
 		int2 pk = S.packet[e];
@@ -1240,8 +1240,14 @@ __global__ void k_giant_rec_forms(GiantSolveCtx S)
 		S.rec[at].w = w | mp | ((points < 4 ? points : 4) << kRecCountShift);
 		entries += 1u;
 		pointSum += (uint32_t)points;
+		if (w & kRecBoxBox)
+		{
+			bb += 1u;
+			bbPoints += (uint32_t)points;
+		}
 	}
-	block_add<2>(S.counters2, { CT_GIANT_ENTRIES, CT_GIANT_POINTS }, { entries, pointSum });
+	block_add<4>(S.counters2, { CT_GIANT_ENTRIES, CT_GIANT_POINTS, CT_GIANT_BB, CT_GIANT_BB_POINTS },
+				 { entries, pointSum, bb, bbPoints });
 }
 
 // lambdaN of the normal pass back from the schedule-ordered copy to the
@@ -1436,8 +1442,10 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		// no contacts: bodies only (integrate, reconstruct)
 		for (int sub = 0; sub < substeps; ++sub)
 		{
+			w.mark_begin(M3DX_K_GIANT_BODIES);
 			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
 			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
+			w.mark_end();
 		}
 		return true;
 	}
@@ -1557,8 +1565,12 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	auto substepLoop = [&]() -> bool {
 	for (int sub = 0; sub < substeps; ++sub)
 	{
+		w.mark_begin(M3DX_K_GIANT_BODIES);
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
+		w.mark_end();
+		w.mark_begin(M3DX_K_GIANT_PREPARE);
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
+		w.mark_end();
 		if (!diff_record_substep(w, sub))
 			return false;
 		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
@@ -1571,7 +1583,9 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		w.mark_end();
 		if (!ok)
 			return false;
+		w.mark_begin(M3DX_K_GIANT_BODIES);
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
+		w.mark_end();
 		w.mark_begin(M3DX_K_GIANT_VELOCITY);
 		ok = coop((const void*)k_giant_pass<VelocityPass>, 1);
 		w.mark_end();

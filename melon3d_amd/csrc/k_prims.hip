@@ -298,6 +298,10 @@ int64_t sort_status_words(int64_t n)
 	return scan_status_words(nb * kRadix);
 }
 
+static thread_local const SortTimer* t_sortTimer = nullptr;
+
+void set_sort_timer(const SortTimer* t) { t_sortTimer = t; }
+
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
 					  void* scratch, uint64_t* status, hipStream_t s)
 {
@@ -309,6 +313,11 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 	int64_t h = (int64_t)nb * kRadix;
 	uint32_t* hist = (uint32_t*)scratch;
 	int passes = (bits + kRadixBits - 1) / kRadixBits;
+	// algorithmic bytes per pass: the keys read for the histogram (8 B), then
+	// every (key, value) pair read and written once (24 B)
+	const SortTimer* timer = t_sortTimer;
+	if (timer != nullptr)
+		timer->begin(timer->ctx, 32.0 * (double)n * passes);
 	uint64_t* kin = keys;
 	uint32_t* vin = vals;
 	uint64_t* kout = keysAlt;
@@ -331,6 +340,8 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 		hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
 		hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
 	}
+	if (timer != nullptr)
+		timer->end(timer->ctx);
 }
 
 } // namespace mx

@@ -26,4 +26,15 @@ int64_t sort_status_words(int64_t n);
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
 					  void* scratch, uint64_t* status, hipStream_t s);
 
+// Kernel timing of the sorts issued by one thread: while a timer is set,
+// radix_sort_pairs calls begin(ctx, bytes) before its passes, with the
+// sort's algorithmic bytes, and end(ctx) after them.
+struct SortTimer
+{
+	void* ctx;
+	void (*begin)(void* ctx, double bytes);
+	void (*end)(void* ctx);
+};
+void set_sort_timer(const SortTimer* t);
+
 } // namespace mx

@@ -52,6 +52,8 @@ enum Counter : int
 	CT_KEPT,          // batched: stable entries kept from the last build
 	CT_PLACED,        // batched: contacts after placement
 	CT_DIFF_CONTACTS, // recorded solver contacts
+	CT_GIANT_BB,      // box-box entries, summed over substeps
+	CT_GIANT_BB_POINTS, // their points, summed over substeps
 	CT_COUNT_
 };
 

@@ -147,7 +147,13 @@ void world_step(World& w, float dt, int substeps)
 {
 	if (dt <= 0.0f)
 		return;
-	if (!world_step_stages(w, dt, substeps < 1 ? 1 : substeps))
+	// sorts are timed (M3DX_K_SORT) while the world's kernel timing is on
+	SortTimer timer = { &w, [](void* c, double bytes) { static_cast<World*>(c)->mark_begin(M3DX_K_SORT, bytes); },
+						[](void* c) { static_cast<World*>(c)->mark_end(); } };
+	set_sort_timer(w.timing ? &timer : nullptr);
+	bool ok = world_step_stages(w, dt, substeps < 1 ? 1 : substeps);
+	set_sort_timer(nullptr);
+	if (!ok)
 		++w.stepFailures;
 }
 
@@ -184,17 +190,37 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	w.tuner_step_end((int32_t)hc[CT_SLOTS]);
 	if (w.timing)
 	{
-		// algorithmic bytes of the timed kernels this step (each field read or
-		// written once per pass):
-		// normals per contact and pass 160 B + 68 B per point; velocity per
-		// contact 472 B + 112 B per point; narrowphase 640 B per regenerated
-		// manifold, 120 B per cached contact
+		// algorithmic bytes of the timed kernels this step, each live field
+		// read or written once per pass (entries and points summed over the
+		// substeps):
+		// - normals, per contact and iteration 160 B + 68 B per point;
+		// - stiction, 16 B per record read + per box-box contact 192 B (two
+		//   body poses read and written) + 72 B per box-box point;
+		// - velocity, per contact 472 B + 112 B per point;
+		// - prepare, per contact 388 B (record, bodies, material, normal,
+		//   pose cache, both bodies' pose, shape, inertia) + 272 B per point
+		//   (anchors, slip and arms read; rows written to the contact and
+		//   the schedule-ordered copy) + 64 B per box-box point (friction
+		//   anchors copied);
+		// - bodies, 184 B per integrated body per substep (integration and
+		//   reconstruction together);
+		// - ordering, 48 B per solver contact and fixpoint round;
+		// - pairs, 32 B per grid entry + 100 B per candidate pair;
+		// - narrowphase, 640 B per regenerated manifold, 120 B per cached
+		//   contact;
+		// - sorts, counted per launch (32 B per element and pass).
 		double ge = (double)hc[CT_GIANT_ENTRIES], gp = (double)hc[CT_GIANT_POINTS];
+		double gbb = (double)hc[CT_GIANT_BB], gbbp = (double)hc[CT_GIANT_BB_POINTS];
 		int iters = w.def.positionIterations > 0 ? w.def.positionIterations : 2;
 		double regen = (double)hc[CT_REGEN];
 		double stepBytes[M3DX_K_COUNT] = {};
 		stepBytes[M3DX_K_GIANT_NORMALS] = iters * (160.0 * ge + 68.0 * gp);
+		stepBytes[M3DX_K_GIANT_FRICTION] = 16.0 * ge + 192.0 * gbb + 72.0 * gbbp;
 		stepBytes[M3DX_K_GIANT_VELOCITY] = 472.0 * ge + 112.0 * gp;
+		stepBytes[M3DX_K_GIANT_PREPARE] = 388.0 * ge + 272.0 * gp + 64.0 * gbbp;
+		stepBytes[M3DX_K_GIANT_BODIES] = 184.0 * substeps * (double)hc[CT_INTEGRATED];
+		stepBytes[M3DX_K_GIANT_ORDER] = 48.0 * (double)w.stats.solvedContacts * (double)hc[CT_COLOR_ROUNDS];
+		stepBytes[M3DX_K_PAIRS] = 32.0 * (double)(w.activeCount + w.stableCount) + 100.0 * (double)hc[CT_CANDIDATES];
 		stepBytes[M3DX_K_NARROW] = 640.0 * regen + 120.0 * std::max(0.0, (double)w.contactCount - regen);
 		w.collect_marks();
 		for (int k = 0; k < M3DX_K_COUNT; ++k)
