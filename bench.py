@@ -106,7 +106,10 @@ def build_scene(lib, w, scene: str, bodies: int):
         ids = scenes.pyramid(lib, w, base=20)
         return None, len(ids)
     if scene == "stream":
-        st = scenes.NativeStream(lib, w, count=bodies, churn=max(1, bodies // 200), seed=777, arena=200.0)
+        # the device takes each step's churn in one bulk call (m3dx_body_respawn:
+        # the same ids and results as the per-body calls the reference makes)
+        st = scenes.NativeStream(lib, w, count=bodies, churn=max(1, bodies // 200), seed=777, arena=200.0,
+                                 bulk=True)
         return st, bodies
     raise SystemExit(f"unknown scene {scene}")
 
@@ -513,7 +516,7 @@ def kernel_window(dev, snap, stream, steps: int, substeps: int):
     cost host time, so they stay out of the timed region; the replay is the
     same computation, the engine being deterministic)."""
     w = dev.import_state(snap)
-    rs = stream.clone_onto(dev, w) if stream is not None else None
+    rs = stream.clone_onto(dev, w, bulk=True) if stream is not None else None
     dev.m3dx_set_kernel_timing(w, 1)
     for _ in range(steps):
         if rs is not None:

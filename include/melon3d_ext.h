@@ -416,6 +416,21 @@ M3D_API int m3dx_world_sphere_cast_batch(m3d_world* world, const m3d_vec3* origi
                                          const m3d_vec3* translations, const uint32_t* maskBits, int32_t count,
                                          m3d_ray_result* out);
 
+/* ------------------------------------------------- bulk spawn/despawn */
+
+/* Bulk despawn and spawn (no counterpart in the reference, which creates and
+   destroys one body per call, src/world.cpp:512-615): the same result as the
+   calls, in this order,
+     for k in 0 .. count-1:
+       m3d_body_destroy(world, destroy[k])                        (destroy != NULL)
+       ids[k] = m3d_body_create(world, &bodyDefs[k], &shapeDefs[k]) (bodyDefs != NULL)
+   - the same ids and slot reuse (LIFO free list), contact removal order,
+   end events, wakes and joint removals - with the contacts of the destroyed
+   bodies found on the device instead of through a host copy of every
+   contact. Destroys are not supported on batches. Returns 0 or -1. */
+M3D_API int m3dx_body_respawn(m3d_world* world, const m3d_body_id* destroy, const m3d_body_def* bodyDefs,
+                              const m3d_shape_def* shapeDefs, int32_t count, m3d_body_id* ids);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

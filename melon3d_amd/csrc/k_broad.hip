@@ -1392,6 +1392,95 @@ bool remove_separated(World& w)
 	return true;
 }
 
+// ------------------------------------------------------------ bulk despawn
+
+// rank[slot] = rank for the destroyed bodies of a bulk despawn (INT32_MAX
+// elsewhere: the array is restored after each call)
+__global__ void k_resp_rank(int32_t* rank, const int2* destroyed, int32_t count, int32_t clear)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k < count)
+		rank[destroyed[k].x] = clear ? INT32_MAX : destroyed[k].y;
+}
+
+__global__ void k_resp_flags(const int2* bodies, int32_t C, const int32_t* rank, uint32_t* flag)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int2 b = bodies[c];
+	flag[c] = (rank[b.x] != INT32_MAX || rank[b.y] != INT32_MAX) ? 1u : 0u;
+}
+
+__global__ void k_resp_compact(const int2* bodies, const float4* mat, int32_t C, const uint32_t* flag,
+							   const uint32_t* offs, int4* out)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C || !flag[c])
+		return;
+	int2 b = bodies[c];
+	out[offs[c]] = make_int4(b.x, b.y, (int32_t)f2u(mat[c].w), c);
+}
+
+bool gather_despawn_contacts(World& w, const std::vector<int2>& destroyed, std::vector<int4>& out)
+{
+	out.clear();
+	int32_t C = w.contactCount;
+	int32_t n = w.deviceBodyCount;
+	int32_t nD = (int32_t)destroyed.size();
+	if (C == 0 || nD == 0)
+		return true;
+	hipStream_t s = w.stream;
+	// the rank array lives across calls, all INT32_MAX between them
+	if ((int64_t)w.respRank.cap < (int64_t)n)
+	{
+		if (!w.respRank.ensure(n))
+			return false;
+		launch_fill_i32(w.respRank.p, INT32_MAX, (int64_t)w.respRank.cap, s);
+	}
+	if (!w.respList.ensure(nD) || !w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.respOut.ensure(C) ||
+		!ensure_prim_scratch(w, C))
+		return false;
+	constexpr int32_t kFirst = 8192; // removed contacts fetched with the count
+	if (w.pinnedResp == nullptr || w.pinnedRespCap < (size_t)std::max(C, kFirst))
+	{
+		if (w.pinnedResp != nullptr)
+			(void)hipHostFree(w.pinnedResp);
+		w.pinnedResp = nullptr;
+		w.pinnedRespCap = (size_t)std::max(C, kFirst) * 3 / 2;
+		if (!hip_ok(hipHostMalloc((void**)&w.pinnedResp, w.pinnedRespCap * sizeof(int4), hipHostMallocDefault),
+					"pinned despawn list"))
+		{
+			w.pinnedRespCap = 0;
+			return false;
+		}
+	}
+	if (!hip_ok(hipMemcpyAsync(w.respList.p, destroyed.data(), nD * sizeof(int2), hipMemcpyHostToDevice, s),
+				"despawn list"))
+		return false;
+	k_resp_rank<<<grid_for(nD), kBlock, 0, s>>>(w.respRank.p, w.respList.p, nD, 0);
+	k_resp_flags<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, C, w.respRank.p, w.u32a.p);
+	scan_exclusive(w.u32a.p, w.u32b.p, C, w.counters.p + CT_RESPAWN_REMOVED, w.scanStatus.p, s);
+	k_resp_compact<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, C, w.u32a.p, w.u32b.p, w.respOut.p);
+	k_resp_rank<<<grid_for(nD), kBlock, 0, s>>>(w.respRank.p, w.respList.p, nD, 1);
+	int32_t first = std::min(C, kFirst);
+	if (!hip_ok(hipMemcpyAsync(w.pinnedResp, w.respOut.p, first * sizeof(int4), hipMemcpyDeviceToHost, s),
+				"despawn contacts") ||
+		!sync_counters(w))
+		return false;
+	int32_t R = (int32_t)w.hostCounters[CT_RESPAWN_REMOVED];
+	if (R > first)
+	{
+		if (!hip_ok(hipMemcpyAsync(w.pinnedResp + first, w.respOut.p + first, (size_t)(R - first) * sizeof(int4),
+								   hipMemcpyDeviceToHost, s),
+					"despawn contacts") ||
+			!hip_ok(hipStreamSynchronize(s), "despawn contacts"))
+			return false;
+	}
+	out.assign(w.pinnedResp, w.pinnedResp + R);
+	return true;
+}
+
 // ------------------------------------------------------------ host sync helpers
 
 __global__ void k_material_refresh(DBodies B, DContacts C, int32_t count, const uint8_t* flag)

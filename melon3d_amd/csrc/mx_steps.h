@@ -54,6 +54,7 @@ enum Counter : int
 	CT_DIFF_CONTACTS, // recorded solver contacts
 	CT_GIANT_BB,      // box-box entries, summed over substeps
 	CT_GIANT_BB_POINTS, // their points, summed over substeps
+	CT_RESPAWN_REMOVED, // between steps: contacts touching the bodies of a bulk despawn
 	CT_COUNT_
 };
 
@@ -121,6 +122,12 @@ bool diff_record_substep(World& w, int sub);
 bool diff_is_recording(const World& w);
 // true (and the error set) when the recording was refused
 bool diff_refused(const World& w);
+
+// ---- bulk despawn (k_broad.hip) ----
+// The contacts touching the bodies of a bulk despawn, as (body a, body b,
+// flags, position) in ascending position: destroyed holds (slot, rank) of
+// each destroyed body, rank = its destroy's place in the call.
+bool gather_despawn_contacts(World& w, const std::vector<int2>& destroyed, std::vector<int4>& out);
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);

@@ -199,6 +199,14 @@ struct World
 	std::vector<int32_t> wakeUpload; // the list being uploaded (kept until the next push)
 	DBuf<int32_t> dWake;
 	bool pendingPermutation = false;
+	// bulk despawn: per-body destroy rank (INT32_MAX between calls), the
+	// call's (slot, rank) list, and the contacts touching those bodies
+	DBuf<int32_t> respRank;
+	DBuf<int2> respList;
+	DBuf<int4> respOut;
+	int4* pinnedResp = nullptr;
+	size_t pinnedRespCap = 0;
+	std::vector<int32_t> respRankHost; // per body slot, -1 between calls
 
 	// ---- device state ----
 	int32_t deviceBodyCap = 0;  // capacity of the body columns

@@ -26,6 +26,8 @@ typedef struct m3ds_api
 	m3d_shape_def (*shape_def_default)(void);
 	m3d_body_id (*body_create)(m3d_world*, const m3d_body_def*, const m3d_shape_def*);
 	void (*body_destroy)(m3d_world*, m3d_body_id);
+	/* bulk despawn + spawn (m3dx_body_respawn), or NULL: one call per body */
+	int (*respawn)(m3d_world*, const m3d_body_id*, const m3d_body_def*, const m3d_shape_def*, int32_t, m3d_body_id*);
 } m3ds_api;
 
 typedef struct m3ds_stream
@@ -38,6 +40,11 @@ typedef struct m3ds_stream
 	int32_t cursor;
 	float arena;
 	m3d_body_id* ring;
+	/* one churn step's arguments for a bulk respawn */
+	m3d_body_id* gone;
+	m3d_body_def* bds;
+	m3d_shape_def* sds;
+	m3d_body_id* born;
 } m3ds_stream;
 
 static double lcg_u(uint32_t* s)
@@ -50,8 +57,8 @@ static double lcg_uniform(uint32_t* s, double a, double b) { return a + (b - a) 
 
 static const double kPi = 3.14159265358979323846;
 
-static m3d_body_id mixed_body(const m3ds_api* api, m3d_world* w, uint32_t* rng, int kind, double x, double y, double z,
-							  double vy)
+static void mixed_defs(const m3ds_api* api, uint32_t* rng, int kind, double x, double y, double z, double vy,
+					   m3d_body_def* bdOut, m3d_shape_def* sdOut)
 {
 	m3d_body_def bd = api->body_def_default();
 	bd.type = M3D_BODY_DYNAMIC;
@@ -91,6 +98,16 @@ static m3d_body_id mixed_body(const m3ds_api* api, m3d_world* w, uint32_t* rng, 
 	bd.orientation.z = (float)(b * sin(2.0 * kPi * u3));
 	bd.orientation.w = (float)(b * cos(2.0 * kPi * u3));
 	sd.friction = (float)lcg_uniform(rng, 0.3, 0.8);
+	*bdOut = bd;
+	*sdOut = sd;
+}
+
+static m3d_body_id mixed_body(const m3ds_api* api, m3d_world* w, uint32_t* rng, int kind, double x, double y, double z,
+							  double vy)
+{
+	m3d_body_def bd;
+	m3d_shape_def sd;
+	mixed_defs(api, rng, kind, x, y, z, vy, &bd, &sd);
 	return api->body_create(w, &bd, &sd);
 }
 
@@ -198,6 +215,18 @@ int m3ds_batch_world(const m3ds_api* api, m3d_world* w, int index, int* isPile)
 	return count;
 }
 
+void m3ds_stream_free(m3ds_stream* st);
+
+static int alloc_churn(m3ds_stream* st, int churn)
+{
+	size_t n = (size_t)(churn > 0 ? churn : 1);
+	st->gone = (m3d_body_id*)malloc(sizeof(m3d_body_id) * n);
+	st->bds = (m3d_body_def*)malloc(sizeof(m3d_body_def) * n);
+	st->sds = (m3d_shape_def*)malloc(sizeof(m3d_shape_def) * n);
+	st->born = (m3d_body_id*)malloc(sizeof(m3d_body_id) * n);
+	return st->gone && st->bds && st->sds && st->born;
+}
+
 /* Streaming scene: count bodies uniform in an arena x 50 x arena volume; each
  * churn step destroys the churn oldest (FIFO ring) and spawns as many at
  * y in [30, 60] with downward speed U[5, 20] m/s. */
@@ -207,9 +236,9 @@ m3ds_stream* m3ds_stream_create(const m3ds_api* api, m3d_world* w, int count, in
 	if (!st)
 		return NULL;
 	st->ring = (m3d_body_id*)malloc(sizeof(m3d_body_id) * (size_t)(count > 0 ? count : 1));
-	if (!st->ring)
+	if (!st->ring || !alloc_churn(st, churn))
 	{
-		free(st);
+		m3ds_stream_free(st);
 		return NULL;
 	}
 	st->api = *api;
@@ -231,6 +260,29 @@ m3ds_stream* m3ds_stream_create(const m3ds_api* api, m3d_world* w, int count, in
 
 void m3ds_stream_churn(m3ds_stream* st)
 {
+	if (st->api.respawn != NULL && st->count > 0)
+	{
+		/* the same draws in the same order, handed over in one call */
+		int n = st->churn;
+		int cur = st->cursor;
+		for (int k = 0; k < n; ++k)
+		{
+			st->gone[k] = st->ring[cur];
+			double x = lcg_uniform(&st->rng, -0.5 * st->arena, 0.5 * st->arena);
+			double y = lcg_uniform(&st->rng, 30.0, 60.0);
+			double z = lcg_uniform(&st->rng, -0.5 * st->arena, 0.5 * st->arena);
+			double vy = -lcg_uniform(&st->rng, 5.0, 20.0);
+			mixed_defs(&st->api, &st->rng, cur % 3, x, y, z, vy, &st->bds[k], &st->sds[k]);
+			cur = (cur + 1) % st->count;
+		}
+		st->api.respawn(st->world, st->gone, st->bds, st->sds, n, st->born);
+		for (int k = 0; k < n; ++k)
+		{
+			st->ring[st->cursor] = st->born[k];
+			st->cursor = (st->cursor + 1) % st->count;
+		}
+		return;
+	}
 	for (int k = 0; k < st->churn && st->count > 0; ++k)
 	{
 		st->api.body_destroy(st->world, st->ring[st->cursor]);
@@ -251,9 +303,9 @@ m3ds_stream* m3ds_stream_clone(const m3ds_stream* src, const m3ds_api* api, m3d_
 	if (!st)
 		return NULL;
 	st->ring = (m3d_body_id*)malloc(sizeof(m3d_body_id) * (size_t)(src->count > 0 ? src->count : 1));
-	if (!st->ring)
+	if (!st->ring || !alloc_churn(st, src->churn))
 	{
-		free(st);
+		m3ds_stream_free(st);
 		return NULL;
 	}
 	memcpy(st->ring, src->ring, sizeof(m3d_body_id) * (size_t)src->count);
@@ -272,6 +324,10 @@ void m3ds_stream_free(m3ds_stream* st)
 	if (st)
 	{
 		free(st->ring);
+		free(st->gone);
+		free(st->bds);
+		free(st->sds);
+		free(st->born);
 		free(st);
 	}
 }

@@ -156,6 +156,11 @@ World::~World()
 	csrVals.release();
 	csrValsAlt.release();
 	dSidecar.release();
+	respRank.release();
+	respList.release();
+	respOut.release();
+	if (pinnedResp != nullptr)
+		(void)hipHostFree(pinnedResp);
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
 }
@@ -1000,6 +1005,8 @@ static void wake_partner(World& w, int32_t i)
 	w.deviceWake.push_back(i);
 }
 
+static void retire_body(World& w, int32_t index);
+
 void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 {
 	World& w = world->w;
@@ -1065,6 +1072,14 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 			w.pendingPermutation = true;
 		}
 	}
+	retire_body(w, index);
+}
+
+// The end of m3d_body_destroy after its contacts (src/world.cpp:595-614):
+// joints on the body go (their other body wakes), the slot is freed with its
+// generation bumped, and the broadphase must rebuild both tiers.
+static void retire_body(World& w, int32_t index)
+{
 	for (int32_t i = 0; i < (int32_t)w.joints.size(); ++i)
 	{
 		DJoint& j = w.joints[i];
@@ -1078,13 +1093,215 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 			w.jointsDirty = true;
 		}
 	}
-	b->inUse = false;
-	b->generation++;
+	HostBody& b = w.bodies[index];
+	b.inUse = false;
+	b.generation++;
 	w.freeBodies.push_back(index);
 	w.gridDirty = true;
 	w.stableDirty = true;
 	w.stableForce = true;
 	mark_dirty(w, index);
+}
+
+// Bulk despawn and spawn: the calls m3d_body_destroy(destroy[k]) then
+// ids[k] = m3d_body_create(&bodyDefs[k], &shapeDefs[k]) for k = 0 ..
+// count - 1, in that order (src/world.cpp:512-615 per call). The slot
+// bookkeeping (free list, generations, joints) runs in call order on the
+// host; creates touch no contact, so the contact removals of all the
+// destroys follow in one pass: the device lists the contacts touching the
+// destroyed bodies, and the host replays the reference's removals on that
+// list alone (per destroy, the body's contacts in descending current
+// position, each swapped with the current last contact; only positions at
+// or beyond the final count and the removed contacts below it take part),
+// which yields the wakes, the end events in the reference's order, and the
+// moves of the surviving tail contacts into the holes.
+int m3dx_body_respawn(m3d_world* world, const m3d_body_id* destroy, const m3d_body_def* bodyDefs,
+					  const m3d_shape_def* shapeDefs, int32_t count, m3d_body_id* ids)
+{
+	if (world == nullptr || count < 0 || (bodyDefs == nullptr) != (shapeDefs == nullptr) ||
+		(bodyDefs != nullptr && ids == nullptr))
+	{
+		set_error("respawn: bad arguments");
+		return -1;
+	}
+	World& w = world->w;
+	if (destroy != nullptr && w.subCount > 0)
+	{
+		set_error("bodies of batched worlds cannot be destroyed");
+		return -1;
+	}
+	if (count == 0)
+		return 0;
+	// the device contacts as the last step and the calls since left them
+	if (destroy != nullptr && !push_host_changes(w))
+		return -1;
+	const int32_t nDev = w.deviceBodyCount;
+	if ((int32_t)w.respRankHost.size() < (int32_t)w.bodies.size())
+		w.respRankHost.resize(w.bodies.size(), -1);
+	// (slot, rank) and the generation before the destroy of every body that
+	// existed on the device when the call began
+	std::vector<int2> destroyed;
+	std::vector<uint32_t> genBefore;
+	std::vector<int32_t> createdSlots;
+	for (int32_t k = 0; k < count; ++k)
+	{
+		if (destroy != nullptr)
+		{
+			HostBody* b = body_checked(w, destroy[k]);
+			if (b != nullptr)
+			{
+				int32_t index = destroy[k].index;
+				// -2: created by this call (no contacts yet); >= 0: already destroyed
+				if (index < nDev && w.respRankHost[index] == -1)
+				{
+					w.respRankHost[index] = (int32_t)destroyed.size();
+					destroyed.push_back(make_int2(index, (int32_t)destroyed.size()));
+					genBefore.push_back(b->generation);
+				}
+				retire_body(w, index);
+			}
+		}
+		if (bodyDefs != nullptr)
+		{
+			ids[k] = m3d_body_create(world, &bodyDefs[k], &shapeDefs[k]);
+			if ((int32_t)w.respRankHost.size() < (int32_t)w.bodies.size())
+				w.respRankHost.resize(w.bodies.size(), -1);
+			if (w.respRankHost[ids[k].index] == -1)
+			{
+				w.respRankHost[ids[k].index] = -2;
+				createdSlots.push_back(ids[k].index);
+			}
+		}
+	}
+	auto reset = [&]() {
+		for (const int2& d : destroyed)
+			w.respRankHost[d.x] = -1;
+		for (int32_t i : createdSlots)
+			w.respRankHost[i] = -1;
+	};
+	if (destroyed.empty())
+	{
+		reset();
+		return 0;
+	}
+	std::vector<int4> rem; // (body a, body b, flags, position), ascending position
+	bool ok = gather_despawn_contacts(w, destroyed, rem);
+	if (ok && !rem.empty())
+	{
+		const int32_t C = w.contactCount;
+		const int32_t R = (int32_t)rem.size();
+		const int32_t n1 = C - R; // final contact count
+		auto rankOf = [&](int32_t b) { return w.respRankHost[b] >= 0 ? w.respRankHost[b] : INT32_MAX; };
+		auto genOf = [&](int32_t b) {
+			int32_t r = w.respRankHost[b];
+			return r >= 0 ? genBefore[r] : w.bodies[b].generation;
+		};
+		// slots: tail slot t >= 0 is position n1 + t; hole h (encoded -1 - h)
+		// is the position of removed entry h (entries below n1 come first)
+		int32_t H = 0;
+		while (H < R && rem[H].w < n1)
+			++H;
+		std::vector<int32_t> tailOcc((size_t)R), holeOcc((size_t)H), cur((size_t)R);
+		for (int32_t t = 0; t < R; ++t)
+			tailOcc[t] = R + t; // surviving tail contact, named by its tail slot
+		for (int32_t i = 0; i < R; ++i)
+		{
+			if (i < H)
+			{
+				holeOcc[i] = i;
+				cur[i] = -1 - i;
+			}
+			else
+			{
+				tailOcc[rem[i].w - n1] = i;
+				cur[i] = rem[i].w - n1;
+			}
+		}
+		auto posOf = [&](int32_t slot) { return slot >= 0 ? n1 + slot : rem[-1 - slot].w; };
+		// entries grouped by the destroy that removes them (the earlier of
+		// their two bodies' destroys)
+		std::vector<int32_t> groupStart(destroyed.size() + 1, 0), order((size_t)R);
+		for (int32_t i = 0; i < R; ++i)
+			++groupStart[std::min(rankOf(rem[i].x), rankOf(rem[i].y)) + 1];
+		for (size_t g = 0; g < destroyed.size(); ++g)
+			groupStart[g + 1] += groupStart[g];
+		{
+			std::vector<int32_t> fill(groupStart.begin(), groupStart.end() - 1);
+			for (int32_t i = 0; i < R; ++i)
+				order[fill[std::min(rankOf(rem[i].x), rankOf(rem[i].y))]++] = i;
+		}
+		int32_t size = C;
+		std::vector<std::pair<int32_t, int32_t>> grp;
+		for (size_t g = 0; g < destroyed.size(); ++g)
+		{
+			const int32_t body = destroyed[g].x;
+			grp.clear();
+			for (int32_t q = groupStart[g]; q < groupStart[g + 1]; ++q)
+				grp.emplace_back(posOf(cur[order[q]]), order[q]);
+			std::sort(grp.begin(), grp.end(), [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+				return a.first > b.first;
+			});
+			for (const auto& pe : grp)
+			{
+				const int32_t i = pe.second;
+				const int4 e = rem[i];
+				const uint32_t cf = (uint32_t)e.z;
+				if (cf & CF_TOUCHING)
+				{
+					const int32_t other = e.x == body ? e.y : e.x;
+					// a partner destroyed later in this call is overwritten by
+					// its own destroy (and a create in its slot)
+					if (!(cf & CF_SENSOR) && w.respRankHost[other] < 0)
+						wake_partner(w, other);
+					m3d_contact_event ev;
+					ev.bodyA = { e.x, genOf(e.x) };
+					ev.bodyB = { e.y, genOf(e.y) };
+					ev.isSensor = (cf & CF_SENSOR) != 0;
+					w.endEvents.push_back(ev);
+				}
+				// swap with the current last contact
+				const int32_t lastSlot = size - 1 - n1;
+				const int32_t o = tailOcc[lastSlot];
+				if (o != i)
+				{
+					const int32_t sl = cur[i];
+					if (sl >= 0)
+						tailOcc[sl] = o;
+					else
+						holeOcc[-1 - sl] = o;
+					if (o < R)
+						cur[o] = sl;
+				}
+				--size;
+			}
+		}
+		std::vector<int2> mv;
+		mv.reserve(H);
+		for (int32_t h = 0; h < H; ++h)
+		{
+			const int32_t o = holeOcc[h];
+			if (o < R)
+			{
+				set_error("respawn: contact removal replay left a hole unfilled");
+				ok = false;
+				break;
+			}
+			mv.push_back(make_int2(rem[h].w, n1 + (o - R)));
+		}
+		// (a copy from pageable memory has left the vector when it returns;
+		// the moves run in stream order before the next use of the contacts)
+		if (ok && !mv.empty())
+			ok = w.moves.ensure(mv.size()) &&
+				 hip_ok(hipMemcpyAsync(w.moves.p, mv.data(), mv.size() * sizeof(int2), hipMemcpyHostToDevice, w.stream),
+						"respawn moves") &&
+				 apply_moves(w, w.moves.p, (int32_t)mv.size());
+		if (ok)
+			w.contactCount = n1;
+	}
+	reset();
+	w.sidecarValid = false;
+	w.sidecarPrefetched = false;
+	return ok ? 0 : -1;
 }
 
 bool m3d_body_is_valid(m3d_world* world, m3d_body_id id) { return body_checked(world->w, id) != nullptr; }

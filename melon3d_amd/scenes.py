@@ -12,7 +12,7 @@ import ctypes as C
 import math
 
 from .api import (M3D_BODY_DYNAMIC, M3D_BODY_STATIC, M3D_SHAPE_BOX, M3D_SHAPE_CAPSULE, M3D_SHAPE_SPHERE, Library,
-                  m3d_quat, m3d_vec3)
+                  m3d_body_def, m3d_body_id, m3d_quat, m3d_shape_def, m3d_vec3)
 
 
 class Lcg:
@@ -81,6 +81,12 @@ def pyramid(L: Library, w, base=20, he=0.5, friction=0.6):
 def mixed_body(L: Library, w, rng: Lcg, kind: int, pos, vel=(0.0, 0.0, 0.0)):
     """One rain-pile body. Draw order: size draws for the kind, then a
     quaternion (3 draws), then friction (1 draw)."""
+    bd, sd = mixed_defs(L, rng, kind, pos, vel)
+    return L.m3d_body_create(w, C.byref(bd), C.byref(sd))
+
+
+def mixed_defs(L: Library, rng: Lcg, kind: int, pos, vel=(0.0, 0.0, 0.0)):
+    """The body and shape definitions of mixed_body (same draws)."""
     bd = L.m3d_body_def_default()
     bd.type = M3D_BODY_DYNAMIC
     bd.position = m3d_vec3(*pos)
@@ -98,7 +104,7 @@ def mixed_body(L: Library, w, rng: Lcg, kind: int, pos, vel=(0.0, 0.0, 0.0)):
         sd.halfExtents = m3d_vec3(rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6))
     bd.orientation = rng.quat()
     sd.friction = rng.uniform(0.3, 0.8)
-    return L.m3d_body_create(w, C.byref(bd), C.byref(sd))
+    return bd, sd
 
 
 def rain_pile(L: Library, w, count=20000, seed=12345, column=(40.0, 40.0, 60.0)):
@@ -135,9 +141,32 @@ class Stream:
             self.ring.append(mixed_body(L, w, self.rng, i % 3, (x, y, z)))
         self.cursor = 0
 
-    def churn_step(self):
+    def churn_step(self, bulk: bool = False):
+        """Destroy the churn oldest and spawn as many; bulk: as one
+        m3dx_body_respawn call with the same draws."""
         L, w, rng = self.L, self.w, self.rng
         n = len(self.ring)
+        if bulk:
+            k = self.churn
+            gone = (m3d_body_id * k)()
+            bds = (m3d_body_def * k)()
+            sds = (m3d_shape_def * k)()
+            born = (m3d_body_id * k)()
+            cur = self.cursor
+            for j in range(k):
+                gone[j] = self.ring[cur]
+                x = rng.uniform(-0.5 * self.arena, 0.5 * self.arena)
+                y = rng.uniform(30.0, 60.0)
+                z = rng.uniform(-0.5 * self.arena, 0.5 * self.arena)
+                vy = -rng.uniform(5.0, 20.0)
+                bds[j], sds[j] = mixed_defs(L, rng, cur % 3, (x, y, z), (0.0, vy, 0.0))
+                cur = (cur + 1) % n
+            if L.m3dx_body_respawn(w, gone, bds, sds, k, born) != 0:
+                raise RuntimeError("m3dx_body_respawn failed: " + L.last_error())
+            for j in range(k):
+                self.ring[self.cursor] = born[j]
+                self.cursor = (self.cursor + 1) % n
+            return
         for _ in range(self.churn):
             L.m3d_body_destroy(w, self.ring[self.cursor])
             x = rng.uniform(-0.5 * self.arena, 0.5 * self.arena)
@@ -152,7 +181,7 @@ class Stream:
 
 class _SceneApi(C.Structure):
     _fields_ = [("body_def_default", C.c_void_p), ("shape_def_default", C.c_void_p),
-                ("body_create", C.c_void_p), ("body_destroy", C.c_void_p)]
+                ("body_create", C.c_void_p), ("body_destroy", C.c_void_p), ("respawn", C.c_void_p)]
 
 
 _driver = None
@@ -183,11 +212,14 @@ def _scene_driver():
     return _driver
 
 
-def _api_table(L: Library) -> _SceneApi:
+def _api_table(L: Library, bulk: bool = False) -> _SceneApi:
+    """Function table of a library for the native driver; bulk: churn through
+    m3dx_body_respawn (one call per churn step) instead of one call per body."""
     t = _SceneApi()
     for name, fn in (("body_def_default", "m3d_body_def_default"), ("shape_def_default", "m3d_shape_def_default"),
                      ("body_create", "m3d_body_create"), ("body_destroy", "m3d_body_destroy")):
         setattr(t, name, C.cast(getattr(L.lib, fn), C.c_void_p).value)
+    t.respawn = C.cast(L.lib.m3dx_body_respawn, C.c_void_p).value if bulk else None
     return t
 
 
@@ -206,8 +238,11 @@ class NativeStream:
     """Stream with its per-step churn in native code (as the reference's own
     bench does it in C++)."""
 
-    def __init__(self, L: Library, w, count=200000, churn=1000, seed=777, arena=200.0, _handle=None):
-        self._table = _api_table(L)
+    def __init__(self, L: Library, w, count=200000, churn=1000, seed=777, arena=200.0, _handle=None,
+                 bulk: bool = False):
+        """bulk: each churn step is one m3dx_body_respawn call (the same
+        bodies, ids and results as the per-body calls)."""
+        self._table = _api_table(L, bulk)
         self._h = _handle if _handle is not None else _scene_driver().m3ds_stream_create(
             C.byref(self._table), _world_ptr(w), count, churn, seed, arena)
         if not self._h:
@@ -216,9 +251,9 @@ class NativeStream:
     def churn_step(self):
         _scene_driver().m3ds_stream_churn(self._h)
 
-    def clone_onto(self, L: Library, w) -> "NativeStream":
+    def clone_onto(self, L: Library, w, bulk: bool = False) -> "NativeStream":
         """Continue this stream on a world imported from this one's snapshot."""
-        t = _api_table(L)
+        t = _api_table(L, bulk)
         h = _scene_driver().m3ds_stream_clone(self._h, C.byref(t), _world_ptr(w))
         s = NativeStream.__new__(NativeStream)
         s._table, s._h = t, h

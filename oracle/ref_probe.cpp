@@ -562,6 +562,23 @@ int m3dx_world_ray_cast_batch(m3d_world*, const m3d_vec3*, const m3d_vec3*, cons
 {
     return -1;
 }
+/* the bulk spawn/despawn is defined as the per-call sequence: on the CPU
+   engine it is that sequence */
+int m3dx_body_respawn(m3d_world* w, const m3d_body_id* destroy, const m3d_body_def* bodyDefs,
+                      const m3d_shape_def* shapeDefs, int32_t count, m3d_body_id* ids)
+{
+    if (w == nullptr || count < 0 || (bodyDefs == nullptr) != (shapeDefs == nullptr) ||
+        (bodyDefs != nullptr && ids == nullptr))
+        return -1;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        if (destroy != nullptr)
+            m3d_body_destroy(w, destroy[k]);
+        if (bodyDefs != nullptr)
+            ids[k] = m3d_body_create(w, &bodyDefs[k], &shapeDefs[k]);
+    }
+    return 0;
+}
 int m3dx_world_ray_cast_batch_noalloc(m3d_world*, const m3d_vec3*, const m3d_vec3*, const uint32_t*, int32_t,
                                       m3d_ray_result*)
 {

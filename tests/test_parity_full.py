@@ -126,7 +126,7 @@ def test_stream_200k_one_step_after_200_churn_steps(ref, ref_libm, dev):
     dw = dev.create_world()
     rws = []
     try:
-        sd = scenes.NativeStream(dev, dw, count=200000, churn=1000)
+        sd = scenes.NativeStream(dev, dw, count=200000, churn=1000, bulk=True)
         for _ in range(200):
             sd.churn_step()
             dev.m3d_world_step(dw, DT, 4)
