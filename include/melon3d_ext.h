@@ -416,6 +416,12 @@ M3D_API int m3dx_world_sphere_cast_batch(m3d_world* world, const m3d_vec3* origi
                                          const m3d_vec3* translations, const uint32_t* maskBits, int32_t count,
                                          m3d_ray_result* out);
 
+/* Transforms of many bodies in one call (no counterpart in the reference,
+   which returns one per call): out[k] = m3d_body_transform(world, ids[k])
+   (include/melon3d.h:470, src/world.cpp:634-645), gathered on the device (a frame of a scene dump,
+   see melon3d_amd/scene_dump.py). Returns 0 or -1. */
+M3D_API int m3dx_world_transforms(m3d_world* world, const m3d_body_id* ids, int32_t count, m3d_transform* out);
+
 /* ------------------------------------------------- bulk spawn/despawn */
 
 /* Bulk despawn and spawn (no counterpart in the reference, which creates and

@@ -335,6 +335,28 @@ __global__ void k_sphere_cast(RayCtx R, int32_t nBodies, const float4* castO, co
 
 using namespace mx;
 
+// ---- body transforms of one frame ----------------------------------------
+
+// out[k] = pose of body idx[k] (7 floats: position, rotation xyzw); idx -1:
+// the identity pose at the origin.
+__global__ void k_gather_poses(DBodies B, const int32_t* idx, int32_t count, float* out)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	int32_t i = idx[k];
+	float4 p = i >= 0 ? B.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	float4 q = i >= 0 ? B.rot[i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+	float* o = out + 7 * (int64_t)k;
+	o[0] = p.x;
+	o[1] = p.y;
+	o[2] = p.z;
+	o[3] = q.x;
+	o[4] = q.y;
+	o[5] = q.z;
+	o[6] = q.w;
+}
+
 // ---- allocation-free ray casts ------------------------------------------
 
 // Staging of m3dx_world_ray_cast_batch_noalloc, one array per GPU for the
@@ -532,6 +554,72 @@ int m3dx_world_ray_cast_batch_noalloc(m3d_world* world, const m3d_vec3* origins,
 			r.fraction = hO[k].w;
 			r.normal = m3d_v3(hD[k].x, hD[k].y, hD[k].z);
 		}
+	}
+	return 0;
+}
+
+// Transforms of many bodies in one call: out[k] = m3d_body_transform(world,
+// ids[k]) (include/melon3d.h:470, src/world.cpp:634-645), gathered on the
+// device after a step instead of through a download of every body.
+int m3dx_world_transforms(m3d_world* world, const m3d_body_id* ids, int32_t count, m3d_transform* out)
+{
+	if (world == nullptr || count < 0 || (count > 0 && (ids == nullptr || out == nullptr)))
+	{
+		set_error("transforms: null arrays");
+		return -1;
+	}
+	World& w = world->w;
+	if (count == 0)
+		return 0;
+	auto valid = [&](m3d_body_id id) {
+		return id.index >= 0 && id.index < (int32_t)w.bodies.size() && w.bodies[id.index].inUse &&
+			   w.bodies[id.index].generation == id.generation;
+	};
+	auto dirty = [&](int32_t i) { return i < (int32_t)w.dirtyFlag.size() && w.dirtyFlag[i] != 0; };
+	if (!w.hostStale)
+	{
+		for (int32_t k = 0; k < count; ++k)
+			out[k] = m3d_body_transform(world, ids[k]);
+		return 0;
+	}
+	std::vector<int32_t> idx((size_t)count);
+	for (int32_t k = 0; k < count; ++k)
+	{
+		int32_t i = ids[k].index;
+		idx[k] = valid(ids[k]) && i < w.deviceBodyCount && !dirty(i) ? i : -1;
+	}
+	hipStream_t s = w.stream;
+	DBuf<char> buf;
+	if (!buf.ensure((size_t)count * (sizeof(int32_t) + 7 * sizeof(float)) + 256))
+		return -1;
+	int32_t* dIdx = (int32_t*)buf.p;
+	float* dOut = (float*)(buf.p + (((size_t)count * sizeof(int32_t) + 255) & ~(size_t)255));
+	std::vector<float> h((size_t)count * 7);
+	bool ok = hip_ok(hipMemcpyAsync(dIdx, idx.data(), count * sizeof(int32_t), hipMemcpyHostToDevice, s), "poses");
+	if (ok)
+	{
+		k_gather_poses<<<(count + 255) / 256, 256, 0, s>>>(make_body_view(w), dIdx, count, dOut);
+		ok = hip_ok(hipGetLastError(), "poses") &&
+			 hip_ok(hipMemcpyAsync(h.data(), dOut, (size_t)count * 7 * sizeof(float), hipMemcpyDeviceToHost, s),
+					"poses") &&
+			 hip_ok(hipStreamSynchronize(s), "poses");
+	}
+	buf.release();
+	if (!ok)
+		return -1;
+	for (int32_t k = 0; k < count; ++k)
+	{
+		int32_t i = ids[k].index;
+		if (idx[k] < 0 && valid(ids[k]))
+		{
+			// a record the host holds ahead of the device
+			out[k].p = w.bodies[i].position;
+			out[k].q = w.bodies[i].rotation;
+			continue;
+		}
+		const float* o = h.data() + 7 * (size_t)k;
+		out[k].p = m3d_v3(o[0], o[1], o[2]);
+		out[k].q = { o[3], o[4], o[5], o[6] };
 	}
 	return 0;
 }

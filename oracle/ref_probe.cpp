@@ -18,22 +18,6 @@
 
 using namespace m3d;
 
-#ifdef M3D_ORACLE_CR_LIBM
-// Strict parity build: the reference's single-precision transcendentals
-// (src/xpbd.cpp:693-694, :882; src/m3d_internal.h:361) resolve to correctly
-// rounded versions (double evaluation, one rounding) inside this library, the
-// same functions the device backend evaluates. glibc's own sinf/cosf are
-// within 0.56 ulp of these; the release-flag build keeps the system libm.
-#include <math.h>
-extern "C" float sinf(float x) { return (float)sin((double)x); }
-extern "C" float cosf(float x) { return (float)cos((double)x); }
-extern "C" float atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
-extern "C" void sincosf(float x, float* s, float* c)
-{
-	*s = (float)sin((double)x);
-	*c = (float)cos((double)x);
-}
-#endif
 
 namespace
 {
@@ -561,6 +545,15 @@ int m3dx_set_pass_blocks(m3d_world*, int) { return -1; }
 int m3dx_world_ray_cast_batch(m3d_world*, const m3d_vec3*, const m3d_vec3*, const uint32_t*, int32_t, m3d_ray_result*)
 {
     return -1;
+}
+/* the CPU engine's transforms are its getter, one body at a time */
+int m3dx_world_transforms(m3d_world* w, const m3d_body_id* ids, int32_t count, m3d_transform* out)
+{
+    if (w == nullptr || count < 0 || (count > 0 && (ids == nullptr || out == nullptr)))
+        return -1;
+    for (int32_t k = 0; k < count; ++k)
+        out[k] = m3d_body_transform(w, ids[k]);
+    return 0;
 }
 /* the bulk spawn/despawn is defined as the per-call sequence: on the CPU
    engine it is that sequence */
