@@ -437,6 +437,17 @@ M3D_API int m3dx_world_transforms(m3d_world* world, const m3d_body_id* ids, int3
 M3D_API int m3dx_body_respawn(m3d_world* world, const m3d_body_id* destroy, const m3d_body_def* bodyDefs,
                               const m3d_shape_def* shapeDefs, int32_t count, m3d_body_id* ids);
 
+/* Many AABB overlap queries in one call on the device (no counterpart in
+   the reference, which runs one query per call with a callback,
+   include/melon3d.h:519-522, src/world.cpp:1394-1416): the bodies whose
+   tight box overlaps boxes[k], in ascending body index (the order the
+   reference's callback sees them), are out[offsets[k] .. offsets[k + 1]).
+   offsets (count + 1 entries) is always filled; at most cap ids are written
+   (out may be NULL with cap 0, to size a second call). Returns the total
+   number of results, or -1. */
+M3D_API int m3dx_world_overlap_aabb_batch(m3d_world* world, const m3d_aabb* boxes, int32_t count,
+                                          int32_t* offsets, m3d_body_id* out, int32_t cap);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

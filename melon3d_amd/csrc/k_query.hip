@@ -32,6 +32,12 @@ struct RayCtx
 	const int32_t* sOver;
 	int32_t nSOver;
 	float cellSize;
+	// bodies in use the grid may not find (created or moved since the tiers
+	// were built, or whose tight box has left the fat box the tiers hold),
+	// tested by every query that needs every overlapping body; null for the
+	// ray cast, which follows the grid as the reference's does
+	const int32_t* extra;
+	const uint32_t* nExtra;
 };
 
 struct RayHit
@@ -264,10 +270,11 @@ __device__ __forceinline__ void cast_test(const RayCtx& R, int32_t bi, m3d_vec3 
 	}
 }
 
-// Sphere casts: candidates are the coarse lists and the bodies of the grid
-// cells the swept sphere's box covers (a body whose tight box meets the
-// sweep has an entry in one of them); sweeps over more than kCastCells cells
-// test every body instead. castO: origin xyz, mask bits; castD: translation
+// Sphere casts: candidates are the coarse lists, the bodies the grid may not
+// find (R.extra) and the bodies of the grid cells the swept sphere's box
+// covers (any other body whose tight box meets the sweep has an entry in one
+// of them: its tight box lies in its fat box); sweeps over more than
+// kCastCells cells test every body instead. castO: origin xyz, mask bits; castD: translation
 // xyz, radius. Outputs as k_ray_cast.
 constexpr int64_t kCastCells = 512;
 
@@ -306,6 +313,9 @@ __global__ void k_sphere_cast(RayCtx R, int32_t nBodies, const float4* castO, co
 			cast_test(R, R.aOver[i], o, radius, d, len, mask, sweep, B);
 		for (int32_t i = 0; i < R.nSOver; ++i)
 			cast_test(R, R.sOver[i], o, radius, d, len, mask, sweep, B);
+		if (R.extra != nullptr)
+			for (uint32_t i = 0; i < *R.nExtra; ++i)
+				cast_test(R, R.extra[i], o, radius, d, len, mask, sweep, B);
 		for (int64_t x = x0; x <= x1; ++x)
 			for (int64_t y = y0; y <= y1; ++y)
 				for (int64_t z = z0; z <= z1; ++z)
@@ -329,6 +339,114 @@ __global__ void k_sphere_cast(RayCtx R, int32_t nBodies, const float4* castO, co
 		outA[k] = v3_f4(end, 2.0f);
 		outB[k] = v3_f4(m3d_v3_zero(), i2f(-1));
 	}
+}
+
+// ---- AABB overlap queries ----------------------------------------------
+
+// flag[i] = body i is in use and the grid may not find it: created after the
+// tiers were built (slot at or beyond gridN, or no fat box), or its tight
+// box is not inside the fat box the tiers hold (moved by the API, or by the
+// step past its fat box).
+__global__ void k_query_extra_flags(DBodies B, int32_t n, int32_t gridN, uint32_t* flag)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i >= n)
+		return;
+	uint32_t f = 0u;
+	if (B.flags[i] & BF_IN_USE)
+	{
+		m3d_aabb t = shape_aabb(load_shape(B, i), load_xf(B, i));
+		float4 lo = B.fatLo[i], hi = B.fatHi[i];
+		bool inside = i < gridN && (B.bp[i] & BP_HAS_FAT) && lo.x <= t.lowerBound.x && lo.y <= t.lowerBound.y &&
+					  lo.z <= t.lowerBound.z && t.upperBound.x <= hi.x && t.upperBound.y <= hi.y &&
+					  t.upperBound.z <= hi.z;
+		f = inside ? 0u : 1u;
+	}
+	flag[i] = f;
+}
+
+__global__ void k_query_extra_compact(const uint32_t* flag, const uint32_t* offs, int32_t n, int32_t* extra)
+{
+	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (i < n && flag[i])
+		extra[offs[i]] = i;
+}
+
+// The bodies whose tight box overlaps query k's box
+// (src/world.cpp:1394-1416), from the coarse lists, R.extra and the grid
+// cells the box covers (every body, for boxes over more than kCastCells
+// cells or brute != 0). A body can be found more than once (cells, tiers,
+// lists); the caller sorts and drops repeats. Pass 0 counts into cnt, pass
+// 1 writes (query << 32 | body) at offs[k].
+__device__ __forceinline__ void overlap_emit(const RayCtx& R, int32_t bi, const m3d_aabb& q, uint64_t k, int pass,
+											 uint32_t& n, uint64_t* out, uint32_t base)
+{
+	if (!(R.B.flags[bi] & BF_IN_USE))
+		return;
+	if (!m3d_aabb_overlap(q, shape_aabb(load_shape(R.B, bi), load_xf(R.B, bi))))
+		return;
+	if (pass)
+		out[base + n] = (k << 32) | (uint32_t)bi;
+	++n;
+}
+
+__global__ void k_overlap(RayCtx R, int32_t nBodies, int brute, const float4* qLo, const float4* qHi, int32_t count,
+						  int pass, uint32_t* cnt, const uint32_t* offs, uint64_t* out)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= count)
+		return;
+	m3d_aabb q;
+	q.lowerBound = f4_v3(qLo[k]);
+	q.upperBound = f4_v3(qHi[k]);
+	uint32_t n = 0u;
+	uint32_t base = pass ? offs[k] : 0u;
+	float inv = 1.0f / R.cellSize;
+	int64_t x0 = cell_coord(q.lowerBound.x, inv), x1 = cell_coord(q.upperBound.x, inv);
+	int64_t y0 = cell_coord(q.lowerBound.y, inv), y1 = cell_coord(q.upperBound.y, inv);
+	int64_t z0 = cell_coord(q.lowerBound.z, inv), z1 = cell_coord(q.upperBound.z, inv);
+	int64_t cells = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+	if (brute || cells > kCastCells || cells <= 0)
+	{
+		for (int32_t bi = 0; bi < nBodies; ++bi)
+			overlap_emit(R, bi, q, (uint64_t)k, pass, n, out, base);
+	}
+	else
+	{
+		for (int32_t i = 0; i < R.nAOver; ++i)
+			overlap_emit(R, R.aOver[i], q, (uint64_t)k, pass, n, out, base);
+		for (int32_t i = 0; i < R.nSOver; ++i)
+			overlap_emit(R, R.sOver[i], q, (uint64_t)k, pass, n, out, base);
+		for (uint32_t i = 0; i < *R.nExtra; ++i)
+			overlap_emit(R, R.extra[i], q, (uint64_t)k, pass, n, out, base);
+		for (int64_t x = x0; x <= x1; ++x)
+			for (int64_t y = y0; y <= y1; ++y)
+				for (int64_t z = z0; z <= z1; ++z)
+				{
+					uint64_t key = pack_cell(x, y, z);
+					for (int64_t i = lower_bound_key(R.aKeys, R.aCount, key); i < R.aCount && R.aKeys[i] == key; ++i)
+						overlap_emit(R, (int32_t)R.aBodies[i], q, (uint64_t)k, pass, n, out, base);
+					for (int64_t i = lower_bound_key(R.sKeys, R.sCount, key); i < R.sCount && R.sKeys[i] == key; ++i)
+						overlap_emit(R, (int32_t)R.sBodies[i], q, (uint64_t)k, pass, n, out, base);
+				}
+	}
+	if (!pass)
+		cnt[k] = n;
+}
+
+__global__ void k_first_of_run(const uint64_t* keys, int64_t n, uint32_t* flag)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_compact_keys(const uint64_t* keys, const uint32_t* flag, const uint32_t* offs, int64_t n,
+							   uint64_t* out)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flag[i])
+		out[offs[i]] = keys[i];
 }
 
 } // namespace mx
@@ -412,13 +530,40 @@ static RayCtx ray_ctx(World& w)
 	R.sOver = w.dStableOversized.p;
 	R.nSOver = (int32_t)w.stableOversized.size();
 	R.cellSize = w.def.gridCellSize;
+	R.extra = nullptr;
+	R.nExtra = nullptr;
 	return R;
+}
+
+// The bodies the grid may not find (RayCtx::extra), listed on the device; the
+// count stays on the device (*nExtra).
+static bool query_extra(World& w, RayCtx& R)
+{
+	int32_t n = w.deviceBodyCount;
+	hipStream_t s = w.stream;
+	if (!w.qFlag.ensure((size_t)std::max(n, 1)) || !w.qOffs.ensure((size_t)n + 1) ||
+		!w.qExtra.ensure((size_t)std::max(n, 1)) || !ensure_prim_scratch(w, std::max(n, 1)))
+		return false;
+	uint32_t* total = w.qOffs.p + n;
+	if (n == 0)
+	{
+		launch_fill_u32(total, 0u, 1, s);
+	}
+	else
+	{
+		k_query_extra_flags<<<(n + 255) / 256, 256, 0, s>>>(make_body_view(w), n, w.gridBodies, w.qFlag.p);
+		scan_exclusive(w.qFlag.p, w.qOffs.p, n, total, w.scanStatus.p, s);
+		k_query_extra_compact<<<(n + 255) / 256, 256, 0, s>>>(w.qFlag.p, w.qOffs.p, n, w.qExtra.p);
+	}
+	R.extra = w.qExtra.p;
+	R.nExtra = total;
+	return true;
 }
 
 // host side of the batched queries: arrays up, kernel, results down
 template <class Launch>
 static int run_query(World& w, const std::vector<float4>& o, const std::vector<float4>& d, m3d_ray_result* out,
-					 Launch&& launch)
+					 bool withExtra, Launch&& launch)
 {
 	int32_t count = (int32_t)o.size();
 	if (w.gridDirty || w.stableDirty)
@@ -436,6 +581,8 @@ static int run_query(World& w, const std::vector<float4>& o, const std::vector<f
 	float4* dA = dD + count;
 	float4* dB = dA + count;
 	RayCtx R = ray_ctx(w);
+	if (withExtra && !query_extra(w, R))
+		return -1;
 	std::vector<float4> a((size_t)count), b((size_t)count);
 	bool ok = hip_ok(hipMemcpyAsync(dO, o.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries") &&
 			  hip_ok(hipMemcpyAsync(dD, d.data(), count * sizeof(float4), hipMemcpyHostToDevice, s), "queries");
@@ -463,8 +610,134 @@ static int run_query(World& w, const std::vector<float4>& o, const std::vector<f
 	return 0;
 }
 
+namespace mx
+{
+
+// Sorted, repeat-free (query << 32 | body) pairs of the bodies whose tight
+// box overlaps each query box (src/world.cpp:1394-1416 per query), on the
+// device: host changes are pushed first (no broadphase refresh: the
+// reference's overlap query reads no grid), candidates come from the grid
+// plus the bodies it may not find, or from every body for a batch (whose
+// cell keys carry the sub-world) and for large boxes.
+bool overlap_query(World& w, const m3d_aabb* boxes, int32_t count, std::vector<uint64_t>& pairs)
+{
+	pairs.clear();
+	if (count <= 0)
+		return true;
+	if (!push_host_changes(w))
+		return false;
+	hipStream_t s = w.stream;
+	int32_t n = w.deviceBodyCount;
+	RayCtx R = ray_ctx(w);
+	if (!query_extra(w, R))
+		return false;
+	std::vector<float4> lo((size_t)count), hi((size_t)count);
+	for (int32_t k = 0; k < count; ++k)
+	{
+		lo[k] = v3_f4(boxes[k].lowerBound, 0.0f);
+		hi[k] = v3_f4(boxes[k].upperBound, 0.0f);
+	}
+	DBuf<char> buf;
+	size_t qb = (size_t)count * sizeof(float4);
+	size_t cb = ((size_t)count + 1) * sizeof(uint32_t);
+	if (!buf.ensure(2 * qb + 2 * cb + 512))
+		return false;
+	float4* dLo = (float4*)buf.p;
+	float4* dHi = dLo + count;
+	uint32_t* cnt = (uint32_t*)(dHi + count);
+	uint32_t* offs = cnt + count + 1;
+	const int brute = w.subCount > 0 ? 1 : 0;
+	const unsigned grid = (unsigned)((count + 127) / 128);
+	bool ok = hip_ok(hipMemcpyAsync(dLo, lo.data(), qb, hipMemcpyHostToDevice, s), "overlap boxes") &&
+			  hip_ok(hipMemcpyAsync(dHi, hi.data(), qb, hipMemcpyHostToDevice, s), "overlap boxes");
+	if (ok)
+	{
+		k_overlap<<<grid, 128, 0, s>>>(R, n, brute, dLo, dHi, count, 0, cnt, nullptr, nullptr);
+		scan_exclusive(cnt, offs, count, w.counters.p + CT_RESPAWN_REMOVED, w.scanStatus.p, s);
+		ok = hip_ok(hipGetLastError(), "overlap count") && sync_counters(w);
+	}
+	int64_t total = ok ? (int64_t)w.hostCounters[CT_RESPAWN_REMOVED] : 0;
+	if (ok && total > 0)
+	{
+		int bitsQ = 1;
+		while ((1ll << bitsQ) < (int64_t)count)
+			++bitsQ;
+		// (growing a buffer frees the old one: none of these is one the
+		// launches below read through R, which holds qExtra and qOffs)
+		ok = w.qPairs.ensure(total) && w.qPairsAlt.ensure(total) && w.qVals.ensure(total) &&
+			 w.qValsAlt.ensure(total) && w.qUFlag.ensure(total) && w.qUOffs.ensure(total + 1) &&
+			 ensure_prim_scratch(w, total);
+		if (ok)
+		{
+			k_overlap<<<grid, 128, 0, s>>>(R, n, brute, dLo, dHi, count, 1, cnt, offs, w.qPairs.p);
+			launch_fill_u32(w.qVals.p, 0u, total, s);
+			radix_sort_pairs(w.qPairs.p, w.qVals.p, w.qPairsAlt.p, w.qValsAlt.p, total, 32 + bitsQ, w.primScratch.p,
+							 w.scanStatus.p, s);
+			unsigned g2 = (unsigned)((total + 255) / 256);
+			k_first_of_run<<<g2, 256, 0, s>>>(w.qPairs.p, total, w.qUFlag.p);
+			scan_exclusive(w.qUFlag.p, w.qUOffs.p, total, w.counters.p + CT_RESPAWN_REMOVED, w.scanStatus.p, s);
+			k_compact_keys<<<g2, 256, 0, s>>>(w.qPairs.p, w.qUFlag.p, w.qUOffs.p, total, w.qPairsAlt.p);
+			ok = hip_ok(hipGetLastError(), "overlap pairs") && sync_counters(w);
+		}
+		if (ok)
+		{
+			int64_t uniq = (int64_t)w.hostCounters[CT_RESPAWN_REMOVED];
+			pairs.resize((size_t)uniq);
+			ok = hip_ok(hipMemcpy(pairs.data(), w.qPairsAlt.p, (size_t)uniq * sizeof(uint64_t), hipMemcpyDeviceToHost),
+						"overlap pairs");
+		}
+	}
+	buf.release();
+	return ok;
+}
+
+} // namespace mx
+
 extern "C"
 {
+
+// Many AABB overlap queries in one call on the device (no counterpart in the
+// reference, which runs one query per call with a callback): the bodies of
+// query k, in ascending body index as the reference's callback would see
+// them (src/world.cpp:1394-1416), are out[offsets[k] .. offsets[k + 1]).
+// offsets (count + 1 entries) is always filled; out receives at most cap ids
+// (out may be null with cap 0). Returns the total number of results, or -1.
+int m3dx_world_overlap_aabb_batch(m3d_world* world, const m3d_aabb* boxes, int32_t count, int32_t* offsets,
+								  m3d_body_id* out, int32_t cap)
+{
+	if (world == nullptr || count < 0 || (count > 0 && (boxes == nullptr || offsets == nullptr)) || cap < 0 ||
+		(cap > 0 && out == nullptr))
+	{
+		set_error("overlap batch: bad arguments");
+		return -1;
+	}
+	World& w = world->w;
+	std::vector<uint64_t> pairs;
+	if (!overlap_query(w, boxes, count, pairs))
+		return -1;
+	if ((int64_t)pairs.size() > INT32_MAX)
+	{
+		set_error("overlap batch: more than 2^31 results");
+		return -1;
+	}
+	size_t j = 0;
+	for (int32_t k = 0; k < count; ++k)
+	{
+		offsets[k] = (int32_t)j;
+		while (j < pairs.size() && (int32_t)(pairs[j] >> 32) == k)
+		{
+			if ((int64_t)j < cap)
+			{
+				int32_t bi = (int32_t)(uint32_t)pairs[j];
+				out[j] = m3d_body_id{ bi, w.bodies[bi].generation };
+			}
+			++j;
+		}
+	}
+	if (count >= 0)
+		offsets[count] = (int32_t)j;
+	return (int32_t)pairs.size();
+}
 
 // m3dx_world_ray_cast_batch without any allocation: device staging in the
 // library's per-GPU `scratch` array (at most kNoAllocMaxRays rays per
@@ -652,7 +925,7 @@ int m3dx_world_ray_cast_batch(m3d_world* world, const m3d_vec3* origins, const m
 		o[k] = make_float4(origins[k].x, origins[k].y, origins[k].z, u2f(m));
 		d[k] = make_float4(translations[k].x, translations[k].y, translations[k].z, 0.0f);
 	}
-	return run_query(w, o, d, out,
+	return run_query(w, o, d, out, false,
 					 [](const RayCtx& R, const float4* dO, const float4* dD, int32_t n, float4* dA, float4* dB,
 						hipStream_t s) { k_ray_cast<<<(n + 127) / 128, 128, 0, s>>>(R, dO, dD, n, dA, dB); });
 }
@@ -687,7 +960,7 @@ int m3dx_world_sphere_cast_batch(m3d_world* world, const m3d_vec3* origins, cons
 		d[k] = make_float4(translations[k].x, translations[k].y, translations[k].z, radii[k]);
 	}
 	int32_t nBodies = w.deviceBodyCount;
-	return run_query(w, o, d, out,
+	return run_query(w, o, d, out, true,
 					 [nBodies](const RayCtx& R, const float4* dO, const float4* dD, int32_t n, float4* dA, float4* dB,
 							   hipStream_t s) {
 						 k_sphere_cast<<<(n + 127) / 128, 128, 0, s>>>(R, nBodies, dO, dD, n, dA, dB);

@@ -129,6 +129,10 @@ bool diff_refused(const World& w);
 // each destroyed body, rank = its destroy's place in the call.
 bool gather_despawn_contacts(World& w, const std::vector<int2>& destroyed, std::vector<int4>& out);
 
+// ---- k_query.hip ----
+// sorted, repeat-free (query << 32 | body) overlap pairs of the query boxes
+bool overlap_query(World& w, const m3d_aabb* boxes, int32_t count, std::vector<uint64_t>& pairs);
+
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);
 

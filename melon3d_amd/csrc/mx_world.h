@@ -159,6 +159,14 @@ struct World
 	// ---- sync state ----
 	bool hostStale = false;          // device bodies newer than the host mirror
 	bool queryRefreshed = false;     // the grid's last refresh was a query's (dt = 0), after the last step moved bodies
+	int32_t gridBodies = 0;          // body slots the grid tiers were last built over
+	// scene queries: bodies the grid may not find (see k_query.hip), query
+	// results before their sort
+	DBuf<int32_t> qExtra;
+	DBuf<uint32_t> qFlag, qOffs;
+	DBuf<uint32_t> qUFlag, qUOffs; // repeat removal (qFlag, qOffs hold the extra list)
+	DBuf<uint64_t> qPairs, qPairsAlt;
+	DBuf<uint32_t> qVals, qValsAlt;
 	std::vector<int32_t> dirtyList;  // bodies to upload at the next step
 	std::vector<uint8_t> dirtyFlag;
 	bool jointsDirty = true;         // joint table to upload

@@ -118,6 +118,7 @@ bool refresh_broadphase(World& w, float dt)
 		}
 		w.gridDirty = false;
 		w.queryRefreshed = dt == 0.0f;
+		w.gridBodies = w.deviceBodyCount;
 		return true;
 	}
 	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
@@ -138,6 +139,7 @@ bool refresh_broadphase(World& w, float dt)
 	}
 	w.gridDirty = false;
 	w.queryRefreshed = dt == 0.0f;
+	w.gridBodies = w.deviceBodyCount;
 	return true;
 }
 

@@ -156,6 +156,15 @@ World::~World()
 	csrVals.release();
 	csrValsAlt.release();
 	dSidecar.release();
+	qExtra.release();
+	qFlag.release();
+	qOffs.release();
+	qUFlag.release();
+	qUOffs.release();
+	qPairs.release();
+	qPairsAlt.release();
+	qVals.release();
+	qValsAlt.release();
 	respRank.release();
 	respList.release();
 	respOut.release();
@@ -1909,14 +1918,16 @@ void m3d_world_overlap_aabb(m3d_world* world, m3d_aabb aabb, m3d_overlap_fn fn, 
 	if (fn == NULL)
 		return;
 	World& w = world->w;
-	ensure_host(w);
-	for (int32_t bi = 0; bi < (int32_t)w.bodies.size(); ++bi)
+	// the matching bodies found on the device, then handed to the callback
+	// in ascending index as the reference's scan does; a body the callback
+	// destroyed before its turn is skipped, as the scan's in-use test would
+	std::vector<uint64_t> pairs;
+	if (!overlap_query(w, &aabb, 1, pairs))
+		return;
+	for (uint64_t p : pairs)
 	{
-		const HostBody& b = w.bodies[bi];
-		if (!b.inUse)
-			continue;
-		m3d_transform xf = { b.position, b.rotation };
-		if (!m3d_aabb_overlap(aabb, shape_aabb(b.shape, xf)))
+		int32_t bi = (int32_t)(uint32_t)p;
+		if (bi >= (int32_t)w.bodies.size() || !w.bodies[bi].inUse)
 			continue;
 		if (!fn(make_id(w, bi), context))
 			return;

@@ -546,6 +546,35 @@ int m3dx_world_ray_cast_batch(m3d_world*, const m3d_vec3*, const m3d_vec3*, cons
 {
     return -1;
 }
+/* the CPU engine's overlap queries are its callback query, one per box */
+int m3dx_world_overlap_aabb_batch(m3d_world* w, const m3d_aabb* boxes, int32_t count, int32_t* offsets,
+                                  m3d_body_id* out, int32_t cap)
+{
+    if (w == nullptr || count < 0 || (count > 0 && (boxes == nullptr || offsets == nullptr)) || cap < 0 ||
+        (cap > 0 && out == nullptr))
+        return -1;
+    struct Sink
+    {
+        m3d_body_id* out;
+        int32_t cap, n;
+    } sink = { out, cap, 0 };
+    for (int32_t k = 0; k < count; ++k)
+    {
+        offsets[k] = sink.n;
+        m3d_world_overlap_aabb(
+            w, boxes[k],
+            [](m3d_body_id id, void* ctx) {
+                Sink* s = (Sink*)ctx;
+                if (s->n < s->cap)
+                    s->out[s->n] = id;
+                ++s->n;
+                return true;
+            },
+            &sink);
+    }
+    offsets[count] = sink.n;
+    return sink.n;
+}
 /* the CPU engine's transforms are its getter, one body at a time */
 int m3dx_world_transforms(m3d_world* w, const m3d_body_id* ids, int32_t count, m3d_transform* out)
 {
