@@ -1,16 +1,11 @@
-// Island solve kernels. Awake islands take one of two paths:
-//
-//   joint islands   one lane runs the reference's serial island schedule,
-//                   joints included (src/xpbd.cpp:898-974, :1167-1369);
-//   all others      together, on a cooperative grid. Each contact gets a
-//                   slot: colored islands (>= 300 contacts) their greedy
-//                   first-fit color 0..7 (src/xpbd.cpp:1179-1235), then
-//                   8 + the serial-order level of their overflow pass; other
-//                   islands 8 + the serial-order level of their contacts.
-//                   Colors and levels are least fixpoints of their
-//                   recurrences, iterated in place until a round changes
-//                   nothing; then the substep loop sweeps the slots with a
-//                   grid barrier between slots.
+// Island solve kernels. Every awake island is solved on one cooperative
+// grid. Each contact gets a slot: colored islands (>= 300 contacts) their
+// greedy first-fit color 0..7 (src/xpbd.cpp:1179-1235), then 8 + the
+// serial-order level of their overflow pass; other islands 8 + the
+// serial-order level of their contacts. Colors and levels are least
+// fixpoints of their recurrences, iterated in place until a round changes
+// nothing; then the substep loop sweeps the slots with a grid barrier
+// between slots.
 //
 // A slot never writes one body twice, so each slot is a plain parallel
 // pass, and every body sees its constraints in the reference's order:
@@ -19,6 +14,15 @@
 // islands in serial contact order (serial arithmetic). Islands are
 // body-disjoint, so sweeping all of them slot by slot together changes
 // nothing for any one of them.
+//
+// Joints (src/xpbd.cpp:413-581) run where the reference runs them in every
+// island (:913-921, :938-941, :966-969 serial; :1288-1291, :1364-1367
+// colored): prepared once per step after the island's inverse inertias are
+// refreshed, their impulses cleared each substep, solved before the
+// contacts in every position iteration and after them in the velocity pass.
+// Their serial order is split into levels (a joint's level is one more than
+// the highest level of the earlier joints on one of its dynamic bodies), one
+// launch per level, so joints of one level touch disjoint bodies.
 #include "mx_steps.h"
 #include "mx_wave.h"
 
@@ -229,155 +233,56 @@ struct IslandCtx
 	DJoint* J;
 	int32_t nJ;
 	const int32_t* jointIsland;
-	uint32_t* colorMask; // per body (joint-island lanes)
 	int32_t* contactColor;
 	m3d_vec3 gravity;
 	float invH;
 	int32_t iterations, substeps, I, NB, NC;
 };
 
-// ------------------------------------------------------ joint islands
+// ------------------------------------------------------------- joints
 
-__global__ void k_solve_joint_islands(IslandCtx X)
+// Bodies of awake islands with joints: world inverse inertia refreshed once
+// per step before the joints are prepared (src/xpbd.cpp:909-921).
+__global__ void k_joint_island_inertia(IslandCtx X)
 {
-	int32_t isl = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (isl >= X.I)
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= X.NB)
 		return;
-	uint32_t m = X.mode[isl];
-	if ((m & (IM_AWAKE | IM_JOINT)) != (IM_AWAKE | IM_JOINT))
+	int32_t b = (int32_t)X.bodyList[k];
+	int32_t isl = X.islandOf[b];
+	if (isl >= 0 && (X.mode[isl] & (IM_AWAKE | IM_JOINT)) == (IM_AWAKE | IM_JOINT))
+		store_iw(X.P.B, b, inv_inertia_world(f4_q(X.P.B.rot[b]), f4_v3(X.P.B.invIL[b])));
+}
+
+// op 0: PrepareJoint of every joint of an awake island (each reads only its
+// own bodies); op 3: the substep's impulse reset.
+__global__ void k_joints_all(IslandCtx X, int op)
+{
+	int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j >= X.nJ || X.jointIsland[j] < 0)
 		return;
-	const PassCtx& P = X.P;
-	uint32_t b0 = X.bodyStart[isl], b1 = X.bodyStart[isl + 1];
-	uint32_t c0 = X.contactStart[isl], c1 = X.contactStart[isl + 1];
-	bool colored = (int32_t)(c1 - c0) >= kBigIslandContacts;
-
-	for (uint32_t k = b0; k < b1; ++k)
+	if (op == 0)
 	{
-		int32_t bi = (int32_t)X.bodyList[k];
-		store_iw(P.B, bi, inv_inertia_world(f4_q(P.B.rot[bi]), f4_v3(P.B.invIL[bi])));
+		pass_joint(X.P, X.J[j], 0);
 	}
-	for (int32_t j = 0; j < X.nJ; ++j)
-		if (X.jointIsland[j] == isl)
-			pass_joint(P, X.J[j], 0);
-
-	int32_t colorCount = 0;
-	if (colored)
+	else
 	{
-		for (uint32_t k = b0; k < b1; ++k)
-			X.colorMask[X.bodyList[k]] = 0;
-		for (uint32_t k = c0; k < c1; ++k)
-		{
-			int32_t c = (int32_t)X.contactList[k];
-			int2 ab = P.C.bodies[c];
-			bool dA = P.B.pos[ab.x].w > 0.0f, dB = P.B.pos[ab.y].w > 0.0f;
-			uint32_t used = (dA ? X.colorMask[ab.x] : 0u) | (dB ? X.colorMask[ab.y] : 0u);
-			int32_t color = (int32_t)__builtin_ctz(~used);
-			if (color >= kMaxGraphColors)
-			{
-				X.contactColor[c] = kMaxGraphColors;
-				continue;
-			}
-			X.contactColor[c] = color;
-			if (dA)
-				X.colorMask[ab.x] |= 1u << color;
-			if (dB)
-				X.colorMask[ab.y] |= 1u << color;
-			colorCount = color + 1 > colorCount ? color + 1 : colorCount;
-		}
+		X.J[j].lambda = 0.0f;
+		X.J[j].motorLambda = 0.0f;
 	}
+}
 
-	for (int sub = 0; sub < X.substeps; ++sub)
-	{
-		for (uint32_t k = b0; k < b1; ++k)
-			integrate_body(P.B, (int32_t)X.bodyList[k], P.h, X.gravity);
-		for (uint32_t k = c0; k < c1; ++k)
-			prepare_contact(P.B, P.C, (int32_t)X.contactList[k], P.h);
-		for (int32_t j = 0; j < X.nJ; ++j)
-		{
-			if (X.jointIsland[j] == isl)
-			{
-				X.J[j].lambda = 0.0f;
-				X.J[j].motorLambda = 0.0f;
-			}
-		}
-		for (int it = 0; it < X.iterations; ++it)
-		{
-			for (int32_t j = 0; j < X.nJ; ++j)
-				if (X.jointIsland[j] == isl)
-					pass_joint(P, X.J[j], 1);
-			if (!colored)
-			{
-				for (uint32_t k = c0; k < c1; ++k)
-					pass_normals_serial(P, (int32_t)X.contactList[k]);
-				continue;
-			}
-			for (int32_t color = 0; color < colorCount; ++color)
-			{
-				int32_t lane[8];
-				int32_t nl = 0;
-				for (uint32_t k = c0; k <= c1; ++k)
-				{
-					bool end = k == c1;
-					if (!end)
-					{
-						int32_t c = (int32_t)X.contactList[k];
-						if (X.contactColor[c] != color)
-							continue;
-						lane[nl++] = c;
-					}
-					if (nl == 8 || (end && nl > 0))
-					{
-						int32_t mp = 0;
-						for (int l = 0; l < nl; ++l)
-						{
-							int32_t pc = f2i(P.C.normal[lane[l]].w);
-							mp = pc > mp ? pc : mp;
-						}
-						for (int l = 0; l < nl; ++l)
-							pass_normals_packet(P, lane[l], mp);
-						nl = 0;
-					}
-				}
-			}
-			for (uint32_t k = c0; k < c1; ++k)
-			{
-				int32_t c = (int32_t)X.contactList[k];
-				if (X.contactColor[c] == kMaxGraphColors)
-					pass_normals_serial(P, c);
-			}
-		}
-		for (int32_t color = 0; color <= kMaxGraphColors; ++color)
-		{
-			if (colored && color == colorCount && color < kMaxGraphColors)
-				color = kMaxGraphColors;
-			for (uint32_t k = c0; k < c1; ++k)
-			{
-				int32_t c = (int32_t)X.contactList[k];
-				if (!colored || X.contactColor[c] == color)
-					pass_friction(P, c);
-			}
-			if (!colored)
-				break;
-		}
-		for (uint32_t k = b0; k < b1; ++k)
-			reconstruct_velocity(P.B, (int32_t)X.bodyList[k], X.invH);
-		for (int32_t color = 0; color <= kMaxGraphColors; ++color)
-		{
-			if (colored && color == colorCount && color < kMaxGraphColors)
-				color = kMaxGraphColors;
-			for (uint32_t k = c0; k < c1; ++k)
-			{
-				int32_t c = (int32_t)X.contactList[k];
-				if (!colored || X.contactColor[c] == color)
-					pass_velocity(P, c);
-			}
-			if (!colored)
-				break;
-		}
-		for (int32_t j = 0; j < X.nJ; ++j)
-			if (X.jointIsland[j] == isl)
-				pass_joint(P, X.J[j], 2);
-	}
+// One level of the joints' serial order: op 1 the position solve, 2 the
+// velocity solve.
+__global__ void k_joint_level(IslandCtx X, const int32_t* order, int32_t begin, int32_t end, int op)
+{
+	int32_t k = begin + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= end)
+		return;
+	int32_t j = order[k];
+	if (X.jointIsland[j] < 0)
+		return;
+	pass_joint(X.P, X.J[j], op);
 }
 
 // ------------------------------------------------------ giant islands
@@ -404,7 +309,7 @@ struct GiantCtx
 
 __device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
 {
-	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & (IM_GRID | IM_JOINT)) == IM_GRID;
+	return (G.X.mode[(int32_t)G.X.entryIsland[k]] & IM_GRID) != 0;
 }
 
 // Barrier of a co-resident grid, in two levels: workgroup b arrives on the
@@ -699,7 +604,7 @@ __global__ void k_giant_adj(IslandCtx X, int4* pred, uint8_t* colored, int32_t* 
 	if (k >= X.NC)
 		return;
 	uint32_t m = X.mode[(int32_t)X.entryIsland[k]];
-	bool g = (m & (IM_GRID | IM_JOINT)) == IM_GRID;
+	bool g = (m & IM_GRID) != 0;
 	bool col = g && (m & IM_COLORED);
 	int2 ab = X.P.C.bodies[X.contactList[k]];
 	bool da = g && X.P.B.pos[ab.x].w > 0.0f;
@@ -1019,7 +924,7 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 // holds only that pass's registers).
 __device__ __forceinline__ bool giant_body(const IslandCtx& X, int32_t b)
 {
-	return (X.mode[X.islandOf[b]] & (IM_GRID | IM_JOINT)) == IM_GRID;
+	return (X.mode[X.islandOf[b]] & IM_GRID) != 0;
 }
 
 __global__ void k_giant_bodies(GiantSolveCtx S, int op)
@@ -1413,7 +1318,6 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	X.J = w.djoints.p;
 	X.nJ = w.djointCount;
 	X.jointIsland = L.jointIsland;
-	X.colorMask = nullptr;
 	X.contactColor = w.contactColor.p;
 	X.gravity = w.def.gravity;
 	X.invH = X.P.h > 0.0f ? 1.0f / X.P.h : 0.0f;
@@ -1423,29 +1327,52 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	X.NB = L.NB;
 	X.NC = L.NC;
 
-	if (w.djointCount > 0)
-	{
-		if (!w.u32a.ensure((size_t)n + 1))
-			return false;
-		X.colorMask = w.u32a.p;
-		w.mark_begin(M3DX_K_JOINT_ISLANDS);
-		k_solve_joint_islands<<<gridk(L.I), 256, 0, s>>>(X);
-		w.mark_end();
-	}
 	if (L.NB == 0)
 		return true;
+	// joints: levels of their serial order (joint_levels, step.cpp)
+	const bool joints = w.djointCount > 0 && w.jointLevelStart.size() > 1;
+	auto joint_levels = [&](int op) {
+		w.mark_begin(M3DX_K_JOINT_ISLANDS);
+		for (size_t l = 0; l + 1 < w.jointLevelStart.size(); ++l)
+		{
+			int32_t b0 = w.jointLevelStart[l], b1 = w.jointLevelStart[l + 1];
+			k_joint_level<<<gridk(b1 - b0), 256, 0, s>>>(X, w.jointOrder.p, b0, b1, op);
+		}
+		w.mark_end();
+	};
+	auto joint_reset = [&]() {
+		if (joints)
+			k_joints_all<<<gridk(w.djointCount), 256, 0, s>>>(X, 3);
+	};
+	if (joints)
+	{
+		w.mark_begin(M3DX_K_JOINT_ISLANDS);
+		k_joint_island_inertia<<<gridk(L.NB), 256, 0, s>>>(X);
+		k_joints_all<<<gridk(w.djointCount), 256, 0, s>>>(X, 0);
+		w.mark_end();
+	}
 
 	GiantSolveCtx S;
 	S.X = X;
 	if (L.NC == 0)
 	{
-		// no contacts: bodies only (integrate, reconstruct)
+		// no contacts: bodies and joints only
 		for (int sub = 0; sub < substeps; ++sub)
 		{
 			w.mark_begin(M3DX_K_GIANT_BODIES);
 			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
+			w.mark_end();
+			if (joints)
+			{
+				joint_reset();
+				for (int it = 0; it < X.iterations; ++it)
+					joint_levels(1);
+			}
+			w.mark_begin(M3DX_K_GIANT_BODIES);
 			k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
 			w.mark_end();
+			if (joints)
+				joint_levels(2);
 		}
 		return true;
 	}
@@ -1574,9 +1501,25 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		if (!diff_record_substep(w, sub))
 			return false;
 		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
-		w.mark_begin(M3DX_K_GIANT_NORMALS);
-		bool ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
-		w.mark_end();
+		bool ok = true;
+		if (joints)
+		{
+			// each position iteration: the joints, then the contacts
+			joint_reset();
+			for (int it = 0; it < X.iterations && ok; ++it)
+			{
+				joint_levels(1);
+				w.mark_begin(M3DX_K_GIANT_NORMALS);
+				ok = coop((const void*)k_giant_pass<NormalsPass>, 1);
+				w.mark_end();
+			}
+		}
+		else
+		{
+			w.mark_begin(M3DX_K_GIANT_NORMALS);
+			ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
+			w.mark_end();
+		}
 		k_giant_lambda_back<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_FRICTION);
 		ok = ok && coop((const void*)k_giant_pass<FrictionPass>, 1);
@@ -1591,6 +1534,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		w.mark_end();
 		if (!ok)
 			return false;
+		if (joints)
+			joint_levels(2);
 	}
 	return true;
 	};

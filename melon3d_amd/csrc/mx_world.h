@@ -225,6 +225,10 @@ struct World
 	int32_t contactCount = 0;
 	DBuf<DJoint> djoints;
 	int32_t djointCount = 0;
+	// joints in levels of their serial order (index order): level l is
+	// jointOrder[jointLevelStart[l] .. jointLevelStart[l + 1])
+	std::vector<int32_t> jointLevelStart;
+	DBuf<int32_t> jointOrder;
 	DBuf<uint64_t> jointedKeys;
 	int32_t jointedCount = 0;
 

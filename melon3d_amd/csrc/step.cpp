@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <unordered_map>
 #include <string.h>
 
 namespace mx
@@ -18,6 +19,54 @@ using Clock = std::chrono::steady_clock;
 static float ms_between(Clock::time_point a, Clock::time_point b)
 {
 	return std::chrono::duration<float, std::milli>(b - a).count();
+}
+
+// Levels of the joints' serial order (index order, src/xpbd.cpp:913-969):
+// a joint's level is one more than the highest level of the earlier joints
+// on one of its dynamic bodies (only those are written), so the joints of
+// one level touch disjoint bodies and running the levels in turn performs
+// every body's joint updates in the serial order.
+static bool joint_levels(World& w)
+{
+	w.jointLevelStart.clear();
+	std::vector<int32_t> level(w.joints.size(), -1);
+	std::unordered_map<int32_t, int32_t> last; // dynamic body -> level of its latest joint
+	int32_t top = -1;
+	for (size_t j = 0; j < w.joints.size(); ++j)
+	{
+		const DJoint& jt = w.joints[j];
+		if (!jt.inUse)
+			continue;
+		const int32_t ends[2] = { jt.bodyA, jt.bodyB };
+		int32_t lv = 0;
+		for (int32_t b : ends)
+		{
+			auto it = w.bodies[b].type == M3D_BODY_DYNAMIC ? last.find(b) : last.end();
+			if (it != last.end())
+				lv = std::max(lv, it->second + 1);
+		}
+		for (int32_t b : ends)
+			if (w.bodies[b].type == M3D_BODY_DYNAMIC)
+				last[b] = lv;
+		level[j] = lv;
+		top = std::max(top, lv);
+	}
+	if (top < 0)
+		return true;
+	w.jointLevelStart.assign((size_t)top + 2, 0);
+	for (int32_t lv : level)
+		if (lv >= 0)
+			++w.jointLevelStart[(size_t)lv + 1];
+	for (int32_t l = 0; l <= top; ++l)
+		w.jointLevelStart[(size_t)l + 1] += w.jointLevelStart[(size_t)l];
+	std::vector<int32_t> order((size_t)w.jointLevelStart.back());
+	std::vector<int32_t> fill(w.jointLevelStart.begin(), w.jointLevelStart.end() - 1);
+	for (size_t j = 0; j < level.size(); ++j)
+		if (level[j] >= 0)
+			order[(size_t)fill[(size_t)level[j]]++] = (int32_t)j;
+	return w.jointOrder.ensure(order.size()) &&
+		   hip_ok(hipMemcpy(w.jointOrder.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+				  "joint levels");
 }
 
 static bool upload_joints(World& w)
@@ -34,6 +83,8 @@ static bool upload_joints(World& w)
 					"joint upload"))
 			return false;
 	}
+	if (!joint_levels(w))
+		return false;
 	w.jointsDirty = false;
 	return true;
 }

@@ -76,6 +76,7 @@ World::~World()
 	subWork.release();
 	newKeys.release();
 	djoints.release();
+	jointOrder.release();
 	jointedKeys.release();
 	activeKeys.release();
 	stableKeys.release();

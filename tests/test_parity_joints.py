@@ -143,11 +143,44 @@ def kinematic_scene(L, w):
     return kin
 
 
-SCENES = {"joints": joint_scene, "kinematic": kinematic_scene}
+def colored_joint_scene(L, w):
+    """A 15 x 15 raft of touching boxes (test/test_main.cpp:960-985: one
+    island of ~650 contacts, solved with the colored schedule) with boxes
+    ball-jointed across the raft and a hinged chain draped over it, so one
+    colored island holds joints too."""
+    _ground(L, w)
+    raft = [scenes.add_box(L, w, (1.0 * gx - 7.0, 0.5, 1.0 * gz - 7.0), (0.501, 0.5, 0.501))
+            for gx in range(15) for gz in range(15)]
+    joints = []
+    for k in range(0, 200, 13):
+        if k % 15 == 14:
+            continue  # the raft's next box in z is in the next row
+        bj = m3d_ball_joint_def()
+        bj.bodyA, bj.bodyB = raft[k], raft[k + 1]
+        bj.localAnchorA = m3d_vec3(0.0, 0.5, 0.5)
+        bj.localAnchorB = m3d_vec3(0.0, 0.5, -0.5)
+        joints.append(L.m3d_joint_create_ball(w, C.byref(bj)))
+    prev = None
+    for i in range(10):
+        link = scenes.add_box(L, w, (-4.0 + 0.9 * i, 1.4 + 0.05 * i, 0.3), (0.4, 0.15, 0.15))
+        if prev is not None:
+            hd = L.m3d_hinge_joint_def_default()
+            hd.bodyA, hd.bodyB = prev, link
+            hd.localAnchorA = m3d_vec3(0.45, 0.0, 0.0)
+            hd.localAnchorB = m3d_vec3(-0.45, 0.0, 0.0)
+            hd.localAxisA = m3d_vec3(0.0, 0.0, 1.0)
+            joints.append(L.m3d_joint_create_hinge(w, C.byref(hd)))
+        prev = link
+    for i in range(6):
+        scenes.add_box(L, w, (2.0 * i - 5.0, 4.0 + 0.7 * i, 0.5 * i - 1.0), (0.45, 0.45, 0.45))
+    return joints
+
+
+SCENES = {"joints": joint_scene, "kinematic": kinematic_scene, "colored_joints": colored_joint_scene}
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scene", ["joints", "kinematic"])
+@pytest.mark.parametrize("scene", ["joints", "kinematic", "colored_joints"])
 @pytest.mark.parametrize("at", [0, 30, 90, 180])
 def test_one_step(ref, dev, scene, at):
     rw = ref.create_world()
@@ -172,7 +205,7 @@ def test_one_step(ref, dev, scene, at):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scene", ["joints", "kinematic"])
+@pytest.mark.parametrize("scene", ["joints", "kinematic", "colored_joints"])
 def test_lockstep(ref, dev, scene):
     rw, dw = ref.create_world(), dev.create_world()
     try:
@@ -187,7 +220,9 @@ def test_lockstep(ref, dev, scene):
             d = state_diff(a, b)
             assert d["pos"] <= TOL and d["rot"] <= TOL, f"step {step}: {d}"
             assert d["awake_equal"], f"step {step}: sleep state"
-            if scene == "joints":
+            if scene == "colored_joints" and step == 5:
+                assert dev.stats(dw).coloredIslands >= 1, "colored path not taken"
+            if scene != "kinematic":
                 angles_r = [ref.m3d_joint_hinge_angle(rw, j) for j in jr]
                 angles_d = [dev.m3d_joint_hinge_angle(dw, j) for j in jd]
                 assert max(abs(x - y) for x, y in zip(angles_r, angles_d)) <= TOL, f"step {step}: hinge angles"
@@ -198,3 +233,41 @@ def test_lockstep(ref, dev, scene):
     finally:
         ref.m3d_world_destroy(rw)
         dev.m3d_world_destroy(dw)
+
+
+@pytest.mark.gpu
+def test_batch_with_joints(ref, dev):
+    """Four sub-worlds of the jointed scene in one batch against four
+    reference worlds (joints are solved per sub-world, on the same grid)."""
+    from parity import bodies_array, rot_angle
+    import numpy as np
+
+    S = 4
+    bw = dev.m3dx_batch_create(C.byref(dev.m3d_world_def_default()), S)
+    assert bw
+    refs = []
+    try:
+        for s_ in range(S):
+            assert dev.m3dx_batch_select(bw, s_) == 0
+            joint_scene(dev, bw)
+            rw = ref.create_world()
+            joint_scene(ref, rw)
+            refs.append(rw)
+        for step in range(120):
+            dev.m3d_world_step(bw, DT, 4)
+            for rw in refs:
+                ref.m3d_world_step(rw, DT, 4)
+        assert dev.m3dx_step_failures(bw) == 0, dev.last_error()
+        snap = dev.export_state(bw)
+        pos, rot, *_ = bodies_array(snap)
+        for s_ in range(S):
+            first, n = C.c_int32(0), C.c_int32(0)
+            assert dev.m3dx_batch_body_range(bw, s_, C.byref(first), C.byref(n)) == 0
+            rp, rr, *_ = bodies_array(ref.export_state(refs[s_]))
+            sl = slice(first.value, first.value + n.value)
+            assert float(np.abs(pos[sl] - rp).max()) <= TOL, s_
+            assert float(rot_angle(rot[sl], rr).max()) <= TOL, s_
+    finally:
+        dev.m3d_world_destroy(bw)
+        for rw in refs:
+            ref.m3d_world_destroy(rw)
