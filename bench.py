@@ -571,14 +571,21 @@ def main():
                     help="only copy 256 MiB device to device (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
+        # 256 MiB device-to-device copies (the runtime's copy kernel): the
+        # known bytes that calibrate FETCH_SIZE and WRITE_SIZE
+        import ctypes as C
 
-        a = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = 256 << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+        hip.hipMemset(a, 1, C.c_size_t(nbytes))
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": 256 << 20}))
+            hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"copy_probe_bytes": nbytes}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "

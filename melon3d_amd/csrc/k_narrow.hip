@@ -40,9 +40,7 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 
 	if (cf & CF_TUNNEL_GUARD)
 	{
-		Manifold m;
-		load_manifold(C, c, m);
-		if (tracked_min_sep(pa, qa, pb, qb, m) < -kLinearSlop)
+		if (tracked_min_sep(pa, qa, pb, qb, StoredManifold(C, c)) < -kLinearSlop)
 		{
 			trans[c] = 0;
 			return;
@@ -59,9 +57,10 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 	if (regen)
 	{
 		wave_inc(&counters[CT_REGEN]);
-		Manifold old;
-		load_manifold(C, c, old);
-		Manifold m = old;
+		// the old manifold is read where it lies; the new one is built here
+		// (slots beyond its count are never stored and keep their contents)
+		const StoredManifold old(C, c);
+		Manifold m;
 		Shape sA = load_shape(B, ia), sB = load_shape(B, ib);
 		float relSpeed = m3d_length(m3d_sub(f4_v3(B.vel[ia]), f4_v3(B.vel[ib])));
 		float margin = kSpeculativeBase + dt * relSpeed;

@@ -737,60 +737,56 @@ MX_HD void collide_shapes(const Shape& sA, m3d_transform xfA, const Shape& sB, m
 {
 	m->pointCount = 0;
 	m->normal = m3d_v3(0.0f, 1.0f, 0.0f);
+	// canonical order (type A <= type B), then flipped back in place: the
+	// same points as the reference's flipped copy (src/collide.cpp:818-836),
+	// and slots beyond the count keep their old values either way
 	bool flip = sA.type > sB.type;
 	const Shape& s0 = flip ? sB : sA;
 	const Shape& s1 = flip ? sA : sB;
 	m3d_transform x0 = flip ? xfB : xfA;
 	m3d_transform x1 = flip ? xfA : xfB;
-	Manifold tmp;
-	Manifold* out = flip ? &tmp : m;
-	if (flip)
-	{
-		tmp.pointCount = 0;
-		tmp.normal = m3d_v3(0.0f, 1.0f, 0.0f);
-	}
 	switch (s0.type)
 	{
 		case M3D_SHAPE_SPHERE:
 			if (s1.type == M3D_SHAPE_SPHERE)
-				collide_spheres(s0, x0, s1, x1, margin, out);
+				collide_spheres(s0, x0, s1, x1, margin, m);
 			else if (s1.type == M3D_SHAPE_CAPSULE)
-				collide_sphere_capsule(s0, x0, s1, x1, margin, out);
+				collide_sphere_capsule(s0, x0, s1, x1, margin, m);
 			else
-				collide_sphere_box(s0, x0, s1, x1, margin, out);
+				collide_sphere_box(s0, x0, s1, x1, margin, m);
 			break;
 		case M3D_SHAPE_CAPSULE:
 			if (s1.type == M3D_SHAPE_CAPSULE)
-				collide_capsules(s0, x0, s1, x1, margin, out);
+				collide_capsules(s0, x0, s1, x1, margin, m);
 			else if (s1.type == M3D_SHAPE_BOX)
-				collide_capsule_box(s0, x0, s1, x1, margin, out);
+				collide_capsule_box(s0, x0, s1, x1, margin, m);
 			break;
 		default:
-			collide_boxes(s0, x0, s1, x1, margin, out);
+			collide_boxes(s0, x0, s1, x1, margin, m);
 			break;
 	}
 	if (!flip)
 	{
 		return;
 	}
-	m->pointCount = tmp.pointCount;
-	m->normal = m3d_neg(tmp.normal);
-	for (int i = 0; i < tmp.pointCount; ++i)
+	m->normal = m3d_neg(m->normal);
+	for (int i = 0; i < m->pointCount; ++i)
 	{
-		MPoint& dst = m->points[i];
-		const MPoint& src = tmp.points[i];
-		dst = src;
-		dst.localAnchorA = src.localAnchorB;
-		dst.localAnchorB = src.localAnchorA;
-		dst.frictionAnchorA = src.frictionAnchorB;
-		dst.frictionAnchorB = src.frictionAnchorA;
+		MPoint& p = m->points[i];
+		m3d_vec3 t = p.localAnchorA;
+		p.localAnchorA = p.localAnchorB;
+		p.localAnchorB = t;
+		t = p.frictionAnchorA;
+		p.frictionAnchorA = p.frictionAnchorB;
+		p.frictionAnchorB = t;
 	}
 }
 
 // Stiction anchors survive regeneration for box-box pairs while the stored
 // pair stretched < 2 cm and the feature drifted < 5 cm; the slip direction
 // carries over for every shape. src/collide.cpp:865-913
-MX_HD void inherit_friction_anchors(const Manifold& old, Manifold& m, m3d_vec3 posA, m3d_quat rotA, m3d_vec3 posB,
+template <class OldManifold>
+MX_HD void inherit_friction_anchors(const OldManifold& old, Manifold& m, m3d_vec3 posA, m3d_quat rotA, m3d_vec3 posB,
 									m3d_quat rotB, bool boxBox)
 {
 	const float slipRelease2 = 0.02f * 0.02f;
@@ -801,7 +797,7 @@ MX_HD void inherit_friction_anchors(const Manifold& old, Manifold& m, m3d_vec3 p
 		MPoint& mp = m.points[i];
 		for (int j = 0; j < old.pointCount; ++j)
 		{
-			const MPoint& op = old.points[j];
+			const MPoint op = old.point(j);
 			if (op.id != mp.id)
 			{
 				continue;

@@ -145,6 +145,7 @@ struct Manifold
 	m3d_vec3 normal;
 	int32_t pointCount;
 	MPoint points[4];
+	MX_HD const MPoint& point(int j) const { return points[j]; }
 };
 
 // ----------------------------------------------------- device layouts
@@ -286,6 +287,41 @@ MX_HD void load_manifold(const DContacts& C, int32_t c, Manifold& m)
 		mp.slipDir = f4_v3(sl);
 	}
 }
+
+// The stored manifold of contact c read where it lies: normal and count at
+// once, a point's fields when point(j) is called (no local copy of the four
+// points, which a kernel would have to keep in scratch memory).
+struct StoredManifold
+{
+	const DContacts* C;
+	int32_t c;
+	m3d_vec3 normal;
+	int32_t pointCount;
+	MX_HD StoredManifold(const DContacts& cs, int32_t ci) : C(&cs), c(ci)
+	{
+		float4 n = cs.normal[ci];
+		normal = f4_v3(n);
+		pointCount = f2i(n.w);
+	}
+	MX_HD MPoint point(int j) const
+	{
+		int32_t p = pidx(*C, c, j);
+		float4 a = C->pA[p], b = C->pB[p], ja = C->pJA[p], jb = C->pJB[p], ra = C->pRA[p], rb = C->pRB[p];
+		float4 fa = C->pFA[p], fb = C->pFB[p], sl = C->pSlip[p];
+		MPoint mp;
+		mp.localAnchorA = f4_v3(a);
+		mp.separationOffset = a.w;
+		mp.localAnchorB = f4_v3(b);
+		mp.lambdaN = ja.w;
+		mp.lambdaT = jb.w;
+		mp.separation = ra.w;
+		mp.id = f2u(rb.w);
+		mp.frictionAnchorA = f4_v3(fa);
+		mp.frictionAnchorB = f4_v3(fb);
+		mp.slipDir = f4_v3(sl);
+		return mp;
+	}
+};
 
 // Store the persistent fields of a manifold's live points. Slots at or
 // beyond pointCount keep their previous contents, as the reference's

@@ -146,7 +146,8 @@ MX_HD m3d_mat3 inv_inertia_world(m3d_quat rot, m3d_vec3 d)
 
 // Anchor-tracked minimum separation at the current poses;
 // src/m3d_internal.h:251-268
-MX_HD float tracked_min_sep(m3d_vec3 pa, m3d_quat qa, m3d_vec3 pb, m3d_quat qb, const Manifold& m)
+template <class M>
+MX_HD float tracked_min_sep(m3d_vec3 pa, m3d_quat qa, m3d_vec3 pb, m3d_quat qb, const M& m)
 {
 	if (m.pointCount == 0)
 	{
@@ -155,7 +156,7 @@ MX_HD float tracked_min_sep(m3d_vec3 pa, m3d_quat qa, m3d_vec3 pb, m3d_quat qb, 
 	float mn = 1e30f;
 	for (int i = 0; i < m.pointCount; ++i)
 	{
-		const MPoint& mp = m.points[i];
+		const MPoint mp = m.point(i);
 		m3d_vec3 rA = m3d_rotate(qa, mp.localAnchorA);
 		m3d_vec3 rB = m3d_rotate(qb, mp.localAnchorB);
 		float sep = mp.separationOffset + m3d_dot(m3d_sub(m3d_add(pb, rB), m3d_add(pa, rA)), m.normal);
@@ -396,9 +397,11 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 	}
 	if (stale)
 	{
-		Manifold old;
-		load_manifold(C, c, old);
-		Manifold m = old; // regenerate in place: stale slots keep old values
+		// the old manifold is read where it lies; the new one is built here
+		// (slots beyond its count are never stored, so they keep their old
+		// contents, as the reference's in-place regeneration leaves them)
+		const StoredManifold old(C, c);
+		Manifold m;
 		Shape sA = load_shape(B, ia), sB = load_shape(B, ib);
 		m3d_transform xfA = { pa, qa }, xfB = { pb, qb };
 		float relSpeed = m3d_length(m3d_sub(f4_v3(B.vel[ia]), f4_v3(B.vel[ib])));
@@ -453,7 +456,10 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 		}
 		if (keepOld)
 		{
-			m = old; // cache pose intentionally not refreshed
+			// the old manifold stays as stored (cache pose intentionally
+			// not refreshed)
+			n = old.normal;
+			count = old.pointCount;
 		}
 		else
 		{
@@ -466,11 +472,11 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 			k[2] = v3_f4(pb, 0.0f);
 			k[3] = q_f4(qb);
 			set_separation_offsets(m, pa, qa, pb, qb);
+			store_manifold(C, c, m);
+			n = m.normal;
+			count = m.pointCount;
 		}
-		store_manifold(C, c, m);
 		C.mat[c] = make_float4(mat.x, mat.y, mat.z, u2f(cf));
-		n = m.normal;
-		count = m.pointCount;
 	}
 
 	// freeze arms, Jacobians and normal mass for this substep
