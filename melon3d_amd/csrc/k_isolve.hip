@@ -1201,9 +1201,10 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 }
 
 // Co-resident grid size of a cooperative kernel (blocks of kGiantThreads),
-// cached per device and kernel.
-template <class K>
-static int coop_blocks(World& w, K kernel)
+// cached per device and kernel: the kernel is a template argument, so each
+// kernel has its own cache (the pass kernels share one function type).
+template <auto kernel>
+static int coop_blocks(World& w)
 {
 	constexpr int kMaxDevices = 64;
 	static std::atomic<int> cache[kMaxDevices];
@@ -1242,11 +1243,11 @@ static int env_blocks(const char* name, int dflt)
 	return v > 0 ? v : dflt;
 }
 
-template <class K>
-static int giant_blocks(World& w, K kernel, int32_t entries, int cap)
+template <auto kernel>
+static int giant_blocks(World& w, int32_t entries, int cap)
 {
 	int b = (int)std::min<int64_t>(((int64_t)entries + kGiantThreads - 1) / kGiantThreads, cap);
-	return std::max(1, std::min(b, coop_blocks(w, kernel)));
+	return std::max(1, std::min(b, coop_blocks<kernel>(w)));
 }
 
 // Captures `body`'s launches on the world's stream into a graph and launches
@@ -1422,7 +1423,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.bitsI = L.bitsI;
 	{
 		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);
-		int blocks = giant_blocks(w, k_giant_order, NC, cap);
+		int blocks = giant_blocks<k_giant_order>(w, NC, cap);
 		void* args[] = { &G };
 		w.mark_begin(M3DX_K_GIANT_ORDER);
 		hipError_t err = hipLaunchCooperativeKernel((const void*)k_giant_order, dim3(blocks), dim3(kGiantThreads), args, 0, s);
@@ -1464,9 +1465,9 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	static const int capEnv = env_blocks("M3D_GIANT_BLOCKS", 0);
 	if (capEnv > 0 && w.tuner.forcedBlocks == 0)
 		w.tuner.forcedBlocks = capEnv;
-	const int coopMax = std::min(coop_blocks(w, k_giant_pass<NormalsPass>),
-								 std::min(coop_blocks(w, k_giant_pass<FrictionPass>),
-										  coop_blocks(w, k_giant_pass<VelocityPass>)));
+	const int coopMax = std::min(coop_blocks<k_giant_pass<NormalsPass>>(w),
+								 std::min(coop_blocks<k_giant_pass<FrictionPass>>(w),
+										  coop_blocks<k_giant_pass<VelocityPass>>(w)));
 	bool measure = false;
 	int blocks = w.tuner_pick(NC, coopMax, measure);
 	w.tuner.lastBlocks = blocks;
