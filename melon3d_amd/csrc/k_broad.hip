@@ -1275,15 +1275,33 @@ bool apply_moves(World& w, const int2* dMoves, int32_t count)
 	return true;
 }
 
-__global__ void k_gather_events(const int2* bodies, const float4* mat, const int32_t* idx, int32_t count, int4* out)
+// count is read on the device (the number of end events of the removal)
+__global__ void k_gather_events(const int2* bodies, const float4* mat, const int32_t* idx, const uint32_t* count,
+								int4* out)
 {
 	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (k >= count)
+	if (k >= (int32_t)*count)
 		return;
 	int32_t c = idx[k];
 	int2 b = bodies[c];
 	uint32_t cf = f2u(mat[c].w);
 	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, c);
+}
+
+static bool ensure_pinned(char*& p, size_t& cap, size_t bytes, const char* what)
+{
+	if (p != nullptr && cap >= bytes)
+		return true;
+	if (p != nullptr)
+		(void)hipHostFree(p);
+	p = nullptr;
+	cap = bytes * 3 / 2 + 4096;
+	if (!hip_ok(hipHostMalloc((void**)&p, cap, hipHostMallocDefault), what))
+	{
+		cap = 0;
+		return false;
+	}
+	return true;
 }
 
 bool remove_separated(World& w)
@@ -1304,6 +1322,20 @@ bool remove_separated(World& w)
 	if (R == 0)
 		return true;
 	int32_t newC = C - R;
+	// One more wait for the rest: the hole/tail/event flags, their counts,
+	// the end events (at most R) and the removal flags of the tail region
+	// [newC, C) all come back together. The pinned block is reused by the
+	// next call only after that call's first wait, which follows every copy
+	// issued here.
+	size_t flagBytes = (size_t)R * sizeof(uint32_t);
+	size_t evBytes = (size_t)R * sizeof(int4);
+	if (!ensure_pinned(w.pinnedRemoval, w.pinnedRemovalCap, flagBytes + evBytes + (size_t)R * sizeof(int32_t),
+					   "removal staging") ||
+		!w.eventIdx.ensure(R) || !w.u64a.ensure((size_t)R * 2))
+		return false;
+	int4* evHost = (int4*)w.pinnedRemoval;
+	uint32_t* flags = (uint32_t*)(w.pinnedRemoval + evBytes);
+	int32_t* order = (int32_t*)(w.pinnedRemoval + evBytes + flagBytes);
 	uint32_t* hole = w.u32b.p;
 	uint32_t* tail = w.u32c.p;
 	uint32_t* ev = w.u32d.p;
@@ -1314,30 +1346,20 @@ bool remove_separated(World& w)
 	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.scanStatus.p, s);
 	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.scanStatus.p, s);
 	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.scanStatus.p, s);
-	if (!sync_counters(w))
+	// end events of touching removed pairs, descending contact order
+	k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
+	k_gather_events<<<grid_for(R), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p,
+												   w.counters.p + CT_END_EVENTS, (int4*)w.u64a.p);
+	if (!hip_ok(hipMemcpyAsync(evHost, w.u64a.p, evBytes, hipMemcpyDeviceToHost, s), "end events") ||
+		!hip_ok(hipMemcpyAsync(flags, rm + newC, flagBytes, hipMemcpyDeviceToHost, s), "removal flags") ||
+		!sync_counters(w))
 		return false;
 	int32_t H = (int32_t)w.hostCounters[CT_HOLES];
 	int32_t NE = (int32_t)w.hostCounters[CT_END_EVENTS];
-	if (H != (int32_t)w.hostCounters[CT_TAILS])
+	if (H != (int32_t)w.hostCounters[CT_TAILS] || NE > R)
 	{
-		set_error("removal bookkeeping mismatch (%d holes, %u tails)", H, w.hostCounters[CT_TAILS]);
+		set_error("removal bookkeeping mismatch (%d holes, %u tails, %d end events)", H, w.hostCounters[CT_TAILS], NE);
 		return false;
-	}
-	// end events of touching removed pairs, descending contact order
-	if (NE > 0)
-	{
-		if (!w.eventIdx.ensure(NE))
-			return false;
-		k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
-	}
-	std::vector<int4> evHost(NE);
-	if (NE > 0)
-	{
-		if (!w.u64a.ensure((size_t)NE * 2))
-			return false;
-		k_gather_events<<<grid_for(NE), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p, NE,
-														 (int4*)w.u64a.p);
-		hipMemcpyAsync(evHost.data(), w.u64a.p, NE * sizeof(int4), hipMemcpyDeviceToHost, s);
 	}
 	if (H > 0)
 	{
@@ -1348,12 +1370,7 @@ bool remove_separated(World& w)
 		// order a replay of that region (R slots) leaves them; the holes below
 		// newC then take them from the top down: the k-th hole (ascending)
 		// receives the k-th of them.
-		std::vector<uint32_t> flags(R);
-		std::vector<int32_t> order(R);
-		if (!w.moves.ensure(H) || !w.tailOrder.ensure(R) ||
-			!hip_ok(hipMemcpyAsync(flags.data(), rm + newC, (size_t)R * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
-					"removal flags") ||
-			!hip_ok(hipStreamSynchronize(s), "removal flags"))
+		if (!w.moves.ensure(H) || !w.tailOrder.ensure(R))
 			return false;
 		for (int32_t i = 0; i < R; ++i)
 			order[i] = newC + i;
@@ -1370,15 +1387,13 @@ bool remove_separated(World& w)
 			set_error("removal replay mismatch (%d survivors, %d holes)", top + 1, H);
 			return false;
 		}
-		if (!hip_ok(hipMemcpyAsync(w.tailOrder.p, order.data(), (size_t)H * sizeof(int32_t), hipMemcpyHostToDevice, s),
+		if (!hip_ok(hipMemcpyAsync(w.tailOrder.p, order, (size_t)H * sizeof(int32_t), hipMemcpyHostToDevice, s),
 					"removal order"))
 			return false;
 		k_make_moves<<<grid_for(C), kBlock, 0, s>>>(hole, holeOff, tail, tailOff, C, w.moves.p);
 		k_set_move_sources<<<grid_for(H), kBlock, 0, s>>>(w.moves.p, w.tailOrder.p, H);
 		k_apply_moves<<<grid_for(4 * (int64_t)H), kBlock, 0, s>>>(contact_view(w), w.moves.p, H);
 	}
-	if (!hip_ok(hipStreamSynchronize(s), "contact removal"))
-		return false;
 	for (int32_t k = NE - 1; k >= 0; --k)
 	{
 		const int4& e = evHost[k];

@@ -118,27 +118,44 @@ __global__ void k_wake_init(const uint32_t* flags, int32_t n, int32_t* wakeAt)
 }
 
 // info.z: +1 begin, +2 begin (sensor), -1/-2 end
-__global__ void k_wake_iter(const int4* info, int32_t T, const uint32_t* flags, int32_t* wakeAt, uint32_t* changed)
+// The wake-on-touch fixpoint in one workgroup: a touching begin t (info.z =
+// 1) with one body awake before t (wake time below t) wakes the other body
+// at t (its wake time lowered to t; static bodies stay). Rounds over every
+// transition until one changes nothing: a wake time only decreases, so the
+// rounds are bounded by the longest wake chain (T + 2 caps them). Wake times
+// are read and lowered at agent scope.
+__global__ void __launch_bounds__(1024) k_wake_fixpoint(const int4* info, const uint32_t* count, const uint32_t* flags,
+														int32_t* wakeAt)
 {
-	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (t >= T)
-		return;
-	int4 e = info[t];
-	if (e.z != 1)
-		return;
-	int32_t a = e.x, b = e.y;
-	int32_t wa = __atomic_load_n(&wakeAt[a], __ATOMIC_RELAXED);
-	int32_t wb = __atomic_load_n(&wakeAt[b], __ATOMIC_RELAXED);
-	bool awA = wa < t, awB = wb < t;
-	if (awA && !awB)
+	__shared__ int changed;
+	const int32_t T = (int32_t)*count;
+	for (int32_t round = 0; round < T + 2; ++round)
 	{
-		if (bf_type(flags[b]) != M3D_BODY_STATIC && atomicMin(&wakeAt[b], t) > t)
-			atomicAdd(changed, 1u);
-	}
-	else if (awB && !awA)
-	{
-		if (bf_type(flags[a]) != M3D_BODY_STATIC && atomicMin(&wakeAt[a], t) > t)
-			atomicAdd(changed, 1u);
+		if (threadIdx.x == 0)
+			changed = 0;
+		__syncthreads();
+		bool mine = false;
+		for (int32_t t = (int32_t)threadIdx.x; t < T; t += (int32_t)blockDim.x)
+		{
+			int4 e = info[t];
+			if (e.z != 1)
+				continue;
+			int32_t a = e.x, b = e.y;
+			int32_t wa = __hip_atomic_load(&wakeAt[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			int32_t wb = __hip_atomic_load(&wakeAt[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			bool awA = wa < t, awB = wb < t;
+			int32_t other = awA && !awB ? b : (awB && !awA ? a : -1);
+			if (other >= 0 && bf_type(flags[other]) != M3D_BODY_STATIC &&
+				__hip_atomic_fetch_min(&wakeAt[other], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > t)
+				mine = true;
+		}
+		if (mine)
+			changed = 1;
+		__syncthreads();
+		bool done = changed == 0;
+		__syncthreads();
+		if (done)
+			break;
 	}
 }
 
@@ -194,16 +211,10 @@ bool narrowphase(World& w, float dt)
 	// wake-on-touch fixpoint
 	int32_t* wakeAt = w.i32c.p;
 	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
-	for (int round = 0; round < 1 << 20; ++round)
-	{
-		hipMemsetAsync(w.counters.p + CT_CHANGED, 0, sizeof(uint32_t), s);
-		k_wake_iter<<<gridn(T), kBlockN, 0, s>>>(info, T, w.bc.flags.p, wakeAt, w.counters.p + CT_CHANGED);
-		if (!sync_counters(w))
-			return false;
-		if (w.hostCounters[CT_CHANGED] == 0)
-			break;
-	}
+	k_wake_fixpoint<<<1, 1024, 0, s>>>(info, w.counters.p + CT_TRANSITIONS, w.bc.flags.p, wakeAt);
 	k_wake_apply<<<gridn(n), kBlockN, 0, s>>>(make_body_view(w), n, wakeAt, w.counters.p);
+	if (!hip_ok(hipStreamSynchronize(s), "transitions"))
+		return false;
 	for (int32_t t = 0; t < T; ++t)
 	{
 		const int4& e = host[t];

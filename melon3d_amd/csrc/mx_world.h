@@ -215,6 +215,13 @@ struct World
 	int4* pinnedResp = nullptr;
 	size_t pinnedRespCap = 0;
 	std::vector<int32_t> respRankHost; // per body slot, -1 between calls
+	// pinned staging of the dirty-body upload, reusable once uploadDone fired
+	char* pinnedUpload = nullptr;
+	size_t pinnedUploadCap = 0;
+	hipEvent_t uploadDone = nullptr;
+	// pinned staging of the contact removal (flags, end events, order)
+	char* pinnedRemoval = nullptr;
+	size_t pinnedRemovalCap = 0;
 
 	// ---- device state ----
 	int32_t deviceBodyCap = 0;  // capacity of the body columns

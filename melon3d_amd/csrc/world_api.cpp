@@ -171,6 +171,12 @@ World::~World()
 	respOut.release();
 	if (pinnedResp != nullptr)
 		(void)hipHostFree(pinnedResp);
+	if (uploadDone != nullptr)
+		(void)hipEventDestroy(uploadDone);
+	if (pinnedUpload != nullptr)
+		(void)hipHostFree(pinnedUpload);
+	if (pinnedRemoval != nullptr)
+		(void)hipHostFree(pinnedRemoval);
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
 }
@@ -423,10 +429,35 @@ bool upload_dirty(World& w)
 	int32_t k = (int32_t)w.dirtyList.size();
 	if (k == 0)
 		return true;
-	std::vector<float4> stage((size_t)k * 18);
-	std::vector<uint32_t> sf(k);
-	std::vector<uint4> sfil(k);
-	std::vector<uint8_t> sbp(k);
+	// one block, laid out as on the device: 18 float4 columns of k, then
+	// filters, flags, slots and tier bits; staged in pinned memory so the
+	// copy and the scatter run behind the host (the stream orders every
+	// later use of the device block; the pinned block is reused only after
+	// its last copy completed)
+	size_t colBytes = (size_t)k * 18 * sizeof(float4);
+	size_t bytes = colBytes + (size_t)k * (sizeof(uint4) + sizeof(uint32_t) + sizeof(int32_t) + 1);
+	if (w.uploadDone != nullptr && !hip_ok(hipEventSynchronize(w.uploadDone), "upload staging"))
+		return false;
+	if (w.pinnedUploadCap < bytes)
+	{
+		if (w.pinnedUpload != nullptr)
+			(void)hipHostFree(w.pinnedUpload);
+		w.pinnedUpload = nullptr;
+		w.pinnedUploadCap = bytes * 3 / 2 + 4096;
+		if (!hip_ok(hipHostMalloc((void**)&w.pinnedUpload, w.pinnedUploadCap, hipHostMallocDefault), "upload staging"))
+		{
+			w.pinnedUploadCap = 0;
+			return false;
+		}
+	}
+	if (w.uploadDone == nullptr && !hip_ok(hipEventCreateWithFlags(&w.uploadDone, hipEventDisableTiming), "upload event"))
+		return false;
+	char* hb = w.pinnedUpload;
+	float4* stage = (float4*)hb;
+	uint4* sfil = (uint4*)(hb + colBytes);
+	uint32_t* sf = (uint32_t*)(sfil + k);
+	int32_t* sidx = (int32_t*)(sf + k);
+	uint8_t* sbp = (uint8_t*)(sidx + k);
 	float4 cols[18];
 	for (int32_t t = 0; t < k; ++t)
 	{
@@ -436,28 +467,22 @@ bool upload_dirty(World& w)
 			stage[(size_t)c * k + t] = cols[c];
 		sf[t] = pack_flags(b);
 		sfil[t] = make_uint4(b.filter.categoryBits, b.filter.maskBits, (uint32_t)b.filter.groupIndex, 0u);
+		sidx[t] = w.dirtyList[t];
 		sbp[t] = b.bp;
 	}
-	size_t bytes = stage.size() * sizeof(float4) + k * (sizeof(uint32_t) + sizeof(uint4) + 1 + sizeof(int32_t)) + 64;
 	if (!w.u64a.ensure(bytes / 8 + 1))
 		return false;
 	char* base = (char*)w.u64a.p;
 	float4* dStage = (float4*)base;
-	uint4* dFil = (uint4*)(base + stage.size() * sizeof(float4));
+	uint4* dFil = (uint4*)(base + colBytes);
 	uint32_t* dFlags = (uint32_t*)(dFil + k);
 	int32_t* dIdx = (int32_t*)(dFlags + k);
 	uint8_t* dBp = (uint8_t*)(dIdx + k);
 	hipStream_t s = w.stream;
-	bool ok = hip_ok(hipMemcpyAsync(dStage, stage.data(), stage.size() * sizeof(float4), hipMemcpyHostToDevice, s),
-					 "upload") &&
-			  hip_ok(hipMemcpyAsync(dFil, sfil.data(), k * sizeof(uint4), hipMemcpyHostToDevice, s), "upload") &&
-			  hip_ok(hipMemcpyAsync(dFlags, sf.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, s), "upload") &&
-			  hip_ok(hipMemcpyAsync(dIdx, w.dirtyList.data(), k * sizeof(int32_t), hipMemcpyHostToDevice, s), "upload") &&
-			  hip_ok(hipMemcpyAsync(dBp, sbp.data(), k, hipMemcpyHostToDevice, s), "upload");
-	if (!ok)
+	if (!hip_ok(hipMemcpyAsync(base, hb, bytes, hipMemcpyHostToDevice, s), "upload"))
 		return false;
 	launch_upload_scatter(w, k, dIdx, dStage, dFlags, dFil, dBp);
-	if (!hip_ok(hipStreamSynchronize(s), "upload scatter"))
+	if (!hip_ok(hipEventRecord(w.uploadDone, s), "upload event"))
 		return false;
 	for (int32_t i : w.dirtyList)
 		w.dirtyFlag[i] = 0;
