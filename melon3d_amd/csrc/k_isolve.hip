@@ -1200,6 +1200,63 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 	}
 }
 
+// The substep's position stage in one cooperative launch: `passes` normal
+// sweeps, their lambdas back to the contacts, the stiction sweep and the
+// velocity reconstruction of the grid bodies, each stage behind a grid
+// barrier (the launch boundaries it replaces cost more than a barrier each).
+// The same work as k_giant_pass<NormalsPass>, k_giant_lambda_back,
+// k_giant_pass<FrictionPass> and k_giant_bodies(op 1) launched in turn.
+__global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx S, int passes)
+{
+	GridSync grid{S.barrier, gridDim.x, 0u};
+	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
+	constexpr int32_t kSlotLds = 2048;
+	__shared__ uint32_t sSlots[kSlotLds];
+	const uint32_t* ss = S.slotStart;
+	if (nSlots + 1 <= kSlotLds)
+	{
+		for (int32_t i = (int32_t)threadIdx.x; i <= nSlots; i += (int32_t)blockDim.x)
+			sSlots[i] = S.slotStart[i];
+		__syncthreads();
+		ss = sSlots;
+	}
+	for (int it = 0; it < passes; ++it)
+		giant_sweep<NormalsPass>(S, ss, grid, nSlots);
+	const int32_t lanes = (int32_t)(gridDim.x * blockDim.x);
+	const int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	{
+		const DContacts& C = S.X.P.C;
+		const int64_t st = S.nrmStride;
+		const int32_t nValid = (int32_t)ss[nSlots];
+		for (int32_t k = tid; k < nValid; k += lanes)
+		{
+			int32_t c = S.rec[k].x;
+			int32_t count = f2i(S.nrm[k].w);
+			for (int j = 0; j < 4 && j < count; ++j)
+				C.pJA[pidx(C, c, j)].w = S.nrm[(9 + j) * st + k].w;
+		}
+	}
+	grid.sync();
+	giant_sweep<FrictionPass>(S, ss, grid, nSlots);
+	{
+		const IslandCtx& X = S.X;
+		for (int32_t k = tid; k < X.NB; k += lanes)
+		{
+			int32_t b = (int32_t)X.bodyList[k];
+			if (giant_body(X, b))
+				reconstruct_velocity(X.P.B, b, X.invH);
+		}
+	}
+	if (threadIdx.x == 0)
+	{
+		uint32_t* leave = S.barrier + (kBarGroups + 1) * kBarStride;
+		uint32_t left = __hip_atomic_fetch_add(leave, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (left + 1 == gridDim.x)
+			for (uint32_t k = 0; k < kBarGroups + 2; ++k)
+				__hip_atomic_store(S.barrier + k * kBarStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
 // Co-resident grid size of a cooperative kernel (blocks of kGiantThreads),
 // cached per device and kernel: the kernel is a template argument, so each
 // kernel has its own cache (the pass kernels share one function type).
@@ -1465,7 +1522,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	static const int capEnv = env_blocks("M3D_GIANT_BLOCKS", 0);
 	if (capEnv > 0 && w.tuner.forcedBlocks == 0)
 		w.tuner.forcedBlocks = capEnv;
-	const int coopMax = std::min(coop_blocks<k_giant_pass<NormalsPass>>(w),
+	const int coopMax = std::min(std::min(coop_blocks<k_giant_pass<NormalsPass>>(w), coop_blocks<k_giant_position>(w)),
 								 std::min(coop_blocks<k_giant_pass<FrictionPass>>(w),
 										  coop_blocks<k_giant_pass<VelocityPass>>(w)));
 	bool measure = false;
@@ -1503,6 +1560,16 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 			return false;
 		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
 		bool ok = true;
+		// one launch for the position stage unless joints interleave with the
+		// normal sweeps or the passes are timed one by one
+		if (!joints && !w.timing)
+		{
+			ok = coop((const void*)k_giant_position, X.iterations);
+			if (!ok)
+				return false;
+		}
+		else
+		{
 		if (joints)
 		{
 			// each position iteration: the joints, then the contacts
@@ -1530,6 +1597,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		w.mark_begin(M3DX_K_GIANT_BODIES);
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 1);
 		w.mark_end();
+		}
 		w.mark_begin(M3DX_K_GIANT_VELOCITY);
 		ok = coop((const void*)k_giant_pass<VelocityPass>, 1);
 		w.mark_end();

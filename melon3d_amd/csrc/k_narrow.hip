@@ -206,18 +206,49 @@ bool narrowphase(World& w, float dt)
 	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(trans, w.u32b.p, C, w.i32b.p);
 	int4* info = (int4*)w.u64a.p;
 	k_trans_info<<<gridn(T), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.i32b.p, T, info);
-	std::vector<int4> host(T);
-	hipMemcpyAsync(host.data(), info, T * sizeof(int4), hipMemcpyDeviceToHost, s);
+	// the transitions come back behind the solver's launches; the step
+	// turns them into events once those are queued (flush_transitions)
+	if (w.pinnedTransCap < (size_t)T)
+	{
+		if (w.pinnedTrans != nullptr)
+			(void)hipHostFree(w.pinnedTrans);
+		w.pinnedTrans = nullptr;
+		w.pinnedTransCap = (size_t)T * 3 / 2 + 256;
+		if (!hip_ok(hipHostMalloc((void**)&w.pinnedTrans, w.pinnedTransCap * sizeof(int4), hipHostMallocDefault),
+					"transition staging"))
+		{
+			w.pinnedTransCap = 0;
+			return false;
+		}
+	}
+	if (w.transReady == nullptr && !hip_ok(hipEventCreateWithFlags(&w.transReady, hipEventDisableTiming), "transitions"))
+		return false;
+	if (!hip_ok(hipMemcpyAsync(w.pinnedTrans, info, T * sizeof(int4), hipMemcpyDeviceToHost, s), "transitions"))
+		return false;
 	// wake-on-touch fixpoint
 	int32_t* wakeAt = w.i32c.p;
 	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
 	k_wake_fixpoint<<<1, 1024, 0, s>>>(info, w.counters.p + CT_TRANSITIONS, w.bc.flags.p, wakeAt);
 	k_wake_apply<<<gridn(n), kBlockN, 0, s>>>(make_body_view(w), n, wakeAt, w.counters.p);
-	if (!hip_ok(hipStreamSynchronize(s), "transitions"))
+	if (!hip_ok(hipEventRecord(w.transReady, s), "transitions"))
+		return false;
+	w.pendingTrans = T;
+	return true;
+}
+
+// Begin and end events of the step's touch transitions, in contact order,
+// after the removal's end events (src/world.cpp:1773-1801).
+bool flush_transitions(World& w)
+{
+	int32_t T = w.pendingTrans;
+	if (T == 0)
+		return true;
+	w.pendingTrans = 0;
+	if (!hip_ok(hipEventSynchronize(w.transReady), "transitions"))
 		return false;
 	for (int32_t t = 0; t < T; ++t)
 	{
-		const int4& e = host[t];
+		const int4& e = w.pinnedTrans[t];
 		m3d_contact_event ce;
 		ce.bodyA = { e.x, w.bodies[e.x].generation };
 		ce.bodyB = { e.y, w.bodies[e.y].generation };

@@ -219,6 +219,11 @@ struct World
 	char* pinnedUpload = nullptr;
 	size_t pinnedUploadCap = 0;
 	hipEvent_t uploadDone = nullptr;
+	// the step's touch transitions (body a, body b, kind), waiting for events
+	int4* pinnedTrans = nullptr;
+	size_t pinnedTransCap = 0;
+	hipEvent_t transReady = nullptr;
+	int32_t pendingTrans = 0;
 	// pinned staging of the contact removal (flags, end events, order)
 	char* pinnedRemoval = nullptr;
 	size_t pinnedRemovalCap = 0;
@@ -378,6 +383,7 @@ bool ensure_device_contacts(World& w, int32_t n);
 bool ensure_contact_columns(World& w, ContactColumns& c, int32_t& cap, int32_t n, size_t keep);
 DContacts columns_view(ContactColumns& cc, int32_t cap);
 bool upload_dirty(World& w);
+bool flush_transitions(World& w);
 bool download_bodies(World& w);
 void ensure_host(World& w);
 void mark_dirty(World& w, int32_t i);

@@ -216,6 +216,7 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	auto t0 = Clock::now();
 	w.beginEvents.clear();
 	w.endEvents.clear();
+	w.pendingTrans = 0;
 	memset(&w.stats, 0, sizeof(w.stats));
 	// a sidecar downloaded before this step (and not used by a destroy since)
 	// no longer matches the device's contacts once the step runs
@@ -237,7 +238,7 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	if (!islands_and_solve(w, dt, substeps))
 		return false;
 	finish_step(w, dt);
-	if (!download_joints(w) || !sync_counters(w))
+	if (!flush_transitions(w) || !download_joints(w) || !sync_counters(w))
 		return false;
 	const uint32_t* hc = w.hostCounters;
 	w.tuner_step_end((int32_t)hc[CT_SLOTS]);

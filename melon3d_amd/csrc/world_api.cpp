@@ -177,6 +177,10 @@ World::~World()
 		(void)hipHostFree(pinnedUpload);
 	if (pinnedRemoval != nullptr)
 		(void)hipHostFree(pinnedRemoval);
+	if (transReady != nullptr)
+		(void)hipEventDestroy(transReady);
+	if (pinnedTrans != nullptr)
+		(void)hipHostFree(pinnedTrans);
 	if (stream != nullptr)
 		(void)hipStreamDestroy(stream);
 }
