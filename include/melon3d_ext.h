@@ -448,6 +448,14 @@ M3D_API int m3dx_body_respawn(m3d_world* world, const m3d_body_id* destroy, cons
 M3D_API int m3dx_world_overlap_aabb_batch(m3d_world* world, const m3d_aabb* boxes, int32_t count,
                                           int32_t* offsets, m3d_body_id* out, int32_t cap);
 
+/* Stable sort of n (key, value) pairs by the low `bits` bits of the keys
+   (keys below 2^bits), ascending: the device radix sort every stage of the
+   step uses, on host arrays, on the device m3dx_set_device names. No
+   counterpart in the reference (its stages sort on the host); a hook for
+   testing the sort. Returns 0 or -1. */
+M3D_API int m3dx_sort_pairs(const uint64_t* keys, const uint32_t* vals, int64_t n, int bits, uint64_t* keysOut,
+                            uint32_t* valsOut);
+
 /* Last error message of this thread ("" when none). */
 M3D_API const char* m3dx_last_error(void);
 

@@ -356,6 +356,7 @@ _SIGS = {
     "m3dx_world_ray_cast_batch_noalloc": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "m3dx_body_respawn": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "m3dx_world_transforms": (C.c_int, [W, C.c_void_p, C.c_int32, C.c_void_p]),
+    "m3dx_sort_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p]),
     "m3dx_world_overlap_aabb_batch": (C.c_int, [W, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
     "m3dx_world_sphere_cast_batch": (C.c_int, [W, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                                  C.c_void_p]),

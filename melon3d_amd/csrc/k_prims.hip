@@ -1,18 +1,18 @@
 // Scan and stable radix sort for gfx950: 256-thread workgroups (4 waves of
-// 64 lanes). The sort ranks each element inside its workgroup with wave
-// ballots (one ballot per digit bit) plus a per-wave digit table in LDS, so
-// the scatter is stable without per-element atomics.
+// 64 lanes). The sort ranks each element inside its tile with wave ballots
+// (one ballot per digit bit) plus per-wave digit counts in LDS, so the
+// scatter is stable without per-element atomics.
 #include "mx_prims.h"
 
 #include <stdio.h>
 
+#include <algorithm>
 #include <atomic>
 
 namespace mx
 {
 
 constexpr int kScanBlock = 1024; // elements per scan workgroup (4 per thread)
-constexpr int kSortBlock = 256;  // elements per sort workgroup (1 per thread)
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 
@@ -217,85 +217,197 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 }
 
 // ------------------------------------------------------------ radix sort
+//
+// LSD radix sort of (key, value) pairs, 8-bit digits, one launch per digit
+// pass after one histogram launch for all passes:
+// - k_radix_digits counts every pass's digits over all keys (LDS tables,
+//   one global atomic per digit and workgroup);
+// - k_radix_pass sorts one digit. A workgroup takes a tile of kSortTile
+//   consecutive pairs (kSortItems per lane, read wave-striped: item j of lane
+//   l of wave w is element w * 64 * kSortItems + j * 64 + l of the tile, so
+//   each load instruction reads 64 consecutive keys), ranks them stably in
+//   that order (per wave: match masks from 8 ballots and a running per-digit
+//   count in LDS; then the waves' counts are offset per digit), finds each
+//   digit's global start for the tile with a decoupled look-back over the
+//   tiles before it (one lane per digit; status words tagged with the call's
+//   epoch as in the scan), stages the tile in LDS in sorted order and writes
+//   it out from there, so consecutive lanes write consecutive addresses of
+//   each digit's run.
+// Stable: within a tile the rank follows the element order, and tiles take
+// their digit offsets in tile order. A tile only waits on lower-numbered
+// tiles, which publish their counts before they wait (see the scan).
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems; // 4096 pairs per tile
+constexpr int kMaxPasses = 8;
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) { return (uint32_t)(k >> shift) & (kRadix - 1); }
 
-// hist layout: [digit][block] so one linear exclusive scan yields the
-// global scatter base of every (digit, block).
-__global__ void __launch_bounds__(256) k_radix_hist(const uint64_t* keys, int64_t n, int shift, uint32_t* hist,
-													int nb)
+__global__ void k_radix_zero(uint32_t* p, int n)
 {
-	__shared__ uint32_t cnt[kRadix];
-	cnt[threadIdx.x] = 0;
-	__syncthreads();
-	int64_t i = (int64_t)blockIdx.x * kSortBlock + threadIdx.x;
+	int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (i < n)
-	{
-		atomicAdd(&cnt[digit_of(keys[i], shift)], 1u);
-	}
-	__syncthreads();
-	hist[(int64_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+		p[i] = 0u;
 }
 
-__global__ void __launch_bounds__(256) k_radix_scatter(const uint64_t* keys, const uint32_t* vals, uint64_t* keysOut,
-													   uint32_t* valsOut, int64_t n, int shift,
-													   const uint32_t* base, int nb)
+// hist[p * 256 + d]: keys whose digit p is d
+__global__ void __launch_bounds__(256) k_radix_digits(const uint64_t* keys, int64_t n, int passes, uint32_t* hist)
 {
+	__shared__ uint32_t cnt[kMaxPasses * kRadix];
+	for (int k = threadIdx.x; k < passes * kRadix; k += 256)
+		cnt[k] = 0u;
+	__syncthreads();
+	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+	{
+		uint64_t key = keys[i];
+		for (int p = 0; p < passes; ++p)
+			atomicAdd(&cnt[p * kRadix + digit_of(key, p * kRadixBits)], 1u);
+	}
+	__syncthreads();
+	for (int k = threadIdx.x; k < passes * kRadix; k += 256)
+		if (cnt[k] != 0u)
+			atomicAdd(&hist[k], cnt[k]);
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_pass(const uint64_t* keys, const uint32_t* vals,
+															uint64_t* keysOut, uint32_t* valsOut, int64_t n, int shift,
+															const uint32_t* digitCount, uint64_t* status,
+															uint32_t epoch)
+{
+	__shared__ uint64_t sKey[kSortTile];
+	__shared__ uint32_t sVal[kSortTile];
 	__shared__ uint32_t waveCnt[4][kRadix];
+	__shared__ uint32_t scanTmp[kRadix];
+	__shared__ uint32_t tileStart[kRadix]; // tile-local start of each digit
+	__shared__ int64_t outBase[kRadix];    // global position of the tile's first key of each digit, minus tileStart
 	const int t = threadIdx.x;
-	const int wave = t >> 6;
-	const int lane = t & 63;
-	for (int k = t; k < 4 * kRadix; k += 256)
-	{
-		(&waveCnt[0][0])[k] = 0;
-	}
+	const int lane = t & 63, wave = t >> 6;
+	const int64_t tile = blockIdx.x;
+	const int64_t tileBase = tile * kSortTile;
+	for (int k = t; k < 4 * kRadix; k += kSortThreads)
+		(&waveCnt[0][0])[k] = 0u;
 	__syncthreads();
 
-	int64_t i = (int64_t)blockIdx.x * kSortBlock + t;
-	bool valid = i < n;
-	uint64_t key = valid ? keys[i] : 0;
-	uint32_t val = valid ? vals[i] : 0;
-	uint32_t d = digit_of(key, shift);
-
-	// lanes of this wave holding the same digit
-	uint64_t same = __ballot(valid);
-	for (int b = 0; b < kRadixBits; ++b)
+	uint64_t key[kSortItems];
+	uint32_t val[kSortItems];
+	uint32_t rank[kSortItems];
+	const int64_t waveBase = tileBase + (int64_t)wave * 64 * kSortItems;
+#pragma unroll
+	for (int j = 0; j < kSortItems; ++j)
 	{
-		bool bit = (d >> b) & 1u;
-		uint64_t m = __ballot(bit);
-		same &= bit ? m : ~m;
+		int64_t i = waveBase + j * 64 + lane;
+		bool valid = i < n;
+		key[j] = valid ? keys[i] : 0ull;
+		val[j] = valid ? vals[i] : 0u;
 	}
-	uint64_t lowerMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	uint32_t rankInWave = (uint32_t)__popcll(same & lowerMask);
-	if (valid && rankInWave == 0)
+	const uint64_t lowerMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+	for (int j = 0; j < kSortItems; ++j)
 	{
-		waveCnt[wave][d] = (uint32_t)__popcll(same);
-	}
-	__syncthreads();
-	if (valid)
-	{
-		uint32_t r = rankInWave;
-		for (int w = 0; w < wave; ++w)
+		int64_t i = waveBase + j * 64 + lane;
+		bool valid = i < n;
+		uint32_t d = digit_of(key[j], shift);
+		uint64_t same = __ballot(valid);
+#pragma unroll
+		for (int b = 0; b < kRadixBits; ++b)
 		{
-			r += waveCnt[w][d];
+			bool bit = (d >> b) & 1u;
+			uint64_t m = __ballot(bit);
+			same &= bit ? m : ~m;
 		}
-		uint32_t dst = base[(int64_t)d * nb + blockIdx.x] + r;
-		keysOut[dst] = key;
-		valsOut[dst] = val;
+		uint32_t before = valid ? waveCnt[wave][d] : 0u;
+		uint32_t below = (uint32_t)__popcll(same & lowerMask);
+		rank[j] = before + below;
+		// the group's lowest lane advances the wave's count of the digit
+		if (valid && below == 0u)
+			waveCnt[wave][d] = before + (uint32_t)__popcll(same);
+		__builtin_amdgcn_wave_barrier();
+	}
+	__syncthreads();
+	// lane t owns digit t: the waves' offsets, the tile's count
+	uint32_t tileCount = 0;
+	{
+		uint32_t c[4];
+#pragma unroll
+		for (int w = 0; w < 4; ++w)
+			c[w] = waveCnt[w][t];
+#pragma unroll
+		for (int w = 0; w < 4; ++w)
+		{
+			waveCnt[w][t] = tileCount;
+			tileCount += c[w];
+		}
+	}
+	uint32_t blockTotal;
+	uint32_t start = block_exclusive_sum(tileCount, scanTmp, &blockTotal);
+	tileStart[t] = start;
+	// the digit's global start: keys of smaller digits, then this digit's
+	// keys in the tiles before this one
+	uint32_t allBefore = block_exclusive_sum(digitCount[t], scanTmp, &blockTotal);
+	uint64_t* st = status + tile * kRadix;
+	uint32_t prefix = 0;
+	if (tile == 0)
+	{
+		__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, tileCount), __ATOMIC_RELAXED,
+						   __HIP_MEMORY_SCOPE_AGENT);
+	}
+	else
+	{
+		__hip_atomic_store(&st[t], pack_status(epoch, kFlagAgg, tileCount), __ATOMIC_RELAXED,
+						   __HIP_MEMORY_SCOPE_AGENT);
+		for (int64_t j = tile - 1; j >= 0; --j)
+		{
+			uint64_t word;
+			for (;;)
+			{
+				word = __hip_atomic_load(&status[j * kRadix + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				if ((uint32_t)(word >> 34) == epoch && ((word >> 32) & 3ull) != 0ull)
+					break;
+				__builtin_amdgcn_s_sleep(1);
+			}
+			prefix += (uint32_t)word;
+			if (((word >> 32) & 3ull) == kFlagPrefix)
+				break;
+		}
+		__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, prefix + tileCount), __ATOMIC_RELAXED,
+						   __HIP_MEMORY_SCOPE_AGENT);
+	}
+	outBase[t] = (int64_t)allBefore + (int64_t)prefix - (int64_t)start;
+	__syncthreads();
+	// stage in tile order of the digit, then write out
+#pragma unroll
+	for (int j = 0; j < kSortItems; ++j)
+	{
+		int64_t i = waveBase + j * 64 + lane;
+		if (i < n)
+		{
+			uint32_t d = digit_of(key[j], shift);
+			uint32_t at = tileStart[d] + waveCnt[wave][d] + rank[j];
+			sKey[at] = key[j];
+			sVal[at] = val[j];
+		}
+	}
+	__syncthreads();
+	const int tileN = (int)(n - tileBase < kSortTile ? n - tileBase : kSortTile);
+	for (int at = t; at < tileN; at += kSortThreads)
+	{
+		uint64_t k = sKey[at];
+		int64_t dst = outBase[digit_of(k, shift)] + at;
+		keysOut[dst] = k;
+		valsOut[dst] = sVal[at];
 	}
 }
 
-size_t sort_scratch_bytes(int64_t n)
+size_t sort_scratch_bytes(int64_t)
 {
-	int64_t nb = (n + kSortBlock - 1) / kSortBlock;
-	int64_t h = nb * kRadix;
-	return (size_t)h * sizeof(uint32_t);
+	return (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
 }
 
 int64_t sort_status_words(int64_t n)
 {
-	int64_t nb = (n + kSortBlock - 1) / kSortBlock;
-	return scan_status_words(nb * kRadix);
+	int64_t tiles = (n + kSortTile - 1) / kSortTile;
+	return tiles * kRadix + 1;
 }
 
 static thread_local const SortTimer* t_sortTimer = nullptr;
@@ -309,31 +421,27 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 	{
 		return;
 	}
-	int nb = (int)((n + kSortBlock - 1) / kSortBlock);
-	int64_t h = (int64_t)nb * kRadix;
+	const int passes = std::min((bits + kRadixBits - 1) / kRadixBits, kMaxPasses);
+	const int64_t tiles = (n + kSortTile - 1) / kSortTile;
 	uint32_t* hist = (uint32_t*)scratch;
-	int passes = (bits + kRadixBits - 1) / kRadixBits;
-	// algorithmic bytes per pass: the keys read for the histogram (8 B), then
-	// every (key, value) pair read and written once (24 B)
+	// algorithmic bytes: the keys read once for the digit counts (8 B), then
+	// every (key, value) pair read and written once per pass (24 B)
 	const SortTimer* timer = t_sortTimer;
 	if (timer != nullptr)
-		timer->begin(timer->ctx, 32.0 * (double)n * passes);
+		timer->begin(timer->ctx, 8.0 * (double)n + 24.0 * (double)n * passes);
+	k_radix_zero<<<(passes * kRadix + 255) / 256, 256, 0, s>>>(hist, passes * kRadix);
+	const int64_t histBlocks = std::min<int64_t>((n + 4095) / 4096, 1024);
+	k_radix_digits<<<(unsigned)histBlocks, 256, 0, s>>>(keys, n, passes, hist);
 	uint64_t* kin = keys;
 	uint32_t* vin = vals;
 	uint64_t* kout = keysAlt;
 	uint32_t* vout = valsAlt;
 	for (int p = 0; p < passes; ++p)
 	{
-		int shift = p * kRadixBits;
-		k_radix_hist<<<nb, 256, 0, s>>>(kin, n, shift, hist, nb);
-		scan_exclusive(hist, hist, h, nullptr, status, s);
-		k_radix_scatter<<<nb, 256, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
-		uint64_t* tk = kin;
-		kin = kout;
-		kout = tk;
-		uint32_t* tv = vin;
-		vin = vout;
-		vout = tv;
+		k_radix_pass<<<(unsigned)tiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p * kRadixBits,
+															  hist + p * kRadix, status, next_scan_epoch());
+		std::swap(kin, kout);
+		std::swap(vin, vout);
 	}
 	if (kin != keys)
 	{

@@ -832,6 +832,52 @@ int m3dx_set_device(int device)
 	return 0;
 }
 
+// The device radix sort on host arrays, on the device m3dx_set_device
+// names, with buffers of its own (a test hook for the sort every stage uses).
+int m3dx_sort_pairs(const uint64_t* keys, const uint32_t* vals, int64_t n, int bits, uint64_t* keysOut,
+					uint32_t* valsOut)
+{
+	if (n < 0 || bits < 0 || bits > 64 || (n > 0 && (keys == nullptr || vals == nullptr || keysOut == nullptr ||
+													valsOut == nullptr)))
+	{
+		set_error("sort pairs: bad arguments");
+		return -1;
+	}
+	if (n == 0)
+		return 0;
+	if (!hip_ok(hipSetDevice(g_device), "sort pairs"))
+		return -1;
+	hipStream_t s = nullptr;
+	uint64_t *k0 = nullptr, *k1 = nullptr, *status = nullptr;
+	uint32_t *v0 = nullptr, *v1 = nullptr;
+	void* scratch = nullptr;
+	const size_t words = (size_t)std::max(scan_status_words(n), sort_status_words(n));
+	bool ok = hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "sort pairs") &&
+			  hip_ok(hipMalloc((void**)&k0, n * sizeof(uint64_t)), "sort pairs") &&
+			  hip_ok(hipMalloc((void**)&k1, n * sizeof(uint64_t)), "sort pairs") &&
+			  hip_ok(hipMalloc((void**)&v0, n * sizeof(uint32_t)), "sort pairs") &&
+			  hip_ok(hipMalloc((void**)&v1, n * sizeof(uint32_t)), "sort pairs") &&
+			  hip_ok(hipMalloc(&scratch, sort_scratch_bytes(n)), "sort pairs") &&
+			  hip_ok(hipMalloc((void**)&status, words * sizeof(uint64_t)), "sort pairs") &&
+			  hip_ok(hipMemset(status, 0, words * sizeof(uint64_t)), "sort pairs") &&
+			  hip_ok(hipMemcpyAsync(k0, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s), "sort pairs") &&
+			  hip_ok(hipMemcpyAsync(v0, vals, n * sizeof(uint32_t), hipMemcpyHostToDevice, s), "sort pairs");
+	if (ok)
+	{
+		radix_sort_pairs(k0, v0, k1, v1, n, bits, scratch, status, s);
+		ok = hip_ok(hipGetLastError(), "sort pairs") &&
+			 hip_ok(hipMemcpyAsync(keysOut, k0, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "sort pairs") &&
+			 hip_ok(hipMemcpyAsync(valsOut, v0, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "sort pairs") &&
+			 hip_ok(hipStreamSynchronize(s), "sort pairs");
+	}
+	for (void* p : { (void*)k0, (void*)k1, (void*)v0, (void*)v1, scratch, (void*)status })
+		if (p != nullptr)
+			(void)hipFree(p);
+	if (s != nullptr)
+		(void)hipStreamDestroy(s);
+	return ok ? 0 : -1;
+}
+
 // src/world.cpp:468-500; returns NULL when the device cannot be set up
 m3d_world* m3d_world_create(const m3d_world_def* def)
 {

@@ -575,6 +575,11 @@ int m3dx_world_overlap_aabb_batch(m3d_world* w, const m3d_aabb* boxes, int32_t c
     offsets[count] = sink.n;
     return sink.n;
 }
+/* the device sort hook has no CPU engine counterpart */
+int m3dx_sort_pairs(const uint64_t*, const uint32_t*, int64_t, int, uint64_t*, uint32_t*)
+{
+    return -1;
+}
 /* the CPU engine's transforms are its getter, one body at a time */
 int m3dx_world_transforms(m3d_world* w, const m3d_body_id* ids, int32_t count, m3d_transform* out)
 {
