@@ -218,29 +218,39 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 
 // ------------------------------------------------------------ radix sort
 //
-// LSD radix sort of (key, value) pairs, 8-bit digits, one launch per digit
-// pass after one histogram launch for all passes:
-// - k_radix_digits counts every pass's digits over all keys (LDS tables,
-//   one global atomic per digit and workgroup);
-// - k_radix_pass sorts one digit. A workgroup takes a tile of kSortTile
-//   consecutive pairs (kSortItems per lane, read wave-striped: item j of lane
-//   l of wave w is element w * 64 * kSortItems + j * 64 + l of the tile, so
-//   each load instruction reads 64 consecutive keys), ranks them stably in
-//   that order (per wave: match masks from 8 ballots and a running per-digit
-//   count in LDS; then the waves' counts are offset per digit), finds each
-//   digit's global start for the tile with a decoupled look-back over the
-//   tiles before it (one lane per digit; status words tagged with the call's
-//   epoch as in the scan), stages the tile in LDS in sorted order and writes
-//   it out from there, so consecutive lanes write consecutive addresses of
-//   each digit's run.
+// LSD radix sort of (key, value) pairs, 8-bit digits, on tiles of kSortTile
+// consecutive pairs (kSortItems per lane, read wave-striped: item j of lane
+// l of wave w is element w * 64 * kSortItems + j * 64 + l of the tile, so
+// each load instruction reads 64 consecutive keys). The pass kernel ranks a
+// tile's pairs stably in that order (per wave: match masks from 8 ballots
+// and a running per-digit count in LDS, its adds issued back to back; then
+// the waves' counts are offset per digit), stages the tile in LDS in sorted
+// order and writes it out from there, so consecutive lanes write
+// consecutive addresses of each digit's run. A tile's output position per
+// digit comes
+// - for sorts of at most kChainedTiles tiles, from one digit count of all
+//   passes over the input (k_radix_digits) and a decoupled look-back over
+//   the tiles before it (one lane per digit, kLookBack predecessors per
+//   round; status words tagged with the call's epoch as in the scan): one
+//   launch per pass;
+// - for larger sorts, from the tiles' digit counts of the pass
+//   (k_radix_count, digit-major) and their exclusive scan: three launches
+//   per pass, none of which waits on another workgroup (a look-back chain
+//   through hundreds of tiles that start together costs more than the
+//   extra read of the keys).
 // Stable: within a tile the rank follows the element order, and tiles take
-// their digit offsets in tile order. A tile only waits on lower-numbered
-// tiles, which publish their counts before they wait (see the scan).
+// their digit offsets in tile order. A chained tile only waits on
+// lower-numbered tiles, which publish their counts before they wait (see
+// the scan).
 
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortThreads = 512;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems; // 4096 pairs per tile
 constexpr int kMaxPasses = 8;
+constexpr int kDigitBlocks = 2048; // workgroups of the digit count, at most
+constexpr int kDigitCopies = 16;   // the digit counts are spread over 16 copies (workgroup b adds to copy b % 16)
+constexpr int64_t kChainedTiles = 32; // sorts of at most this many tiles take the chained passes
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) { return (uint32_t)(k >> shift) & (kRadix - 1); }
 
@@ -251,47 +261,176 @@ __global__ void k_radix_zero(uint32_t* p, int n)
 		p[i] = 0u;
 }
 
-// hist[p * 256 + d]: keys whose digit p is d
+// Lanes of this wave that hold digit d (among the valid lanes).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid)
+{
+	uint64_t same = __ballot(valid);
+#pragma unroll
+	for (int b = 0; b < kRadixBits; ++b)
+	{
+		bool bit = (d >> b) & 1u;
+		uint64_t m = __ballot(bit);
+		same &= bit ? m : ~m;
+	}
+	return same;
+}
+
+// Keys whose digit p is d, summed over the copies c of
+// hist[(c * passes + p) * 256 + d]. Each workgroup counts chunks of 1024 keys
+// (4 per lane) in LDS; a wave whose keys share one digit (a pass over bits
+// the keys have in common) adds once, other waves add per lane. Workgroups
+// add their counts to one of kDigitCopies copies: global adds to one address
+// serialize, so one copy would take a few thousand in a row.
 __global__ void __launch_bounds__(256) k_radix_digits(const uint64_t* keys, int64_t n, int passes, uint32_t* hist)
 {
 	__shared__ uint32_t cnt[kMaxPasses * kRadix];
 	for (int k = threadIdx.x; k < passes * kRadix; k += 256)
 		cnt[k] = 0u;
 	__syncthreads();
-	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+	const int lane = threadIdx.x & 63;
+	const int64_t chunks = (n + 1023) / 1024;
+	for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x)
 	{
-		uint64_t key = keys[i];
-		for (int p = 0; p < passes; ++p)
-			atomicAdd(&cnt[p * kRadix + digit_of(key, p * kRadixBits)], 1u);
+		uint64_t key[4];
+		bool valid[4];
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+		{
+			int64_t i = c * 1024 + q * 256 + threadIdx.x;
+			valid[q] = i < n;
+			key[q] = valid[q] ? keys[i] : 0ull;
+		}
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+		{
+			uint64_t live = __ballot(valid[q]);
+			if (live == 0ull)
+				continue;
+			const int first = __ffsll((unsigned long long)live) - 1;
+			for (int p = 0; p < passes; ++p)
+			{
+				uint32_t d = digit_of(key[q], p * kRadixBits);
+				uint32_t d0 = (uint32_t)__shfl((int)d, first);
+				if (__ballot(valid[q] && d != d0) == 0ull)
+				{
+					if (lane == first)
+						atomicAdd(&cnt[p * kRadix + d0], (uint32_t)__popcll(live));
+				}
+				else if (valid[q])
+					atomicAdd(&cnt[p * kRadix + d], 1u);
+			}
+		}
 	}
 	__syncthreads();
+	uint32_t* copy = hist + (size_t)(blockIdx.x % kDigitCopies) * passes * kRadix;
 	for (int k = threadIdx.x; k < passes * kRadix; k += 256)
 		if (cnt[k] != 0u)
-			atomicAdd(&hist[k], cnt[k]);
+			atomicAdd(&copy[k], cnt[k]);
 }
 
+// Exclusive scans of two values over the first 256 lanes of the workgroup
+// (lane t holds digit t's values; the other lanes pass zeros); tmp holds 8
+// words. Every lane of the workgroup calls it.
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t* tmp, uint32_t& ea, uint32_t& eb)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	uint32_t ia = wave_inclusive_sum(a), ib = wave_inclusive_sum(b);
+	if (lane == 63 && wave < 4)
+	{
+		tmp[wave] = ia;
+		tmp[4 + wave] = ib;
+	}
+	__syncthreads();
+	uint32_t oa = 0, ob = 0;
+	for (int w = 0; w < wave && w < 4; ++w)
+	{
+		oa += tmp[w];
+		ob += tmp[4 + w];
+	}
+	ea = oa + ia - a;
+	eb = ob + ib - b;
+}
+
+constexpr int kLookBack = 16; // predecessor tiles read per look-back round
+
+// Per-tile digit counts of one pass, digit-major: counts[d * tiles + tile]
+// (their exclusive scan is each tile's output position per digit).
+__global__ void __launch_bounds__(kSortThreads) k_radix_count(const uint64_t* keys, int64_t n, int shift,
+															 uint32_t* counts)
+{
+	__shared__ uint32_t cnt[kRadix];
+	const int t = threadIdx.x;
+	const int lane = t & 63, wave = t >> 6;
+	const int64_t tile = blockIdx.x;
+	for (int k = t; k < kRadix; k += kSortThreads)
+		cnt[k] = 0u;
+	uint64_t key[kSortItems];
+	const int64_t waveBase = tile * kSortTile + (int64_t)wave * 64 * kSortItems;
+#pragma unroll
+	for (int j = 0; j < kSortItems; ++j)
+	{
+		int64_t i = waveBase + j * 64 + lane;
+		key[j] = i < n ? keys[i] : 0ull;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < kSortItems; ++j)
+	{
+		bool valid = waveBase + j * 64 + lane < n;
+		uint64_t live = __ballot(valid);
+		if (live == 0ull)
+			continue;
+		const int first = __ffsll((unsigned long long)live) - 1;
+		uint32_t d = digit_of(key[j], shift);
+		uint32_t d0 = (uint32_t)__shfl((int)d, first);
+		if (__ballot(valid && d != d0) == 0ull)
+		{
+			if (lane == first)
+				atomicAdd(&cnt[d0], (uint32_t)__popcll(live));
+		}
+		else if (valid)
+			atomicAdd(&cnt[d], 1u);
+	}
+	__syncthreads();
+	for (int k = t; k < kRadix; k += kSortThreads)
+		counts[(int64_t)k * gridDim.x + tile] = cnt[k];
+}
+
+// Chained: digitCount is the digit's count over the input in kDigitCopies
+// copies copyStride apart, and the tile finds its offsets by look-back.
+// Otherwise digitCount holds the scanned per-tile counts (k_radix_count):
+// the tile's output position of each digit.
+template <bool Chained>
 __global__ void __launch_bounds__(kSortThreads) k_radix_pass(const uint64_t* keys, const uint32_t* vals,
 															uint64_t* keysOut, uint32_t* valsOut, int64_t n, int shift,
-															const uint32_t* digitCount, uint64_t* status,
-															uint32_t epoch)
+															const uint32_t* digitCount, int copyStride,
+															uint64_t* status, uint32_t epoch)
 {
 	__shared__ uint64_t sKey[kSortTile];
 	__shared__ uint32_t sVal[kSortTile];
-	__shared__ uint32_t waveCnt[4][kRadix];
-	__shared__ uint32_t scanTmp[kRadix];
+	__shared__ uint32_t waveCnt[kSortWaves][kRadix];
+	__shared__ uint32_t scanTmp[8];
 	__shared__ uint32_t tileStart[kRadix]; // tile-local start of each digit
 	__shared__ int64_t outBase[kRadix];    // global position of the tile's first key of each digit, minus tileStart
 	const int t = threadIdx.x;
 	const int lane = t & 63, wave = t >> 6;
 	const int64_t tile = blockIdx.x;
 	const int64_t tileBase = tile * kSortTile;
-	for (int k = t; k < 4 * kRadix; k += kSortThreads)
+	for (int k = t; k < kSortWaves * kRadix; k += kSortThreads)
 		(&waveCnt[0][0])[k] = 0u;
-	__syncthreads();
 
+	// the digit's count over the whole input (lane t < 256), read with the keys
+	uint32_t copies[Chained ? kDigitCopies : 1];
+	if constexpr (Chained)
+	{
+#pragma unroll
+		for (int c = 0; c < kDigitCopies; ++c)
+			copies[c] = t < kRadix ? digitCount[c * copyStride + t] : 0u;
+	}
+	else
+		copies[0] = t < kRadix ? digitCount[(int64_t)t * gridDim.x + tile] : 0u;
 	uint64_t key[kSortItems];
 	uint32_t val[kSortItems];
-	uint32_t rank[kSortItems];
 	const int64_t waveBase = tileBase + (int64_t)wave * 64 * kSortItems;
 #pragma unroll
 	for (int j = 0; j < kSortItems; ++j)
@@ -301,86 +440,65 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_pass(const uint64_t* key
 		key[j] = valid ? keys[i] : 0ull;
 		val[j] = valid ? vals[i] : 0u;
 	}
+	__syncthreads();
+	// Stable rank within the wave, in item order (j, then lane): the
+	// lowest lane of each digit group adds the group's size to the wave's
+	// count of that digit and gets the count before it. The adds of
+	// successive items are issued back to back; LDS executes one wave's
+	// operations in order, so each returns the count its item follows.
 	const uint64_t lowerMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	uint32_t below[kSortItems], leader[kSortItems], old[kSortItems];
 #pragma unroll
 	for (int j = 0; j < kSortItems; ++j)
 	{
-		int64_t i = waveBase + j * 64 + lane;
-		bool valid = i < n;
+		bool valid = waveBase + j * 64 + lane < n;
 		uint32_t d = digit_of(key[j], shift);
-		uint64_t same = __ballot(valid);
-#pragma unroll
-		for (int b = 0; b < kRadixBits; ++b)
-		{
-			bool bit = (d >> b) & 1u;
-			uint64_t m = __ballot(bit);
-			same &= bit ? m : ~m;
-		}
-		uint32_t before = valid ? waveCnt[wave][d] : 0u;
-		uint32_t below = (uint32_t)__popcll(same & lowerMask);
-		rank[j] = before + below;
-		// the group's lowest lane advances the wave's count of the digit
-		if (valid && below == 0u)
-			waveCnt[wave][d] = before + (uint32_t)__popcll(same);
-		__builtin_amdgcn_wave_barrier();
+		uint64_t same = digit_peers(d, valid);
+		below[j] = (uint32_t)__popcll(same & lowerMask);
+		leader[j] = valid ? (uint32_t)(__ffsll((unsigned long long)same) - 1) : (uint32_t)lane;
+		old[j] = 0u;
+		if (valid && below[j] == 0u)
+			old[j] = atomicAdd(&waveCnt[wave][d], (uint32_t)__popcll(same));
 	}
-	__syncthreads();
-	// lane t owns digit t: the waves' offsets, the tile's count
-	uint32_t tileCount = 0;
-	{
-		uint32_t c[4];
+	uint32_t rank[kSortItems];
 #pragma unroll
-		for (int w = 0; w < 4; ++w)
+	for (int j = 0; j < kSortItems; ++j)
+		rank[j] = (uint32_t)__shfl((int)old[j], (int)leader[j]) + below[j];
+	__syncthreads();
+	// lane t < 256 owns digit t: the waves' offsets, the tile's count
+	uint32_t tileCount = 0;
+	if (t < kRadix)
+	{
+		uint32_t c[kSortWaves];
+#pragma unroll
+		for (int w = 0; w < kSortWaves; ++w)
 			c[w] = waveCnt[w][t];
 #pragma unroll
-		for (int w = 0; w < 4; ++w)
+		for (int w = 0; w < kSortWaves; ++w)
 		{
 			waveCnt[w][t] = tileCount;
 			tileCount += c[w];
 		}
 	}
-	uint32_t blockTotal;
-	uint32_t start = block_exclusive_sum(tileCount, scanTmp, &blockTotal);
-	tileStart[t] = start;
-	// the digit's global start: keys of smaller digits, then this digit's
-	// keys in the tiles before this one
-	uint32_t allBefore = block_exclusive_sum(digitCount[t], scanTmp, &blockTotal);
-	uint64_t* st = status + tile * kRadix;
-	uint32_t prefix = 0;
-	if (tile == 0)
+	// tile-local start of the digit; keys of smaller digits in the whole input
+	uint32_t total = 0;
+	if constexpr (Chained)
 	{
-		__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, tileCount), __ATOMIC_RELAXED,
-						   __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+		for (int c = 0; c < kDigitCopies; ++c)
+			total += copies[c];
 	}
-	else
-	{
-		__hip_atomic_store(&st[t], pack_status(epoch, kFlagAgg, tileCount), __ATOMIC_RELAXED,
-						   __HIP_MEMORY_SCOPE_AGENT);
-		for (int64_t j = tile - 1; j >= 0; --j)
-		{
-			uint64_t word;
-			for (;;)
-			{
-				word = __hip_atomic_load(&status[j * kRadix + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-				if ((uint32_t)(word >> 34) == epoch && ((word >> 32) & 3ull) != 0ull)
-					break;
-				__builtin_amdgcn_s_sleep(1);
-			}
-			prefix += (uint32_t)word;
-			if (((word >> 32) & 3ull) == kFlagPrefix)
-				break;
-		}
-		__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, prefix + tileCount), __ATOMIC_RELAXED,
-						   __HIP_MEMORY_SCOPE_AGENT);
-	}
-	outBase[t] = (int64_t)allBefore + (int64_t)prefix - (int64_t)start;
+	uint32_t start, allBefore;
+	block_scan2(tileCount, total, scanTmp, start, allBefore);
+	if (t < kRadix)
+		tileStart[t] = start;
 	__syncthreads();
-	// stage in tile order of the digit, then write out
+	// stage in tile order of the digit (while the lanes of the digits look
+	// back), then write out
 #pragma unroll
 	for (int j = 0; j < kSortItems; ++j)
 	{
-		int64_t i = waveBase + j * 64 + lane;
-		if (i < n)
+		if (waveBase + j * 64 + lane < n)
 		{
 			uint32_t d = digit_of(key[j], shift);
 			uint32_t at = tileStart[d] + waveCnt[wave][d] + rank[j];
@@ -388,20 +506,80 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_pass(const uint64_t* key
 			sVal[at] = val[j];
 		}
 	}
+	if (!Chained && t < kRadix)
+		outBase[t] = (int64_t)copies[0] - (int64_t)start;
+	if (Chained && t < kRadix)
+	{
+		// this digit's keys in the tiles before this one: decoupled
+		// look-back, kLookBack predecessors per round (their aggregates are
+		// summed up to the nearest inclusive prefix; a predecessor not yet
+		// published ends the round early and is read again)
+		uint64_t* st = status + tile * kRadix;
+		uint32_t prefix = 0;
+		if (tile == 0)
+		{
+			__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, tileCount), __ATOMIC_RELAXED,
+							   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else
+		{
+			__hip_atomic_store(&st[t], pack_status(epoch, kFlagAgg, tileCount), __ATOMIC_RELAXED,
+							   __HIP_MEMORY_SCOPE_AGENT);
+			int64_t j = tile - 1;
+			for (;;)
+			{
+				uint64_t word[kLookBack];
+#pragma unroll
+				for (int q = 0; q < kLookBack; ++q)
+					word[q] = j - q >= 0 ? __hip_atomic_load(&status[(j - q) * kRadix + t], __ATOMIC_RELAXED,
+															 __HIP_MEMORY_SCOPE_AGENT)
+										 : pack_status(epoch, kFlagPrefix, 0u);
+				int used = 0;
+				bool done = false;
+#pragma unroll
+				for (int q = 0; q < kLookBack; ++q)
+				{
+					if (done || used < q)
+						break;
+					uint64_t w = word[q];
+					uint64_t flag = (w >> 32) & 3ull;
+					if ((uint32_t)(w >> 34) != epoch || flag == 0ull)
+						break;
+					prefix += (uint32_t)w;
+					used = q + 1;
+					done = flag == kFlagPrefix;
+				}
+				if (done)
+					break;
+				j -= used;
+				if (used < kLookBack)
+					__builtin_amdgcn_s_sleep(1);
+			}
+			__hip_atomic_store(&st[t], pack_status(epoch, kFlagPrefix, prefix + tileCount), __ATOMIC_RELAXED,
+							   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		outBase[t] = (int64_t)allBefore + (int64_t)prefix - (int64_t)start;
+	}
 	__syncthreads();
 	const int tileN = (int)(n - tileBase < kSortTile ? n - tileBase : kSortTile);
-	for (int at = t; at < tileN; at += kSortThreads)
+#pragma unroll
+	for (int q = 0; q < kSortItems; ++q)
 	{
-		uint64_t k = sKey[at];
-		int64_t dst = outBase[digit_of(k, shift)] + at;
-		keysOut[dst] = k;
-		valsOut[dst] = sVal[at];
+		int at = q * kSortThreads + t;
+		if (at < tileN)
+		{
+			uint64_t k = sKey[at];
+			int64_t dst = outBase[digit_of(k, shift)] + at;
+			keysOut[dst] = k;
+			valsOut[dst] = sVal[at];
+		}
 	}
 }
 
-size_t sort_scratch_bytes(int64_t)
+size_t sort_scratch_bytes(int64_t n)
 {
-	return (size_t)kMaxPasses * kRadix * sizeof(uint32_t);
+	int64_t tiles = (n + kSortTile - 1) / kSortTile;
+	return std::max((size_t)kDigitCopies * kMaxPasses * kRadix, (size_t)tiles * kRadix) * sizeof(uint32_t);
 }
 
 int64_t sort_status_words(int64_t n)
@@ -425,23 +603,47 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 	const int64_t tiles = (n + kSortTile - 1) / kSortTile;
 	uint32_t* hist = (uint32_t*)scratch;
 	// algorithmic bytes: the keys read once for the digit counts (8 B), then
-	// every (key, value) pair read and written once per pass (24 B)
+	// every (key, value) pair read and written once per pass (24 B; the
+	// per-tile counts of larger sorts read the keys once more per pass)
 	const SortTimer* timer = t_sortTimer;
 	if (timer != nullptr)
 		timer->begin(timer->ctx, 8.0 * (double)n + 24.0 * (double)n * passes);
-	k_radix_zero<<<(passes * kRadix + 255) / 256, 256, 0, s>>>(hist, passes * kRadix);
-	const int64_t histBlocks = std::min<int64_t>((n + 4095) / 4096, 1024);
-	k_radix_digits<<<(unsigned)histBlocks, 256, 0, s>>>(keys, n, passes, hist);
 	uint64_t* kin = keys;
 	uint32_t* vin = vals;
 	uint64_t* kout = keysAlt;
 	uint32_t* vout = valsAlt;
-	for (int p = 0; p < passes; ++p)
+	if (tiles <= kChainedTiles)
 	{
-		k_radix_pass<<<(unsigned)tiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p * kRadixBits,
-															  hist + p * kRadix, status, next_scan_epoch());
-		std::swap(kin, kout);
-		std::swap(vin, vout);
+		// few tiles: one launch per pass, offsets by look-back from the digit
+		// counts of all passes
+		const int histWords = kDigitCopies * passes * kRadix;
+		k_radix_zero<<<(histWords + 255) / 256, 256, 0, s>>>(hist, histWords);
+		const int64_t histBlocks = std::min<int64_t>((n + 1023) / 1024, kDigitBlocks);
+		k_radix_digits<<<(unsigned)histBlocks, 256, 0, s>>>(keys, n, passes, hist);
+		for (int p = 0; p < passes; ++p)
+		{
+			k_radix_pass<true><<<(unsigned)tiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p * kRadixBits,
+																		hist + p * kRadix, passes * kRadix, status,
+																		next_scan_epoch());
+			std::swap(kin, kout);
+			std::swap(vin, vout);
+		}
+	}
+	else
+	{
+		// many tiles (a look-back chain through tiles that run at once would
+		// set the pace): per pass, the tiles' digit counts, their scan, and
+		// the pass from the scanned offsets
+		uint32_t* offs = (uint32_t*)scratch;
+		for (int p = 0; p < passes; ++p)
+		{
+			k_radix_count<<<(unsigned)tiles, kSortThreads, 0, s>>>(kin, n, p * kRadixBits, offs);
+			scan_exclusive(offs, offs, tiles * kRadix, nullptr, status, s);
+			k_radix_pass<false><<<(unsigned)tiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p * kRadixBits,
+																		 offs, 0, status, 0u);
+			std::swap(kin, kout);
+			std::swap(vin, vout);
+		}
 	}
 	if (kin != keys)
 	{

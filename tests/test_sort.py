@@ -41,6 +41,8 @@ CASES = [
     (12289, 1, "random"),
     (100_003, 24, "random"),
     (100_003, 40, "few"),
+    (131_072, 16, "random"),
+    (135_169, 24, "few"),
     (262_144, 32, "equal"),
     (1_000_000, 64, "random"),
     (1_000_000, 20, "few"),
