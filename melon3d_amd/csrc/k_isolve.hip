@@ -722,6 +722,7 @@ struct GiantSolveCtx
 	uint32_t* pos;             // record position of each sorted entry (per step)
 	uint32_t* cnt;             // point count of each sorted entry (per substep)
 	uint32_t* entryAt;         // sorted entry of each record position (per step)
+	uint32_t* atOf;            // record position of each grid contact, by contact (per step)
 	// contact-side inputs of the three passes for each scheduled contact at
 	// its record position (block b of record k at b * nrmStride + k):
 	// 0 normal, 1-4 anchors A (+ separation offset), 5-8 anchors B (+ wN),
@@ -1045,6 +1046,7 @@ __global__ __launch_bounds__(kGroupThreads) void k_giant_group(GiantSolveCtx S)
 	{
 		S.pos[e] = at;
 		S.entryAt[at] = (uint32_t)e;
+		S.atOf[c] = at;
 		int4 rc = make_rec(P, c, 0);
 		rc.w = (rc.w & kRecBoxBox) | ((f2u(P.C.mat[c].w) & CF_CURVED) ? kRecCurved : 0);
 		S.rec[at] = rc;
@@ -1100,16 +1102,22 @@ struct CopySink
 
 // Every substep: the contact's constraint rows for the substep, written to
 // the contact and to the schedule-ordered copy the three passes read. Lanes
-// take record positions in order: the copy is written coalesced, and the
-// lanes of a wave share point count and curvature.
+// take the grid contacts in island-list order (ascending contact within an
+// island): the contact's columns are read and written coalesced and the
+// bodies of neighbouring lanes are near each other; only the copy is
+// written at scattered record positions (in record order, lanes of one
+// wave would each touch another line of every contact column, since a
+// slot holds one contact in every few of an island).
 __global__ void k_giant_prepare(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
-	int32_t at = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (at >= (int32_t)S.slotStart[S.counters[CT_SLOTS]])
+	const IslandCtx& X = S.X;
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= X.NC || (X.mode[(int32_t)X.entryIsland[k]] & IM_GRID) == 0)
 		return;
+	int32_t c = (int32_t)X.contactList[k];
+	int32_t at = (int32_t)S.atOf[c];
 	int4 r = S.rec[at];
-	int32_t c = r.x;
 	CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0 };
 	prepare_contact(P.B, P.C, c, P.h, sink);
 	float4 nc = P.C.normal[c];
@@ -1447,7 +1455,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		++bitsS;
 	if (!w.s64a.ensure(two) || !w.s64b.ensure(two) || !w.s32a.ensure(two) || !w.s32b.ensure(two) ||
 		!w.s32c.ensure((size_t)n + 2) || !w.sColor.ensure(NC) || !w.sLevel.ensure(NC) ||
-		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPosG.ensure(3 * (size_t)NC) ||
+		!w.sSlotStart.ensure((size_t)maxS + 1) || !w.sPacket.ensure(NC) || !w.sRecG.ensure(NC) || !w.sPosG.ensure(3 * (size_t)NC) || !w.sAtOf.ensure((size_t)std::max(w.contactCount, 1)) ||
 		!w.sNrm.ensure(kSchedBlocks * (size_t)NC) || !w.sPred.ensure(NC) || !w.sColored.ensure(NC) ||
 		!w.sBucket.ensure(2 * (size_t)(maxS + 1) * kBuckets) || !w.sCtr.ensure(16) || !w.gridBar.ensure(2 * (size_t)kBarWords) || !w.sDirty.ensure(two + 4) ||
 		!ensure_prim_scratch(w, std::max<int64_t>((int64_t)two + n + 2, NC)))
@@ -1509,6 +1517,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.pos = w.sPosG.p;
 	S.cnt = w.sPosG.p + NC;
 	S.entryAt = w.sPosG.p + 2 * (size_t)NC;
+	S.atOf = w.sAtOf.p;
 	launch_fill_u32(S.bucketCount, 0u, (int64_t)(maxS + 1) * kBuckets, s);
 	const unsigned groupGrid = (unsigned)((NC + kGroupThreads - 1) / kGroupThreads);
 	k_giant_group<false><<<groupGrid, kGroupThreads, 0, s>>>(S);

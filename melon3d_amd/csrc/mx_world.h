@@ -276,6 +276,7 @@ struct World
 	DBuf<int32_t> tailOrder;      // removal: sources of the holes, in the reference's order
 	DBuf<uint32_t> gridBar;       // grid path: barrier counters of the passes and of the ordering
 	DBuf<uint32_t> sPosG;         // grid path: record position and point count by sorted entry, entry by position
+	DBuf<uint32_t> sAtOf;         // grid path: record position by contact
 	DBuf<float4> sNrm;            // grid path: normal-pass inputs by schedule position (17 x entries)
 	DBuf<uint32_t> sBucket;       // giant islands: per (slot, group) counts and cursors
 	DBuf<int4> sPred;             // giant islands: predecessor ranges per entry

@@ -123,6 +123,7 @@ World::~World()
 	sPacket.release();
 	sRecG.release();
 	sPosG.release();
+	sAtOf.release();
 	gridBar.release();
 	tailOrder.release();
 	sNrm.release();
