@@ -290,10 +290,12 @@ M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
    world of its own with the same bodies would (same pairs, contact order,
    islands and results). Bodies are created with m3d_body_create into the
    selected sub-world, sub-worlds filled in ascending order; body ids and
-   every m3d_body_* getter and setter work as for a plain world. Not
-   supported on a batch: m3d_body_destroy (an error, no-op), joints between
-   sub-worlds (creation returns the null id), and per-sub-world contact
-   events (end events of removed contacts are not reported). A batch exports
+   every m3d_body_* getter and setter work as for a plain world. Contact
+   events hold those of every sub-world: the events of one sub-world's
+   bodies, in the batch's order, are the events its own world would report,
+   in its order. Not supported on a batch: m3d_body_destroy (an error,
+   no-op) and joints between sub-worlds (creation returns the null id). A
+   batch exports
    its full state (contacts grouped by sub-world, see
    m3dx_batch_contact_counts) for inspection; importing it gives a plain
    world. Queries see the bodies of all sub-worlds and use the broadphase of

@@ -2093,6 +2093,28 @@ __global__ void k_place_new(PlaceCtx X, const uint64_t* keys, int32_t K, int32_t
 	dst[k] = place_tail(X, sw, l - nc);
 }
 
+// Touching contacts a batch step removes (end events), and the gather of
+// them: (body a, body b, sensor, sub-world of body a).
+__global__ void k_end_flags(const uint32_t* rm, const float4* mat, int32_t C, uint32_t* ev)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	ev[c] = (rm[c] && (f2u(mat[c].w) & CF_TOUCHING)) ? 1u : 0u;
+}
+
+__global__ void k_gather_events_sub(const int2* bodies, const float4* mat, const int32_t* bodySub, const int32_t* idx,
+									const uint32_t* count, int4* out)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (k >= (int32_t)*count)
+		return;
+	int32_t c = idx[k];
+	int2 b = bodies[c];
+	uint32_t cf = f2u(mat[c].w);
+	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, bodySub[b.x]);
+}
+
 bool remove_and_place(World& w)
 {
 	hipStream_t s = w.stream;
@@ -2128,6 +2150,25 @@ bool remove_and_place(World& w)
 		return false;
 	if (w.hostCounters[CT_REMOVED] == 0 && K == 0)
 		return true;
+	// end events of the touching removed contacts (at most R), back with the
+	// placement's last wait
+	int4* evHost = nullptr;
+	if (w.hostCounters[CT_REMOVED] > 0)
+	{
+		const int32_t Rm = (int32_t)w.hostCounters[CT_REMOVED];
+		if (!w.bEvFlag.ensure(C0) || !w.bEvOff.ensure(C0) || !w.bEvIdx.ensure(Rm) || !w.bEvOut.ensure(Rm) ||
+			!ensure_pinned(w.pinnedRemoval, w.pinnedRemovalCap, (size_t)Rm * sizeof(int4), "end events"))
+			return false;
+		evHost = (int4*)w.pinnedRemoval;
+		k_end_flags<<<grid_for(C0), kBlock, 0, s>>>(rm, w.cc.mat.p, C0, w.bEvFlag.p);
+		scan_exclusive(w.bEvFlag.p, w.bEvOff.p, C0, w.counters.p + CT_END_EVENTS, w.scanStatus.p, s);
+		k_compact_indices<<<grid_for(C0), kBlock, 0, s>>>(w.bEvFlag.p, w.bEvOff.p, C0, w.bEvIdx.p);
+		k_gather_events_sub<<<grid_for(Rm), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bodySub.p, w.bEvIdx.p,
+															w.counters.p + CT_END_EVENTS, w.bEvOut.p);
+		if (!hip_ok(hipMemcpyAsync(evHost, w.bEvOut.p, (size_t)Rm * sizeof(int4), hipMemcpyDeviceToHost, s),
+					"end events"))
+			return false;
+	}
 	k_sub_sizes<<<grid_for(S + 1), kBlock, 0, s>>>(w.subContactStart.p, removed, added, S, newC);
 	scan_exclusive(newC, newStart, S + 1, w.counters.p + CT_PLACED, w.scanStatus.p, s);
 	// new keys are grouped by sub-world: exclusive scan of the counts
@@ -2193,7 +2234,31 @@ bool remove_and_place(World& w)
 	std::swap(w.cc, w.cc2);
 	std::swap(w.contactCap, w.contactCap2);
 	w.contactCount = total;
-	return hip_ok(hipStreamSynchronize(s), "contact placement");
+	if (!hip_ok(hipStreamSynchronize(s), "contact placement"))
+		return false;
+	// each sub-world's end events in its own world's order: its removals in
+	// descending contact position (src/world.cpp:1722-1734); sub-worlds in
+	// ascending order (their contacts are ascending runs)
+	if (evHost != nullptr)
+	{
+		const int32_t NE = (int32_t)w.hostCounters[CT_END_EVENTS];
+		for (int32_t hi = 0; hi < NE;)
+		{
+			int32_t lo = hi;
+			while (hi < NE && evHost[hi].w == evHost[lo].w)
+				++hi;
+			for (int32_t k = hi - 1; k >= lo; --k)
+			{
+				const int4& e = evHost[k];
+				m3d_contact_event ce;
+				ce.bodyA = { e.x, w.bodies[e.x].generation };
+				ce.bodyB = { e.y, w.bodies[e.y].generation };
+				ce.isSensor = e.z != 0;
+				w.endEvents.push_back(ce);
+			}
+		}
+	}
+	return true;
 }
 
 } // namespace mx

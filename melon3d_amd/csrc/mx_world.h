@@ -305,6 +305,10 @@ struct World
 	std::vector<uint8_t> subDirtyHost;  // stable-tier dirt raised by API calls
 	bool anySubDirtyHost = false;
 	DBuf<int32_t> bodySub;              // device copy of bodySubHost
+	// end events of a batch step's removals: flags, offsets, indices, gathered events
+	DBuf<uint32_t> bEvFlag, bEvOff;
+	DBuf<int32_t> bEvIdx;
+	DBuf<int4> bEvOut;
 	DBuf<uint32_t> subStable;           // per sub-world SUB_* bits | steps since build << 8
 	DBuf<uint8_t> subDirtyUpload;
 	DBuf<int32_t> subContactStart;      // subCount + 1, contacts by sub-world

@@ -124,6 +124,10 @@ World::~World()
 	sRecG.release();
 	sPosG.release();
 	sAtOf.release();
+	bEvFlag.release();
+	bEvOff.release();
+	bEvIdx.release();
+	bEvOut.release();
 	gridBar.release();
 	tailOrder.release();
 	sNrm.release();

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from melon3d_amd import scenes
-from parity import bodies_array, rot_angle
+from parity import bodies_array, events, rot_angle
 
 DT = 1.0 / 60.0
 
@@ -57,6 +57,25 @@ def _plain(dev, w):
     return bodies_array(snap), rows
 
 
+def _ranges(dev, w, S):
+    out = []
+    for s in range(S):
+        first, n = C.c_int32(0), C.c_int32(0)
+        assert dev.m3dx_batch_body_range(w, s, C.byref(first), C.byref(n)) == 0
+        out.append((first.value, n.value))
+    return out
+
+
+def _events_by_sub(ev, ranges):
+    """A batch's event list split by sub-world (of body A), body indices local."""
+    starts = np.array([r[0] for r in ranges])
+    out = [[] for _ in ranges]
+    for a, b, sensor in ev:
+        s = int(np.searchsorted(starts, a, side="right")) - 1
+        out[s].append((a - starts[s], b - starts[s], sensor))
+    return out
+
+
 def _assert_same(batch_part, plain_part, s, step):
     (bb, brows), (pb, prows) = batch_part, plain_part
     for k, name in enumerate(("position", "rotation", "linear velocity", "angular velocity", "in use", "awake")):
@@ -71,7 +90,10 @@ def _assert_same(batch_part, plain_part, s, step):
 def test_batch_equals_plain_worlds(dev, first, S, steps, every):
     """Bitwise equality with S plain worlds, including sleep (the second case
     runs 4 s, long enough for piles and pyramids to sleep, which rebuilds the
-    stable tiers of sub-worlds at different steps)."""
+    stable tiers of sub-worlds at different steps). Every step, each
+    sub-world's begin and end events (those of its bodies, in the batch's
+    order) are its plain twin's, in order: the end events of removed
+    contacts, then of touches that ended."""
     bw, info = scenes.batch(dev, S, first=first)
     plain = []
     for s in range(S):
@@ -79,12 +101,21 @@ def test_batch_equals_plain_worlds(dev, first, S, steps, every):
         n, pile = scenes.batch_world(dev, w, first + s)
         assert (n, pile) == info[s]
         plain.append(w)
+    ranges = _ranges(dev, bw, S)
     asleep = 0
+    ended = 0
     try:
         for step in range(1, steps + 1):
             dev.m3d_world_step(bw, DT, 4)
             for w in plain:
                 dev.m3d_world_step(w, DT, 4)
+            bb, be = events(dev, bw)
+            sb, se = _events_by_sub(bb, ranges), _events_by_sub(be, ranges)
+            for s in range(S):
+                pb, pe = events(dev, plain[s])
+                assert sb[s] == pb, f"sub-world {s} step {step}: begin events differ"
+                assert se[s] == pe, f"sub-world {s} step {step}: end events differ"
+            ended += len(be)
             if step % every == 0 or step == steps:
                 parts = _split(dev, bw, S)
                 for s in range(S):
@@ -94,8 +125,59 @@ def test_batch_equals_plain_worlds(dev, first, S, steps, every):
         dev.m3d_world_destroy(bw)
         for w in plain:
             dev.m3d_world_destroy(w)
+    if steps < 240:
+        assert ended > 0, "no end events"
     if steps >= 240:
         assert asleep > 0, "no body fell asleep: the stable-tier path went untested"
+
+
+@pytest.mark.gpu
+def test_batch_removal_end_events(dev):
+    """Touching contacts removed by the step (bodies teleported away from the
+    heap they rest on) give each sub-world its plain twin's end events, in
+    its order (descending contact position), sub-worlds in ascending order."""
+    from melon3d_amd.api import m3d_body_id, m3d_vec3
+
+    S, first = 4, 3
+    bw, info = scenes.batch(dev, S, first=first)
+    plain = []
+    for s in range(S):
+        w = dev.create_world()
+        scenes.batch_world(dev, w, first + s)
+        plain.append(w)
+    ranges = _ranges(dev, bw, S)
+    removed = 0
+    try:
+        for step in range(1, 91):
+            if step in (40, 70):
+                # every third body of each sub-world jumps 500 m up (its slot
+                # k in the sub-world: the same body in the twin)
+                for s in range(S):
+                    b0, nb = ranges[s]
+                    for k in range(1 + step % 3, nb, 3):
+                        for w, idx in ((bw, b0 + k), (plain[s], k)):
+                            bid = m3d_body_id(idx, 1)  # never destroyed: first generation
+                            t = dev.m3d_body_transform(w, bid)
+                            dev.m3d_body_set_transform(w, bid, m3d_vec3(t.p.x, t.p.y + 500.0, t.p.z), t.q)
+            dev.m3d_world_step(bw, DT, 4)
+            for w in plain:
+                dev.m3d_world_step(w, DT, 4)
+            bb, be = events(dev, bw)
+            sb, se = _events_by_sub(bb, ranges), _events_by_sub(be, ranges)
+            for s in range(S):
+                pb, pe = events(dev, plain[s])
+                assert sb[s] == pb, f"sub-world {s} step {step}: begin events differ"
+                assert se[s] == pe, f"sub-world {s} step {step}: end events differ"
+            if step in (40, 70):
+                removed += len(be)
+        parts = _split(dev, bw, S)
+        for s in range(S):
+            _assert_same(parts[s], _plain(dev, plain[s]), s, 90)
+    finally:
+        dev.m3d_world_destroy(bw)
+        for w in plain:
+            dev.m3d_world_destroy(w)
+    assert removed > 0, "the teleports removed no touching contact"
 
 
 @pytest.mark.gpu
