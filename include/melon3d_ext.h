@@ -290,12 +290,15 @@ M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
    world of its own with the same bodies would (same pairs, contact order,
    islands and results). Bodies are created with m3d_body_create into the
    selected sub-world, sub-worlds filled in ascending order; body ids and
-   every m3d_body_* getter and setter work as for a plain world. Contact
-   events hold those of every sub-world: the events of one sub-world's
-   bodies, in the batch's order, are the events its own world would report,
-   in its order. Not supported on a batch: m3d_body_destroy (an error,
-   no-op) and joints between sub-worlds (creation returns the null id). A
-   batch exports
+   every m3d_body_* getter and setter work as for a plain world.
+   m3d_body_destroy destroys in the body's sub-world as in a world of its
+   own (its contacts leave the sub-world's run in the reference's order, the
+   slot goes to the sub-world's free list, which creates into that sub-world
+   reuse while it is the selected one). Contact events hold those of every
+   sub-world: the events of one sub-world's bodies, in the batch's order,
+   are the events its own world would report, in its order. Not supported
+   on a batch: joints between sub-worlds (creation returns the null id) and
+   m3dx_body_respawn's destroys. A batch exports
    its full state (contacts grouped by sub-world, see
    m3dx_batch_contact_counts) for inspection; importing it gives a plain
    world. Queries see the bodies of all sub-worlds and use the broadphase of

@@ -1715,11 +1715,12 @@ static int bits_for(uint64_t range)
 	return b;
 }
 
+// flags[t]: SUB_DIRTY and SUB_FORCE bits set by the host since the last step
 __global__ void k_sub_or_dirty(uint32_t* subStable, const uint8_t* flags, int32_t S)
 {
 	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (t < S && flags[t])
-		subStable[t] |= SUB_DIRTY;
+		subStable[t] |= (uint32_t)flags[t] & (SUB_DIRTY | SUB_FORCE);
 }
 
 bool upload_batch_state(World& w)
@@ -2115,12 +2116,40 @@ __global__ void k_gather_events_sub(const int2* bodies, const float4* mat, const
 	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, bodySub[b.x]);
 }
 
-bool remove_and_place(World& w)
+// Contacts of one body (a destroy in a batch).
+__global__ void k_body_contact_flags(const int2* bodies, int32_t C, int32_t body, uint32_t* flags)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C)
+		return;
+	int2 ab = bodies[c];
+	flags[c] = (ab.x == body || ab.y == body) ? 1u : 0u;
+}
+
+static bool remove_and_place_impl(World& w, int32_t destroyed, std::vector<int4>* events);
+
+bool remove_and_place(World& w) { return remove_and_place_impl(w, -1, nullptr); }
+
+// m3d_body_destroy's contact removal in a batch (src/world.cpp:576-593 in
+// the body's sub-world): the body's contacts leave its sub-world's run in
+// descending position, each swapped with the run's current last contact;
+// the touching ones come back as (body a, body b, sensor, sub-world) in that
+// order.
+bool batch_destroy_contacts(World& w, int32_t body, std::vector<int4>& events)
+{
+	events.clear();
+	launch_fill_u32(w.counters.p + CT_REMOVED, 0u, 1, w.stream);
+	return remove_and_place_impl(w, body, &events);
+}
+
+// The batch step's removals (or, with destroyed >= 0, the removal of that
+// body's contacts) and placement of the new contacts, per sub-world run.
+static bool remove_and_place_impl(World& w, int32_t destroyed, std::vector<int4>* events)
 {
 	hipStream_t s = w.stream;
 	int32_t S = w.subCount;
 	int32_t C0 = w.contactCount;
-	int32_t K = w.newCount;
+	int32_t K = destroyed >= 0 ? 0 : w.newCount;
 	if (C0 == 0 && K == 0)
 		return true;
 	if (!w.subWork.ensure(5 * (size_t)S + 9) || !w.u32a.ensure(std::max(C0, 1)) || !w.u32b.ensure(std::max(C0, 1)) ||
@@ -2140,7 +2169,10 @@ bool remove_and_place(World& w)
 	uint32_t* rm = w.u32a.p;
 	if (C0 > 0)
 	{
-		k_remove_flags<<<grid_for(C0), kBlock, 0, s>>>(body_view(w), oldBodies, C0, rm);
+		if (destroyed >= 0)
+			k_body_contact_flags<<<grid_for(C0), kBlock, 0, s>>>(oldBodies, C0, destroyed, rm);
+		else
+			k_remove_flags<<<grid_for(C0), kBlock, 0, s>>>(body_view(w), oldBodies, C0, rm);
 		k_count_removed<<<reduce_grid(C0, kBlock), kBlock, 0, s>>>(rm, oldBodies, w.bodySub.p, C0, removed,
 																  w.counters.p);
 	}
@@ -2250,6 +2282,11 @@ bool remove_and_place(World& w)
 			for (int32_t k = hi - 1; k >= lo; --k)
 			{
 				const int4& e = evHost[k];
+				if (events != nullptr)
+				{
+					events->push_back(e);
+					continue;
+				}
 				m3d_contact_event ce;
 				ce.bodyA = { e.x, w.bodies[e.x].generation };
 				ce.bodyB = { e.y, w.bodies[e.y].generation };

@@ -113,6 +113,7 @@ bool stage_new_contacts(World& w, const uint64_t* keys, int32_t K);
 // separated-pair removal and new-contact placement per sub-world, each sub-world
 // as src/world.cpp:1705-1735 and :1540-1568 would lay out a world of its own
 bool remove_and_place(World& w);
+bool batch_destroy_contacts(World& w, int32_t body, std::vector<int4>& events);
 
 // ---- k_diff.hip ----
 // records the step's solve inputs when differentiable recording is on

@@ -301,6 +301,7 @@ struct World
 	int32_t subCurrent = 0;
 	std::vector<int32_t> subBodyStart;  // subCount + 1
 	std::vector<int32_t> bodySubHost;   // per body
+	std::vector<std::vector<int32_t>> subFreeBodies; // per sub-world: its destroyed slots (LIFO)
 	bool bodySubDirty = false;
 	std::vector<uint8_t> subDirtyHost;  // stable-tier dirt raised by API calls
 	bool anySubDirtyHost = false;

@@ -637,7 +637,7 @@ void set_stable_dirty(World& w, int32_t i)
 {
 	if (w.subCount > 0)
 	{
-		w.subDirtyHost[w.bodySubHost[i]] = 1;
+		w.subDirtyHost[w.bodySubHost[i]] |= (uint8_t)SUB_DIRTY;
 		w.anySubDirtyHost = true;
 	}
 	else
@@ -1020,10 +1020,15 @@ m3d_body_id m3d_body_create(m3d_world* world, const m3d_body_def* bd, const m3d_
 {
 	World& w = world->w;
 	int32_t index;
-	if (!w.freeBodies.empty())
+	// a batch reuses a destroyed slot of the sub-world being filled (each
+	// sub-world's own free list, as its own world would)
+	std::vector<int32_t>& freeList =
+		w.subCount > 0 && w.subCurrent < (int32_t)w.subFreeBodies.size() ? w.subFreeBodies[w.subCurrent] : w.freeBodies;
+	const bool reused = !freeList.empty();
+	if (reused)
 	{
-		index = w.freeBodies.back();
-		w.freeBodies.pop_back();
+		index = freeList.back();
+		freeList.pop_back();
 	}
 	else
 	{
@@ -1065,7 +1070,7 @@ m3d_body_id m3d_body_create(m3d_world* world, const m3d_body_def* bd, const m3d_
 	update_inv_inertia(b);
 	b.bp = 0; // recycled slot: stale fat AABB and tier membership reset
 	w.gridDirty = true;
-	if (w.subCount > 0)
+	if (w.subCount > 0 && !reused)
 	{
 		// bodies of a sub-world are one index range, in sub-world order
 		w.bodySubHost.push_back(w.subCurrent);
@@ -1103,13 +1108,30 @@ void m3d_body_destroy(m3d_world* world, m3d_body_id id)
 	HostBody* b = body_checked(w, id);
 	if (b == nullptr)
 		return;
+	int32_t index = id.index;
 	if (w.subCount > 0)
 	{
-		// a recycled slot would break the per-sub-world body ranges
-		set_error("bodies of batched worlds cannot be destroyed");
+		// the body's sub-world as a world of its own: its contacts leave the
+		// sub-world's run on the device (touching ones give end events and,
+		// unless sensors, wake the partner), then the slot joins the
+		// sub-world's free list and its stable tier is rebuilt at the next step
+		std::vector<int4> gone;
+		if (!batch_destroy_contacts(w, index, gone))
+			return;
+		for (const int4& e : gone)
+		{
+			int32_t other = e.x == index ? e.y : e.x;
+			if (e.z == 0)
+				wake_partner(w, other);
+			m3d_contact_event ev;
+			ev.bodyA = make_id(w, e.x);
+			ev.bodyB = make_id(w, e.y);
+			ev.isSensor = e.z != 0;
+			w.endEvents.push_back(ev);
+		}
+		retire_body(w, index);
 		return;
 	}
-	int32_t index = id.index;
 	w.destroyedSinceStep = true;
 	if (ensure_sidecar(w))
 	{
@@ -1186,10 +1208,23 @@ static void retire_body(World& w, int32_t index)
 	HostBody& b = w.bodies[index];
 	b.inUse = false;
 	b.generation++;
-	w.freeBodies.push_back(index);
 	w.gridDirty = true;
-	w.stableDirty = true;
-	w.stableForce = true;
+	if (w.subCount > 0)
+	{
+		// the sub-world's free list; its stable tier rebuilt at the next step
+		int32_t sub = w.bodySubHost[index];
+		if ((int32_t)w.subFreeBodies.size() < w.subCount)
+			w.subFreeBodies.resize(w.subCount);
+		w.subFreeBodies[sub].push_back(index);
+		w.subDirtyHost[sub] |= (uint8_t)(SUB_DIRTY | SUB_FORCE);
+		w.anySubDirtyHost = true;
+	}
+	else
+	{
+		w.freeBodies.push_back(index);
+		w.stableDirty = true;
+		w.stableForce = true;
+	}
 	mark_dirty(w, index);
 }
 

@@ -268,7 +268,72 @@ def test_batch_against_reference(dev, ref):
 
 
 @pytest.mark.gpu
-def test_batch_rejects_destroy_and_cross_joints(dev):
+def test_batch_destroy_equals_plain_worlds(dev):
+    """m3d_body_destroy in a batch is a destroy in the body's sub-world: after
+    destroys at two steps (contacts removed in the sub-world's order, end
+    events, partner wakes, the stable tier rebuilt) and creates into the last
+    sub-world (its freed slots reused last-in first-out), every sub-world
+    equals its plain twin: ids, bodies, contacts and events."""
+    from melon3d_amd.api import m3d_body_id
+
+    S, first = 4, 11
+    bw, info = scenes.batch(dev, S, first=first)
+    plain = []
+    for s in range(S):
+        w = dev.create_world()
+        scenes.batch_world(dev, w, first + s)
+        plain.append(w)
+    ranges = _ranges(dev, bw, S)
+    destroyed = []
+    try:
+        for step in range(1, 101):
+            if step in (30, 56):
+                for s in range(S):
+                    b0, nb = ranges[s]
+                    for k in range(2 + step % 5, nb, 5):
+                        if (s, k) in destroyed:
+                            continue
+                        destroyed.append((s, k))
+                        dev.m3d_body_destroy(bw, m3d_body_id(b0 + k, 1))
+                        dev.m3d_body_destroy(plain[s], m3d_body_id(k, 1))
+                        assert not dev.m3d_body_is_valid(bw, m3d_body_id(b0 + k, 1))
+                    be = events(dev, bw)[1]
+                    pe = events(dev, plain[s])[1]
+                    assert _events_by_sub(be, ranges)[s] == pe, f"sub-world {s}: destroy end events differ"
+            if step == 70:
+                # the last sub-world is the selected one: creates reuse its slots
+                s = S - 1
+                b0 = ranges[s][0]
+                gens = []
+                for j in range(8):
+                    pos = (0.3 * j, 8.0 + 1.1 * j, 0.2)
+                    a = scenes.add_box(dev, bw, pos, (0.4, 0.4, 0.4))
+                    p = scenes.add_box(dev, plain[s], pos, (0.4, 0.4, 0.4))
+                    assert (a.index - b0, a.generation) == (p.index, p.generation), j
+                    gens.append(a.generation)
+                assert gens[0] == 2, "the first create did not reuse a destroyed slot"
+            dev.m3d_world_step(bw, DT, 4)
+            for w in plain:
+                dev.m3d_world_step(w, DT, 4)
+            bb, be = events(dev, bw)
+            sb, se = _events_by_sub(bb, ranges), _events_by_sub(be, ranges)
+            for s in range(S):
+                pb, pe = events(dev, plain[s])
+                assert sb[s] == pb, f"sub-world {s} step {step}: begin events differ"
+                assert se[s] == pe, f"sub-world {s} step {step}: end events differ"
+            if step % 10 == 0 or step in (30, 31, 56, 57, 70, 71):
+                parts = _split(dev, bw, S)
+                for s in range(S):
+                    _assert_same(parts[s], _plain(dev, plain[s]), s, step)
+    finally:
+        dev.m3d_world_destroy(bw)
+        for w in plain:
+            dev.m3d_world_destroy(w)
+    assert len(destroyed) > 20
+
+
+@pytest.mark.gpu
+def test_batch_rejects_cross_joints(dev):
     bw, info = scenes.batch(dev, 2, first=0)
     try:
         first, n = C.c_int32(0), C.c_int32(0)
@@ -276,8 +341,6 @@ def test_batch_rejects_destroy_and_cross_joints(dev):
         from melon3d_amd.api import m3d_ball_joint_def, m3d_body_id
         a = m3d_body_id(1, 1)
         b = m3d_body_id(first.value + 1, 1)
-        dev.m3d_body_destroy(bw, a)
-        assert dev.m3d_body_is_valid(bw, a)
         jd = m3d_ball_joint_def()
         jd.bodyA, jd.bodyB = a, b
         assert dev.m3d_joint_create_ball(bw, C.byref(jd)).index == -1
