@@ -1108,7 +1108,7 @@ struct CopySink
 // written at scattered record positions (in record order, lanes of one
 // wave would each touch another line of every contact column, since a
 // slot holds one contact in every few of an island).
-__global__ void k_giant_prepare(GiantSolveCtx S)
+__global__ void __launch_bounds__(256) k_giant_prepare(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
 	const IslandCtx& X = S.X;

@@ -20,7 +20,7 @@ MX_HD void add_point(Manifold* m, m3d_transform xfA, m3d_transform xfB, m3d_vec3
 	{
 		return;
 	}
-	MPoint& mp = m->points[m->pointCount++];
+	MPoint mp;
 	mp.localAnchorA = m3d_inv_transform_point(xfA, onA);
 	mp.localAnchorB = m3d_inv_transform_point(xfB, onB);
 	mp.frictionAnchorA = mp.localAnchorA;
@@ -31,6 +31,7 @@ MX_HD void add_point(Manifold* m, m3d_transform xfA, m3d_transform xfB, m3d_vec3
 	mp.lambdaT = 0.0f;
 	mp.separationOffset = 0.0f;
 	mp.slipDir = m3d_v3_zero();
+	m->set_point(m->pointCount++, mp);
 }
 
 // src/collide.cpp:36-47
@@ -388,27 +389,55 @@ struct ClipV
 	uint32_t id;
 };
 
+// A clipping polygon of up to 8 vertices kept in vector registers (an
+// element taken at a run-time index is a register lane, not a memory slot).
+typedef float mx_float8 __attribute__((ext_vector_type(8)));
+typedef uint32_t mx_uint8 __attribute__((ext_vector_type(8)));
+struct ClipPoly
+{
+	mx_float8 x, y, z;
+	mx_uint8 id;
+	MX_HD ClipV get(int i) const
+	{
+		ClipV c;
+		c.v = m3d_v3(x[i], y[i], z[i]);
+		c.id = id[i];
+		return c;
+	}
+	MX_HD void put(int i, const ClipV& c)
+	{
+		x[i] = c.v.x;
+		y[i] = c.v.y;
+		z[i] = c.v.z;
+		id[i] = c.id;
+	}
+};
+
+// Column i of a rotation (a box's axis i) at a run-time index.
+MX_HD m3d_vec3 mat3_col(const m3d_mat3& m, int i) { return i == 0 ? m.cx : (i == 1 ? m.cy : m.cz); }
+
 // Sutherland-Hodgman against dot(x, n) <= off, at most 8 output vertices;
 // src/collide.cpp:437-464
-MX_HD int clip_polygon(const ClipV* in, int inCount, m3d_vec3 n, float off, uint32_t planeId, ClipV* out)
+MX_HD int clip_polygon(const ClipPoly& in, int inCount, m3d_vec3 n, float off, uint32_t planeId, ClipPoly& out)
 {
 	int oc = 0;
 	for (int i = 0; i < inCount; ++i)
 	{
-		const ClipV& v1 = in[i];
-		const ClipV& v2 = in[i + 1 < inCount ? i + 1 : 0];
+		const ClipV v1 = in.get(i);
+		const ClipV v2 = in.get(i + 1 < inCount ? i + 1 : 0);
 		float d1 = m3d_dot(v1.v, n) - off;
 		float d2 = m3d_dot(v2.v, n) - off;
 		if (d1 <= 0.0f)
 		{
-			out[oc++] = v1;
+			out.put(oc++, v1);
 		}
 		if (d1 * d2 < 0.0f && oc < 8)
 		{
 			float t = d1 / (d1 - d2);
-			out[oc].v = m3d_lerp(v1.v, v2.v, t);
-			out[oc].id = ((v1.id ^ v2.id) << 8) | planeId;
-			++oc;
+			ClipV x;
+			x.v = m3d_lerp(v1.v, v2.v, t);
+			x.id = ((v1.id ^ v2.id) << 8) | planeId;
+			out.put(oc++, x);
 		}
 		if (oc >= 8)
 		{
@@ -549,24 +578,36 @@ MX_HD void collide_boxes(const Shape& a, m3d_transform xfA, const Shape& b, m3d_
 	}
 
 	// face contact: clip the incident face against the reference side planes
+	// (the axes of either box are picked with compile-time indices)
 	m3d_transform xfRef = refIsA ? xfA : xfB;
 	m3d_transform xfInc = refIsA ? xfB : xfA;
-	const m3d_mat3& rotInc = refIsA ? rotB : rotA;
-	const m3d_vec3* refAx = refIsA ? axA : axB;
-	const m3d_vec3* incAx = refIsA ? axB : axA;
+	// picked component by component (a choice between two whole matrices
+	// or vectors would put both in scratch memory)
+	auto sel = [](bool c, m3d_vec3 a, m3d_vec3 b) { return m3d_v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); };
+	m3d_mat3 rotInc, rotRef;
+	rotInc.cx = sel(refIsA, rotB.cx, rotA.cx);
+	rotInc.cy = sel(refIsA, rotB.cy, rotA.cy);
+	rotInc.cz = sel(refIsA, rotB.cz, rotA.cz);
+	rotRef.cx = sel(refIsA, rotA.cx, rotB.cx);
+	rotRef.cy = sel(refIsA, rotA.cy, rotB.cy);
+	rotRef.cz = sel(refIsA, rotA.cz, rotB.cz);
+	auto refAx = [&](int i) { return mat3_col(rotRef, i); };
+	auto incAx = [&](int i) { return mat3_col(rotInc, i); };
 	m3d_vec3 refExt = refIsA ? eA : eB;
 	m3d_vec3 incExt = refIsA ? eB : eA;
 	m3d_vec3 dRI = refIsA ? d : m3d_neg(d);
 
-	float refSign = m3d_dot(refAx[refFace], dRI) >= 0.0f ? 1.0f : -1.0f;
-	m3d_vec3 refN = m3d_scale(refSign, refAx[refFace]);
+	const m3d_vec3 refAxis = refAx(refFace);
+	float refSign = m3d_dot(refAxis, dRI) >= 0.0f ? 1.0f : -1.0f;
+	m3d_vec3 refN = m3d_scale(refSign, refAxis);
 
 	int incFace = 0;
 	float minDot = FLT_MAX;
 	float incSign = 1.0f;
+#pragma unroll
 	for (int i = 0; i < 3; ++i)
 	{
-		float dt = m3d_dot(incAx[i], refN);
+		float dt = m3d_dot(incAx(i), refN);
 		if (dt < minDot)
 		{
 			minDot = dt;
@@ -588,27 +629,33 @@ MX_HD void collide_boxes(const Shape& a, m3d_transform xfA, const Shape& b, m3d_
 	v3_set(uL, iu, v3_get(incExt, iu));
 	v3_set(vL, iv, v3_get(incExt, iv));
 
-	ClipV poly[8], tmp[8];
+	ClipPoly poly, tmp;
 	{
 		m3d_vec3 cW = m3d_transform_point(xfInc, cL);
 		m3d_vec3 uW = m3d_mat3_mulv(&rotInc, uL);
 		m3d_vec3 vW = m3d_mat3_mulv(&rotInc, vL);
-		poly[0].v = m3d_add(m3d_add(cW, uW), vW);
-		poly[0].id = 0;
-		poly[1].v = m3d_add(m3d_sub(cW, uW), vW);
-		poly[1].id = 1;
-		poly[2].v = m3d_sub(m3d_sub(cW, uW), vW);
-		poly[2].id = 2;
-		poly[3].v = m3d_sub(m3d_add(cW, uW), vW);
-		poly[3].id = 3;
+		ClipV q;
+		q.v = m3d_add(m3d_add(cW, uW), vW);
+		q.id = 0;
+		poly.put(0, q);
+		q.v = m3d_add(m3d_sub(cW, uW), vW);
+		q.id = 1;
+		poly.put(1, q);
+		q.v = m3d_sub(m3d_sub(cW, uW), vW);
+		q.id = 2;
+		poly.put(2, q);
+		q.v = m3d_sub(m3d_add(cW, uW), vW);
+		q.id = 3;
+		poly.put(3, q);
 	}
 	int count = 4;
 	int ru = refFace + 1 < 3 ? refFace + 1 : refFace - 2;
 	int rv = refFace + 2 < 3 ? refFace + 2 : refFace - 1;
+#pragma unroll
 	for (int k = 0; k < 2; ++k)
 	{
 		int ai = k == 0 ? ru : rv;
-		m3d_vec3 sn = refAx[ai];
+		m3d_vec3 sn = refAx(ai);
 		float cp = m3d_dot(xfRef.p, sn);
 		count = clip_polygon(poly, count, sn, cp + v3_get(refExt, ai), (uint32_t)(2 * k + 4), tmp);
 		if (count == 0)
@@ -623,15 +670,16 @@ MX_HD void collide_boxes(const Shape& a, m3d_transform xfA, const Shape& b, m3d_
 	}
 
 	float faceOff = m3d_dot(xfRef.p, refN) + v3_get(refExt, refFace);
-	ClipV kept[8];
-	float seps[8];
+	ClipPoly kept;
+	mx_float8 seps;
 	int kc = 0;
 	for (int i = 0; i < count; ++i)
 	{
-		float s = m3d_dot(poly[i].v, refN) - faceOff;
+		const ClipV pv = poly.get(i);
+		float s = m3d_dot(pv.v, refN) - faceOff;
 		if (s <= margin)
 		{
-			kept[kc] = poly[i];
+			kept.put(kc, pv);
 			seps[kc] = s;
 			++kc;
 		}
@@ -641,91 +689,89 @@ MX_HD void collide_boxes(const Shape& a, m3d_transform xfA, const Shape& b, m3d_
 		return;
 	}
 
-	// reduce to 4: deepest, farthest from it, then extreme signed areas
-	int order[8];
-	int red = 0;
+	m->normal = refIsA ? refN : m3d_neg(refN);
+	const uint32_t flip = refIsA ? 0u : 0x40000000u;
+	// one kept vertex as a manifold point: the incident vertex and its
+	// projection on the reference face
+	auto emit = [&](const ClipV& kv, float sep) {
+		m3d_vec3 pInc = kv.v;
+		m3d_vec3 pRef = m3d_mul_add(pInc, -sep, refN);
+		uint32_t id = (kv.id & 0x00FFFFFFu) | ((uint32_t)refFace << 24) | flip;
+		if (refIsA)
+		{
+			add_point(m, xfA, xfB, pRef, pInc, sep, id);
+		}
+		else
+		{
+			add_point(m, xfA, xfB, pInc, pRef, sep, id);
+		}
+	};
 	if (kc <= 4)
 	{
 		for (int i = 0; i < kc; ++i)
 		{
-			order[red++] = i;
+			emit(kept.get(i), seps[i]);
+		}
+		return;
+	}
+	// reduce to 4: deepest, farthest from it, then extreme signed areas
+	int i0 = 0;
+	float s0 = seps[0];
+	for (int i = 1; i < kc; ++i)
+	{
+		if (seps[i] < s0)
+		{
+			s0 = seps[i];
+			i0 = i;
 		}
 	}
-	else
+	const m3d_vec3 k0 = kept.get(i0).v;
+	int i1 = -1;
+	float best = -1.0f;
+	for (int i = 0; i < kc; ++i)
 	{
-		int i0 = 0;
-		for (int i = 1; i < kc; ++i)
+		if (i == i0)
 		{
-			if (seps[i] < seps[i0])
-			{
-				i0 = i;
-			}
+			continue;
 		}
-		int i1 = -1;
-		float best = -1.0f;
-		for (int i = 0; i < kc; ++i)
+		float dd = m3d_length_sq(m3d_sub(kept.get(i).v, k0));
+		if (dd > best)
 		{
-			if (i == i0)
-			{
-				continue;
-			}
-			float dd = m3d_length_sq(m3d_sub(kept[i].v, kept[i0].v));
-			if (dd > best)
-			{
-				best = dd;
-				i1 = i;
-			}
-		}
-		int i2 = -1, i3 = -1;
-		float bestArea = 0.0f, worstArea = 0.0f;
-		for (int i = 0; i < kc; ++i)
-		{
-			if (i == i0 || i == i1)
-			{
-				continue;
-			}
-			m3d_vec3 e1 = m3d_sub(kept[i1].v, kept[i0].v);
-			m3d_vec3 e2 = m3d_sub(kept[i].v, kept[i0].v);
-			float area = m3d_dot(m3d_cross(e1, e2), refN);
-			if (area > bestArea)
-			{
-				bestArea = area;
-				i2 = i;
-			}
-			if (area < worstArea)
-			{
-				worstArea = area;
-				i3 = i;
-			}
-		}
-		order[red++] = i0;
-		order[red++] = i1;
-		if (i2 >= 0)
-		{
-			order[red++] = i2;
-		}
-		if (i3 >= 0)
-		{
-			order[red++] = i3;
+			best = dd;
+			i1 = i;
 		}
 	}
-
-	m->normal = refIsA ? refN : m3d_neg(refN);
-	uint32_t flip = refIsA ? 0u : 0x40000000u;
-	for (int k = 0; k < red; ++k)
+	const m3d_vec3 e1 = m3d_sub(kept.get(i1).v, k0);
+	int i2 = -1, i3 = -1;
+	float bestArea = 0.0f, worstArea = 0.0f;
+	for (int i = 0; i < kc; ++i)
 	{
-		int i = order[k];
-		m3d_vec3 pInc = kept[i].v;
-		m3d_vec3 pRef = m3d_mul_add(pInc, -seps[i], refN);
-		uint32_t id = (kept[i].id & 0x00FFFFFFu) | ((uint32_t)refFace << 24) | flip;
-		if (refIsA)
+		if (i == i0 || i == i1)
 		{
-			add_point(m, xfA, xfB, pRef, pInc, seps[i], id);
+			continue;
 		}
-		else
+		m3d_vec3 e2 = m3d_sub(kept.get(i).v, k0);
+		float area = m3d_dot(m3d_cross(e1, e2), refN);
+		if (area > bestArea)
 		{
-			add_point(m, xfA, xfB, pInc, pRef, seps[i], id);
+			bestArea = area;
+			i2 = i;
 		}
+		if (area < worstArea)
+		{
+			worstArea = area;
+			i3 = i;
+		}
+	}
+	emit(kept.get(i0), seps[i0]);
+	emit(kept.get(i1), seps[i1]);
+	if (i2 >= 0)
+	{
+		emit(kept.get(i2), seps[i2]);
+	}
+	if (i3 >= 0)
+	{
+		emit(kept.get(i3), seps[i3]);
 	}
 }
 
@@ -772,13 +818,14 @@ MX_HD void collide_shapes(const Shape& sA, m3d_transform xfA, const Shape& sB, m
 	m->normal = m3d_neg(m->normal);
 	for (int i = 0; i < m->pointCount; ++i)
 	{
-		MPoint& p = m->points[i];
+		MPoint p = m->point(i);
 		m3d_vec3 t = p.localAnchorA;
 		p.localAnchorA = p.localAnchorB;
 		p.localAnchorB = t;
 		t = p.frictionAnchorA;
 		p.frictionAnchorA = p.frictionAnchorB;
 		p.frictionAnchorB = t;
+		m->set_point(i, p);
 	}
 }
 
@@ -794,7 +841,7 @@ MX_HD void inherit_friction_anchors(const OldManifold& old, Manifold& m, m3d_vec
 	m3d_vec3 n = m.normal;
 	for (int i = 0; i < m.pointCount; ++i)
 	{
-		MPoint& mp = m.points[i];
+		MPoint mp = m.point(i);
 		for (int j = 0; j < old.pointCount; ++j)
 		{
 			const MPoint op = old.point(j);
@@ -819,6 +866,7 @@ MX_HD void inherit_friction_anchors(const OldManifold& old, Manifold& m, m3d_vec
 			}
 			break;
 		}
+		m.set_point(i, mp);
 	}
 }
 

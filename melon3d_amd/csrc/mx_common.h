@@ -140,12 +140,55 @@ struct MPoint
 	m3d_vec3 slipDir;
 };
 
+// A manifold under construction: point j's fields are lane j of 4-wide
+// vector registers, so a point taken or stored at a run-time index stays in
+// registers (an array of MPoint would go to per-lane scratch memory).
+typedef float mx_float4v __attribute__((ext_vector_type(4)));
+typedef uint32_t mx_uint4v __attribute__((ext_vector_type(4)));
 struct Manifold
 {
 	m3d_vec3 normal;
 	int32_t pointCount;
-	MPoint points[4];
-	MX_HD const MPoint& point(int j) const { return points[j]; }
+	mx_float4v ax, ay, az, bx, by, bz, fax, fay, faz, fbx, fby, fbz, sep, lamN, lamT, off, slx, sly, slz;
+	mx_uint4v id;
+	MX_HD MPoint point(int j) const
+	{
+		MPoint p;
+		p.localAnchorA = m3d_v3(ax[j], ay[j], az[j]);
+		p.localAnchorB = m3d_v3(bx[j], by[j], bz[j]);
+		p.frictionAnchorA = m3d_v3(fax[j], fay[j], faz[j]);
+		p.frictionAnchorB = m3d_v3(fbx[j], fby[j], fbz[j]);
+		p.separation = sep[j];
+		p.id = id[j];
+		p.lambdaN = lamN[j];
+		p.lambdaT = lamT[j];
+		p.separationOffset = off[j];
+		p.slipDir = m3d_v3(slx[j], sly[j], slz[j]);
+		return p;
+	}
+	MX_HD void set_point(int j, const MPoint& p)
+	{
+		ax[j] = p.localAnchorA.x;
+		ay[j] = p.localAnchorA.y;
+		az[j] = p.localAnchorA.z;
+		bx[j] = p.localAnchorB.x;
+		by[j] = p.localAnchorB.y;
+		bz[j] = p.localAnchorB.z;
+		fax[j] = p.frictionAnchorA.x;
+		fay[j] = p.frictionAnchorA.y;
+		faz[j] = p.frictionAnchorA.z;
+		fbx[j] = p.frictionAnchorB.x;
+		fby[j] = p.frictionAnchorB.y;
+		fbz[j] = p.frictionAnchorB.z;
+		sep[j] = p.separation;
+		id[j] = p.id;
+		lamN[j] = p.lambdaN;
+		lamT[j] = p.lambdaT;
+		off[j] = p.separationOffset;
+		slx[j] = p.slipDir.x;
+		sly[j] = p.slipDir.y;
+		slz[j] = p.slipDir.z;
+	}
 };
 
 // ----------------------------------------------------- device layouts
@@ -274,7 +317,7 @@ MX_HD void load_manifold(const DContacts& C, int32_t c, Manifold& m)
 		int32_t p = pidx(C, c, j);
 		float4 a = C.pA[p], b = C.pB[p], ja = C.pJA[p], jb = C.pJB[p], ra = C.pRA[p], rb = C.pRB[p];
 		float4 fa = C.pFA[p], fb = C.pFB[p], sl = C.pSlip[p];
-		MPoint& mp = m.points[j];
+		MPoint mp;
 		mp.localAnchorA = f4_v3(a);
 		mp.separationOffset = a.w;
 		mp.localAnchorB = f4_v3(b);
@@ -285,6 +328,7 @@ MX_HD void load_manifold(const DContacts& C, int32_t c, Manifold& m)
 		mp.frictionAnchorA = f4_v3(fa);
 		mp.frictionAnchorB = f4_v3(fb);
 		mp.slipDir = f4_v3(sl);
+		m.set_point(j, mp);
 	}
 }
 
@@ -333,7 +377,7 @@ MX_HD void store_manifold(const DContacts& C, int32_t c, const Manifold& m)
 	for (int j = 0; j < m.pointCount; ++j)
 	{
 		int32_t p = pidx(C, c, j);
-		const MPoint& mp = m.points[j];
+		const MPoint mp = m.point(j);
 		C.pA[p] = v3_f4(mp.localAnchorA, mp.separationOffset);
 		float4 b = C.pB[p];
 		C.pB[p] = make_float4(mp.localAnchorB.x, mp.localAnchorB.y, mp.localAnchorB.z, b.w);

@@ -184,11 +184,12 @@ MX_HD void set_separation_offsets(Manifold& m, m3d_vec3 pa, m3d_quat qa, m3d_vec
 {
 	for (int i = 0; i < m.pointCount; ++i)
 	{
-		MPoint& mp = m.points[i];
+		MPoint mp = m.point(i);
 		m3d_vec3 rA = m3d_rotate(qa, mp.localAnchorA);
 		m3d_vec3 rB = m3d_rotate(qb, mp.localAnchorB);
 		float ref = m3d_dot(m3d_sub(m3d_add(pb, rB), m3d_add(pa, rA)), m.normal);
 		mp.separationOffset = mp.separation - ref;
+		m.set_point(i, mp);
 	}
 }
 
