@@ -279,7 +279,7 @@ __device__ void init_inertia(const StepIn& in, StepState<T>& S, int32_t i)
 // state: per body 13 values (x, q, v, w) in, then out; null in = recorded.
 // mass, friction: per body, null = recorded.
 template <class T>
-__global__ void k_diff_replay(RecView V, StepState<T> G, const T* stateIn, T* stateOut, const T* mass, const T* mu,
+__global__ void __launch_bounds__(64) k_diff_replay(RecView V, StepState<T> G, const T* stateIn, T* stateOut, const T* mass, const T* mu,
 							  T* muScratch)
 {
 	// one world per wave: worlds branch differently, so they do not share one
@@ -348,7 +348,7 @@ __global__ void k_diff_replay(RecView V, StepState<T> G, const T* stateIn, T* st
 // Taped replay of one step (float values), from the recorded state: leaves
 // per body x, q, v, w, mass, friction (ids i * 15 + k, in that order), the
 // output node of every state component in outIds.
-__global__ void k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tapes, uint32_t* outIds,
+__global__ void __launch_bounds__(64) k_diff_tape(RecView V, StepState<Ad<float>> G, Tape<float>* tapes, uint32_t* outIds,
 							Ad<float>* muScratch)
 {
 	typedef Ad<float> A;
