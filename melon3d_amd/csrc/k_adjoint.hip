@@ -54,7 +54,7 @@ typedef Dual<float, double> DF;
 typedef dd::Body<DF> DBody;
 typedef dd::Body<float> FBody;
 
-constexpr int kAdjThreads = 512;
+constexpr int kAdjThreads = 256;
 constexpr int kAdjTasks = 2048;      // results of one chunk of (op, component) tasks
 constexpr int kAdjMaxContacts = 3072; // per world (level lists live in LDS)
 constexpr int kAdjMaxBodies = 2048;
