@@ -708,8 +708,11 @@ struct ContactGeo
 {
 	float4 n;     // normal xyz, point count bits
 	float4 mat;   // friction, restitution, rolling, CF_* bits
-	float4 pA[4]; // localAnchorA, separationOffset
-	float4 pB[4]; // localAnchorB
+	// point j's localAnchorA and separationOffset, localAnchorB: lane j of
+	// vector registers (a run-time j does not put the geometry in scratch)
+	mx_float4v pax, pay, paz, paw, pbx, pby, pbz, pbw;
+	MX_HD float4 pa(int j) const { return make_float4(pax[j], pay[j], paz[j], paw[j]); }
+	MX_HD float4 pb(int j) const { return make_float4(pbx[j], pby[j], pbz[j], pbw[j]); }
 	float radiusA, radiusB, hhA, hhB;
 	int tA, tB;
 	int count;
@@ -725,8 +728,15 @@ MX_HD ContactGeo contact_geo(const StepIn& in, int32_t c)
 	g.count = f2i(g.n.w);
 	for (int j = 0; j < 4; ++j)
 	{
-		g.pA[j] = in.pA[c * 4 + j];
-		g.pB[j] = in.pB[c * 4 + j];
+		const float4 a = in.pA[c * 4 + j], b = in.pB[c * 4 + j];
+		g.pax[j] = a.x;
+		g.pay[j] = a.y;
+		g.paz[j] = a.z;
+		g.paw[j] = a.w;
+		g.pbx[j] = b.x;
+		g.pby[j] = b.y;
+		g.pbz[j] = b.z;
+		g.pbw[j] = b.w;
 	}
 	g.radiusA = in.shape[ab.x].w;
 	g.radiusB = in.shape[ab.y].w;
@@ -753,8 +763,8 @@ MX_HD PointPrep<T> prepare_point(const ContactGeo& g, int j, Q4<T> qa, Q4<T> qb,
 {
 	V3<T> n = lift<T>(g.n);
 	PointPrep<T> r;
-	r.rA = contact_arm(g.tA, g.radiusA, g.hhA, qa, g.pA[j], n, 1.0f);
-	r.rB = contact_arm(g.tB, g.radiusB, g.hhB, qb, g.pB[j], n, -1.0f);
+	r.rA = contact_arm(g.tA, g.radiusA, g.hhA, qa, g.pa(j), n, 1.0f);
+	r.rB = contact_arm(g.tB, g.radiusB, g.hhB, qb, g.pb(j), n, -1.0f);
 	V3<T> crA = cross(r.rA, n);
 	V3<T> crB = cross(r.rB, n);
 	r.jA = mat3_mulv(iwa, crA);
@@ -790,9 +800,9 @@ MX_UNROLL_POINTS
 		T w = wN[j];
 		if (!(val(w) > (F)0))
 			continue;
-		float4 a4 = g.pA[j];
+		float4 a4 = g.pa(j);
 		V3<T> rA = rotate(qa, lift<T>(a4));
-		V3<T> rB = rotate(qb, lift<T>(g.pB[j]));
+		V3<T> rB = rotate(qb, lift<T>(g.pb(j)));
 		T sep = Num<T>::c(a4.w) + dot(sub(add(pb, rB), add(pa, rA)), n);
 		if (!(val(sep) < (F)0))
 			continue;
@@ -847,8 +857,8 @@ MX_UNROLL_POINTS
 			{
 				if (j >= g.count)
 					break;
-				fA[j] = g.pA[j];
-				fB[j] = g.pB[j];
+				fA[j] = g.pa(j);
+				fB[j] = g.pb(j);
 			}
 			return;
 		}
@@ -884,8 +894,8 @@ MX_UNROLL_POINTS
 		position_correction(a, pt, rA);
 		if (val(needed) > (F)2 * val(allowed))
 		{
-			fA[j] = g.pA[j];
-			fB[j] = g.pB[j];
+			fA[j] = g.pa(j);
+			fB[j] = g.pb(j);
 		}
 	}
 }
@@ -957,8 +967,8 @@ MX_UNROLL_POINTS
 		V3<T> rAf, rBf;
 		if (curved)
 		{
-			rAf = contact_arm(g.tA, g.radiusA, g.hhA, a.q, g.pA[j], n, 1.0f);
-			rBf = contact_arm(g.tB, g.radiusB, g.hhB, b.q, g.pB[j], n, -1.0f);
+			rAf = contact_arm(g.tA, g.radiusA, g.hhA, a.q, g.pa(j), n, 1.0f);
+			rBf = contact_arm(g.tB, g.radiusB, g.hhB, b.q, g.pb(j), n, -1.0f);
 		}
 		else
 		{
