@@ -78,21 +78,19 @@ def measured_traffic(scene: str, size: int, kernel: str, window=None):
     `traffic_bytes_per_launch <kernel> <bytes> <warmup> <steps>`, the profiled
     run's window, which must be this run's `window` when one is given. None
     when not measured."""
-    files = {("stream", 200000): "stream200k_window.md", ("batch", 4096): "batch4096_kernels.md",
-             ("diff", 256): "diff256_adjoint.md"}
-    if (scene, size) not in files:
-        return None
-    path = os.path.join(ROOT, "profiles", files[(scene, size)])
-    try:
-        with open(path) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    if window is not None and (len(parts) < 5 or (int(parts[3]), int(parts[4])) != tuple(window)):
-                        continue
-                    return float(parts[2])
-    except OSError:
-        return None
+    files = {("stream", 200000): ("stream200k_window.md", "stream200k_steady.md"),
+             ("batch", 4096): ("batch4096_kernels.md",), ("diff", 256): ("diff256_adjoint.md",)}
+    for name in files.get((scene, size), ()):
+        try:
+            with open(os.path.join(ROOT, "profiles", name)) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                        if window is not None and (len(parts) < 5 or (int(parts[3]), int(parts[4])) != tuple(window)):
+                            continue
+                        return float(parts[2])
+        except OSError:
+            continue
     return None
 
 
