@@ -1228,18 +1228,24 @@ __global__ void k_set_move_sources(int2* moves, const int32_t* src, int32_t H)
 // a quarter of the contact's own columns: four consecutive lanes move one
 // contact, so a wave moves 16 contacts' points as contiguous runs (a
 // contact's four points are adjacent in each column).
-__device__ __forceinline__ void copy_contact_part(const DContacts& S, int32_t s, const DContacts& D, int32_t d, int j)
+// Part j of contact s to slot d: point j (unless point == false) and a
+// quarter of the contact's own fields.
+__device__ __forceinline__ void copy_contact_part(const DContacts& S, int32_t s, const DContacts& D, int32_t d, int j,
+												  bool point = true)
 {
-	int32_t pd = pidx(D, d, j), ps = pidx(S, s, j);
-	D.pA[pd] = S.pA[ps];
-	D.pB[pd] = S.pB[ps];
-	D.pJA[pd] = S.pJA[ps];
-	D.pJB[pd] = S.pJB[ps];
-	D.pRA[pd] = S.pRA[ps];
-	D.pRB[pd] = S.pRB[ps];
-	D.pFA[pd] = S.pFA[ps];
-	D.pFB[pd] = S.pFB[ps];
-	D.pSlip[pd] = S.pSlip[ps];
+	if (point)
+	{
+		int32_t pd = pidx(D, d, j), ps = pidx(S, s, j);
+		D.pA[pd] = S.pA[ps];
+		D.pB[pd] = S.pB[ps];
+		D.pJA[pd] = S.pJA[ps];
+		D.pJB[pd] = S.pJB[ps];
+		D.pRA[pd] = S.pRA[ps];
+		D.pRB[pd] = S.pRB[ps];
+		D.pFA[pd] = S.pFA[ps];
+		D.pFB[pd] = S.pFB[ps];
+		D.pSlip[pd] = S.pSlip[ps];
+	}
 	cache_of(D, d)[j] = cache_of(S, s)[j];
 	switch (j)
 	{
@@ -2073,7 +2079,10 @@ __global__ void k_place_old(PlaceCtx X, DContacts src, DContacts dstC)
 		d = (int32_t)X.newStart[sw] + (int32_t)l;
 	else
 		d = place_tail(X, sw, l - X.newC[sw]);
-	copy_contact_part(src, c, dstC, d, (int)(t & 3));
+	// point slots at or beyond the count are never read (a regeneration
+	// writes its points before any stage reads them): not copied
+	const int j = (int)(t & 3);
+	copy_contact_part(src, c, dstC, d, j, j < f2i(src.normal[c].w));
 }
 
 __global__ void k_place_new(PlaceCtx X, const uint64_t* keys, int32_t K, int32_t* dst)
