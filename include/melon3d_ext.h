@@ -322,14 +322,15 @@ M3D_API int m3dx_batch_contact_counts(m3d_world* world, int32_t* out, int32_t ca
    friction, and the solver contacts with their manifolds (the fixed contact
    set of that step). The adjoint is that of the step's solve (the four
    substeps of integrate, prepare, normal projection, stiction, velocity
-   reconstruction and velocity pass, in the serial-island order) with the
-   contact set and manifolds held fixed; a recorded step's output state is
-   the next step's input. Requires a world created with enableSleep = false;
-   bodies must not be created or destroyed while recording. Plain worlds and
-   batches (one lane per sub-world) both work. A step the adjoint does not
-   cover (an island of 300 or more solver contacts, which the engine solves
-   with the colored schedule of src/xpbd.cpp:1167-1369, or a world with
-   joints) is refused: that step runs as usual but is not recorded,
+   reconstruction and velocity pass, in the order the engine ran them: the
+   serial order of src/xpbd.cpp:898-974, or for an island of 300 or more
+   solver contacts the colored schedule of src/xpbd.cpp:1167-1369 with its
+   packet arithmetic and overflow pass) with the contact set, manifolds and
+   coloring held fixed; a recorded step's output state is the next step's
+   input. Requires a world created with enableSleep = false; bodies must not
+   be created or destroyed while recording. Plain worlds and batches (one
+   lane per sub-world) both work. A step the adjoint does not cover (a world
+   with joints) is refused: that step runs as usual but is not recorded,
    recording stops, and m3dx_diff_record(world, 0) and every m3dx_diff_*
    call on the records return -1 with the reason in m3dx_last_error. */
 /* enable != 0 starts recording (dropping earlier records), 0 stops it.

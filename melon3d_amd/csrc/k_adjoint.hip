@@ -2,11 +2,14 @@
 //
 // One workgroup per world and recorded step. The workgroup first replays the
 // step's solve forward in float with the ops of mx_diff.h, which is the
-// engine's arithmetic in the engine's order: the serial contact order of a
-// pass (src/xpbd.cpp:898-974) is split into levels, a contact's level being
-// one more than the highest level of the earlier contacts that share one of
-// its dynamic bodies, so the contacts of one level write disjoint bodies and
-// run in parallel with the serial result. Every op's inputs are recorded.
+// engine's arithmetic in the engine's order: the contact order of a pass
+// (serial islands: src/xpbd.cpp:898-974; colored islands: their colors in
+// turn, then the overflow contacts, :1269-1368, the colored normal
+// projection in its packet form) is split into levels, a contact's level
+// being one more than the highest level of the earlier contacts that share
+// one of its dynamic bodies, so the contacts of one level write disjoint
+// bodies and run in parallel with the serial result. Every op's inputs are
+// recorded.
 //
 // The substeps are then walked backwards, each pass's levels in reverse. The
 // vector-Jacobian product of one op (one contact or body) is assembled from
@@ -312,7 +315,7 @@ MX_HD void fwd_substep(const AW& A, const StepIn& ins, const SubConst& K)
 				st4(r + 10, qb);
 				for (int j = 0; j < 4; ++j)
 					r[14 + j] = lN[j];
-				dd::normals_op(g, pa, qa, pb, qb, A.im[ab.x], A.im[ab.y], lN, wN, jA, jB, K.maxCorrection);
+				dd::normals_any(ins, c, g, pa, qa, pb, qb, A.im[ab.x], A.im[ab.y], lN, wN, jA, jB, K.maxCorrection);
 				for (int j = 0; j < g.count; ++j)
 					A.lamN[c * 4 + j] = lN[j];
 				if (g.dynA)
@@ -740,7 +743,7 @@ MX_HD double nrm_eval(const AW& A, const StepIn& ins, const SubConst& K, int it,
 		jA[j] = dv3(pr + j * 13 + 6, k + 1, d);
 		jB[j] = dv3(pr + j * 13 + 9, k + 4, d);
 	}
-	dd::normals_op(g, pa, qa, pb, qb, imA, imB, lN, wN, jA, jB, K.maxCorrection);
+	dd::normals_any(ins, c, g, pa, qa, pb, qb, imA, imB, lN, wN, jA, jB, K.maxCorrection);
 	double r = 0.0;
 	if (g.dynA)
 		r += td3(A.aSt + ab.x * 13, pa) + td4(A.aSt + ab.x * 13 + 3, qa);
@@ -1042,36 +1045,47 @@ MX_HD void adj_world(const RecView& V, const AdjLayout& L, float* slab, double* 
 		A.il[i * 3 + 2] = IL.z;
 		sLast[i] = -1;
 	}
-	// packed pairs for the level pass: a | b << 12 | dynamic bits
+	// packed pairs for the level pass: a | b << 12 | dynamic bits | the
+	// contact's pass-order key << 26
 	for (int32_t c = tid; c < nc; c += T)
 	{
 		int2 ab = in.bodies[c];
 		A.mu[c] = dd::sqrt_(in.bmat[ab.x].y * in.bmat[ab.y].y);
 		sOrder[c] = ab.x | (ab.y << 12) | (dd::is_dynamic(in, ab.x) ? 1 << 24 : 0) |
-					(dd::is_dynamic(in, ab.y) ? 1 << 25 : 0);
+					(dd::is_dynamic(in, ab.y) ? 1 << 25 : 0) | (dd::sched_key(in, c) << 26);
 	}
 	ADJ_SYNC();
-	// levels of the serial contact order
+	// levels of the step's contact order: key by key (a colored island's
+	// colors, then its overflow contacts; every other contact has key 0),
+	// ascending contacts within a key
 	if (tid == 0)
 	{
-		int32_t maxL = -1;
+		int32_t maxL = -1, keys = 1;
 		for (int32_t c = 0; c < nc; ++c)
 		{
-			int32_t pk = sOrder[c];
-			int32_t a = pk & 0xFFF, b = (pk >> 12) & 0xFFF;
-			bool da = (pk >> 24) & 1, db = (pk >> 25) & 1;
-			int32_t l = 0;
-			if (da)
-				l = l > sLast[a] + 1 ? l : sLast[a] + 1;
-			if (db)
-				l = l > sLast[b] + 1 ? l : sLast[b] + 1;
-			sLevel[c] = l;
-			if (da)
-				sLast[a] = l;
-			if (db)
-				sLast[b] = l;
-			maxL = maxL > l ? maxL : l;
+			int32_t kc = ((sOrder[c] >> 26) & 0xF) + 1;
+			keys = keys > kc ? keys : kc;
 		}
+		for (int32_t key = 0; key < keys; ++key)
+			for (int32_t c = 0; c < nc; ++c)
+			{
+				int32_t pk = sOrder[c];
+				if (((pk >> 26) & 0xF) != key)
+					continue;
+				int32_t a = pk & 0xFFF, b = (pk >> 12) & 0xFFF;
+				bool da = (pk >> 24) & 1, db = (pk >> 25) & 1;
+				int32_t l = 0;
+				if (da)
+					l = l > sLast[a] + 1 ? l : sLast[a] + 1;
+				if (db)
+					l = l > sLast[b] + 1 ? l : sLast[b] + 1;
+				sLevel[c] = l;
+				if (da)
+					sLast[a] = l;
+				if (db)
+					sLast[b] = l;
+				maxL = maxL > l ? maxL : l;
+			}
 		int32_t nL = maxL + 1;
 		for (int32_t l = 0; l <= nL; ++l)
 			sStart[l] = 0;

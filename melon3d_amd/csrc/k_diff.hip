@@ -70,15 +70,22 @@ __global__ void k_diff_gather(const int2* bodies, int32_t n, const uint32_t* fla
 	idx[k] = c;
 }
 
-// the manifold of every recorded contact after a substep's prepare
-__global__ void k_diff_gather_sub(DContacts C, const int32_t* idx, int32_t nc, int64_t base, float4* on, float4* om,
-								  float4* oA, float4* oB, float4* oFA, float4* oFB, float4* oS)
+// the manifold of every recorded contact after a substep's prepare, and its
+// schedule word: the normal projection's form from the contact's solve
+// record (bits 0-7: the packet's largest point count, or the serial form)
+// and its pass-order key, the color of a colored island's contact (8: the
+// overflow pass) or 0
+__global__ void k_diff_gather_sub(DContacts C, const int32_t* idx, int32_t nc, int64_t base, const uint32_t* atOf,
+								  const int4* rec, const int32_t* contactColor, float4* on, float4* om, float4* oA,
+								  float4* oB, float4* oFA, float4* oFB, float4* oS, int32_t* osched)
 {
 	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (k >= nc)
 		return;
 	int32_t c = idx[k];
 	int64_t o = base + k;
+	int32_t color = contactColor[c];
+	osched[o] = (rec[atOf[c]].w & 0xFF) | ((color < 0 ? 0 : color) << 8);
 	on[o] = C.normal[c];
 	om[o] = C.mat[c];
 	for (int j = 0; j < 4; ++j)
@@ -104,24 +111,17 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps)
 	DiffState* D = w.diff;
 	if (D == nullptr || !D->recording)
 		return true;
-	// The adjoint replays the serial island schedule (src/xpbd.cpp:898-974).
-	// An island of kBigIslandContacts or more solver contacts is solved by
-	// the colored schedule instead (src/xpbd.cpp:1167-1369, packets with
-	// invW = 1 / wN and the overflow pass), and joints are not replayed:
-	// such a step is not recorded, recording stops, and the records refuse
-	// every later use, so no gradient of a trajectory the engine did not run
-	// is ever returned. The step itself runs as usual.
-	if (w.djointCount > 0 || w.stats.coloredIslands > 0)
+	// Joints are not replayed by the adjoint: such a step is not recorded,
+	// recording stops, and the records refuse every later use, so no
+	// gradient of a trajectory the engine did not run is ever returned. The
+	// step itself runs as usual. Colored islands (src/xpbd.cpp:1167-1369) are
+	// replayed with the colors and packet forms the solve used (recorded per
+	// substep by diff_record_substep).
+	if (w.djointCount > 0)
 	{
 		char msg[256];
-		if (w.djointCount > 0)
-			snprintf(msg, sizeof msg, "differentiable recording refused at step %d: the world has joints",
-					 (int)D->steps.size());
-		else
-			snprintf(msg, sizeof msg,
-					 "differentiable recording refused at step %d: %d island(s) with %d or more solver contacts take "
-					 "the colored schedule, which the adjoint does not replay",
-					 (int)D->steps.size(), (int)w.stats.coloredIslands, (int)kBigIslandContacts);
+		snprintf(msg, sizeof msg, "differentiable recording refused at step %d: the world has joints",
+				 (int)D->steps.size());
 		D->recording = false;
 		D->refusedStep = (int32_t)D->steps.size();
 		D->refusal = msg;
@@ -188,7 +188,7 @@ bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps)
 	size_t ncs = nc1 * (size_t)substeps;
 	if (!R->bodies.ensure(nc1) || !R->idx.ensure(nc1) || !R->normal.ensure(ncs) || !R->cmat.ensure(ncs) ||
 		!R->pA.ensure(4 * ncs) || !R->pB.ensure(4 * ncs) || !R->pFA.ensure(4 * ncs) || !R->pFB.ensure(4 * ncs) ||
-		!R->pSlip.ensure(4 * ncs))
+		!R->pSlip.ensure(4 * ncs) || !R->sched.ensure(ncs))
 		return false;
 	if (NC > 0)
 		k_diff_gather<<<gridk(C), 256, 0, s>>>(w.cc.bodies.p, C, flag, offs, w.subCount > 0 ? w.bodySub.p : nullptr,
@@ -206,7 +206,7 @@ bool diff_refused(const World& w)
 	return true;
 }
 
-bool diff_record_substep(World& w, int sub)
+bool diff_record_substep(World& w, int sub, const uint32_t* atOf, const int4* rec)
 {
 	DiffState* D = w.diff;
 	if (D == nullptr || !D->recording || D->steps.empty())
@@ -215,9 +215,9 @@ bool diff_record_substep(World& w, int sub)
 	if (R->nc == 0 || sub >= R->substeps)
 		return true;
 	int64_t base = (int64_t)sub * R->nc;
-	k_diff_gather_sub<<<gridk(R->nc), 256, 0, w.stream>>>(make_contact_view(w), R->idx.p, R->nc, base, R->normal.p,
-														  R->cmat.p, R->pA.p, R->pB.p, R->pFA.p, R->pFB.p,
-														  R->pSlip.p);
+	k_diff_gather_sub<<<gridk(R->nc), 256, 0, w.stream>>>(make_contact_view(w), R->idx.p, R->nc, base, atOf, rec,
+														  w.contactColor.p, R->normal.p, R->cmat.p, R->pA.p, R->pB.p,
+														  R->pFA.p, R->pFB.p, R->pSlip.p, R->sched.p);
 	return true;
 }
 
@@ -548,6 +548,7 @@ static int jvp_host(World& w, int first, int count, const double* stateIn, const
 	std::vector<uint32_t> flags;
 	std::vector<int32_t> cstart;
 	std::vector<int2> bodies;
+	std::vector<int32_t> sched;
 	std::vector<char> mem;
 	for (int k = 0; k < count; ++k)
 	{
@@ -564,7 +565,7 @@ static int jvp_host(World& w, int first, int count, const double* stateIn, const
 				  down(cstart, R->contactStart, (size_t)W + 1) && down(bodies, R->bodies, (size_t)R->nc) &&
 				  down(normal, R->normal, ncs) && down(cmat, R->cmat, ncs) && down(pA, R->pA, 4 * ncs) &&
 				  down(pB, R->pB, 4 * ncs) && down(pFA, R->pFA, 4 * ncs) && down(pFB, R->pFB, 4 * ncs) &&
-				  down(pSlip, R->pSlip, 4 * ncs);
+				  down(pSlip, R->pSlip, 4 * ncs) && down(sched, R->sched, ncs);
 		if (!ok)
 		{
 			set_error("jvp: record download failed");
@@ -594,6 +595,7 @@ static int jvp_host(World& w, int first, int count, const double* stateIn, const
 			in.pFA = pFA.data() + 4 * (int64_t)c0;
 			in.pFB = pFB.data() + 4 * (int64_t)c0;
 			in.pSlip = pSlip.data() + 4 * (int64_t)c0;
+			in.sched = sched.data() + c0;
 			in.cStride = R->nc;
 			in.dt = R->dt;
 			in.substeps = R->substeps;

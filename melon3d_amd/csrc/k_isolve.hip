@@ -1565,9 +1565,9 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		w.mark_begin(M3DX_K_GIANT_PREPARE);
 		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_end();
-		if (!diff_record_substep(w, sub))
-			return false;
 		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
+		if (!diff_record_substep(w, sub, S.atOf, S.rec))
+			return false;
 		bool ok = true;
 		// one launch for the position stage unless joints interleave with the
 		// normal sweeps or the passes are timed one by one

@@ -1,8 +1,9 @@
 // Differentiable replay of the XPBD solve of one step (the four substeps of
-// src/xpbd.cpp:898-974 for serial islands: integrate, prepare, normal
-// projection iterations, stiction, velocity reconstruction, velocity pass)
-// over a fixed contact set: the solver contacts and manifolds the engine's
-// narrowphase produced for that step.
+// src/xpbd.cpp:898-974 for serial islands and :1167-1369 for colored ones:
+// integrate, prepare, normal projection iterations, stiction, velocity
+// reconstruction, velocity pass) over a fixed contact set and coloring: the
+// solver contacts and manifolds the engine's narrowphase produced for that
+// step, and the colors and packet forms its solve used.
 //
 // Everything is written once over a generic scalar T:
 //   float      the same expressions in the same order as mx_solver.h, so a
@@ -580,9 +581,12 @@ struct StepIn
 	const float4* pFA;    // friction anchors at step start
 	const float4* pFB;
 	const float4* pSlip;  // slip direction
-	// the manifold fields above hold one block per substep (the manifold
-	// each substep's prepare left, re-collided or not): substep s reads the
-	// block at s * cStride contacts
+	// per contact: the schedule word (normal projection form, pass-order
+	// key; sched_form, sched_key); null: every contact serial, key 0
+	const int32_t* sched;
+	// the manifold fields above and the schedule words hold one block per
+	// substep (the manifold each substep's prepare left, re-collided or not):
+	// substep s reads the block at s * cStride contacts
 	int64_t cStride;
 	float dt;
 	int substeps;
@@ -813,6 +817,136 @@ MX_UNROLL_POINTS
 		axis_correction(pb, qb, imB, dl, n, jB[j]);
 		axis_correction(pa, qa, imA, -dl, n, jA[j]);
 	}
+}
+
+// SolvePacketNormals (src/xpbd.cpp:1047-1141) seen from one lane: the
+// contacts of one color are solved side by side in packets of 8, each lane
+// taking its points in sequence up to the packet's largest point count
+// (maxPoints). A point beyond the lane's own count is the zero point with
+// invW = 1 and a zero gate: it moves nothing, yet both rotations are still
+// renormalized. Both bodies' lane copies are updated (a static body's copy
+// only by the renormalization; the caller keeps the dynamic ones). The
+// expressions are those of mx_solver.h solve_normals_packet_in, in order.
+template <class T>
+MX_HD void normals_packet_op(const ContactGeo& g, int maxPoints, V3<T>& pa, Q4<T>& qa, V3<T>& pb, Q4<T>& qb, T imA,
+							 T imB, T* lambdaN, const T* wN, const V3<T>* jA, const V3<T>* jB, float maxCorrection)
+{
+	typedef typename Num<T>::F F;
+	const T zero = Num<T>::c(0.0);
+	const V3<T> zero3 = v3<T>(zero, zero, zero);
+	const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	const F nx = (F)g.n.x, ny = (F)g.n.y, nz = (F)g.n.z;
+MX_UNROLL_POINTS
+	for (int j = 0; j < 4; ++j)
+	{
+		if (j >= maxPoints)
+			break;
+		const bool on = j < g.count;
+		const float4 a4 = on ? g.pa(j) : zero4;
+		const float4 b4 = on ? g.pb(j) : zero4;
+		const V3<T> ja = on ? jA[j] : zero3;
+		const V3<T> jb = on ? jB[j] : zero3;
+		const F active = on ? (F)1 : (F)0;
+		const T invW = (on && val(wN[j]) > (F)0) ? (F)1 / wN[j] : Num<T>::c(1.0);
+
+		F vx = (F)a4.x, vy = (F)a4.y, vz = (F)a4.z;
+		T tx = (F)2 * (qa.y * vz - qa.z * vy);
+		T ty = (F)2 * (qa.z * vx - qa.x * vz);
+		T tz = (F)2 * (qa.x * vy - qa.y * vx);
+		T rax = vx + qa.w * tx + (qa.y * tz - qa.z * ty);
+		T ray = vy + qa.w * ty + (qa.z * tx - qa.x * tz);
+		T raz = vz + qa.w * tz + (qa.x * ty - qa.y * tx);
+		vx = (F)b4.x;
+		vy = (F)b4.y;
+		vz = (F)b4.z;
+		tx = (F)2 * (qb.y * vz - qb.z * vy);
+		ty = (F)2 * (qb.z * vx - qb.x * vz);
+		tz = (F)2 * (qb.x * vy - qb.y * vx);
+		T rbx = vx + qb.w * tx + (qb.y * tz - qb.z * ty);
+		T rby = vy + qb.w * ty + (qb.z * tx - qb.x * tz);
+		T rbz = vz + qb.w * tz + (qb.x * ty - qb.y * tx);
+
+		T sep = (F)a4.w + (pb.x + rbx - pa.x - rax) * nx + (pb.y + rby - pa.y - ray) * ny +
+				(pb.z + rbz - pa.z - raz) * nz;
+		const F gate = val(sep) < (F)0 ? active : (F)0;
+		T capped = val(sep) < (F)(-maxCorrection) ? Num<T>::c(-maxCorrection) : sep;
+		T dl = -capped * invW * gate;
+		if (on)
+			lambdaN[j] = lambdaN[j] + dl;
+
+		pb.x = pb.x + dl * imB * nx;
+		pb.y = pb.y + dl * imB * ny;
+		pb.z = pb.z + dl * imB * nz;
+		pa.x = pa.x - dl * imA * nx;
+		pa.y = pa.y - dl * imA * ny;
+		pa.z = pa.z - dl * imA * nz;
+
+		T dwx = dl * jb.x, dwy = dl * jb.y, dwz = dl * jb.z;
+		T dqx = dwx * qb.w + dwy * qb.z - dwz * qb.y;
+		T dqy = -dwx * qb.z + dwy * qb.w + dwz * qb.x;
+		T dqz = dwx * qb.y - dwy * qb.x + dwz * qb.w;
+		T dqw = -(dwx * qb.x + dwy * qb.y + dwz * qb.z);
+		T x = qb.x + (F)0.5 * dqx, y = qb.y + (F)0.5 * dqy, z = qb.z + (F)0.5 * dqz, w = qb.w + (F)0.5 * dqw;
+		T s = (F)1.5 - (F)0.5 * (x * x + y * y + z * z + w * w);
+		qb.x = x * s;
+		qb.y = y * s;
+		qb.z = z * s;
+		qb.w = w * s;
+
+		dwx = -dl * ja.x;
+		dwy = -dl * ja.y;
+		dwz = -dl * ja.z;
+		dqx = dwx * qa.w + dwy * qa.z - dwz * qa.y;
+		dqy = -dwx * qa.z + dwy * qa.w + dwz * qa.x;
+		dqz = dwx * qa.y - dwy * qa.x + dwz * qa.w;
+		dqw = -(dwx * qa.x + dwy * qa.y + dwz * qa.z);
+		x = qa.x + (F)0.5 * dqx;
+		y = qa.y + (F)0.5 * dqy;
+		z = qa.z + (F)0.5 * dqz;
+		w = qa.w + (F)0.5 * dqw;
+		s = (F)1.5 - (F)0.5 * (x * x + y * y + z * z + w * w);
+		qa.x = x * s;
+		qa.y = y * s;
+		qa.z = z * s;
+		qa.w = w * s;
+	}
+}
+
+// A contact's schedule word in a step's record (one per contact and
+// substep): bits 0-7 the normal projection's form, kFormSerial for the
+// serial form or the packet's largest point count (0-4) for the packet form;
+// bits 8-11 the pass-order key: the contact's color (0-7, 8 for the overflow
+// pass) in a colored island, 0 in any other island.
+constexpr int32_t kFormSerial = 0xFF;
+constexpr int kOrderKeys = 9;
+MX_HD int sched_form(const StepIn& in, int32_t c) { return in.sched != nullptr ? (in.sched[c] & 0xFF) : kFormSerial; }
+MX_HD int sched_key(const StepIn& in, int32_t c) { return in.sched != nullptr ? ((in.sched[c] >> 8) & 0xF) : 0; }
+
+// One more than the largest pass-order key of the step's contacts: the passes
+// run key by key, ascending contacts within a key. That is every island's own
+// order (src/xpbd.cpp:898-974 serial; :1269-1368 colored: colors in turn, then
+// the overflow contacts), since islands touch disjoint dynamic bodies.
+MX_HD int sched_keys(const StepIn& in)
+{
+	int k = 0;
+	for (int32_t c = 0; c < in.nc; ++c)
+	{
+		int kc = sched_key(in, c) + 1;
+		k = kc > k ? kc : k;
+	}
+	return k;
+}
+
+// The normal projection of contact c in its form.
+template <class T>
+MX_HD void normals_any(const StepIn& in, int32_t c, const ContactGeo& g, V3<T>& pa, Q4<T>& qa, V3<T>& pb, Q4<T>& qb,
+					   T imA, T imB, T* lambdaN, const T* wN, const V3<T>* jA, const V3<T>* jB, float maxCorrection)
+{
+	const int form = sched_form(in, c);
+	if (form == kFormSerial)
+		normals_op(g, pa, qa, pb, qb, imA, imB, lambdaN, wN, jA, jB, maxCorrection);
+	else
+		normals_packet_op(g, form, pa, qa, pb, qb, imA, imB, lambdaN, wN, jA, jB, maxCorrection);
 }
 
 // src/xpbd.cpp:85-89
@@ -1120,7 +1254,7 @@ MX_HD void normals(const StepIn& in, StepState<T>& S, int32_t c, float maxCorrec
 		jA[j] = S.jA[p];
 		jB[j] = S.jB[p];
 	}
-	normals_op(g, pa, qa, pb, qb, S.invMass[ia], S.invMass[ib], lN, wN, jA, jB, maxCorrection);
+	normals_any(in, c, g, pa, qa, pb, qb, S.invMass[ia], S.invMass[ib], lN, wN, jA, jB, maxCorrection);
 	for (int j = 0; j < g.count; ++j)
 		S.lambdaN[c * 4 + j] = lN[j];
 	if (g.dynA)
@@ -1203,12 +1337,15 @@ MX_HD void velocity(const StepIn& in, StepState<T>& S, int32_t c, float h, float
 	store_body(S, ib, b, in);
 }
 
-// One substep (src/xpbd.cpp:898-974) with the manifold block of `in`:
-// integrate, prepare, iterations x normals, stiction, reconstruct, velocity.
+// One substep (src/xpbd.cpp:898-974 serial, :1269-1368 colored) with the
+// manifold block of `in`: integrate, prepare, iterations x normals,
+// stiction, reconstruct, velocity; the contact passes in the step's pass
+// order (sched_keys).
 template <class T>
 MX_HD void solve_substep(const StepIn& in, StepState<T>& S, int iterations, float h, float invH, float maxCorrection,
 						 float gravityLen)
 {
+	const int keys = sched_keys(in);
 	for (int32_t c = 0; c < in.nc; ++c)
 		for (int j = 0; j < 4; ++j)
 		{
@@ -1221,18 +1358,24 @@ MX_HD void solve_substep(const StepIn& in, StepState<T>& S, int iterations, floa
 	for (int32_t c = 0; c < in.nc; ++c)
 		prepare(in, S, c);
 	for (int it = 0; it < iterations; ++it)
+		for (int k = 0; k < keys; ++k)
+			for (int32_t c = 0; c < in.nc; ++c)
+				if (sched_key(in, c) == k)
+					normals(in, S, c, maxCorrection);
+	for (int k = 0; k < keys; ++k)
 		for (int32_t c = 0; c < in.nc; ++c)
-			normals(in, S, c, maxCorrection);
-	for (int32_t c = 0; c < in.nc; ++c)
-	{
-		int2 ab = in.bodies[c];
-		if (bf_shape(in.flags[ab.x]) == M3D_SHAPE_BOX && bf_shape(in.flags[ab.y]) == M3D_SHAPE_BOX)
-			friction(in, S, c);
-	}
+		{
+			int2 ab = in.bodies[c];
+			if (sched_key(in, c) == k && bf_shape(in.flags[ab.x]) == M3D_SHAPE_BOX &&
+				bf_shape(in.flags[ab.y]) == M3D_SHAPE_BOX)
+				friction(in, S, c);
+		}
 	for (int32_t i = 0; i < in.nb; ++i)
 		reconstruct(in, S, i, invH);
-	for (int32_t c = 0; c < in.nc; ++c)
-		velocity(in, S, c, h, gravityLen);
+	for (int k = 0; k < keys; ++k)
+		for (int32_t c = 0; c < in.nc; ++c)
+			if (sched_key(in, c) == k)
+				velocity(in, S, c, h, gravityLen);
 }
 
 // The manifold block of substep s (blocks of cStride contacts).
@@ -1246,6 +1389,8 @@ MX_HD StepIn substep_in(const StepIn& in, int s)
 	ins.pFA = in.pFA + 4 * s * in.cStride;
 	ins.pFB = in.pFB + 4 * s * in.cStride;
 	ins.pSlip = in.pSlip + 4 * s * in.cStride;
+	if (in.sched != nullptr)
+		ins.sched = in.sched + s * in.cStride;
 	return ins;
 }
 

@@ -36,6 +36,7 @@ struct DiffStepRec
 	// one block per substep: the manifold the substep's prepare left
 	DBuf<float4> normal, cmat;            // [s * nc + c]
 	DBuf<float4> pA, pB, pFA, pFB, pSlip; // [(s * nc + c) * 4 + j]
+	DBuf<int32_t> sched;                  // [s * nc + c]: schedule word (mx_diff.h)
 	void release()
 	{
 		DBuf<float4>* f[] = { &pos, &rot, &vel, &angVel, &invIL, &shape, &bmat, &normal, &cmat, &pA, &pB, &pFA, &pFB, &pSlip };
@@ -46,6 +47,7 @@ struct DiffStepRec
 		contactStart.release();
 		bodies.release();
 		idx.release();
+		sched.release();
 	}
 };
 
@@ -53,7 +55,7 @@ struct DiffState
 {
 	bool recording = false;
 	// step at which recording was refused (-1: none): a step whose solve the
-	// adjoint does not cover (a colored island, or joints) stops the
+	// adjoint does not cover (joints) stops the
 	// recording, the step itself runs as usual, and every later use of the
 	// records fails with `refusal`
 	int32_t refusedStep = -1;
@@ -120,6 +122,7 @@ struct RecView
 	const int32_t* contactStart;
 	const int2* bodies;
 	const float4 *normal, *cmat, *pA, *pB, *pFA, *pFB, *pSlip;
+	const int32_t* sched;
 	int64_t cStride;
 	int32_t worlds;
 	float dt;
@@ -150,6 +153,7 @@ inline RecView rec_view(const DiffState* D, const DiffStepRec* R, int iterations
 	V.pFA = R->pFA.p;
 	V.pFB = R->pFB.p;
 	V.pSlip = R->pSlip.p;
+	V.sched = R->sched.p;
 	V.cStride = R->nc;
 	V.worlds = D->worlds;
 	V.dt = R->dt;
@@ -183,6 +187,7 @@ MX_HD StepIn world_in(const RecView& V, int32_t wi)
 	in.pFA = V.pFA + 4 * (int64_t)c0;
 	in.pFB = V.pFB + 4 * (int64_t)c0;
 	in.pSlip = V.pSlip + 4 * (int64_t)c0;
+	in.sched = V.sched != nullptr ? V.sched + c0 : nullptr;
 	in.cStride = V.cStride;
 	in.dt = V.dt;
 	in.substeps = V.substeps;

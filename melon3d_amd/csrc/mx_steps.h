@@ -118,8 +118,9 @@ bool batch_destroy_contacts(World& w, int32_t body, std::vector<int4>& events);
 // ---- k_diff.hip ----
 // records the step's solve inputs when differentiable recording is on
 bool diff_record_step(World& w, const IslandLists& L, float dt, int substeps);
-// records the manifold each substep's prepare left
-bool diff_record_substep(World& w, int sub);
+// records the manifold each substep's prepare left and the contacts' schedule
+// words (after the solve records hold the substep's packet forms)
+bool diff_record_substep(World& w, int sub, const uint32_t* atOf, const int4* rec);
 bool diff_is_recording(const World& w);
 // true (and the error set) when the recording was refused
 bool diff_refused(const World& w);

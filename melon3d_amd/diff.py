@@ -200,8 +200,8 @@ class _RecordedWorld:
 def record_rollout(L: Library, scene: DiffScene, v0, friction, mass, steps: int, substeps: int) -> _RecordedWorld:
     """Builds the scene's batch with these parameters and steps it `steps`
     times with recording on. Raises if a step failed or the recording was
-    refused (a colored island or joints: the adjoint would not describe the
-    steps the engine ran)."""
+    refused (joints: the adjoint would not describe the steps the engine
+    ran)."""
     owner = _RecordedWorld(L, scene.build(L, v0, friction, mass))
     w = owner.w
     if L.m3dx_diff_record(w, 1) != 0:

@@ -299,39 +299,302 @@ def test_torch_function_second_backward_and_release(dev):
     assert not release.alive
 
 
+def _raft(dev, settle):
+    """The 15 x 15 raft of touching boxes on the ground with five boxes
+    dropping onto it (test/test_main.cpp:960-985, sleep off): one island of
+    about 650 contacts, which the engine solves with the colored schedule
+    (greedy colors in 8-lane packets, then the overflow pass). Returns the
+    world, its dynamic bodies' indices in the recorded state arrays and their
+    ids."""
+    from melon3d_amd import scenes
+
+    wd = dev.m3d_world_def_default()
+    wd.enableSleep = False
+    w = dev.create_world(wd)
+    scenes.add_ground(dev, w)
+    ids = [scenes.add_box(dev, w, (-7.0 + i, 0.5, -7.0 + k), (0.501, 0.5, 0.501)) for i in range(15) for k in range(15)]
+    ids += [scenes.add_box(dev, w, (2.0 * i - 4.0, 3.0 + 0.7 * i, 0.5 * i - 1.0), (0.45, 0.45, 0.45)) for i in range(5)]
+    for _ in range(settle):
+        dev.m3d_world_step(w, DT, 4)
+    return w, np.arange(1, 231), ids
+
+
+def _record_raft(dev, settle, steps):
+    w, dyn, ids = _raft(dev, settle)
+    assert dev.m3dx_diff_record(w, 1) == 0, dev.last_error()
+    colored = []
+    for _ in range(steps):
+        dev.m3d_world_step(w, DT, 4)
+        colored.append(dev.stats(w).coloredIslands)
+    assert dev.m3dx_diff_record(w, 0) == 0, dev.last_error()
+    assert dev.m3dx_diff_steps(w) == steps
+    assert dev.m3dx_step_failures(w) == 0, dev.last_error()
+    assert min(colored) >= 1, colored
+    return w, dyn, ids
+
+
 @pytest.mark.gpu
-def test_recording_refuses_colored_islands(dev):
-    """A 15 x 15 raft of boxes on the ground (test/test_main.cpp:960-985) is
-    one island of well over 300 contacts, which the engine solves with the
-    colored schedule: recording refuses it instead of returning the gradient
-    of the serial schedule. The steps still run as usual."""
+def test_colored_raft_replay_is_the_engine(dev):
+    """Colored steps replayed in float reproduce the engine bit for bit: the
+    single-lane replay and the parallel adjoint's forward kernels, each step
+    from its recorded start against the next recorded start, and 8 chained
+    steps against the engine's final poses. The steps record the boxes
+    landing on the raft (steps 41-48), so packets of mixed point counts and
+    the overflow pass both occur."""
+    import ctypes as C
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    steps = 8
+    w, dyn, ids = _record_raft(dev, 40, steps)
+    try:
+        n = 231
+        for t in range(steps - 1):
+            nxt = np.zeros((n, 13), np.float32)
+            assert dev.m3dx_diff_state(w, t + 1, ptr(nxt)) >= 300
+            out = np.zeros((n, 13), np.float32)
+            assert dev.m3dx_diff_replay(w, t, 1, 0, None, None, None, ptr(out)) == 0, dev.last_error()
+            assert np.array_equal(out[dyn], nxt[dyn]), t
+            par = np.zeros((n, 13), np.float32)
+            assert dev.m3dx_diff_forward(w, t, 1, ptr(par)) == 0, dev.last_error()
+            assert np.array_equal(par[dyn], nxt[dyn]), t
+        out = np.zeros((n, 13), np.float32)
+        assert dev.m3dx_diff_forward(w, 0, steps, ptr(out)) == 0
+        engine = np.array([[getattr(dev.m3d_body_position(w, b), a) for a in "xyz"] for b in ids], np.float32)
+        assert np.array_equal(out[dyn, 0:3], engine)
+    finally:
+        dev.m3d_world_destroy(w)
+
+
+@pytest.mark.gpu
+def test_colored_raft_adjoint(dev):
+    """The adjoint of colored steps (4 steps of boxes landing on the raft).
+
+    - The parallel adjoint equals the reverse-mode tape to float rounding and
+      is the same for checkpoint intervals 1 and 10.
+    - Dot-product test: <J^T y, u> from the adjoint equals <y, J u> from FP64
+      forward tangents on the host CPU along the same float path, for random
+      directions in the initial velocities, mass and friction.
+    - The derivative code against FP64 central differences: FP64 tangents
+      along the double replay equal central differences of the double
+      replay of a linear loss on the final poses, to 1e-2 of the
+      directions' RMS derivative. The stencil is h = 1e-10 (velocities in
+      m/s): the velocity pass applies friction only where the tangential
+      speed exceeds 1e-9 (src/xpbd.cpp:326), and on resting boxes the
+      previous friction impulse leaves it just below that, so any larger
+      stencil crosses into the friction branch and measures that branch's
+      derivative instead (central differences at 1e-6 .. 1e-9 agree with
+      each other and not with either path's tangents). At this stencil the
+      double replay's rounding (about 1e-15 m on coordinates of up to 7 m)
+      leaves central differences good to a few 1e-3.
+    The float and double replays themselves drift apart by about 1e-4 m over
+    these steps, which moves their derivatives by about 1 %: the adjoint is
+    the derivative of the engine's own float path. The thrown lattice below
+    compares the adjoint with central differences directly."""
+    import ctypes as C
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    steps = 4
+    w, dyn, _ = _record_raft(dev, 40, steps)
+    try:
+        n = 231
+        rng = np.random.default_rng(17)
+        seed = np.zeros((n, 13))
+        seed[dyn] = rng.normal(size=(len(dyn), 13))
+        seed32 = seed.astype(np.float32)
+        # tape against the parallel adjoint, and checkpoint intervals
+        at, mt, ft = (np.zeros((n, 13), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32))
+        assert dev.m3dx_diff_backward_tape(w, ptr(seed32), ptr(at), ptr(mt), ptr(ft)) == 0, dev.last_error()
+        res = []
+        for k in (1, 10):
+            assert dev.m3dx_diff_set_checkpoint(w, k) == 0
+            r = (np.zeros((n, 13), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32))
+            assert dev.m3dx_diff_backward(w, ptr(seed32), ptr(r[0]), ptr(r[1]), ptr(r[2])) == 0, dev.last_error()
+            res.append(r)
+        for a, b in zip(res[0], res[1]):
+            assert np.array_equal(a, b)
+        ap, mp, fp = res[0]
+        for name, a, b in (("state", ap[dyn], at[dyn]), ("mass", mp[dyn], mt[dyn]), ("friction", fp, ft)):
+            a = a.astype(np.float64)
+            b = b.astype(np.float64)
+            err = np.abs(a - b).max() / (np.abs(b).max() + 1e-12)
+            assert err <= 1e-4, (name, err)
+        # FP64 adjoint, then the dot-product test against FP64 CPU tangents
+        adj = np.zeros((n, 13))
+        mg = np.zeros(n)
+        fg = np.zeros(n)
+        assert dev.m3dx_diff_backward_f64(w, ptr(seed), ptr(adj), ptr(mg), ptr(fg)) == 0, dev.last_error()
+        st0 = np.zeros((n, 13), np.float32)
+        assert dev.m3dx_diff_state(w, 0, ptr(st0)) >= 300
+        st0 = st0.astype(np.float64)
+        for _ in range(3):
+            u = np.zeros((n, 13))
+            u[dyn, 7:13] = rng.normal(size=(len(dyn), 6))
+            dmass = np.zeros(n)
+            dmass[dyn] = rng.normal(size=len(dyn)) * 10.0
+            dfric = rng.normal(size=n) * 0.05
+            out = np.zeros((n, 13))
+            dout = np.zeros((n, 13))
+            assert dev.m3dx_diff_jvp_cpu(w, 0, steps, 0, ptr(st0), ptr(u), None, ptr(dmass), None, ptr(dfric),
+                                         ptr(out), ptr(dout)) == 0, dev.last_error()
+            jvp = (seed * dout).sum()
+            vjp = (adj * u).sum() + (mg * dmass).sum() + (fg * dfric).sum()
+            assert abs(vjp - jvp) <= 1e-9 * abs(jvp), (vjp, jvp)
+        # FP64 tangents of the double replay against its central differences
+        y = np.zeros((n, 13))
+        y[dyn, 0:7] = rng.normal(size=(len(dyn), 7))
+        zero = np.zeros(n)
+
+        def loss(x0):
+            o = np.zeros((n, 13))
+            assert dev.m3dx_diff_replay(w, 0, steps, 1, ptr(np.ascontiguousarray(x0)), None, None, ptr(o)) == 0
+            return (y * o).sum()
+
+        tangents, fds = [], []
+        for _ in range(6):
+            u = np.zeros((n, 13))
+            u[dyn, 7:13] = rng.normal(size=(len(dyn), 6))
+            out = np.zeros((n, 13))
+            dout = np.zeros((n, 13))
+            assert dev.m3dx_diff_jvp_cpu(w, 0, steps, 1, ptr(st0), ptr(u), None, ptr(zero), None, ptr(zero),
+                                         ptr(out), ptr(dout)) == 0, dev.last_error()
+            tangents.append((y * dout).sum())
+            fds.append((loss(st0 + 1e-10 * u) - loss(st0 - 1e-10 * u)) / 2e-10)
+        tangents, fds = np.array(tangents), np.array(fds)
+        scale = np.sqrt((fds ** 2).mean())
+        assert np.abs(tangents - fds).max() <= 1e-2 * scale, (tangents, fds)
+    finally:
+        dev.m3d_world_destroy(w)
+
+
+def _lattice(dev, rng):
+    """A 6 x 6 x 6 lattice of unit spheres and cubes (alternating, 2 cm
+    gaps) thrown with random linear and angular velocities, sleep off: one
+    island of more than 500 contacts, solved with the colored schedule, whose
+    contacts are in flight (sliding) rather than resting."""
+    import ctypes as C
+
+    from melon3d_amd.api import M3D_BODY_DYNAMIC, M3D_SHAPE_BOX, M3D_SHAPE_SPHERE, m3d_vec3
+
+    wd = dev.m3d_world_def_default()
+    wd.enableSleep = False
+    w = dev.create_world(wd)
+    for i in range(6):
+        for j in range(6):
+            for k in range(6):
+                bd = dev.m3d_body_def_default()
+                bd.type = M3D_BODY_DYNAMIC
+                bd.position = m3d_vec3(1.02 * i, 1.02 * j, 1.02 * k)
+                bd.linearVelocity = m3d_vec3(*rng.normal(size=3))
+                bd.angularVelocity = m3d_vec3(*rng.normal(size=3))
+                sd = dev.m3d_shape_def_default()
+                if (i + j + k) % 2 == 0:
+                    sd.type = M3D_SHAPE_SPHERE
+                    sd.radius = 0.5
+                else:
+                    sd.type = M3D_SHAPE_BOX
+                    sd.halfExtents = m3d_vec3(0.5, 0.5, 0.5)
+                sd.friction = 0.6
+                dev.m3d_body_create(w, C.byref(bd), C.byref(sd))
+    return w
+
+
+@pytest.mark.gpu
+def test_colored_lattice_adjoint_matches_fp64_central_differences(dev):
+    """Directional derivatives of a linear loss on the final poses after 2
+    colored steps of the thrown lattice, along random directions in the
+    initial linear and angular velocities, against FP64 central differences
+    of the double replay (h = 1e-5), wherever central differences at h and
+    h / 2 agree to 1e-4 (no branch flips inside the stencil; at least half
+    the directions qualify):
+    - the adjoint (FP64 along the engine's float path): rel err <= 1e-3;
+    - FP64 tangents along the double replay itself: rel err <= 1e-6.
+    Over a third step a branch decision of the float and double replays
+    differs (their final poses part by 0.16 m) and the two paths'
+    derivatives by about 10 %, so the horizon stays at 2 steps."""
+    import ctypes as C
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    rng = np.random.default_rng(29)
+    w = _lattice(dev, rng)
+    try:
+        steps = 2
+        assert dev.m3dx_diff_record(w, 1) == 0, dev.last_error()
+        colored = []
+        for _ in range(steps):
+            dev.m3d_world_step(w, DT, 4)
+            colored.append(dev.stats(w).coloredIslands)
+        assert dev.m3dx_diff_record(w, 0) == 0, dev.last_error()
+        assert min(colored) >= 1, colored
+        n = 216
+        st0 = np.zeros((n, 13), np.float32)
+        assert dev.m3dx_diff_state(w, 0, ptr(st0)) >= 300
+        st0 = st0.astype(np.float64)
+        y = rng.normal(size=(n, 13))
+        y[:, 7:13] = 0.0
+        g = np.zeros((n, 13))
+        assert dev.m3dx_diff_backward_f64(w, ptr(y), ptr(g), None, None) == 0, dev.last_error()
+        zero = np.zeros(n)
+
+        def loss(x0):
+            o = np.zeros((n, 13))
+            assert dev.m3dx_diff_replay(w, 0, steps, 1, ptr(np.ascontiguousarray(x0)), None, None, ptr(o)) == 0
+            return (y * o).sum()
+
+        smooth = 0
+        for _ in range(8):
+            u = np.zeros((n, 13))
+            u[:, 7:13] = rng.normal(size=(n, 6))
+            fd1 = (loss(st0 + 1e-5 * u) - loss(st0 - 1e-5 * u)) / 2e-5
+            fd2 = (loss(st0 + 5e-6 * u) - loss(st0 - 5e-6 * u)) / 1e-5
+            if abs(fd1 - fd2) > 1e-4 * abs(fd1):
+                continue
+            smooth += 1
+            adj = (g * u).sum()
+            assert abs(adj - fd1) <= 1e-3 * abs(fd1), (adj, fd1, fd2)
+            out = np.zeros((n, 13))
+            dout = np.zeros((n, 13))
+            assert dev.m3dx_diff_jvp_cpu(w, 0, steps, 1, ptr(st0), ptr(u), None, ptr(zero), None, ptr(zero),
+                                         ptr(out), ptr(dout)) == 0, dev.last_error()
+            tan = (y * dout).sum()
+            assert abs(tan - fd1) <= 1e-6 * abs(fd1), (tan, fd1, fd2)
+        assert smooth >= 4, smooth
+    finally:
+        dev.m3d_world_destroy(w)
+
+
+@pytest.mark.gpu
+def test_recording_refuses_joints(dev):
+    """Joints are not replayed by the adjoint: a world holding one is
+    refused instead of returning the gradient of a solve without them. The
+    steps still run as usual."""
     import ctypes as C
 
     from melon3d_amd import scenes
+    from melon3d_amd.api import m3d_ball_joint_def, m3d_vec3
 
     wd = dev.m3d_world_def_default()
     wd.enableSleep = False
     w = dev.create_world(wd)
     try:
         scenes.add_ground(dev, w)
-        raft = [scenes.add_box(dev, w, (-7.0 + i, 0.5, -7.0 + k), (0.501, 0.5, 0.501)) for i in range(15) for k in range(15)]
+        a = scenes.add_box(dev, w, (0.0, 0.5, 0.0), (0.5, 0.5, 0.5))
+        b = scenes.add_box(dev, w, (0.0, 3.0, 0.0), (0.3, 0.3, 0.3))
+        bj = m3d_ball_joint_def()
+        bj.bodyA, bj.bodyB = a, b
+        bj.localAnchorA = m3d_vec3(0.0, 1.5, 0.0)
+        bj.localAnchorB = m3d_vec3(0.0, -1.0, 0.0)
+        dev.m3d_joint_create_ball(w, C.byref(bj))
         assert dev.m3dx_diff_record(w, 1) == 0
         for _ in range(5):
             dev.m3d_world_step(w, 1.0 / 60.0, 4)
-        assert dev.stats(w).coloredIslands >= 1
         assert dev.m3dx_step_failures(w) == 0
         assert dev.m3dx_diff_steps(w) == 0
         assert dev.m3dx_diff_record(w, 0) == -1
-        assert "colored schedule" in dev.last_error()
-        n = 226
-        seed = np.zeros(n * 13)
-        out = np.zeros(n * 13)
-        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        assert "joints" in dev.last_error()
+        seed = np.zeros(3 * 13)
+        out = np.zeros(3 * 13)
+        ptr = lambda v: v.ctypes.data_as(C.c_void_p)
         assert dev.m3dx_diff_backward_f64(w, ptr(seed), ptr(out), None, None) == -1
-        assert "colored schedule" in dev.last_error()
-        # the engine kept stepping: the raft rests on the ground
-        ys = [dev.m3d_body_position(w, b).y for b in raft]
-        assert max(abs(y - 0.5) for y in ys) < 0.01
+        assert "joints" in dev.last_error()
+        # the engine kept stepping: the box rests on the ground
+        assert abs(dev.m3d_body_position(w, a).y - 0.5) < 0.01
     finally:
         dev.m3d_world_destroy(w)
 
