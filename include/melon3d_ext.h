@@ -279,6 +279,14 @@ M3D_API int32_t m3dx_step_failures(m3d_world* world);
 M3D_API int m3dx_set_step_graphs(m3d_world* world, int enable);
 /* Steps whose substep loop ran as a graph. */
 M3D_API int64_t m3dx_step_graph_launches(m3d_world* world);
+/* Grid barriers of the cooperative solver passes in XCD mode (default on
+   where the device's XCC_ID register names its 8 XCDs, checked once per
+   device; M3D_XCD_BARRIER=0 turns it off for the process): the workgroups of
+   a launch are grouped by the XCD they run on, and each barrier writes back
+   and invalidates each XCD's L2 once instead of once per workgroup. enable =
+   0 fences every workgroup at the agent scope. Results do not depend on it.
+   Returns 0 or -1. */
+M3D_API int m3dx_set_xcd_barrier(m3d_world* world, int enable);
 /* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
 M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
 

@@ -335,6 +335,7 @@ _SIGS = {
     "m3dx_pass_blocks": (C.c_int, [W]),
     "m3dx_step_failures": (C.c_int32, [W]),
     "m3dx_set_step_graphs": (C.c_int, [W, C.c_int]),
+    "m3dx_set_xcd_barrier": (C.c_int, [W, C.c_int]),
     "m3dx_step_graph_launches": (C.c_int64, [W]),
     "m3dx_kernel_timing": (C.c_int, [W, C.POINTER(m3dx_kernel_time)]),
     "m3dx_batch_create": (W, [C.POINTER(m3d_world_def), C.c_int32]),

@@ -237,6 +237,7 @@ struct IslandCtx
 	m3d_vec3 gravity;
 	float invH;
 	int32_t iterations, substeps, I, NB, NC;
+	uint32_t xcdBar; // grid barriers in XCD mode (GridSync)
 };
 
 // ------------------------------------------------------------- joints
@@ -324,38 +325,146 @@ __device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
 // group's last arrival passes them on (acquire-release) to the top counter,
 // and the acquire fence drops stale cache lines before a workgroup reads
 // others' writes.
+//
+// XCD mode (xcd != 0). The MI355X's 8 XCDs each have their own L2. An
+// agent-scope release writes the whole L2 back (buffer_wbl2) and an acquire
+// invalidates it (buffer_inv sc1); with every workgroup doing both, a barrier
+// cost 3.3 us at 32 workgroups and 10.3 us at 256 (the write-backs serialize
+// in each L2). In XCD mode the groups are the XCDs themselves: at the launch
+// every workgroup reads its XCD (the XCC_ID register) and counts itself in a
+// census, one barrier of the plain kind makes the census final, and a group's
+// size is its census count (the dispatcher's XCD for workgroup b rotates from
+// launch to launch, so b % 8 is not the XCD). Every wave waits for its own
+// stores to reach the L2 before the workgroup arrives, and only the group's
+// last arrival writes the L2 back (the group's stores with it) before
+// arriving on the top counter: one write-back per XCD. Every workgroup still
+// invalidates for itself after the top counter: an invalidation orders only
+// the issuing CU's later loads, and a variant with one invalidation per XCD
+// handed on by a flag let other CUs read stale lines (a coherence probe with
+// concurrent readers counted them; none with per-workgroup invalidation).
+// Measured: 3.0 us at 64 workgroups (4.3 plain), 4.7 us at 256 (10.3).
+// Every group counter receives exactly its census count of arrivals per
+// barrier and the top counter one per nonempty group, so the waits end as in
+// the plain kind.
 constexpr uint32_t kBarGroups = 8;
-constexpr uint32_t kBarStride = 32;                         // u32 per counter
-constexpr uint32_t kBarWords = (kBarGroups + 2) * kBarStride; // groups, top, leave
+constexpr uint32_t kBarStride = 32; // u32 per counter
+// counters (kBarStride words apart): group arrivals, top, leave, the XCD
+// census, and the census barrier's group arrivals and top
+constexpr uint32_t kBarTop = kBarGroups;
+constexpr uint32_t kBarLeave = kBarGroups + 1;
+constexpr uint32_t kBarCensus0 = kBarGroups + 2;
+constexpr uint32_t kBarCGroup0 = 2 * kBarGroups + 2;
+constexpr uint32_t kBarCTop = 3 * kBarGroups + 2;
+constexpr uint32_t kBarCounters = 3 * kBarGroups + 3;
+constexpr uint32_t kBarWords = kBarCounters * kBarStride;
+
+__device__ __forceinline__ uint32_t xcc_id()
+{
+	// hwreg(HW_REG_XCC_ID, 0, 4)
+	return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11));
+}
+
+// Lane 0 of every workgroup: one two-level barrier of the plain kind over
+// groups b % kBarGroups on the counters at groupBase / top (epoch `e`).
+__device__ __forceinline__ void plain_barrier_lane0(uint32_t* groupBase, uint32_t* top, uint32_t e)
+{
+	const uint32_t blocks = gridDim.x;
+	const uint32_t g = blockIdx.x % kBarGroups;
+	const uint32_t size = blocks / kBarGroups + (g < blocks % kBarGroups ? 1u : 0u);
+	const uint32_t groups = blocks < kBarGroups ? blocks : kBarGroups;
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+	uint32_t old = __hip_atomic_fetch_add(groupBase + g * kBarStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if (old + 1u == (e + 1u) * size)
+	{
+		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+		__hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	while (__hip_atomic_load(top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (e + 1u) * groups)
+		__builtin_amdgcn_s_sleep(1);
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
 
 struct GridSync
 {
 	uint32_t* base; // kBarWords words, zero at the launch
-	uint32_t blocks;
 	uint32_t epoch;
+	uint32_t xcd;    // XCD mode
+	uint32_t grp;    // this workgroup's group
+	uint32_t size;   // workgroups in the group
+	uint32_t groups; // nonempty groups
 
-	__device__ uint32_t* top() const { return base + kBarGroups * kBarStride; }
-
-	__device__ void arrive_lane0() const
+	// Called by every thread of the workgroup at the launch.
+	__device__ static GridSync make(uint32_t* base, uint32_t xcdMode)
 	{
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-		const uint32_t g = blockIdx.x % kBarGroups;
-		const uint32_t size = blocks / kBarGroups + (g < blocks % kBarGroups ? 1u : 0u);
-		uint32_t old = __hip_atomic_fetch_add(base + g * kBarStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		GridSync g;
+		g.base = base;
+		g.epoch = 0u;
+		g.xcd = xcdMode;
+		const uint32_t blocks = gridDim.x;
+		if (xcdMode == 0u)
+		{
+			g.grp = blockIdx.x % kBarGroups;
+			g.size = blocks / kBarGroups + (g.grp < blocks % kBarGroups ? 1u : 0u);
+			g.groups = blocks < kBarGroups ? blocks : kBarGroups;
+			return g;
+		}
+		__shared__ uint32_t census[3];
+		if (threadIdx.x == 0)
+		{
+			const uint32_t x = xcc_id() % kBarGroups;
+			__hip_atomic_fetch_add(base + (kBarCensus0 + x) * kBarStride, 1u, __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+			plain_barrier_lane0(base + kBarCGroup0 * kBarStride, base + kBarCTop * kBarStride, 0u);
+			uint32_t nonempty = 0u, mine = 0u;
+			for (uint32_t k = 0; k < kBarGroups; ++k)
+			{
+				uint32_t c = __hip_atomic_load(base + (kBarCensus0 + k) * kBarStride, __ATOMIC_RELAXED,
+											   __HIP_MEMORY_SCOPE_AGENT);
+				nonempty += c != 0u ? 1u : 0u;
+				mine = k == x ? c : mine;
+			}
+			census[0] = x;
+			census[1] = mine;
+			census[2] = nonempty;
+		}
+		__syncthreads();
+		g.grp = census[0];
+		g.size = census[1];
+		g.groups = census[2];
+		return g;
+	}
+
+	__device__ uint32_t* top() const { return base + kBarTop * kBarStride; }
+
+	__device__ void arrive_lane0()
+	{
+		if (xcd == 0u)
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+		uint32_t old = __hip_atomic_fetch_add(base + grp * kBarStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (old + 1u == (epoch + 1u) * size)
 		{
-			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+			if (xcd != 0u)
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+			else
+				__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
 			__hip_atomic_fetch_add(top(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
 	}
 
 	__device__ void wait_lane0() const
 	{
-		const uint32_t groups = blocks < kBarGroups ? blocks : kBarGroups;
 		const uint32_t target = (epoch + 1u) * groups;
 		while (__hip_atomic_load(top(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
 			__builtin_amdgcn_s_sleep(1);
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+	}
+
+	// every wave: its stores have reached the L2 (XCD mode; the agent release
+	// of the plain kind covers the workgroup)
+	__device__ void stores_done() const
+	{
+		if (xcd != 0u)
+			__builtin_amdgcn_s_waitcnt(0);
 	}
 
 	// Barrier; then returns *word as read by one lane after it (the same
@@ -364,6 +473,7 @@ struct GridSync
 	__device__ uint32_t sync_read(const uint32_t* word)
 	{
 		__shared__ uint32_t value;
+		stores_done();
 		__syncthreads();
 		if (threadIdx.x == 0)
 		{
@@ -383,6 +493,7 @@ struct GridSync
 	// workgroups' writes of this phase may be issued in between.
 	__device__ void arrive()
 	{
+		stores_done();
 		__syncthreads();
 		if (threadIdx.x == 0)
 			arrive_lane0();
@@ -394,6 +505,18 @@ struct GridSync
 			wait_lane0();
 		++epoch;
 		__syncthreads();
+	}
+
+	// the last workgroup to leave the launch zeroes the counters for the next
+	__device__ void leave() const
+	{
+		if (threadIdx.x != 0)
+			return;
+		uint32_t left =
+			__hip_atomic_fetch_add(base + kBarLeave * kBarStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (left + 1 == gridDim.x)
+			for (uint32_t k = 0; k < kBarCounters; ++k)
+				__hip_atomic_store(base + k * kBarStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
 };
 
@@ -559,7 +682,7 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 
 __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 {
-	GridSync grid{G.bar, gridDim.x, 0u};
+	GridSync grid = GridSync::make(G.bar, G.X.xcdBar);
 	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
 	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt);
@@ -1183,7 +1306,7 @@ __global__ void k_giant_lambda_back(GiantSolveCtx S)
 template <class Pass>
 __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, int passes)
 {
-	GridSync grid{S.barrier, gridDim.x, 0u};
+	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar);
 	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
 	// slot starts in LDS when they fit
 	constexpr int32_t kSlotLds = 2048;
@@ -1198,14 +1321,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 	}
 	for (int it = 0; it < passes; ++it)
 		giant_sweep<Pass>(S, ss, grid, nSlots);
-	if (threadIdx.x == 0)
-	{
-		uint32_t* leave = S.barrier + (kBarGroups + 1) * kBarStride;
-		uint32_t left = __hip_atomic_fetch_add(leave, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (left + 1 == gridDim.x)
-			for (uint32_t k = 0; k < kBarGroups + 2; ++k)
-				__hip_atomic_store(S.barrier + k * kBarStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	}
+	grid.leave();
 }
 
 // The substep's position stage in one cooperative launch: `passes` normal
@@ -1216,7 +1332,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 // k_giant_pass<FrictionPass> and k_giant_bodies(op 1) launched in turn.
 __global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx S, int passes)
 {
-	GridSync grid{S.barrier, gridDim.x, 0u};
+	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar);
 	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
 	constexpr int32_t kSlotLds = 2048;
 	__shared__ uint32_t sSlots[kSlotLds];
@@ -1255,14 +1371,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx 
 				reconstruct_velocity(X.P.B, b, X.invH);
 		}
 	}
-	if (threadIdx.x == 0)
-	{
-		uint32_t* leave = S.barrier + (kBarGroups + 1) * kBarStride;
-		uint32_t left = __hip_atomic_fetch_add(leave, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (left + 1 == gridDim.x)
-			for (uint32_t k = 0; k < kBarGroups + 2; ++k)
-				__hip_atomic_store(S.barrier + k * kBarStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	}
+	grid.leave();
 }
 
 // Co-resident grid size of a cooperative kernel (blocks of kGiantThreads),
@@ -1289,6 +1398,55 @@ static int coop_blocks(World& w)
 	if (cacheable)
 		cache[w.device].store(v, std::memory_order_relaxed);
 	return v;
+}
+
+// Workgroup placement probe: the XCC_ID register of each workgroup.
+__global__ void k_xcc_probe(uint32_t* out, int32_t n)
+{
+	if (threadIdx.x == 0 && (int32_t)blockIdx.x < n)
+		out[blockIdx.x] = xcc_id();
+}
+
+// Whether grid barriers may take XCD mode on this device (GridSync): over a
+// grid of 4096 workgroups the XCC_ID register reads 0 to 7 and every value
+// occurs (8 XCDs, each its own L2). Checked once per device;
+// M3D_XCD_BARRIER=0 turns the mode off for the process.
+constexpr int kXcdMaxDevices = 64;
+static std::atomic<int> gXcdState[kXcdMaxDevices]; // 0 unknown, 1 on, 2 off
+
+static bool xcd_barrier_ok(World& w)
+{
+	if (w.device < 0 || w.device >= kXcdMaxDevices)
+		return false;
+	int st = gXcdState[w.device].load(std::memory_order_relaxed);
+	if (st != 0)
+		return st == 1;
+	const char* e = getenv("M3D_XCD_BARRIER");
+	bool ok = !(e != nullptr && atoi(e) == 0);
+	if (ok)
+	{
+		constexpr int32_t n = 4096;
+		uint32_t* d = nullptr;
+		std::vector<uint32_t> h(n, 0xFFFFFFFFu);
+		ok = hipMalloc(&d, n * sizeof(uint32_t)) == hipSuccess;
+		if (ok)
+		{
+			k_xcc_probe<<<n, 64, 0, w.stream>>>(d, n);
+			ok = hipMemcpyAsync(h.data(), d, n * sizeof(uint32_t), hipMemcpyDeviceToHost, w.stream) == hipSuccess &&
+				 hipStreamSynchronize(w.stream) == hipSuccess;
+			(void)hipFree(d);
+		}
+		uint32_t seen = 0;
+		for (int32_t b = 0; ok && b < n; ++b)
+		{
+			ok = h[b] < kBarGroups;
+			seen |= ok ? 1u << h[b] : 0u;
+		}
+		ok = ok && seen == (1u << kBarGroups) - 1u;
+		(void)hipGetLastError();
+	}
+	gXcdState[w.device].store(ok ? 1 : 2, std::memory_order_relaxed);
+	return ok;
 }
 
 // Workgroups of a giant-island launch: enough lanes for every entry, at
@@ -1392,6 +1550,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	X.I = L.I;
 	X.NB = L.NB;
 	X.NC = L.NC;
+	X.xcdBar = w.xcdBarrier && xcd_barrier_ok(w) ? 1u : 0u;
 
 	if (L.NB == 0)
 		return true;

@@ -374,6 +374,9 @@ struct World
 	// the giant substep loop of a batch as one hipGraph (captured each step,
 	// the executable graph updated in place)
 	bool stepGraphs = true;
+	// grid barriers of the solver passes in XCD mode when the device allows
+	// it (one L2 write-back and invalidation per XCD; k_isolve.hip GridSync)
+	bool xcdBarrier = true;
 	int validatedBlocks = 0; // workgroups a direct cooperative launch accepted
 	hipGraphExec_t stepGraphExec = nullptr;
 	int64_t stepGraphLaunches = 0;

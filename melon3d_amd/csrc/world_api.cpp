@@ -2607,6 +2607,14 @@ int m3dx_set_step_graphs(m3d_world* world, int enable)
 
 int64_t m3dx_step_graph_launches(m3d_world* world) { return world != nullptr ? world->w.stepGraphLaunches : -1; }
 
+int m3dx_set_xcd_barrier(m3d_world* world, int enable)
+{
+	if (world == nullptr)
+		return -1;
+	world->w.xcdBarrier = enable != 0;
+	return 0;
+}
+
 int m3dx_set_kernel_timing(m3d_world* world, int enable)
 {
 	if (world == nullptr)

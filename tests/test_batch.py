@@ -210,6 +210,37 @@ def test_pass_workgroups_change_nothing(dev):
 
 
 @pytest.mark.gpu
+def test_xcd_barrier_changes_nothing(dev):
+    """The solver passes' grid barriers in XCD mode (one L2 write-back and
+    invalidation per XCD) and with agent-scope fences in every workgroup
+    give bit-identical batches after 30 steps, with 256 workgroups asked
+    for (as many as the runtime lets run together, over 100: a dozen or more
+    per XCD) and with 24 (a group of 3 on each XCD), and every step
+    succeeds."""
+    S, steps = 48, 30
+    for blocks in (256, 24):
+        worlds = []
+        try:
+            for xcd in (0, 1):
+                bw, _ = scenes.batch(dev, S, first=300)
+                assert dev.m3dx_set_pass_blocks(bw, blocks) == 0
+                assert dev.m3dx_set_xcd_barrier(bw, xcd) == 0
+                worlds.append(bw)
+            for step in range(steps):
+                for bw in worlds:
+                    dev.m3d_world_step(bw, DT, 4)
+            assert [dev.m3dx_step_failures(bw) for bw in worlds] == [0, 0], dev.last_error()
+            used = dev.m3dx_pass_blocks(worlds[1])
+            assert used == blocks if blocks < 100 else used > 100, used
+            parts = [_split(dev, bw, S) for bw in worlds]
+            for s in range(S):
+                _assert_same(parts[1][s], parts[0][s], s, steps)
+        finally:
+            for bw in worlds:
+                dev.m3d_world_destroy(bw)
+
+
+@pytest.mark.gpu
 def test_step_graph_changes_nothing(dev):
     """The batch's substep loop run as a captured hipGraph (the default once
     the workgroup count is settled) equals the directly launched loop bit for
