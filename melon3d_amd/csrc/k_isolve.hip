@@ -392,6 +392,7 @@ struct GridSync
 	uint32_t grp;    // this workgroup's group
 	uint32_t size;   // workgroups in the group
 	uint32_t groups; // nonempty groups
+	uint32_t vblock; // this workgroup's place in the launch's XCD-major order (XCD mode), else blockIdx.x
 
 	// Called by every thread of the workgroup at the launch.
 	__device__ static GridSync make(uint32_t* base, uint32_t xcdMode)
@@ -406,31 +407,35 @@ struct GridSync
 			g.grp = blockIdx.x % kBarGroups;
 			g.size = blocks / kBarGroups + (g.grp < blocks % kBarGroups ? 1u : 0u);
 			g.groups = blocks < kBarGroups ? blocks : kBarGroups;
+			g.vblock = blockIdx.x;
 			return g;
 		}
-		__shared__ uint32_t census[3];
+		__shared__ uint32_t census[4];
 		if (threadIdx.x == 0)
 		{
 			const uint32_t x = xcc_id() % kBarGroups;
-			__hip_atomic_fetch_add(base + (kBarCensus0 + x) * kBarStride, 1u, __ATOMIC_RELAXED,
-								   __HIP_MEMORY_SCOPE_AGENT);
+			const uint32_t rank = __hip_atomic_fetch_add(base + (kBarCensus0 + x) * kBarStride, 1u, __ATOMIC_RELAXED,
+														 __HIP_MEMORY_SCOPE_AGENT);
 			plain_barrier_lane0(base + kBarCGroup0 * kBarStride, base + kBarCTop * kBarStride, 0u);
-			uint32_t nonempty = 0u, mine = 0u;
+			uint32_t nonempty = 0u, mine = 0u, before = 0u;
 			for (uint32_t k = 0; k < kBarGroups; ++k)
 			{
 				uint32_t c = __hip_atomic_load(base + (kBarCensus0 + k) * kBarStride, __ATOMIC_RELAXED,
 											   __HIP_MEMORY_SCOPE_AGENT);
 				nonempty += c != 0u ? 1u : 0u;
 				mine = k == x ? c : mine;
+				before += k < x ? c : 0u;
 			}
 			census[0] = x;
 			census[1] = mine;
 			census[2] = nonempty;
+			census[3] = before + rank;
 		}
 		__syncthreads();
 		g.grp = census[0];
 		g.size = census[1];
 		g.groups = census[2];
+		g.vblock = census[3];
 		return g;
 	}
 
@@ -962,14 +967,17 @@ __device__ __forceinline__ void sweep_run<NormalsPass>(const GiantSolveCtx& S, i
 constexpr int32_t kSmallSlot = 256;
 
 // First entry of slot s for this lane, or -1: every lane of the grid for
-// large slots, the lanes of workgroup 0 for small ones. ss: slot starts.
+// large slots, the lanes of the first workgroup for small ones. ss: slot
+// starts; tid: the lane's place in the launch's XCD-major order (GridSync
+// vblock), so consecutive records of a slot go to workgroups of one XCD and
+// the body and record lines they share are fetched into one L2.
 __device__ __forceinline__ int32_t giant_lane_entry(const uint32_t* ss, int32_t s, int32_t tid, int32_t& stride)
 {
 	int32_t st = (int32_t)ss[s], en = (int32_t)ss[s + 1];
 	int32_t e;
 	if (en - st <= kSmallSlot)
 	{
-		if (blockIdx.x != 0)
+		if (tid >= (int32_t)blockDim.x)
 			return -1;
 		e = st + (int32_t)threadIdx.x;
 		stride = (int32_t)blockDim.x;
@@ -989,7 +997,7 @@ template <class Pass>
 __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync& grid, int32_t nSlots)
 {
 	const PassCtx& P = S.X.P;
-	const int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	const int32_t tid = (int32_t)(grid.vblock * blockDim.x + threadIdx.x);
 	int32_t stride = 1;
 	int4 r = make_int4(0, 0, 0, 0);
 	typename Pass::In d;
@@ -1025,7 +1033,7 @@ __device__ void giant_sweep(const GiantSolveCtx& S, const uint32_t* ss, GridSync
 		bool big = st != en && !small;
 		if (big)
 			grid.arrive();
-		else if (st != en && blockIdx.x == 0)
+		else if (st != en && grid.vblock == 0)
 			__syncthreads();
 		e = eNext;
 		stride = strideNext;
