@@ -1464,8 +1464,11 @@ static bool xcd_barrier_ok(World& w)
 // can be co-resident. The passes' count comes from the world's tuner.
 
 // The coloring fixpoint does several loads per entry and round, so it
-// takes one lane per entry up to kOrderMaxBlocks workgroups.
-constexpr int kOrderMaxBlocks = 128;
+// takes one lane per entry up to kOrderMaxBlocks workgroups (a round is
+// bound by its loads: on a 6 M-entry batch 128 workgroups took 2.95 ms per
+// step, 512 took 1.39 and 1024 1.35; on a 200 k-entry world 512 and 128
+// were even and 1024 slower).
+constexpr int kOrderMaxBlocks = 512;
 
 static int env_blocks(const char* name, int dflt)
 {
