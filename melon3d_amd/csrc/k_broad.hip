@@ -352,8 +352,10 @@ __device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool vali
 	__syncthreads();
 }
 
+// Entries at positions >= room are not stored (their cell bounds still
+// count): the caller emits again into a larger buffer.
 __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated, float cellSize, const uint32_t* cnt,
-							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, uint32_t* counters)
+							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, int64_t room, uint32_t* counters)
 {
 	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
 	bool any = false;
@@ -372,8 +374,11 @@ __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated,
 				for (int64_t z = z0; z <= z1; ++z)
 				{
 					uint64_t k = body_cell_key(B, i, x, y, z);
-					keys[o] = k;
-					bodies[o] = (uint32_t)i;
+					if ((int64_t)o < room)
+					{
+						keys[o] = k;
+						bodies[o] = (uint32_t)i;
+					}
 					++o;
 					KeyFields f = key_fields(k, batched);
 					uint32_t ux = f.x, uy = f.y, uz = f.z;
@@ -459,6 +464,14 @@ __global__ void k_mark_cells(DBodies B, int32_t n, uint8_t includeBit, int gated
 	B.bp[i] = in ? (uint8_t)(bp | cellBit) : (uint8_t)(bp & ~cellBit);
 }
 
+// empty cell bounds (min words all ones, max words zero) before an emit
+__global__ void k_bounds_init(uint32_t* counters)
+{
+	const uint32_t i = threadIdx.x;
+	if (i < 6u)
+		counters[CT_UX_MIN + i] = (i & 1u) ? 0u : 0xFFFFFFFFu;
+}
+
 // compression of keys whose field bounds sit in the counters block
 static KeyCompress key_compress_from(const uint32_t* hc, int32_t subCount, int* bitsOut)
 {
@@ -512,45 +525,69 @@ bool build_tier(World& w, int tier)
 		over.clear();
 		return true;
 	}
-	if (!w.u32a.ensure(n) || !w.u32b.ensure(n) || !w.u32c.ensure(n) || !w.i32a.ensure(n) || !ensure_prim_scratch(w, n))
+	// Entries go straight into the tier's buffers (room for at least four
+	// per body, or what an earlier build needed), so one counter readback
+	// returns the entry count, the oversized count and the cell bounds
+	// together; a tier with more entries than room is emitted once more.
+	size_t target = std::max<size_t>(std::min(keys.cap, vals.cap), 4 * (size_t)n + 1024);
+	if (!w.u32a.ensure(n) || !w.u32b.ensure(n) || !w.u32c.ensure(n) || !w.u32d.ensure(n) || !w.i32a.ensure(n) ||
+		!ensure_prim_scratch(w, n) || !keys.ensure(target) || !vals.ensure(target))
 		return false;
+	int64_t room = (int64_t)std::min(keys.cap, vals.cap);
 	uint32_t* cnt = w.u32a.p;
 	uint32_t* offs = w.u32b.p;
 	uint32_t* ov = w.u32c.p;
+	uint32_t* ovOff = w.u32d.p;
 	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, ov);
 	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
 	// oversized list in body order
-	uint32_t* ovOff = w.u32d.ensure(n) ? w.u32d.p : nullptr;
-	if (ovOff == nullptr)
-		return false;
 	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
 	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
-	if (!sync_counters(w))
+	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, cellBit, cnt);
+	k_bounds_init<<<1, 64, 0, s>>>(w.counters.p);
+	k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p,
+														   vals.p, room, w.counters.p);
+	int32_t* head = (int32_t*)w.hostCounters + kCountersSize + tier * kOverHead;
+	const int32_t nHead = std::min(n, kOverHead);
+	if (!hip_ok(hipMemcpyAsync(head, w.i32a.p, (size_t)nHead * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+				"oversized ids") ||
+		!sync_counters(w))
 		return false;
 	int64_t E = w.hostCounters[CT_ENTRIES];
 	int32_t nov = (int32_t)w.hostCounters[CT_OVERSIZED];
-	over.resize(nov);
+	over.assign(head, head + std::min(nov, nHead));
+	if (E > room || nov > nHead)
+	{
+		if (E > room)
+		{
+			// the same entries again (their cell bounds are min / max over
+			// the same keys, so the counters keep their values)
+			if (!keys.ensure(E) || !vals.ensure(E))
+				return false;
+			k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs,
+																   keys.p, vals.p, E, w.counters.p);
+		}
+		if (nov > nHead)
+		{
+			over.resize(nov);
+			if (!hip_ok(hipMemcpyAsync(over.data() + nHead, w.i32a.p + nHead, (size_t)(nov - nHead) * sizeof(int32_t),
+									   hipMemcpyDeviceToHost, s),
+						"oversized ids"))
+				return false;
+		}
+		if (!sync_counters(w))
+			return false;
+	}
 	if (nov > 0)
 	{
 		if (!dOver.ensure(nov))
 			return false;
 		hipMemcpyAsync(dOver.p, w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
-		hipMemcpyAsync(over.data(), w.i32a.p, nov * sizeof(int32_t), hipMemcpyDeviceToHost, s);
 	}
-	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, cellBit, cnt);
 	count = E;
 	if (E == 0)
-	{
-		return hip_ok(hipStreamSynchronize(s), "tier");
-	}
-	if (!keys.ensure(E) || !vals.ensure(E) || !w.sortKeysAlt.ensure(E) || !w.sortValsAlt.ensure(E) ||
-		!ensure_prim_scratch(w, E))
-		return false;
-	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
-	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
-	k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p,
-														   vals.p, w.counters.p);
-	if (!sync_counters(w))
+		return true;
+	if (!w.sortKeysAlt.ensure(E) || !w.sortValsAlt.ensure(E) || !ensure_prim_scratch(w, E))
 		return false;
 	int bits = 0;
 	KeyCompress kc = key_compress_from(w.hostCounters, w.subCount, &bits);
@@ -969,6 +1006,22 @@ __global__ void k_create(DBodies B, DContacts C, const uint64_t* keys, int32_t c
 	}
 }
 
+static bool ensure_pinned(char*& p, size_t& cap, size_t bytes, const char* what)
+{
+	if (p != nullptr && cap >= bytes)
+		return true;
+	if (p != nullptr)
+		(void)hipHostFree(p);
+	p = nullptr;
+	cap = bytes * 3 / 2 + 4096;
+	if (!hip_ok(hipHostMalloc((void**)&p, cap, hipHostMallocDefault), what))
+	{
+		cap = 0;
+		return false;
+	}
+	return true;
+}
+
 bool enumerate_and_create(World& w)
 {
 	w.newCount = 0;
@@ -1000,46 +1053,7 @@ bool enumerate_and_create(World& w)
 
 	int64_t E = w.activeCount;
 	int64_t SB = w.stableCount;
-	int64_t gridTotal = 0;
-	RunInfo R;
-	uint32_t* itemCnt = nullptr;
-	uint32_t* itemOffs = nullptr;
-	if (E > 0)
-	{
-		// items: at most one per active entry plus one per stable entry
-		int64_t T = E + SB;
-		if (!w.pairRuns.ensure(7 * (size_t)E + 2 * (size_t)T) || !ensure_prim_scratch(w, T))
-			return false;
-		uint32_t* base = w.pairRuns.p;
-		uint32_t* flag = base;
-		uint32_t* runEx = base + E;
-		R.flag = flag;
-		R.runEx = runEx;
-		R.runStart = base + 2 * E;
-		R.runEnd = base + 3 * E;
-		R.runStabLo = base + 4 * E;
-		R.runItems = base + 5 * E;
-		uint32_t* runBase = base + 6 * E;
-		R.runBase = runBase;
-		itemCnt = base + 7 * E;
-		itemOffs = itemCnt + T;
-		k_run_flags<<<grid_for(E), kBlock, 0, s>>>(w.activeKeys.p, E, flag);
-		scan_exclusive(flag, runEx, E, nullptr, w.scanStatus.p, s);
-		launch_fill_u32(R.runItems, 0u, E, s);
-		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
-		scan_exclusive(R.runItems, runBase, E, nullptr, w.scanStatus.p, s);
-		launch_fill_u32(itemCnt, 0u, T, s);
-		w.mark_begin(M3DX_K_PAIRS);
-		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
-		if (SB > 0)
-			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
-		w.mark_end();
-		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.scanStatus.p, s);
-		if (!sync_counters(w))
-			return false;
-		gridTotal = w.hostCounters[CT_CANDIDATES];
-	}
-
+	int64_t T = E + SB; // grid items: at most one per active entry plus one per stable entry
 	// Coarse-list segments in the reference's serial order
 	// (src/world.cpp:1643-1702). Batched: each oversized body meets only its
 	// own sub-world's bodies and oversized lists, which keeps every
@@ -1107,32 +1121,73 @@ bool enumerate_and_create(World& w)
 		size_t words = (segs.size() * sizeof(OverSeg) + 7) / 8;
 		if (!w.overSegs.ensure(words))
 			return false;
-		w.overSegHost.resize(words);
-		memcpy(w.overSegHost.data(), segs.data(), segs.size() * sizeof(OverSeg));
-		hipMemcpyAsync(w.overSegs.p, w.overSegHost.data(), words * 8, hipMemcpyHostToDevice, s);
+		// pinned, so the upload does not wait for the stream (the earlier
+		// step's upload from the same memory has ended at its readbacks)
+		if (!ensure_pinned(w.pinnedSegs, w.pinnedSegsCap, words * 8, "coarse-list segments"))
+			return false;
+		memcpy(w.pinnedSegs, segs.data(), segs.size() * sizeof(OverSeg));
+		if (!hip_ok(hipMemcpyAsync(w.overSegs.p, w.pinnedSegs, words * 8, hipMemcpyHostToDevice, s),
+					"coarse-list segments"))
+			return false;
 		O.seg = (const OverSeg*)w.overSegs.p;
 	}
 	else
 	{
 		O.seg = nullptr;
 	}
-	int64_t total = gridTotal;
+	// Both candidate counts (grid items and admitted coarse-list pairs) come
+	// back with one counter readback.
+	if (E > 0 && (!w.pairRuns.ensure(7 * (size_t)E + 2 * (size_t)T)))
+		return false;
+	if (ovTotal > 0 && (!w.u32c.ensure(ovTotal) || !w.u32d.ensure(ovTotal)))
+		return false;
+	if ((E > 0 || ovTotal > 0) && !ensure_prim_scratch(w, std::max(E > 0 ? T : 0, ovTotal)))
+		return false;
+	RunInfo R;
+	uint32_t* itemCnt = nullptr;
+	uint32_t* itemOffs = nullptr;
+	if (E > 0)
+	{
+		uint32_t* base = w.pairRuns.p;
+		uint32_t* flag = base;
+		uint32_t* runEx = base + E;
+		R.flag = flag;
+		R.runEx = runEx;
+		R.runStart = base + 2 * E;
+		R.runEnd = base + 3 * E;
+		R.runStabLo = base + 4 * E;
+		R.runItems = base + 5 * E;
+		uint32_t* runBase = base + 6 * E;
+		R.runBase = runBase;
+		itemCnt = base + 7 * E;
+		itemOffs = itemCnt + T;
+		k_run_flags<<<grid_for(E), kBlock, 0, s>>>(w.activeKeys.p, E, flag);
+		scan_exclusive(flag, runEx, E, nullptr, w.scanStatus.p, s);
+		launch_fill_u32(R.runItems, 0u, E, s);
+		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
+		scan_exclusive(R.runItems, runBase, E, nullptr, w.scanStatus.p, s);
+		launch_fill_u32(itemCnt, 0u, T, s);
+		w.mark_begin(M3DX_K_PAIRS);
+		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
+		if (SB > 0)
+			k_pairs_b<<<grid_for(SB), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
+		w.mark_end();
+		scan_exclusive(itemCnt, itemOffs, T, w.counters.p + CT_CANDIDATES, w.scanStatus.p, s);
+	}
 	uint32_t* ovFlags = nullptr;
 	uint32_t* ovOffs = nullptr;
-	int64_t ovAdmitted = 0;
 	if (ovTotal > 0)
 	{
-		if (!w.u32c.ensure(ovTotal) || !w.u32d.ensure(ovTotal) || !ensure_prim_scratch(w, ovTotal))
-			return false;
 		ovFlags = w.u32c.p;
 		ovOffs = w.u32d.p;
 		k_over_flags<<<grid_for(ovTotal), kBlock, 0, s>>>(P, O, ovTotal, ovFlags);
 		scan_exclusive(ovFlags, ovOffs, ovTotal, w.counters.p + CT_SEG_COUNT, w.scanStatus.p, s);
-		if (!sync_counters(w))
-			return false;
-		ovAdmitted = w.hostCounters[CT_SEG_COUNT];
-		total += ovAdmitted;
 	}
+	if ((E > 0 || ovTotal > 0) && !sync_counters(w))
+		return false;
+	int64_t gridTotal = E > 0 ? (int64_t)w.hostCounters[CT_CANDIDATES] : 0;
+	int64_t ovAdmitted = ovTotal > 0 ? (int64_t)w.hostCounters[CT_SEG_COUNT] : 0;
+	int64_t total = gridTotal + ovAdmitted;
 	if (total == 0)
 	{
 		return true;
@@ -1193,16 +1248,22 @@ __global__ void k_remove_flags(DBodies B, const int2* bodies, int32_t C, uint32_
 	flags[c] = f;
 }
 
-__global__ void k_hole_tail_flags(const uint32_t* rm, const float4* mat, int32_t C, int32_t newC, uint32_t* hole,
-								  uint32_t* tail, uint32_t* ev)
+// Holes (removed below newC = C - *removed), tails (kept at or above it),
+// end-event flags, and the removal flags of the tail region [newC, C)
+// packed from 0 (tailRm).
+__global__ void k_hole_tail_flags(const uint32_t* rm, const float4* mat, int32_t C, const uint32_t* removed,
+								  uint32_t* hole, uint32_t* tail, uint32_t* ev, uint32_t* tailRm)
 {
 	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (c >= C)
 		return;
+	const int32_t newC = C - (int32_t)*removed;
 	uint32_t r = rm[c];
 	hole[c] = (r && c < newC) ? 1u : 0u;
 	tail[c] = (!r && c >= newC) ? 1u : 0u;
 	ev[c] = (r && (f2u(mat[c].w) & CF_TOUCHING)) ? 1u : 0u;
+	if (c >= newC)
+		tailRm[c - newC] = r;
 }
 
 __global__ void k_make_moves(const uint32_t* hole, const uint32_t* holeOff, const uint32_t* tail,
@@ -1294,22 +1355,6 @@ __global__ void k_gather_events(const int2* bodies, const float4* mat, const int
 	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, c);
 }
 
-static bool ensure_pinned(char*& p, size_t& cap, size_t bytes, const char* what)
-{
-	if (p != nullptr && cap >= bytes)
-		return true;
-	if (p != nullptr)
-		(void)hipHostFree(p);
-	p = nullptr;
-	cap = bytes * 3 / 2 + 4096;
-	if (!hip_ok(hipHostMalloc((void**)&p, cap, hipHostMallocDefault), what))
-	{
-		cap = 0;
-		return false;
-	}
-	return true;
-}
-
 bool remove_separated(World& w)
 {
 	hipStream_t s = w.stream;
@@ -1317,55 +1362,73 @@ bool remove_separated(World& w)
 	if (C == 0)
 		return true;
 	if (!w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.u32c.ensure(C) || !w.u32d.ensure(C) || !w.i32a.ensure(C) ||
-		!w.i32b.ensure(C) || !w.i32c.ensure(C) || !ensure_prim_scratch(w, C))
+		!w.i32b.ensure(C) || !w.i32c.ensure(C) || !w.i32d.ensure(C) || !w.eventIdx.ensure(C) ||
+		!w.u64a.ensure((size_t)C * 2) || !ensure_prim_scratch(w, C))
+		return false;
+	// One readback returns the removed count, the hole / tail / end-event
+	// counts, the end events and the removal flags of the tail region when
+	// at most `spec` contacts go (spec follows twice the last count); more
+	// take a second copy. The pinned block is reused by the next call only
+	// after that call's readback, which follows every copy issued here.
+	const int32_t spec = std::min(C, std::max<int32_t>(4096, w.removalSpec));
+	size_t specBytes = (size_t)spec * (sizeof(int4) + 2 * sizeof(uint32_t));
+	if (!ensure_pinned(w.pinnedRemoval, w.pinnedRemovalCap, specBytes, "removal staging"))
 		return false;
 	uint32_t* rm = w.u32a.p;
-	k_remove_flags<<<grid_for(C), kBlock, 0, s>>>(body_view(w), w.cc.bodies.p, C, rm);
-	scan_exclusive(rm, w.u32b.p, C, w.counters.p + CT_REMOVED, w.scanStatus.p, s);
-	if (!sync_counters(w))
-		return false;
-	int32_t R = (int32_t)w.hostCounters[CT_REMOVED];
-	if (R == 0)
-		return true;
-	int32_t newC = C - R;
-	// One more wait for the rest: the hole/tail/event flags, their counts,
-	// the end events (at most R) and the removal flags of the tail region
-	// [newC, C) all come back together. The pinned block is reused by the
-	// next call only after that call's first wait, which follows every copy
-	// issued here.
-	size_t flagBytes = (size_t)R * sizeof(uint32_t);
-	size_t evBytes = (size_t)R * sizeof(int4);
-	if (!ensure_pinned(w.pinnedRemoval, w.pinnedRemovalCap, flagBytes + evBytes + (size_t)R * sizeof(int32_t),
-					   "removal staging") ||
-		!w.eventIdx.ensure(R) || !w.u64a.ensure((size_t)R * 2))
-		return false;
-	int4* evHost = (int4*)w.pinnedRemoval;
-	uint32_t* flags = (uint32_t*)(w.pinnedRemoval + evBytes);
-	int32_t* order = (int32_t*)(w.pinnedRemoval + evBytes + flagBytes);
 	uint32_t* hole = w.u32b.p;
 	uint32_t* tail = w.u32c.p;
 	uint32_t* ev = w.u32d.p;
-	k_hole_tail_flags<<<grid_for(C), kBlock, 0, s>>>(rm, w.cc.mat.p, C, newC, hole, tail, ev);
+	uint32_t* tailRm = (uint32_t*)w.i32d.p;
 	uint32_t* holeOff = (uint32_t*)w.i32a.p;
 	uint32_t* tailOff = (uint32_t*)w.i32b.p;
 	uint32_t* evOff = (uint32_t*)w.i32c.p;
+	k_remove_flags<<<grid_for(C), kBlock, 0, s>>>(body_view(w), w.cc.bodies.p, C, rm);
+	scan_exclusive(rm, hole, C, w.counters.p + CT_REMOVED, w.scanStatus.p, s); // offsets unused: the count
+	k_hole_tail_flags<<<grid_for(C), kBlock, 0, s>>>(rm, w.cc.mat.p, C, w.counters.p + CT_REMOVED, hole, tail, ev,
+													 tailRm);
 	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.scanStatus.p, s);
 	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.scanStatus.p, s);
 	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.scanStatus.p, s);
 	// end events of touching removed pairs, descending contact order
 	k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
-	k_gather_events<<<grid_for(R), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p,
+	k_gather_events<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p,
 												   w.counters.p + CT_END_EVENTS, (int4*)w.u64a.p);
-	if (!hip_ok(hipMemcpyAsync(evHost, w.u64a.p, evBytes, hipMemcpyDeviceToHost, s), "end events") ||
-		!hip_ok(hipMemcpyAsync(flags, rm + newC, flagBytes, hipMemcpyDeviceToHost, s), "removal flags") ||
+	int4* evHost = (int4*)w.pinnedRemoval;
+	uint32_t* flags = (uint32_t*)(w.pinnedRemoval + (size_t)spec * sizeof(int4));
+	int32_t* order = (int32_t*)(w.pinnedRemoval + (size_t)spec * (sizeof(int4) + sizeof(uint32_t)));
+	if (!hip_ok(hipMemcpyAsync(evHost, w.u64a.p, (size_t)spec * sizeof(int4), hipMemcpyDeviceToHost, s),
+				"end events") ||
+		!hip_ok(hipMemcpyAsync(flags, tailRm, (size_t)spec * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+				"removal flags") ||
 		!sync_counters(w))
 		return false;
+	int32_t R = (int32_t)w.hostCounters[CT_REMOVED];
+	w.removalSpec = 2 * R;
+	if (R == 0)
+		return true;
+	int32_t newC = C - R;
 	int32_t H = (int32_t)w.hostCounters[CT_HOLES];
 	int32_t NE = (int32_t)w.hostCounters[CT_END_EVENTS];
 	if (H != (int32_t)w.hostCounters[CT_TAILS] || NE > R)
 	{
 		set_error("removal bookkeeping mismatch (%d holes, %u tails, %d end events)", H, w.hostCounters[CT_TAILS], NE);
 		return false;
+	}
+	if (R > spec)
+	{
+		// a larger block (its contents are not kept) and both lists again
+		if (!ensure_pinned(w.pinnedRemoval, w.pinnedRemovalCap, (size_t)R * (sizeof(int4) + 2 * sizeof(uint32_t)),
+						   "removal staging"))
+			return false;
+		evHost = (int4*)w.pinnedRemoval;
+		flags = (uint32_t*)(w.pinnedRemoval + (size_t)R * sizeof(int4));
+		order = (int32_t*)(w.pinnedRemoval + (size_t)R * (sizeof(int4) + sizeof(uint32_t)));
+		if (!hip_ok(hipMemcpyAsync(evHost, w.u64a.p, (size_t)NE * sizeof(int4), hipMemcpyDeviceToHost, s),
+					"end events") ||
+			!hip_ok(hipMemcpyAsync(flags, tailRm, (size_t)R * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+					"removal flags") ||
+			!hip_ok(hipStreamSynchronize(s), "removal staging"))
+			return false;
 	}
 	if (H > 0)
 	{
@@ -1880,8 +1943,7 @@ bool build_stable_batched(World& w)
 	if (!w.stableKeysNext.ensure(E) || !w.stableBodiesNext.ensure(E) || !w.sortKeysAlt.ensure(E) ||
 		!w.sortValsAlt.ensure(E) || !ensure_prim_scratch(w, E))
 		return false;
-	uint32_t init[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
-	hipMemcpyAsync(w.counters.p + CT_UX_MIN, init, sizeof(init), hipMemcpyHostToDevice, s);
+	k_bounds_init<<<1, 64, 0, s>>>(w.counters.p);
 	if (eKeep > 0)
 	{
 		k_compact_entries<<<grid_for(oldE), kBlock, 0, s>>>(w.stableKeys.p, w.stableBodies.p, keep, keepOff, oldE,
@@ -1891,7 +1953,7 @@ bool build_stable_batched(World& w)
 	if (eNew > 0)
 		k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, offs,
 												   w.stableKeysNext.p + eKeep, w.stableBodiesNext.p + eKeep,
-												   w.counters.p);
+												   eNew, w.counters.p);
 	if (!sync_counters(w))
 		return false;
 	int bits = 0;

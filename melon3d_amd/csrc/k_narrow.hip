@@ -99,10 +99,10 @@ __global__ void k_trans_compact(const int8_t* trans, const uint32_t* offs, int32
 }
 
 __global__ void k_trans_info(const int2* bodies, const float4* mat, const int8_t* trans, const int32_t* list,
-							 int32_t T, int4* out)
+							 const uint32_t* count, int4* out)
 {
 	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (t >= T)
+	if (t >= (int32_t)*count)
 		return;
 	int32_t c = list[t];
 	int2 b = bodies[c];
@@ -188,7 +188,7 @@ bool narrowphase(World& w, float dt)
 	if (C == 0)
 		return true;
 	if (!w.u8a.ensure(C) || !w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.i32a.ensure(C) ||
-		!ensure_prim_scratch(w, C))
+		!ensure_prim_scratch(w, C) || !w.transList.ensure(C) || !w.transInfo.ensure(C) || !w.i32c.ensure(n))
 		return false;
 	int8_t* trans = (int8_t*)w.u8a.p;
 	w.mark_begin(M3DX_K_NARROW);
@@ -196,18 +196,34 @@ bool narrowphase(World& w, float dt)
 	w.mark_end();
 	k_trans_flags<<<gridn(C), kBlockN, 0, s>>>(trans, C, w.u32a.p);
 	scan_exclusive(w.u32a.p, w.u32b.p, C, w.counters.p + CT_TRANSITIONS, w.scanStatus.p, s);
-	if (!sync_counters(w))
-		return false;
+	// The transitions are listed and described on the device (their count
+	// read there); the count comes back with the island stage's first
+	// readback, which issues their download (queue_transitions).
+	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(trans, w.u32b.p, C, w.transList.p);
+	k_trans_info<<<gridn(C), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.transList.p,
+											  w.counters.p + CT_TRANSITIONS, w.transInfo.p);
+	// wake-on-touch fixpoint
+	int32_t* wakeAt = w.i32c.p;
+	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
+	k_wake_fixpoint<<<1, 1024, 0, s>>>(w.transInfo.p, w.counters.p + CT_TRANSITIONS, w.bc.flags.p, wakeAt);
+	k_wake_apply<<<gridn(n), kBlockN, 0, s>>>(make_body_view(w), n, wakeAt, w.counters.p);
+	w.transInFlight = true;
+	return true;
+}
+
+// Download of the step's transitions once their count is in the host
+// counters: it runs behind the launches queued so far, and the step turns
+// them into events after the solver's launches are queued
+// (flush_transitions).
+bool queue_transitions(World& w)
+{
+	if (!w.transInFlight)
+		return true;
+	w.transInFlight = false;
 	int32_t T = (int32_t)w.hostCounters[CT_TRANSITIONS];
 	if (T == 0)
 		return true;
-	if (!w.i32b.ensure(T) || !w.u64a.ensure((size_t)T * 2) || !w.i32c.ensure(n))
-		return false;
-	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(trans, w.u32b.p, C, w.i32b.p);
-	int4* info = (int4*)w.u64a.p;
-	k_trans_info<<<gridn(T), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.i32b.p, T, info);
-	// the transitions come back behind the solver's launches; the step
-	// turns them into events once those are queued (flush_transitions)
+	hipStream_t s = w.stream;
 	if (w.pinnedTransCap < (size_t)T)
 	{
 		if (w.pinnedTrans != nullptr)
@@ -223,14 +239,9 @@ bool narrowphase(World& w, float dt)
 	}
 	if (w.transReady == nullptr && !hip_ok(hipEventCreateWithFlags(&w.transReady, hipEventDisableTiming), "transitions"))
 		return false;
-	if (!hip_ok(hipMemcpyAsync(w.pinnedTrans, info, T * sizeof(int4), hipMemcpyDeviceToHost, s), "transitions"))
-		return false;
-	// wake-on-touch fixpoint
-	int32_t* wakeAt = w.i32c.p;
-	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
-	k_wake_fixpoint<<<1, 1024, 0, s>>>(info, w.counters.p + CT_TRANSITIONS, w.bc.flags.p, wakeAt);
-	k_wake_apply<<<gridn(n), kBlockN, 0, s>>>(make_body_view(w), n, wakeAt, w.counters.p);
-	if (!hip_ok(hipEventRecord(w.transReady, s), "transitions"))
+	if (!hip_ok(hipMemcpyAsync(w.pinnedTrans, w.transInfo.p, T * sizeof(int4), hipMemcpyDeviceToHost, s),
+				"transitions") ||
+		!hip_ok(hipEventRecord(w.transReady, s), "transitions"))
 		return false;
 	w.pendingTrans = T;
 	return true;
@@ -240,6 +251,9 @@ bool narrowphase(World& w, float dt)
 // after the removal's end events (src/world.cpp:1773-1801).
 bool flush_transitions(World& w)
 {
+	// a step that read no counters back after the narrowphase
+	if (w.transInFlight && !(sync_counters(w) && queue_transitions(w)))
+		return false;
 	int32_t T = w.pendingTrans;
 	if (T == 0)
 		return true;

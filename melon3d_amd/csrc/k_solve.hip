@@ -390,7 +390,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		k_uf_joints<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p, parent);
 	k_uf_roots<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, n, w.u32a.p);
 	scan_exclusive(w.u32a.p, w.u32b.p, n, w.counters.p + CT_ISLANDS, w.scanStatus.p, s);
-	if (!sync_counters(w))
+	if (!sync_counters(w) || !queue_transitions(w))
 		return false;
 	// no awake dynamic body: the reference skips island building entirely
 	if (!w.hostCounters[CT_ANY_AWAKE_DYN])

@@ -59,6 +59,11 @@ enum Counter : int
 };
 
 constexpr int kCountersSize = 64;
+// The pinned host mirror holds, after the counters block, the first
+// kOverHead oversized body ids of each tier: a tier build downloads them
+// with its counter readback (more oversized bodies take a second copy).
+constexpr int kOverHead = 256;
+constexpr int kHostCountersWords = kCountersSize + 2 * kOverHead;
 
 // island mode bits
 constexpr uint32_t IM_AWAKE = 1u;
@@ -137,6 +142,8 @@ bool overlap_query(World& w, const m3d_aabb* boxes, int32_t count, std::vector<u
 
 // ---- k_narrow.hip ----
 bool narrowphase(World& w, float dt);
+// issues the download of the narrowphase's transitions (counters read back)
+bool queue_transitions(World& w);
 
 // ---- k_solve.hip ----
 bool islands_and_solve(World& w, float dt, int substeps);

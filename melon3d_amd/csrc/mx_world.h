@@ -224,9 +224,16 @@ struct World
 	size_t pinnedTransCap = 0;
 	hipEvent_t transReady = nullptr;
 	int32_t pendingTrans = 0;
+	bool transInFlight = false; // listed on the device, count not yet read back
+	DBuf<int32_t> transList;    // contacts with a transition, ascending
+	DBuf<int4> transInfo;       // (body a, body b, kind, contact) per transition
 	// pinned staging of the contact removal (flags, end events, order)
 	char* pinnedRemoval = nullptr;
 	size_t pinnedRemovalCap = 0;
+	int32_t removalSpec = 0; // removed contacts the staging expects (twice the last count)
+	// pinned staging of the coarse-list candidate segments
+	char* pinnedSegs = nullptr;
+	size_t pinnedSegsCap = 0;
 
 	// ---- device state ----
 	int32_t deviceBodyCap = 0;  // capacity of the body columns
@@ -260,7 +267,6 @@ struct World
 	DBuf<uint64_t> scanStatus;    // scan look-back words (nothing else writes them)
 	DBuf<uint32_t> pairRuns;      // pair enumeration: run tables, item counts and offsets
 	DBuf<uint64_t> overSegs;      // coarse-list candidate segments
-	std::vector<uint64_t> overSegHost;
 	DBuf<uint64_t> hashKeys;
 	DBuf<uint32_t> hashVals;
 	DBuf<int32_t> islandOf;       // per body: island id (-1)

@@ -159,8 +159,9 @@ bool refresh_broadphase(World& w, float dt)
 		// per sub-world stable-tier timing; the active tier is rebuilt for all
 		launch_stable_plan(w);
 		launch_refresh(w, dt, false);
-		build_tier(w, 0);
-		if (!sync_counters(w))
+		// the tier build reads the counters back after the plan (with no
+		// bodies it launches nothing and reads nothing)
+		if (!build_tier(w, 0) || (w.deviceBodyCount == 0 && !sync_counters(w)))
 			return false;
 		if (w.hostCounters[CT_SUB_REBUILDS] > 0)
 		{
@@ -174,10 +175,12 @@ bool refresh_broadphase(World& w, float dt)
 	}
 	bool rebuildStable = w.stableForce || (w.stableDirty && w.stepsSinceStableBuild >= 4);
 	launch_refresh(w, dt, rebuildStable);
-	build_tier(w, 0);
+	if (!build_tier(w, 0))
+		return false;
 	if (rebuildStable)
 	{
-		build_tier(w, 1);
+		if (!build_tier(w, 1))
+			return false;
 		// inStableTier = isStable for every slot
 		w.stableDirty = false;
 		w.stableForce = false;
@@ -217,6 +220,7 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	w.beginEvents.clear();
 	w.endEvents.clear();
 	w.pendingTrans = 0;
+	w.transInFlight = false;
 	memset(&w.stats, 0, sizeof(w.stats));
 	// a sidecar downloaded before this step (and not used by a destroy since)
 	// no longer matches the device's contacts once the step runs

@@ -100,6 +100,8 @@ World::~World()
 	i32c.release();
 	i32d.release();
 	i32e.release();
+	transList.release();
+	transInfo.release();
 	u8a.release();
 	primScratch.release();
 	pairRuns.release();
@@ -182,6 +184,8 @@ World::~World()
 		(void)hipHostFree(pinnedUpload);
 	if (pinnedRemoval != nullptr)
 		(void)hipHostFree(pinnedRemoval);
+	if (pinnedSegs != nullptr)
+		(void)hipHostFree(pinnedSegs);
 	if (transReady != nullptr)
 		(void)hipEventDestroy(transReady);
 	if (pinnedTrans != nullptr)
@@ -905,14 +909,14 @@ m3d_world* m3d_world_create(const m3d_world_def* def)
 	bool ok = hip_ok(hipSetDevice(w.device), "hipSetDevice") &&
 			  hip_ok(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking), "stream") &&
 			  w.counters.ensure(kCountersSize) &&
-			  hip_ok(hipHostMalloc((void**)&w.hostCounters, kCountersSize * sizeof(uint32_t), hipHostMallocDefault),
+			  hip_ok(hipHostMalloc((void**)&w.hostCounters, kHostCountersWords * sizeof(uint32_t), hipHostMallocDefault),
 					 "pinned counters");
 	if (!ok)
 	{
 		delete wp;
 		return nullptr;
 	}
-	memset(w.hostCounters, 0, kCountersSize * sizeof(uint32_t));
+	memset(w.hostCounters, 0, kHostCountersWords * sizeof(uint32_t));
 	memset(&w.profile, 0, sizeof(w.profile));
 	return wp;
 }
