@@ -287,6 +287,19 @@ M3D_API int64_t m3dx_step_graph_launches(m3d_world* world);
    0 fences every workgroup at the agent scope. Results do not depend on it.
    Returns 0 or -1. */
 M3D_API int m3dx_set_xcd_barrier(m3d_world* world, int enable);
+/* Launches of the grid-barrier kernels (the coloring fixpoint and the
+   solver passes). cooperative = 0 (the default): plain launches, sized to
+   what every CU's occupancy holds at once so the workgroups are co-resident;
+   1: cooperative launches (M3D_COOP_LAUNCH=1 sets this for every world).
+   waitTicks > 0 bounds every grid-barrier wait, in ticks of the device's
+   100 MHz wall clock (default 2^28, about 2.7 s); a wait beyond it fails the
+   step (m3d_world_step counts the failure and the error names it; the step's
+   solve is void) and turns the world to cooperative launches. 0 keeps the
+   bound. Returns 0 or -1. */
+M3D_API int m3dx_set_grid_launch(m3d_world* world, int cooperative, uint32_t waitTicks);
+/* 1 when the world launches its grid kernels cooperatively, 0 when not, -1
+   for no world. */
+M3D_API int m3dx_grid_launch_mode(m3d_world* world);
 /* Copies the M3DX_K_COUNT accumulators. Returns M3DX_K_COUNT. */
 M3D_API int m3dx_kernel_timing(m3d_world* world, m3dx_kernel_time* out);
 

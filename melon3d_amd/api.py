@@ -336,6 +336,8 @@ _SIGS = {
     "m3dx_step_failures": (C.c_int32, [W]),
     "m3dx_set_step_graphs": (C.c_int, [W, C.c_int]),
     "m3dx_set_xcd_barrier": (C.c_int, [W, C.c_int]),
+    "m3dx_set_grid_launch": (C.c_int, [W, C.c_int, C.c_uint32]),
+    "m3dx_grid_launch_mode": (C.c_int, [W]),
     "m3dx_step_graph_launches": (C.c_int64, [W]),
     "m3dx_kernel_timing": (C.c_int, [W, C.POINTER(m3dx_kernel_time)]),
     "m3dx_batch_create": (W, [C.POINTER(m3d_world_def), C.c_int32]),

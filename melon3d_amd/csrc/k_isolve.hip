@@ -237,7 +237,8 @@ struct IslandCtx
 	m3d_vec3 gravity;
 	float invH;
 	int32_t iterations, substeps, I, NB, NC;
-	uint32_t xcdBar; // grid barriers in XCD mode (GridSync)
+	uint32_t xcdBar;     // grid barriers in XCD mode (GridSync)
+	uint32_t barTimeout; // longest grid-barrier wait, in wall-clock ticks (GridSync)
 };
 
 // ------------------------------------------------------------- joints
@@ -306,6 +307,7 @@ struct GiantCtx
 	uint32_t* slotVals;
 	uint8_t* dirty;           // 2 x NC: entries to re-evaluate in the next round (zeroed)
 	int bitsI;
+	int32_t maxS;             // slot bound: slots are 0 .. maxS - 1
 };
 
 __device__ __forceinline__ bool giant_entry(const GiantCtx& G, int32_t k)
@@ -364,9 +366,39 @@ __device__ __forceinline__ uint32_t xcc_id()
 	return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11));
 }
 
+// Bounded waits. The grid's workgroups are co-resident by its size (at most
+// what the occupancy of every CU holds), so every wait ends; should one not
+// end within `limit` wall-clock ticks (s_memrealtime, 100 MHz), the waiting
+// lane sets the step's abort word and every later wait of the step returns
+// at once: the launch drains and the step reports the failure. Returns
+// false when the wait was cut short.
+__device__ __forceinline__ bool bounded_wait(const uint32_t* word, uint32_t target, uint32_t* abortw, uint32_t limit)
+{
+	uint64_t t0 = 0;
+	for (uint32_t k = 0;; ++k)
+	{
+		uint32_t v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t a = __hip_atomic_load(abortw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (v >= target)
+			return true;
+		if (a != 0u)
+			return false;
+		uint64_t t = __builtin_amdgcn_s_memrealtime();
+		if (k == 0)
+			t0 = t;
+		else if (t - t0 > (uint64_t)limit)
+		{
+			__hip_atomic_store(abortw, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			return false;
+		}
+		__builtin_amdgcn_s_sleep(1);
+	}
+}
+
 // Lane 0 of every workgroup: one two-level barrier of the plain kind over
 // groups b % kBarGroups on the counters at groupBase / top (epoch `e`).
-__device__ __forceinline__ void plain_barrier_lane0(uint32_t* groupBase, uint32_t* top, uint32_t e)
+__device__ __forceinline__ void plain_barrier_lane0(uint32_t* groupBase, uint32_t* top, uint32_t e, uint32_t* abortw,
+													uint32_t limit)
 {
 	const uint32_t blocks = gridDim.x;
 	const uint32_t g = blockIdx.x % kBarGroups;
@@ -379,8 +411,7 @@ __device__ __forceinline__ void plain_barrier_lane0(uint32_t* groupBase, uint32_
 		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
 		__hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
-	while (__hip_atomic_load(top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (e + 1u) * groups)
-		__builtin_amdgcn_s_sleep(1);
+	(void)bounded_wait(top, (e + 1u) * groups, abortw, limit);
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
@@ -393,14 +424,18 @@ struct GridSync
 	uint32_t size;   // workgroups in the group
 	uint32_t groups; // nonempty groups
 	uint32_t vblock; // this workgroup's place in the launch's XCD-major order (XCD mode), else blockIdx.x
+	uint32_t* abortw; // the step's abort word (bounded_wait)
+	uint32_t limit;   // wait bound, wall-clock ticks
 
 	// Called by every thread of the workgroup at the launch.
-	__device__ static GridSync make(uint32_t* base, uint32_t xcdMode)
+	__device__ static GridSync make(uint32_t* base, uint32_t xcdMode, uint32_t* abortw, uint32_t limit)
 	{
 		GridSync g;
 		g.base = base;
 		g.epoch = 0u;
 		g.xcd = xcdMode;
+		g.abortw = abortw;
+		g.limit = limit;
 		const uint32_t blocks = gridDim.x;
 		if (xcdMode == 0u)
 		{
@@ -416,7 +451,7 @@ struct GridSync
 			const uint32_t x = xcc_id() % kBarGroups;
 			const uint32_t rank = __hip_atomic_fetch_add(base + (kBarCensus0 + x) * kBarStride, 1u, __ATOMIC_RELAXED,
 														 __HIP_MEMORY_SCOPE_AGENT);
-			plain_barrier_lane0(base + kBarCGroup0 * kBarStride, base + kBarCTop * kBarStride, 0u);
+			plain_barrier_lane0(base + kBarCGroup0 * kBarStride, base + kBarCTop * kBarStride, 0u, abortw, limit);
 			uint32_t nonempty = 0u, mine = 0u, before = 0u;
 			for (uint32_t k = 0; k < kBarGroups; ++k)
 			{
@@ -456,12 +491,11 @@ struct GridSync
 		}
 	}
 
-	__device__ void wait_lane0() const
+	__device__ bool wait_lane0() const
 	{
-		const uint32_t target = (epoch + 1u) * groups;
-		while (__hip_atomic_load(top(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-			__builtin_amdgcn_s_sleep(1);
+		bool ok = bounded_wait(top(), (epoch + 1u) * groups, abortw, limit);
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+		return ok;
 	}
 
 	// every wave: its stores have reached the L2 (XCD mode; the agent release
@@ -474,7 +508,7 @@ struct GridSync
 
 	// Barrier; then returns *word as read by one lane after it (the same
 	// value for every lane of the grid as long as nobody writes *word
-	// before the next barrier).
+	// before the next barrier), or 0 when the wait was cut short.
 	__device__ uint32_t sync_read(const uint32_t* word)
 	{
 		__shared__ uint32_t value;
@@ -483,9 +517,9 @@ struct GridSync
 		if (threadIdx.x == 0)
 		{
 			arrive_lane0();
-			wait_lane0();
+			bool ok = wait_lane0();
 			if (word)
-				value = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				value = ok ? __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
 		}
 		++epoch;
 		__syncthreads();
@@ -507,7 +541,7 @@ struct GridSync
 	__device__ void wait()
 	{
 		if (threadIdx.x == 0)
-			wait_lane0();
+			(void)wait_lane0();
 		++epoch;
 		__syncthreads();
 	}
@@ -687,7 +721,7 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 
 __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 {
-	GridSync grid = GridSync::make(G.bar, G.X.xcdBar);
+	GridSync grid = GridSync::make(G.bar, G.X.xcdBar, G.counters + CT_GRID_ABORT, G.X.barTimeout);
 	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
 	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
 	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt);
@@ -703,6 +737,9 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 		{
 			int32_t col = G.color[k];
 			int32_t slot = col < kMaxGraphColors ? col : kMaxGraphColors + G.level[k];
+			// within the slot arrays whatever the rounds left (a fixpoint
+			// cut short by an aborted barrier); complete rounds stay below
+			slot = slot < 0 ? 0 : (slot >= G.maxS ? G.maxS - 1 : slot);
 			if (G.colored[k])
 				G.X.contactColor[G.X.contactList[k]] = col;
 			G.X.P.C.hint[G.X.contactList[k]] = make_int2(col, G.level[k]);
@@ -1314,7 +1351,7 @@ __global__ void k_giant_lambda_back(GiantSolveCtx S)
 template <class Pass>
 __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, int passes)
 {
-	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar);
+	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar, S.counters2 + CT_GRID_ABORT, S.X.barTimeout);
 	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
 	// slot starts in LDS when they fit
 	constexpr int32_t kSlotLds = 2048;
@@ -1340,7 +1377,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_pass(GiantSolveCtx S, i
 // k_giant_pass<FrictionPass> and k_giant_bodies(op 1) launched in turn.
 __global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx S, int passes)
 {
-	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar);
+	GridSync grid = GridSync::make(S.barrier, S.X.xcdBar, S.counters2 + CT_GRID_ABORT, S.X.barTimeout);
 	const int32_t nSlots = (int32_t)S.counters[CT_SLOTS];
 	constexpr int32_t kSlotLds = 2048;
 	__shared__ uint32_t sSlots[kSlotLds];
@@ -1477,6 +1514,22 @@ static int env_blocks(const char* name, int dflt)
 	return v > 0 ? v : dflt;
 }
 
+// Launch of a grid-barrier kernel. Its size never exceeds what every CU's
+// occupancy holds at once (giant_blocks, the pass tuner's limit), so the
+// workgroups are co-resident with a plain launch, whose dispatch follows the
+// stream's previous kernel directly; a cooperative launch (which also
+// guarantees it) first waits about 11 us for the runtime's switch of queues
+// on each side. A world whose barrier wait ever exceeded its bound
+// (CT_GRID_ABORT) uses cooperative launches from then on, as does every
+// world with M3D_COOP_LAUNCH=1.
+static hipError_t grid_launch(World& w, const void* kernel, int blocks, void** args, hipStream_t s)
+{
+	static const bool forceCoop = env_blocks("M3D_COOP_LAUNCH", 0) == 1;
+	if (w.coopLaunch || forceCoop)
+		return hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+	return hipLaunchKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+}
+
 template <auto kernel>
 static int giant_blocks(World& w, int32_t entries, int cap)
 {
@@ -1562,6 +1615,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	X.NB = L.NB;
 	X.NC = L.NC;
 	X.xcdBar = w.xcdBarrier && xcd_barrier_ok(w) ? 1u : 0u;
+	X.barTimeout = w.barTimeout;
 
 	if (L.NB == 0)
 		return true;
@@ -1656,12 +1710,13 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.slotVals = w.s32b.p;
 	G.dirty = w.sDirty.p;
 	G.bitsI = L.bitsI;
+	G.maxS = maxS;
 	{
 		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);
 		int blocks = giant_blocks<k_giant_order>(w, NC, cap);
 		void* args[] = { &G };
 		w.mark_begin(M3DX_K_GIANT_ORDER);
-		hipError_t err = hipLaunchCooperativeKernel((const void*)k_giant_order, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+		hipError_t err = grid_launch(w, (const void*)k_giant_order, blocks, args, s);
 		w.mark_end();
 		if (!hip_ok(err,
 					"giant order"))
@@ -1716,7 +1771,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		void* args[] = { &S, &passes };
 		for (;;)
 		{
-			hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(kGiantThreads), args, 0, s);
+			hipError_t e = grid_launch(w, kernel, blocks, args, s);
 			if (e != hipErrorCooperativeLaunchTooLarge || blocks <= 1)
 				return hip_ok(e, "giant pass");
 			(void)hipGetLastError();

@@ -55,6 +55,7 @@ enum Counter : int
 	CT_GIANT_BB,      // box-box entries, summed over substeps
 	CT_GIANT_BB_POINTS, // their points, summed over substeps
 	CT_RESPAWN_REMOVED, // between steps: contacts touching the bodies of a bulk despawn
+	CT_GRID_ABORT,      // a grid barrier's wait exceeded its bound (the step failed)
 	CT_COUNT_
 };
 

@@ -383,6 +383,10 @@ struct World
 	// grid barriers of the solver passes in XCD mode when the device allows
 	// it (one L2 write-back and invalidation per XCD; k_isolve.hip GridSync)
 	bool xcdBarrier = true;
+	// grid-barrier kernels: cooperative launches (else plain launches of a
+	// co-resident size), and the longest barrier wait in wall-clock ticks
+	bool coopLaunch = false;
+	uint32_t barTimeout = 1u << 28;
 	int validatedBlocks = 0; // workgroups a direct cooperative launch accepted
 	hipGraphExec_t stepGraphExec = nullptr;
 	int64_t stepGraphLaunches = 0;

@@ -245,6 +245,14 @@ static bool world_step_stages(World& w, float dt, int substeps)
 	if (!flush_transitions(w) || !download_joints(w) || !sync_counters(w))
 		return false;
 	const uint32_t* hc = w.hostCounters;
+	if (hc[CT_GRID_ABORT] != 0u)
+	{
+		// the grid was not co-resident: this step's solve is void
+		w.coopLaunch = true;
+		set_error("a grid barrier wait exceeded %u ticks; the world launches its grid kernels cooperatively from now on",
+				  w.barTimeout);
+		return false;
+	}
 	w.tuner_step_end((int32_t)hc[CT_SLOTS]);
 	if (w.timing)
 	{

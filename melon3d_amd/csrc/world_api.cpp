@@ -2619,6 +2619,23 @@ int m3dx_set_xcd_barrier(m3d_world* world, int enable)
 	return 0;
 }
 
+int m3dx_set_grid_launch(m3d_world* world, int cooperative, uint32_t waitTicks)
+{
+	if (world == nullptr)
+		return -1;
+	world->w.coopLaunch = cooperative != 0;
+	if (waitTicks > 0)
+		world->w.barTimeout = waitTicks;
+	return 0;
+}
+
+int m3dx_grid_launch_mode(m3d_world* world)
+{
+	if (world == nullptr)
+		return -1;
+	return world->w.coopLaunch ? 1 : 0;
+}
+
 int m3dx_set_kernel_timing(m3d_world* world, int enable)
 {
 	if (world == nullptr)

@@ -621,6 +621,8 @@ int32_t m3dx_step_failures(m3d_world*) { return 0; }
 int m3dx_set_step_graphs(m3d_world*, int) { return -1; }
 int64_t m3dx_step_graph_launches(m3d_world*) { return 0; }
 int m3dx_set_xcd_barrier(m3d_world*, int) { return -1; }
+int m3dx_set_grid_launch(m3d_world*, int, uint32_t) { return -1; }
+int m3dx_grid_launch_mode(m3d_world*) { return -1; }
 
 int m3dx_kernel_timing(m3d_world*, m3dx_kernel_time* out)
 {
