@@ -1247,14 +1247,25 @@ struct CopySink
 	int64_t st;
 	uint32_t at;
 	bool flat, boxBox;
-	__device__ void point(int j, int32_t p, const float4& a4, const float4& b4, const float4& ja, const float4& jb,
+	uint32_t* cnt; // the record's point count, at the entry's sorted position
+	mutable float4 slip[4], fa[4], fb[4];
+	__device__ void preload(int j, int32_t p) const
+	{
+		slip[j] = C->pSlip[p];
+		if (boxBox)
+		{
+			fa[j] = C->pFA[p];
+			fb[j] = C->pFB[p];
+		}
+	}
+	__device__ void point(int j, int32_t, const float4& a4, const float4& b4, const float4& ja, const float4& jb,
 						  const float4& ra, const float4& rb) const
 	{
 		nrm[(1 + j) * st + at] = a4;
 		nrm[(5 + j) * st + at] = b4;
 		nrm[(9 + j) * st + at] = ja;
 		nrm[(13 + j) * st + at] = jb;
-		nrm[(18 + j) * st + at] = C->pSlip[p];
+		nrm[(18 + j) * st + at] = slip[j];
 		if (flat || (boxBox && j == 0))
 		{
 			nrm[(22 + j) * st + at] = ra;
@@ -1262,9 +1273,15 @@ struct CopySink
 		}
 		if (boxBox)
 		{
-			nrm[(30 + j) * st + at] = C->pFA[p];
-			nrm[(34 + j) * st + at] = C->pFB[p];
+			nrm[(30 + j) * st + at] = fa[j];
+			nrm[(34 + j) * st + at] = fb[j];
 		}
+	}
+	__device__ void contact(const float4& nc, const float4& mat) const
+	{
+		*cnt = (uint32_t)f2i(nc.w);
+		nrm[at] = nc;
+		nrm[17 * st + at] = mat;
 	}
 };
 
@@ -1286,12 +1303,9 @@ __global__ void __launch_bounds__(256) k_giant_prepare(GiantSolveCtx S)
 	int32_t c = (int32_t)X.contactList[k];
 	int32_t at = (int32_t)S.atOf[c];
 	int4 r = S.rec[at];
-	CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0 };
+	CopySink sink = { S.nrm, &P.C, S.nrmStride, (uint32_t)at, (r.w & kRecCurved) == 0, (r.w & kRecBoxBox) != 0,
+					  S.cnt + S.entryAt[at] };
 	prepare_contact(P.B, P.C, c, P.h, sink);
-	float4 nc = P.C.normal[c];
-	S.cnt[S.entryAt[at]] = (uint32_t)f2i(nc.w);
-	S.nrm[at] = nc;
-	S.nrm[17 * S.nrmStride + at] = P.C.mat[c];
 }
 
 // Every substep: the record's form (the packet's largest point count, or the

@@ -364,13 +364,17 @@ MX_HD float min3(float a, float b, float c) { return fminf(a, fminf(b, c)); }
 
 // Per-substep contact preparation; src/xpbd.cpp:733-863.
 // Receives each point's prepared rows (anchor A, anchor B with wN, angular
-// Jacobians A and B, arms A and B) as they are stored; the default drops them.
+// Jacobians A and B, arms A and B) as they are stored, after a preload call
+// per point made with the contact's other loads, and the contact's normal and
+// material words as they stand at the end; the default drops them.
 struct NoPrepSink
 {
+	MX_HD void preload(int, int32_t) const {}
 	MX_HD void point(int, int32_t, const float4&, const float4&, const float4&, const float4&, const float4&,
 					 const float4&) const
 	{
 	}
+	MX_HD void contact(const float4&, const float4&) const {}
 };
 
 template <class Sink = NoPrepSink>
@@ -480,34 +484,62 @@ MX_HD void prepare_contact(const DBodies& B, const DContacts& C, int32_t c, floa
 		C.mat[c] = make_float4(mat.x, mat.y, mat.z, u2f(cf));
 	}
 
-	// freeze arms, Jacobians and normal mass for this substep
+	// freeze arms, Jacobians and normal mass for this substep. Every point's
+	// inputs are loaded before the first store: on this device a load issued
+	// after a store is waited for together with the store's completion, so
+	// loads between the points' stores would cost a memory round trip each.
 	m3d_mat3 iwA = load_iw(B, ia), iwB = load_iw(B, ib);
 	float4 SA = B.shape[ia], SB = B.shape[ib];
 	int tA = bf_shape(B.flags[ia]), tB = bf_shape(B.flags[ib]);
 	float hhA = B.mat[ia].x, hhB = B.mat[ib].x;
-	for (int j = 0; j < count; ++j)
+	float4 a4v[4], b4v[4], rav[4], rbv[4];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
 	{
-		int32_t p = pidx(C, c, j);
-		float4 a4 = C.pA[p], b4 = C.pB[p];
-		m3d_vec3 rA = contact_arm(tA, SA.w, hhA, qa, f4_v3(a4), n, 1.0f);
-		m3d_vec3 rB = contact_arm(tB, SB.w, hhB, qb, f4_v3(b4), n, -1.0f);
-		m3d_vec3 crA = m3d_cross(rA, n);
-		m3d_vec3 crB = m3d_cross(rB, n);
-		m3d_vec3 jA = m3d_mat3_mulv(&iwA, crA);
-		m3d_vec3 jB = m3d_mat3_mulv(&iwB, crB);
-		float wN = imA + imB + m3d_dot(crA, jA) + m3d_dot(crB, jB);
-		const float4 bw = make_float4(b4.x, b4.y, b4.z, wN);
-		const float4 ja = v3_f4(jA, 0.0f); // lambdaN = 0
-		const float4 jb = v3_f4(jB, 0.0f); // lambdaT = 0
-		C.pB[p] = bw;
-		C.pJA[p] = ja;
-		C.pJB[p] = jb;
-		float4 ra = C.pRA[p], rb = C.pRB[p];
-		const float4 ra2 = v3_f4(rA, ra.w), rb2 = v3_f4(rB, rb.w);
-		C.pRA[p] = ra2;
-		C.pRB[p] = rb2;
-		sink.point(j, p, a4, bw, ja, jb, ra2, rb2);
+		if (j < count)
+		{
+			int32_t p = pidx(C, c, j);
+			a4v[j] = C.pA[p];
+			b4v[j] = C.pB[p];
+			rav[j] = C.pRA[p];
+			rbv[j] = C.pRB[p];
+			sink.preload(j, p);
+		}
 	}
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		if (j < count)
+		{
+			int32_t p = pidx(C, c, j);
+			float4 a4 = a4v[j], b4 = b4v[j];
+			m3d_vec3 rA = contact_arm(tA, SA.w, hhA, qa, f4_v3(a4), n, 1.0f);
+			m3d_vec3 rB = contact_arm(tB, SB.w, hhB, qb, f4_v3(b4), n, -1.0f);
+			m3d_vec3 crA = m3d_cross(rA, n);
+			m3d_vec3 crB = m3d_cross(rB, n);
+			m3d_vec3 jA = m3d_mat3_mulv(&iwA, crA);
+			m3d_vec3 jB = m3d_mat3_mulv(&iwB, crB);
+			float wN = imA + imB + m3d_dot(crA, jA) + m3d_dot(crB, jB);
+			const float4 bw = make_float4(b4.x, b4.y, b4.z, wN);
+			const float4 ja = v3_f4(jA, 0.0f); // lambdaN = 0
+			const float4 jb = v3_f4(jB, 0.0f); // lambdaT = 0
+			C.pB[p] = bw;
+			C.pJA[p] = ja;
+			C.pJB[p] = jb;
+			float4 ra = rav[j], rb = rbv[j];
+			const float4 ra2 = v3_f4(rA, ra.w), rb2 = v3_f4(rB, rb.w);
+			C.pRA[p] = ra2;
+			C.pRB[p] = rb2;
+			sink.point(j, p, a4, bw, ja, jb, ra2, rb2);
+		}
+	}
+	if (stale)
+	{
+		// as the regeneration (or the kept manifold) left them
+		nc = C.normal[c];
+		mat = C.mat[c];
+	}
+	sink.contact(nc, mat);
 }
 
 // ------------------------------------------------------ normal projection
