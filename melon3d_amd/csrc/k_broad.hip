@@ -133,11 +133,10 @@ MX_HD m3d_aabb fat_lo_hi(float4 lo, float4 hi)
 	return a;
 }
 
-__global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uint32_t* counters)
+// One body's broadphase refresh; true when its fat AABB was refitted.
+__device__ __forceinline__ bool refresh_body(const DBodies& B, int32_t i, float dt, int rebuildStable)
 {
-	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i >= n)
-		return;
+	bool refit = false;
 	uint32_t fl = B.flags[i];
 	bool inUse = (fl & BF_IN_USE) != 0;
 	bool awake = (fl & BF_AWAKE) != 0;
@@ -176,7 +175,7 @@ __global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uin
 			B.fatLo[i] = v3_f4(fat.lowerBound, 0.0f);
 			B.fatHi[i] = v3_f4(fat.upperBound, 0.0f);
 			bp |= BP_HAS_FAT | BP_MOVED;
-			wave_inc(&counters[CT_REFAT]);
+			refit = true;
 		}
 	}
 	else if (stable && rebuildStable && !(bp & BP_HAS_FAT))
@@ -198,6 +197,19 @@ __global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uin
 	}
 	B.bp[i] = bp;
 	B.sb[i] = sb;
+	return refit;
+}
+
+// The refit count is summed per workgroup (one atomic each) over a grid of
+// at most kRefreshGroups workgroups: an atomic from every wave (or from
+// every workgroup of a 200k-body grid) on one counter serializes.
+constexpr unsigned kRefreshGroups = 256;
+__global__ void k_refresh(DBodies B, int32_t n, float dt, int rebuildStable, uint32_t* counters)
+{
+	uint32_t refits = 0;
+	for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int32_t)(gridDim.x * blockDim.x))
+		refits += refresh_body(B, i, dt, rebuildStable) ? 1u : 0u;
+	block_add<1>(counters, { CT_REFAT }, { refits });
 }
 
 void launch_refresh(World& w, float dt, bool rebuildStable)
@@ -205,7 +217,8 @@ void launch_refresh(World& w, float dt, bool rebuildStable)
 	int32_t n = w.deviceBodyCount;
 	if (n > 0)
 	{
-		k_refresh<<<grid_for(n), kBlock, 0, w.stream>>>(body_view(w), n, dt, rebuildStable ? 1 : 0, w.counters.p);
+		const unsigned g = std::min<unsigned>(grid_for(n), kRefreshGroups);
+		k_refresh<<<g, kBlock, 0, w.stream>>>(body_view(w), n, dt, rebuildStable ? 1 : 0, w.counters.p);
 	}
 }
 
@@ -322,8 +335,12 @@ __global__ void k_compact_indices(const uint32_t* flags, const uint32_t* offs, i
 // *gmin = min(*gmin, mn) and *gmax = max(*gmax, mx) over the workgroup's
 // lanes with valid set: wave shuffles, then one pair of global atomics per
 // workgroup (callers loop over their elements on reduce_grid workgroups).
-// Every lane of the workgroup calls it.
-__device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool valid, uint32_t* gmin, uint32_t* gmax)
+// With part set, the workgroup's (min, max) goes to part[0], part[1]
+// instead (all ones and zero for none), for k_bounds_reduce: the atomics of
+// hundreds of workgroups on one line serialize. Every lane of the workgroup
+// calls it.
+__device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool valid, uint32_t* gmin, uint32_t* gmax,
+											 uint32_t* part = nullptr)
 {
 	__shared__ uint32_t smin[16], smax[16];
 	mn = wave_min(valid ? mn : 0xFFFFFFFFu);
@@ -343,7 +360,12 @@ __device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool vali
 			mn = smin[k] < mn ? smin[k] : mn;
 			mx = smax[k] > mx ? smax[k] : mx;
 		}
-		if (mn <= mx)
+		if (part != nullptr)
+		{
+			part[0] = mn;
+			part[1] = mx;
+		}
+		else if (mn <= mx)
 		{
 			atomicMin(gmin, mn);
 			atomicMax(gmax, mx);
@@ -352,10 +374,49 @@ __device__ __forceinline__ void block_minmax(uint32_t mn, uint32_t mx, bool vali
 	__syncthreads();
 }
 
+// Cell bounds from the per-workgroup partials of k_tier_emit: one
+// workgroup, nb partials of six words (x min, x max, y min, y max, z min,
+// z max), written to the counters block.
+__global__ void __launch_bounds__(256) k_bounds_reduce(const uint32_t* part, int32_t nb, uint32_t* counters)
+{
+	__shared__ uint32_t red[6][4];
+	uint32_t v[6] = { 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u };
+	for (int32_t b = (int32_t)threadIdx.x; b < nb; b += (int32_t)blockDim.x)
+	{
+#pragma unroll
+		for (int k = 0; k < 6; ++k)
+		{
+			uint32_t x = part[6 * (int64_t)b + k];
+			v[k] = (k & 1) ? (x > v[k] ? x : v[k]) : (x < v[k] ? x : v[k]);
+		}
+	}
+	const uint32_t wv = threadIdx.x >> 6;
+#pragma unroll
+	for (int k = 0; k < 6; ++k)
+	{
+		v[k] = (k & 1) ? wave_max(v[k]) : wave_min(v[k]);
+		if (__lane_id() == 0)
+			red[k][wv] = v[k];
+	}
+	__syncthreads();
+	if (threadIdx.x < 6u)
+	{
+		const int k = (int)threadIdx.x;
+		const uint32_t nw = (blockDim.x + 63) >> 6;
+		uint32_t r = red[k][0];
+		for (uint32_t w = 1; w < nw; ++w)
+			r = (k & 1) ? (red[k][w] > r ? red[k][w] : r) : (red[k][w] < r ? red[k][w] : r);
+		counters[CT_UX_MIN + k] = r;
+	}
+}
+
 // Entries at positions >= room are not stored (their cell bounds still
 // count): the caller emits again into a larger buffer.
+// part: per-workgroup bounds (six words each, then k_bounds_reduce), or null
+// for atomics on the counters.
 __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated, float cellSize, const uint32_t* cnt,
-							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, int64_t room, uint32_t* counters)
+							const uint32_t* offs, uint64_t* keys, uint32_t* bodies, int64_t room, uint32_t* counters,
+							uint32_t* part)
 {
 	uint32_t mnx = 0xFFFFFFFFu, mxx = 0, mny = 0xFFFFFFFFu, mxy = 0, mnz = 0xFFFFFFFFu, mxz = 0;
 	bool any = false;
@@ -391,9 +452,10 @@ __global__ void k_tier_emit(DBodies B, int32_t n, uint8_t includeBit, int gated,
 					any = true;
 				}
 	}
-	block_minmax(mnx, mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX]);
-	block_minmax(mny, mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX]);
-	block_minmax(mnz, mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX]);
+	uint32_t* pb = part != nullptr ? part + 6 * (int64_t)blockIdx.x : nullptr;
+	block_minmax(mnx, mxx, any, &counters[CT_UX_MIN], &counters[CT_UX_MAX], pb);
+	block_minmax(mny, mxy, any, &counters[CT_UY_MIN], &counters[CT_UY_MAX], pb ? pb + 2 : nullptr);
+	block_minmax(mnz, mxz, any, &counters[CT_UZ_MIN], &counters[CT_UZ_MAX], pb ? pb + 4 : nullptr);
 }
 
 // Sort keys: the occupied box of cell fields (and the sub-world, most
@@ -531,7 +593,8 @@ bool build_tier(World& w, int tier)
 	// together; a tier with more entries than room is emitted once more.
 	size_t target = std::max<size_t>(std::min(keys.cap, vals.cap), 4 * (size_t)n + 1024);
 	if (!w.u32a.ensure(n) || !w.u32b.ensure(n) || !w.u32c.ensure(n) || !w.u32d.ensure(n) || !w.i32a.ensure(n) ||
-		!ensure_prim_scratch(w, n) || !keys.ensure(target) || !vals.ensure(target))
+		!ensure_prim_scratch(w, n) || !keys.ensure(target) || !vals.ensure(target) ||
+		!w.boundParts.ensure(6 * (size_t)reduce_grid(n, kBlock)))
 		return false;
 	int64_t room = (int64_t)std::min(keys.cap, vals.cap);
 	uint32_t* cnt = w.u32a.p;
@@ -544,9 +607,10 @@ bool build_tier(World& w, int tier)
 	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
 	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
 	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, cellBit, cnt);
-	k_bounds_init<<<1, 64, 0, s>>>(w.counters.p);
-	k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p,
-														   vals.p, room, w.counters.p);
+	const unsigned emitGrid = reduce_grid(n, kBlock);
+	k_tier_emit<<<emitGrid, kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p, vals.p, room,
+											w.counters.p, w.boundParts.p);
+	k_bounds_reduce<<<1, 256, 0, s>>>(w.boundParts.p, (int32_t)emitGrid, w.counters.p);
 	int32_t* head = (int32_t*)w.hostCounters + kCountersSize + tier * kOverHead;
 	const int32_t nHead = std::min(n, kOverHead);
 	if (!hip_ok(hipMemcpyAsync(head, w.i32a.p, (size_t)nHead * sizeof(int32_t), hipMemcpyDeviceToHost, s),
@@ -564,8 +628,9 @@ bool build_tier(World& w, int tier)
 			// the same keys, so the counters keep their values)
 			if (!keys.ensure(E) || !vals.ensure(E))
 				return false;
-			k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs,
-																   keys.p, vals.p, E, w.counters.p);
+			k_tier_emit<<<emitGrid, kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, offs, keys.p, vals.p,
+													E, w.counters.p, w.boundParts.p);
+			k_bounds_reduce<<<1, 256, 0, s>>>(w.boundParts.p, (int32_t)emitGrid, w.counters.p);
 		}
 		if (nov > nHead)
 		{
@@ -1953,7 +2018,7 @@ bool build_stable_batched(World& w)
 	if (eNew > 0)
 		k_tier_emit<<<reduce_grid(n, kBlock), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, offs,
 												   w.stableKeysNext.p + eKeep, w.stableBodiesNext.p + eKeep,
-												   eNew, w.counters.p);
+												   eNew, w.counters.p, nullptr);
 	if (!sync_counters(w))
 		return false;
 	int bits = 0;

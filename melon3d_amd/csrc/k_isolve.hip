@@ -1293,6 +1293,15 @@ struct CopySink
 // written at scattered record positions (in record order, lanes of one
 // wave would each touch another line of every contact column, since a
 // slot holds one contact in every few of an island).
+// Workgroup size of prepare: one wave up to kPrepareWaveEntries grid
+// contacts, 256 lanes above. A step of a streaming world holds a few tens of
+// thousands of grid contacts: 256-lane workgroups would leave most CUs idle
+// and put four of the few waves on each busy CU, while single waves spread
+// over every CU (the kernel then takes the time of its slowest wave, whose
+// manifold regenerations dominate; 0.30 -> 0.27 ms per step on the 200k
+// stream). With millions of contacts (a batch of worlds) the larger
+// workgroups are faster (6.35 -> 6.08 ms per step).
+constexpr int32_t kPrepareWaveEntries = 65536;
 __global__ void __launch_bounds__(256) k_giant_prepare(GiantSolveCtx S)
 {
 	const PassCtx& P = S.X.P;
@@ -1795,6 +1804,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 			w.tuner.pendingFailed = measure;
 		}
 	};
+	const int prepThreads = NC <= kPrepareWaveEntries ? 64 : 256;
 	auto substepLoop = [&]() -> bool {
 	for (int sub = 0; sub < substeps; ++sub)
 	{
@@ -1802,7 +1812,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
 		w.mark_end();
 		w.mark_begin(M3DX_K_GIANT_PREPARE);
-		k_giant_prepare<<<gridk(NC), 256, 0, s>>>(S);
+		k_giant_prepare<<<(unsigned)((NC + prepThreads - 1) / prepThreads), prepThreads, 0, s>>>(S);
 		w.mark_end();
 		k_giant_rec_forms<<<reduce_grid(NC, 256), 256, 0, s>>>(S);
 		if (!diff_record_substep(w, sub, S.atOf, S.rec))

@@ -19,18 +19,15 @@ DContacts make_contact_view(World& w);
 constexpr int kBlockN = 256;
 static inline unsigned gridn(int64_t n) { return (unsigned)((n + kBlockN - 1) / kBlockN); }
 
-__global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t count, float dt, int8_t* trans,
-												uint32_t* counters)
+// One contact's narrowphase; true when its manifold was regenerated.
+__device__ __forceinline__ bool narrow_contact(const DBodies& B, const DContacts& C, int32_t c, float dt, int8_t* trans)
 {
-	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (c >= count)
-		return;
 	int2 ab = C.bodies[c];
 	int32_t ia = ab.x, ib = ab.y;
 	if (!(B.sb[ia] & SB_AWAKE) && !(B.sb[ib] & SB_AWAKE))
 	{
 		trans[c] = 0;
-		return;
+		return false;
 	}
 	float4 mat = C.mat[c];
 	uint32_t cf = f2u(mat.w);
@@ -43,7 +40,7 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 		if (tracked_min_sep(pa, qa, pb, qb, StoredManifold(C, c)) < -kLinearSlop)
 		{
 			trans[c] = 0;
-			return;
+			return false;
 		}
 		cf &= ~CF_TUNNEL_GUARD;
 	}
@@ -56,7 +53,6 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 	}
 	if (regen)
 	{
-		wave_inc(&counters[CT_REGEN]);
 		// the old manifold is read where it lies; the new one is built here
 		// (slots beyond its count are never stored and keep their contents)
 		const StoredManifold old(C, c);
@@ -82,6 +78,17 @@ __global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t 
 	cf = now ? (cf | CF_TOUCHING) : (cf & ~CF_TOUCHING);
 	trans[c] = (int8_t)(now && !was ? 1 : (!now && was ? -1 : 0));
 	C.mat[c] = make_float4(mat.x, mat.y, mat.z, u2f(cf));
+	return regen;
+}
+
+// The regeneration count is summed per workgroup (one atomic each): an
+// atomic from every wave on one counter serializes.
+__global__ void __launch_bounds__(256) k_narrow(DBodies B, DContacts C, int32_t count, float dt, int8_t* trans,
+												uint32_t* counters)
+{
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	bool regen = c < count && narrow_contact(B, C, c, dt, trans);
+	block_add<1>(counters, { CT_REGEN }, { regen ? 1u : 0u });
 }
 
 __global__ void k_trans_flags(const int8_t* trans, int32_t count, uint32_t* flags)

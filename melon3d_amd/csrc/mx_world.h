@@ -266,6 +266,7 @@ struct World
 	DBuf<char> primScratch;       // radix sort histograms
 	DBuf<uint64_t> scanStatus;    // scan look-back words (nothing else writes them)
 	DBuf<uint32_t> pairRuns;      // pair enumeration: run tables, item counts and offsets
+	DBuf<uint32_t> boundParts;    // tier builds: per-workgroup cell bounds (six words each)
 	DBuf<uint64_t> overSegs;      // coarse-list candidate segments
 	DBuf<uint64_t> hashKeys;
 	DBuf<uint32_t> hashVals;

@@ -105,6 +105,7 @@ World::~World()
 	u8a.release();
 	primScratch.release();
 	pairRuns.release();
+	boundParts.release();
 	overSegs.release();
 	hashKeys.release();
 	scanStatus.release();
