@@ -98,11 +98,17 @@ __device__ __forceinline__ void block_add(uint32_t* ctr, const int (&idx)[N], co
 	}
 }
 
+// Rounds of the keyed helpers below: each round sums the lanes of one key
+// into one atomic; the lanes left after them (keys shared by few lanes, as
+// the one-body islands of a scattered world, which would take a round per
+// lane) issue their own atomics together.
+constexpr int kKeyedRounds = 4;
+
 // ctr[key] += v for every lane whose key is not ~0u.
 __device__ __forceinline__ void wave_add_keyed(uint32_t* ctr, uint32_t key, uint32_t v)
 {
 	uint64_t todo = __ballot(key != ~0u);
-	while (todo)
+	for (int round = 0; round < kKeyedRounds && todo; ++round)
 	{
 		int leader = __ffsll((unsigned long long)todo) - 1;
 		uint32_t lk = __shfl(key, leader);
@@ -112,13 +118,15 @@ __device__ __forceinline__ void wave_add_keyed(uint32_t* ctr, uint32_t key, uint
 			atomicAdd(&ctr[lk], s);
 		todo &= ~__ballot(mine);
 	}
+	if ((todo >> __lane_id()) & 1ull)
+		atomicAdd(&ctr[key], v);
 }
 
 // ctr[key] = min(ctr[key], v) for every lane whose key is not ~0u.
 __device__ __forceinline__ void wave_min_keyed(uint32_t* ctr, uint32_t key, uint32_t v)
 {
 	uint64_t todo = __ballot(key != ~0u);
-	while (todo)
+	for (int round = 0; round < kKeyedRounds && todo; ++round)
 	{
 		int leader = __ffsll((unsigned long long)todo) - 1;
 		uint32_t lk = __shfl(key, leader);
@@ -128,6 +136,8 @@ __device__ __forceinline__ void wave_min_keyed(uint32_t* ctr, uint32_t key, uint
 			atomicMin(&ctr[lk], m);
 		todo &= ~__ballot(mine);
 	}
+	if ((todo >> __lane_id()) & 1ull)
+		atomicMin(&ctr[key], v);
 }
 
 } // namespace mx
