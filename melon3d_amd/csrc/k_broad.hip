@@ -1865,11 +1865,18 @@ bool upload_batch_state(World& w)
 	int32_t n = (int32_t)w.bodySubHost.size();
 	if (w.bodySubDirty && n > 0)
 	{
-		if (!w.bodySub.ensure(n))
+		const int32_t S = w.subCount;
+		if (!w.bodySub.ensure(n) || !w.subBodyStartD.ensure((size_t)S + 1))
 			return false;
 		if (!hip_ok(hipMemcpyAsync(w.bodySub.p, w.bodySubHost.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s),
+					"sub-world upload") ||
+			!hip_ok(hipMemcpyAsync(w.subBodyStartD.p, w.subBodyStart.data(), ((size_t)S + 1) * sizeof(int32_t),
+								   hipMemcpyHostToDevice, s),
 					"sub-world upload"))
 			return false;
+		w.maxSubBodies = 0;
+		for (int32_t k = 0; k < S; ++k)
+			w.maxSubBodies = std::max(w.maxSubBodies, w.subBodyStart[k + 1] - w.subBodyStart[k]);
 		w.bodySubDirty = false;
 	}
 	if (w.anySubDirtyHost)

@@ -97,6 +97,71 @@ __global__ void k_uf_joints(const DJoint* J, int32_t nJ, const uint32_t* flags, 
 		uf_union(parent, jt.bodyA, jt.bodyB);
 }
 
+// Batched worlds: one workgroup per sub-world does its union-find in LDS (a
+// sub-world's bodies and contacts are contiguous ranges, and no contact
+// joins two sub-worlds). Same result as k_uf_init, k_uf_contacts and
+// k_uf_roots: each body's root is its island's smallest body.
+constexpr int32_t kUfLocalBodies = 4096;
+
+__device__ __forceinline__ int32_t uf_find_lds(int32_t* par, int32_t x)
+{
+	for (;;)
+	{
+		int32_t p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (p == x)
+			return x;
+		int32_t gp = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (gp != p)
+			__hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		x = p;
+	}
+}
+
+__global__ void __launch_bounds__(256) k_uf_sub(const int2* bodies, const float4* mat, const uint8_t* sb,
+												const int32_t* subBodyStart, const int32_t* subContactStart,
+												int32_t* parent, uint32_t* rootFlag)
+{
+	__shared__ int32_t par[kUfLocalBodies];
+	const int32_t sw = (int32_t)blockIdx.x;
+	const int32_t b0 = subBodyStart[sw], nb = subBodyStart[sw + 1] - b0;
+	const int32_t c0 = subContactStart[sw], c1 = subContactStart[sw + 1];
+	if (nb > kUfLocalBodies)
+		return; // the host sends such batches to the global kernels
+	for (int32_t i = (int32_t)threadIdx.x; i < nb; i += (int32_t)blockDim.x)
+		par[i] = i;
+	__syncthreads();
+	for (int32_t c = c0 + (int32_t)threadIdx.x; c < c1; c += (int32_t)blockDim.x)
+	{
+		uint32_t cf = f2u(mat[c].w);
+		if (!(cf & CF_TOUCHING) || (cf & CF_SENSOR))
+			continue;
+		int2 ab = bodies[c];
+		int32_t a = ab.x - b0, b = ab.y - b0;
+		if ((uint32_t)a >= (uint32_t)nb || (uint32_t)b >= (uint32_t)nb)
+			continue;
+		if (!((sb[ab.x] & SB_DYNAMIC) && (sb[ab.y] & SB_DYNAMIC)))
+			continue;
+		for (;;)
+		{
+			int32_t ra = uf_find_lds(par, a);
+			int32_t rb = uf_find_lds(par, b);
+			if (ra == rb)
+				break;
+			int32_t lo = ra < rb ? ra : rb;
+			int32_t hi = ra < rb ? rb : ra;
+			if (atomicCAS(&par[hi], hi, lo) == hi)
+				break;
+		}
+	}
+	__syncthreads();
+	for (int32_t i = (int32_t)threadIdx.x; i < nb; i += (int32_t)blockDim.x)
+	{
+		int32_t r = uf_find_lds(par, i);
+		parent[b0 + i] = b0 + r;
+		rootFlag[b0 + i] = ((sb[b0 + i] & SB_DYNAMIC) && r == i) ? 1u : 0u;
+	}
+}
+
 __global__ void k_uf_roots(int32_t* parent, const uint8_t* sb, int32_t n, uint32_t* rootFlag)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -383,12 +448,21 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		return false;
 	int32_t* parent = w.i32a.p;
 	k_any_awake_dyn<<<grids(n), kBlockS, 0, s>>>(w.bc.sb.p, n, w.counters.p);
-	k_uf_init<<<grids(n), kBlockS, 0, s>>>(parent, n);
-	if (C > 0)
-		k_uf_contacts<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, C, parent);
-	if (w.djointCount > 0)
-		k_uf_joints<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p, parent);
-	k_uf_roots<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, n, w.u32a.p);
+	if (w.subCount > 0 && w.djointCount == 0 && w.maxSubBodies <= kUfLocalBodies &&
+		w.subBodyStart.size() == (size_t)w.subCount + 1 && w.subBodyStart[w.subCount] == n)
+	{
+		k_uf_sub<<<(unsigned)w.subCount, 256, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, w.subBodyStartD.p,
+													  w.subContactStart.p, parent, w.u32a.p);
+	}
+	else
+	{
+		k_uf_init<<<grids(n), kBlockS, 0, s>>>(parent, n);
+		if (C > 0)
+			k_uf_contacts<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, C, parent);
+		if (w.djointCount > 0)
+			k_uf_joints<<<grids(w.djointCount), kBlockS, 0, s>>>(w.djoints.p, w.djointCount, w.bc.flags.p, parent);
+		k_uf_roots<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, n, w.u32a.p);
+	}
 	scan_exclusive(w.u32a.p, w.u32b.p, n, w.counters.p + CT_ISLANDS, w.scanStatus.p, s);
 	if (!sync_counters(w) || !queue_transitions(w))
 		return false;

@@ -313,6 +313,8 @@ struct World
 	std::vector<uint8_t> subDirtyHost;  // stable-tier dirt raised by API calls
 	bool anySubDirtyHost = false;
 	DBuf<int32_t> bodySub;              // device copy of bodySubHost
+	DBuf<int32_t> subBodyStartD;        // device copy of subBodyStart
+	int32_t maxSubBodies = 0;           // largest sub-world's body count
 	// end events of a batch step's removals: flags, offsets, indices, gathered events
 	DBuf<uint32_t> bEvFlag, bEvOff;
 	DBuf<int32_t> bEvIdx;

@@ -106,6 +106,7 @@ World::~World()
 	primScratch.release();
 	pairRuns.release();
 	boundParts.release();
+	subBodyStartD.release();
 	overSegs.release();
 	hashKeys.release();
 	scanStatus.release();
