@@ -673,8 +673,13 @@ __device__ void mark_successors(const GiantCtx& G, int32_t k, int4 pr, uint8_t* 
 // the other half). Round r flags changes in ctr[r % 3] and clears
 // ctr[(r + 1) % 3], which no lane reads again before it is reused two
 // rounds later.
-template <int WHICH>
-__device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* ctr, int32_t tid, int32_t nt)
+// The entries taken are [k0, k1) (all of them on the grid, one sub-world's
+// on a workgroup: no predecessor or successor of an entry lies in another
+// sub-world), tid and nt the lane's place and count over them; Sync is the
+// grid barrier or the workgroup's (BlockSync).
+template <int WHICH, class Sync>
+__device__ int32_t fixpoint_rounds(const GiantCtx& G, Sync& grid, uint32_t* ctr, int32_t tid, int32_t nt, int32_t k0,
+								   int32_t k1)
 {
 	const int32_t NC = G.X.NC;
 	for (int32_t r = 0; r < (1 << 24); ++r)
@@ -684,7 +689,7 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 		uint8_t* cur = G.dirty + (size_t)(r & 1) * NC;
 		uint8_t* next = G.dirty + (size_t)((r + 1) & 1) * NC;
 		uint32_t changes = 0;
-		for (int32_t k = tid; k < NC; k += nt)
+		for (int32_t k = k0 + tid; k < k1; k += nt)
 		{
 			if (r > 0)
 			{
@@ -719,18 +724,12 @@ __device__ int32_t fixpoint_rounds(const GiantCtx& G, GridSync& grid, uint32_t* 
 	return -1;
 }
 
-__global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
+// Slot keys, (slot, island) for giant entries and a sentinel above all
+// others, of the entries [k0, k1); returns the lane's largest slot + 1.
+__device__ uint32_t order_keys(const GiantCtx& G, int32_t tid, int32_t nt, int32_t k0, int32_t k1)
 {
-	GridSync grid = GridSync::make(G.bar, G.X.xcdBar, G.counters + CT_GRID_ABORT, G.X.barTimeout);
-	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
-	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt);
-	int32_t rl = fixpoint_rounds<1>(G, grid, G.ctr + 3, tid, nt);
-	if (tid == 0)
-		G.counters[CT_COLOR_ROUNDS] = (uint32_t)(rc + rl);
-	// slot keys: (slot, island) for giant entries, a sentinel above all others
 	uint32_t maxSlot = 0;
-	for (int32_t k = tid; k < G.X.NC; k += nt)
+	for (int32_t k = k0 + tid; k < k1; k += nt)
 	{
 		uint64_t key = ~0ull;
 		if (giant_entry(G, k))
@@ -749,6 +748,19 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 		G.slotKeys[k] = key;
 		G.slotVals[k] = G.X.contactList[k];
 	}
+	return maxSlot;
+}
+
+__global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
+{
+	GridSync grid = GridSync::make(G.bar, G.X.xcdBar, G.counters + CT_GRID_ABORT, G.X.barTimeout);
+	int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	int32_t nt = (int32_t)(gridDim.x * blockDim.x);
+	int32_t rc = fixpoint_rounds<0>(G, grid, G.ctr, tid, nt, 0, G.X.NC);
+	int32_t rl = fixpoint_rounds<1>(G, grid, G.ctr + 3, tid, nt, 0, G.X.NC);
+	if (tid == 0)
+		G.counters[CT_COLOR_ROUNDS] = (uint32_t)(rc + rl);
+	uint32_t maxSlot = order_keys(G, tid, nt, 0, G.X.NC);
 	// one atomic per wave (all lanes reach here)
 	for (int off = 32; off > 0; off >>= 1)
 	{
@@ -757,6 +769,64 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_order(GiantCtx G)
 	}
 	if (__lane_id() == 0 && maxSlot > 0u)
 		atomicMax(&G.counters[CT_SLOTS], maxSlot);
+}
+
+// Batched worlds: a sub-world's islands are contiguous island ids and their
+// entries a contiguous range, so one workgroup per sub-world runs the same
+// fixpoints over its range with workgroup barriers (the grid barriers of
+// k_giant_order cost microseconds each, over rounds of a whole batch).
+struct BlockSync
+{
+	__device__ uint32_t sync_read(const uint32_t* word)
+	{
+		__syncthreads();
+		uint32_t v = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		__syncthreads();
+		return v;
+	}
+};
+
+// start[s] = first entry of sub-world s or later (entries ascend by
+// sub-world), start[S] = NC.
+__global__ void k_sub_entry_start(const GiantCtx G, const int32_t* bodySub, int32_t S, int32_t* start)
+{
+	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	const int32_t NC = G.X.NC;
+	if (k > NC)
+		return;
+	int32_t hi = k < NC ? bodySub[G.X.P.C.bodies[G.X.contactList[k]].x] : S;
+	int32_t lo = k > 0 ? bodySub[G.X.P.C.bodies[G.X.contactList[k - 1]].x] : -1;
+	for (int32_t sw = lo + 1; sw <= hi; ++sw)
+		start[sw] = k;
+}
+
+__global__ void __launch_bounds__(kGiantThreads) k_giant_order_sub(GiantCtx G, const int32_t* subEntryStart)
+{
+	__shared__ uint32_t ctr[6];
+	const int32_t k0 = subEntryStart[blockIdx.x], k1 = subEntryStart[blockIdx.x + 1];
+	if (threadIdx.x < 6u)
+		ctr[threadIdx.x] = 0u;
+	__syncthreads();
+	BlockSync sync;
+	const int32_t tid = (int32_t)threadIdx.x, nt = (int32_t)blockDim.x;
+	int32_t rc = fixpoint_rounds<0>(G, sync, ctr, tid, nt, k0, k1);
+	int32_t rl = fixpoint_rounds<1>(G, sync, ctr + 3, tid, nt, k0, k1);
+	uint32_t maxSlot = order_keys(G, tid, nt, k0, k1);
+	maxSlot = wave_max(maxSlot);
+	__shared__ uint32_t wmax[kGiantThreads / 64];
+	if (__lane_id() == 0)
+		wmax[threadIdx.x >> 6] = maxSlot;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		for (uint32_t w = 1; w < (blockDim.x + 63) / 64; ++w)
+			maxSlot = wmax[w] > maxSlot ? wmax[w] : maxSlot;
+		if (maxSlot > 0u)
+			atomicMax(&G.counters[CT_SLOTS], maxSlot);
+		// rounds of the sub-world that took the most
+		if (k1 > k0)
+			atomicMax(&G.counters[CT_COLOR_ROUNDS], (uint32_t)(rc + rl));
+	}
 }
 
 // Adjacency pairs (body, entry) of the giant entries, and the fixpoint's
@@ -1734,6 +1804,17 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.dirty = w.sDirty.p;
 	G.bitsI = L.bitsI;
 	G.maxS = maxS;
+	if (w.subCount > 0)
+	{
+		// batches: one workgroup per sub-world
+		if (!w.subEntryStart.ensure((size_t)w.subCount + 1))
+			return false;
+		k_sub_entry_start<<<gridk((int64_t)NC + 1), 256, 0, s>>>(G, w.bodySub.p, w.subCount, w.subEntryStart.p);
+		w.mark_begin(M3DX_K_GIANT_ORDER);
+		k_giant_order_sub<<<(unsigned)w.subCount, kGiantThreads, 0, s>>>(G, w.subEntryStart.p);
+		w.mark_end();
+	}
+	else
 	{
 		static const int cap = env_blocks("M3D_ORDER_BLOCKS", kOrderMaxBlocks);
 		int blocks = giant_blocks<k_giant_order>(w, NC, cap);

@@ -314,6 +314,7 @@ struct World
 	bool anySubDirtyHost = false;
 	DBuf<int32_t> bodySub;              // device copy of bodySubHost
 	DBuf<int32_t> subBodyStartD;        // device copy of subBodyStart
+	DBuf<int32_t> subEntryStart;        // grid solve of a batch: first solver entry of each sub-world
 	int32_t maxSubBodies = 0;           // largest sub-world's body count
 	// end events of a batch step's removals: flags, offsets, indices, gathered events
 	DBuf<uint32_t> bEvFlag, bEvOff;

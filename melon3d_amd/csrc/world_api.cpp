@@ -107,6 +107,7 @@ World::~World()
 	pairRuns.release();
 	boundParts.release();
 	subBodyStartD.release();
+	subEntryStart.release();
 	overSegs.release();
 	hashKeys.release();
 	scanStatus.release();
