@@ -657,7 +657,14 @@ bool build_tier(World& w, int tier)
 	int bits = 0;
 	KeyCompress kc = key_compress_from(w.hostCounters, w.subCount, &bits);
 	k_key_compress<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
-	radix_sort_pairs(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p, w.scanStatus.p, s);
+	// an odd pass count leaves the tier in the ping-pong buffers, which then
+	// become the tier's (no copy back)
+	if (radix_sort_pairs_in_place_or_alt(keys.p, vals.p, w.sortKeysAlt.p, w.sortValsAlt.p, E, bits, w.primScratch.p,
+										 w.scanStatus.p, s))
+	{
+		std::swap(keys, w.sortKeysAlt);
+		std::swap(vals, w.sortValsAlt);
+	}
 	k_key_expand<<<grid_for(E), kBlock, 0, s>>>(keys.p, E, kc);
 	return true;
 }

@@ -595,9 +595,19 @@ void set_sort_timer(const SortTimer* t) { t_sortTimer = t; }
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
 					  void* scratch, uint64_t* status, hipStream_t s)
 {
+	if (radix_sort_pairs_in_place_or_alt(keys, vals, keysAlt, valsAlt, n, bits, scratch, status, s))
+	{
+		hipMemcpyAsync(keys, keysAlt, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+		hipMemcpyAsync(vals, valsAlt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+	}
+}
+
+bool radix_sort_pairs_in_place_or_alt(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n,
+									  int bits, void* scratch, uint64_t* status, hipStream_t s)
+{
 	if (n <= 1 || bits <= 0)
 	{
-		return;
+		return false;
 	}
 	const int passes = std::min((bits + kRadixBits - 1) / kRadixBits, kMaxPasses);
 	const int64_t tiles = (n + kSortTile - 1) / kSortTile;
@@ -645,13 +655,9 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_
 			std::swap(vin, vout);
 		}
 	}
-	if (kin != keys)
-	{
-		hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
-		hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-	}
 	if (timer != nullptr)
 		timer->end(timer->ctx);
+	return kin != keys;
 }
 
 } // namespace mx

@@ -25,6 +25,10 @@ int64_t sort_status_words(int64_t n);
 // Results end in keys/vals; keysAlt/valsAlt are ping-pong buffers of n.
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n, int bits,
 					  void* scratch, uint64_t* status, hipStream_t s);
+// The same sort without the copy back after an odd number of passes: true
+// when the results are in keysAlt/valsAlt.
+bool radix_sort_pairs_in_place_or_alt(uint64_t* keys, uint32_t* vals, uint64_t* keysAlt, uint32_t* valsAlt, int64_t n,
+									  int bits, void* scratch, uint64_t* status, hipStream_t s);
 
 // Kernel timing of the sorts issued by one thread: while a timer is set,
 // radix_sort_pairs calls begin(ctx, bytes) before its passes, with the
