@@ -970,6 +970,10 @@ struct GiantSolveCtx
 	// contacts, which only the contact itself reads later in the substep.
 	float4* nrm;
 	int64_t nrmStride;
+	// the substep returns lambdaN from the copy to the contacts (the last
+	// one only: each substep's prepare zeroes it again, and within a
+	// substep every pass reads it from the copy)
+	int lambdaBack;
 };
 
 constexpr int kSchedBlocks = 38;
@@ -1486,6 +1490,7 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx 
 		giant_sweep<NormalsPass>(S, ss, grid, nSlots);
 	const int32_t lanes = (int32_t)(gridDim.x * blockDim.x);
 	const int32_t tid = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (S.lambdaBack)
 	{
 		const DContacts& C = S.X.P.C;
 		const int64_t st = S.nrmStride;
@@ -1497,8 +1502,8 @@ __global__ void __launch_bounds__(kGiantThreads) k_giant_position(GiantSolveCtx 
 			for (int j = 0; j < 4 && j < count; ++j)
 				C.pJA[pidx(C, c, j)].w = S.nrm[(9 + j) * st + k].w;
 		}
+		grid.sync();
 	}
-	grid.sync();
 	giant_sweep<FrictionPass>(S, ss, grid, nSlots);
 	{
 		const IslandCtx& X = S.X;
@@ -1736,6 +1741,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	}
 
 	GiantSolveCtx S;
+	S.lambdaBack = 1;
 	S.X = X;
 	if (L.NC == 0)
 	{
@@ -1889,6 +1895,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	auto substepLoop = [&]() -> bool {
 	for (int sub = 0; sub < substeps; ++sub)
 	{
+		S.lambdaBack = sub == substeps - 1 ? 1 : 0;
 		w.mark_begin(M3DX_K_GIANT_BODIES);
 		k_giant_bodies<<<gridk(L.NB), 256, 0, s>>>(S, 0);
 		w.mark_end();
@@ -1927,7 +1934,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 			ok = coop((const void*)k_giant_pass<NormalsPass>, X.iterations);
 			w.mark_end();
 		}
-		k_giant_lambda_back<<<gridk(NC), 256, 0, s>>>(S);
+		if (S.lambdaBack)
+			k_giant_lambda_back<<<gridk(NC), 256, 0, s>>>(S);
 		w.mark_begin(M3DX_K_GIANT_FRICTION);
 		ok = ok && coop((const void*)k_giant_pass<FrictionPass>, 1);
 		w.mark_end();
