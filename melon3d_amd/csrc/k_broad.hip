@@ -602,9 +602,8 @@ bool build_tier(World& w, int tier)
 	uint32_t* ov = w.u32c.p;
 	uint32_t* ovOff = w.u32d.p;
 	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, w.def.gridCellSize, cnt, ov);
-	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
-	// oversized list in body order
-	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
+	// entry offsets, and the oversized list in body order
+	scan_exclusive2(cnt, offs, w.counters.p + CT_ENTRIES, ov, ovOff, w.counters.p + CT_OVERSIZED, n, w.scanStatus.p, s);
 	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
 	k_mark_cells<<<grid_for(n), kBlock, 0, s>>>(B, n, includeBit, 0, cellBit, cnt);
 	const unsigned emitGrid = reduce_grid(n, kBlock);
@@ -1458,9 +1457,8 @@ bool remove_separated(World& w)
 	scan_exclusive(rm, hole, C, w.counters.p + CT_REMOVED, w.scanStatus.p, s); // offsets unused: the count
 	k_hole_tail_flags<<<grid_for(C), kBlock, 0, s>>>(rm, w.cc.mat.p, C, w.counters.p + CT_REMOVED, hole, tail, ev,
 													 tailRm);
-	scan_exclusive(hole, holeOff, C, w.counters.p + CT_HOLES, w.scanStatus.p, s);
-	scan_exclusive(tail, tailOff, C, w.counters.p + CT_TAILS, w.scanStatus.p, s);
-	scan_exclusive(ev, evOff, C, w.counters.p + CT_END_EVENTS, w.scanStatus.p, s);
+	scan_exclusive3(hole, holeOff, w.counters.p + CT_HOLES, tail, tailOff, w.counters.p + CT_TAILS, ev, evOff,
+					w.counters.p + CT_END_EVENTS, C, w.scanStatus.p, s);
 	// end events of touching removed pairs, descending contact order
 	k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
 	k_gather_events<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p,
@@ -1968,8 +1966,7 @@ bool build_stable_batched(World& w)
 	uint32_t* keep = (uint32_t*)w.i32d.p;
 	uint32_t* keepOff = (uint32_t*)w.i32e.p;
 	k_tier_count<<<grid_for(n), kBlock, 0, s>>>(B, n, BP_STABLE, 1, w.def.gridCellSize, cnt, ov);
-	scan_exclusive(cnt, offs, n, w.counters.p + CT_ENTRIES, w.scanStatus.p, s);
-	scan_exclusive(ov, ovOff, n, w.counters.p + CT_OVERSIZED, w.scanStatus.p, s);
+	scan_exclusive2(cnt, offs, w.counters.p + CT_ENTRIES, ov, ovOff, w.counters.p + CT_OVERSIZED, n, w.scanStatus.p, s);
 	k_compact_indices<<<grid_for(n), kBlock, 0, s>>>(ov, ovOff, n, w.i32a.p);
 	if (oldE > 0)
 	{

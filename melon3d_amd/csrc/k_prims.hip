@@ -189,6 +189,136 @@ __global__ void __launch_bounds__(256) k_scan_onepass(const uint32_t* in, uint32
 	}
 }
 
+// K independent exclusive scans of n elements in one launch (K <= 3): the
+// same single-pass scan per array, wave k looking back over array k's status
+// words (status + k * tiles). Saves the launches of scans that follow one
+// flag kernel.
+template <int K>
+struct ScanArrays
+{
+	const uint32_t* in[K];
+	uint32_t* out[K];
+	uint32_t* total[K];
+};
+
+template <int K>
+__global__ void __launch_bounds__(256) k_scan_multi(ScanArrays<K> A, int64_t n, uint64_t* status, int64_t tiles,
+													uint32_t epoch)
+{
+	static_assert(K >= 1 && K <= 3, "one look-back wave per array");
+	__shared__ uint32_t waveSum[K][4];
+	__shared__ uint32_t tilePrefix[K];
+	const int t = threadIdx.x;
+	const int lane = t & 63, wave = t >> 6;
+	const int64_t tile = blockIdx.x;
+	const int64_t base = tile * kScanTile + (int64_t)t * kScanItems;
+	uint32_t v[K][kScanItems];
+	uint32_t sum[K], incl[K];
+#pragma unroll
+	for (int a = 0; a < K; ++a)
+	{
+		sum[a] = 0u;
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
+		{
+			v[a][k] = base + k < n ? A.in[a][base + k] : 0u;
+			sum[a] += v[a][k];
+		}
+		incl[a] = wave_inclusive_sum(sum[a]);
+		if (lane == 63)
+			waveSum[a][wave] = incl[a];
+	}
+	__syncthreads();
+	uint32_t waveOff[K], agg[K];
+#pragma unroll
+	for (int a = 0; a < K; ++a)
+	{
+		waveOff[a] = 0u;
+		agg[a] = 0u;
+		for (int w = 0; w < 4; ++w)
+		{
+			waveOff[a] += w < wave ? waveSum[a][w] : 0u;
+			agg[a] += waveSum[a][w];
+		}
+	}
+	if (wave < K)
+	{
+		// wave k publishes and looks back for array k
+		uint32_t myAgg = agg[0];
+#pragma unroll
+		for (int a = 1; a < K; ++a)
+			myAgg = wave == a ? agg[a] : myAgg;
+		uint64_t* st = status + (int64_t)wave * tiles;
+		uint32_t prefix = 0;
+		if (tile == 0)
+		{
+			if (lane == 0)
+				__hip_atomic_store(&st[0], pack_status(epoch, kFlagPrefix, myAgg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else
+		{
+			if (lane == 0)
+				__hip_atomic_store(&st[tile], pack_status(epoch, kFlagAgg, myAgg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+			int64_t j = tile - 1;
+			for (;;)
+			{
+				const int64_t idx = j - lane;
+				uint64_t word;
+				uint64_t preMask;
+				int first;
+				for (;;)
+				{
+					word = idx >= 0 ? __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+									: pack_status(epoch, kFlagPrefix, 0u);
+					uint64_t flag = (word >> 32) & 3ull;
+					bool ready = (uint32_t)(word >> 34) == epoch && flag != 0ull;
+					uint64_t readyMask = __ballot(ready);
+					preMask = __ballot(ready && flag == kFlagPrefix);
+					first = preMask ? __ffsll((unsigned long long)preMask) - 1 : 64;
+					uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+					if ((readyMask & need) == need)
+						break;
+				}
+				uint32_t mine = lane <= first ? (uint32_t)word : 0u;
+				for (int off = 32; off > 0; off >>= 1)
+					mine += __shfl_xor(mine, off);
+				prefix += mine;
+				if (first < 64)
+					break;
+				j -= 64;
+			}
+			if (lane == 0)
+				__hip_atomic_store(&st[tile], pack_status(epoch, kFlagPrefix, prefix + myAgg), __ATOMIC_RELAXED,
+								   __HIP_MEMORY_SCOPE_AGENT);
+		}
+		if (lane == 0)
+		{
+			tilePrefix[wave] = prefix;
+			uint32_t* tot = A.total[0];
+#pragma unroll
+			for (int a = 1; a < K; ++a)
+				tot = wave == a ? A.total[a] : tot;
+			if (tot != nullptr && (tile + 1) * kScanTile >= n)
+				*tot = prefix + myAgg;
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int a = 0; a < K; ++a)
+	{
+		uint32_t run = tilePrefix[a] + waveOff[a] + (incl[a] - sum[a]);
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
+		{
+			if (base + k < n)
+				A.out[a][base + k] = run;
+			run += v[a][k];
+		}
+	}
+}
+
 static uint32_t next_scan_epoch()
 {
 	static std::atomic<uint32_t> epoch{ 0 };
@@ -197,7 +327,8 @@ static uint32_t next_scan_epoch()
 
 int64_t scan_status_words(int64_t n)
 {
-	return (n + kScanTile - 1) / kScanTile + 1;
+	// room for the three arrays of scan_exclusive3
+	return 3 * ((n + kScanTile - 1) / kScanTile) + 1;
 }
 
 void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint64_t* status, hipStream_t s)
@@ -214,6 +345,47 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tota
 	// in and out may alias: each workgroup reads its whole tile before it
 	// writes it, and reads no other tile's input
 	k_scan_onepass<<<(unsigned)tiles, 256, 0, s>>>(in, out, n, total, status, next_scan_epoch());
+}
+
+template <int K>
+static void scan_exclusive_k(const uint32_t* const* in, uint32_t* const* out, int64_t n, uint32_t* const* total,
+							 uint64_t* status, hipStream_t s)
+{
+	if (n <= 0)
+	{
+		for (int a = 0; a < K; ++a)
+			if (total[a] != nullptr)
+				hipMemsetAsync(total[a], 0, sizeof(uint32_t), s);
+		return;
+	}
+	ScanArrays<K> A;
+	for (int a = 0; a < K; ++a)
+	{
+		A.in[a] = in[a];
+		A.out[a] = out[a];
+		A.total[a] = total[a];
+	}
+	int64_t tiles = (n + kScanTile - 1) / kScanTile;
+	k_scan_multi<K><<<(unsigned)tiles, 256, 0, s>>>(A, n, status, tiles, next_scan_epoch());
+}
+
+void scan_exclusive2(const uint32_t* in0, uint32_t* out0, uint32_t* total0, const uint32_t* in1, uint32_t* out1,
+					 uint32_t* total1, int64_t n, uint64_t* status, hipStream_t s)
+{
+	const uint32_t* in[2] = { in0, in1 };
+	uint32_t* out[2] = { out0, out1 };
+	uint32_t* total[2] = { total0, total1 };
+	scan_exclusive_k<2>(in, out, n, total, status, s);
+}
+
+void scan_exclusive3(const uint32_t* in0, uint32_t* out0, uint32_t* total0, const uint32_t* in1, uint32_t* out1,
+					 uint32_t* total1, const uint32_t* in2, uint32_t* out2, uint32_t* total2, int64_t n,
+					 uint64_t* status, hipStream_t s)
+{
+	const uint32_t* in[3] = { in0, in1, in2 };
+	uint32_t* out[3] = { out0, out1, out2 };
+	uint32_t* total[3] = { total0, total1, total2 };
+	scan_exclusive_k<3>(in, out, n, total, status, s);
 }
 
 // ------------------------------------------------------------ radix sort

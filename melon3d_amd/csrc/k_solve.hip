@@ -531,8 +531,8 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	radix_sort_pairs(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p, w.scanStatus.p, s);
 	if (NC > 0)
 		radix_sort_pairs(w.u64c.p, contactListVals, w.u64a.p, w.sortValsAlt.p, NC, bitsI, w.primScratch.p, w.scanStatus.p, s);
-	scan_exclusive(islandBodies, islandBodies, I + 1, nullptr, w.scanStatus.p, s);
-	scan_exclusive(islandContacts, islandContacts, I + 1, nullptr, w.scanStatus.p, s);
+	scan_exclusive2(islandBodies, islandBodies, nullptr, islandContacts, islandContacts, nullptr, I + 1, w.scanStatus.p,
+					s);
 
 	if (NB > 0)
 	{

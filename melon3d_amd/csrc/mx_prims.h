@@ -16,6 +16,13 @@ int64_t scan_status_words(int64_t n);
 // out[i] = sum(in[0..i)); *total (device pointer, may be null) = sum(in).
 // in and out may alias.
 void scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint64_t* status, hipStream_t s);
+// Two or three independent exclusive scans of n elements each in one launch
+// (status holds scan_status_words(n) words; in and out of an array may alias).
+void scan_exclusive2(const uint32_t* in0, uint32_t* out0, uint32_t* total0, const uint32_t* in1, uint32_t* out1,
+					 uint32_t* total1, int64_t n, uint64_t* status, hipStream_t s);
+void scan_exclusive3(const uint32_t* in0, uint32_t* out0, uint32_t* total0, const uint32_t* in1, uint32_t* out1,
+					 uint32_t* total1, const uint32_t* in2, uint32_t* out2, uint32_t* total2, int64_t n,
+					 uint64_t* status, hipStream_t s);
 
 // Bytes of histogram scratch and status words radix_sort_pairs needs for n
 // elements.
