@@ -65,6 +65,27 @@ static void launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s)
 		k_fill_u64<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
 }
 
+// Up to four fills of 32-bit words in one launch (a fill is a launch of a
+// few microseconds; several small ones in a row cost more than the work).
+__global__ void k_fill_multi(FillSegs F)
+{
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	for (int k = 0; k < F.count; ++k)
+	{
+		if (i < F.n[k])
+			F.p[k][i] = F.v[k];
+	}
+}
+
+void launch_fill_multi(const FillSegs& F, hipStream_t s)
+{
+	int64_t m = 0;
+	for (int k = 0; k < F.count; ++k)
+		m = std::max(m, F.n[k]);
+	if (m > 0)
+		k_fill_multi<<<grid_for(m), kBlock, 0, s>>>(F);
+}
+
 static DBodies body_view(World& w)
 {
 	DBodies B;
@@ -1232,12 +1253,14 @@ bool enumerate_and_create(World& w)
 		R.runBase = runBase;
 		itemCnt = base + 7 * E;
 		itemOffs = itemCnt + T;
+		FillSegs F;
+		F.add(R.runItems, 0u, E);
+		F.add(itemCnt, 0u, T);
+		launch_fill_multi(F, s);
 		k_run_flags<<<grid_for(E), kBlock, 0, s>>>(w.activeKeys.p, E, flag);
 		scan_exclusive(flag, runEx, E, nullptr, w.scanStatus.p, s);
-		launch_fill_u32(R.runItems, 0u, E, s);
 		k_run_info<<<grid_for(E), kBlock, 0, s>>>(P, R);
 		scan_exclusive(R.runItems, runBase, E, nullptr, w.scanStatus.p, s);
-		launch_fill_u32(itemCnt, 0u, T, s);
 		w.mark_begin(M3DX_K_PAIRS);
 		k_pairs_a<<<grid_for(E), kBlock, 0, s>>>(P, R, 0, itemCnt, nullptr, nullptr);
 		if (SB > 0)
@@ -1281,8 +1304,13 @@ bool enumerate_and_create(World& w)
 	if (!w.u64c.ensure(csz) || !w.u32a.ensure(std::max<int64_t>(csz, total)) || !w.u32b.ensure(total) ||
 		!w.u32c.ensure(total) || !ensure_prim_scratch(w, total))
 		return false;
-	launch_fill_u64(w.u64c.p, kEmptyKey, csz, s);
-	launch_fill_u32(w.u32a.p, 0xFFFFFFFFu, csz, s);
+	{
+		// the key table (all ones: kEmptyKey) and its first-occurrence slots
+		FillSegs F;
+		F.add((uint32_t*)w.u64c.p, 0xFFFFFFFFu, 2 * (int64_t)csz);
+		F.add(w.u32a.p, 0xFFFFFFFFu, csz);
+		launch_fill_multi(F, s);
+	}
 	k_first_insert<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1);
 	k_first_flag<<<grid_for(total), kBlock, 0, s>>>(w.u64a.p, total, w.u64c.p, w.u32a.p, csz - 1, w.u32b.p);
 	scan_exclusive(w.u32b.p, w.u32c.p, total, w.counters.p + CT_NEW_CONTACTS, w.scanStatus.p, s);

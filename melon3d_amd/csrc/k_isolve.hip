@@ -1790,9 +1790,13 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
 	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, w.scanStatus.p, s);
 	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
-	launch_fill_u32(w.sCtr.p, 0u, 16, s); // fixpoint counters [0, 6)
-	launch_fill_u32(w.gridBar.p, 0u, 2 * (int64_t)kBarWords, s); // the passes' and the order's barriers
-	launch_fill_u32((uint32_t*)w.sDirty.p, 0u, (int64_t)(two + 3) / 4, s);
+	{
+		FillSegs F;
+		F.add(w.sCtr.p, 0u, 16);                                   // fixpoint counters [0, 6)
+		F.add(w.gridBar.p, 0u, 2 * (int64_t)kBarWords);            // the passes' and the order's barriers
+		F.add((uint32_t*)w.sDirty.p, 0u, (int64_t)(two + 3) / 4); // fixpoint worklists
+		launch_fill_multi(F, s);
+	}
 
 	GiantCtx G;
 	G.X = X;

@@ -477,14 +477,18 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	if (!w.islandAwakeD.ensure(I + 1) || !w.u32d.ensure(2 * (size_t)(I + 1)) || !w.islandMode.ensure(I + 1))
 		return false;
 	uint32_t* islandAwake = w.islandAwakeD.p;
-	launch_fill_u32(islandAwake, 0u, I, s);
-	k_island_of<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, w.u32b.p, n, w.islandOf.p, islandAwake);
-	k_island_wake<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), n, w.islandOf.p, islandAwake, w.counters.p);
-
 	// island body lists (ascending) and contact lists (ascending)
 	uint32_t* islandBodies = w.u32d.p; // counts, then exclusive starts
 	uint32_t* islandContacts = w.u32d.p + I + 1;
-	launch_fill_u32(islandBodies, 0u, 2 * (int64_t)(I + 1), s);
+	{
+		FillSegs F;
+		F.add(islandAwake, 0u, I);
+		F.add(islandBodies, 0u, 2 * (int64_t)(I + 1));
+		launch_fill_multi(F, s);
+	}
+	k_island_of<<<grids(n), kBlockS, 0, s>>>(parent, w.bc.sb.p, w.u32b.p, n, w.islandOf.p, islandAwake);
+	k_island_wake<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), n, w.islandOf.p, islandAwake, w.counters.p);
+
 	int bitsI = 1;
 	while ((1ll << bitsI) <= I)
 		++bitsI;
@@ -556,8 +560,12 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	// sleeping
 	if (!w.i32d.ensure(I) || !w.u32a.ensure(I))
 		return false;
-	launch_fill_i32(w.i32d.p, 0x7FFFFFFF, I, s);
-	launch_fill_u32(w.u32a.p, f2u(FLT_MAX), I, s);
+	{
+		FillSegs F;
+		F.add((uint32_t*)w.i32d.p, 0x7FFFFFFFu, I);
+		F.add(w.u32a.p, f2u(FLT_MAX), I);
+		launch_fill_multi(F, s);
+	}
 	k_sleep_prep<<<grids(n), kBlockS, 0, s>>>(w.bc.flags.p, w.islandOf.p, islandAwake, n, w.i32d.p);
 	k_sleep_update<<<grids(n), kBlockS, 0, s>>>(make_body_view(w), w.islandOf.p, islandAwake, n, dt, w.i32d.p,
 												w.u32a.p);

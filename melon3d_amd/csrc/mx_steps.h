@@ -98,6 +98,22 @@ bool enumerate_and_create(World& w);
 bool remove_separated(World& w);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+// Up to four fills of 32-bit words, one launch.
+struct FillSegs
+{
+	uint32_t* p[4];
+	uint32_t v[4];
+	int64_t n[4];
+	int count = 0;
+	void add(uint32_t* ptr, uint32_t value, int64_t words)
+	{
+		p[count] = ptr;
+		v[count] = value;
+		n[count] = words;
+		++count;
+	}
+};
+void launch_fill_multi(const FillSegs& F, hipStream_t s);
 bool apply_moves(World& w, const int2* dMoves, int32_t count);
 void launch_material_refresh(World& w, const uint8_t* dFlags);
 // applies and clears the host's list of bodies to wake
