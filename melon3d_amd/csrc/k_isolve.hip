@@ -1788,8 +1788,18 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	k_giant_adj<<<gridk(NC), 256, 0, s>>>(X, w.sPred.p, w.sColored.p, w.sColor.p, w.sLevel.p, n, w.s64a.p, w.s32a.p,
 										  bodyCount);
 	scan_exclusive(bodyCount, bodyCount, (int64_t)n + 1, nullptr, w.scanStatus.p, s);
-	radix_sort_pairs(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN, w.primScratch.p, w.scanStatus.p, s);
-	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, w.s64a.p, w.s32a.p, bodyCount, (int64_t)two, n, w.sPred.p);
+	// The two sorts below leave an odd pass count's result in their
+	// ping-pong buffers (no copy back): the adjacency lands in one pair of
+	// the (s64, s32) buffers and the slot keys go to the other, which the
+	// slot sort then ping-pongs with the adjacency's pair (free once the
+	// fixpoints are done).
+	const bool adjAlt = radix_sort_pairs_in_place_or_alt(w.s64a.p, w.s32a.p, w.s64b.p, w.s32b.p, (int64_t)two, bitsN,
+														 w.primScratch.p, w.scanStatus.p, s);
+	uint64_t* adjK = adjAlt ? w.s64b.p : w.s64a.p;
+	uint32_t* adjV = adjAlt ? w.s32b.p : w.s32a.p;
+	uint64_t* slotK = adjAlt ? w.s64a.p : w.s64b.p;
+	uint32_t* slotV = adjAlt ? w.s32a.p : w.s32b.p;
+	k_giant_pred<<<gridk((int64_t)two), 256, 0, s>>>(X, adjK, adjV, bodyCount, (int64_t)two, n, w.sPred.p);
 	{
 		FillSegs F;
 		F.add(w.sCtr.p, 0u, 16);                                   // fixpoint counters [0, 6)
@@ -1801,7 +1811,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	GiantCtx G;
 	G.X = X;
 	G.adjStart = bodyCount;
-	G.adj = w.s32a.p;
+	G.adj = adjV;
 	G.pred = w.sPred.p;
 	G.colored = w.sColored.p;
 	G.color = w.sColor.p;
@@ -1809,8 +1819,8 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	G.ctr = w.sCtr.p;
 	G.bar = w.gridBar.p + kBarWords;
 	G.counters = w.counters.p;
-	G.slotKeys = w.s64b.p;
-	G.slotVals = w.s32b.p;
+	G.slotKeys = slotK;
+	G.slotVals = slotV;
 	G.dirty = w.sDirty.p;
 	G.bitsI = L.bitsI;
 	G.maxS = maxS;
@@ -1836,11 +1846,16 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 					"giant order"))
 			return false;
 	}
-	radix_sort_pairs(w.s64b.p, w.s32b.p, w.s64a.p, w.s32a.p, NC, L.bitsI + bitsS + 1, w.primScratch.p, w.scanStatus.p, s);
-	k_giant_slot_starts<<<gridk(maxS + 1), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, maxS, w.sSlotStart.p);
-	k_giant_packets<<<gridk(NC), 256, 0, s>>>(w.s64b.p, NC, L.bitsI, w.sPacket.p);
+	if (radix_sort_pairs_in_place_or_alt(slotK, slotV, adjK, adjV, NC, L.bitsI + bitsS + 1, w.primScratch.p,
+										 w.scanStatus.p, s))
+	{
+		slotK = adjK;
+		slotV = adjV;
+	}
+	k_giant_slot_starts<<<gridk(maxS + 1), 256, 0, s>>>(slotK, NC, L.bitsI, maxS, w.sSlotStart.p);
+	k_giant_packets<<<gridk(NC), 256, 0, s>>>(slotK, NC, L.bitsI, w.sPacket.p);
 
-	S.entries = w.s32b.p;
+	S.entries = slotV;
 	S.packet = w.sPacket.p;
 	S.slotStart = w.sSlotStart.p;
 	S.counters = w.counters.p;
@@ -1849,7 +1864,7 @@ bool solve_islands(World& w, const IslandLists& L, float dt, int substeps)
 	S.rec = w.sRecG.p;
 	S.nrm = w.sNrm.p;
 	S.nrmStride = NC;
-	S.keys = w.s64b.p;
+	S.keys = slotK;
 	S.bitsI = L.bitsI;
 	S.bucketCount = w.sBucket.p;
 	S.bucketCursor = w.sBucket.p + (size_t)(maxS + 1) * kBuckets;
