@@ -532,9 +532,20 @@ bool islands_and_solve(World& w, float dt, int substeps)
 	w.stats.solvedPoints = (int32_t)w.hostCounters[CT_SOLVED_POINTS];
 	w.stats.boxBoxPoints = (int32_t)w.hostCounters[CT_BB_POINTS];
 	w.stats.coloredIslands = (int32_t)w.hostCounters[CT_COLORED_ISLANDS];
-	radix_sort_pairs(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p, w.scanStatus.p, s);
-	if (NC > 0)
-		radix_sort_pairs(w.u64c.p, contactListVals, w.u64a.p, w.sortValsAlt.p, NC, bitsI, w.primScratch.p, w.scanStatus.p, s);
+	// Odd pass counts leave the lists in the ping-pong buffers (no copy
+	// back): the body sort's in sortValsAlt, the contact sort's in u64a and
+	// i32d (both free by then: the body keys are not read again, and i32d
+	// only fed the compaction).
+	uint64_t* entryIsland = w.u64c.p;
+	if (radix_sort_pairs_in_place_or_alt(w.u64b.p, bodyListVals, w.u64a.p, w.sortValsAlt.p, NB, bitsI, w.primScratch.p,
+										 w.scanStatus.p, s))
+		bodyListVals = w.sortValsAlt.p;
+	if (NC > 0 && radix_sort_pairs_in_place_or_alt(w.u64c.p, contactListVals, w.u64a.p, (uint32_t*)w.i32d.p, NC, bitsI,
+												   w.primScratch.p, w.scanStatus.p, s))
+	{
+		entryIsland = w.u64a.p;
+		contactListVals = (uint32_t*)w.i32d.p;
+	}
 	scan_exclusive2(islandBodies, islandBodies, nullptr, islandContacts, islandContacts, nullptr, I + 1, w.scanStatus.p,
 					s);
 
@@ -545,7 +556,7 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		L.bodyStart = islandBodies;
 		L.contactList = contactListVals;
 		L.contactStart = islandContacts;
-		L.entryIsland = w.u64c.p;
+		L.entryIsland = entryIsland;
 		L.mode = mode;
 		L.islandOf = w.islandOf.p;
 		L.jointIsland = w.i32e.p;
