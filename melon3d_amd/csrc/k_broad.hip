@@ -1441,17 +1441,16 @@ bool apply_moves(World& w, const int2* dMoves, int32_t count)
 	return true;
 }
 
-// count is read on the device (the number of end events of the removal)
-__global__ void k_gather_events(const int2* bodies, const float4* mat, const int32_t* idx, const uint32_t* count,
-								int4* out)
+// (body a, body b, sensor, contact) of each flagged contact, at its offset
+__global__ void k_gather_flagged_events(const int2* bodies, const float4* mat, const uint32_t* flag,
+										const uint32_t* offs, int32_t C, int4* out)
 {
-	int32_t k = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (k >= (int32_t)*count)
+	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (c >= C || !flag[c])
 		return;
-	int32_t c = idx[k];
 	int2 b = bodies[c];
 	uint32_t cf = f2u(mat[c].w);
-	out[k] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, c);
+	out[offs[c]] = make_int4(b.x, b.y, (cf & CF_SENSOR) ? 1 : 0, c);
 }
 
 bool remove_separated(World& w)
@@ -1461,7 +1460,7 @@ bool remove_separated(World& w)
 	if (C == 0)
 		return true;
 	if (!w.u32a.ensure(C) || !w.u32b.ensure(C) || !w.u32c.ensure(C) || !w.u32d.ensure(C) || !w.i32a.ensure(C) ||
-		!w.i32b.ensure(C) || !w.i32c.ensure(C) || !w.i32d.ensure(C) || !w.eventIdx.ensure(C) ||
+		!w.i32b.ensure(C) || !w.i32c.ensure(C) || !w.i32d.ensure(C) ||
 		!w.u64a.ensure((size_t)C * 2) || !ensure_prim_scratch(w, C))
 		return false;
 	// One readback returns the removed count, the hole / tail / end-event
@@ -1487,10 +1486,9 @@ bool remove_separated(World& w)
 													 tailRm);
 	scan_exclusive3(hole, holeOff, w.counters.p + CT_HOLES, tail, tailOff, w.counters.p + CT_TAILS, ev, evOff,
 					w.counters.p + CT_END_EVENTS, C, w.scanStatus.p, s);
-	// end events of touching removed pairs, descending contact order
-	k_compact_indices<<<grid_for(C), kBlock, 0, s>>>(ev, evOff, C, w.eventIdx.p);
-	k_gather_events<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.eventIdx.p,
-												   w.counters.p + CT_END_EVENTS, (int4*)w.u64a.p);
+	// end events of touching removed pairs, ascending contact order (the
+	// host reads them back to front)
+	k_gather_flagged_events<<<grid_for(C), kBlock, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, ev, evOff, C, (int4*)w.u64a.p);
 	int4* evHost = (int4*)w.pinnedRemoval;
 	uint32_t* flags = (uint32_t*)(w.pinnedRemoval + (size_t)spec * sizeof(int4));
 	int32_t* order = (int32_t*)(w.pinnedRemoval + (size_t)spec * (sizeof(int4) + sizeof(uint32_t)));

@@ -98,20 +98,16 @@ __global__ void k_trans_flags(const int8_t* trans, int32_t count, uint32_t* flag
 		flags[c] = trans[c] != 0 ? 1u : 0u;
 }
 
-__global__ void k_trans_compact(const int8_t* trans, const uint32_t* offs, int32_t count, int32_t* list)
+// The transitions in contact order: list[t] = contact, out[t] = (body a,
+// body b, kind, contact) with kind +-1 (begin / end), +-2 for sensors.
+__global__ void k_trans_compact(const int2* bodies, const float4* mat, const int8_t* trans, const uint32_t* offs,
+								int32_t count, int32_t* list, int4* out)
 {
 	int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (c < count && trans[c] != 0)
-		list[offs[c]] = c;
-}
-
-__global__ void k_trans_info(const int2* bodies, const float4* mat, const int8_t* trans, const int32_t* list,
-							 const uint32_t* count, int4* out)
-{
-	int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (t >= (int32_t)*count)
+	if (c >= count || trans[c] == 0)
 		return;
-	int32_t c = list[t];
+	const uint32_t t = offs[c];
+	list[t] = c;
 	int2 b = bodies[c];
 	uint32_t cf = f2u(mat[c].w);
 	out[t] = make_int4(b.x, b.y, (int)trans[c] * (((cf & CF_SENSOR) != 0) ? 2 : 1), c);
@@ -206,9 +202,8 @@ bool narrowphase(World& w, float dt)
 	// The transitions are listed and described on the device (their count
 	// read there); the count comes back with the island stage's first
 	// readback, which issues their download (queue_transitions).
-	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(trans, w.u32b.p, C, w.transList.p);
-	k_trans_info<<<gridn(C), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.transList.p,
-											  w.counters.p + CT_TRANSITIONS, w.transInfo.p);
+	k_trans_compact<<<gridn(C), kBlockN, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, trans, w.u32b.p, C, w.transList.p,
+												 w.transInfo.p);
 	// wake-on-touch fixpoint
 	int32_t* wakeAt = w.i32c.p;
 	k_wake_init<<<gridn(n), kBlockN, 0, s>>>(w.bc.flags.p, n, wakeAt);
