@@ -29,17 +29,16 @@ static inline unsigned grids(int64_t n) { return (unsigned)((n + kBlockS - 1) / 
 
 // ----------------------------------------------------------- islands
 
-__global__ void k_any_awake_dyn(const uint8_t* sb, int32_t n, uint32_t* counters)
+// Flags an awake dynamic body in the counters; with parent set, also starts
+// the union-find forest (every body its own root).
+__global__ void k_any_awake_dyn(const uint8_t* sb, int32_t n, uint32_t* counters, int32_t* parent)
 {
 	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i < n && (sb[i] & (SB_DYNAMIC | SB_AWAKE)) == (SB_DYNAMIC | SB_AWAKE))
+	if (i >= n)
+		return;
+	if ((sb[i] & (SB_DYNAMIC | SB_AWAKE)) == (SB_DYNAMIC | SB_AWAKE))
 		counters[CT_ANY_AWAKE_DYN] = 1u;
-}
-
-__global__ void k_uf_init(int32_t* parent, int32_t n)
-{
-	int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i < n)
+	if (parent != nullptr)
 		parent[i] = i;
 }
 
@@ -99,7 +98,7 @@ __global__ void k_uf_joints(const DJoint* J, int32_t nJ, const uint32_t* flags, 
 
 // Batched worlds: one workgroup per sub-world does its union-find in LDS (a
 // sub-world's bodies and contacts are contiguous ranges, and no contact
-// joins two sub-worlds). Same result as k_uf_init, k_uf_contacts and
+// joins two sub-worlds). Same result as the forest start in k_any_awake_dyn, k_uf_contacts and
 // k_uf_roots: each body's root is its island's smallest body.
 constexpr int32_t kUfLocalBodies = 4096;
 
@@ -447,16 +446,16 @@ bool islands_and_solve(World& w, float dt, int substeps)
 		!ensure_prim_scratch(w, m + 1))
 		return false;
 	int32_t* parent = w.i32a.p;
-	k_any_awake_dyn<<<grids(n), kBlockS, 0, s>>>(w.bc.sb.p, n, w.counters.p);
-	if (w.subCount > 0 && w.djointCount == 0 && w.maxSubBodies <= kUfLocalBodies &&
-		w.subBodyStart.size() == (size_t)w.subCount + 1 && w.subBodyStart[w.subCount] == n)
+	const bool ufLocal = w.subCount > 0 && w.djointCount == 0 && w.maxSubBodies <= kUfLocalBodies &&
+						 w.subBodyStart.size() == (size_t)w.subCount + 1 && w.subBodyStart[w.subCount] == n;
+	k_any_awake_dyn<<<grids(n), kBlockS, 0, s>>>(w.bc.sb.p, n, w.counters.p, ufLocal ? nullptr : parent);
+	if (ufLocal)
 	{
 		k_uf_sub<<<(unsigned)w.subCount, 256, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, w.subBodyStartD.p,
 													  w.subContactStart.p, parent, w.u32a.p);
 	}
 	else
 	{
-		k_uf_init<<<grids(n), kBlockS, 0, s>>>(parent, n);
 		if (C > 0)
 			k_uf_contacts<<<grids(C), kBlockS, 0, s>>>(w.cc.bodies.p, w.cc.mat.p, w.bc.sb.p, C, parent);
 		if (w.djointCount > 0)
